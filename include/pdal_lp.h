@@ -1,0 +1,48 @@
+/*
+ * Additional C entry points of libcp_al.so that have no counterpart in the
+ * reference API.  They exist for tests and measurement only; a drop-in caller
+ * of cp_al.h never needs them.
+ */
+#pragma once
+
+#include "cp_al_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Reseed the process-global power-iteration generator to std::mt19937(1).
+ * The reference keeps that generator for the life of the process
+ * (LP/CP_AL/src/utils.cu:27-28), so the k-th solve of a process starts its
+ * power iteration from the k-th block of draws; tests call this before each
+ * solve to make runs independent of call order. */
+void pdal_cp_al_reset_rng(void);
+
+/* Set up the device state for `prob` (upload, transpose, rescale, power
+ * iteration) and launch `iters` plain iterations kernel by kernel, timing each
+ * fused SpMV kernel with hipEvents on the solver's stream.  out[0..6] =
+ * {avg ms primal (At.y + primal update), avg ms dual (A.x_bar + dual update),
+ * avg ms predictor (0 if beta == 0), avg ms per iteration, ms of one
+ * evaluation pass, lanes per row for At, lanes per row for A}.
+ * Returns 0 on success, negative on error. */
+int pdal_cp_al_time_kernels(const lp_problem_t *prob, const cp_al_parameters_t *params, int iters, double *out);
+
+/* Device-resident solve session.  create: upload, transpose, rescale and power
+ * iteration for `prob` (which the caller keeps alive until free).  solve: run
+ * the main loop from the problem's start point and return a result exactly as
+ * solve_lp_problem would (free it with cp_al_result_free); may be called any
+ * number of times.  time_kernels: as pdal_cp_al_time_kernels on the session's
+ * current iterate.  setup_seconds: wall time of create.  NULL / negative on
+ * error. */
+void *pdal_cp_al_session_create(lp_problem_t *prob, const cp_al_parameters_t *params);
+cp_al_result_t *pdal_cp_al_session_solve(void *session);
+int pdal_cp_al_session_time_kernels(void *session, int iters, double *out);
+double pdal_cp_al_session_setup_seconds(void *session);
+void pdal_cp_al_session_free(void *session);
+
+/* reference: LP/CP_AL/src/utils.cu:230-253 (exported there through utils.h) */
+const char *termination_reason_to_string(termination_reason_t reason);
+
+#ifdef __cplusplus
+}
+#endif

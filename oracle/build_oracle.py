@@ -1,0 +1,38 @@
+"""Build the CPU oracle shared objects (test infrastructure, gcc only).
+
+Output: oracle/_build/libcp_al_oracle.so.  The oracle exports the same C names
+as the product library; it is built with hidden visibility and -Bsymbolic so a
+process can load both without the two binding to each other's symbols.
+
+The reference itself cannot be compiled here (CUDA / cuSPARSE / cuBLAS sources,
+PSLP fetched over the network), so there is no oracle/_ref build.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parent
+OUT = HERE / "_build"
+
+FLAGS = ["-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-fvisibility=hidden",
+         "-Wl,-Bsymbolic", f"-I{ROOT / 'include'}"]
+
+
+def build() -> Path:
+    OUT.mkdir(exist_ok=True)
+    lib = OUT / "libcp_al_oracle.so"
+    tmp = OUT / "libcp_al_oracle.so.tmp"
+    cmd = ["g++", *FLAGS, str(HERE / "cp_al_oracle.cpp"), "-o", str(tmp)]
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, lib)
+    return lib
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

@@ -1,0 +1,910 @@
+/*
+ * CPU FP64 oracle for the CP_AL (LP) restarted reflected-Halpern solver.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker that the HIP path in
+ * pdal_amd/ is compared against; nothing in the product links, loads or calls
+ * it.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use
+ * it (bench.py times it as the CPU baseline, "kind": "port").
+ *
+ * It exports the same C names as libcp_al.so (create_lp_problem,
+ * solve_lp_problem, ...) so tests can drive both through one ctypes wrapper,
+ * plus cp_al_oracle_reset_rng().  It is built as its own shared object with
+ * hidden visibility and -Bsymbolic so its symbols never bind to the product's.
+ *
+ * Statement-by-statement restatement of the reference default path:
+ *   optimize ................. LP/CP_AL/src/solver.cu:1175-1736
+ *   initialize_solver_state .. solver.cu:1795-2273 (norms :1999-2067)
+ *   iteration kernels ........ solver.cu:2340-2459, launch order :2523-2661
+ *   fixed-point error ........ solver.cu:3611-3686
+ *   perform_restart .......... solver.cu:2821-3101 (target "current", PID branch)
+ *   refresh_step_schedule .... solver.cu:3103-3126 (alpha mode)
+ *   step size / primal weight  solver.cu:3538-3609
+ *   create_result_from_state . solver.cu:3828-3909
+ *   power iteration .......... LP/CP_AL/src/utils.cu:87-228 (global mt19937(1), :27-28)
+ *   termination / restart .... utils.cu:275-359
+ *   defaults ................. utils.cu:361-394
+ *   residual ................. utils.cu:649-682, 812-917
+ *   matrix filter (verbose) .. utils.cu:396-478
+ *   conversions .............. utils.cu:1032-1192, cp_al.c:26-209
+ *   rescaling ................ LP/CP_AL/src/preconditioner.cu:162-475, kernels :477-736
+ *
+ * Supported environment knobs (read where the reference reads them):
+ *   CP_AL_LAMBDA, CP_AL_STEP_MARGIN, CP_AL_REFLECTION_COEFFICIENT,
+ *   CP_AL_RESTART_SUFFICIENT / _NECESSARY / _ARTIFICIAL, CP_AL_CR_SCALING,
+ *   CP_AL_CR_ITERS.  The sigma-mode, adaptive-lambda, alpha-trial,
+ *   direct-pulse, restart-select and dynamic-eval controllers are not part of
+ *   this oracle (nor of the HIP path).
+ *
+ * Deliberate differences from the reference, shared with the HIP path:
+ *   - timing uses steady_clock wall time, not clock() CPU time;
+ *   - presolve is not available (PSLP is not vendored): a request is ignored
+ *     with a warning;
+ *   - result->absolute_{primal,dual}_residual are filled in;
+ *   - errors return NULL instead of exit().
+ */
+#include "cp_al.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#define ORACLE_API extern "C" __attribute__((visibility("default")))
+
+namespace {
+
+std::mt19937 g_gen(1);                              // utils.cu:27
+std::normal_distribution<double> g_dist(0.0, 1.0);  // utils.cu:28
+
+double env_double(const char *name, double def)  // solver.cu:377-391, utils.cu:33-46
+{
+    const char *raw = getenv(name);
+    if (!raw || !raw[0]) return def;
+    char *end = nullptr;
+    const double v = strtod(raw, &end);
+    if (end == raw || !std::isfinite(v)) return def;
+    return v;
+}
+
+int env_int(const char *name, int def)  // solver.cu:393-407
+{
+    const char *raw = getenv(name);
+    if (!raw || !raw[0]) return def;
+    char *end = nullptr;
+    const long v = strtol(raw, &end, 10);
+    if (end == raw) return def;
+    return (int)v;
+}
+
+struct Csr {
+    int rows = 0, cols = 0;
+    std::vector<int> rp, ci;
+    std::vector<double> v;
+};
+
+void spmv(const Csr &A, const double *x, double *y)
+{
+    for (int i = 0; i < A.rows; ++i) {
+        double s = 0.0;
+        for (int k = A.rp[i]; k < A.rp[i + 1]; ++k) s += A.v[k] * x[A.ci[k]];
+        y[i] = s;
+    }
+}
+
+double nrm2(const std::vector<double> &x)
+{
+    double s = 0.0;
+    for (double a : x) s += a * a;
+    return std::sqrt(s);
+}
+
+double dot(const std::vector<double> &a, const std::vector<double> &b)
+{
+    double s = 0.0;
+    for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+    return s;
+}
+
+double inf_norm(const std::vector<double> &x)
+{
+    double m = 0.0;
+    for (double a : x) m = std::max(m, std::fabs(a));
+    return m;
+}
+
+/* CSR -> CSR of the transpose; rows stay ascending inside each column
+ * (the order cusparseCsr2cscEx2 produces, solver.cu:1859-1890). */
+Csr transpose(const Csr &A)
+{
+    Csr T;
+    T.rows = A.cols;
+    T.cols = A.rows;
+    T.rp.assign(A.cols + 1, 0);
+    for (int k = 0; k < A.rp[A.rows]; ++k) T.rp[A.ci[k] + 1]++;
+    for (int j = 0; j < A.cols; ++j) T.rp[j + 1] += T.rp[j];
+    T.ci.resize(A.rp[A.rows]);
+    T.v.resize(A.rp[A.rows]);
+    std::vector<int> next(T.rp.begin(), T.rp.end() - 1);
+    for (int i = 0; i < A.rows; ++i)
+        for (int k = A.rp[i]; k < A.rp[i + 1]; ++k) {
+            const int p = next[A.ci[k]]++;
+            T.ci[p] = i;
+            T.v[p] = A.v[k];
+        }
+    return T;
+}
+
+struct State {
+    int n = 0, m = 0;
+    double obj_const = 0.0;
+    Csr A, At;
+    std::vector<double> c, lb, ub, l, u, l_fin, u_fin;
+    std::vector<double> x0, x, xT, xH, xR, dual_prod, dual_slack, dual_res, dx;
+    std::vector<double> y0, y, yT, yH, yR, prim_prod, primal_slack, primal_res, dy;
+    std::vector<double> E, D;  // cumulative constraint / variable rescaling
+    double b_scale = 1.0, c_scale = 1.0;
+    double c_norm = 0.0, b_norm = 0.0;
+
+    double step_size = 0.0, primal_weight = 1.0, al_lambda = 0.0, al_penalty = 0.0;
+    double lambda_est = 1.0;
+    double pw_error_sum = 0.0, pw_last_error = 0.0, best_pw = 1.0, best_gap = INFINITY;
+    int total_count = 0, inner_count = 0;
+    double fp = 0.0, fp0 = 0.0, last_trial_fp = INFINITY;
+
+    double abs_p = 0, rel_p = 0, abs_d = 0, rel_d = 0, p_obj = 0, d_obj = 0, gap = 0, rel_gap = 0;
+    termination_reason_t reason = TERMINATION_REASON_UNSPECIFIED;
+    double rescaling_time = 0.0, solve_time = 0.0;
+};
+
+/* ---------------- preconditioner.cu ---------------- */
+
+void scale_problem(State &s, const std::vector<double> &e, const std::vector<double> &d,
+                   const std::vector<double> &inv_e, const std::vector<double> &inv_d)
+{   // preconditioner.cu:125-160 and kernels :477-531
+    for (int j = 0; j < s.n; ++j) {
+        s.c[j] *= inv_d[j];
+        s.lb[j] *= d[j];
+        s.ub[j] *= d[j];
+        s.x0[j] *= d[j];
+    }
+    for (int i = 0; i < s.m; ++i) {
+        s.l[i] *= inv_e[i];
+        s.u[i] *= inv_e[i];
+        s.y0[i] *= e[i];
+    }
+    for (int i = 0; i < s.m; ++i)
+        for (int k = s.A.rp[i]; k < s.A.rp[i + 1]; ++k) s.A.v[k] *= inv_d[s.A.ci[k]] * inv_e[i];
+    for (int j = 0; j < s.n; ++j)
+        for (int k = s.At.rp[j]; k < s.At.rp[j + 1]; ++k) s.At.v[k] *= inv_d[j] * inv_e[s.At.ci[k]];
+}
+
+void row_absmax(const Csr &M, std::vector<double> &out)  // preconditioner.cu:533-551
+{
+    for (int i = 0; i < M.rows; ++i) {
+        double mx = 0.0;
+        for (int k = M.rp[i]; k < M.rp[i + 1]; ++k) {
+            double v = std::fabs(M.v[k]);
+            if (!std::isfinite(v)) v = 0.0;
+            if (v > mx) mx = v;
+        }
+        out[i] = mx;
+    }
+}
+
+double pow_fast(double v, double p)  // preconditioner.cu:652-661
+{
+    if (p == 2.0) return v * v;
+    if (p == 1.0) return v;
+    if (p == 0.5) return std::sqrt(v);
+    return std::pow(v, p);
+}
+
+void row_powsum(const Csr &M, double degree, std::vector<double> &out)  // :663-681
+{
+    for (int i = 0; i < M.rows; ++i) {
+        double acc = 0.0;
+        for (int k = M.rp[i]; k < M.rp[i + 1]; ++k) {
+            double v = std::fabs(M.v[k]);
+            if (!std::isfinite(v)) v = 0.0;
+            acc += pow_fast(v, degree);
+        }
+        out[i] = acc;
+    }
+}
+
+void clamp_sqrt_accum(std::vector<double> &sf, std::vector<double> &inv, std::vector<double> &cum)
+{   // preconditioner.cu:683-697, SCALING_EPSILON 1e-12
+    for (size_t t = 0; t < sf.size(); ++t) {
+        const double v = sf[t];
+        const double q = (v < 1e-12) ? 1.0 : std::sqrt(v);
+        cum[t] *= q;
+        sf[t] = q;
+        inv[t] = 1.0 / q;
+    }
+}
+
+void cr_log_update(const Csr &M, const std::vector<double> &other, std::vector<double> &log_scale)
+{   // preconditioner.cu:553-588 (log_delta only feeds diagnostics / early stop)
+    for (int r = 0; r < M.rows; ++r) {
+        const int b = M.rp[r], e = M.rp[r + 1];
+        if (e - b <= 0) {
+            log_scale[r] = 0.0;
+            continue;
+        }
+        double acc = 0.0;
+        for (int k = b; k < e; ++k) {
+            double v = std::fabs(M.v[k]);
+            if (!std::isfinite(v) || v < 1e-300) v = 1e-300;
+            acc += -std::log(v) - other[M.ci[k]];
+        }
+        log_scale[r] = acc / (double)(e - b);
+    }
+}
+
+void cr_exp_to_scale(std::vector<double> &ls, std::vector<double> &inv, std::vector<double> &cum)
+{   // preconditioner.cu:611-625
+    for (size_t i = 0; i < ls.size(); ++i) {
+        double f = std::exp(ls[i]);
+        f = std::fmin(std::fmax(f, 1e-30), 1e30);
+        const double sc = 1.0 / f;
+        ls[i] = sc;
+        inv[i] = f;
+        cum[i] *= sc;
+    }
+}
+
+void rescale_problem(const cp_al_parameters_t *p, State &s)  // preconditioner.cu:379-475
+{
+    std::vector<double> e(s.m), d(s.n), inv_e(s.m), inv_d(s.n);
+    s.E.assign(s.m, 1.0);
+    s.D.assign(s.n, 1.0);
+
+    const char *cr_raw = getenv("CP_AL_CR_SCALING");
+    const bool cr_forced = cr_raw && cr_raw[0] != '\0' && atoi(cr_raw) != 0;
+    int cr_iters = cr_forced ? 20 : 0;
+    const char *cr_iters_raw = getenv("CP_AL_CR_ITERS");
+    if (cr_iters > 0 && cr_iters_raw && cr_iters_raw[0] != '\0') cr_iters = atoi(cr_iters_raw);
+    cr_iters = cr_iters < 0 ? 0 : cr_iters;
+    if (cr_iters > 0) {  // curtis_reid_rescaling, :162-268
+        std::fill(e.begin(), e.end(), 0.0);
+        std::fill(d.begin(), d.end(), 0.0);
+        for (int it = 0; it < cr_iters; ++it) {
+            cr_log_update(s.A, d, e);
+            cr_log_update(s.At, e, d);
+        }
+        cr_exp_to_scale(e, inv_e, s.E);
+        cr_exp_to_scale(d, inv_d, s.D);
+        scale_problem(s, e, d, inv_e, inv_d);
+    }
+    for (int it = 0; it < p->l_inf_ruiz_iterations; ++it) {  // ruiz_rescaling, :270-296
+        row_absmax(s.A, e);
+        clamp_sqrt_accum(e, inv_e, s.E);
+        row_absmax(s.At, d);
+        clamp_sqrt_accum(d, inv_d, s.D);
+        scale_problem(s, e, d, inv_e, inv_d);
+    }
+    if (p->has_pock_chambolle_alpha) {  // pock_chambolle_rescaling, :298-323
+        row_powsum(s.A, p->pock_chambolle_alpha, e);
+        clamp_sqrt_accum(e, inv_e, s.E);
+        row_powsum(s.At, 2.0 - p->pock_chambolle_alpha, d);
+        clamp_sqrt_accum(d, inv_d, s.D);
+        scale_problem(s, e, d, inv_e, inv_d);
+    }
+    s.b_scale = 1.0;
+    s.c_scale = 1.0;
+    if (p->bound_objective_rescaling) {  // bound_objective_rescaling, :325-377 + kernels :699-729
+        double sq = 0.0;
+        for (int i = 0; i < s.m; ++i) {
+            const double L = s.l[i], U = s.u[i];
+            const bool fL = std::isfinite(L), fU = std::isfinite(U);
+            double acc = 0.0;
+            if (fL && (!fU || std::fabs(L - U) > 1e-12)) acc += L * L;
+            if (fU) acc += U * U;
+            sq += acc;
+        }
+        const double bnd_norm = std::sqrt(sq);
+        const double obj_norm = nrm2(s.c);
+        const double cs = 1.0 / (bnd_norm + 1.0);
+        const double os = 1.0 / (obj_norm + 1.0);
+        for (int i = 0; i < s.m; ++i) {
+            s.l[i] *= cs;
+            s.u[i] *= cs;
+            s.y0[i] *= os;
+        }
+        for (int j = 0; j < s.n; ++j) {
+            s.lb[j] *= cs;
+            s.ub[j] *= cs;
+            s.c[j] *= os;
+            s.x0[j] *= cs;
+        }
+        s.b_scale = cs;
+        s.c_scale = os;
+    }
+}
+
+/* ---------------- utils.cu ---------------- */
+
+double estimate_max_singular_value(const State &s, int max_iter, double tol)  // utils.cu:87-228
+{
+    const int m = s.m;
+    std::vector<double> v(m), nv(m), w(s.n);
+    for (int i = 0; i < m; ++i) v[i] = g_dist(g_gen);
+    double sigma_sq = 1.0;
+    for (int it = 0; it < max_iter; ++it) {
+        nv = v;
+        const double inv = 1.0 / nrm2(nv);
+        for (double &a : nv) a *= inv;
+        spmv(s.At, nv.data(), w.data());
+        spmv(s.A, w.data(), v.data());
+        sigma_sq = dot(nv, v);
+        const double neg = -sigma_sq;
+        for (int i = 0; i < m; ++i) nv[i] = neg * nv[i] + v[i];
+        if (nrm2(nv) < tol) break;
+    }
+    return std::sqrt(sigma_sq);
+}
+
+void compute_residual(State &s, norm_type_t norm)  // utils.cu:812-917
+{
+    spmv(s.A, s.xT.data(), s.prim_prod.data());
+    spmv(s.At, s.yT.data(), s.dual_prod.data());
+    for (int i = 0; i < s.m; ++i) {  // compute_residual_kernel, :649-682
+        const double cl = std::fmax(s.l[i], std::fmin(s.prim_prod[i], s.u[i]));
+        s.primal_res[i] = (s.prim_prod[i] - cl) * s.E[i];
+        s.primal_slack[i] = std::fmax(s.yT[i], 0.0) * s.l_fin[i] + std::fmin(s.yT[i], 0.0) * s.u_fin[i];
+    }
+    for (int j = 0; j < s.n; ++j)
+        s.dual_res[j] = (s.c[j] - s.dual_prod[j] - s.dual_slack[j]) * s.D[j];
+
+    if (norm == NORM_TYPE_L_INF) {
+        s.abs_p = inf_norm(s.primal_res);
+        s.abs_d = inf_norm(s.dual_res);
+    } else {
+        s.abs_p = nrm2(s.primal_res) / s.b_scale;
+        s.abs_d = nrm2(s.dual_res);
+    }
+    s.abs_d /= s.c_scale;
+    const double bc = s.b_scale * s.c_scale;
+    s.p_obj = dot(s.c, s.xT) / bc + s.obj_const;
+    const double base = dot(s.dual_slack, s.xT);
+    double sum = 0.0;
+    for (double a : s.primal_slack) sum += a;
+    s.d_obj = (base + sum) / bc + s.obj_const;
+    s.rel_p = s.abs_p / (1.0 + s.b_norm);
+    s.rel_d = s.abs_d / (1.0 + s.c_norm);
+    s.gap = std::fabs(s.p_obj - s.d_obj);
+    s.rel_gap = s.gap / (1.0 + std::fabs(s.p_obj) + std::fabs(s.d_obj));
+}
+
+/* ---------------- solver.cu ---------------- */
+
+void refresh_step_schedule(State &s)  // solver.cu:3103-3126, alpha mode
+{
+    const double margin = env_double("CP_AL_STEP_MARGIN", 0.998 * 0.998);
+    s.al_lambda = std::fmin(std::fmax(s.al_lambda, 0.0), 0.95);
+    const double eta_sq = margin * (1.0 - s.al_lambda) / std::fmax(s.lambda_est, 1e-12);
+    s.step_size = std::sqrt(std::fmax(eta_sq, 1e-24));
+    const double dual_step = s.step_size * std::fmax(s.primal_weight, 1e-12);
+    s.al_penalty = dual_step * s.al_lambda / std::fmax(1.0 - s.al_lambda, 1e-12);
+}
+
+void iteration(State &s, int k_offset, double rho, bool major)
+{
+    const double tau = s.step_size / s.primal_weight;  // sync_step_sizes_to_gpu, solver.cu:1738-1749
+    const double sigma = s.step_size * s.primal_weight;
+    const double beta = s.al_penalty;
+    const int k = s.inner_count + k_offset;
+    const double w = (double)k / (double)(k + 1);
+
+    // compute_next_primal_solution, solver.cu:2523-2607
+    const double *dual_src = s.y.data();
+    if (beta > 0.0) {
+        spmv(s.A, s.x.data(), s.prim_prod.data());
+        for (int i = 0; i < s.m; ++i) {  // compute_box_al_predictor_kernel, :2459-2479
+            const double z = std::fmax(s.l[i], std::fmin(s.prim_prod[i] - s.y[i] / beta, s.u[i]));
+            s.yR[i] = s.y[i] - beta * (s.prim_prod[i] - z);
+        }
+        dual_src = s.yR.data();
+    }
+    spmv(s.At, dual_src, s.dual_prod.data());
+    for (int j = 0; j < s.n; ++j) {  // kernels :2340-2397
+        const double xj = s.x[j];
+        const double temp = xj - tau * (s.c[j] - s.dual_prod[j]);
+        const double proj = std::fmax(s.lb[j], std::fmin(temp, s.ub[j]));
+        if (major) {
+            s.xT[j] = proj;
+            s.dual_slack[j] = (proj - temp) / tau;
+            s.xH[j] = w * proj + (1.0 - w) * s.x0[j];
+        }
+        s.xR[j] = 2.0 * proj - xj;
+        const double refl = rho * s.xR[j] + (1.0 - rho) * xj;
+        s.x[j] = w * refl + (1.0 - w) * s.x0[j];
+    }
+
+    // compute_next_dual_solution, solver.cu:2609-2661
+    spmv(s.A, s.xR.data(), s.prim_prod.data());
+    const double prox = sigma + beta;
+    for (int i = 0; i < s.m; ++i) {  // kernels :2399-2457
+        const double yi = s.y[i];
+        const double pp = s.prim_prod[i];
+        const double zb = std::fmax(s.l[i], std::fmin(pp - yi / prox, s.u[i]));
+        const double yn = yi - sigma * (pp - zb);
+        if (major) {
+            s.yT[i] = yn;
+            s.yH[i] = w * yn + (1.0 - w) * s.y0[i];
+            s.yR[i] = 2.0 * yn - yi;
+        }
+        const double refl = rho * (2.0 * yn - yi) + (1.0 - rho) * yi;
+        s.y[i] = w * refl + (1.0 - w) * s.y0[i];
+    }
+}
+
+void compute_fixed_point_error(State &s)  // solver.cu:3611-3686 (non-sigma branch)
+{
+    for (int j = 0; j < s.n; ++j) s.dx[j] = s.xR[j] - s.xT[j];
+    for (int i = 0; i < s.m; ++i) s.dy[i] = s.yR[i] - s.yT[i];
+    spmv(s.At, s.dy.data(), s.dual_prod.data());
+    spmv(s.A, s.dx.data(), s.prim_prod.data());
+    const double dn = nrm2(s.dy), pn = nrm2(s.dx), an = nrm2(s.prim_prod);
+    const double movement = pn * pn * s.primal_weight + dn * dn / s.primal_weight;
+    const double cross = dot(s.dual_prod, s.dx);
+    const double interaction = 2 * s.step_size * cross;
+    const double aug = s.step_size * s.al_penalty * an * an;
+    s.fp = std::sqrt(std::fmax(movement + interaction - aug, 0.0));
+}
+
+bool should_restart(State &s, const restart_parameters_t &rp, int K)  // utils.cu:319-359
+{
+    const double suff = env_double("CP_AL_RESTART_SUFFICIENT", rp.sufficient_reduction_for_restart);
+    const double nec = env_double("CP_AL_RESTART_NECESSARY", rp.necessary_reduction_for_restart);
+    const double art = env_double("CP_AL_RESTART_ARTIFICIAL", rp.artificial_restart_threshold);
+    bool r = false;
+    if (s.total_count == K) {
+        r = true;
+    } else if (s.total_count > K) {
+        if (s.fp <= suff * s.fp0) r = true;
+        if (s.fp <= nec * s.fp0 && s.fp > s.last_trial_fp) r = true;
+        if (s.inner_count >= art * s.total_count) r = true;
+    }
+    s.last_trial_fp = s.fp;
+    return r;
+}
+
+void perform_restart(State &s, const cp_al_parameters_t *p)  // solver.cu:2821-3101
+{
+    double pd = 0.0, dd = 0.0;
+    for (int j = 0; j < s.n; ++j) {
+        const double t = s.xT[j] - s.x0[j];
+        pd += t * t;
+    }
+    for (int i = 0; i < s.m; ++i) {
+        const double t = s.yT[i] - s.y0[i];
+        dd += t * t;
+    }
+    pd = std::sqrt(pd);
+    dd = std::sqrt(dd);
+    const double ratio = s.rel_d / std::fmax(s.rel_p, 1e-300);
+    if (pd > 1e-16 && dd > 1e-16 && pd < 1e12 && dd < 1e12 && ratio > 1e-8 && ratio < 1e8) {
+        const double err = std::log(dd) - std::log(pd) - std::log(s.primal_weight);
+        s.pw_error_sum *= p->restart_params.i_smooth;
+        s.pw_error_sum += err;
+        const double de = err - s.pw_last_error;
+        s.primal_weight *= std::exp(p->restart_params.k_p * err + p->restart_params.k_i * s.pw_error_sum +
+                                    p->restart_params.k_d * de);
+        s.pw_last_error = err;
+    } else {
+        s.primal_weight = s.best_pw;
+        s.pw_error_sum = 0.0;
+        s.pw_last_error = 0.0;
+    }
+    const double g = std::fabs(std::log10(std::fmax(s.rel_d, 1e-300) / std::fmax(s.rel_p, 1e-300)));
+    if (g < s.best_gap) {
+        s.best_gap = g;
+        s.best_pw = s.primal_weight;
+    }
+    refresh_step_schedule(s);
+    s.x0 = s.xT;
+    s.x = s.xT;
+    s.y0 = s.yT;
+    s.y = s.yT;
+    s.inner_count = 0;
+    s.last_trial_fp = INFINITY;
+}
+
+void filter_matrix(lp_problem_t *pr, const cp_al_parameters_t *p)  // utils.cu:396-478
+{
+    const int rows = pr->num_constraints, nnz = pr->constraint_matrix_num_nonzeros;
+    if (rows == 0 || nnz == 0) return;
+    int kept = 0;
+    for (int k = 0; k < nnz; ++k)
+        if (std::fabs(pr->constraint_matrix_values[k]) > p->matrix_zero_tol) ++kept;
+    if (kept == nnz) return;
+    int *rp = (int *)malloc((size_t)(rows + 1) * sizeof(int));
+    int *ci = (int *)malloc((size_t)std::max(kept, 1) * sizeof(int));
+    double *v = (double *)malloc((size_t)std::max(kept, 1) * sizeof(double));
+    int pos = 0;
+    rp[0] = 0;
+    for (int i = 0; i < rows; ++i) {
+        for (int k = pr->constraint_matrix_row_pointers[i]; k < pr->constraint_matrix_row_pointers[i + 1]; ++k) {
+            const double a = pr->constraint_matrix_values[k];
+            if (std::fabs(a) <= p->matrix_zero_tol) continue;
+            ci[pos] = pr->constraint_matrix_col_indices[k];
+            v[pos] = a;
+            ++pos;
+        }
+        rp[i + 1] = pos;
+    }
+    free(pr->constraint_matrix_row_pointers);
+    free(pr->constraint_matrix_col_indices);
+    free(pr->constraint_matrix_values);
+    pr->constraint_matrix_row_pointers = rp;
+    pr->constraint_matrix_col_indices = ci;
+    pr->constraint_matrix_values = v;
+    pr->constraint_matrix_num_nonzeros = kept;
+}
+
+void init_state(State &s, const lp_problem_t *w, const cp_al_parameters_t *p)  // solver.cu:1795-2273
+{
+    s.n = w->num_variables;
+    s.m = w->num_constraints;
+    s.obj_const = w->objective_constant;
+    const int nnz = w->constraint_matrix_num_nonzeros;
+    s.A.rows = s.m;
+    s.A.cols = s.n;
+    s.A.rp.assign(w->constraint_matrix_row_pointers, w->constraint_matrix_row_pointers + s.m + 1);
+    s.A.ci.assign(w->constraint_matrix_col_indices, w->constraint_matrix_col_indices + nnz);
+    s.A.v.assign(w->constraint_matrix_values, w->constraint_matrix_values + nnz);
+    s.At = transpose(s.A);
+    s.c.assign(w->objective_vector, w->objective_vector + s.n);
+    s.lb.assign(w->variable_lower_bound, w->variable_lower_bound + s.n);
+    s.ub.assign(w->variable_upper_bound, w->variable_upper_bound + s.n);
+    s.l.assign(w->constraint_lower_bound, w->constraint_lower_bound + s.m);
+    s.u.assign(w->constraint_upper_bound, w->constraint_upper_bound + s.m);
+    for (auto *v : {&s.x0, &s.x, &s.xT, &s.xH, &s.xR, &s.dual_prod, &s.dual_slack, &s.dual_res, &s.dx})
+        v->assign(s.n, 0.0);
+    for (auto *v : {&s.y0, &s.y, &s.yT, &s.yH, &s.yR, &s.prim_prod, &s.primal_slack, &s.primal_res, &s.dy})
+        v->assign(s.m, 0.0);
+    if (w->primal_start) s.x0.assign(w->primal_start, w->primal_start + s.n);
+    if (w->dual_start) s.y0.assign(w->dual_start, w->dual_start + s.m);
+
+    const auto t0 = std::chrono::steady_clock::now();
+    rescale_problem(p, s);
+    s.rescaling_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+
+    s.x = s.x0;
+    s.y = s.y0;
+    s.xT = s.x0;
+    s.xH = s.x0;
+    s.yT = s.y0;
+    s.yH = s.y0;
+    s.l_fin.resize(s.m);
+    s.u_fin.resize(s.m);
+    for (int i = 0; i < s.m; ++i) {  // fill_finite_bounds_kernel, :2323-2338
+        s.l_fin[i] = std::isfinite(s.l[i]) ? s.l[i] : 0.0;
+        s.u_fin[i] = std::isfinite(s.u[i]) ? s.u[i] : 0.0;
+    }
+    // norms of the unscaled working problem, :1999-2067
+    const bool linf = p->optimality_norm == NORM_TYPE_L_INF;
+    double sq = 0.0, mx = 0.0;
+    for (int j = 0; j < s.n; ++j) {
+        const double cj = w->objective_vector[j];
+        if (linf) mx = std::max(mx, std::fabs(cj));
+        else sq += cj * cj;
+    }
+    s.c_norm = linf ? mx : std::sqrt(sq);
+    sq = 0.0;
+    mx = 0.0;
+    for (int i = 0; i < s.m; ++i) {
+        const double lo = w->constraint_lower_bound[i], hi = w->constraint_upper_bound[i];
+        if (linf) {
+            if (std::isfinite(lo) && lo != hi) mx = std::max(mx, std::fabs(lo));
+            if (std::isfinite(hi)) mx = std::max(mx, std::fabs(hi));
+        } else {
+            if (std::isfinite(lo) && lo != hi) sq += lo * lo;
+            if (std::isfinite(hi)) sq += hi * hi;
+        }
+    }
+    s.b_norm = linf ? mx : std::sqrt(sq);
+}
+
+}  // namespace
+
+/* ---------------- public C API (cp_al.c, utils.cu conversions) ---------------- */
+
+ORACLE_API void cp_al_oracle_reset_rng(void)
+{
+    g_gen.seed(1);
+    g_dist.reset();
+}
+
+ORACLE_API void set_default_parameters(cp_al_parameters_t *p)  // utils.cu:361-394
+{
+    p->l_inf_ruiz_iterations = 10;
+    p->has_pock_chambolle_alpha = true;
+    p->pock_chambolle_alpha = 1.0;
+    p->bound_objective_rescaling = true;
+    p->verbose = false;
+    p->termination_evaluation_frequency = 200;
+    p->feasibility_polishing = false;
+    p->reflection_coefficient = env_double("CP_AL_REFLECTION_COEFFICIENT", 1.0);
+    p->reflection_coefficient = std::fmax(0.0, std::fmin(p->reflection_coefficient, 2.0));
+    p->sv_max_iter = 5000;
+    p->sv_tol = 1e-4;
+    p->termination_criteria.eps_optimal_relative = 1e-4;
+    p->termination_criteria.eps_feasible_relative = 1e-4;
+    p->termination_criteria.time_sec_limit = 3600.0;
+    p->termination_criteria.iteration_limit = INT32_MAX;
+    p->termination_criteria.eps_feas_polish_relative = 1e-6;
+    p->restart_params.artificial_restart_threshold = 0.36;
+    p->restart_params.sufficient_reduction_for_restart = 0.2;
+    p->restart_params.necessary_reduction_for_restart = 0.5;
+    p->restart_params.k_p = 0.99;
+    p->restart_params.k_i = 0.01;
+    p->restart_params.k_d = 0.0;
+    p->restart_params.i_smooth = 0.3;
+    p->optimality_norm = NORM_TYPE_L2;
+    p->presolve = true;
+    p->matrix_zero_tol = 1e-9;
+}
+
+static void fill_or_copy(double **dst, int n, const double *src, double fill)  // utils.cu:1032-1040
+{
+    *dst = (double *)malloc((size_t)std::max(n, 1) * sizeof(double));
+    if (src) memcpy(*dst, src, (size_t)n * sizeof(double));
+    else
+        for (int i = 0; i < n; ++i) (*dst)[i] = fill;
+}
+
+ORACLE_API lp_problem_t *create_lp_problem(const double *c, const matrix_desc_t *A, const double *con_lb,
+                                           const double *con_ub, const double *var_lb, const double *var_ub,
+                                           const double *obj_const)
+{   // cp_al.c:26-116, conversions utils.cu:1042-1192
+    if (!A) return nullptr;
+    lp_problem_t *p = (lp_problem_t *)calloc(1, sizeof(lp_problem_t));
+    const int m = A->m, n = A->n;
+    p->num_variables = n;
+    p->num_constraints = m;
+    int *rp = (int *)calloc((size_t)m + 1, sizeof(int));
+    int *ci = nullptr;
+    double *v = nullptr;
+    int nnz = 0;
+    switch (A->fmt) {
+    case matrix_dense: {
+        for (long i = 0; i < (long)m * n; ++i)
+            if (std::fabs(A->data.dense.A[i]) > 0.0) ++nnz;
+        ci = (int *)malloc((size_t)std::max(nnz, 1) * sizeof(int));
+        v = (double *)malloc((size_t)std::max(nnz, 1) * sizeof(double));
+        int nz = 0;
+        for (int i = 0; i < m; ++i) {
+            rp[i] = nz;
+            for (int j = 0; j < n; ++j) {
+                const double a = A->data.dense.A[(long)i * n + j];
+                if (std::fabs(a) > 0.0) {
+                    ci[nz] = j;
+                    v[nz] = a;
+                    ++nz;
+                }
+            }
+        }
+        rp[m] = nz;
+        break;
+    }
+    case matrix_csr:
+        nnz = A->data.csr.nnz;
+        ci = (int *)malloc((size_t)std::max(nnz, 1) * sizeof(int));
+        v = (double *)malloc((size_t)std::max(nnz, 1) * sizeof(double));
+        memcpy(rp, A->data.csr.row_ptr, (size_t)(m + 1) * sizeof(int));
+        memcpy(ci, A->data.csr.col_ind, (size_t)nnz * sizeof(int));
+        memcpy(v, A->data.csr.vals, (size_t)nnz * sizeof(double));
+        break;
+    case matrix_csc:
+    case matrix_coo: {
+        const bool csc = A->fmt == matrix_csc;
+        nnz = csc ? A->data.csc.nnz : A->data.coo.nnz;
+        // row index of each stored entry, then a stable counting sort by row
+        std::vector<int> rows(nnz), cols(nnz);
+        if (csc) {
+            for (int j = 0; j < n; ++j)
+                for (int k = A->data.csc.col_ptr[j]; k < A->data.csc.col_ptr[j + 1]; ++k) {
+                    rows[k] = A->data.csc.row_ind[k];
+                    cols[k] = j;
+                }
+        } else {
+            for (int k = 0; k < nnz; ++k) {
+                rows[k] = A->data.coo.row_ind[k];
+                cols[k] = A->data.coo.col_ind[k];
+            }
+        }
+        for (int k = 0; k < nnz; ++k)
+            if (rows[k] < 0 || rows[k] >= m || cols[k] < 0 || cols[k] >= n) {
+                fprintf(stderr, "[interface] %s: index out of range\n", csc ? "CSC" : "COO");
+                free(rp);
+                free(p);
+                return nullptr;
+            }
+        const double *vals = csc ? A->data.csc.vals : A->data.coo.vals;
+        for (int k = 0; k < nnz; ++k) rp[rows[k] + 1]++;
+        for (int i = 0; i < m; ++i) rp[i + 1] += rp[i];
+        ci = (int *)malloc((size_t)std::max(nnz, 1) * sizeof(int));
+        v = (double *)malloc((size_t)std::max(nnz, 1) * sizeof(double));
+        std::vector<int> next(rp, rp + m);
+        for (int k = 0; k < nnz; ++k) {
+            const int pos = next[rows[k]]++;
+            ci[pos] = cols[k];
+            v[pos] = vals[k];
+        }
+        break;
+    }
+    default:
+        fprintf(stderr, "[interface] unsupported matrix format %d.\n", (int)A->fmt);
+        free(rp);
+        free(p);
+        return nullptr;
+    }
+    p->constraint_matrix_row_pointers = rp;
+    p->constraint_matrix_col_indices = ci;
+    p->constraint_matrix_values = v;
+    p->constraint_matrix_num_nonzeros = nnz;
+    p->objective_constant = obj_const ? *obj_const : 0.0;
+    fill_or_copy(&p->objective_vector, n, c, 0.0);
+    fill_or_copy(&p->variable_lower_bound, n, var_lb, -INFINITY);
+    fill_or_copy(&p->variable_upper_bound, n, var_ub, INFINITY);
+    fill_or_copy(&p->constraint_lower_bound, m, con_lb, -INFINITY);
+    fill_or_copy(&p->constraint_upper_bound, m, con_ub, INFINITY);
+    return p;
+}
+
+ORACLE_API void set_start_values(lp_problem_t *p, const double *primal, const double *dual)  // cp_al.c:146-178
+{
+    if (!p) return;
+    free(p->primal_start);
+    free(p->dual_start);
+    p->primal_start = nullptr;
+    p->dual_start = nullptr;
+    if (primal) {
+        p->primal_start = (double *)malloc((size_t)std::max(p->num_variables, 1) * sizeof(double));
+        memcpy(p->primal_start, primal, (size_t)p->num_variables * sizeof(double));
+    }
+    if (dual) {
+        p->dual_start = (double *)malloc((size_t)std::max(p->num_constraints, 1) * sizeof(double));
+        memcpy(p->dual_start, dual, (size_t)p->num_constraints * sizeof(double));
+    }
+}
+
+ORACLE_API void cp_al_result_free(cp_al_result_t *r)
+{
+    if (!r) return;
+    free(r->primal_solution);
+    free(r->dual_solution);
+    free(r->reduced_cost);
+    free(r);
+}
+
+ORACLE_API void lp_problem_free(lp_problem_t *p)
+{
+    if (!p) return;
+    free(p->constraint_matrix_row_pointers);
+    free(p->constraint_matrix_col_indices);
+    free(p->constraint_matrix_values);
+    free(p->variable_lower_bound);
+    free(p->variable_upper_bound);
+    free(p->objective_vector);
+    free(p->constraint_lower_bound);
+    free(p->constraint_upper_bound);
+    free(p->primal_start);
+    free(p->dual_start);
+    free(p);
+}
+
+ORACLE_API cp_al_result_t *optimize(const cp_al_parameters_t *params, lp_problem_t *prob)
+{   // solver.cu:1175-1736
+    if (!params || !prob) return nullptr;
+    if (params->termination_evaluation_frequency < 3) {
+        fprintf(stderr, "Error: termination_evaluation_frequency must be >= 3 (got %d).\n",
+                params->termination_evaluation_frequency);
+        return nullptr;
+    }
+    if (params->verbose) filter_matrix(prob, params);
+
+    State s;
+    init_state(s, prob, params);
+
+    // initialize_step_size_and_primal_weight, solver.cu:3538-3609
+    s.lambda_est = 1.0;
+    if (s.A.rp[s.m] != 0) {
+        const double sv = estimate_max_singular_value(s, params->sv_max_iter, params->sv_tol);
+        s.lambda_est = sv * sv;
+    }
+    s.primal_weight = params->bound_objective_rescaling ? 1.0 : (s.c_norm + 1.0) / (s.b_norm + 1.0);
+    s.al_lambda = env_double("CP_AL_LAMBDA", 0.05);
+    refresh_step_schedule(s);
+    s.best_pw = s.primal_weight;
+
+    const auto t_start = std::chrono::steady_clock::now();
+    int charged = s.total_count;
+    bool do_restart = false;
+    const int K = params->termination_evaluation_frequency;
+    const double rho = params->reflection_coefficient;
+    while (charged < params->termination_criteria.iteration_limit) {
+        if (do_restart) {
+            compute_fixed_point_error(s);
+            s.fp0 = s.fp;
+            do_restart = false;
+        }
+        for (int i = 1; i <= K; ++i) iteration(s, i, rho, i == 1 || i == K);
+        compute_fixed_point_error(s);
+        compute_residual(s, params->optimality_norm);
+        s.inner_count += K;
+        s.total_count += K;
+        charged += K;
+
+        // check_termination_criteria, utils.cu:299-317
+        s.solve_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+        const auto &tc = params->termination_criteria;
+        if (s.rel_d < tc.eps_feasible_relative && s.rel_p < tc.eps_feasible_relative &&
+            s.rel_gap < tc.eps_optimal_relative)
+            s.reason = TERMINATION_REASON_OPTIMAL;
+        else if (s.total_count >= tc.iteration_limit)
+            s.reason = TERMINATION_REASON_ITERATION_LIMIT;
+        else if (s.solve_time >= tc.time_sec_limit)
+            s.reason = TERMINATION_REASON_TIME_LIMIT;
+        if (s.reason == TERMINATION_REASON_UNSPECIFIED && charged >= tc.iteration_limit)
+            s.reason = TERMINATION_REASON_ITERATION_LIMIT;
+        if (s.reason != TERMINATION_REASON_UNSPECIFIED) break;
+
+        do_restart = should_restart(s, params->restart_params, K);
+        if (do_restart) perform_restart(s, params);
+    }
+    if (s.reason == TERMINATION_REASON_UNSPECIFIED) {
+        s.reason = TERMINATION_REASON_ITERATION_LIMIT;
+        compute_residual(s, params->optimality_norm);
+    }
+
+    // create_result_from_state, solver.cu:3828-3909
+    cp_al_result_t *r = (cp_al_result_t *)calloc(1, sizeof(cp_al_result_t));
+    spmv(s.At, s.yT.data(), s.dual_prod.data());
+    r->primal_solution = (double *)malloc((size_t)std::max(s.n, 1) * sizeof(double));
+    r->dual_solution = (double *)malloc((size_t)std::max(s.m, 1) * sizeof(double));
+    r->reduced_cost = (double *)malloc((size_t)std::max(s.n, 1) * sizeof(double));
+    for (int j = 0; j < s.n; ++j) {
+        double rc = (s.c[j] - s.dual_prod[j]) * s.D[j] / s.c_scale;
+        if (!std::isfinite(s.lb[j])) rc = std::fmin(rc, 0.0);
+        if (!std::isfinite(s.ub[j])) rc = std::fmax(rc, 0.0);
+        r->reduced_cost[j] = rc;
+        r->primal_solution[j] = s.xT[j] / s.D[j] / s.b_scale;
+    }
+    for (int i = 0; i < s.m; ++i) r->dual_solution[i] = s.yT[i] / s.E[i] / s.c_scale;
+    r->num_variables = prob->num_variables;
+    r->num_constraints = prob->num_constraints;
+    r->num_nonzeros = prob->constraint_matrix_num_nonzeros;
+    r->total_count = charged;
+    r->rescaling_time_sec = s.rescaling_time;
+    r->cumulative_time_sec = s.solve_time;
+    r->absolute_primal_residual = s.abs_p;
+    r->relative_primal_residual = s.rel_p;
+    r->absolute_dual_residual = s.abs_d;
+    r->relative_dual_residual = s.rel_d;
+    r->primal_objective_value = s.p_obj;
+    r->dual_objective_value = s.d_obj;
+    r->objective_gap = s.gap;
+    r->relative_objective_gap = s.rel_gap;
+    r->termination_reason = s.reason;
+    return r;
+}
+
+ORACLE_API cp_al_result_t *solve_lp_problem(lp_problem_t *prob, const cp_al_parameters_t *params)
+{   // cp_al.c:180-209
+    if (!prob) {
+        fprintf(stderr, "[interface] solve_lp_problem: invalid arguments.\n");
+        return nullptr;
+    }
+    cp_al_parameters_t local;
+    if (params) local = *params;
+    else set_default_parameters(&local);
+    return optimize(&local, prob);
+}

@@ -1,0 +1,90 @@
+"""Build the native libraries of pdal_amd in-tree.
+
+Products (HIP, gfx950):
+  pdal_amd/lib/libcp_al.so   -- CP_AL LP solver behind include/cp_al.h
+  pdal_amd/lib/cp_al         -- the LP runner command line
+Checkers (CPU, gcc), built by oracle/build_oracle.py, are separate.
+
+Run `python -m pdal_amd.build` (or __graft_entry__.build()).
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent.parent
+CSRC = ROOT / "pdal_amd" / "csrc"
+LIB = ROOT / "pdal_amd" / "lib"
+INCLUDE = ROOT / "include"
+
+ARCH = os.environ.get("PDAL_OFFLOAD_ARCH", "gfx950")
+
+# Packed-FP32 instructions are kept out of every HIP build.
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-ffp-contract=off",
+    "-fvisibility=hidden",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+    f"-I{INCLUDE}",
+]
+
+CP_AL_SOURCES = [
+    CSRC / "lp" / "cp_al_api.cpp",
+    CSRC / "lp" / "cp_al_solver.hip",
+    CSRC / "lp" / "mps_parser.cpp",
+]
+
+
+def _hipcc() -> str:
+    exe = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    return exe
+
+
+def _run(cmd: list[str]) -> None:
+    print("+", " ".join(str(c) for c in cmd), flush=True)
+    subprocess.run([str(c) for c in cmd], check=True)
+
+
+def _compile_objects(sources, outdir: Path, extra=()) -> list[Path]:
+    outdir.mkdir(parents=True, exist_ok=True)
+    objs = []
+    for src in sources:
+        obj = outdir / (src.stem + src.suffix.replace(".", "_") + ".o")
+        lang = ["-x", "hip"] if src.suffix == ".hip" else ["-x", "c++"]
+        _run([_hipcc(), *HIP_FLAGS, *extra, *lang, "-c", src, "-o", obj])
+        objs.append(obj)
+    return objs
+
+
+def build_cp_al() -> Path:
+    objdir = ROOT / "pdal_amd" / "_build" / "cp_al"
+    objs = _compile_objects(CP_AL_SOURCES, objdir)
+    LIB.mkdir(parents=True, exist_ok=True)
+    out = LIB / "libcp_al.so"
+    tmp = LIB / "libcp_al.so.tmp"
+    _run([_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-Bsymbolic", *objs, "-lz",
+          "-o", tmp])
+    os.replace(tmp, out)
+    # LP runner CLI linked against the shared library
+    cli_src = CSRC / "lp" / "cli.cpp"
+    if cli_src.exists():
+        exe = LIB / "cp_al"
+        _run(["g++", "-O2", "-std=c++17", f"-I{INCLUDE}", cli_src, f"-L{LIB}", "-lcp_al",
+              f"-Wl,-rpath,$ORIGIN", "-o", exe])
+    return out
+
+
+def build_all() -> None:
+    build_cp_al()
+
+
+if __name__ == "__main__":
+    build_all()
+    sys.exit(0)

@@ -1,0 +1,34 @@
+// Internal declarations shared by the translation units of libcp_al.so.
+#pragma once
+
+#include "cp_al_types.h"
+
+namespace pdal {
+namespace lp {
+
+// Environment helpers with the reference's parsing rules (solver.cu:377-413).
+double env_double(const char *name, double def);
+int env_int(const char *name, int def);
+bool env_set(const char *name);
+
+// Warn once per solve about reference controllers this build does not run.
+void warn_unsupported_controllers();
+
+// Plain-path solve on the current HIP device (throws pdal::HipError).
+cp_al_result_t *optimize_gpu(const cp_al_parameters_t *params, lp_problem_t *problem);
+
+// Reseed the process-global power-iteration generator to mt19937(1).
+void reset_rng();
+
+// Set up the device state and time `iters` iterations kernel by kernel.
+int time_kernels_gpu(const cp_al_parameters_t *params, const lp_problem_t *problem, int iters, double *out);
+
+// Device-resident session: set up once, solve many times (pdal_lp.h).
+void *session_create(const cp_al_parameters_t *params, lp_problem_t *problem);
+cp_al_result_t *session_solve(void *session);
+int session_time_kernels(void *session, int iters, double *out);
+double session_setup_seconds(void *session);
+void session_free(void *session);
+
+}  // namespace lp
+}  // namespace pdal

@@ -1,0 +1,898 @@
+// CP_AL optimize(): host driver of the MI355X plain path.
+//
+// Reference: LP/CP_AL/src/solver.cu:1175-1736 (main loop), :1795-2273 (state
+// set-up), :3538-3609 (step size and primal weight), :2821-3126 (restart and
+// step schedule), :3611-3686 (fixed-point error), :3828-3909 (result);
+// LP/CP_AL/src/utils.cu:87-228 (power iteration), :275-359 (termination and
+// restart tests), :812-917 (residual); LP/CP_AL/src/preconditioner.cu (scaling).
+//
+// MI355X design: one HIP stream; a block of K = eval_freq iterations plus the
+// evaluation pass is captured once into a hipGraph and replayed every block.
+// Each iteration is two launches (three with the AL predictor): the At.y SpMV
+// fused with the primal update and the A.x_bar SpMV fused with the dual update.
+// The evaluation pass is two dual-RHS SpMVs with fused deterministic
+// reductions; the host reads 13 doubles per block (one synchronisation per
+// eval_freq iterations) and runs the scalar restart / PID / termination logic,
+// which is a handful of flops.
+#include "cp_al.h"
+#include "cp_al_internal.hpp"
+
+#include "../common/hip_util.hpp"
+#include "cp_al_kernels.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <vector>
+
+namespace pdal {
+namespace lp {
+
+namespace {
+
+std::mt19937 g_gen(1);                              // utils.cu:27 (process-global)
+std::normal_distribution<double> g_dist(0.0, 1.0);  // utils.cu:28
+
+inline int blocks_for(long work) { return work > 0 ? (int)((work + kBlock - 1) / kBlock) : 1; }
+
+struct HostCsr {
+    int rows = 0, cols = 0;
+    std::vector<int> rp, ci;
+    std::vector<double> v;
+};
+
+// Stable CSR transpose: rows stay ascending inside each column, the order the
+// reference gets from cusparseCsr2cscEx2 (solver.cu:1859-1890).
+HostCsr transpose_csr(int m, int n, const int *rp, const int *ci, const double *v)
+{
+    HostCsr T;
+    T.rows = n;
+    T.cols = m;
+    const int nnz = rp[m];
+    T.rp.assign((size_t)n + 1, 0);
+    for (int k = 0; k < nnz; ++k) T.rp[ci[k] + 1]++;
+    for (int j = 0; j < n; ++j) T.rp[j + 1] += T.rp[j];
+    T.ci.resize(nnz);
+    T.v.resize(nnz);
+    std::vector<int> next(T.rp.begin(), T.rp.end() - 1);
+    for (int i = 0; i < m; ++i)
+        for (int k = rp[i]; k < rp[i + 1]; ++k) {
+            const int p = next[ci[k]]++;
+            T.ci[p] = i;
+            T.v[p] = v[k];
+        }
+    return T;
+}
+
+struct DevCsr {
+    DBuf<int> rp, ci;
+    DBuf<double> v;
+    int rows = 0, nnz = 0, G = 4;
+    CsrView view() const { return CsrView{rp.get(), ci.get(), v.get(), rows}; }
+};
+
+int pick_group(const char *env, long nnz, int rows)
+{   // lanes per row: a power of two giving about two nonzeros per lane
+    if (const char *raw = getenv(env)) {
+        const int g = atoi(raw);
+        if (g == 1 || g == 2 || g == 4 || g == 8 || g == 16 || g == 32 || g == 64) return g;
+    }
+    const double mean = rows > 0 ? (double)nnz / rows : 0.0;
+    int g = 1;
+    while (g < 64 && 2.0 * g <= mean / 2.0) g *= 2;
+    return g;
+}
+
+struct Eval {
+    double cross, pn2, dres2, cx, lx, rx2, dresmax;
+    double ax2, dn2, pres2, contrib, ry2, presmax;
+};
+
+class Solver {
+  public:
+    Solver(const cp_al_parameters_t *p, const lp_problem_t *w) : P_(*p), W_(w) {}
+    ~Solver()
+    {
+        if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+        if (stream_) (void)hipStreamDestroy(stream_);
+    }
+    cp_al_result_t *run(const lp_problem_t *original);
+
+    // Session use: set up once, then solve repeatedly from the start point.
+    void setup();
+    cp_al_result_t *solve_once(const lp_problem_t *original);
+    void time_kernels(int iters, double *out);
+    double setup_seconds() const { return setup_time_; }
+    void reset_iterates();
+
+  private:
+    // set-up
+    void upload();
+    void rescale();
+    double power_iteration();
+    void init_steps();
+    // iteration
+    void launch_iteration(int k_offset, bool major);
+    void launch_eval();
+    void launch_block(int K);
+    void record_graph(int K);
+    void sync_scalars();
+    Eval read_eval();
+    void fixed_point_error(const Eval &e);
+    void residual(const Eval &e);
+    bool should_restart(int K);
+    void perform_restart(const Eval &e);
+    void refresh_step_schedule();
+    cp_al_result_t *make_result(const lp_problem_t *original, int charged);
+    void display() const;
+
+    cp_al_parameters_t P_;
+    const lp_problem_t *W_;
+    hipStream_t stream_ = nullptr;
+    int n_ = 0, m_ = 0;
+    long nnz_ = 0;
+    double obj_const_ = 0.0;
+    DevCsr A_, At_;
+    DBuf<double> c_, lb_, ub_, l_, u_, E_, D_;
+    DBuf<double> x0_, x_, xT_, xH_, xR_, dslack_;
+    DBuf<double> y0_, y_, yT_, yH_, yR_;
+    DBuf<double> x_start_, y_start_;  // scaled start point, kept for repeated solves
+    DBuf<double> parts_at_, parts_a_, evalq_;
+    DBuf<IterScalars> sc_;
+    int eval_grid_at_ = 1, eval_grid_a_ = 1;
+    PinnedBuf<double> h_eval_{16};
+    PinnedBuf<IterScalars> h_sc_{1};
+    hipGraphExec_t graph_exec_ = nullptr;
+    int graph_K_ = 0;
+    bool graph_pred_ = false;
+    bool use_graph_ = true;
+
+    double b_scale_ = 1.0, c_scale_ = 1.0, c_norm_ = 0.0, b_norm_ = 0.0;
+    double step_size_ = 0.0, pw_ = 1.0, al_lambda_ = 0.0, al_penalty_ = 0.0, lambda_est_ = 1.0;
+    double pw_err_sum_ = 0.0, pw_last_err_ = 0.0, best_pw_ = 1.0, best_gap_ = INFINITY;
+    int total_ = 0, inner_ = 0;
+    double fp_ = 0.0, fp0_ = 0.0, last_trial_fp_ = INFINITY;
+    double abs_p_ = 0, rel_p_ = 0, abs_d_ = 0, rel_d_ = 0, p_obj_ = 0, d_obj_ = 0, gap_ = 0, rel_gap_ = 0;
+    termination_reason_t reason_ = TERMINATION_REASON_UNSPECIFIED;
+    double rescale_time_ = 0.0, solve_time_ = 0.0, setup_time_ = 0.0;
+    double pw_init_ = 1.0;
+    Eval last_eval_{};
+};
+
+void Solver::upload()
+{
+    const lp_problem_t *w = W_;
+    n_ = w->num_variables;
+    m_ = w->num_constraints;
+    nnz_ = w->constraint_matrix_num_nonzeros;
+    obj_const_ = w->objective_constant;
+    PDAL_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    use_graph_ = !(getenv("PDAL_NO_GRAPH") && atoi(getenv("PDAL_NO_GRAPH")) != 0);
+
+    HostCsr T = transpose_csr(m_, n_, w->constraint_matrix_row_pointers, w->constraint_matrix_col_indices,
+                              w->constraint_matrix_values);
+    auto put_csr = [&](DevCsr &D, int rows, const int *rp, const int *ci, const double *v, const char *env) {
+        D.rows = rows;
+        D.nnz = (int)nnz_;
+        D.rp.alloc((size_t)rows + 1);
+        D.ci.alloc(nnz_);
+        D.v.alloc(nnz_);
+        D.rp.upload(rp, (size_t)rows + 1, stream_);
+        D.ci.upload(ci, nnz_, stream_);
+        D.v.upload(v, nnz_, stream_);
+        D.G = pick_group(env, nnz_, rows);
+    };
+    put_csr(A_, m_, w->constraint_matrix_row_pointers, w->constraint_matrix_col_indices,
+            w->constraint_matrix_values, "PDAL_SPMV_G_A");
+    put_csr(At_, n_, T.rp.data(), T.ci.data(), T.v.data(), "PDAL_SPMV_G_AT");
+
+    for (auto *b : {&c_, &lb_, &ub_, &D_, &x0_, &x_, &xT_, &xH_, &xR_, &dslack_}) b->alloc(n_);
+    for (auto *b : {&l_, &u_, &E_, &y0_, &y_, &yT_, &yH_, &yR_}) b->alloc(m_);
+    c_.upload(w->objective_vector, n_, stream_);
+    lb_.upload(w->variable_lower_bound, n_, stream_);
+    ub_.upload(w->variable_upper_bound, n_, stream_);
+    l_.upload(w->constraint_lower_bound, m_, stream_);
+    u_.upload(w->constraint_upper_bound, m_, stream_);
+    for (auto *b : {&x0_, &x_, &xT_, &xH_, &xR_, &dslack_})
+        PDAL_HIP(hipMemsetAsync(b->get(), 0, (size_t)n_ * sizeof(double), stream_));
+    for (auto *b : {&y0_, &y_, &yT_, &yH_, &yR_})
+        PDAL_HIP(hipMemsetAsync(b->get(), 0, (size_t)m_ * sizeof(double), stream_));
+    if (w->primal_start) x0_.upload(w->primal_start, n_, stream_);
+    if (w->dual_start) y0_.upload(w->dual_start, m_, stream_);
+
+    // evaluation partials: fixed grids so reductions are deterministic
+    eval_grid_at_ = std::max(1, std::min(2048, blocks_for((long)n_ * At_.G)));
+    eval_grid_a_ = std::max(1, std::min(2048, blocks_for((long)m_ * A_.G)));
+    parts_at_.alloc((size_t)eval_grid_at_ * kEvalStride);
+    parts_a_.alloc((size_t)eval_grid_a_ * kEvalStride);
+    evalq_.alloc(16);
+    sc_.alloc(1);
+    PDAL_HIP(hipMemsetAsync(sc_.get(), 0, sizeof(IterScalars), stream_));
+    PDAL_HIP(hipStreamSynchronize(stream_));  // host CSR copies are released below
+}
+
+void Solver::rescale()
+{   // rescale_problem, preconditioner.cu:379-475
+    const auto t0 = std::chrono::steady_clock::now();
+    DBuf<double> e(m_), d(n_), inv_e(m_), inv_d(n_);
+    fill_kernel<<<blocks_for(m_), kBlock, 0, stream_>>>(E_.get(), 1.0, m_);
+    fill_kernel<<<blocks_for(n_), kBlock, 0, stream_>>>(D_.get(), 1.0, n_);
+    PDAL_LAUNCHED();
+    auto scale_problem = [&]() {
+        scale_vars_kernel<<<blocks_for(n_), kBlock, 0, stream_>>>(c_.get(), lb_.get(), ub_.get(), x0_.get(), d.get(),
+                                                                 inv_d.get(), n_);
+        scale_cons_kernel<<<blocks_for(m_), kBlock, 0, stream_>>>(l_.get(), u_.get(), y0_.get(), e.get(),
+                                                                 inv_e.get(), m_);
+        scale_rows_kernel<<<blocks_for(m_), kBlock, 0, stream_>>>(A_.rp.get(), A_.ci.get(), A_.v.get(), m_,
+                                                                 inv_e.get(), inv_d.get());
+        scale_rows_kernel<<<blocks_for(n_), kBlock, 0, stream_>>>(At_.rp.get(), At_.ci.get(), At_.v.get(), n_,
+                                                                 inv_d.get(), inv_e.get());
+        PDAL_LAUNCHED();
+    };
+
+    const char *cr_raw = getenv("CP_AL_CR_SCALING");
+    const bool cr_forced = cr_raw && cr_raw[0] != '\0' && atoi(cr_raw) != 0;
+    int cr_iters = cr_forced ? 20 : 0;
+    const char *cr_iters_raw = getenv("CP_AL_CR_ITERS");
+    if (cr_iters > 0 && cr_iters_raw && cr_iters_raw[0] != '\0') cr_iters = atoi(cr_iters_raw);
+    cr_iters = std::max(cr_iters, 0);
+    if (cr_iters > 0) {  // curtis_reid_rescaling, :162-268
+        if (P_.verbose) printf("  Curtis-Reid scaling (%d iterations)\n", cr_iters);
+        PDAL_HIP(hipMemsetAsync(e.get(), 0, (size_t)m_ * sizeof(double), stream_));
+        PDAL_HIP(hipMemsetAsync(d.get(), 0, (size_t)n_ * sizeof(double), stream_));
+        for (int it = 0; it < cr_iters; ++it) {
+            cr_log_update_kernel<<<blocks_for(m_), kBlock, 0, stream_>>>(A_.view(), d.get(), e.get());
+            cr_log_update_kernel<<<blocks_for(n_), kBlock, 0, stream_>>>(At_.view(), e.get(), d.get());
+        }
+        cr_exp_kernel<<<blocks_for(m_), kBlock, 0, stream_>>>(e.get(), inv_e.get(), E_.get(), m_);
+        cr_exp_kernel<<<blocks_for(n_), kBlock, 0, stream_>>>(d.get(), inv_d.get(), D_.get(), n_);
+        PDAL_LAUNCHED();
+        scale_problem();
+    }
+    if (P_.l_inf_ruiz_iterations > 0 && P_.verbose)
+        printf("  Ruiz scaling (%d iterations)\n", P_.l_inf_ruiz_iterations);
+    for (int it = 0; it < P_.l_inf_ruiz_iterations; ++it) {  // ruiz_rescaling, :270-296
+        row_absmax_kernel<<<blocks_for(m_), kBlock, 0, stream_>>>(A_.view(), e.get());
+        clamp_sqrt_accum_kernel<<<blocks_for(m_), kBlock, 0, stream_>>>(e.get(), inv_e.get(), E_.get(), m_);
+        row_absmax_kernel<<<blocks_for(n_), kBlock, 0, stream_>>>(At_.view(), d.get());
+        clamp_sqrt_accum_kernel<<<blocks_for(n_), kBlock, 0, stream_>>>(d.get(), inv_d.get(), D_.get(), n_);
+        PDAL_LAUNCHED();
+        scale_problem();
+    }
+    if (P_.has_pock_chambolle_alpha) {  // pock_chambolle_rescaling, :298-323
+        if (P_.verbose) printf("  Pock-Chambolle scaling (alpha=%.4f)\n", P_.pock_chambolle_alpha);
+        row_powsum_kernel<<<blocks_for(m_), kBlock, 0, stream_>>>(A_.view(), P_.pock_chambolle_alpha, e.get());
+        clamp_sqrt_accum_kernel<<<blocks_for(m_), kBlock, 0, stream_>>>(e.get(), inv_e.get(), E_.get(), m_);
+        row_powsum_kernel<<<blocks_for(n_), kBlock, 0, stream_>>>(At_.view(), 2.0 - P_.pock_chambolle_alpha,
+                                                                  d.get());
+        clamp_sqrt_accum_kernel<<<blocks_for(n_), kBlock, 0, stream_>>>(d.get(), inv_d.get(), D_.get(), n_);
+        PDAL_LAUNCHED();
+        scale_problem();
+    }
+    b_scale_ = 1.0;
+    c_scale_ = 1.0;
+    if (P_.bound_objective_rescaling) {  // bound_objective_rescaling, :325-377
+        if (P_.verbose) printf("  Bound-objective scaling\n");
+        const int grid = std::max(1, std::min(1024, blocks_for(std::max(m_, n_))));
+        DBuf<double> parts((size_t)grid * 2), out(2);
+        bound_contrib_partials_kernel<<<grid, kBlock, 0, stream_>>>(l_.get(), u_.get(), m_, c_.get(), n_,
+                                                                    parts.get());
+        reduce_partials_kernel<<<1, kBlock, 0, stream_>>>(parts.get(), grid, 2, 2, 2, out.get());
+        PDAL_LAUNCHED();
+        double h[2];
+        out.download(h, 2, stream_);
+        PDAL_HIP(hipStreamSynchronize(stream_));
+        const double cs = 1.0 / (std::sqrt(h[0]) + 1.0);
+        const double os = 1.0 / (std::sqrt(h[1]) + 1.0);
+        bound_obj_scale_kernel<<<blocks_for((long)m_ + n_), kBlock, 0, stream_>>>(
+            l_.get(), u_.get(), y0_.get(), m_, lb_.get(), ub_.get(), c_.get(), x0_.get(), n_, cs, os);
+        PDAL_LAUNCHED();
+        b_scale_ = cs;
+        c_scale_ = os;
+    }
+    PDAL_HIP(hipStreamSynchronize(stream_));
+    rescale_time_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+template <class F>
+void dispatch_group(int G, F &&f)
+{
+    switch (G) {
+    case 1: f(std::integral_constant<int, 1>{}); break;
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    case 8: f(std::integral_constant<int, 8>{}); break;
+    case 16: f(std::integral_constant<int, 16>{}); break;
+    case 32: f(std::integral_constant<int, 32>{}); break;
+    default: f(std::integral_constant<int, 64>{}); break;
+    }
+}
+
+double Solver::power_iteration()
+{   // estimate_maximum_singular_value, utils.cu:87-228
+    const int max_iter = P_.sv_max_iter;
+    std::vector<double> h(m_);
+    for (int i = 0; i < m_; ++i) h[i] = g_dist(g_gen);
+    DBuf<double> v(m_), vh(m_), w(n_);
+    v.upload(h.data(), m_, stream_);
+    const int gridA = std::max(1, std::min(2048, blocks_for((long)m_ * A_.G)));
+    const int gridR = std::max(1, std::min(2048, blocks_for(m_)));
+    DBuf<double> partsA(gridA), partsR((size_t)gridR * 2), parts0(gridR);
+    DBuf<PowerState> ps(1);
+    PowerState init{};
+    ps.upload(&init, 1, stream_);
+    // ||v0||
+    sumsq_partials_kernel<<<gridR, kBlock, 0, stream_>>>(v.get(), m_, parts0.get());
+    DBuf<double> tmp(1);
+    reduce_partials_kernel<<<1, kBlock, 0, stream_>>>(parts0.get(), gridR, 1, 1, 1, tmp.get());
+    PDAL_LAUNCHED();
+    double ss = 0.0;
+    tmp.download(&ss, 1, stream_);
+    PDAL_HIP(hipStreamSynchronize(stream_));
+    init.vnorm = std::sqrt(ss);
+    ps.upload(&init, 1, stream_);
+
+    auto one_iter = [&]() {
+        power_normalize_kernel<<<blocks_for(m_), kBlock, 0, stream_>>>(v.get(), vh.get(), m_, ps.get(), max_iter);
+        dispatch_group(At_.G, [&](auto g) {
+            constexpr int G = decltype(g)::value;
+            spmv_gated_kernel<G><<<blocks_for((long)n_ * G), kBlock, 0, stream_>>>(At_.view(), vh.get(), w.get(),
+                                                                                  ps.get(), max_iter);
+        });
+        dispatch_group(A_.G, [&](auto g) {
+            constexpr int G = decltype(g)::value;
+            power_av_dot_kernel<G><<<gridA, kBlock, 0, stream_>>>(A_.view(), w.get(), vh.get(), v.get(),
+                                                                 partsA.get(), ps.get(), max_iter);
+        });
+        power_finish_dot_kernel<<<1, kBlock, 0, stream_>>>(partsA.get(), gridA, ps.get(), max_iter);
+        power_resid_kernel<<<gridR, kBlock, 0, stream_>>>(v.get(), vh.get(), m_, partsR.get(), ps.get(), max_iter);
+        power_finish_resid_kernel<<<1, kBlock, 0, stream_>>>(partsR.get(), gridR, ps.get(), max_iter, P_.sv_tol);
+        PDAL_LAUNCHED();
+    };
+    constexpr int kChunk = 32;
+    hipGraphExec_t exec = nullptr;
+    if (use_graph_) {
+        hipGraph_t graph;
+        PDAL_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeGlobal));
+        for (int i = 0; i < kChunk; ++i) one_iter();
+        PDAL_HIP(hipStreamEndCapture(stream_, &graph));
+        PDAL_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        PDAL_HIP(hipGraphDestroy(graph));
+    }
+    PowerState st{};
+    for (int done_iters = 0; done_iters < max_iter; done_iters += kChunk) {
+        if (exec) PDAL_HIP(hipGraphLaunch(exec, stream_));
+        else
+            for (int i = 0; i < kChunk; ++i) one_iter();
+        ps.download(&st, 1, stream_);
+        PDAL_HIP(hipStreamSynchronize(stream_));
+        if (st.done || st.iter >= max_iter) break;
+    }
+    if (exec) (void)hipGraphExecDestroy(exec);
+    return std::sqrt(st.sigma_sq);
+}
+
+void Solver::refresh_step_schedule()
+{   // solver.cu:3103-3126 (alpha mode)
+    const double margin = env_double("CP_AL_STEP_MARGIN", 0.998 * 0.998);
+    al_lambda_ = std::fmin(std::fmax(al_lambda_, 0.0), 0.95);
+    const double eta_sq = margin * (1.0 - al_lambda_) / std::fmax(lambda_est_, 1e-12);
+    step_size_ = std::sqrt(std::fmax(eta_sq, 1e-24));
+    const double dual_step = step_size_ * std::fmax(pw_, 1e-12);
+    al_penalty_ = dual_step * al_lambda_ / std::fmax(1.0 - al_lambda_, 1e-12);
+}
+
+void Solver::init_steps()
+{   // initialize_step_size_and_primal_weight, solver.cu:3538-3609
+    lambda_est_ = 1.0;
+    if (nnz_ != 0) {
+        const double sv = power_iteration();
+        lambda_est_ = sv * sv;
+    }
+    pw_ = P_.bound_objective_rescaling ? 1.0 : (c_norm_ + 1.0) / (b_norm_ + 1.0);
+    al_lambda_ = env_double("CP_AL_LAMBDA", 0.05);
+    refresh_step_schedule();
+    best_pw_ = pw_;
+    warn_unsupported_controllers();
+    if (P_.verbose)
+        fprintf(stderr, "[box-cpal] pw_init=%.3e al_lambda=%.3e eta=%.3e refl=%.3e al_penalty=%.3e lambdaA=%.3e\n",
+                pw_, al_lambda_, step_size_, P_.reflection_coefficient, al_penalty_, lambda_est_);
+}
+
+void Solver::setup()
+{   // initialize_solver_state, solver.cu:1795-2273
+    const auto t0 = std::chrono::steady_clock::now();
+    upload();
+    rescale();
+    x_start_.alloc(n_);
+    y_start_.alloc(m_);
+    // anchor = current = T-output = Halpern point = start (solver.cu:1954-1965)
+    auto copy = [&](DBuf<double> &dst, const DBuf<double> &src, int len) {
+        PDAL_HIP(hipMemcpyAsync(dst.get(), src.get(), (size_t)len * sizeof(double), hipMemcpyDeviceToDevice,
+                                stream_));
+    };
+    copy(x_start_, x0_, n_);
+    copy(y_start_, y0_, m_);
+    // norms of the unscaled working problem (solver.cu:1999-2067)
+    const lp_problem_t *w = W_;
+    const bool linf = P_.optimality_norm == NORM_TYPE_L_INF;
+    double sq = 0.0, mx = 0.0;
+    for (int j = 0; j < n_; ++j) {
+        const double cj = w->objective_vector[j];
+        if (linf) mx = std::max(mx, std::fabs(cj));
+        else sq += cj * cj;
+    }
+    c_norm_ = linf ? mx : std::sqrt(sq);
+    sq = 0.0;
+    mx = 0.0;
+    for (int i = 0; i < m_; ++i) {
+        const double lo = w->constraint_lower_bound[i], hi = w->constraint_upper_bound[i];
+        if (linf) {
+            if (std::isfinite(lo) && lo != hi) mx = std::max(mx, std::fabs(lo));
+            if (std::isfinite(hi)) mx = std::max(mx, std::fabs(hi));
+        } else {
+            if (std::isfinite(lo) && lo != hi) sq += lo * lo;
+            if (std::isfinite(hi)) sq += hi * hi;
+        }
+    }
+    b_norm_ = linf ? mx : std::sqrt(sq);
+    PDAL_HIP(hipStreamSynchronize(stream_));
+    init_steps();
+    pw_init_ = pw_;
+    setup_time_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void Solver::reset_iterates()
+{   // anchor = current = T-output = Halpern point = start (solver.cu:1954-1965)
+    auto copy = [&](DBuf<double> &dst, const DBuf<double> &src, int len) {
+        PDAL_HIP(hipMemcpyAsync(dst.get(), src.get(), (size_t)len * sizeof(double), hipMemcpyDeviceToDevice,
+                                stream_));
+    };
+    for (auto *b : {&x0_, &x_, &xT_, &xH_}) copy(*b, x_start_, n_);
+    for (auto *b : {&y0_, &y_, &yT_, &yH_}) copy(*b, y_start_, m_);
+    for (auto *b : {&xR_, &dslack_}) PDAL_HIP(hipMemsetAsync(b->get(), 0, (size_t)n_ * sizeof(double), stream_));
+    PDAL_HIP(hipMemsetAsync(yR_.get(), 0, (size_t)m_ * sizeof(double), stream_));
+    pw_ = pw_init_;
+    al_lambda_ = env_double("CP_AL_LAMBDA", 0.05);
+    refresh_step_schedule();
+    best_pw_ = pw_;
+    pw_err_sum_ = 0.0;
+    pw_last_err_ = 0.0;
+    best_gap_ = INFINITY;
+    total_ = 0;
+    inner_ = 0;
+    fp_ = fp0_ = 0.0;
+    last_trial_fp_ = INFINITY;
+    abs_p_ = rel_p_ = abs_d_ = rel_d_ = p_obj_ = d_obj_ = gap_ = rel_gap_ = 0.0;
+    reason_ = TERMINATION_REASON_UNSPECIFIED;
+    solve_time_ = 0.0;
+    last_eval_ = Eval{};
+    sync_scalars();
+}
+
+void Solver::sync_scalars()
+{   // sync_step_sizes_to_gpu + sync_inner_count_to_gpu, solver.cu:1738-1755
+    IterScalars &s = *h_sc_.get();
+    s.tau = step_size_ / pw_;
+    s.sigma = step_size_ * pw_;
+    s.beta = al_penalty_;
+    s.inner = inner_;
+    s.pad = 0;
+    PDAL_HIP(hipMemcpyAsync(sc_.get(), h_sc_.get(), sizeof(IterScalars), hipMemcpyHostToDevice, stream_));
+}
+
+void Solver::launch_iteration(int k_offset, bool major)
+{
+    const double rho = P_.reflection_coefficient;
+    const bool pred = al_penalty_ > 0.0;
+    if (pred) {
+        PredictorArgs a{A_.view(), x_.get(), y_.get(), l_.get(), u_.get(), yR_.get(), sc_.get()};
+        dispatch_group(A_.G, [&](auto g) {
+            constexpr int G = decltype(g)::value;
+            predictor_kernel<G><<<blocks_for((long)m_ * G), kBlock, 0, stream_>>>(a);
+        });
+    }
+    PrimalArgs pa{At_.view(), pred ? yR_.get() : y_.get(), c_.get(), lb_.get(), ub_.get(), x0_.get(), x_.get(),
+                  xR_.get(), xT_.get(), xH_.get(), dslack_.get(), sc_.get(), rho};
+    dispatch_group(At_.G, [&](auto g) {
+        constexpr int G = decltype(g)::value;
+        if (major) primal_kernel<G, true><<<blocks_for((long)n_ * G), kBlock, 0, stream_>>>(pa, k_offset);
+        else primal_kernel<G, false><<<blocks_for((long)n_ * G), kBlock, 0, stream_>>>(pa, k_offset);
+    });
+    DualArgs da{A_.view(), xR_.get(), l_.get(), u_.get(), y0_.get(), y_.get(), yT_.get(), yH_.get(), yR_.get(),
+                sc_.get(), rho};
+    dispatch_group(A_.G, [&](auto g) {
+        constexpr int G = decltype(g)::value;
+        if (major) dual_kernel<G, true><<<blocks_for((long)m_ * G), kBlock, 0, stream_>>>(da, k_offset);
+        else dual_kernel<G, false><<<blocks_for((long)m_ * G), kBlock, 0, stream_>>>(da, k_offset);
+    });
+    PDAL_LAUNCHED();
+}
+
+void Solver::launch_eval()
+{
+    EvalAtArgs ea{At_.view(), yR_.get(), yT_.get(), xR_.get(), xT_.get(), x0_.get(), c_.get(), dslack_.get(),
+                  D_.get(), parts_at_.get()};
+    dispatch_group(At_.G, [&](auto g) {
+        constexpr int G = decltype(g)::value;
+        eval_at_kernel<G><<<eval_grid_at_, kBlock, 0, stream_>>>(ea);
+    });
+    EvalAArgs eb{A_.view(), xR_.get(), xT_.get(), yR_.get(), yT_.get(), y0_.get(), l_.get(), u_.get(), E_.get(),
+                 parts_a_.get()};
+    dispatch_group(A_.G, [&](auto g) {
+        constexpr int G = decltype(g)::value;
+        eval_a_kernel<G><<<eval_grid_a_, kBlock, 0, stream_>>>(eb);
+    });
+    reduce_partials_kernel<<<1, kBlock, 0, stream_>>>(parts_at_.get(), eval_grid_at_, kEvalStride, kEvalAtSums,
+                                                      QAT_N, evalq_.get());
+    reduce_partials_kernel<<<1, kBlock, 0, stream_>>>(parts_a_.get(), eval_grid_a_, kEvalStride, kEvalASums, QA_N,
+                                                      evalq_.get() + 8);
+    PDAL_LAUNCHED();
+}
+
+void Solver::record_graph(int K)
+{
+    if (graph_exec_) {
+        PDAL_HIP(hipGraphExecDestroy(graph_exec_));
+        graph_exec_ = nullptr;
+    }
+    hipGraph_t graph;
+    PDAL_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeGlobal));
+    for (int i = 1; i <= K; ++i) launch_iteration(i, i == 1 || i == K);
+    launch_eval();
+    PDAL_HIP(hipStreamEndCapture(stream_, &graph));
+    PDAL_HIP(hipGraphInstantiate(&graph_exec_, graph, nullptr, nullptr, 0));
+    PDAL_HIP(hipGraphDestroy(graph));
+    graph_K_ = K;
+    graph_pred_ = al_penalty_ > 0.0;
+}
+
+void Solver::launch_block(int K)
+{   // solver.cu:1301-1341: iteration 1 and K use the major kernels
+    if (!use_graph_) {
+        for (int i = 1; i <= K; ++i) launch_iteration(i, i == 1 || i == K);
+        launch_eval();
+        return;
+    }
+    if (!graph_exec_ || graph_K_ != K || graph_pred_ != (al_penalty_ > 0.0)) record_graph(K);
+    PDAL_HIP(hipGraphLaunch(graph_exec_, stream_));
+}
+
+Eval Solver::read_eval()
+{
+    PDAL_HIP(hipMemcpyAsync(h_eval_.get(), evalq_.get(), 16 * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    PDAL_HIP(hipStreamSynchronize(stream_));
+    const double *q = h_eval_.get();
+    Eval e;
+    e.cross = q[QAT_CROSS];
+    e.pn2 = q[QAT_PN2];
+    e.dres2 = q[QAT_DRES2];
+    e.cx = q[QAT_CX];
+    e.lx = q[QAT_LX];
+    e.rx2 = q[QAT_RX2];
+    e.dresmax = q[QAT_DRESMAX];
+    e.ax2 = q[8 + QA_AX2];
+    e.dn2 = q[8 + QA_DN2];
+    e.pres2 = q[8 + QA_PRES2];
+    e.contrib = q[8 + QA_CONTRIB];
+    e.ry2 = q[8 + QA_RY2];
+    e.presmax = q[8 + QA_PRESMAX];
+    return e;
+}
+
+void Solver::fixed_point_error(const Eval &e)
+{   // solver.cu:3658-3686 (alpha-mode branch)
+    const double dn = std::sqrt(e.dn2), pn = std::sqrt(e.pn2), an = std::sqrt(e.ax2);
+    const double movement = pn * pn * pw_ + dn * dn / pw_;
+    const double interaction = 2 * step_size_ * e.cross;
+    const double aug = step_size_ * al_penalty_ * an * an;
+    fp_ = std::sqrt(std::fmax(movement + interaction - aug, 0.0));
+}
+
+void Solver::residual(const Eval &e)
+{   // compute_residual, utils.cu:812-917
+    if (P_.optimality_norm == NORM_TYPE_L_INF) {
+        abs_p_ = e.presmax;
+        abs_d_ = e.dresmax;
+    } else {
+        abs_p_ = std::sqrt(e.pres2) / b_scale_;
+        abs_d_ = std::sqrt(e.dres2);
+    }
+    abs_d_ /= c_scale_;
+    const double bc = b_scale_ * c_scale_;
+    p_obj_ = e.cx / bc + obj_const_;
+    d_obj_ = (e.lx + e.contrib) / bc + obj_const_;
+    rel_p_ = abs_p_ / (1.0 + b_norm_);
+    rel_d_ = abs_d_ / (1.0 + c_norm_);
+    gap_ = std::fabs(p_obj_ - d_obj_);
+    rel_gap_ = gap_ / (1.0 + std::fabs(p_obj_) + std::fabs(d_obj_));
+}
+
+bool Solver::should_restart(int K)
+{   // should_do_adaptive_restart, utils.cu:319-359
+    const restart_parameters_t &rp = P_.restart_params;
+    const double suff = env_double("CP_AL_RESTART_SUFFICIENT", rp.sufficient_reduction_for_restart);
+    const double nec = env_double("CP_AL_RESTART_NECESSARY", rp.necessary_reduction_for_restart);
+    const double art = env_double("CP_AL_RESTART_ARTIFICIAL", rp.artificial_restart_threshold);
+    bool r = false;
+    if (total_ == K) {
+        r = true;
+    } else if (total_ > K) {
+        if (fp_ <= suff * fp0_) r = true;
+        if (fp_ <= nec * fp0_ && fp_ > last_trial_fp_) r = true;
+        if (inner_ >= art * total_) r = true;
+    }
+    last_trial_fp_ = fp_;
+    return r;
+}
+
+void Solver::perform_restart(const Eval &e)
+{   // solver.cu:2821-3101, restart target "current", PID primal weight
+    const double pd = std::sqrt(e.rx2), dd = std::sqrt(e.ry2);
+    const double ratio = rel_d_ / std::fmax(rel_p_, 1e-300);
+    const restart_parameters_t &rp = P_.restart_params;
+    if (pd > 1e-16 && dd > 1e-16 && pd < 1e12 && dd < 1e12 && ratio > 1e-8 && ratio < 1e8) {
+        const double err = std::log(dd) - std::log(pd) - std::log(pw_);
+        pw_err_sum_ *= rp.i_smooth;
+        pw_err_sum_ += err;
+        const double de = err - pw_last_err_;
+        pw_ *= std::exp(rp.k_p * err + rp.k_i * pw_err_sum_ + rp.k_d * de);
+        pw_last_err_ = err;
+    } else {
+        pw_ = best_pw_;
+        pw_err_sum_ = 0.0;
+        pw_last_err_ = 0.0;
+    }
+    const double g = std::fabs(std::log10(std::fmax(rel_d_, 1e-300) / std::fmax(rel_p_, 1e-300)));
+    if (g < best_gap_) {
+        best_gap_ = g;
+        best_pw_ = pw_;
+    }
+    refresh_step_schedule();
+    restart_copy_kernel<<<blocks_for((long)n_ + m_), kBlock, 0, stream_>>>(xT_.get(), x0_.get(), x_.get(), n_,
+                                                                          yT_.get(), y0_.get(), y_.get(), m_);
+    PDAL_LAUNCHED();
+    inner_ = 0;
+    last_trial_fp_ = INFINITY;
+}
+
+int print_frequency(int iter)
+{   // get_print_frequency, utils.cu:637-647
+    int step = 10;
+    long long threshold = 1000;
+    while (iter >= threshold) {
+        step *= 10;
+        threshold *= 10;
+    }
+    return step;
+}
+
+void Solver::display() const
+{   // display_iteration_stats, utils.cu:613-635
+    if (!P_.verbose) return;
+    if (total_ % print_frequency(total_) == 0)
+        printf("%6d %.1e | %8.1e  %8.1e | %.1e %.1e %.1e | %.1e %.1e %.1e | %.1e \n", total_, solve_time_, p_obj_,
+               d_obj_, abs_p_, abs_d_, gap_, rel_p_, rel_d_, rel_gap_, fp_);
+}
+
+cp_al_result_t *Solver::make_result(const lp_problem_t *original, int charged)
+{   // create_result_from_state, solver.cu:3828-3909
+    DBuf<double> rc(n_), xo(n_), yo(m_);
+    dispatch_group(At_.G, [&](auto g) {
+        constexpr int G = decltype(g)::value;
+        reduced_cost_kernel<G><<<blocks_for((long)n_ * G), kBlock, 0, stream_>>>(
+            At_.view(), yT_.get(), c_.get(), D_.get(), lb_.get(), ub_.get(), c_scale_, rc.get());
+    });
+    unscale_kernel<<<blocks_for((long)n_ + m_), kBlock, 0, stream_>>>(xT_.get(), D_.get(), b_scale_, xo.get(), n_,
+                                                                     yT_.get(), E_.get(), c_scale_, yo.get(), m_);
+    PDAL_LAUNCHED();
+    cp_al_result_t *r = (cp_al_result_t *)calloc(1, sizeof(cp_al_result_t));
+    r->primal_solution = (double *)malloc((size_t)std::max(n_, 1) * sizeof(double));
+    r->dual_solution = (double *)malloc((size_t)std::max(m_, 1) * sizeof(double));
+    r->reduced_cost = (double *)malloc((size_t)std::max(n_, 1) * sizeof(double));
+    xo.download(r->primal_solution, n_, stream_);
+    yo.download(r->dual_solution, m_, stream_);
+    rc.download(r->reduced_cost, n_, stream_);
+    PDAL_HIP(hipStreamSynchronize(stream_));
+    r->num_variables = original->num_variables;
+    r->num_constraints = original->num_constraints;
+    r->num_nonzeros = original->constraint_matrix_num_nonzeros;
+    r->total_count = charged;
+    r->rescaling_time_sec = rescale_time_;
+    r->cumulative_time_sec = solve_time_;
+    r->absolute_primal_residual = abs_p_;
+    r->relative_primal_residual = rel_p_;
+    r->absolute_dual_residual = abs_d_;
+    r->relative_dual_residual = rel_d_;
+    r->primal_objective_value = p_obj_;
+    r->dual_objective_value = d_obj_;
+    r->objective_gap = gap_;
+    r->relative_objective_gap = rel_gap_;
+    r->termination_reason = reason_;
+    return r;
+}
+
+cp_al_result_t *Solver::run(const lp_problem_t *original)
+{
+    setup();
+    return solve_once(original);
+}
+
+cp_al_result_t *Solver::solve_once(const lp_problem_t *original)
+{   // optimize main loop, solver.cu:1197-1736
+    reset_iterates();
+    display();
+    const auto t_start = std::chrono::steady_clock::now();
+    int charged = total_;
+    bool do_restart = false;
+    const int K = P_.termination_evaluation_frequency;
+    const termination_criteria_t &tc = P_.termination_criteria;
+    bool have_eval = false;
+    while (charged < tc.iteration_limit) {
+        if (do_restart) {  // :1279-1284, same vectors as the last evaluation
+            fixed_point_error(last_eval_);
+            fp0_ = fp_;
+            do_restart = false;
+        }
+        sync_scalars();
+        launch_block(K);
+        const Eval e = read_eval();
+        last_eval_ = e;
+        have_eval = true;
+        fixed_point_error(e);
+        residual(e);
+        inner_ += K;
+        total_ += K;
+        charged += K;
+        if (total_ % print_frequency(total_) == 0) display();
+        solve_time_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+        if (rel_d_ < tc.eps_feasible_relative && rel_p_ < tc.eps_feasible_relative &&
+            rel_gap_ < tc.eps_optimal_relative)
+            reason_ = TERMINATION_REASON_OPTIMAL;
+        else if (total_ >= tc.iteration_limit)
+            reason_ = TERMINATION_REASON_ITERATION_LIMIT;
+        else if (solve_time_ >= tc.time_sec_limit)
+            reason_ = TERMINATION_REASON_TIME_LIMIT;
+        if (reason_ == TERMINATION_REASON_UNSPECIFIED && charged >= tc.iteration_limit)
+            reason_ = TERMINATION_REASON_ITERATION_LIMIT;
+        if (reason_ != TERMINATION_REASON_UNSPECIFIED) break;
+        do_restart = should_restart(K);
+        if (do_restart) {
+            if (P_.verbose) printf("restart iter %d fp %.12e\n", total_, fp_);
+            perform_restart(e);
+        }
+    }
+    if (reason_ == TERMINATION_REASON_UNSPECIFIED) {  // :1704-1709
+        reason_ = TERMINATION_REASON_ITERATION_LIMIT;
+        if (!have_eval) {
+            launch_eval();
+            last_eval_ = read_eval();
+        }
+        residual(last_eval_);
+        display();
+    }
+    return make_result(original, charged);
+}
+
+// Launch `iters` plain iterations one kernel at a time with hipEvents around
+// each SpMV-fused kernel; out = {avg ms primal, avg ms dual, avg ms predictor
+// (0 when beta == 0), avg ms per whole iteration, avg ms eval pass}.
+void Solver::time_kernels(int iters, double *out)
+{
+    hipEvent_t ev[8];
+    for (auto &e : ev) PDAL_HIP(hipEventCreate(&e));
+    double tp = 0, td = 0, tpr = 0;
+    const double rho = P_.reflection_coefficient;
+    const bool pred = al_penalty_ > 0.0;
+    sync_scalars();
+    for (int it = 0; it < iters; ++it) {
+        PDAL_HIP(hipEventRecord(ev[0], stream_));
+        if (pred) {
+            PredictorArgs a{A_.view(), x_.get(), y_.get(), l_.get(), u_.get(), yR_.get(), sc_.get()};
+            dispatch_group(A_.G, [&](auto g) {
+                constexpr int G = decltype(g)::value;
+                predictor_kernel<G><<<blocks_for((long)m_ * G), kBlock, 0, stream_>>>(a);
+            });
+        }
+        PDAL_HIP(hipEventRecord(ev[1], stream_));
+        PrimalArgs pa{At_.view(), pred ? yR_.get() : y_.get(), c_.get(), lb_.get(), ub_.get(), x0_.get(), x_.get(),
+                      xR_.get(), xT_.get(), xH_.get(), dslack_.get(), sc_.get(), rho};
+        dispatch_group(At_.G, [&](auto g) {
+            constexpr int G = decltype(g)::value;
+            primal_kernel<G, false><<<blocks_for((long)n_ * G), kBlock, 0, stream_>>>(pa, it + 1);
+        });
+        PDAL_HIP(hipEventRecord(ev[2], stream_));
+        DualArgs da{A_.view(), xR_.get(), l_.get(), u_.get(), y0_.get(), y_.get(), yT_.get(), yH_.get(),
+                    yR_.get(), sc_.get(), rho};
+        dispatch_group(A_.G, [&](auto g) {
+            constexpr int G = decltype(g)::value;
+            dual_kernel<G, false><<<blocks_for((long)m_ * G), kBlock, 0, stream_>>>(da, it + 1);
+        });
+        PDAL_HIP(hipEventRecord(ev[3], stream_));
+        PDAL_LAUNCHED();
+        PDAL_HIP(hipEventSynchronize(ev[3]));
+        float a = 0, b = 0, c = 0;
+        PDAL_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
+        PDAL_HIP(hipEventElapsedTime(&b, ev[1], ev[2]));
+        PDAL_HIP(hipEventElapsedTime(&c, ev[2], ev[3]));
+        tpr += a;
+        tp += b;
+        td += c;
+    }
+    PDAL_HIP(hipEventRecord(ev[4], stream_));
+    launch_eval();
+    PDAL_HIP(hipEventRecord(ev[5], stream_));
+    PDAL_HIP(hipEventSynchronize(ev[5]));
+    float te = 0;
+    PDAL_HIP(hipEventElapsedTime(&te, ev[4], ev[5]));
+    out[0] = tp / iters;
+    out[1] = td / iters;
+    out[2] = pred ? tpr / iters : 0.0;
+    out[3] = (tp + td + (pred ? tpr : 0.0)) / iters;
+    out[4] = te;
+    out[5] = At_.G;
+    out[6] = A_.G;
+    for (auto &e : ev) (void)hipEventDestroy(e);
+}
+
+}  // namespace
+
+void reset_rng()
+{
+    g_gen.seed(1);
+    g_dist.reset();
+}
+
+cp_al_result_t *optimize_gpu(const cp_al_parameters_t *params, lp_problem_t *problem)
+{
+    Solver s(params, problem);
+    return s.run(problem);
+}
+
+int time_kernels_gpu(const cp_al_parameters_t *params, const lp_problem_t *problem, int iters, double *out)
+{
+    Solver s(params, problem);
+    s.setup();
+    s.reset_iterates();
+    s.time_kernels(iters, out);
+    return 0;
+}
+
+struct Session {
+    cp_al_parameters_t params;
+    lp_problem_t *problem;  // borrowed: must outlive the session
+    Solver solver;
+    Session(const cp_al_parameters_t *p, lp_problem_t *pr) : params(*p), problem(pr), solver(&params, pr) {}
+};
+
+void *session_create(const cp_al_parameters_t *params, lp_problem_t *problem)
+{
+    auto *s = new Session(params, problem);
+    try {
+        s->solver.setup();
+    } catch (...) {
+        delete s;
+        throw;
+    }
+    return s;
+}
+
+cp_al_result_t *session_solve(void *h) { return static_cast<Session *>(h)->solver.solve_once(static_cast<Session *>(h)->problem); }
+
+int session_time_kernels(void *h, int iters, double *out)
+{
+    auto *s = static_cast<Session *>(h);
+    s->solver.time_kernels(iters, out);
+    return 0;
+}
+
+double session_setup_seconds(void *h) { return static_cast<Session *>(h)->solver.setup_seconds(); }
+
+void session_free(void *h) { delete static_cast<Session *>(h); }
+
+}  // namespace lp
+}  // namespace pdal

@@ -1,0 +1,159 @@
+"""HIP CP_AL path vs the CPU FP64 oracle, through the cp_al.h C ABI.
+
+Bars (from the build's contract):
+  * iterates (x_T, y_T after every eval block up to 1000 iterations) within
+    1e-10 relative of the oracle;
+  * same termination status and objective within 1e-6 relative of the oracle
+    and of the planted optimum at the solve tolerances;
+  * edge cases the reference's interface admits: infinite and two-sided
+    bounds, empty rows and columns, every input format, warm start, the AL
+    predictor (alpha > 0) and the L-inf optimality norm.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from pdal_amd.generators import config0, planted_lp
+
+pytestmark = pytest.mark.gpu
+
+ITER_RTOL = 1e-10
+OBJ_RTOL = 1e-6
+
+
+def rel_err(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def run_both(oracle_lp, gpu_lp, args, **params):
+    out = []
+    for lib in (oracle_lp, gpu_lp):
+        lib.reset_rng()
+        p = lib.default_params(**params)
+        print(f"[case] lib={lib.path.name} params={params}", flush=True)
+        out.append(lib.solve(**args, params=p))
+    return out
+
+
+@pytest.mark.parametrize("lam", ["0", "0.05"])
+@pytest.mark.parametrize("iters", [200, 400, 600, 1000])
+def test_iterates_track_oracle(oracle_lp, gpu_lp, monkeypatch, lam, iters):
+    monkeypatch.setenv("CP_AL_LAMBDA", lam)
+    inst = config0(seed=3)
+    o, g = run_both(oracle_lp, gpu_lp, inst.lp_args(), iteration_limit=iters, eps_optimal_relative=1e-14,
+                    eps_feasible_relative=1e-14)
+    assert o.status == g.status == "ITERATION_LIMIT"
+    assert o.iterations == g.iterations == iters
+    assert rel_err(g.x, o.x) <= ITER_RTOL
+    assert rel_err(g.y, o.y) <= ITER_RTOL
+    assert rel_err(g.reduced_cost, o.reduced_cost) <= ITER_RTOL
+    for f in ("relative_primal_residual", "relative_dual_residual", "relative_objective_gap"):
+        assert abs(getattr(g, f) - getattr(o, f)) <= 1e-8 * max(abs(getattr(o, f)), 1e-300) + 1e-14
+
+
+@pytest.mark.parametrize("tol", [1e-4, 1e-8])
+@pytest.mark.parametrize("seed", [0, 1])
+def test_solve_matches_oracle_and_planted(oracle_lp, gpu_lp, monkeypatch, tol, seed):
+    monkeypatch.setenv("CP_AL_LAMBDA", "0")
+    inst = config0(seed=seed)
+    o, g = run_both(oracle_lp, gpu_lp, inst.lp_args(), eps_optimal_relative=tol, eps_feasible_relative=tol)
+    assert o.status == g.status == "OPTIMAL"
+    assert g.kkt < tol
+    assert abs(g.iterations - o.iterations) <= 2 * 200, (g.iterations, o.iterations)
+    assert abs(g.primal_objective - o.primal_objective) <= OBJ_RTOL * abs(o.primal_objective)
+    if tol <= 1e-8:
+        assert abs(g.primal_objective - inst.optimal_value) <= OBJ_RTOL * abs(inst.optimal_value)
+
+
+def test_kat1_docs_example(oracle_lp, gpu_lp):
+    """LP/CP_AL/docs/C_API.md:47-88: x=(1,2), y=(1,-1,0), objective 3."""
+    A = np.array([[1.0, 2.0], [0.0, 1.0], [3.0, 2.0]])
+    args = dict(A=sp.csr_matrix(A), c=np.array([1.0, 1.0]), con_lb=np.array([5.0, -np.inf, -np.inf]),
+                con_ub=np.array([5.0, 2.0, 8.0]))
+    o, g = run_both(oracle_lp, gpu_lp, args, eps_optimal_relative=1e-9, eps_feasible_relative=1e-9)
+    for r in (o, g):
+        assert r.status == "OPTIMAL"
+        np.testing.assert_allclose(r.x, [1.0, 2.0], atol=1e-6)
+        np.testing.assert_allclose(r.y, [1.0, -1.0, 0.0], atol=1e-6)
+        assert abs(r.primal_objective - 3.0) < 1e-6
+    assert rel_err(g.x, o.x) < 1e-8
+
+
+def _edge_instance(seed=7):
+    rng = np.random.default_rng(seed)
+    m, n = 60, 90
+    A = sp.random(m, n, density=0.08, random_state=seed, format="lil")
+    A[5, :] = 0.0  # empty row
+    A[:, 7] = 0.0  # empty column
+    A = sp.csr_matrix(A)
+    A.eliminate_zeros()
+    x0 = rng.uniform(0.0, 1.0, n)
+    ax = A @ x0
+    l = ax - rng.uniform(0.0, 1.0, m)
+    u = ax + rng.uniform(0.0, 1.0, m)
+    l[::4] = -np.inf
+    u[1::5] = np.inf
+    l[5], u[5] = -1.0, 1.0
+    lb = np.where(rng.random(n) < 0.3, -np.inf, -rng.random(n))
+    ub = np.where(rng.random(n) < 0.3, np.inf, 1.0 + rng.random(n))
+    c = rng.standard_normal(n)
+    return dict(A=A, c=c, con_lb=l, con_ub=u, var_lb=lb, var_ub=ub, obj_const=1.25)
+
+
+@pytest.mark.parametrize("lam", ["0", "0.05"])
+@pytest.mark.parametrize("norm", ["l2", "linf"])
+def test_edge_bounds_empty_rows(oracle_lp, gpu_lp, monkeypatch, lam, norm):
+    monkeypatch.setenv("CP_AL_LAMBDA", lam)
+    args = _edge_instance()
+    o, g = run_both(oracle_lp, gpu_lp, args, iteration_limit=600, optimality_norm=norm,
+                    eps_optimal_relative=1e-14, eps_feasible_relative=1e-14)
+    assert rel_err(g.x, o.x) <= ITER_RTOL
+    assert rel_err(g.y, o.y) <= ITER_RTOL
+    assert abs(g.primal_objective - o.primal_objective) <= 1e-9 * max(1.0, abs(o.primal_objective))
+
+
+def test_input_formats_agree(gpu_lp):
+    args = _edge_instance(11)
+    A = args.pop("A")
+    base = None
+    for fmt in ("csr", "csc", "coo", "dense"):
+        gpu_lp.reset_rng()
+        p = gpu_lp.default_params(iteration_limit=400)
+        r = gpu_lp.solve(A=A, **args, params=p, fmt=fmt)
+        if base is None:
+            base = r
+        else:
+            assert rel_err(r.x, base.x) == 0.0
+            assert rel_err(r.y, base.y) == 0.0
+
+
+def test_warm_start_and_eval_freq(oracle_lp, gpu_lp, monkeypatch):
+    monkeypatch.setenv("CP_AL_LAMBDA", "0")
+    inst = planted_lp(300, 600, 6, seed=5)
+    rng = np.random.default_rng(0)
+    xs = inst.x_star + 0.01 * rng.standard_normal(inst.n)
+    ys = inst.y_star + 0.01 * rng.standard_normal(inst.m)
+    out = []
+    for lib in (oracle_lp, gpu_lp):
+        lib.reset_rng()
+        p = lib.default_params(iteration_limit=90, termination_evaluation_frequency=3,
+                               eps_optimal_relative=1e-14, eps_feasible_relative=1e-14)
+        out.append(lib.solve(**inst.lp_args(), params=p, primal_start=xs, dual_start=ys))
+    o, g = out
+    assert o.iterations == g.iterations == 90
+    assert rel_err(g.x, o.x) <= ITER_RTOL
+    assert rel_err(g.y, o.y) <= ITER_RTOL
+
+
+def test_zero_iteration_limit(oracle_lp, gpu_lp):
+    inst = planted_lp(50, 100, 4, seed=2)
+    o, g = run_both(oracle_lp, gpu_lp, inst.lp_args(), iteration_limit=0)
+    assert o.status == g.status == "ITERATION_LIMIT"
+    assert o.iterations == g.iterations == 0
+    np.testing.assert_array_equal(g.x, 0.0)
+    assert abs(g.relative_primal_residual - o.relative_primal_residual) <= 1e-12 * o.relative_primal_residual
+
