@@ -908,3 +908,53 @@ ORACLE_API cp_al_result_t *solve_lp_problem(lp_problem_t *prob, const cp_al_para
     else set_default_parameters(&local);
     return optimize(&local, prob);
 }
+
+/* ---------------- test hooks (pin individual oracle functions) ---------------- */
+
+// sigma_max(A) by the oracle's power iteration (utils.cu:87-228), generator reset first.
+ORACLE_API double cp_al_oracle_sigma_max(int m, int n, const int *rp, const int *ci, const double *v, int max_iter,
+                                         double tol)
+{
+    State s;
+    s.m = m;
+    s.n = n;
+    s.A.rows = m;
+    s.A.cols = n;
+    s.A.rp.assign(rp, rp + m + 1);
+    s.A.ci.assign(ci, ci + rp[m]);
+    s.A.v.assign(v, v + rp[m]);
+    s.At = transpose(s.A);
+    cp_al_oracle_reset_rng();
+    return estimate_max_singular_value(s, max_iter, tol);
+}
+
+// The oracle's rescaling (preconditioner.cu:379-475) of a problem; outputs the
+// scaled matrix values (CSR order of the input), cumulative E and D, and the
+// bound / objective scales.
+ORACLE_API void cp_al_oracle_rescale(const lp_problem_t *w, const cp_al_parameters_t *p, double *vals_out,
+                                     double *E_out, double *D_out, double *scales_out)
+{
+    State s;
+    s.n = w->num_variables;
+    s.m = w->num_constraints;
+    const int nnz = w->constraint_matrix_num_nonzeros;
+    s.A.rows = s.m;
+    s.A.cols = s.n;
+    s.A.rp.assign(w->constraint_matrix_row_pointers, w->constraint_matrix_row_pointers + s.m + 1);
+    s.A.ci.assign(w->constraint_matrix_col_indices, w->constraint_matrix_col_indices + nnz);
+    s.A.v.assign(w->constraint_matrix_values, w->constraint_matrix_values + nnz);
+    s.At = transpose(s.A);
+    s.c.assign(w->objective_vector, w->objective_vector + s.n);
+    s.lb.assign(w->variable_lower_bound, w->variable_lower_bound + s.n);
+    s.ub.assign(w->variable_upper_bound, w->variable_upper_bound + s.n);
+    s.l.assign(w->constraint_lower_bound, w->constraint_lower_bound + s.m);
+    s.u.assign(w->constraint_upper_bound, w->constraint_upper_bound + s.m);
+    s.x0.assign(s.n, 0.0);
+    s.y0.assign(s.m, 0.0);
+    rescale_problem(p, s);
+    std::copy(s.A.v.begin(), s.A.v.end(), vals_out);
+    std::copy(s.E.begin(), s.E.end(), E_out);
+    std::copy(s.D.begin(), s.D.end(), D_out);
+    scales_out[0] = s.b_scale;
+    scales_out[1] = s.c_scale;
+}
