@@ -1,6 +1,6 @@
 """Build the CPU oracle shared objects (test infrastructure, gcc only).
 
-Output: oracle/_build/libcp_al_oracle.so.  The oracle exports the same C names
+Output: oracle/_build/libcp_al_oracle.so and libfa_cp_oracle.so.  The oracle exports the same C names
 as the product library; it is built with hidden visibility and -Bsymbolic so a
 process can load both without the two binding to each other's symbols.
 
@@ -22,15 +22,20 @@ FLAGS = ["-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-fvisibi
          "-Wl,-Bsymbolic", f"-I{ROOT / 'include'}"]
 
 
-def build() -> Path:
+def _one(src: str, name: str) -> Path:
     OUT.mkdir(exist_ok=True)
-    lib = OUT / "libcp_al_oracle.so"
-    tmp = OUT / "libcp_al_oracle.so.tmp"
-    cmd = ["g++", *FLAGS, str(HERE / "cp_al_oracle.cpp"), "-o", str(tmp)]
+    lib = OUT / name
+    tmp = OUT / (name + ".tmp")
+    cmd = ["g++", *FLAGS, str(HERE / src), "-o", str(tmp)]
     print("+", " ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, lib)
     return lib
+
+
+def build() -> Path:
+    _one("fa_cp_oracle.cpp", "libfa_cp_oracle.so")
+    return _one("cp_al_oracle.cpp", "libcp_al_oracle.so")
 
 
 if __name__ == "__main__":

@@ -1,0 +1,1223 @@
+/*
+ * CPU FP64 oracle for the FA_CP (QP) restarted reflected-Halpern solver.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker the HIP path (libfa_cp.so) is compared
+ * against.  Nothing in the product links, loads or calls it; only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg use it.  It exports
+ * the same C names as libfa_cp.so plus fa_cp_oracle_reset_rng().
+ *
+ * Statement-by-statement restatement of QP/FA_CP:
+ *   optimize ............................ src/solver.cu:74-168
+ *   rescale_problem (host) .............. src/preconditioner.c:36-331
+ *   initialize_solver_state ............. src/solver_state.cu:494-877
+ *   quadratic term, Q.x, xQx ............ src/solver_state.cu:220-460
+ *   detect_q_type ....................... src/utils.cu:1576-1616
+ *   power iterations (A, Q max/min) ..... src/utils.cu:65-315 (mt19937(1) and rand())
+ *   termination / restart / defaults .... src/utils.cu:412-537
+ *   residual, infeasibility ............. src/utils.cu:705-1132
+ *   fa_cp_update, BB inner solve, dual,
+ *   Halpern, restart, sigma rules,
+ *   fixed-point error, result ........... src/fa_cp_core_op.cu:93-1239
+ *   problem construction, conversions ... src/fa_cp.c:28-376, src/utils.cu:1135-1358
+ *
+ * Deliberate differences shared with the HIP path: wall-clock timing, no
+ * trajectory CSV, no feasibility polishing, restart-select env knobs ignored,
+ * errors return NULL instead of exit().
+ */
+#include "fa_cp.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#define ORACLE_API extern "C" __attribute__((visibility("default")))
+
+namespace {
+
+std::mt19937 g_gen(1);                              // utils.cu:28
+std::normal_distribution<double> g_dist(0.0, 1.0);  // utils.cu:29
+
+using Vec = std::vector<double>;
+
+struct Csr {
+    int rows = 0, cols = 0;
+    std::vector<int> rp, ci;
+    Vec v;
+    int nnz() const { return rp.empty() ? 0 : rp[rows]; }
+};
+
+void spmv(const Csr &A, const double *x, double *y)
+{
+    for (int i = 0; i < A.rows; ++i) {
+        double s = 0.0;
+        for (int k = A.rp[i]; k < A.rp[i + 1]; ++k) s += A.v[k] * x[A.ci[k]];
+        y[i] = s;
+    }
+}
+
+double nrm2(const Vec &x)
+{
+    double s = 0.0;
+    for (double a : x) s += a * a;
+    return std::sqrt(s);
+}
+
+double dot(const Vec &a, const Vec &b)
+{
+    double s = 0.0;
+    for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+    return s;
+}
+
+double inf_norm(const Vec &x)
+{
+    double m = 0.0;
+    for (double a : x) m = std::max(m, std::fabs(a));
+    return m;
+}
+
+double sum(const Vec &x)
+{
+    double s = 0.0;
+    for (double a : x) s += a;
+    return s;
+}
+
+Csr transpose(const Csr &A)
+{
+    Csr T;
+    T.rows = A.cols;
+    T.cols = A.rows;
+    T.rp.assign(A.cols + 1, 0);
+    const int nnz = A.nnz();
+    for (int k = 0; k < nnz; ++k) T.rp[A.ci[k] + 1]++;
+    for (int j = 0; j < A.cols; ++j) T.rp[j + 1] += T.rp[j];
+    T.ci.resize(nnz);
+    T.v.resize(nnz);
+    std::vector<int> next(T.rp.begin(), T.rp.end() - 1);
+    for (int i = 0; i < A.rows; ++i)
+        for (int k = A.rp[i]; k < A.rp[i + 1]; ++k) {
+            const int p = next[A.ci[k]]++;
+            T.ci[p] = i;
+            T.v[p] = A.v[k];
+        }
+    return T;
+}
+
+Csr from_component(const CsrComponent *c, int rows, int cols, int nnz)
+{
+    Csr M;
+    M.rows = rows;
+    M.cols = cols;
+    M.rp.assign(rows + 1, 0);
+    if (c && c->row_ptr && nnz > 0) {
+        M.rp.assign(c->row_ptr, c->row_ptr + rows + 1);
+        M.ci.assign(c->col_ind, c->col_ind + nnz);
+        M.v.assign(c->val, c->val + nnz);
+    }
+    return M;
+}
+
+double sane_positive(double v, double fb) { return (std::isfinite(v) && v > 0.0) ? v : fb; }
+double score_component(double v) { return std::isfinite(v) ? std::fmax(v, 1e-300) : INFINITY; }
+double clampd(double v, double lo, double hi) { return std::fmin(std::fmax(v, lo), hi); }
+
+int print_frequency(int iter)  // utils.cu:690-703
+{
+    int step = 10;
+    long long threshold = 1000;
+    while (iter >= threshold) {
+        step *= 10;
+        threshold *= 10;
+    }
+    return step;
+}
+
+struct S {
+    fa_cp_parameters_t P{};
+    int n = 0, m = 0, r = 0;
+    double obj_const = 0.0;
+    Csr A, At, Q, R, Rt;
+    quad_obj_type_t qtype = FA_CP_NON_Q;
+    Vec qdiag;
+    double q_norm = 0.0, nonconvexity = 0.0;
+    bool is_lp = true;
+    Vec c, lb, ub, l, u, lfin, ufin, E, D;
+    double b_scale = 1.0, c_scale = 1.0, c_norm = 0.0, b_norm = 0.0;
+
+    Vec x0, x, xT, xR, dual_prod, dslack, dres, dx, qx;
+    Vec y0, y, yT, yR, prim_prod, al_u, pslack, pres, dy;
+    Vec rx, ry;  // R x and scratch
+
+    // inner solver
+    bool has_inner = false;
+    int inner_limit = 0;
+    double inner_tol = 0.0;
+    long inner_total = 0;
+    Vec grad, dir, gbuf;
+
+    // AL sigma state
+    bool al_available = false, use_al = false;
+    double al_sigma = 0.0, best_al_sigma = 0.0, best_sigma_score = INFINITY, last_sigma_score = INFINITY;
+    double last_sigma_rp = INFINITY, last_sigma_rd = INFINITY, last_sigma_gap = INFINITY;
+    int sigma_update_count = 0, sigma_dir = 1;
+
+    double step = 0.0, pw = 1.0, best_pw = 1.0, pw_err_sum = 0.0, pw_last_err = 0.0, best_gap = INFINITY;
+    int total = 0, inner = 0;
+    bool major = false;
+    double fp = INFINITY, fp0 = INFINITY, last_trial_fp = INFINITY;
+    double abs_p = 0, rel_p = 0, abs_d = 0, rel_d = 0, p_obj = 0, d_obj = 0, gap = 0, rel_gap = 0;
+    double max_p_ray = 0, max_d_ray = 0, p_ray_obj = 0, d_ray_obj = 0;
+    termination_reason_t reason = TERMINATION_REASON_UNSPECIFIED;
+    double rescale_time = 0.0, solve_time = 0.0;
+};
+
+/* ---------------- preconditioner.c (host) ---------------- */
+
+void scale_problem(S &s, const Vec &e, const Vec &d)
+{   // preconditioner.c:92-138
+    for (int i = 0; i < s.n; ++i) {
+        s.c[i] /= d[i];
+        s.ub[i] *= d[i];
+        s.lb[i] *= d[i];
+    }
+    for (int i = 0; i < s.m; ++i) {
+        s.l[i] /= e[i];
+        s.u[i] /= e[i];
+    }
+    for (int row = 0; row < s.m; ++row)
+        for (int k = s.A.rp[row]; k < s.A.rp[row + 1]; ++k) s.A.v[k] /= (e[row] * d[s.A.ci[k]]);
+    for (int row = 0; row < s.n; ++row)
+        for (int k = s.Q.rp[row]; k < s.Q.rp[row + 1]; ++k) s.Q.v[k] /= (d[row] * d[s.Q.ci[k]]);
+    if (s.R.nnz() > 0)
+        for (int row = 0; row < s.r; ++row)
+            for (int k = s.R.rp[row]; k < s.R.rp[row + 1]; ++k) s.R.v[k] /= d[s.R.ci[k]];
+}
+
+void rescale(S &s)
+{   // rescale_problem, preconditioner.c:291-331
+    s.E.assign(s.m, 1.0);
+    s.D.assign(s.n, 1.0);
+    Vec e(s.m), d(s.n);
+    for (int it = 0; it < s.P.l_inf_ruiz_iterations; ++it) {  // ruiz_rescaling, :140-203
+        std::fill(d.begin(), d.end(), 0.0);
+        std::fill(e.begin(), e.end(), 0.0);
+        for (int row = 0; row < s.m; ++row)
+            for (int k = s.A.rp[row]; k < s.A.rp[row + 1]; ++k) {
+                const double v = std::fabs(s.A.v[k]);
+                const int col = s.A.ci[k];
+                if (v > d[col]) d[col] = v;
+                if (v > e[row]) e[row] = v;
+            }
+        for (int row = 0; row < s.n; ++row)
+            for (int k = s.Q.rp[row]; k < s.Q.rp[row + 1]; ++k) {
+                const double v = std::fabs(s.Q.v[k]);
+                if (v > d[s.Q.ci[k]]) d[s.Q.ci[k]] = v;
+            }
+        for (int i = 0; i < s.n; ++i) d[i] = (d[i] < 1e-12) ? 1.0 : std::sqrt(d[i]);
+        for (int i = 0; i < s.m; ++i) e[i] = (e[i] < 1e-12) ? 1.0 : std::sqrt(e[i]);
+        scale_problem(s, e, d);
+        for (int i = 0; i < s.n; ++i) s.D[i] *= d[i];
+        for (int i = 0; i < s.m; ++i) s.E[i] *= e[i];
+    }
+    if (s.P.has_pock_chambolle_alpha) {  // pock_chambolle_rescaling, :205-247
+        const double a = s.P.pock_chambolle_alpha;
+        std::fill(d.begin(), d.end(), 0.0);
+        std::fill(e.begin(), e.end(), 0.0);
+        for (int row = 0; row < s.m; ++row)
+            for (int k = s.A.rp[row]; k < s.A.rp[row + 1]; ++k) {
+                const double v = std::fabs(s.A.v[k]);
+                d[s.A.ci[k]] += std::pow(v, 2.0 - a);
+                e[row] += std::pow(v, a);
+            }
+        for (int row = 0; row < s.n; ++row)
+            for (int k = s.Q.rp[row]; k < s.Q.rp[row + 1]; ++k) d[s.Q.ci[k]] += std::pow(std::fabs(s.Q.v[k]), 2.0 - a);
+        for (int i = 0; i < s.n; ++i) d[i] = (d[i] < 1e-12) ? 1.0 : std::sqrt(d[i]);
+        for (int i = 0; i < s.m; ++i) e[i] = (e[i] < 1e-12) ? 1.0 : std::sqrt(e[i]);
+        scale_problem(s, e, d);
+        for (int i = 0; i < s.n; ++i) s.D[i] *= d[i];
+        for (int i = 0; i < s.m; ++i) s.E[i] *= e[i];
+    }
+    if (s.P.bound_objective_rescaling) {  // bound_obj_rescaling, :249-289
+        double bsq = 0.0;
+        for (int i = 0; i < s.m; ++i) {
+            if (std::isfinite(s.l[i]) && s.l[i] != s.u[i]) bsq += s.l[i] * s.l[i];
+            if (std::isfinite(s.u[i])) bsq += s.u[i] * s.u[i];
+        }
+        double csq = 0.0;
+        for (int i = 0; i < s.n; ++i) csq += s.c[i] * s.c[i];
+        s.b_scale = 1.0 / (std::sqrt(bsq) + 1.0);
+        s.c_scale = 1.0 / (std::sqrt(csq) + 1.0);
+        for (int i = 0; i < s.m; ++i) {
+            s.l[i] *= s.b_scale;
+            s.u[i] *= s.b_scale;
+        }
+        for (int i = 0; i < s.n; ++i) {
+            s.lb[i] *= s.b_scale;
+            s.ub[i] *= s.b_scale;
+            s.c[i] *= s.c_scale;
+        }
+        for (double &q : s.Q.v) q *= s.c_scale / s.b_scale;
+        if (s.R.nnz() > 0) {
+            const double f = std::sqrt(s.c_scale / s.b_scale);
+            for (double &q : s.R.v) q *= f;
+        }
+    }
+}
+
+/* ---------------- Q helpers (solver_state.cu) ---------------- */
+
+quad_obj_type_t detect(const S &s)
+{   // detect_q_type, utils.cu:1576-1616 (called with the scaled problem)
+    const int ns = s.Q.nnz(), nr = s.R.nnz();
+    if (ns == 0 && nr == 0) return FA_CP_NON_Q;
+    if (nr > 0) return ns > 0 ? FA_CP_LOW_RANK_PLUS_SPARSE_Q : FA_CP_LOW_RANK_Q;
+    for (int i = 0; i < s.n; ++i)
+        for (int k = s.Q.rp[i]; k < s.Q.rp[i + 1]; ++k)
+            if (s.Q.ci[k] != i) return FA_CP_SPARSE_Q;
+    return FA_CP_DIAG_Q;
+}
+
+void obj_product(S &s, const Vec &xv)
+{   // update_obj_product, solver_state.cu:371-442
+    switch (s.qtype) {
+    case FA_CP_NON_Q: return;
+    case FA_CP_SPARSE_Q: spmv(s.Q, xv.data(), s.qx.data()); return;
+    case FA_CP_DIAG_Q:
+        for (int i = 0; i < s.n; ++i) s.qx[i] = s.qdiag[i] * xv[i];
+        return;
+    case FA_CP_LOW_RANK_Q:
+        spmv(s.R, xv.data(), s.rx.data());
+        spmv(s.Rt, s.rx.data(), s.qx.data());
+        return;
+    case FA_CP_LOW_RANK_PLUS_SPARSE_Q: {
+        spmv(s.Q, xv.data(), s.qx.data());
+        spmv(s.R, xv.data(), s.rx.data());
+        for (int i = 0; i < s.n; ++i) {
+            double t = 0.0;
+            for (int k = s.Rt.rp[i]; k < s.Rt.rp[i + 1]; ++k) t += s.Rt.v[k] * s.rx[s.Rt.ci[k]];
+            s.qx[i] = t + s.qx[i];
+        }
+        return;
+    }
+    default: return;
+    }
+}
+
+double max_singular_value(const S &s)
+{   // estimate_maximum_singular_value, utils.cu:65-170
+    const int m = s.m;
+    Vec v(m), nv(m), w(s.n);
+    for (int i = 0; i < m; ++i) v[i] = g_dist(g_gen);
+    double sig = 1.0;
+    for (int it = 0; it < s.P.sv_max_iter; ++it) {
+        nv = v;
+        const double inv = 1.0 / nrm2(nv);
+        for (double &a : nv) a *= inv;
+        spmv(s.At, nv.data(), w.data());
+        spmv(s.A, w.data(), v.data());
+        sig = dot(nv, v);
+        const double neg = -sig;
+        for (int i = 0; i < m; ++i) nv[i] = neg * nv[i] + v[i];
+        if (nrm2(nv) < s.P.sv_tol) break;
+    }
+    return std::sqrt(sig);
+}
+
+double q_max_eig(const S &s)
+{   // estimate_maximum_eigenvalue, utils.cu:172-238
+    const int n = s.n;
+    Vec v(n), av(n);
+    for (int i = 0; i < n; ++i) v[i] = (double)rand() / RAND_MAX;
+    double lam = 0.0;
+    for (int it = 0; it < s.P.sv_max_iter; ++it) {
+        const double inv = 1.0 / nrm2(v);
+        for (double &a : v) a *= inv;
+        spmv(s.Q, v.data(), av.data());
+        const double old = lam;
+        lam = dot(v, av);
+        if (it > 0 && std::fabs(lam - old) < s.P.sv_tol) break;
+        v = av;
+    }
+    return lam;
+}
+
+double q_min_eig(const S &s, double lmax)
+{   // estimate_minimum_eigenvalue, utils.cu:240-315
+    const int n = s.n;
+    Vec v(n), av(n), sh(n);
+    for (int i = 0; i < n; ++i) v[i] = (double)rand() / RAND_MAX;
+    double mu = 0.0;
+    for (int it = 0; it < s.P.sv_max_iter; ++it) {
+        const double inv = 1.0 / nrm2(v);
+        for (double &a : v) a *= inv;
+        spmv(s.Q, v.data(), av.data());
+        for (int i = 0; i < n; ++i) sh[i] = lmax * v[i] + (-av[i]);
+        const double old = mu;
+        mu = dot(v, sh);
+        if (it > 0 && std::fabs(mu - old) < s.P.sv_tol) break;
+        v = sh;
+    }
+    return lmax - mu;
+}
+
+/* ---------------- core op (fa_cp_core_op.cu) ---------------- */
+
+bool major_kernel(const S &s) { return s.major || ((s.total + 2) % print_frequency(s.total + 2)) == 0; }
+
+void al_effective_dual_product(S &s, const Vec &xv, double sigma)
+{   // compute_al_effective_dual_product, :275-303 and kernel :181-192
+    spmv(s.A, xv.data(), s.prim_prod.data());
+    for (int i = 0; i < s.m; ++i) {
+        const double sh = s.prim_prod[i] - s.y[i] / sigma;
+        const double pr = std::fmax(s.l[i], std::fmin(sh, s.u[i]));
+        s.al_u[i] = s.y[i] - sigma * (s.prim_prod[i] - pr);
+    }
+    spmv(s.At, s.al_u.data(), s.dual_prod.data());
+}
+
+void primal_bb(S &s, double tau, double al_sigma)
+{   // primal_BB_step_size_update, :445-503
+    const double inv = 1.0 / tau;
+    int it = 1;
+    double alpha = 1.0 / inv;
+    if (s.use_al) al_effective_dual_product(s, s.x, al_sigma);
+    obj_product(s, s.x);
+    for (int i = 0; i < s.n; ++i) {  // bb_init kernel :381-399
+        const double g = s.qx[i] + s.c[i] - s.dual_prod[i];
+        double nx = s.x[i] - alpha * g;
+        nx = std::fmax(s.lb[i], std::fmin(nx, s.ub[i]));
+        s.xT[i] = nx;
+        s.grad[i] = g;
+        s.dir[i] = nx - s.x[i];
+    }
+    while (it < s.inner_limit) {
+        const double nd = nrm2(s.dir);
+        if (nd <= s.inner_tol) break;
+        if (s.use_al) al_effective_dual_product(s, s.xT, al_sigma);
+        obj_product(s, s.xT);
+        for (int i = 0; i < s.n; ++i) {  // :401-415
+            const double last = s.grad[i];
+            const double cur = s.qx[i] + s.c[i] - s.dual_prod[i] + inv * (s.xT[i] - s.x[i]);
+            s.gbuf[i] = cur - last;
+            s.grad[i] = cur;
+        }
+        const double tmp = dot(s.dir, s.gbuf);
+        alpha = nd * nd / tmp;
+        for (int i = 0; i < s.n; ++i) {  // :417-430
+            const double cs = s.xT[i];
+            double nx = cs - alpha * s.grad[i];
+            nx = std::fmax(s.lb[i], std::fmin(nx, s.ub[i]));
+            s.dir[i] = nx - cs;
+            s.xT[i] = nx;
+        }
+        ++it;
+    }
+    for (int i = 0; i < s.n; ++i) s.xR[i] = 2.0 * s.xT[i] - s.x[i];
+    s.inner_total += it - 1;
+}
+
+void fa_cp_update(S &s)
+{   // :531-630
+    double tau = s.step / s.pw;
+    if (s.nonconvexity < 0) {
+        tau = std::fmax(tau, -1.01 * std::fmin(0.0, s.nonconvexity));
+        tau /= 2;
+    }
+    const double sigma = s.step * s.pw;
+    const double al_sigma = s.use_al ? s.al_sigma : 0.0;
+    const bool mk = major_kernel(s);
+    if (!s.use_al) spmv(s.At, s.y.data(), s.dual_prod.data());
+    if (s.use_al) {
+        primal_bb(s, tau, al_sigma);
+    } else {
+        switch (s.qtype) {
+        case FA_CP_NON_Q:  // lp_primal_update, kernels :93-121
+            for (int i = 0; i < s.n; ++i) {
+                const double xi = s.x[i];
+                const double t = xi - tau * (s.c[i] - s.dual_prod[i]);
+                const double p = std::fmax(s.lb[i], std::fmin(t, s.ub[i]));
+                s.xR[i] = 2.0 * p - xi;
+                if (mk) {
+                    s.xT[i] = p;
+                    s.dslack[i] = (p - t) / tau;
+                }
+            }
+            break;
+        case FA_CP_DIAG_Q:  // diag_q_primal_update, kernels :123-154
+            for (int i = 0; i < s.n; ++i) {
+                const double xi = s.x[i];
+                const double t = (xi - tau * (s.c[i] - s.dual_prod[i])) / (1.0 + tau * s.qdiag[i]);
+                const double p = std::fmax(s.lb[i], std::fmin(t, s.ub[i]));
+                s.xR[i] = 2.0 * p - xi;
+                if (mk) s.xT[i] = p;
+            }
+            break;
+        default: primal_bb(s, tau, 0.0); break;
+        }
+    }
+    spmv(s.A, s.xR.data(), s.prim_prod.data());
+    if (s.use_al) {  // kernels :194-223
+        for (int i = 0; i < s.m; ++i) {
+            const double den = al_sigma + sigma;
+            const double sh = s.prim_prod[i] - s.y[i] / den;
+            const double pr = std::fmax(s.l[i], std::fmin(sh, s.u[i]));
+            const double nd = s.y[i] + sigma * (pr - s.prim_prod[i]);
+            if (mk) s.yT[i] = nd;
+            s.yR[i] = 2.0 * nd - s.y[i];
+        }
+    } else {  // kernels :156-179
+        for (int i = 0; i < s.m; ++i) {
+            const double t = s.y[i] / sigma - s.prim_prod[i];
+            const double p = std::fmax(-s.u[i], std::fmin(t, -s.l[i]));
+            if (mk) {
+                s.yT[i] = (t - p) * sigma;
+                s.yR[i] = 2.0 * s.yT[i] - s.y[i];
+            } else {
+                s.yR[i] = 2.0 * (t - p) * sigma - s.y[i];
+            }
+        }
+    }
+}
+
+void halpern(S &s)
+{   // halpern_update, :632-640 and kernel :225-242
+    const double w = (double)(s.inner + 1) / (s.inner + 2);
+    const double rho = s.P.reflection_coefficient;
+    for (int i = 0; i < s.n; ++i) {
+        const double rf = rho * s.xR[i] + (1.0 - rho) * s.x[i];
+        s.x[i] = w * rf + (1.0 - w) * s.x0[i];
+    }
+    for (int i = 0; i < s.m; ++i) {
+        const double rf = rho * s.yR[i] + (1.0 - rho) * s.y[i];
+        s.y[i] = w * rf + (1.0 - w) * s.y0[i];
+    }
+}
+
+void fixed_point_error(S &s)
+{   // :1042-1089
+    for (int i = 0; i < s.n; ++i) s.dx[i] = s.xR[i] - s.x[i];
+    for (int i = 0; i < s.m; ++i) s.dy[i] = s.yR[i] - s.y[i];
+    spmv(s.At, s.dy.data(), s.dual_prod.data());
+    const double dn = nrm2(s.dy), pn = nrm2(s.dx);
+    const double movement = pn * pn * s.pw + dn * dn / s.pw;
+    const double cross = dot(s.dual_prod, s.dx);
+    s.fp = std::sqrt(std::fmax(movement + 2 * s.step * cross, 0.0));
+    if (s.has_inner) s.inner_tol = std::fmin(s.inner_tol, std::fmax(0.0005 * pn / s.step * s.pw, 1e-9));
+}
+
+void compute_residual(S &s, norm_type_t norm)
+{   // utils.cu:889-1026
+    spmv(s.A, s.xT.data(), s.prim_prod.data());
+    spmv(s.At, s.yT.data(), s.dual_prod.data());
+    obj_product(s, s.xT);
+    for (int i = 0; i < s.m; ++i) {
+        const double cl = std::fmax(s.l[i], std::fmin(s.prim_prod[i], s.u[i]));
+        s.pres[i] = (s.prim_prod[i] - cl) * s.E[i];
+        s.pslack[i] = std::fmax(s.yT[i], 0.0) * s.lfin[i] + std::fmin(s.yT[i], 0.0) * s.ufin[i];
+    }
+    if (s.is_lp) {  // compute_lp_residual_kernel :705-733
+        for (int j = 0; j < s.n; ++j) s.dres[j] = (s.c[j] - s.dual_prod[j] - s.dslack[j]) * s.D[j];
+    } else {  // compute_qp_residual_kernel :735-770
+        const double tr = s.step / s.pw;
+        for (int j = 0; j < s.n; ++j) {
+            const double g = s.qx[j] + s.c[j] - s.dual_prod[j];
+            const double t = s.xT[j] - tr * g;
+            const double p = std::fmax(s.lb[j], std::fmin(s.ub[j], t));
+            const double ds = (p - t) / tr;
+            s.dres[j] = (g - ds) * s.D[j];
+            s.dslack[j] = ds;
+        }
+    }
+    const bool linf = norm == NORM_TYPE_L_INF;
+    s.abs_p = (linf ? inf_norm(s.pres) : nrm2(s.pres)) / s.b_scale;
+    s.abs_d = (linf ? inf_norm(s.dres) : nrm2(s.dres)) / s.c_scale;
+    const double half = s.qtype == FA_CP_NON_Q ? 0.0 : 0.5 * dot(s.xT, s.qx);
+    const double bc = s.b_scale * s.c_scale;
+    s.p_obj = (dot(s.c, s.xT) + half) / bc + s.obj_const;
+    s.d_obj = (dot(s.dslack, s.xT) + sum(s.pslack) - half) / bc + s.obj_const;
+    s.rel_p = s.abs_p / (1.0 + s.b_norm);
+    double den;
+    if (s.is_lp) {
+        den = 1.0 + s.c_norm;
+    } else {  // recover_primal_obj_dual_product :772-781 mutates both in place
+        for (int j = 0; j < s.n; ++j) {
+            s.dual_prod[j] = s.dual_prod[j] * s.D[j];
+            s.qx[j] = s.qx[j] * s.D[j];
+        }
+        const double qn = linf ? inf_norm(s.qx) : nrm2(s.qx);
+        const double an = linf ? inf_norm(s.dual_prod) : nrm2(s.dual_prod);
+        den = 1.0 + std::fmax(s.c_norm, std::fmax(qn / s.c_scale, an / s.c_scale));
+    }
+    s.rel_d = s.abs_d / den;
+    s.gap = std::fabs(s.p_obj - s.d_obj);
+    s.rel_gap = s.gap / (1.0 + std::fabs(s.p_obj) + std::fabs(s.d_obj));
+}
+
+void infeasibility_info(S &s)
+{   // compute_infeasibility_information, utils.cu:1028-1132
+    for (int i = 0; i < s.n; ++i) {
+        if (std::isfinite(s.lb[i])) s.dx[i] = std::fmax(s.dx[i], 0.0);
+        if (std::isfinite(s.ub[i])) s.dx[i] = std::fmin(s.dx[i], 0.0);
+    }
+    for (int i = 0; i < s.m; ++i) {
+        if (!std::isfinite(s.l[i])) s.dy[i] = std::fmin(s.dy[i], 0.0);
+        if (!std::isfinite(s.u[i])) s.dy[i] = std::fmax(s.dy[i], 0.0);
+    }
+    const double pinf = inf_norm(s.dx);
+    if (pinf > 0.0) {
+        const double sc = 1.0 / pinf;
+        for (double &a : s.dx) a *= sc;
+    }
+    const double dinf = inf_norm(s.dy);
+    spmv(s.A, s.dx.data(), s.prim_prod.data());
+    spmv(s.At, s.dy.data(), s.dual_prod.data());
+    const double bc = s.b_scale * s.c_scale;
+    s.p_ray_obj = dot(s.c, s.dx) / bc;
+    for (int i = 0; i < s.m; ++i)
+        s.pslack[i] = std::fmax(s.dy[i], 0.0) * s.lfin[i] + std::fmin(s.dy[i], 0.0) * s.ufin[i];
+    // the reference passes dual_product as the "dual slack" input and writes dual_slack
+    for (int j = 0; j < s.n; ++j) {
+        const double lbf = std::isfinite(s.lb[j]) ? s.lb[j] : 0.0;
+        const double ubf = std::isfinite(s.ub[j]) ? s.ub[j] : 0.0;
+        s.dslack[j] = std::fmax(-s.dual_prod[j], 0.0) * lbf + std::fmin(-s.dual_prod[j], 0.0) * ubf;
+    }
+    s.d_ray_obj = (sum(s.pslack) + sum(s.dslack)) / bc;
+    for (int i = 0; i < s.m; ++i) {
+        const double pp = s.prim_prod[i];
+        s.pslack[i] = (std::fmax(0.0, -pp) * (double)std::isfinite(s.l[i]) +
+                       std::fmax(0.0, pp) * (double)std::isfinite(s.u[i])) *
+                      s.E[i];
+    }
+    for (int j = 0; j < s.n; ++j) {
+        const double dp = -s.dual_prod[j];
+        s.dslack[j] = (std::fmax(0.0, dp) * (double)!std::isfinite(s.lb[j]) -
+                       std::fmin(0.0, dp) * (double)!std::isfinite(s.ub[j])) *
+                      s.D[j];
+    }
+    s.max_p_ray = inf_norm(s.pslack);
+    if (!s.is_lp && s.qtype != FA_CP_NON_Q) {
+        obj_product(s, s.dx);
+        s.max_p_ray = std::fmax(s.max_p_ray, inf_norm(s.qx) / s.c_scale);
+    }
+    const double dsn = inf_norm(s.dslack);
+    s.max_d_ray = dsn;
+    const double f = std::fmax(dinf, dsn);
+    if (f > 0.0) {
+        s.max_d_ray /= f;
+        s.d_ray_obj /= f;
+    } else {
+        s.max_d_ray = 0.0;
+        s.d_ray_obj = 0.0;
+    }
+}
+
+void check_termination(S &s)
+{   // utils.cu:438-462
+    const auto &tc = s.P.termination_criteria;
+    if (s.rel_d < tc.eps_feasible_relative && s.rel_p < tc.eps_feasible_relative && s.rel_gap < tc.eps_optimal_relative)
+        s.reason = TERMINATION_REASON_OPTIMAL;
+    else if (s.d_ray_obj > 0.0 && s.max_d_ray / s.d_ray_obj <= tc.eps_infeasible)
+        s.reason = TERMINATION_REASON_PRIMAL_INFEASIBLE;
+    else if (s.p_ray_obj < 0.0 && s.max_p_ray / (-s.p_ray_obj) <= tc.eps_infeasible)
+        s.reason = TERMINATION_REASON_DUAL_INFEASIBLE;
+    else if (s.total >= tc.iteration_limit)
+        s.reason = TERMINATION_REASON_ITERATION_LIMIT;
+    else if (s.solve_time >= tc.time_sec_limit)
+        s.reason = TERMINATION_REASON_TIME_LIMIT;
+}
+
+bool should_restart(S &s)
+{   // utils.cu:464-492
+    const auto &rp = s.P.restart_params;
+    const int K = s.P.termination_evaluation_frequency;
+    bool r = false;
+    if (s.total == K) {
+        r = true;
+    } else if (s.total > K) {
+        if (s.fp <= rp.sufficient_reduction_for_restart * s.fp0) r = true;
+        if (s.fp <= rp.necessary_reduction_for_restart * s.fp0 && s.fp > s.last_trial_fp) r = true;
+        if (s.inner >= rp.artificial_restart_threshold * s.total) r = true;
+    }
+    s.last_trial_fp = s.fp;
+    return r;
+}
+
+void set_sigma_active(S &s, double sigma, double zero_thr)
+{   // :667-676
+    if (!std::isfinite(sigma) || sigma <= zero_thr) {
+        s.al_sigma = 0.0;
+        s.use_al = false;
+    } else {
+        s.al_sigma = sigma;
+        s.use_al = s.al_available;
+    }
+}
+
+double clipped_update(const S &s, double cur, double ratio)
+{   // :700-713
+    const double gain = sane_positive(s.P.sigma_update_gain, 0.5);
+    double lo = sane_positive(s.P.sigma_clip_low, 0.6), hi = sane_positive(s.P.sigma_clip_high, 1.7);
+    if (lo > hi) std::swap(lo, hi);
+    ratio = sane_positive(ratio, 1.0);
+    return cur * clampd(std::pow(ratio, gain), lo, hi);
+}
+
+void update_al_sigma(S &s, double pd, double dd, double normal_motion)
+{   // update_al_sigma_at_restart, :722-906
+    const auto &P = s.P;
+    const bool bal = P.sigma_update_rule == FA_CP_SIGMA_UPDATE_BALANCED_GUARDED;
+    const bool guarded = P.sigma_update_rule == FA_CP_SIGMA_UPDATE_GUARDED || bal;
+    if (!s.al_available || P.sigma_update_rule == FA_CP_SIGMA_UPDATE_FIXED) return;
+    if (!guarded && !s.use_al) return;
+    const double ref = bal ? sane_positive(s.step * s.pw, 1.0) : 1.0;
+    double smin = sane_positive(P.sigma_min, 1e-8) * ref;
+    double smax = sane_positive(P.sigma_max, 10.0) * ref;
+    const double zthr = sane_positive(P.sigma_zero_threshold, sane_positive(P.sigma_min, 1e-8)) * ref;
+    if (smin > smax) std::swap(smin, smax);
+    if (zthr > smin) smin = zthr;
+    const double cur = s.use_al ? clampd(sane_positive(s.al_sigma, smin), smin, smax) : 0.0;
+    set_sigma_active(s, cur, zthr);
+    const double rp = sane_positive(s.rel_p, 1e-300), rd = sane_positive(s.rel_d, 1e-300),
+                 rg = sane_positive(s.rel_gap, 1e-300);
+    const double score = std::fmax(score_component(s.rel_p), std::fmax(score_component(s.rel_d), score_component(s.rel_gap)));
+    if (!std::isfinite(score)) return;
+    if (!std::isfinite(s.best_sigma_score) || score < s.best_sigma_score) {
+        s.best_sigma_score = score;
+        s.best_al_sigma = cur;
+    }
+    if (P.sigma_rollback_tol > 0.0 && (!guarded || cur > zthr) && std::isfinite(s.best_sigma_score) &&
+        s.best_sigma_score > 0.0 && score > P.sigma_rollback_tol * s.best_sigma_score &&
+        std::isfinite(s.best_al_sigma) && (guarded ? s.best_al_sigma >= 0.0 : s.best_al_sigma > 0.0)) {
+        const double best = guarded && s.best_al_sigma <= zthr ? 0.0 : clampd(s.best_al_sigma, smin, smax);
+        set_sigma_active(s, best, zthr);
+        s.last_sigma_score = score;
+        s.last_sigma_rp = rp;
+        s.last_sigma_rd = rd;
+        s.last_sigma_gap = rg;
+        s.sigma_update_count++;
+        return;
+    }
+    double prop = cur;
+    switch (P.sigma_update_rule) {
+    case FA_CP_SIGMA_UPDATE_RESIDUAL_BALANCE: prop = clipped_update(s, cur, rp / rd); break;
+    case FA_CP_SIGMA_UPDATE_KKT_BALANCE: prop = clipped_update(s, cur, rp / std::fmax(rd, rg)); break;
+    case FA_CP_SIGMA_UPDATE_DISPLACEMENT:
+        if (std::isfinite(pd) && std::isfinite(dd) && pd > 1e-16 && dd > 1e-16) {
+            const double target = clampd(dd / pd, smin, smax);
+            prop = clipped_update(s, cur, target / cur);
+        }
+        break;
+    case FA_CP_SIGMA_UPDATE_PROGRESS: {
+        const double lo = std::fmax(s.last_sigma_rd, s.last_sigma_gap);
+        const double other = std::fmax(rd, rg);
+        if (std::isfinite(s.last_sigma_rp) && std::isfinite(lo) && s.last_sigma_rp > 0.0 && lo > 0.0)
+            prop = clipped_update(s, cur, (lo / other) / (s.last_sigma_rp / rp));
+        else
+            prop = clipped_update(s, cur, rp / std::fmax(rd, rg));
+        break;
+    }
+    case FA_CP_SIGMA_UPDATE_GREEDY_SCORE:
+        if (s.sigma_update_count == 0) s.sigma_dir = (rp >= std::fmax(rd, rg)) ? 1 : -1;
+        else if (std::isfinite(score) && std::isfinite(s.last_sigma_score) && score > s.last_sigma_score)
+            s.sigma_dir *= -1;
+        prop = clipped_update(s, cur, s.sigma_dir > 0 ? P.sigma_clip_high : P.sigma_clip_low);
+        break;
+    case FA_CP_SIGMA_UPDATE_GUARDED:
+    case FA_CP_SIGMA_UPDATE_BALANCED_GUARDED: {
+        const double nthr = sane_positive(P.sigma_normal_motion_threshold, 1e-10);
+        const bool normal_ok = !std::isfinite(normal_motion) || normal_motion > nthr;
+        const bool zero_ok = !std::isfinite(normal_motion) || normal_motion > std::fmax(nthr, 2e-1);
+        const bool bottleneck = rp >= sane_positive(P.sigma_activation_threshold, 2.0) *
+                                          sane_positive(std::fmax(rd, rg), 1e-300);
+        const bool zero_stalled =
+            cur <= zthr && std::isfinite(s.last_sigma_score) && score > 0.9 * s.last_sigma_score;
+        const bool primal_signal = rp >= 0.5 * sane_positive(rd, 1e-300);
+        const double act = sane_positive(P.sigma_activation_value, 0.1) * ref;
+        const bool scale_ok = !bal || act >= 1e-6;
+        const bool activate = cur <= zthr ? zero_ok && zero_stalled && primal_signal && scale_ok
+                                          : normal_ok && bottleneck;
+        if (cur <= zthr) prop = activate ? act : 0.0;
+        else if (!normal_ok) prop = clipped_update(s, cur, P.sigma_clip_low);
+        else if (activate) prop = clipped_update(s, cur, rp / std::fmax(rd, rg));
+        else prop = clipped_update(s, cur, P.sigma_clip_low);
+        break;
+    }
+    default: break;
+    }
+    if (prop > zthr) prop = clampd(prop, smin, smax);
+    else prop = 0.0;
+    set_sigma_active(s, prop, zthr);
+    s.last_sigma_score = score;
+    s.last_sigma_rp = rp;
+    s.last_sigma_rd = rd;
+    s.last_sigma_gap = rg;
+    s.sigma_update_count++;
+}
+
+void perform_restart(S &s)
+{   // :908-1020 (restart target: the T-output)
+    for (int i = 0; i < s.n; ++i) s.dx[i] = s.xT[i] - s.x0[i];
+    for (int i = 0; i < s.m; ++i) s.dy[i] = s.yT[i] - s.y0[i];
+    const double pd = nrm2(s.dx), dd = nrm2(s.dy);
+    double normal = INFINITY;
+    if (pd > 1e-16 && std::isfinite(pd)) {
+        spmv(s.A, s.dx.data(), s.prim_prod.data());
+        normal = nrm2(s.prim_prod) / pd;
+    }
+    const double ratio = s.rel_d / s.rel_p;
+    const auto &rp = s.P.restart_params;
+    if (pd > 1e-16 && dd > 1e-16 && pd < 1e12 && dd < 1e12 && ratio > 1e-8 && ratio < 1e8) {
+        const double err = std::log(dd) - std::log(pd) - std::log(s.pw);
+        s.pw_err_sum *= rp.i_smooth;
+        s.pw_err_sum += err;
+        const double de = err - s.pw_last_err;
+        s.pw *= std::exp(rp.k_p * err + rp.k_i * s.pw_err_sum + rp.k_d * de);
+        s.pw_last_err = err;
+    } else {
+        s.pw = s.best_pw;
+        s.pw_err_sum = 0.0;
+        s.pw_last_err = 0.0;
+    }
+    const double g = std::fabs(std::log10(s.rel_d / s.rel_p));
+    if (g < s.best_gap) {
+        s.best_gap = g;
+        s.best_pw = s.pw;
+    }
+    update_al_sigma(s, pd, dd, normal);
+    s.x0 = s.xT;
+    s.y0 = s.yT;
+    s.x = s.xT;
+    s.y = s.yT;
+    s.inner = 0;
+    s.last_trial_fp = INFINITY;
+}
+
+void init_state(S &s, const qp_problem_t *w)
+{   // solver_state.cu:494-877 after rescale_problem (preconditioner.c)
+    s.n = w->num_variables;
+    s.m = w->num_constraints;
+    s.r = w->num_rank_lowrank_obj;
+    s.obj_const = w->objective_constant;
+    s.A = from_component(w->constraint_matrix, s.m, s.n, w->constraint_matrix_num_nonzeros);
+    s.Q = from_component(w->objective_sparse_matrix, s.n, s.n, w->objective_sparse_matrix_num_nonzeros);
+    s.R = from_component(w->objective_lowrank_matrix, s.r, s.n, w->objective_lowrank_matrix_num_nonzeros);
+    s.c.assign(w->objective_vector, w->objective_vector + s.n);
+    s.lb.assign(w->variable_lower_bound, w->variable_lower_bound + s.n);
+    s.ub.assign(w->variable_upper_bound, w->variable_upper_bound + s.n);
+    s.l.assign(w->constraint_lower_bound, w->constraint_lower_bound + s.m);
+    s.u.assign(w->constraint_upper_bound, w->constraint_upper_bound + s.m);
+    const auto t0 = std::chrono::steady_clock::now();
+    rescale(s);
+    s.rescale_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    s.At = transpose(s.A);
+    s.Rt = transpose(s.R);
+    s.al_available = s.P.use_al_qp;
+    s.use_al = s.P.use_al_qp;
+    s.al_sigma = s.P.al_sigma;
+    s.best_al_sigma = s.P.al_sigma;
+    for (Vec *v : {&s.x0, &s.x, &s.xT, &s.xR, &s.dual_prod, &s.dslack, &s.dres, &s.dx, &s.qx, &s.grad, &s.dir, &s.gbuf})
+        v->assign(s.n, 0.0);
+    for (Vec *v : {&s.y0, &s.y, &s.yT, &s.yR, &s.prim_prod, &s.al_u, &s.pslack, &s.pres, &s.dy}) v->assign(s.m, 0.0);
+    s.rx.assign(s.r, 0.0);
+    if (w->primal_start)
+        for (int i = 0; i < s.n; ++i) s.x0[i] = s.x[i] = s.xT[i] = w->primal_start[i] * s.D[i] * s.b_scale;
+    if (w->dual_start)
+        for (int i = 0; i < s.m; ++i) s.y0[i] = s.y[i] = s.yT[i] = w->dual_start[i] * s.E[i] * s.c_scale;
+    s.lfin.resize(s.m);
+    s.ufin.resize(s.m);
+    for (int i = 0; i < s.m; ++i) {
+        s.lfin[i] = std::isfinite(s.l[i]) ? s.l[i] : 0.0;
+        s.ufin[i] = std::isfinite(s.u[i]) ? s.u[i] : 0.0;
+    }
+    const bool linf = s.P.optimality_norm == NORM_TYPE_L_INF;
+    double sq = 0.0, mx = 0.0;
+    for (int i = 0; i < s.n; ++i) {
+        if (linf) mx = std::max(mx, std::fabs(w->objective_vector[i]));
+        else sq += w->objective_vector[i] * w->objective_vector[i];
+    }
+    s.c_norm = linf ? mx : std::sqrt(sq);
+    sq = mx = 0.0;
+    for (int i = 0; i < s.m; ++i) {
+        const double lo = w->constraint_lower_bound[i], hi = w->constraint_upper_bound[i];
+        if (linf) {
+            if (std::isfinite(lo) && lo != hi) mx = std::max(mx, std::fabs(lo));
+            if (std::isfinite(hi)) mx = std::max(mx, std::fabs(hi));
+        } else {
+            if (std::isfinite(lo) && lo != hi) sq += lo * lo;
+            if (std::isfinite(hi)) sq += hi * hi;
+        }
+    }
+    s.b_norm = linf ? mx : std::sqrt(sq);
+    // quadratic term (solver_state.cu:220-303, 462-492)
+    s.qtype = detect(s);
+    if (s.qtype == FA_CP_DIAG_Q) {
+        s.qdiag.assign(s.n, 0.0);
+        for (int i = 0; i < s.n; ++i)
+            for (int k = s.Q.rp[i]; k < s.Q.rp[i + 1]; ++k)
+                if (s.Q.ci[k] < s.n) s.qdiag[s.Q.ci[k]] = s.Q.v[k] + 1e-12;
+    }
+    if (s.qtype == FA_CP_SPARSE_Q) {
+        s.q_norm = q_max_eig(s);
+        s.nonconvexity = q_min_eig(s, s.q_norm);
+    } else if (s.qtype == FA_CP_DIAG_Q) {
+        double mxe = 0.0, mne = 0.0;
+        for (double d : s.qdiag) {
+            mxe = std::fmax(mxe, std::fabs(d));
+            mne = std::fmin(mne, d);
+        }
+        s.q_norm = mxe;
+        s.nonconvexity = mne;
+    }
+    if (s.al_available && s.P.al_sigma <= s.P.sigma_zero_threshold) {
+        s.use_al = false;
+        s.al_sigma = 0.0;
+    }
+    // initialize_inner_solver, solver_state.cu:305-360
+    s.has_inner = s.use_al || s.al_available;
+    const bool simple = s.qtype == FA_CP_NON_Q || s.qtype == FA_CP_DIAG_Q;
+    if (!simple) s.has_inner = true;
+    if (!simple || s.al_available) {
+        s.inner_limit = 1000;
+        s.inner_tol = 1e-3;
+    }
+    s.is_lp = s.qtype == FA_CP_NON_Q;
+}
+
+}  // namespace
+
+/* ---------------- public C API ---------------- */
+
+ORACLE_API void fa_cp_oracle_reset_rng(void)
+{
+    g_gen.seed(1);
+    g_dist.reset();
+    srand(1);
+}
+
+ORACLE_API void set_default_parameters(fa_cp_parameters_t *p)  // utils.cu:494-537
+{
+    memset(p, 0, sizeof(*p));
+    p->l_inf_ruiz_iterations = 10;
+    p->has_pock_chambolle_alpha = true;
+    p->pock_chambolle_alpha = 1.0;
+    p->bound_objective_rescaling = true;
+    p->trajectory_csv = nullptr;
+    p->use_al_qp = true;
+    p->al_sigma = 0.0;
+    p->sigma_update_rule = FA_CP_SIGMA_UPDATE_GUARDED;
+    p->sigma_update_gain = 0.5;
+    p->sigma_clip_low = 0.6;
+    p->sigma_clip_high = 1.7;
+    p->sigma_min = 1e-8;
+    p->sigma_max = 10.0;
+    p->sigma_rollback_tol = 1.25;
+    p->sigma_zero_threshold = 1e-8;
+    p->sigma_activation_threshold = 2.0;
+    p->sigma_normal_motion_threshold = 1e-10;
+    p->sigma_activation_value = 0.1;
+    p->verbose = 1;
+    p->termination_evaluation_frequency = 200;
+    p->feasibility_polishing = false;
+    p->reflection_coefficient = 1.0;
+    p->sv_max_iter = 5000;
+    p->sv_tol = 1e-4;
+    p->termination_criteria.eps_optimal_relative = 1e-4;
+    p->termination_criteria.eps_feasible_relative = 1e-4;
+    p->termination_criteria.eps_infeasible = 1e-10;
+    p->termination_criteria.time_sec_limit = 3600.0;
+    p->termination_criteria.iteration_limit = INT32_MAX;
+    p->termination_criteria.eps_feas_polish_relative = 1e-6;
+    p->restart_params.artificial_restart_threshold = 0.36;
+    p->restart_params.sufficient_reduction_for_restart = 0.2;
+    p->restart_params.necessary_reduction_for_restart = 0.8;
+    p->restart_params.k_p = 0.99;
+    p->restart_params.k_i = 0.01;
+    p->restart_params.k_d = 0.0;
+    p->restart_params.i_smooth = 0.3;
+    p->optimality_norm = NORM_TYPE_L_INF;
+}
+
+static bool to_csr(const matrix_desc_t *d, int rows_expected, CsrComponent *out, int *nnz_out)
+{   // fa_cp.c:60-270 with utils.cu:1135-1358 (zero_tolerance rules)
+    const int m = d->m, n = d->n;
+    std::vector<int> ri, cj;
+    Vec vv;
+    switch (d->fmt) {
+    case matrix_dense: {
+        const double tol = d->zero_tolerance > 0 ? d->zero_tolerance : 1e-12;
+        for (int i = 0; i < m; ++i)
+            for (int j = 0; j < n; ++j) {
+                const double a = d->data.dense.A[(long)i * n + j];
+                if (std::fabs(a) > tol) {
+                    ri.push_back(i);
+                    cj.push_back(j);
+                    vv.push_back(a);
+                }
+            }
+        break;
+    }
+    case matrix_csr: {
+        const int nnz = d->data.csr.nnz;
+        out->row_ptr = (int *)malloc((size_t)(m + 1) * sizeof(int));
+        out->col_ind = (int *)malloc((size_t)std::max(nnz, 1) * sizeof(int));
+        out->val = (double *)malloc((size_t)std::max(nnz, 1) * sizeof(double));
+        memcpy(out->row_ptr, d->data.csr.row_ptr, (size_t)(m + 1) * sizeof(int));
+        if (nnz) {
+            memcpy(out->col_ind, d->data.csr.col_ind, (size_t)nnz * sizeof(int));
+            memcpy(out->val, d->data.csr.vals, (size_t)nnz * sizeof(double));
+        }
+        *nnz_out = nnz;
+        return true;
+    }
+    case matrix_csc:
+    case matrix_coo: {
+        const bool csc = d->fmt == matrix_csc;
+        const double tol = d->zero_tolerance > 0 ? d->zero_tolerance : 0.0;
+        const int nnz = csc ? d->data.csc.nnz : d->data.coo.nnz;
+        const double *vals = csc ? d->data.csc.vals : d->data.coo.vals;
+        for (int k = 0, j = 0; k < nnz; ++k) {
+            int r, c;
+            if (csc) {
+                while (j < n && k >= d->data.csc.col_ptr[j + 1]) ++j;
+                r = d->data.csc.row_ind[k];
+                c = j;
+            } else {
+                r = d->data.coo.row_ind[k];
+                c = d->data.coo.col_ind[k];
+            }
+            if (r < 0 || r >= m || c < 0 || c >= n) return false;
+            if (tol > 0 && std::fabs(vals[k]) <= tol) continue;
+            ri.push_back(r);
+            cj.push_back(c);
+            vv.push_back(vals[k]);
+        }
+        break;
+    }
+    default: return false;
+    }
+    (void)rows_expected;
+    const int nnz = (int)vv.size();
+    out->row_ptr = (int *)calloc((size_t)m + 1, sizeof(int));
+    out->col_ind = (int *)malloc((size_t)std::max(nnz, 1) * sizeof(int));
+    out->val = (double *)malloc((size_t)std::max(nnz, 1) * sizeof(double));
+    for (int k = 0; k < nnz; ++k) out->row_ptr[ri[k] + 1]++;
+    for (int i = 0; i < m; ++i) out->row_ptr[i + 1] += out->row_ptr[i];
+    std::vector<int> next(out->row_ptr, out->row_ptr + m);
+    for (int k = 0; k < nnz; ++k) {
+        const int p = next[ri[k]]++;
+        out->col_ind[p] = cj[k];
+        out->val[p] = vv[k];
+    }
+    *nnz_out = nnz;
+    return true;
+}
+
+static void fill_or_copy(double **dst, int n, const double *src, double fill)
+{
+    *dst = (double *)malloc((size_t)std::max(n, 1) * sizeof(double));
+    if (src) memcpy(*dst, src, (size_t)n * sizeof(double));
+    else
+        for (int i = 0; i < n; ++i) (*dst)[i] = fill;
+}
+
+static void free_component(CsrComponent *c)
+{
+    if (!c) return;
+    free(c->row_ptr);
+    free(c->col_ind);
+    free(c->val);
+    free(c);
+}
+
+ORACLE_API void qp_problem_free(qp_problem_t *p)
+{
+    if (!p) return;
+    free_component(p->constraint_matrix);
+    free_component(p->objective_sparse_matrix);
+    free_component(p->objective_lowrank_matrix);
+    free(p->variable_lower_bound);
+    free(p->variable_upper_bound);
+    free(p->objective_vector);
+    free(p->constraint_lower_bound);
+    free(p->constraint_upper_bound);
+    free(p->primal_start);
+    free(p->dual_start);
+    free(p);
+}
+
+ORACLE_API qp_problem_t *create_qp_problem(const double *c, const matrix_desc_t *Q, const matrix_desc_t *R,
+                                           const matrix_desc_t *A, const double *con_lb, const double *con_ub,
+                                           const double *var_lb, const double *var_ub, const double *obj_const)
+{   // fa_cp.c:28-288
+    int n, m = 0;
+    if (A) {
+        n = A->n;
+        m = A->m;
+    } else if (Q) {
+        n = Q->n;
+    } else if (R) {
+        n = R->n;
+    } else {
+        fprintf(stderr, "[interface] Error: At least one matrix (A, Q, or R) must be provided.\n");
+        return nullptr;
+    }
+    qp_problem_t *p = (qp_problem_t *)calloc(1, sizeof(qp_problem_t));
+    p->num_variables = n;
+    p->num_constraints = m;
+    p->constraint_matrix = (CsrComponent *)calloc(1, sizeof(CsrComponent));
+    p->objective_sparse_matrix = (CsrComponent *)calloc(1, sizeof(CsrComponent));
+    p->objective_lowrank_matrix = (CsrComponent *)calloc(1, sizeof(CsrComponent));
+    auto empty = [](CsrComponent *cc) { cc->row_ptr = (int *)calloc(1, sizeof(int)); };
+    bool ok = true;
+    if (A) ok = ok && to_csr(A, m, p->constraint_matrix, &p->constraint_matrix_num_nonzeros);
+    else empty(p->constraint_matrix);
+    if (Q) ok = ok && to_csr(Q, n, p->objective_sparse_matrix, &p->objective_sparse_matrix_num_nonzeros);
+    else empty(p->objective_sparse_matrix);
+    if (R) {
+        p->num_rank_lowrank_obj = R->m;
+        ok = ok && to_csr(R, R->m, p->objective_lowrank_matrix, &p->objective_lowrank_matrix_num_nonzeros);
+    } else {
+        empty(p->objective_lowrank_matrix);
+    }
+    if (!ok) {
+        fprintf(stderr, "[interface] matrix conversion failed.\n");
+        qp_problem_free(p);
+        return nullptr;
+    }
+    p->objective_constant = obj_const ? *obj_const : 0.0;
+    fill_or_copy(&p->objective_vector, n, c, 0.0);
+    fill_or_copy(&p->variable_lower_bound, n, var_lb, -INFINITY);
+    fill_or_copy(&p->variable_upper_bound, n, var_ub, INFINITY);
+    fill_or_copy(&p->constraint_lower_bound, m, con_lb, -INFINITY);
+    fill_or_copy(&p->constraint_upper_bound, m, con_ub, INFINITY);
+    return p;
+}
+
+ORACLE_API void set_start_values(qp_problem_t *p, const double *primal, const double *dual)
+{
+    if (!p) return;
+    free(p->primal_start);
+    free(p->dual_start);
+    p->primal_start = p->dual_start = nullptr;
+    if (primal) {
+        p->primal_start = (double *)malloc((size_t)std::max(p->num_variables, 1) * sizeof(double));
+        memcpy(p->primal_start, primal, (size_t)p->num_variables * sizeof(double));
+    }
+    if (dual) {
+        p->dual_start = (double *)malloc((size_t)std::max(p->num_constraints, 1) * sizeof(double));
+        memcpy(p->dual_start, dual, (size_t)p->num_constraints * sizeof(double));
+    }
+}
+
+ORACLE_API void fa_cp_result_free(fa_cp_result_t *r)
+{
+    if (!r) return;
+    free(r->primal_solution);
+    free(r->dual_solution);
+    free(r->reduced_cost);
+    free(r);
+}
+
+ORACLE_API fa_cp_result_t *optimize(const fa_cp_parameters_t *params, const qp_problem_t *prob)
+{   // solver.cu:74-168
+    if (!params || !prob) return nullptr;
+    S s;
+    s.P = *params;
+    init_state(s, prob);
+    if (s.nonconvexity < 0) s.inner_limit = 1;
+    // initialize_step_size_and_primal_weight, :1022-1040
+    if (s.A.nnz() == 0) s.step = 1.0;
+    else s.step = 0.998 / max_singular_value(s);
+    s.pw = s.P.bound_objective_rescaling ? 1.0 : (s.c_norm + 1.0) / (s.b_norm + 1.0);
+    s.best_pw = s.pw;
+
+    const auto t0 = std::chrono::steady_clock::now();
+    const int K = s.P.termination_evaluation_frequency;
+    bool do_restart = false;
+    while (s.total < s.P.termination_criteria.iteration_limit) {
+        const bool consider = s.major || s.total == 0;
+        bool planned = false;
+        if (consider || s.total % print_frequency(s.total) == 0) {
+            compute_residual(s, s.P.optimality_norm);
+            if (s.major && s.total < 3 * K) infeasibility_info(s);
+            s.solve_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            check_termination(s);
+            if (consider && s.reason == TERMINATION_REASON_UNSPECIFIED) planned = should_restart(s);
+            if (s.reason != TERMINATION_REASON_UNSPECIFIED) break;
+        }
+        if (consider) {
+            do_restart = planned;
+            if (do_restart) perform_restart(s);
+        }
+        s.major = ((s.total + 1) % K) == 0;
+        fa_cp_update(s);
+        if (s.major || do_restart) {
+            fixed_point_error(s);
+            if (do_restart) {
+                s.fp0 = s.fp;
+                do_restart = false;
+            }
+        }
+        halpern(s);
+        s.inner++;
+        s.total++;
+    }
+    if (s.reason == TERMINATION_REASON_UNSPECIFIED) {
+        s.reason = TERMINATION_REASON_ITERATION_LIMIT;
+        compute_residual(s, s.P.optimality_norm);
+    }
+    // create_result_from_state, :1171-1239
+    fa_cp_result_t *r = (fa_cp_result_t *)calloc(1, sizeof(fa_cp_result_t));
+    spmv(s.At, s.yT.data(), s.dual_prod.data());
+    r->primal_solution = (double *)malloc((size_t)std::max(s.n, 1) * sizeof(double));
+    r->dual_solution = (double *)malloc((size_t)std::max(s.m, 1) * sizeof(double));
+    r->reduced_cost = (double *)malloc((size_t)std::max(s.n, 1) * sizeof(double));
+    for (int j = 0; j < s.n; ++j) {
+        r->reduced_cost[j] = (s.c[j] - s.dual_prod[j]) * s.D[j] / s.c_scale;
+        r->primal_solution[j] = s.xT[j] / s.D[j] / s.b_scale;
+    }
+    for (int i = 0; i < s.m; ++i) r->dual_solution[i] = s.yT[i] / s.E[i] / s.c_scale;
+    r->num_variables = prob->num_variables;
+    r->num_constraints = prob->num_constraints;
+    r->num_nonzeros = prob->constraint_matrix_num_nonzeros;
+    r->total_count = s.total;
+    r->total_inner_count = (int)s.inner_total;
+    r->rescaling_time_sec = s.rescale_time;
+    r->cumulative_time_sec = s.solve_time;
+    r->absolute_primal_residual = s.abs_p;
+    r->relative_primal_residual = s.rel_p;
+    r->absolute_dual_residual = s.abs_d;
+    r->relative_dual_residual = s.rel_d;
+    r->primal_objective_value = s.p_obj;
+    r->dual_objective_value = s.d_obj;
+    r->objective_gap = s.gap;
+    r->relative_objective_gap = s.rel_gap;
+    r->max_primal_ray_infeasibility = s.max_p_ray;
+    r->max_dual_ray_infeasibility = s.max_d_ray;
+    r->primal_ray_linear_objective = s.p_ray_obj;
+    r->dual_ray_objective = s.d_ray_obj;
+    r->final_al_sigma = s.al_sigma;
+    r->sigma_update_count = s.sigma_update_count;
+    r->sigma_update_rule = s.al_available ? s.P.sigma_update_rule : FA_CP_SIGMA_UPDATE_FIXED;
+    r->termination_reason = s.reason;
+    return r;
+}
+
+ORACLE_API fa_cp_result_t *solve_qp_problem(const qp_problem_t *prob, const fa_cp_parameters_t *params)
+{   // fa_cp.c:352-376
+    if (!prob) {
+        fprintf(stderr, "[interface] solve_qp_problem: invalid arguments.\n");
+        return nullptr;
+    }
+    fa_cp_parameters_t local;
+    if (params) local = *params;
+    else set_default_parameters(&local);
+    return optimize(&local, prob);
+}

@@ -60,6 +60,18 @@ __device__ __forceinline__ void row_dot2(const CsrView &M, int row, int lane, co
     s_b = group_sum<G>(t);
 }
 
+// Plain y = M x (used to time lane counts at set-up).
+template <int G>
+__global__ void __launch_bounds__(kBlock) spmv_kernel(CsrView M, const double *__restrict__ x, double *__restrict__ y)
+{
+    const int tid = blockIdx.x * kBlock + threadIdx.x;
+    const int row = tid / G;
+    const int lane = tid & (G - 1);
+    if (row >= M.rows) return;
+    const double s = row_dot<G>(M, row, lane, x);
+    if (lane == 0) y[row] = s;
+}
+
 struct PrimalArgs {
     CsrView At;                      // n rows
     const double *__restrict__ yv;   // y, or the AL predictor u when beta > 0

@@ -113,6 +113,7 @@ class Solver {
     // set-up
     void upload();
     void rescale();
+    void autotune_lanes();
     double power_iteration();
     void init_steps();
     // iteration
@@ -376,6 +377,53 @@ double Solver::power_iteration()
     return std::sqrt(st.sigma_sq);
 }
 
+// Rows-per-wavefront autotuner: time a plain SpMV of each matrix for every
+// lane count in {1,..,32} (hipEvents, 3 launches after a warm-up) and keep the
+// fastest.  An explicit PDAL_SPMV_G_A / PDAL_SPMV_G_AT setting wins.
+void Solver::autotune_lanes()
+{
+    if (nnz_ == 0 || (long)nnz_ < env_int("PDAL_AUTOTUNE_MIN_NNZ", 1 << 18)) return;
+    DBuf<double> xa(std::max(n_, m_)), ya(std::max(n_, m_));
+    PDAL_HIP(hipMemsetAsync(xa.get(), 0, (size_t)std::max(n_, m_) * sizeof(double), stream_));
+    hipEvent_t e0, e1;
+    PDAL_HIP(hipEventCreate(&e0));
+    PDAL_HIP(hipEventCreate(&e1));
+    auto tune = [&](DevCsr &M, const char *env) {
+        if (getenv(env)) return;
+        float best = 1e30f;
+        int best_g = M.G;
+        for (int G : {1, 2, 4, 8, 16, 32}) {
+            auto launch = [&]() {
+                dispatch_group(G, [&](auto g) {
+                    constexpr int GG = decltype(g)::value;
+                    spmv_kernel<GG><<<blocks_for((long)M.rows * GG), kBlock, 0, stream_>>>(M.view(), xa.get(), ya.get());
+                });
+            };
+            launch();
+            PDAL_HIP(hipEventRecord(e0, stream_));
+            for (int r = 0; r < 3; ++r) launch();
+            PDAL_HIP(hipEventRecord(e1, stream_));
+            PDAL_LAUNCHED();
+            PDAL_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            PDAL_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) {
+                best = ms;
+                best_g = G;
+            }
+        }
+        M.G = best_g;
+    };
+    tune(At_, "PDAL_SPMV_G_AT");
+    tune(A_, "PDAL_SPMV_G_A");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    eval_grid_at_ = std::max(1, std::min(2048, blocks_for((long)n_ * At_.G)));
+    eval_grid_a_ = std::max(1, std::min(2048, blocks_for((long)m_ * A_.G)));
+    parts_at_.alloc((size_t)eval_grid_at_ * kEvalStride);
+    parts_a_.alloc((size_t)eval_grid_a_ * kEvalStride);
+}
+
 void Solver::refresh_step_schedule()
 {   // solver.cu:3103-3126 (alpha mode)
     const double margin = env_double("CP_AL_STEP_MARGIN", 0.998 * 0.998);
@@ -408,6 +456,7 @@ void Solver::setup()
     const auto t0 = std::chrono::steady_clock::now();
     upload();
     rescale();
+    autotune_lanes();
     x_start_.alloc(n_);
     y_start_.alloc(m_);
     // anchor = current = T-output = Halpern point = start (solver.cu:1954-1965)

@@ -121,3 +121,99 @@ def config3_sizes(count: int = 4096, seed: int = 12345, lo: int = 200, hi: int =
 def config3_instance(i: int, m: int) -> PlantedLP:
     """Instance i of the batched family: n = 2m, 8 nonzeros per column, seed i."""
     return planted_lp(int(m), 2 * int(m), 8, seed=i)
+
+
+@dataclass
+class PlantedQP:
+    """min 1/2 x^T (Q + R^T R) x + c^T x  s.t. l <= A x <= u, lb <= x <= ub with
+    a planted KKT point in the solver's sign convention (dual residual
+    Qx + c - A^T y - lam = 0, y >= 0 on lower-active rows, y <= 0 on
+    upper-active rows, lam >= 0 at x = lb, lam <= 0 at x = ub)."""
+    A: sp.csr_matrix
+    M: sp.csr_matrix          # low-rank factor: Q_total = M^T M + reg I
+    reg: float
+    c: np.ndarray
+    l: np.ndarray
+    u: np.ndarray
+    lb: np.ndarray
+    ub: np.ndarray
+    x_star: np.ndarray
+    y_star: np.ndarray
+    lam_star: np.ndarray
+    seed: int
+
+    @property
+    def n(self) -> int:
+        return self.A.shape[1]
+
+    @property
+    def m(self) -> int:
+        return self.A.shape[0]
+
+    def q_explicit(self) -> sp.csr_matrix:
+        Q = (self.M.T @ self.M + self.reg * sp.identity(self.n, format="csr")).tocsr()
+        Q.sort_indices()
+        return Q
+
+    def q_diag_part(self) -> sp.csr_matrix:
+        return (self.reg * sp.identity(self.n, format="csr")).tocsr()
+
+    @property
+    def optimal_value(self) -> float:
+        x = self.x_star
+        return float(0.5 * (np.sum((self.M @ x) ** 2) + self.reg * x @ x) + self.c @ x)
+
+    def qp_args(self, form: str = "explicit") -> dict:
+        """Keyword arguments of FaCp.solve: "explicit" passes Q = M^T M + reg I;
+        "lowrank" passes Q = reg I and R = M."""
+        base = dict(c=self.c, A=self.A, con_lb=self.l, con_ub=self.u, var_lb=self.lb, var_ub=self.ub)
+        if form == "explicit":
+            return dict(base, Q=self.q_explicit())
+        return dict(base, Q=self.q_diag_part(), R=self.M)
+
+
+def planted_qp(n: int, m: int, r: int, seed: int, a_nnz_col: int = 8, m_nnz_col: int = 5, reg: float = 1e-3,
+               box: float = 10.0, bound_frac: float = 0.3) -> PlantedQP:
+    """Planted convex QP of the config-2 family (n=1e5, m=5e4, r=2e4 at full size)."""
+    rng = np.random.default_rng(seed)
+
+    def sparse_cols(rows, cols, k):
+        counts = np.full(cols, k, dtype=np.int64)
+        indptr = np.zeros(cols + 1, dtype=np.int64)
+        np.cumsum(counts, out=indptr[1:])
+        cidx = np.repeat(np.arange(cols, dtype=np.int64), counts)
+        ridx = _distinct_rows(rng, cidx, rows)
+        vals = rng.standard_normal(cidx.size)
+        S = sp.csc_matrix((vals, ridx, indptr), shape=(rows, cols)).tocsr()
+        S.sort_indices()
+        return S
+
+    A = sparse_cols(m, n, a_nnz_col)
+    M = sparse_cols(r, n, m_nnz_col)
+    x = rng.uniform(0.0, box, n)
+    perm = rng.permutation(n)
+    nb = int(n * bound_frac)
+    at_lo, at_hi = perm[: nb // 2], perm[nb // 2: nb]
+    x[at_lo] = 0.0
+    x[at_hi] = box
+    lam = np.zeros(n)
+    lam[at_lo] = rng.uniform(0.1, 1.0, at_lo.size)
+    lam[at_hi] = -rng.uniform(0.1, 1.0, at_hi.size)
+    ax = A @ x
+    kind = rng.integers(0, 3, m)  # 0: lower-active (y>0), 1: upper-active (y<0), 2: slack (y=0)
+    y = np.zeros(m)
+    y[kind == 0] = rng.uniform(0.1, 1.0, int(np.sum(kind == 0)))
+    y[kind == 1] = -rng.uniform(0.1, 1.0, int(np.sum(kind == 1)))
+    width = rng.uniform(0.5, 2.0, m)
+    l = np.where(kind == 0, ax, ax - width)
+    u = np.where(kind == 1, ax, ax + width)
+    qx = M.T @ (M @ x) + reg * x
+    c = -qx + A.T @ y + lam
+    return PlantedQP(A=A, M=M, reg=reg, c=c, l=l, u=u, lb=np.zeros(n), ub=np.full(n, box), x_star=x, y_star=y,
+                     lam_star=lam, seed=seed)
+
+
+def config2(seed: int = 2, scale: float = 1.0) -> PlantedQP:
+    """n=1e5, m=5e4, M 2e4 x n (5 nnz/col), A 8 nnz/col; `scale` shrinks the sizes."""
+    n = int(round(100_000 * scale))
+    return planted_qp(n, n // 2, max(n // 5, 1), seed)
