@@ -14,6 +14,7 @@
 #pragma once
 
 #include "../common/device_reduce.hpp"
+#include "../common/spmv.hpp"
 
 namespace pdal {
 namespace lp {
@@ -25,52 +26,6 @@ struct IterScalars {  // device copy of the step schedule, written once per bloc
     int inner;        // inner_count at block start (d_inner_count)
     int pad;
 };
-
-struct CsrView {
-    const int *__restrict__ rp;
-    const int *__restrict__ ci;
-    const double *__restrict__ v;
-    int rows;
-};
-
-template <int G>
-__device__ __forceinline__ double row_dot(const CsrView &M, int row, int lane, const double *__restrict__ x)
-{
-    double s = 0.0;
-    const int b = M.rp[row], e = M.rp[row + 1];
-    for (int k = b + lane; k < e; k += G) s += M.v[k] * x[M.ci[k]];
-    return group_sum<G>(s);
-}
-
-template <int G>
-__device__ __forceinline__ void row_dot2(const CsrView &M, int row, int lane, const double *__restrict__ xa,
-                                         const double *__restrict__ xb, double &s_diff, double &s_b)
-{   // s_diff = sum a*(xa - xb) (the difference is rounded first, as a materialised
-    // delta vector would be); s_b = sum a*xb.
-    double d = 0.0, t = 0.0;
-    const int b = M.rp[row], e = M.rp[row + 1];
-    for (int k = b + lane; k < e; k += G) {
-        const int c = M.ci[k];
-        const double a = M.v[k];
-        const double xbv = xb[c];
-        d += a * (xa[c] - xbv);
-        t += a * xbv;
-    }
-    s_diff = group_sum<G>(d);
-    s_b = group_sum<G>(t);
-}
-
-// Plain y = M x (used to time lane counts at set-up).
-template <int G>
-__global__ void __launch_bounds__(kBlock) spmv_kernel(CsrView M, const double *__restrict__ x, double *__restrict__ y)
-{
-    const int tid = blockIdx.x * kBlock + threadIdx.x;
-    const int row = tid / G;
-    const int lane = tid & (G - 1);
-    if (row >= M.rows) return;
-    const double s = row_dot<G>(M, row, lane, x);
-    if (lane == 0) y[row] = s;
-}
 
 struct PrimalArgs {
     CsrView At;                      // n rows

@@ -299,20 +299,6 @@ void Solver::rescale()
     rescale_time_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
-template <class F>
-void dispatch_group(int G, F &&f)
-{
-    switch (G) {
-    case 1: f(std::integral_constant<int, 1>{}); break;
-    case 2: f(std::integral_constant<int, 2>{}); break;
-    case 4: f(std::integral_constant<int, 4>{}); break;
-    case 8: f(std::integral_constant<int, 8>{}); break;
-    case 16: f(std::integral_constant<int, 16>{}); break;
-    case 32: f(std::integral_constant<int, 32>{}); break;
-    default: f(std::integral_constant<int, 64>{}); break;
-    }
-}
-
 double Solver::power_iteration()
 {   // estimate_maximum_singular_value, utils.cu:87-228
     const int max_iter = P_.sv_max_iter;
