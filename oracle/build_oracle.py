@@ -1,6 +1,8 @@
 """Build the CPU oracle shared objects (test infrastructure, gcc only).
 
-Output: oracle/_build/libcp_al_oracle.so and libfa_cp_oracle.so.  The oracle exports the same C names
+Output: oracle/_build/libcp_al_oracle.so, libfa_cp_oracle.so and
+libfa_cp_oracle_rev.so (the FA_CP oracle summing in reverse order, the
+summation-order noise floor of a trajectory).  The oracle exports the same C names
 as the product library; it is built with hidden visibility and -Bsymbolic so a
 process can load both without the two binding to each other's symbols.
 
@@ -22,11 +24,11 @@ FLAGS = ["-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-fvisibi
          "-Wl,-Bsymbolic", f"-I{ROOT / 'include'}"]
 
 
-def _one(src: str, name: str) -> Path:
+def _one(src: str, name: str, defines=()) -> Path:
     OUT.mkdir(exist_ok=True)
     lib = OUT / name
     tmp = OUT / (name + ".tmp")
-    cmd = ["g++", *FLAGS, str(HERE / src), "-o", str(tmp)]
+    cmd = ["g++", *FLAGS, *defines, str(HERE / src), "-o", str(tmp)]
     print("+", " ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, lib)
@@ -35,6 +37,7 @@ def _one(src: str, name: str) -> Path:
 
 def build() -> Path:
     _one("fa_cp_oracle.cpp", "libfa_cp_oracle.so")
+    _one("fa_cp_oracle.cpp", "libfa_cp_oracle_rev.so", ["-DFA_CP_ORACLE_REVERSE_SUMS"])
     return _one("cp_al_oracle.cpp", "libcp_al_oracle.so")
 
 

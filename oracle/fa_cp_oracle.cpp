@@ -23,6 +23,11 @@
  * Deliberate differences shared with the HIP path: wall-clock timing, no
  * trajectory CSV, no feasibility polishing, restart-select env knobs ignored,
  * errors return NULL instead of exit().
+ *
+ * Built a second time with -DFA_CP_ORACLE_REVERSE_SUMS, every SpMV row, dot
+ * product and norm sums its terms in reverse order.  That variant is the same
+ * algorithm with a different (equally valid) rounding; tests use the distance
+ * between the two builds as the summation-order noise floor of a trajectory.
  */
 #include "fa_cp.h"
 
@@ -56,7 +61,11 @@ void spmv(const Csr &A, const double *x, double *y)
 {
     for (int i = 0; i < A.rows; ++i) {
         double s = 0.0;
+#ifdef FA_CP_ORACLE_REVERSE_SUMS
+        for (int k = A.rp[i + 1] - 1; k >= A.rp[i]; --k) s += A.v[k] * x[A.ci[k]];
+#else
         for (int k = A.rp[i]; k < A.rp[i + 1]; ++k) s += A.v[k] * x[A.ci[k]];
+#endif
         y[i] = s;
     }
 }
@@ -64,14 +73,22 @@ void spmv(const Csr &A, const double *x, double *y)
 double nrm2(const Vec &x)
 {
     double s = 0.0;
+#ifdef FA_CP_ORACLE_REVERSE_SUMS
+    for (size_t i = x.size(); i-- > 0;) s += x[i] * x[i];
+#else
     for (double a : x) s += a * a;
+#endif
     return std::sqrt(s);
 }
 
 double dot(const Vec &a, const Vec &b)
 {
     double s = 0.0;
+#ifdef FA_CP_ORACLE_REVERSE_SUMS
+    for (size_t i = a.size(); i-- > 0;) s += a[i] * b[i];
+#else
     for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+#endif
     return s;
 }
 

@@ -3,6 +3,7 @@
 Products (HIP, gfx950):
   pdal_amd/lib/libcp_al.so   -- CP_AL LP solver behind include/cp_al.h
   pdal_amd/lib/cp_al         -- the LP runner command line
+  pdal_amd/lib/libfa_cp.so   -- FA_CP QP solver behind include/fa_cp.h
 Checkers (CPU, gcc), built by oracle/build_oracle.py, are separate.
 
 Run `python -m pdal_amd.build` (or __graft_entry__.build()).
@@ -81,8 +82,26 @@ def build_cp_al() -> Path:
     return out
 
 
+FA_CP_SOURCES = [
+    CSRC / "qp" / "fa_cp_api.cpp",
+    CSRC / "qp" / "fa_cp_solver.hip",
+]
+
+
+def build_fa_cp() -> Path:
+    objdir = ROOT / "pdal_amd" / "_build" / "fa_cp"
+    objs = _compile_objects(FA_CP_SOURCES, objdir)
+    LIB.mkdir(parents=True, exist_ok=True)
+    out = LIB / "libfa_cp.so"
+    tmp = LIB / "libfa_cp.so.tmp"
+    _run([_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-Bsymbolic", *objs, "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
 def build_all() -> None:
     build_cp_al()
+    build_fa_cp()
 
 
 if __name__ == "__main__":

@@ -17,6 +17,7 @@
 #include "cp_al.h"
 #include "cp_al_internal.hpp"
 
+#include "../common/csr_host.hpp"
 #include "../common/hip_util.hpp"
 #include "cp_al_kernels.hpp"
 
@@ -36,56 +37,6 @@ namespace {
 
 std::mt19937 g_gen(1);                              // utils.cu:27 (process-global)
 std::normal_distribution<double> g_dist(0.0, 1.0);  // utils.cu:28
-
-inline int blocks_for(long work) { return work > 0 ? (int)((work + kBlock - 1) / kBlock) : 1; }
-
-struct HostCsr {
-    int rows = 0, cols = 0;
-    std::vector<int> rp, ci;
-    std::vector<double> v;
-};
-
-// Stable CSR transpose: rows stay ascending inside each column, the order the
-// reference gets from cusparseCsr2cscEx2 (solver.cu:1859-1890).
-HostCsr transpose_csr(int m, int n, const int *rp, const int *ci, const double *v)
-{
-    HostCsr T;
-    T.rows = n;
-    T.cols = m;
-    const int nnz = rp[m];
-    T.rp.assign((size_t)n + 1, 0);
-    for (int k = 0; k < nnz; ++k) T.rp[ci[k] + 1]++;
-    for (int j = 0; j < n; ++j) T.rp[j + 1] += T.rp[j];
-    T.ci.resize(nnz);
-    T.v.resize(nnz);
-    std::vector<int> next(T.rp.begin(), T.rp.end() - 1);
-    for (int i = 0; i < m; ++i)
-        for (int k = rp[i]; k < rp[i + 1]; ++k) {
-            const int p = next[ci[k]]++;
-            T.ci[p] = i;
-            T.v[p] = v[k];
-        }
-    return T;
-}
-
-struct DevCsr {
-    DBuf<int> rp, ci;
-    DBuf<double> v;
-    int rows = 0, nnz = 0, G = 4;
-    CsrView view() const { return CsrView{rp.get(), ci.get(), v.get(), rows}; }
-};
-
-int pick_group(const char *env, long nnz, int rows)
-{   // lanes per row: a power of two giving about two nonzeros per lane
-    if (const char *raw = getenv(env)) {
-        const int g = atoi(raw);
-        if (g == 1 || g == 2 || g == 4 || g == 8 || g == 16 || g == 32 || g == 64) return g;
-    }
-    const double mean = rows > 0 ? (double)nnz / rows : 0.0;
-    int g = 1;
-    while (g < 64 && 2.0 * g <= mean / 2.0) g *= 2;
-    return g;
-}
 
 struct Eval {
     double cross, pn2, dres2, cx, lx, rx2, dresmax;

@@ -1,0 +1,19 @@
+// Internal interface between the FA_CP C API (fa_cp_api.cpp, host-only C++)
+// and the HIP solver (fa_cp_solver.hip).
+#pragma once
+
+#include "fa_cp.h"
+
+namespace pdal {
+namespace qp {
+
+// Solve on the current HIP device; throws pdal::HipError / std::exception on failure.
+fa_cp_result_t *optimize_gpu(const fa_cp_parameters_t *params, const qp_problem_t *prob);
+
+// Re-seed the power-iteration generators (std::mt19937(1) + normal cache, srand(1)).
+void reset_rng();
+
+const char *termination_reason_name(termination_reason_t r);
+
+}  // namespace qp
+}  // namespace pdal

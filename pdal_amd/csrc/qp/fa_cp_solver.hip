@@ -1,0 +1,1168 @@
+// FA_CP optimize(): host driver of the MI355X QP path.
+//
+// Reference: QP/FA_CP/src/solver.cu:74-168 (main loop), preconditioner.c:36-331
+// (scaling, here on the device), solver_state.cu:220-877 (state, Q handling,
+// inner solver), utils.cu:65-315 (power iterations), :438-537 (termination,
+// restart, defaults), :705-1132 (residual, infeasibility), fa_cp_core_op.cu:
+// 93-1239 (iteration kernels, BB inner solve, Halpern, restart, sigma rules,
+// fixed-point error, result).
+//
+// MI355X design: one HIP stream, FP64 throughout.  An iteration without an
+// inner solve is two launches: the At.y SpMV fused with the LP / diagonal-Q
+// prox and the A.x_bar SpMV fused with the dual update, each also applying the
+// Halpern step when no fixed-point evaluation needs the pre-Halpern point.
+// The projected-BB inner solve runs device-resident: its step size, direction
+// norm and stopping test live in a BBState on the device, every inner kernel
+// is gated on the state's done flag, and the host launches inner steps in
+// chunks and reads one flag per chunk.  Checkpoints (residual, infeasibility,
+// restart, fixed-point error) are SpMVs with fused deterministic reductions;
+// the host reads a few doubles and runs the scalar logic.
+#include "fa_cp.h"
+#include "fa_cp_internal.hpp"
+
+#include "../common/csr_host.hpp"
+#include "../common/hip_util.hpp"
+#include "fa_cp_kernels.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <utility>
+#include <vector>
+
+namespace pdal {
+namespace qp {
+
+namespace {
+
+std::mt19937 g_gen(1);                              // QP/FA_CP/src/utils.cu:28 (process-global)
+std::normal_distribution<double> g_dist(0.0, 1.0);  // utils.cu:29
+
+constexpr int kMaxPartBlocks = 2048;
+
+inline int stride_grid(long len) { return std::max(1, std::min(1024, blocks_for(len))); }
+inline int row_grid(long rows, int G) { return std::max(1, std::min(kMaxPartBlocks, blocks_for(rows * G))); }
+
+int print_frequency(int iter)  // utils.cu:690-703
+{
+    int step = 10;
+    long long threshold = 1000;
+    while (iter >= threshold) {
+        step *= 10;
+        threshold *= 10;
+    }
+    return step;
+}
+
+double sane_positive(double v, double fb) { return (std::isfinite(v) && v > 0.0) ? v : fb; }
+double score_component(double v) { return std::isfinite(v) ? std::fmax(v, 1e-300) : INFINITY; }
+double clampd_h(double v, double lo, double hi) { return std::fmin(std::fmax(v, lo), hi); }
+
+HostCsr host_component(const CsrComponent *c, int rows, int cols, int nnz)
+{
+    HostCsr M;
+    M.rows = rows;
+    M.cols = cols;
+    M.rp.assign((size_t)rows + 1, 0);
+    if (c && c->row_ptr && nnz > 0) {
+        M.rp.assign(c->row_ptr, c->row_ptr + rows + 1);
+        M.ci.assign(c->col_ind, c->col_ind + nnz);
+        M.v.assign(c->val, c->val + nnz);
+    }
+    return M;
+}
+
+// Scalar slots of the device reduction array.
+enum Slot {
+    S_RA = 0,                 // RA_N = 3
+    S_RT = S_RA + RA_N,       // RT_N = 9
+    S_FP = S_RT + RT_N,       // cross, pn2
+    S_DN2 = S_FP + FP_N,      // ||dy||^2
+    S_PROJ = S_DN2 + 1,       // max |dx|, max |dy|
+    S_RAY = S_PROJ + 2,       // RY_N = 5
+    S_QXMAX = S_RAY + RY_N,   // max |Q dx|
+    S_PD2 = S_QXMAX + 1,
+    S_DD2 = S_PD2 + 1,
+    S_NORMAL = S_DD2 + 1,
+    S_PW0 = S_NORMAL + 1,     // power iteration: sumsq, <v,Mv>, (unused), resid
+    S_BOUND = S_PW0 + 4,      // bound-objective sums
+    S_COUNT = S_BOUND + 2
+};
+
+class Solver {
+  public:
+    Solver(const fa_cp_parameters_t &p, const qp_problem_t *w) : P_(p), W_(w) {}
+    ~Solver()
+    {
+        if (stream_) (void)hipStreamDestroy(stream_);
+    }
+    fa_cp_result_t *run();
+
+  private:
+    // ---- set-up ----
+    void upload();
+    void rescale();
+    void scale_all(const double *e, const double *d);
+    void init_state();
+    double max_singular_value();
+    double q_max_eig();
+    double q_min_eig(double lmax);
+
+    // ---- device helpers ----
+    void reduce(int nparts, int nsum, int nq, int slot)
+    {
+        reduce_partials_kernel<<<1, kBlock, 0, stream_>>>(parts_.get(), nparts, kEvalStride, nsum, nq,
+                                                          dscal_.get() + slot);
+        PDAL_LAUNCHED();
+    }
+    void fetch(int lo, int cnt)
+    {
+        PDAL_HIP(hipMemcpyAsync(hscal_.get() + lo, dscal_.get() + lo, (size_t)cnt * sizeof(double),
+                                hipMemcpyDeviceToHost, stream_));
+        PDAL_HIP(hipStreamSynchronize(stream_));
+    }
+    double hs(int slot) { return hscal_[slot]; }
+    void spmv(const DevCsr &M, const double *x, double *y, const BBState *gate = nullptr)
+    {
+        dispatch_group(M.G, [&](auto g) {
+            constexpr int G = decltype(g)::value;
+            gated_spmv_kernel<G><<<blocks_for((long)M.rows * G), kBlock, 0, stream_>>>(M.view(), x, y, gate);
+        });
+        PDAL_LAUNCHED();
+    }
+    void obj_product(const double *xv);
+    void copy(double *dst, const double *src, int len)
+    {
+        if (len > 0)
+            PDAL_HIP(hipMemcpyAsync(dst, src, (size_t)len * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    }
+
+    // ---- iteration ----
+    StepScalars scalars() const;
+    bool major_kernel() const { return major_ || ((total_ + 2) % print_frequency(total_ + 2)) == 0; }
+    void update(bool fuse_halpern);
+    void primal_bb(const StepScalars &sc, bool fuse_halpern);
+    void bb_inner_step(const StepScalars &sc);
+    void halpern();
+    void fixed_point_error();
+    void compute_residual();
+    void infeasibility_info();
+    void check_termination();
+    bool should_restart();
+    void set_sigma_active(double sigma, double zero_thr);
+    double clipped_update(double cur, double ratio) const;
+    void update_al_sigma(double pd, double dd, double normal);
+    void perform_restart();
+    fa_cp_result_t *make_result();
+    void print_initial_info() const;
+    void display_stats() const;
+
+    fa_cp_parameters_t P_;
+    const qp_problem_t *W_;
+    hipStream_t stream_ = nullptr;
+
+    int n_ = 0, m_ = 0, r_ = 0;
+    double obj_const_ = 0.0;
+    HostCsr hQ_;  // structure of Q for the type test
+    DevCsr A_, At_, Q_, Qt_, R_, Rt_;
+    quad_obj_type_t qtype_ = FA_CP_NON_Q;
+    bool is_lp_ = true;
+    DBuf<double> c_, lb_, ub_, l_, u_, E_, D_, qdiag_;
+    double b_scale_ = 1.0, c_scale_ = 1.0, c_norm_ = 0.0, b_norm_ = 0.0, q_norm_ = 0.0, nonconvexity_ = 0.0;
+
+    DBuf<double> x0_, x_, xT_, xR_, dual_prod_, dslack_, dx_, qx_, grad_, dir_;
+    DBuf<double> y0_, y_, yT_, yR_, prim_prod_, al_u_, pslack_, dy_, rx_;
+    DBuf<double> parts_, bbparts_, dscal_, scratch_e_, scratch_d_;
+    PinnedBuf<double> hscal_{S_COUNT};
+    DBuf<BBState> dbb_;
+    PinnedBuf<BBState> hbb_{1};
+
+    // inner solver
+    bool has_inner_ = false;
+    int inner_limit_ = 0;
+    double inner_tol_ = 0.0;
+    int bb_chunk_ = 4;
+
+    // AL sigma state (fa_cp_core_op.cu:642-906)
+    bool al_available_ = false, use_al_ = false;
+    double al_sigma_ = 0.0, best_al_sigma_ = 0.0, best_sigma_score_ = INFINITY, last_sigma_score_ = INFINITY;
+    double last_sigma_rp_ = INFINITY, last_sigma_rd_ = INFINITY, last_sigma_gap_ = INFINITY;
+    int sigma_update_count_ = 0, sigma_dir_ = 1;
+
+    double step_ = 0.0, pw_ = 1.0, best_pw_ = 1.0, pw_err_sum_ = 0.0, pw_last_err_ = 0.0, best_gap_ = INFINITY;
+    int total_ = 0, inner_ = 0;
+    bool major_ = false;
+    double fp_ = INFINITY, fp0_ = INFINITY, last_trial_fp_ = INFINITY;
+    double abs_p_ = 0, rel_p_ = 0, abs_d_ = 0, rel_d_ = 0, p_obj_ = 0, d_obj_ = 0, gap_ = 0, rel_gap_ = 0;
+    double max_p_ray_ = 0, max_d_ray_ = 0, p_ray_obj_ = 0, d_ray_obj_ = 0;
+    termination_reason_t reason_ = TERMINATION_REASON_UNSPECIFIED;
+    double rescale_time_ = 0.0, solve_time_ = 0.0;
+};
+
+// ---------------------------------------------------------------------------
+// Set-up
+
+void Solver::upload()
+{
+    const qp_problem_t *w = W_;
+    n_ = w->num_variables;
+    m_ = w->num_constraints;
+    r_ = w->num_rank_lowrank_obj;
+    obj_const_ = w->objective_constant;
+    PDAL_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+
+    HostCsr hA = host_component(w->constraint_matrix, m_, n_, w->constraint_matrix_num_nonzeros);
+    hQ_ = host_component(w->objective_sparse_matrix, n_, n_, w->objective_sparse_matrix_num_nonzeros);
+    HostCsr hR = host_component(w->objective_lowrank_matrix, r_, n_, w->objective_lowrank_matrix_num_nonzeros);
+    HostCsr hAt = transpose_csr(m_, n_, hA.rp.data(), hA.ci.data(), hA.v.data());
+    HostCsr hQt = transpose_csr(n_, n_, hQ_.rp.data(), hQ_.ci.data(), hQ_.v.data());
+    HostCsr hRt = transpose_csr(r_, n_, hR.rp.data(), hR.ci.data(), hR.v.data());
+    auto put = [&](DevCsr &Dm, const HostCsr &H, const char *env) {
+        Dm.rows = H.rows;
+        Dm.nnz = H.nnz();
+        Dm.rp.alloc((size_t)H.rows + 1);
+        Dm.ci.alloc(Dm.nnz);
+        Dm.v.alloc(Dm.nnz);
+        Dm.rp.upload(H.rp.data(), (size_t)H.rows + 1, stream_);
+        Dm.ci.upload(H.ci.data(), Dm.nnz, stream_);
+        Dm.v.upload(H.v.data(), Dm.nnz, stream_);
+        Dm.G = pick_group(env, Dm.nnz, H.rows);
+    };
+    put(A_, hA, "PDAL_QP_G_A");
+    put(At_, hAt, "PDAL_QP_G_AT");
+    put(Q_, hQ_, "PDAL_QP_G_Q");
+    put(Qt_, hQt, nullptr);
+    put(R_, hR, "PDAL_QP_G_R");
+    put(Rt_, hRt, "PDAL_QP_G_RT");
+    PDAL_HIP(hipStreamSynchronize(stream_));  // host matrices go out of scope
+
+    auto vec = [&](DBuf<double> &b, const double *src, int len) {
+        b.alloc(len);
+        b.upload(src, len, stream_);
+    };
+    vec(c_, w->objective_vector, n_);
+    vec(lb_, w->variable_lower_bound, n_);
+    vec(ub_, w->variable_upper_bound, n_);
+    vec(l_, w->constraint_lower_bound, m_);
+    vec(u_, w->constraint_upper_bound, m_);
+    E_.alloc(m_);
+    D_.alloc(n_);
+    scratch_e_.alloc(m_);
+    scratch_d_.alloc(n_);
+    parts_.alloc((size_t)kMaxPartBlocks * kEvalStride);
+    bbparts_.alloc(kMaxPartBlocks);
+    dscal_.alloc(S_COUNT);
+    dbb_.alloc(1);
+    PDAL_HIP(hipMemsetAsync(dscal_.get(), 0, S_COUNT * sizeof(double), stream_));
+    PDAL_HIP(hipMemsetAsync(dbb_.get(), 0, sizeof(BBState), stream_));
+    for (DBuf<double> *v : {&x0_, &x_, &xT_, &xR_, &dual_prod_, &dslack_, &dx_, &qx_, &grad_, &dir_}) {
+        v->alloc(n_);
+        PDAL_HIP(hipMemsetAsync(v->get(), 0, (size_t)std::max(n_, 1) * sizeof(double), stream_));
+    }
+    for (DBuf<double> *v : {&y0_, &y_, &yT_, &yR_, &prim_prod_, &al_u_, &pslack_, &dy_}) {
+        v->alloc(m_);
+        PDAL_HIP(hipMemsetAsync(v->get(), 0, (size_t)std::max(m_, 1) * sizeof(double), stream_));
+    }
+    rx_.alloc(r_);
+    PDAL_HIP(hipMemsetAsync(rx_.get(), 0, (size_t)std::max(r_, 1) * sizeof(double), stream_));
+}
+
+void Solver::scale_all(const double *e, const double *d)
+{   // scale_problem, preconditioner.c:92-138, plus the transposed copies
+    scale_vectors_kernel<<<blocks_for((long)n_ + m_), kBlock, 0, stream_>>>(c_.get(), lb_.get(), ub_.get(), d,
+                                                                            D_.get(), n_, l_.get(), u_.get(), e,
+                                                                            E_.get(), m_);
+    divide_rows_kernel<0><<<blocks_for(m_), kBlock, 0, stream_>>>(A_.view(), A_.v.get(), e, d);
+    divide_rows_kernel<0><<<blocks_for(n_), kBlock, 0, stream_>>>(At_.view(), At_.v.get(), d, e);
+    if (Q_.nnz > 0) {
+        divide_rows_kernel<0><<<blocks_for(n_), kBlock, 0, stream_>>>(Q_.view(), Q_.v.get(), d, d);
+        divide_rows_kernel<0><<<blocks_for(n_), kBlock, 0, stream_>>>(Qt_.view(), Qt_.v.get(), d, d);
+    }
+    if (R_.nnz > 0) {
+        divide_rows_kernel<1><<<blocks_for(r_), kBlock, 0, stream_>>>(R_.view(), R_.v.get(), nullptr, d);
+        divide_rows_kernel<2><<<blocks_for(n_), kBlock, 0, stream_>>>(Rt_.view(), Rt_.v.get(), d, nullptr);
+    }
+    PDAL_LAUNCHED();
+}
+
+void Solver::rescale()
+{   // rescale_problem, preconditioner.c:291-331
+    fill_kernel<<<blocks_for(m_), kBlock, 0, stream_>>>(E_.get(), 1.0, m_);
+    fill_kernel<<<blocks_for(n_), kBlock, 0, stream_>>>(D_.get(), 1.0, n_);
+    double *e = scratch_e_.get(), *d = scratch_d_.get();
+    const int rows = std::max(m_, n_);
+    for (int it = 0; it < P_.l_inf_ruiz_iterations; ++it) {  // ruiz_rescaling, :140-203
+        ruiz_factors_kernel<<<blocks_for(rows), kBlock, 0, stream_>>>(A_.view(), At_.view(), Qt_.view(), e, d);
+        PDAL_LAUNCHED();
+        scale_all(e, d);
+    }
+    if (P_.has_pock_chambolle_alpha) {  // pock_chambolle_rescaling, :205-247
+        pc_factors_kernel<<<blocks_for(rows), kBlock, 0, stream_>>>(A_.view(), At_.view(), Qt_.view(),
+                                                                    P_.pock_chambolle_alpha, e, d);
+        PDAL_LAUNCHED();
+        scale_all(e, d);
+    }
+    if (P_.bound_objective_rescaling) {  // bound_obj_rescaling, :249-289
+        const int nb = stride_grid(std::max(m_, n_));
+        bound_obj_sums_kernel<<<nb, kBlock, 0, stream_>>>(l_.get(), u_.get(), m_, c_.get(), n_, parts_.get());
+        PDAL_LAUNCHED();
+        reduce(nb, 2, 2, S_BOUND);
+        fetch(S_BOUND, 2);
+        b_scale_ = 1.0 / (std::sqrt(hs(S_BOUND)) + 1.0);
+        c_scale_ = 1.0 / (std::sqrt(hs(S_BOUND + 1)) + 1.0);
+        bound_obj_apply_kernel<<<blocks_for((long)m_ + n_), kBlock, 0, stream_>>>(
+            l_.get(), u_.get(), m_, lb_.get(), ub_.get(), c_.get(), n_, b_scale_, c_scale_);
+        const double fq = c_scale_ / b_scale_;
+        if (Q_.nnz > 0) {
+            mul_all_kernel<<<blocks_for(Q_.nnz), kBlock, 0, stream_>>>(Q_.v.get(), Q_.nnz, fq);
+            mul_all_kernel<<<blocks_for(Qt_.nnz), kBlock, 0, stream_>>>(Qt_.v.get(), Qt_.nnz, fq);
+        }
+        if (R_.nnz > 0) {
+            const double fr = std::sqrt(c_scale_ / b_scale_);
+            mul_all_kernel<<<blocks_for(R_.nnz), kBlock, 0, stream_>>>(R_.v.get(), R_.nnz, fr);
+            mul_all_kernel<<<blocks_for(Rt_.nnz), kBlock, 0, stream_>>>(Rt_.v.get(), Rt_.nnz, fr);
+        }
+        PDAL_LAUNCHED();
+    }
+    PDAL_HIP(hipStreamSynchronize(stream_));
+}
+
+double Solver::max_singular_value()
+{   // estimate_maximum_singular_value, utils.cu:65-170
+    std::vector<double> v0(m_);
+    for (int i = 0; i < m_; ++i) v0[i] = g_dist(g_gen);
+    double *v = al_u_.get(), *nv = dy_.get(), *w = dx_.get();  // free before the iteration starts
+    PDAL_HIP(hipMemcpyAsync(v, v0.data(), (size_t)m_ * sizeof(double), hipMemcpyHostToDevice, stream_));
+    const int nbm = stride_grid(m_);
+    double sig = 1.0;
+    for (int it = 0; it < P_.sv_max_iter; ++it) {
+        dot_sumsq_kernel<<<nbm, kBlock, 0, stream_>>>(v, v, m_, parts_.get());
+        reduce(nbm, 2, 2, S_PW0);
+        normalize_dev_kernel<<<blocks_for(m_), kBlock, 0, stream_>>>(v, nv, m_, dscal_.get() + S_PW0);
+        spmv(At_, nv, w);
+        spmv(A_, w, v);
+        dot_sumsq_kernel<<<nbm, kBlock, 0, stream_>>>(nv, v, m_, parts_.get());
+        reduce(nbm, 2, 2, S_PW0 + 1);
+        sv_resid_kernel<<<nbm, kBlock, 0, stream_>>>(nv, v, m_, dscal_.get() + S_PW0 + 1, parts_.get());
+        reduce(nbm, 1, 1, S_PW0 + 3);
+        PDAL_LAUNCHED();
+        fetch(S_PW0, 4);
+        sig = hs(S_PW0 + 1);
+        if (std::sqrt(hs(S_PW0 + 3)) < P_.sv_tol) break;
+    }
+    PDAL_HIP(hipMemsetAsync(al_u_.get(), 0, (size_t)std::max(m_, 1) * sizeof(double), stream_));
+    PDAL_HIP(hipMemsetAsync(dy_.get(), 0, (size_t)std::max(m_, 1) * sizeof(double), stream_));
+    PDAL_HIP(hipMemsetAsync(dx_.get(), 0, (size_t)std::max(n_, 1) * sizeof(double), stream_));
+    return std::sqrt(sig);
+}
+
+double Solver::q_max_eig()
+{   // estimate_maximum_eigenvalue, utils.cu:172-238 (glibc rand() start)
+    std::vector<double> v0(n_);
+    for (int i = 0; i < n_; ++i) v0[i] = (double)rand() / RAND_MAX;
+    double *v = grad_.get(), *av = dir_.get();
+    PDAL_HIP(hipMemcpyAsync(v, v0.data(), (size_t)n_ * sizeof(double), hipMemcpyHostToDevice, stream_));
+    const int nb = stride_grid(n_);
+    double lam = 0.0;
+    for (int it = 0; it < P_.sv_max_iter; ++it) {
+        dot_sumsq_kernel<<<nb, kBlock, 0, stream_>>>(v, v, n_, parts_.get());
+        reduce(nb, 2, 2, S_PW0);
+        normalize_dev_kernel<<<blocks_for(n_), kBlock, 0, stream_>>>(v, v, n_, dscal_.get() + S_PW0);
+        spmv(Q_, v, av);
+        dot_sumsq_kernel<<<nb, kBlock, 0, stream_>>>(v, av, n_, parts_.get());
+        reduce(nb, 2, 2, S_PW0 + 1);
+        PDAL_LAUNCHED();
+        fetch(S_PW0 + 1, 1);
+        const double old = lam;
+        lam = hs(S_PW0 + 1);
+        if (it > 0 && std::fabs(lam - old) < P_.sv_tol) break;
+        std::swap(v, av);
+    }
+    return lam;
+}
+
+double Solver::q_min_eig(double lmax)
+{   // estimate_minimum_eigenvalue, utils.cu:240-315
+    std::vector<double> v0(n_);
+    for (int i = 0; i < n_; ++i) v0[i] = (double)rand() / RAND_MAX;
+    double *v = grad_.get(), *av = dir_.get(), *sh = dx_.get();
+    PDAL_HIP(hipMemcpyAsync(v, v0.data(), (size_t)n_ * sizeof(double), hipMemcpyHostToDevice, stream_));
+    const int nb = stride_grid(n_);
+    double mu = 0.0;
+    for (int it = 0; it < P_.sv_max_iter; ++it) {
+        dot_sumsq_kernel<<<nb, kBlock, 0, stream_>>>(v, v, n_, parts_.get());
+        reduce(nb, 2, 2, S_PW0);
+        normalize_dev_kernel<<<blocks_for(n_), kBlock, 0, stream_>>>(v, v, n_, dscal_.get() + S_PW0);
+        spmv(Q_, v, av);
+        shift_dot_kernel<<<nb, kBlock, 0, stream_>>>(v, av, lmax, sh, n_, parts_.get());
+        reduce(nb, 1, 1, S_PW0 + 1);
+        PDAL_LAUNCHED();
+        fetch(S_PW0 + 1, 1);
+        const double old = mu;
+        mu = hs(S_PW0 + 1);
+        if (it > 0 && std::fabs(mu - old) < P_.sv_tol) break;
+        std::swap(v, sh);
+    }
+    for (double *b : {grad_.get(), dir_.get(), dx_.get()})
+        PDAL_HIP(hipMemsetAsync(b, 0, (size_t)std::max(n_, 1) * sizeof(double), stream_));
+    return lmax - mu;
+}
+
+void Solver::init_state()
+{   // initialize_solver_state, solver_state.cu:494-877
+    const qp_problem_t *w = W_;
+    al_available_ = P_.use_al_qp;
+    use_al_ = P_.use_al_qp;
+    al_sigma_ = P_.al_sigma;
+    best_al_sigma_ = P_.al_sigma;
+    if (w->primal_start || w->dual_start) {  // solver_state.cu:630-657
+        if (w->primal_start) x_.upload(w->primal_start, n_, stream_);
+        if (w->dual_start) y_.upload(w->dual_start, m_, stream_);
+        start_scale_kernel<<<blocks_for((long)n_ + m_), kBlock, 0, stream_>>>(
+            x_.get(), D_.get(), b_scale_, w->primal_start ? n_ : 0, y_.get(), E_.get(), c_scale_,
+            w->dual_start ? m_ : 0);
+        PDAL_LAUNCHED();
+        copy(x0_.get(), x_.get(), n_);
+        copy(xT_.get(), x_.get(), n_);
+        copy(y0_.get(), y_.get(), m_);
+        copy(yT_.get(), y_.get(), m_);
+    }
+    const bool linf = P_.optimality_norm == NORM_TYPE_L_INF;
+    double sq = 0.0, mx = 0.0;
+    for (int i = 0; i < n_; ++i) {
+        const double cv = w->objective_vector[i];
+        if (linf) mx = std::max(mx, std::fabs(cv));
+        else sq += cv * cv;
+    }
+    c_norm_ = linf ? mx : std::sqrt(sq);
+    sq = mx = 0.0;
+    for (int i = 0; i < m_; ++i) {
+        const double lo = w->constraint_lower_bound[i], hi = w->constraint_upper_bound[i];
+        if (linf) {
+            if (std::isfinite(lo) && lo != hi) mx = std::max(mx, std::fabs(lo));
+            if (std::isfinite(hi)) mx = std::max(mx, std::fabs(hi));
+        } else {
+            if (std::isfinite(lo) && lo != hi) sq += lo * lo;
+            if (std::isfinite(hi)) sq += hi * hi;
+        }
+    }
+    b_norm_ = linf ? mx : std::sqrt(sq);
+
+    // detect_q_type, utils.cu:1576-1616
+    if (Q_.nnz == 0 && R_.nnz == 0) qtype_ = FA_CP_NON_Q;
+    else if (R_.nnz > 0) qtype_ = Q_.nnz > 0 ? FA_CP_LOW_RANK_PLUS_SPARSE_Q : FA_CP_LOW_RANK_Q;
+    else {
+        qtype_ = FA_CP_DIAG_Q;
+        for (int i = 0; i < n_ && qtype_ == FA_CP_DIAG_Q; ++i)
+            for (int k = hQ_.rp[i]; k < hQ_.rp[i + 1]; ++k)
+                if (hQ_.ci[k] != i) {
+                    qtype_ = FA_CP_SPARSE_Q;
+                    break;
+                }
+    }
+    hQ_ = HostCsr{};
+    if (qtype_ == FA_CP_DIAG_Q) {  // solver_state.cu:462-492
+        qdiag_.alloc(n_);
+        PDAL_HIP(hipMemsetAsync(qdiag_.get(), 0, (size_t)std::max(n_, 1) * sizeof(double), stream_));
+        qdiag_kernel<<<blocks_for(n_), kBlock, 0, stream_>>>(Q_.view(), n_, qdiag_.get());
+        PDAL_LAUNCHED();
+        std::vector<double> qd(n_);
+        qdiag_.download(qd.data(), n_, stream_);
+        PDAL_HIP(hipStreamSynchronize(stream_));
+        double mxe = 0.0, mne = 0.0;
+        for (double dv : qd) {
+            mxe = std::fmax(mxe, std::fabs(dv));
+            mne = std::fmin(mne, dv);
+        }
+        q_norm_ = mxe;
+        nonconvexity_ = mne;
+    } else if (qtype_ == FA_CP_SPARSE_Q) {
+        q_norm_ = q_max_eig();
+        nonconvexity_ = q_min_eig(q_norm_);
+    }
+    if (al_available_ && P_.al_sigma <= P_.sigma_zero_threshold) {
+        use_al_ = false;
+        al_sigma_ = 0.0;
+    }
+    // initialize_inner_solver, solver_state.cu:305-360
+    has_inner_ = use_al_ || al_available_;
+    const bool simple = qtype_ == FA_CP_NON_Q || qtype_ == FA_CP_DIAG_Q;
+    if (!simple) has_inner_ = true;
+    if (!simple || al_available_) {
+        inner_limit_ = 1000;
+        inner_tol_ = 1e-3;
+    }
+    is_lp_ = qtype_ == FA_CP_NON_Q;
+}
+
+// ---------------------------------------------------------------------------
+// Products and iteration
+
+void Solver::obj_product(const double *xv)
+{   // update_obj_product, solver_state.cu:371-442
+    switch (qtype_) {
+    case FA_CP_SPARSE_Q:
+        dispatch_group(Q_.G, [&](auto g) {
+            constexpr int G = decltype(g)::value;
+            qx_kernel<G, 0><<<blocks_for((long)n_ * G), kBlock, 0, stream_>>>(Q_.view(), Rt_.view(), xv, nullptr,
+                                                                             qx_.get(), n_, nullptr);
+        });
+        break;
+    case FA_CP_DIAG_Q:
+        diag_mul_kernel<<<blocks_for(n_), kBlock, 0, stream_>>>(qdiag_.get(), xv, qx_.get(), n_);
+        break;
+    case FA_CP_LOW_RANK_Q:
+        spmv(R_, xv, rx_.get());
+        dispatch_group(Rt_.G, [&](auto g) {
+            constexpr int G = decltype(g)::value;
+            qx_kernel<G, 1><<<blocks_for((long)n_ * G), kBlock, 0, stream_>>>(Q_.view(), Rt_.view(), xv, rx_.get(),
+                                                                             qx_.get(), n_, nullptr);
+        });
+        break;
+    case FA_CP_LOW_RANK_PLUS_SPARSE_Q:
+        dispatch_group(Q_.G, [&](auto g) {
+            constexpr int G = decltype(g)::value;
+            qx_kernel<G, 0><<<blocks_for((long)n_ * G), kBlock, 0, stream_>>>(Q_.view(), Rt_.view(), xv, nullptr,
+                                                                             qx_.get(), n_, nullptr);
+        });
+        spmv(R_, xv, rx_.get());
+        dispatch_group(Rt_.G, [&](auto g) {
+            constexpr int G = decltype(g)::value;
+            qx_kernel<G, 2><<<blocks_for((long)n_ * G), kBlock, 0, stream_>>>(Q_.view(), Rt_.view(), xv, rx_.get(),
+                                                                             qx_.get(), n_, nullptr);
+        });
+        break;
+    default: return;
+    }
+    PDAL_LAUNCHED();
+}
+
+StepScalars Solver::scalars() const
+{   // fa_cp_update, fa_cp_core_op.cu:531-545
+    StepScalars sc{};
+    double tau = step_ / pw_;
+    if (nonconvexity_ < 0) {
+        tau = std::fmax(tau, -1.01 * std::fmin(0.0, nonconvexity_));
+        tau /= 2;
+    }
+    sc.tau = tau;
+    sc.sigma = step_ * pw_;
+    sc.al_sigma = use_al_ ? al_sigma_ : 0.0;
+    sc.weight = (double)(inner_ + 1) / (inner_ + 2);
+    sc.rho = P_.reflection_coefficient;
+    sc.tau_res = step_ / pw_;
+    return sc;
+}
+
+void Solver::bb_inner_step(const StepScalars &sc)
+{   // one pass of the loop body in primal_BB_step_size_update, :452-500
+    BBState *bb = dbb_.get();
+    if (use_al_) {
+        dispatch_group(A_.G, [&](auto g) {
+            constexpr int G = decltype(g)::value;
+            al_effective_dual_kernel<G><<<blocks_for((long)m_ * G), kBlock, 0, stream_>>>(
+                A_.view(), xT_.get(), y_.get(), l_.get(), u_.get(), prim_prod_.get(), al_u_.get(), sc, bb);
+        });
+        spmv(At_, al_u_.get(), dual_prod_.get(), bb);
+    }
+    BBArgs a{c_.get(), lb_.get(), ub_.get(), x_.get(), dual_prod_.get(), qx_.get(), xT_.get(), grad_.get(),
+             dir_.get(), bbparts_.get(), bb, sc, n_};
+    int nb = 0;
+    auto grad = [&](auto gtag, auto mtag, const DevCsr &M) {
+        constexpr int G = decltype(gtag)::value;
+        constexpr int MODE = decltype(mtag)::value;
+        nb = std::max(1, std::min(kMaxPartBlocks, blocks_for((long)n_ * G)));
+        bb_grad_kernel<G, MODE><<<nb, kBlock, 0, stream_>>>(a, Q_.view(), M.view(), rx_.get(), qdiag_.get());
+    };
+    switch (qtype_) {
+    case FA_CP_SPARSE_Q:
+        dispatch_group(Q_.G, [&](auto g) { grad(g, std::integral_constant<int, 0>{}, Rt_); });
+        break;
+    case FA_CP_LOW_RANK_Q:
+        spmv(R_, xT_.get(), rx_.get(), bb);
+        dispatch_group(Rt_.G, [&](auto g) { grad(g, std::integral_constant<int, 1>{}, Rt_); });
+        break;
+    case FA_CP_LOW_RANK_PLUS_SPARSE_Q:
+        dispatch_group(Q_.G, [&](auto g) {
+            constexpr int G = decltype(g)::value;
+            qx_kernel<G, 0><<<blocks_for((long)n_ * G), kBlock, 0, stream_>>>(Q_.view(), Rt_.view(), xT_.get(),
+                                                                             nullptr, qx_.get(), n_, bb);
+        });
+        spmv(R_, xT_.get(), rx_.get(), bb);
+        dispatch_group(Rt_.G, [&](auto g) { grad(g, std::integral_constant<int, 2>{}, Rt_); });
+        break;
+    case FA_CP_DIAG_Q: grad(std::integral_constant<int, 1>{}, std::integral_constant<int, 4>{}, Rt_); break;
+    default: grad(std::integral_constant<int, 1>{}, std::integral_constant<int, 3>{}, Rt_); break;
+    }
+    PDAL_LAUNCHED();
+    bb_alpha_kernel<<<1, kBlock, 0, stream_>>>(bbparts_.get(), nb, bb);
+    const int nbd = stride_grid(n_);
+    bb_dir_kernel<<<nbd, kBlock, 0, stream_>>>(a);
+    bb_check_kernel<false><<<1, kBlock, 0, stream_>>>(bbparts_.get(), nbd, bb, inner_tol_, inner_limit_);
+    PDAL_LAUNCHED();
+}
+
+void Solver::primal_bb(const StepScalars &sc, bool fuse_halpern)
+{   // primal_BB_step_size_update, fa_cp_core_op.cu:445-503
+    if (use_al_) {  // compute_al_effective_dual_product at x (:275-303)
+        dispatch_group(A_.G, [&](auto g) {
+            constexpr int G = decltype(g)::value;
+            al_effective_dual_kernel<G><<<blocks_for((long)m_ * G), kBlock, 0, stream_>>>(
+                A_.view(), x_.get(), y_.get(), l_.get(), u_.get(), prim_prod_.get(), al_u_.get(), sc, nullptr);
+        });
+        spmv(At_, al_u_.get(), dual_prod_.get());
+    }
+    obj_product(x_.get());
+    BBArgs a{c_.get(), lb_.get(), ub_.get(), x_.get(), dual_prod_.get(), qx_.get(), xT_.get(), grad_.get(),
+             dir_.get(), bbparts_.get(), dbb_.get(), sc, n_};
+    const int nbd = stride_grid(n_);
+    bb_init_kernel<<<nbd, kBlock, 0, stream_>>>(a);
+    bb_check_kernel<true><<<1, kBlock, 0, stream_>>>(bbparts_.get(), nbd, dbb_.get(), inner_tol_, inner_limit_);
+    PDAL_LAUNCHED();
+    // Inner steps in chunks, each ending with one read of the done flag.  The
+    // first chunk is sized by the previous outer iteration's inner count; steps
+    // launched after the loop has stopped return at once (gated kernels).
+    int chunk = bb_chunk_;
+    while (true) {
+        for (int k = 0; k < chunk; ++k) bb_inner_step(sc);
+        PDAL_HIP(hipMemcpyAsync(hbb_.get(), dbb_.get(), sizeof(BBState), hipMemcpyDeviceToHost, stream_));
+        PDAL_HIP(hipStreamSynchronize(stream_));
+        if (hbb_[0].done) break;
+        chunk = 8;
+    }
+    bb_chunk_ = std::max(1, std::min(64, hbb_[0].it - 1));
+    if (fuse_halpern)
+        bb_final_kernel<true><<<blocks_for(n_), kBlock, 0, stream_>>>(x_.get(), xT_.get(), xR_.get(), x0_.get(), n_,
+                                                                     dbb_.get(), sc);
+    else
+        bb_final_kernel<false><<<blocks_for(n_), kBlock, 0, stream_>>>(x_.get(), xT_.get(), xR_.get(), x0_.get(),
+                                                                      n_, dbb_.get(), sc);
+    PDAL_LAUNCHED();
+}
+
+void Solver::update(bool fuse)
+{   // fa_cp_update, fa_cp_core_op.cu:531-630 (+ the Halpern step when fused)
+    const StepScalars sc = scalars();
+    const bool mk = major_kernel();
+    const bool simple = !use_al_ && (qtype_ == FA_CP_NON_Q || qtype_ == FA_CP_DIAG_Q);
+    if (simple) {
+        SimplePrimalArgs a{At_.view(), y_.get(), c_.get(), lb_.get(), ub_.get(), qdiag_.get(), x0_.get(),
+                           x_.get(), xR_.get(), xT_.get(), dslack_.get(), dual_prod_.get(), sc};
+        const bool diag = qtype_ == FA_CP_DIAG_Q;
+        dispatch_group(At_.G, [&](auto g) {
+            constexpr int G = decltype(g)::value;
+            const int nb = blocks_for((long)n_ * G);
+            const int key = (mk ? 4 : 0) | (diag ? 2 : 0) | (fuse ? 1 : 0);
+            switch (key) {
+            case 0: simple_primal_kernel<G, false, false, false><<<nb, kBlock, 0, stream_>>>(a); break;
+            case 1: simple_primal_kernel<G, false, false, true><<<nb, kBlock, 0, stream_>>>(a); break;
+            case 2: simple_primal_kernel<G, false, true, false><<<nb, kBlock, 0, stream_>>>(a); break;
+            case 3: simple_primal_kernel<G, false, true, true><<<nb, kBlock, 0, stream_>>>(a); break;
+            case 4: simple_primal_kernel<G, true, false, false><<<nb, kBlock, 0, stream_>>>(a); break;
+            case 5: simple_primal_kernel<G, true, false, true><<<nb, kBlock, 0, stream_>>>(a); break;
+            case 6: simple_primal_kernel<G, true, true, false><<<nb, kBlock, 0, stream_>>>(a); break;
+            default: simple_primal_kernel<G, true, true, true><<<nb, kBlock, 0, stream_>>>(a); break;
+            }
+        });
+        PDAL_LAUNCHED();
+    } else {
+        if (!use_al_) spmv(At_, y_.get(), dual_prod_.get());
+        primal_bb(sc, fuse);
+    }
+    DualArgs d{A_.view(), xR_.get(), l_.get(), u_.get(), y0_.get(), y_.get(), yR_.get(), yT_.get(), sc};
+    dispatch_group(A_.G, [&](auto g) {
+        constexpr int G = decltype(g)::value;
+        const int nb = blocks_for((long)m_ * G);
+        const int key = (mk ? 4 : 0) | (use_al_ ? 2 : 0) | (fuse ? 1 : 0);
+        switch (key) {
+        case 0: dual_kernel<G, false, false, false><<<nb, kBlock, 0, stream_>>>(d); break;
+        case 1: dual_kernel<G, false, false, true><<<nb, kBlock, 0, stream_>>>(d); break;
+        case 2: dual_kernel<G, false, true, false><<<nb, kBlock, 0, stream_>>>(d); break;
+        case 3: dual_kernel<G, false, true, true><<<nb, kBlock, 0, stream_>>>(d); break;
+        case 4: dual_kernel<G, true, false, false><<<nb, kBlock, 0, stream_>>>(d); break;
+        case 5: dual_kernel<G, true, false, true><<<nb, kBlock, 0, stream_>>>(d); break;
+        case 6: dual_kernel<G, true, true, false><<<nb, kBlock, 0, stream_>>>(d); break;
+        default: dual_kernel<G, true, true, true><<<nb, kBlock, 0, stream_>>>(d); break;
+        }
+    });
+    PDAL_LAUNCHED();
+}
+
+void Solver::halpern()
+{   // halpern_update, fa_cp_core_op.cu:632-640
+    halpern_kernel<<<blocks_for((long)n_ + m_), kBlock, 0, stream_>>>(x_.get(), xR_.get(), x0_.get(), n_, y_.get(),
+                                                                      yR_.get(), y0_.get(), m_, scalars());
+    PDAL_LAUNCHED();
+}
+
+void Solver::fixed_point_error()
+{   // compute_fixed_point_error, fa_cp_core_op.cu:1042-1089
+    const int nbm = stride_grid(m_);
+    diff_norm_kernel<<<nbm, kBlock, 0, stream_>>>(yR_.get(), y_.get(), dy_.get(), m_, parts_.get());
+    reduce(nbm, 1, 1, S_DN2);
+    int nb = 1;
+    dispatch_group(At_.G, [&](auto g) {
+        constexpr int G = decltype(g)::value;
+        nb = row_grid(n_, G);
+        fp_at_kernel<G><<<nb, kBlock, 0, stream_>>>(At_.view(), yR_.get(), y_.get(), xR_.get(), x_.get(), dx_.get(),
+                                                     dual_prod_.get(), parts_.get());
+    });
+    PDAL_LAUNCHED();
+    reduce(nb, 2, 2, S_FP);
+    fetch(S_FP, 3);
+    const double dn = std::sqrt(hs(S_DN2)), pn = std::sqrt(hs(S_FP + FP_PN2));
+    const double movement = pn * pn * pw_ + dn * dn / pw_;
+    const double cross = hs(S_FP + FP_CROSS);
+    fp_ = std::sqrt(std::fmax(movement + 2 * step_ * cross, 0.0));
+    if (has_inner_) inner_tol_ = std::fmin(inner_tol_, std::fmax(0.0005 * pn / step_ * pw_, 1e-9));
+}
+
+void Solver::compute_residual()
+{   // compute_residual, utils.cu:889-1026
+    int nba = 1, nbt = 1;
+    dispatch_group(A_.G, [&](auto g) {
+        constexpr int G = decltype(g)::value;
+        nba = row_grid(m_, G);
+        res_a_kernel<G><<<nba, kBlock, 0, stream_>>>(A_.view(), xT_.get(), yT_.get(), l_.get(), u_.get(), E_.get(),
+                                                     parts_.get());
+    });
+    PDAL_LAUNCHED();
+    reduce(nba, 2, RA_N, S_RA);
+    obj_product(xT_.get());
+    const StepScalars sc = scalars();
+    dispatch_group(At_.G, [&](auto g) {
+        constexpr int G = decltype(g)::value;
+        nbt = row_grid(n_, G);
+        if (is_lp_)
+            res_at_kernel<G, false><<<nbt, kBlock, 0, stream_>>>(At_.view(), yT_.get(), xT_.get(), qx_.get(),
+                                                                 c_.get(), lb_.get(), ub_.get(), D_.get(),
+                                                                 dslack_.get(), sc, parts_.get());
+        else
+            res_at_kernel<G, true><<<nbt, kBlock, 0, stream_>>>(At_.view(), yT_.get(), xT_.get(), qx_.get(), c_.get(),
+                                                                lb_.get(), ub_.get(), D_.get(), dslack_.get(), sc,
+                                                                parts_.get());
+    });
+    PDAL_LAUNCHED();
+    reduce(nbt, 6, RT_N, S_RT);
+    fetch(S_RA, RA_N + RT_N);
+    const bool linf = P_.optimality_norm == NORM_TYPE_L_INF;
+    const double *ra = hscal_.get() + S_RA, *rt = hscal_.get() + S_RT;
+    abs_p_ = (linf ? ra[RA_PRESMAX] : std::sqrt(ra[RA_PRES2])) / b_scale_;
+    abs_d_ = (linf ? rt[RT_DRESMAX] : std::sqrt(rt[RT_DRES2])) / c_scale_;
+    const double half = qtype_ == FA_CP_NON_Q ? 0.0 : 0.5 * rt[RT_XQX];
+    const double bc = b_scale_ * c_scale_;
+    p_obj_ = (rt[RT_CX] + half) / bc + obj_const_;
+    d_obj_ = (rt[RT_SX] + ra[RA_CONTRIB] - half) / bc + obj_const_;
+    rel_p_ = abs_p_ / (1.0 + b_norm_);
+    double den;
+    if (is_lp_) {
+        den = 1.0 + c_norm_;
+    } else {  // recover_primal_obj_dual_product, :772-781
+        const double qn = linf ? rt[RT_QXDMAX] : std::sqrt(rt[RT_QXD2]);
+        const double an = linf ? rt[RT_DPDMAX] : std::sqrt(rt[RT_DPD2]);
+        den = 1.0 + std::fmax(c_norm_, std::fmax(qn / c_scale_, an / c_scale_));
+    }
+    rel_d_ = abs_d_ / den;
+    gap_ = std::fabs(p_obj_ - d_obj_);
+    rel_gap_ = gap_ / (1.0 + std::fabs(p_obj_) + std::fabs(d_obj_));
+}
+
+void Solver::infeasibility_info()
+{   // compute_infeasibility_information, utils.cu:1028-1132
+    const int nb = stride_grid(std::max(m_, n_));
+    ray_project_kernel<<<nb, kBlock, 0, stream_>>>(dx_.get(), lb_.get(), ub_.get(), n_, dy_.get(), l_.get(), u_.get(),
+                                                   m_, parts_.get());
+    PDAL_LAUNCHED();
+    reduce(nb, 0, 2, S_PROJ);
+    fetch(S_PROJ, 2);
+    const double pinf = hs(S_PROJ), dinf = hs(S_PROJ + 1);
+    if (pinf > 0.0) {
+        scale_kernel<<<blocks_for(n_), kBlock, 0, stream_>>>(dx_.get(), 1.0 / pinf, n_);
+        PDAL_LAUNCHED();
+    }
+    spmv(A_, dx_.get(), prim_prod_.get());
+    spmv(At_, dy_.get(), dual_prod_.get());
+    ray_terms_kernel<<<nb, kBlock, 0, stream_>>>(c_.get(), dx_.get(), dual_prod_.get(), lb_.get(), ub_.get(),
+                                                 D_.get(), dslack_.get(), n_, dy_.get(), prim_prod_.get(), l_.get(),
+                                                 u_.get(), E_.get(), pslack_.get(), m_, parts_.get());
+    PDAL_LAUNCHED();
+    reduce(nb, 3, RY_N, S_RAY);
+    const bool quad = !is_lp_ && qtype_ != FA_CP_NON_Q;
+    if (quad) {
+        obj_product(dx_.get());
+        const int nbn = stride_grid(n_);
+        absmax_kernel<<<nbn, kBlock, 0, stream_>>>(qx_.get(), n_, parts_.get());
+        PDAL_LAUNCHED();
+        reduce(nbn, 0, 1, S_QXMAX);
+    }
+    fetch(S_RAY, RY_N + 1);
+    const double bc = b_scale_ * c_scale_;
+    const double *ry = hscal_.get() + S_RAY;
+    p_ray_obj_ = ry[RY_CDX] / bc;
+    d_ray_obj_ = (ry[RY_PSUM] + ry[RY_DSUM]) / bc;
+    max_p_ray_ = ry[RY_PMAX];
+    if (quad) max_p_ray_ = std::fmax(max_p_ray_, hs(S_QXMAX) / c_scale_);
+    const double dsn = ry[RY_DMAX];
+    max_d_ray_ = dsn;
+    const double f = std::fmax(dinf, dsn);
+    if (f > 0.0) {
+        max_d_ray_ /= f;
+        d_ray_obj_ /= f;
+    } else {
+        max_d_ray_ = 0.0;
+        d_ray_obj_ = 0.0;
+    }
+}
+
+void Solver::check_termination()
+{   // check_termination_criteria, utils.cu:438-462
+    const auto &tc = P_.termination_criteria;
+    if (rel_d_ < tc.eps_feasible_relative && rel_p_ < tc.eps_feasible_relative && rel_gap_ < tc.eps_optimal_relative)
+        reason_ = TERMINATION_REASON_OPTIMAL;
+    else if (d_ray_obj_ > 0.0 && max_d_ray_ / d_ray_obj_ <= tc.eps_infeasible)
+        reason_ = TERMINATION_REASON_PRIMAL_INFEASIBLE;
+    else if (p_ray_obj_ < 0.0 && max_p_ray_ / (-p_ray_obj_) <= tc.eps_infeasible)
+        reason_ = TERMINATION_REASON_DUAL_INFEASIBLE;
+    else if (total_ >= tc.iteration_limit)
+        reason_ = TERMINATION_REASON_ITERATION_LIMIT;
+    else if (solve_time_ >= tc.time_sec_limit)
+        reason_ = TERMINATION_REASON_TIME_LIMIT;
+}
+
+bool Solver::should_restart()
+{   // should_do_adaptive_restart, utils.cu:464-492
+    const auto &rp = P_.restart_params;
+    const int K = P_.termination_evaluation_frequency;
+    bool r = false;
+    if (total_ == K) {
+        r = true;
+    } else if (total_ > K) {
+        if (fp_ <= rp.sufficient_reduction_for_restart * fp0_) r = true;
+        if (fp_ <= rp.necessary_reduction_for_restart * fp0_ && fp_ > last_trial_fp_) r = true;
+        if (inner_ >= rp.artificial_restart_threshold * total_) r = true;
+    }
+    last_trial_fp_ = fp_;
+    return r;
+}
+
+void Solver::set_sigma_active(double sigma, double zero_thr)
+{   // fa_cp_core_op.cu:667-676
+    if (!std::isfinite(sigma) || sigma <= zero_thr) {
+        al_sigma_ = 0.0;
+        use_al_ = false;
+    } else {
+        al_sigma_ = sigma;
+        use_al_ = al_available_;
+    }
+}
+
+double Solver::clipped_update(double cur, double ratio) const
+{   // fa_cp_core_op.cu:700-713
+    const double gain = sane_positive(P_.sigma_update_gain, 0.5);
+    double lo = sane_positive(P_.sigma_clip_low, 0.6), hi = sane_positive(P_.sigma_clip_high, 1.7);
+    if (lo > hi) std::swap(lo, hi);
+    ratio = sane_positive(ratio, 1.0);
+    return cur * clampd_h(std::pow(ratio, gain), lo, hi);
+}
+
+void Solver::update_al_sigma(double pd, double dd, double normal_motion)
+{   // update_al_sigma_at_restart, fa_cp_core_op.cu:722-906
+    const auto &P = P_;
+    const bool bal = P.sigma_update_rule == FA_CP_SIGMA_UPDATE_BALANCED_GUARDED;
+    const bool guarded = P.sigma_update_rule == FA_CP_SIGMA_UPDATE_GUARDED || bal;
+    if (!al_available_ || P.sigma_update_rule == FA_CP_SIGMA_UPDATE_FIXED) return;
+    if (!guarded && !use_al_) return;
+    const double ref = bal ? sane_positive(step_ * pw_, 1.0) : 1.0;
+    double smin = sane_positive(P.sigma_min, 1e-8) * ref;
+    double smax = sane_positive(P.sigma_max, 10.0) * ref;
+    const double zthr = sane_positive(P.sigma_zero_threshold, sane_positive(P.sigma_min, 1e-8)) * ref;
+    if (smin > smax) std::swap(smin, smax);
+    if (zthr > smin) smin = zthr;
+    const double cur = use_al_ ? clampd_h(sane_positive(al_sigma_, smin), smin, smax) : 0.0;
+    set_sigma_active(cur, zthr);
+    const double rp = sane_positive(rel_p_, 1e-300), rd = sane_positive(rel_d_, 1e-300),
+                 rg = sane_positive(rel_gap_, 1e-300);
+    const double score =
+        std::fmax(score_component(rel_p_), std::fmax(score_component(rel_d_), score_component(rel_gap_)));
+    if (!std::isfinite(score)) return;
+    if (!std::isfinite(best_sigma_score_) || score < best_sigma_score_) {
+        best_sigma_score_ = score;
+        best_al_sigma_ = cur;
+    }
+    if (P.sigma_rollback_tol > 0.0 && (!guarded || cur > zthr) && std::isfinite(best_sigma_score_) &&
+        best_sigma_score_ > 0.0 && score > P.sigma_rollback_tol * best_sigma_score_ && std::isfinite(best_al_sigma_) &&
+        (guarded ? best_al_sigma_ >= 0.0 : best_al_sigma_ > 0.0)) {
+        const double best = guarded && best_al_sigma_ <= zthr ? 0.0 : clampd_h(best_al_sigma_, smin, smax);
+        set_sigma_active(best, zthr);
+        last_sigma_score_ = score;
+        last_sigma_rp_ = rp;
+        last_sigma_rd_ = rd;
+        last_sigma_gap_ = rg;
+        sigma_update_count_++;
+        return;
+    }
+    double prop = cur;
+    switch (P.sigma_update_rule) {
+    case FA_CP_SIGMA_UPDATE_RESIDUAL_BALANCE: prop = clipped_update(cur, rp / rd); break;
+    case FA_CP_SIGMA_UPDATE_KKT_BALANCE: prop = clipped_update(cur, rp / std::fmax(rd, rg)); break;
+    case FA_CP_SIGMA_UPDATE_DISPLACEMENT:
+        if (std::isfinite(pd) && std::isfinite(dd) && pd > 1e-16 && dd > 1e-16) {
+            const double target = clampd_h(dd / pd, smin, smax);
+            prop = clipped_update(cur, target / cur);
+        }
+        break;
+    case FA_CP_SIGMA_UPDATE_PROGRESS: {
+        const double lo = std::fmax(last_sigma_rd_, last_sigma_gap_);
+        const double other = std::fmax(rd, rg);
+        if (std::isfinite(last_sigma_rp_) && std::isfinite(lo) && last_sigma_rp_ > 0.0 && lo > 0.0)
+            prop = clipped_update(cur, (lo / other) / (last_sigma_rp_ / rp));
+        else
+            prop = clipped_update(cur, rp / std::fmax(rd, rg));
+        break;
+    }
+    case FA_CP_SIGMA_UPDATE_GREEDY_SCORE:
+        if (sigma_update_count_ == 0) sigma_dir_ = (rp >= std::fmax(rd, rg)) ? 1 : -1;
+        else if (std::isfinite(score) && std::isfinite(last_sigma_score_) && score > last_sigma_score_)
+            sigma_dir_ *= -1;
+        prop = clipped_update(cur, sigma_dir_ > 0 ? P.sigma_clip_high : P.sigma_clip_low);
+        break;
+    case FA_CP_SIGMA_UPDATE_GUARDED:
+    case FA_CP_SIGMA_UPDATE_BALANCED_GUARDED: {
+        const double nthr = sane_positive(P.sigma_normal_motion_threshold, 1e-10);
+        const bool normal_ok = !std::isfinite(normal_motion) || normal_motion > nthr;
+        const bool zero_ok = !std::isfinite(normal_motion) || normal_motion > std::fmax(nthr, 2e-1);
+        const bool bottleneck =
+            rp >= sane_positive(P.sigma_activation_threshold, 2.0) * sane_positive(std::fmax(rd, rg), 1e-300);
+        const bool zero_stalled = cur <= zthr && std::isfinite(last_sigma_score_) && score > 0.9 * last_sigma_score_;
+        const bool primal_signal = rp >= 0.5 * sane_positive(rd, 1e-300);
+        const double act = sane_positive(P.sigma_activation_value, 0.1) * ref;
+        const bool scale_ok = !bal || act >= 1e-6;
+        const bool activate =
+            cur <= zthr ? zero_ok && zero_stalled && primal_signal && scale_ok : normal_ok && bottleneck;
+        if (cur <= zthr) prop = activate ? act : 0.0;
+        else if (!normal_ok) prop = clipped_update(cur, P.sigma_clip_low);
+        else if (activate) prop = clipped_update(cur, rp / std::fmax(rd, rg));
+        else prop = clipped_update(cur, P.sigma_clip_low);
+        break;
+    }
+    default: break;
+    }
+    if (prop > zthr) prop = clampd_h(prop, smin, smax);
+    else prop = 0.0;
+    set_sigma_active(prop, zthr);
+    last_sigma_score_ = score;
+    last_sigma_rp_ = rp;
+    last_sigma_rd_ = rd;
+    last_sigma_gap_ = rg;
+    sigma_update_count_++;
+}
+
+void Solver::perform_restart()
+{   // perform_restart, fa_cp_core_op.cu:908-1020 (restart target: the T-output)
+    const int nbn = stride_grid(n_), nbm = stride_grid(m_);
+    diff_norm_kernel<<<nbn, kBlock, 0, stream_>>>(xT_.get(), x0_.get(), dx_.get(), n_, parts_.get());
+    reduce(nbn, 1, 1, S_PD2);
+    diff_norm_kernel<<<nbm, kBlock, 0, stream_>>>(yT_.get(), y0_.get(), dy_.get(), m_, parts_.get());
+    reduce(nbm, 1, 1, S_DD2);
+    PDAL_LAUNCHED();
+    fetch(S_PD2, 2);
+    const double pd = std::sqrt(hs(S_PD2)), dd = std::sqrt(hs(S_DD2));
+    double normal = INFINITY;
+    if (pd > 1e-16 && std::isfinite(pd)) {
+        int nb = 1;
+        dispatch_group(A_.G, [&](auto g) {
+            constexpr int G = decltype(g)::value;
+            nb = row_grid(m_, G);
+            spmv_norm_kernel<G><<<nb, kBlock, 0, stream_>>>(A_.view(), dx_.get(), prim_prod_.get(), parts_.get());
+        });
+        PDAL_LAUNCHED();
+        reduce(nb, 1, 1, S_NORMAL);
+        fetch(S_NORMAL, 1);
+        normal = std::sqrt(hs(S_NORMAL)) / pd;
+    }
+    const double ratio = rel_d_ / rel_p_;
+    const auto &rp = P_.restart_params;
+    if (pd > 1e-16 && dd > 1e-16 && pd < 1e12 && dd < 1e12 && ratio > 1e-8 && ratio < 1e8) {
+        const double err = std::log(dd) - std::log(pd) - std::log(pw_);
+        pw_err_sum_ *= rp.i_smooth;
+        pw_err_sum_ += err;
+        const double de = err - pw_last_err_;
+        pw_ *= std::exp(rp.k_p * err + rp.k_i * pw_err_sum_ + rp.k_d * de);
+        pw_last_err_ = err;
+    } else {
+        pw_ = best_pw_;
+        pw_err_sum_ = 0.0;
+        pw_last_err_ = 0.0;
+    }
+    const double g = std::fabs(std::log10(rel_d_ / rel_p_));
+    if (g < best_gap_) {
+        best_gap_ = g;
+        best_pw_ = pw_;
+    }
+    update_al_sigma(pd, dd, normal);
+    restart_copy_kernel<<<blocks_for((long)n_ + m_), kBlock, 0, stream_>>>(xT_.get(), x0_.get(), x_.get(), n_,
+                                                                           yT_.get(), y0_.get(), y_.get(), m_);
+    PDAL_LAUNCHED();
+    inner_ = 0;
+    last_trial_fp_ = INFINITY;
+}
+
+fa_cp_result_t *Solver::make_result()
+{   // create_result_from_state, fa_cp_core_op.cu:1171-1239
+    spmv(At_, yT_.get(), dual_prod_.get());
+    // reuse iterate buffers that are no longer needed as outputs
+    double *xo = grad_.get(), *rc = dir_.get(), *yo = al_u_.get();
+    result_kernel<<<blocks_for((long)n_ + m_), kBlock, 0, stream_>>>(xT_.get(), D_.get(), b_scale_, xo, c_.get(),
+                                                                     dual_prod_.get(), c_scale_, rc, n_, yT_.get(),
+                                                                     E_.get(), yo, m_);
+    PDAL_LAUNCHED();
+    PDAL_HIP(hipMemcpyAsync(hbb_.get(), dbb_.get(), sizeof(BBState), hipMemcpyDeviceToHost, stream_));
+    fa_cp_result_t *r = (fa_cp_result_t *)calloc(1, sizeof(fa_cp_result_t));
+    r->primal_solution = (double *)malloc((size_t)std::max(n_, 1) * sizeof(double));
+    r->dual_solution = (double *)malloc((size_t)std::max(m_, 1) * sizeof(double));
+    r->reduced_cost = (double *)malloc((size_t)std::max(n_, 1) * sizeof(double));
+    PDAL_HIP(hipMemcpyAsync(r->primal_solution, xo, (size_t)n_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    PDAL_HIP(hipMemcpyAsync(r->reduced_cost, rc, (size_t)n_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    PDAL_HIP(hipMemcpyAsync(r->dual_solution, yo, (size_t)m_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    PDAL_HIP(hipStreamSynchronize(stream_));
+    r->num_variables = W_->num_variables;
+    r->num_constraints = W_->num_constraints;
+    r->num_nonzeros = W_->constraint_matrix_num_nonzeros;
+    r->total_count = total_;
+    r->total_inner_count = (int)hbb_[0].total;
+    r->rescaling_time_sec = rescale_time_;
+    r->cumulative_time_sec = solve_time_;
+    r->absolute_primal_residual = abs_p_;
+    r->relative_primal_residual = rel_p_;
+    r->absolute_dual_residual = abs_d_;
+    r->relative_dual_residual = rel_d_;
+    r->primal_objective_value = p_obj_;
+    r->dual_objective_value = d_obj_;
+    r->objective_gap = gap_;
+    r->relative_objective_gap = rel_gap_;
+    r->max_primal_ray_infeasibility = max_p_ray_;
+    r->max_dual_ray_infeasibility = max_d_ray_;
+    r->primal_ray_linear_objective = p_ray_obj_;
+    r->dual_ray_objective = d_ray_obj_;
+    r->final_al_sigma = al_sigma_;
+    r->sigma_update_count = sigma_update_count_;
+    r->sigma_update_rule = al_available_ ? P_.sigma_update_rule : FA_CP_SIGMA_UPDATE_FIXED;
+    r->termination_reason = reason_;
+    return r;
+}
+
+void Solver::print_initial_info() const
+{   // print_initial_info, utils.cu:560-659 (the settings that differ from the defaults are omitted)
+    if (P_.verbose < 2) return;
+    printf("---------------------------------------------------------------------------------------\n");
+    printf("                          FA_CP (MI355X / gfx950 HIP build)\n");
+    printf("---------------------------------------------------------------------------------------\n");
+    printf("problem: %d rows, %d columns, %d nonzeros\n", W_->num_constraints, W_->num_variables,
+           W_->constraint_matrix_num_nonzeros);
+    printf("settings:\n");
+    printf("  iter_limit         : %d\n", P_.termination_criteria.iteration_limit);
+    printf("  time_limit         : %.2f sec\n", P_.termination_criteria.time_sec_limit);
+    printf("  eps_opt            : %.1e\n", P_.termination_criteria.eps_optimal_relative);
+    printf("  eps_feas           : %.1e\n", P_.termination_criteria.eps_feasible_relative);
+    printf("  eps_infeas_detect  : %.1e\n", P_.termination_criteria.eps_infeasible);
+}
+
+void Solver::display_stats() const
+{   // display_iteration_stats, utils.cu:680-688
+    if (P_.verbose < 2) return;
+    if (total_ % print_frequency(total_) == 0)
+        printf("%6d %.1e | %8.1e  %8.1e | %.1e %.1e %.1e | %.1e %.1e %.1e \n", total_, solve_time_, p_obj_, d_obj_,
+               abs_p_, abs_d_, gap_, rel_p_, rel_d_, rel_gap_);
+}
+
+fa_cp_result_t *Solver::run()
+{   // optimize, QP/FA_CP/src/solver.cu:74-168
+    print_initial_info();
+    const auto t_setup = std::chrono::steady_clock::now();
+    upload();
+    rescale();
+    rescale_time_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_setup).count();
+    init_state();
+    if (nonconvexity_ < 0) inner_limit_ = 1;
+    // initialize_step_size_and_primal_weight, fa_cp_core_op.cu:1022-1040
+    step_ = A_.nnz == 0 ? 1.0 : 0.998 / max_singular_value();
+    pw_ = P_.bound_objective_rescaling ? 1.0 : (c_norm_ + 1.0) / (b_norm_ + 1.0);
+    best_pw_ = pw_;
+    PDAL_HIP(hipStreamSynchronize(stream_));
+
+    const auto t0 = std::chrono::steady_clock::now();
+    const int K = P_.termination_evaluation_frequency;
+    bool do_restart = false;
+    while (total_ < P_.termination_criteria.iteration_limit) {
+        const bool consider = major_ || total_ == 0;
+        bool planned = false;
+        if (consider || total_ % print_frequency(total_) == 0) {
+            compute_residual();
+            if (major_ && total_ < 3 * K) infeasibility_info();
+            solve_time_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            check_termination();
+            if (consider && reason_ == TERMINATION_REASON_UNSPECIFIED) planned = should_restart();
+            display_stats();
+            if (reason_ != TERMINATION_REASON_UNSPECIFIED) break;
+        }
+        if (consider) {
+            do_restart = planned;
+            if (do_restart) perform_restart();
+        }
+        major_ = ((total_ + 1) % K) == 0;
+        const bool need_fp = major_ || do_restart;
+        update(!need_fp);
+        if (need_fp) {
+            fixed_point_error();
+            if (do_restart) {
+                fp0_ = fp_;
+                do_restart = false;
+            }
+            halpern();
+        }
+        inner_++;
+        total_++;
+    }
+    if (reason_ == TERMINATION_REASON_UNSPECIFIED) {
+        reason_ = TERMINATION_REASON_ITERATION_LIMIT;
+        compute_residual();
+        display_stats();
+    }
+    return make_result();
+}
+
+}  // namespace
+
+fa_cp_result_t *optimize_gpu(const fa_cp_parameters_t *params, const qp_problem_t *prob)
+{
+    Solver s(*params, prob);
+    return s.run();
+}
+
+void reset_rng()
+{
+    g_gen.seed(1);
+    g_dist.reset();
+    srand(1);
+}
+
+const char *termination_reason_name(termination_reason_t r)
+{
+    switch (r) {
+    case TERMINATION_REASON_OPTIMAL: return "OPTIMAL";
+    case TERMINATION_REASON_PRIMAL_INFEASIBLE: return "PRIMAL_INFEASIBLE";
+    case TERMINATION_REASON_DUAL_INFEASIBLE: return "DUAL_INFEASIBLE";
+    case TERMINATION_REASON_INFEASIBLE_OR_UNBOUNDED: return "INFEASIBLE_OR_UNBOUNDED";
+    case TERMINATION_REASON_TIME_LIMIT: return "TIME_LIMIT";
+    case TERMINATION_REASON_ITERATION_LIMIT: return "ITERATION_LIMIT";
+    case TERMINATION_REASON_FEAS_POLISH_SUCCESS: return "FEAS_POLISH_SUCCESS";
+    default: return "UNSPECIFIED";
+    }
+}
+
+}  // namespace qp
+}  // namespace pdal

@@ -15,6 +15,8 @@ os.environ.setdefault("PDAL_QUIET", "1")
 
 ORACLE_LP = ROOT / "oracle" / "_build" / "libcp_al_oracle.so"
 PRODUCT_LP = ROOT / "pdal_amd" / "lib" / "libcp_al.so"
+ORACLE_QP = ROOT / "oracle" / "_build" / "libfa_cp_oracle.so"
+ORACLE_QP_REV = ROOT / "oracle" / "_build" / "libfa_cp_oracle_rev.so"
 
 
 def pytest_configure(config):
@@ -35,6 +37,31 @@ def gpu_lp():
     """The MI355X product library; the test fails (no fallback) if it is missing."""
     from pdal_amd.lp import load_cp_al
     return load_cp_al()
+
+
+@pytest.fixture(scope="session")
+def oracle_qp():
+    """CPU FA_CP oracle library (test infrastructure only)."""
+    from pdal_amd.qp import FaCp
+    if not ORACLE_QP.exists():
+        pytest.fail(f"oracle not built: run python oracle/build_oracle.py ({ORACLE_QP})")
+    return FaCp(ORACLE_QP)
+
+
+@pytest.fixture(scope="session")
+def oracle_qp_rev():
+    """The FA_CP oracle built to sum in reverse order (same algorithm, other rounding)."""
+    from pdal_amd.qp import FaCp
+    if not ORACLE_QP_REV.exists():
+        pytest.fail(f"oracle not built: run python oracle/build_oracle.py ({ORACLE_QP_REV})")
+    return FaCp(ORACLE_QP_REV)
+
+
+@pytest.fixture(scope="session")
+def gpu_qp():
+    """The MI355X FA_CP library; the test fails (no fallback) if it is missing."""
+    from pdal_amd.qp import load_fa_cp
+    return load_fa_cp()
 
 
 @pytest.fixture(autouse=True)
