@@ -1,0 +1,58 @@
+/*
+ * Batched CP_AL: solve many independent LPs in one device-resident run.
+ *
+ * No counterpart in the reference API (the reference solves one LP per
+ * process call, LP/CP_AL/include/cp_al.h:28-47).  Every instance follows the
+ * single-instance algorithm of solve_lp_problem exactly (same rescaling, step
+ * sizes, blocks of termination_evaluation_frequency iterations, restart and
+ * termination rules), with its own state kept on the device, so a caller that
+ * looped over solve_lp_problem gets the same per-instance results.  Two
+ * differences: the power-iteration start vector of every instance is drawn
+ * from a freshly seeded std::mt19937(1) (as after pdal_cp_al_reset_rng), and
+ * time_sec_limit bounds the whole batch (instances still running when it
+ * expires end with TIME_LIMIT).  The presolve and the environment-driven
+ * controllers of the reference are not applied, as in solve_lp_problem here.
+ *
+ * Lifetime: create uploads, rescales and estimates every instance's step size
+ * once; solve may be called repeatedly (each call restarts every instance
+ * from its start point); the problems may be freed after create.
+ */
+#pragma once
+
+#include "cp_al_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pdal_cp_al_batch pdal_cp_al_batch_t;
+
+/* Set up a batch of `count` problems (primal_start / dual_start of each
+ * problem, when present, are its start point).  params NULL = defaults.
+ * Returns NULL on error (invalid parameters, > 2^31-1 total rows, columns or
+ * nonzeros, device failure). */
+pdal_cp_al_batch_t *pdal_cp_al_batch_create(lp_problem_t *const *problems, int count,
+                                            const cp_al_parameters_t *params);
+
+/* Replace the start points of all instances: x is the concatenation of the
+ * instances' primal starts (sum of num_variables entries), y of their dual
+ * starts; either may be NULL to keep the current one.  Returns 0 on success. */
+int pdal_cp_al_batch_set_starts(pdal_cp_al_batch_t *batch, const double *x, const double *y);
+
+/* Solve every instance; returns the number that reached OPTIMAL, or -1. */
+int pdal_cp_al_batch_solve(pdal_cp_al_batch_t *batch);
+
+/* Result of instance i of the last solve (free with cp_al_result_free);
+ * NULL before the first solve or for an invalid index. */
+cp_al_result_t *pdal_cp_al_batch_result(pdal_cp_al_batch_t *batch, int i);
+
+/* out[0..7] = {set-up seconds, last solve seconds (wall clock, device
+ * resident), hipGraph replays, instances, OPTIMAL instances, sum of
+ * iteration counts, lanes per row of A, lanes per row of At}. */
+void pdal_cp_al_batch_stats(const pdal_cp_al_batch_t *batch, double *out);
+
+void pdal_cp_al_batch_free(pdal_cp_al_batch_t *batch);
+
+#ifdef __cplusplus
+}
+#endif

@@ -39,6 +39,7 @@ HIP_FLAGS = [
 CP_AL_SOURCES = [
     CSRC / "lp" / "cp_al_api.cpp",
     CSRC / "lp" / "cp_al_solver.hip",
+    CSRC / "lp" / "batch_solver.hip",
     CSRC / "lp" / "mps_parser.cpp",
 ]
 

@@ -60,7 +60,7 @@ __device__ __forceinline__ void block_reduce_store(double (&q)[NSUM + NMAX], dou
 
 // Combine `nparts` rows of `stride` partials (row b = block b) into out[0..NQ):
 // entries [0,nsum) summed, [nsum,nq) maxed.  Launch with one 256-thread block.
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     reduce_partials_kernel(const double *__restrict__ parts, int nparts, int stride, int nsum, int nq,
                            double *__restrict__ out)
 {

@@ -50,7 +50,7 @@ __device__ __forceinline__ void row_dot2(const CsrView &M, int row, int lane, co
 
 // Plain y = M x.
 template <int G>
-__global__ void __launch_bounds__(kBlock) spmv_kernel(CsrView M, const double *__restrict__ x, double *__restrict__ y)
+static __global__ void __launch_bounds__(kBlock) spmv_kernel(CsrView M, const double *__restrict__ x, double *__restrict__ y)
 {
     const int tid = blockIdx.x * kBlock + threadIdx.x;
     const int row = tid / G;

@@ -1,0 +1,732 @@
+// Batched CP_AL: many independent LPs in one device-resident solve.
+//
+// Each instance follows exactly the single-instance CP_AL schedule (blocks of
+// K = termination_evaluation_frequency iterations, major kernels at iterations
+// 1 and K, evaluation, termination, adaptive restart with the PID primal
+// weight; LP/CP_AL/src/solver.cu:1197-1736) with its own step sizes, restart
+// state and termination.  The instances are stored block-diagonally (see
+// batch_kernels.hpp); one hipGraph holds a block of K iterations, the
+// evaluation, the per-instance segmented reductions, the per-instance control
+// kernel and the restart copy, and is replayed until every instance has
+// stopped.  The host only polls one integer (the active count) per replay,
+// two replays behind, so the GPU never waits on the host.
+//
+// Set-up per batch: concatenation and one stable transpose on the host;
+// Ruiz / Pock-Chambolle scaling with the single-instance kernels (row and
+// column operations never cross instance boundaries), per-instance
+// bound-objective scaling, and a batched power iteration for each instance's
+// sigma_max(A)^2 (each instance starts from the first m_i draws of a freshly
+// seeded std::mt19937(1) normal stream, i.e. as if the generator were reset
+// before each single solve).
+#include "cp_al.h"
+#include "cp_al_internal.hpp"
+#include "pdal_batch.h"
+
+#include "../common/csr_host.hpp"
+#include "../common/hip_util.hpp"
+#include "batch_kernels.hpp"
+#include "cp_al_kernels.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <vector>
+
+namespace pdal {
+namespace lp {
+namespace batch {  // inner scope: the batched kernels hide the single-instance ones of the same name
+
+namespace {
+
+struct Table {
+    DBuf<int> inst, row0;
+    std::vector<int> first, count;  // per instance: first block, number of blocks
+    DBuf<int> d_first, d_count;
+    int blocks = 0, G = 1;
+};
+
+inline int blocks_for_l(long work) { return work > 0 ? (int)((work + kBlock - 1) / kBlock) : 1; }
+
+}  // namespace
+
+class BatchSolver {
+  public:
+    BatchSolver(lp_problem_t *const *probs, int count, const cp_al_parameters_t &p);
+    ~BatchSolver();
+    void setup();
+    void set_starts(const double *x, const double *y);  // concatenated, unscaled; nullptr = keep
+    int solve();
+    cp_al_result_t *result(int i);
+    void stats(double *out) const;
+    int count() const { return N_; }
+
+  private:
+    void build_table(Table &t, const std::vector<long> &off, int G);
+    void rescale();
+    void power_iteration();
+    void autotune();
+    void reset();
+    void launch_block();
+    void record_graph();
+
+    std::vector<const lp_problem_t *> probs_;
+    cp_al_parameters_t P_;
+    int N_ = 0;
+    long m_ = 0, n_ = 0, nnz_ = 0;
+    std::vector<long> row_off_, col_off_;  // N+1 prefix offsets
+    hipStream_t stream_ = nullptr;
+    DevCsr A_, At_;
+    DBuf<int> row_inst_, col_inst_, row_end_a_, row_end_at_;
+    DBuf<long> d_row_off_, d_col_off_;
+    Table ta_, tat_;
+    DBuf<double> c_, lb_, ub_, l_, u_, E_, D_;
+    DBuf<double> x0_, x_, xT_, xH_, xR_, dslack_, y0_, y_, yT_, yH_, yR_;
+    DBuf<double> xs_, ys_;  // scaled starts (kept for repeated solves)
+    DBuf<double> parts_at_, parts_a_, evalq_;
+    DBuf<InstState> st_;
+    std::vector<InstState> init_;
+    DBuf<int> active_;
+    PinnedBuf<int> h_active_{4};
+    hipGraphExec_t graph_ = nullptr;
+    ControlParams cp_{};
+    bool pred_ = false;
+    double rho_ = 1.0;
+    double setup_s_ = 0.0, solve_s_ = 0.0;
+    long replays_ = 0;
+    std::vector<InstState> final_;
+    hipEvent_t ev_[2] = {nullptr, nullptr};
+    bool res_ready_ = false;
+    DBuf<double> rc_all_, xo_all_, yo_all_;
+
+    void copy_d(double *dst, const double *src, long len)
+    {
+        if (len > 0)
+            PDAL_HIP(hipMemcpyAsync(dst, src, (size_t)len * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    }
+};
+
+BatchSolver::BatchSolver(lp_problem_t *const *probs, int count, const cp_al_parameters_t &p)
+    : probs_(probs, probs + count), P_(p), N_(count)
+{
+}
+
+BatchSolver::~BatchSolver()
+{
+    if (graph_) (void)hipGraphExecDestroy(graph_);
+    for (auto &e : ev_)
+        if (e) (void)hipEventDestroy(e);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void BatchSolver::build_table(Table &t, const std::vector<long> &off, int G)
+{
+    const int rpb = kBlock / G;
+    std::vector<int> inst, row0;
+    t.first.assign(N_, 0);
+    t.count.assign(N_, 0);
+    for (int i = 0; i < N_; ++i) {
+        t.first[i] = (int)inst.size();
+        for (long r = off[i]; r < off[i + 1]; r += rpb) {
+            inst.push_back(i);
+            row0.push_back((int)r);
+        }
+        t.count[i] = (int)inst.size() - t.first[i];
+    }
+    t.blocks = (int)inst.size();
+    t.G = G;
+    t.inst.alloc(std::max<size_t>(inst.size(), 1));
+    t.row0.alloc(std::max<size_t>(row0.size(), 1));
+    t.inst.upload(inst.data(), inst.size(), stream_);
+    t.row0.upload(row0.data(), row0.size(), stream_);
+    t.d_first.alloc(N_);
+    t.d_count.alloc(N_);
+    t.d_first.upload(t.first.data(), N_, stream_);
+    t.d_count.upload(t.count.data(), N_, stream_);
+    PDAL_HIP(hipStreamSynchronize(stream_));
+}
+
+void BatchSolver::setup()
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    PDAL_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    for (auto &e : ev_) PDAL_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    // --- concatenate (block diagonal) ---
+    row_off_.assign(N_ + 1, 0);
+    col_off_.assign(N_ + 1, 0);
+    for (int i = 0; i < N_; ++i) {
+        const lp_problem_t *w = probs_[i];
+        row_off_[i + 1] = row_off_[i] + w->num_constraints;
+        col_off_[i + 1] = col_off_[i] + w->num_variables;
+        nnz_ += w->constraint_matrix_num_nonzeros;
+    }
+    m_ = row_off_[N_];
+    n_ = col_off_[N_];
+    if (m_ > INT32_MAX - 1 || n_ > INT32_MAX - 1 || nnz_ > INT32_MAX - 1)
+        throw HipError("batch exceeds int32 indexing (rows, columns or nonzeros)");
+    std::vector<int> rp((size_t)m_ + 1), ci(nnz_), rinst(m_), cinst(n_);
+    std::vector<double> v(nnz_), c(n_), lb(n_), ub(n_), l(m_), u(m_), xs(n_, 0.0), ys(m_, 0.0);
+    long k = 0;
+    rp[0] = 0;
+    for (int i = 0; i < N_; ++i) {
+        const lp_problem_t *w = probs_[i];
+        const long r0 = row_off_[i], c0 = col_off_[i];
+        for (int r = 0; r < w->num_constraints; ++r) {
+            for (int q = w->constraint_matrix_row_pointers[r]; q < w->constraint_matrix_row_pointers[r + 1]; ++q) {
+                ci[k] = (int)(c0 + w->constraint_matrix_col_indices[q]);
+                v[k] = w->constraint_matrix_values[q];
+                ++k;
+            }
+            rp[r0 + r + 1] = (int)k;
+            rinst[r0 + r] = i;
+            l[r0 + r] = w->constraint_lower_bound[r];
+            u[r0 + r] = w->constraint_upper_bound[r];
+            if (w->dual_start) ys[r0 + r] = w->dual_start[r];
+        }
+        for (int j = 0; j < w->num_variables; ++j) {
+            cinst[c0 + j] = i;
+            c[c0 + j] = w->objective_vector[j];
+            lb[c0 + j] = w->variable_lower_bound[j];
+            ub[c0 + j] = w->variable_upper_bound[j];
+            if (w->primal_start) xs[c0 + j] = w->primal_start[j];
+        }
+    }
+    HostCsr T = transpose_csr((int)m_, (int)n_, rp.data(), ci.data(), v.data());
+    auto put = [&](DevCsr &Dm, int rows, const int *hrp, const int *hci, const double *hv) {
+        Dm.rows = rows;
+        Dm.nnz = (int)nnz_;
+        Dm.rp.alloc((size_t)rows + 1);
+        Dm.ci.alloc(nnz_);
+        Dm.v.alloc(nnz_);
+        Dm.rp.upload(hrp, (size_t)rows + 1, stream_);
+        Dm.ci.upload(hci, nnz_, stream_);
+        Dm.v.upload(hv, nnz_, stream_);
+    };
+    put(A_, (int)m_, rp.data(), ci.data(), v.data());
+    put(At_, (int)n_, T.rp.data(), T.ci.data(), T.v.data());
+    row_inst_.alloc(m_);
+    col_inst_.alloc(n_);
+    row_inst_.upload(rinst.data(), m_, stream_);
+    col_inst_.upload(cinst.data(), n_, stream_);
+    std::vector<int> rea(N_), reat(N_);
+    for (int i = 0; i < N_; ++i) {
+        rea[i] = (int)row_off_[i + 1];
+        reat[i] = (int)col_off_[i + 1];
+    }
+    row_end_a_.alloc(N_);
+    row_end_at_.alloc(N_);
+    row_end_a_.upload(rea.data(), N_, stream_);
+    row_end_at_.upload(reat.data(), N_, stream_);
+    d_row_off_.alloc(N_ + 1);
+    d_col_off_.alloc(N_ + 1);
+    d_row_off_.upload(row_off_.data(), N_ + 1, stream_);
+    d_col_off_.upload(col_off_.data(), N_ + 1, stream_);
+    for (auto *b : {&c_, &lb_, &ub_, &D_, &x0_, &x_, &xT_, &xH_, &xR_, &dslack_, &xs_}) b->alloc(n_);
+    for (auto *b : {&l_, &u_, &E_, &y0_, &y_, &yT_, &yH_, &yR_, &ys_}) b->alloc(m_);
+    c_.upload(c.data(), n_, stream_);
+    lb_.upload(lb.data(), n_, stream_);
+    ub_.upload(ub.data(), n_, stream_);
+    l_.upload(l.data(), m_, stream_);
+    u_.upload(u.data(), m_, stream_);
+    xs_.upload(xs.data(), n_, stream_);
+    ys_.upload(ys.data(), m_, stream_);
+    st_.alloc(N_);
+    active_.alloc(1);
+    evalq_.alloc((size_t)N_ * kEvalSlots);
+    PDAL_HIP(hipMemsetAsync(evalq_.get(), 0, (size_t)N_ * kEvalSlots * sizeof(double), stream_));
+    PDAL_HIP(hipStreamSynchronize(stream_));
+
+    // --- per-instance norms of the unscaled problems (solver.cu:1999-2067) ---
+    init_.assign(N_, InstState{});
+    const bool linf = P_.optimality_norm == NORM_TYPE_L_INF;
+    for (int i = 0; i < N_; ++i) {
+        const lp_problem_t *w = probs_[i];
+        double sq = 0.0, mx = 0.0;
+        for (int j = 0; j < w->num_variables; ++j) {
+            const double cj = w->objective_vector[j];
+            if (linf) mx = std::max(mx, std::fabs(cj));
+            else sq += cj * cj;
+        }
+        init_[i].c_norm = linf ? mx : std::sqrt(sq);
+        sq = mx = 0.0;
+        for (int r = 0; r < w->num_constraints; ++r) {
+            const double lo = w->constraint_lower_bound[r], hi = w->constraint_upper_bound[r];
+            if (linf) {
+                if (std::isfinite(lo) && lo != hi) mx = std::max(mx, std::fabs(lo));
+                if (std::isfinite(hi)) mx = std::max(mx, std::fabs(hi));
+            } else {
+                if (std::isfinite(lo) && lo != hi) sq += lo * lo;
+                if (std::isfinite(hi)) sq += hi * hi;
+            }
+        }
+        init_[i].b_norm = linf ? mx : std::sqrt(sq);
+        init_[i].obj_const = w->objective_constant;
+    }
+
+    // the starts are scaled together with the problem (as x0/y0 in the single solver)
+    copy_d(x0_.get(), xs_.get(), n_);
+    copy_d(y0_.get(), ys_.get(), m_);
+    rescale();
+    copy_d(xs_.get(), x0_.get(), n_);
+    copy_d(ys_.get(), y0_.get(), m_);
+
+    // --- tables, lanes and step sizes ---
+    PDAL_HIP(hipMemsetAsync(xR_.get(), 0, (size_t)n_ * sizeof(double), stream_));
+    PDAL_HIP(hipMemsetAsync(yR_.get(), 0, (size_t)m_ * sizeof(double), stream_));
+    autotune();
+    parts_at_.alloc((size_t)tat_.blocks * kPartStride);
+    parts_a_.alloc((size_t)ta_.blocks * kPartStride);
+    power_iteration();
+
+    const restart_parameters_t &rp_ = P_.restart_params;
+    cp_.step_margin = env_double("CP_AL_STEP_MARGIN", 0.998 * 0.998);
+    cp_.suff = env_double("CP_AL_RESTART_SUFFICIENT", rp_.sufficient_reduction_for_restart);
+    cp_.nec = env_double("CP_AL_RESTART_NECESSARY", rp_.necessary_reduction_for_restart);
+    cp_.art = env_double("CP_AL_RESTART_ARTIFICIAL", rp_.artificial_restart_threshold);
+    cp_.k_p = rp_.k_p;
+    cp_.k_i = rp_.k_i;
+    cp_.k_d = rp_.k_d;
+    cp_.i_smooth = rp_.i_smooth;
+    cp_.eps_feas = P_.termination_criteria.eps_feasible_relative;
+    cp_.eps_opt = P_.termination_criteria.eps_optimal_relative;
+    cp_.iteration_limit = P_.termination_criteria.iteration_limit;
+    cp_.K = P_.termination_evaluation_frequency;
+    cp_.linf = linf ? 1 : 0;
+    rho_ = P_.reflection_coefficient;
+    const double al_lambda = env_double("CP_AL_LAMBDA", 0.05);
+    for (int i = 0; i < N_; ++i) {  // initialize_step_size_and_primal_weight, solver.cu:3538-3609
+        InstState &s = init_[i];
+        s.pw = P_.bound_objective_rescaling ? 1.0 : (s.c_norm + 1.0) / (s.b_norm + 1.0);
+        s.al_lambda = al_lambda;
+        s.al_lambda = std::fmin(std::fmax(s.al_lambda, 0.0), 0.95);
+        const double eta_sq = cp_.step_margin * (1.0 - s.al_lambda) / std::fmax(s.lambda_est, 1e-12);
+        s.step = std::sqrt(std::fmax(eta_sq, 1e-24));
+        const double dual_step = s.step * std::fmax(s.pw, 1e-12);
+        s.beta = dual_step * s.al_lambda / std::fmax(1.0 - s.al_lambda, 1e-12);
+        s.tau = s.step / s.pw;
+        s.sigma = s.step * s.pw;
+        s.best_pw = s.pw;
+        s.best_gap = INFINITY;
+        s.last_trial_fp = INFINITY;
+        s.active = 1;
+    }
+    pred_ = al_lambda > 0.0 && std::fmin(al_lambda, 0.95) > 0.0;
+    warn_unsupported_controllers();
+    if (env_set("CP_AL_CR_SCALING"))
+        fprintf(stderr, "[box-cpal] CP_AL_CR_SCALING is not applied by the batched solver.\n");
+    setup_s_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void BatchSolver::rescale()
+{   // rescale_problem (preconditioner.cu:379-475) over the block-diagonal system
+    DBuf<double> e(m_), d(n_), inv_e(m_), inv_d(n_);
+    const int mi = (int)m_, ni = (int)n_;
+    fill_kernel<<<blocks_for_l(m_), kBlock, 0, stream_>>>(E_.get(), 1.0, mi);
+    fill_kernel<<<blocks_for_l(n_), kBlock, 0, stream_>>>(D_.get(), 1.0, ni);
+    auto scale_problem = [&]() {
+        scale_vars_kernel<<<blocks_for_l(n_), kBlock, 0, stream_>>>(c_.get(), lb_.get(), ub_.get(), x0_.get(),
+                                                                   d.get(), inv_d.get(), ni);
+        scale_cons_kernel<<<blocks_for_l(m_), kBlock, 0, stream_>>>(l_.get(), u_.get(), y0_.get(), e.get(),
+                                                                   inv_e.get(), mi);
+        scale_rows_kernel<<<blocks_for_l(m_), kBlock, 0, stream_>>>(A_.rp.get(), A_.ci.get(), A_.v.get(), mi,
+                                                                   inv_e.get(), inv_d.get());
+        scale_rows_kernel<<<blocks_for_l(n_), kBlock, 0, stream_>>>(At_.rp.get(), At_.ci.get(), At_.v.get(), ni,
+                                                                   inv_d.get(), inv_e.get());
+        PDAL_LAUNCHED();
+    };
+    for (int it = 0; it < P_.l_inf_ruiz_iterations; ++it) {
+        row_absmax_kernel<<<blocks_for_l(m_), kBlock, 0, stream_>>>(A_.view(), e.get());
+        clamp_sqrt_accum_kernel<<<blocks_for_l(m_), kBlock, 0, stream_>>>(e.get(), inv_e.get(), E_.get(), mi);
+        row_absmax_kernel<<<blocks_for_l(n_), kBlock, 0, stream_>>>(At_.view(), d.get());
+        clamp_sqrt_accum_kernel<<<blocks_for_l(n_), kBlock, 0, stream_>>>(d.get(), inv_d.get(), D_.get(), ni);
+        PDAL_LAUNCHED();
+        scale_problem();
+    }
+    if (P_.has_pock_chambolle_alpha) {
+        row_powsum_kernel<<<blocks_for_l(m_), kBlock, 0, stream_>>>(A_.view(), P_.pock_chambolle_alpha, e.get());
+        clamp_sqrt_accum_kernel<<<blocks_for_l(m_), kBlock, 0, stream_>>>(e.get(), inv_e.get(), E_.get(), mi);
+        row_powsum_kernel<<<blocks_for_l(n_), kBlock, 0, stream_>>>(At_.view(), 2.0 - P_.pock_chambolle_alpha,
+                                                                    d.get());
+        clamp_sqrt_accum_kernel<<<blocks_for_l(n_), kBlock, 0, stream_>>>(d.get(), inv_d.get(), D_.get(), ni);
+        PDAL_LAUNCHED();
+        scale_problem();
+    }
+    std::vector<double> scales(2 * (size_t)N_, 1.0);
+    if (P_.bound_objective_rescaling) {
+        DBuf<double> ds(2 * (size_t)N_);
+        seg_bound_sums_kernel<<<N_, kBlock, 0, stream_>>>(d_row_off_.get(), d_col_off_.get(), l_.get(), u_.get(),
+                                                          c_.get(), ds.get());
+        seg_bound_apply_kernel<<<blocks_for_l(m_ + n_), kBlock, 0, stream_>>>(
+            row_inst_.get(), l_.get(), u_.get(), y0_.get(), m_, col_inst_.get(), lb_.get(), ub_.get(), c_.get(),
+            x0_.get(), n_, ds.get());
+        PDAL_LAUNCHED();
+        ds.download(scales.data(), 2 * (size_t)N_, stream_);
+    }
+    PDAL_HIP(hipStreamSynchronize(stream_));
+    for (int i = 0; i < N_; ++i) {
+        init_[i].b_scale = scales[2 * i];
+        init_[i].c_scale = scales[2 * i + 1];
+    }
+}
+
+void BatchSolver::autotune()
+{   // rows-per-wavefront: time one plain SpMV pass over the whole batch per lane count
+    hipEvent_t a, b;
+    PDAL_HIP(hipEventCreate(&a));
+    PDAL_HIP(hipEventCreate(&b));
+    auto tune = [&](const DevCsr &M, const std::vector<long> &off, const char *env, Table &t, const double *x,
+                    double *y, const int *row_end) {
+        std::vector<int> cands = {1, 2, 4, 8, 16, 32};
+        if (const char *raw = getenv(env)) {
+            const int g = atoi(raw);
+            if (g == 1 || g == 2 || g == 4 || g == 8 || g == 16 || g == 32 || g == 64) cands = {g};
+        }
+        float best = 1e30f;
+        int bestG = cands[0];
+        for (int G : cands) {
+            if (cands.size() == 1) break;
+            Table tt;
+            build_table(tt, off, G);
+            const BlockTable bt{tt.inst.get(), tt.row0.get(), row_end};
+            auto launch = [&]() {
+                dispatch_group(G, [&](auto g) {
+                    constexpr int GG = decltype(g)::value;
+                    table_spmv_kernel<GG><<<tt.blocks, kBlock, 0, stream_>>>(M.view(), bt, x, y);
+                });
+                PDAL_LAUNCHED();
+            };
+            launch();
+            PDAL_HIP(hipEventRecord(a, stream_));
+            for (int r = 0; r < 3; ++r) launch();
+            PDAL_HIP(hipEventRecord(b, stream_));
+            PDAL_HIP(hipEventSynchronize(b));
+            float ms = 0.0f;
+            PDAL_HIP(hipEventElapsedTime(&ms, a, b));
+            if (ms < best) {
+                best = ms;
+                bestG = G;
+            }
+        }
+        build_table(t, off, bestG);
+    };
+    // x and y buffers hold zeros here; the timing depends only on the sparsity
+    tune(A_, row_off_, "PDAL_BATCH_G_A", ta_, xR_.get(), yR_.get(), row_end_a_.get());
+    tune(At_, col_off_, "PDAL_BATCH_G_AT", tat_, yR_.get(), xR_.get(), row_end_at_.get());
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+}
+
+void BatchSolver::power_iteration()
+{   // estimate_maximum_singular_value (utils.cu:87-228) for every instance at once
+    const int max_iter = P_.sv_max_iter;
+    long mmax = 0;
+    for (int i = 0; i < N_; ++i) mmax = std::max(mmax, row_off_[i + 1] - row_off_[i]);
+    std::vector<double> stream(mmax);
+    {   // a freshly seeded generator: every instance takes its prefix of this stream
+        std::mt19937 gen(1);
+        std::normal_distribution<double> dist(0.0, 1.0);
+        for (long i = 0; i < mmax; ++i) stream[i] = dist(gen);
+    }
+    std::vector<double> v0(m_);
+    std::vector<PowerState> ps0(N_);
+    for (int i = 0; i < N_; ++i) {
+        const long mi = row_off_[i + 1] - row_off_[i];
+        std::copy(stream.begin(), stream.begin() + mi, v0.begin() + row_off_[i]);
+        const long nnz_i = probs_[i]->constraint_matrix_num_nonzeros;
+        ps0[i] = PowerState{0.0, 1.0, 0, nnz_i == 0 ? 1 : 0};  // no power iteration for an empty A
+    }
+    DBuf<double> v(m_), vh(m_), w(n_);
+    DBuf<PowerState> ps(N_);
+    v.upload(v0.data(), m_, stream_);
+    ps.upload(ps0.data(), N_, stream_);
+    power_init_kernel<<<N_, kBlock, 0, stream_>>>(d_row_off_.get(), v.get(), ps.get());
+    PDAL_LAUNCHED();
+    const BlockTable bta{ta_.inst.get(), ta_.row0.get(), row_end_a_.get()};
+    const BlockTable btat{tat_.inst.get(), tat_.row0.get(), row_end_at_.get()};
+    const double tol = P_.sv_tol;
+    auto one = [&]() {
+        power_normalize_kernel<<<blocks_for_l(m_), kBlock, 0, stream_>>>(row_inst_.get(), v.get(), vh.get(), m_,
+                                                                        ps.get(), max_iter);
+        dispatch_group(tat_.G, [&](auto g) {
+            constexpr int G = decltype(g)::value;
+            power_at_kernel<G><<<tat_.blocks, kBlock, 0, stream_>>>(At_.view(), btat, vh.get(), w.get(), ps.get(),
+                                                                   max_iter);
+        });
+        dispatch_group(ta_.G, [&](auto g) {
+            constexpr int G = decltype(g)::value;
+            power_a_kernel<G><<<ta_.blocks, kBlock, 0, stream_>>>(A_.view(), bta, w.get(), vh.get(), v.get(),
+                                                                 parts_a_.get(), ps.get(), max_iter);
+        });
+        power_sigma_kernel<<<N_, kBlock, 0, stream_>>>(parts_a_.get(), ta_.d_first.get(), ta_.d_count.get(), ps.get(),
+                                                       max_iter);
+        power_resid_kernel<<<N_, kBlock, 0, stream_>>>(d_row_off_.get(), v.get(), vh.get(), ps.get(), max_iter, tol);
+        PDAL_LAUNCHED();
+    };
+    constexpr int kChunk = 32;
+    hipGraph_t graph;
+    hipGraphExec_t exec = nullptr;
+    PDAL_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeGlobal));
+    for (int i = 0; i < kChunk; ++i) one();
+    PDAL_HIP(hipStreamEndCapture(stream_, &graph));
+    PDAL_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    PDAL_HIP(hipGraphDestroy(graph));
+    std::vector<PowerState> h(N_);
+    for (int done_iters = 0; done_iters < max_iter; done_iters += kChunk) {
+        PDAL_HIP(hipGraphLaunch(exec, stream_));
+        ps.download(h.data(), N_, stream_);
+        PDAL_HIP(hipStreamSynchronize(stream_));
+        bool all = true;
+        for (const PowerState &p : h) all = all && (p.done || p.iter >= max_iter);
+        if (all) break;
+    }
+    (void)hipGraphExecDestroy(exec);
+    for (int i = 0; i < N_; ++i) {
+        const bool empty = probs_[i]->constraint_matrix_num_nonzeros == 0;
+        const double sv = std::sqrt(h[i].sigma_sq);
+        init_[i].lambda_est = empty ? 1.0 : sv * sv;
+    }
+}
+
+void BatchSolver::reset()
+{   // anchor = current = T-output = Halpern point = start (solver.cu:1954-1965)
+    for (auto *b : {&x0_, &x_, &xT_, &xH_}) copy_d(b->get(), xs_.get(), n_);
+    for (auto *b : {&y0_, &y_, &yT_, &yH_}) copy_d(b->get(), ys_.get(), m_);
+    for (auto *b : {&xR_, &dslack_}) PDAL_HIP(hipMemsetAsync(b->get(), 0, (size_t)n_ * sizeof(double), stream_));
+    PDAL_HIP(hipMemsetAsync(yR_.get(), 0, (size_t)m_ * sizeof(double), stream_));
+    st_.upload(init_.data(), N_, stream_);
+}
+
+void BatchSolver::launch_block()
+{
+    const int K = cp_.K;
+    const BlockTable bta{ta_.inst.get(), ta_.row0.get(), row_end_a_.get()};
+    const BlockTable btat{tat_.inst.get(), tat_.row0.get(), row_end_at_.get()};
+    for (int i = 1; i <= K; ++i) {
+        const bool major = i == 1 || i == K;
+        if (pred_) {
+            PredictorArgs pa{A_.view(), bta, x_.get(), y_.get(), l_.get(), u_.get(), yR_.get(), st_.get()};
+            dispatch_group(ta_.G, [&](auto g) {
+                constexpr int G = decltype(g)::value;
+                predictor_kernel<G><<<ta_.blocks, kBlock, 0, stream_>>>(pa);
+            });
+        }
+        PrimalArgs pa{At_.view(), btat, pred_ ? yR_.get() : y_.get(), c_.get(), lb_.get(), ub_.get(), x0_.get(),
+                      x_.get(), xR_.get(), xT_.get(), xH_.get(), dslack_.get(), st_.get(), rho_};
+        dispatch_group(tat_.G, [&](auto g) {
+            constexpr int G = decltype(g)::value;
+            if (major) primal_kernel<G, true><<<tat_.blocks, kBlock, 0, stream_>>>(pa, i);
+            else primal_kernel<G, false><<<tat_.blocks, kBlock, 0, stream_>>>(pa, i);
+        });
+        DualArgs da{A_.view(), bta, xR_.get(), l_.get(), u_.get(), y0_.get(), y_.get(), yT_.get(), yH_.get(),
+                    yR_.get(), st_.get(), rho_};
+        dispatch_group(ta_.G, [&](auto g) {
+            constexpr int G = decltype(g)::value;
+            if (major) dual_kernel<G, true><<<ta_.blocks, kBlock, 0, stream_>>>(da, i);
+            else dual_kernel<G, false><<<ta_.blocks, kBlock, 0, stream_>>>(da, i);
+        });
+        PDAL_LAUNCHED();
+    }
+    EvalAtArgs ea{At_.view(), btat, yR_.get(), yT_.get(), xR_.get(), xT_.get(), x0_.get(), c_.get(), dslack_.get(),
+                  D_.get(), st_.get(), parts_at_.get()};
+    dispatch_group(tat_.G, [&](auto g) {
+        constexpr int G = decltype(g)::value;
+        eval_at_kernel<G><<<tat_.blocks, kBlock, 0, stream_>>>(ea);
+    });
+    EvalAArgs eb{A_.view(), bta, xR_.get(), xT_.get(), yR_.get(), yT_.get(), y0_.get(), l_.get(), u_.get(), E_.get(),
+                 st_.get(), parts_a_.get()};
+    dispatch_group(ta_.G, [&](auto g) {
+        constexpr int G = decltype(g)::value;
+        eval_a_kernel<G><<<ta_.blocks, kBlock, 0, stream_>>>(eb);
+    });
+    seg_reduce_kernel<<<N_, kBlock, 0, stream_>>>(parts_at_.get(), tat_.d_first.get(), tat_.d_count.get(), kAtSums,
+                                                  kAtQ, 0, evalq_.get(), st_.get());
+    seg_reduce_kernel<<<N_, kBlock, 0, stream_>>>(parts_a_.get(), ta_.d_first.get(), ta_.d_count.get(), kASums, kAQ,
+                                                  8, evalq_.get(), st_.get());
+    control_kernel<<<blocks_for_l(N_), kBlock, 0, stream_>>>(st_.get(), evalq_.get(), N_, cp_);
+    restart_copy_kernel<<<blocks_for_l(n_ + m_), kBlock, 0, stream_>>>(col_inst_.get(), xT_.get(), x0_.get(),
+                                                                      x_.get(), n_, row_inst_.get(), yT_.get(),
+                                                                      y0_.get(), y_.get(), m_, st_.get());
+    active_count_kernel<<<1, kBlock, 0, stream_>>>(st_.get(), N_, active_.get());
+    PDAL_LAUNCHED();
+}
+
+void BatchSolver::record_graph()
+{
+    hipGraph_t graph;
+    PDAL_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeGlobal));
+    launch_block();
+    PDAL_HIP(hipStreamEndCapture(stream_, &graph));
+    PDAL_HIP(hipGraphInstantiate(&graph_, graph, nullptr, nullptr, 0));
+    PDAL_HIP(hipGraphDestroy(graph));
+}
+
+int BatchSolver::solve()
+{
+    reset();
+    res_ready_ = false;
+    if (!graph_) record_graph();
+    PDAL_HIP(hipStreamSynchronize(stream_));
+    const auto t0 = std::chrono::steady_clock::now();
+    const double tlim = P_.termination_criteria.time_sec_limit;
+    // Two replays in flight: replay r+1 is queued before the active count of
+    // replay r is read.  Replays after the last instance stopped are no-ops.
+    long launched = 0, checked = 0;
+    bool timed_out = false;
+    while (true) {
+        while (launched < checked + 2) {
+            PDAL_HIP(hipGraphLaunch(graph_, stream_));
+            PDAL_HIP(hipMemcpyAsync(h_active_.get() + (launched & 1), active_.get(), sizeof(int),
+                                    hipMemcpyDeviceToHost, stream_));
+            PDAL_HIP(hipEventRecord(ev_[launched & 1], stream_));
+            ++launched;
+        }
+        PDAL_HIP(hipEventSynchronize(ev_[checked & 1]));
+        const int active = h_active_[checked & 1];
+        ++checked;
+        if (active == 0) break;
+        if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() >= tlim) {
+            timed_out = true;
+            break;
+        }
+    }
+    PDAL_HIP(hipStreamSynchronize(stream_));
+    solve_s_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    replays_ = launched;
+    final_.resize(N_);
+    st_.download(final_.data(), N_, stream_);
+    PDAL_HIP(hipStreamSynchronize(stream_));
+    int optimal = 0;
+    for (InstState &s : final_) {
+        if (s.active) s.reason = timed_out ? TERMINATION_REASON_TIME_LIMIT : TERMINATION_REASON_ITERATION_LIMIT;
+        optimal += s.reason == TERMINATION_REASON_OPTIMAL;
+    }
+    return optimal;
+}
+
+void BatchSolver::set_starts(const double *x, const double *y)
+{   // unscaled starts -> scaled space: x * D * b_scale, y * E * c_scale
+    DBuf<double> hx, hy;
+    if (x) {
+        hx.alloc(n_);
+        hx.upload(x, n_, stream_);
+    }
+    if (y) {
+        hy.alloc(m_);
+        hy.upload(y, m_, stream_);
+    }
+    st_.upload(init_.data(), N_, stream_);
+    if (x)
+        scale_start_kernel<<<blocks_for_l(n_), kBlock, 0, stream_>>>(col_inst_.get(), hx.get(), D_.get(), xs_.get(),
+                                                                    n_, row_inst_.get(), nullptr, E_.get(), nullptr,
+                                                                    0, st_.get());
+    if (y)
+        scale_start_kernel<<<blocks_for_l(m_), kBlock, 0, stream_>>>(col_inst_.get(), nullptr, D_.get(), nullptr, 0,
+                                                                    row_inst_.get(), hy.get(), E_.get(), ys_.get(),
+                                                                    m_, st_.get());
+    PDAL_LAUNCHED();
+    PDAL_HIP(hipStreamSynchronize(stream_));
+}
+
+cp_al_result_t *BatchSolver::result(int i)
+{   // create_result_from_state (solver.cu:3828-3909) for instance i
+    if (i < 0 || i >= N_ || final_.empty()) return nullptr;
+    const long r0 = row_off_[i], c0 = col_off_[i];
+    const int mi = (int)(row_off_[i + 1] - r0), ni = (int)(col_off_[i + 1] - c0);
+    // results of all instances are computed on the first request
+    if (!res_ready_) {
+        rc_all_.alloc(n_);
+        xo_all_.alloc(n_);
+        yo_all_.alloc(m_);
+        const BlockTable btat{tat_.inst.get(), tat_.row0.get(), row_end_at_.get()};
+        dispatch_group(tat_.G, [&](auto g) {
+            constexpr int G = decltype(g)::value;
+            result_x_kernel<G><<<tat_.blocks, kBlock, 0, stream_>>>(At_.view(), btat, yT_.get(), xT_.get(), c_.get(),
+                                                                   D_.get(), lb_.get(), ub_.get(), st_.get(),
+                                                                   rc_all_.get(), xo_all_.get());
+        });
+        result_y_kernel<<<blocks_for_l(m_), kBlock, 0, stream_>>>(row_inst_.get(), yT_.get(), E_.get(), st_.get(),
+                                                                 yo_all_.get(), m_);
+        PDAL_LAUNCHED();
+        res_ready_ = true;
+    }
+    cp_al_result_t *r = (cp_al_result_t *)calloc(1, sizeof(cp_al_result_t));
+    r->primal_solution = (double *)malloc((size_t)std::max(ni, 1) * sizeof(double));
+    r->dual_solution = (double *)malloc((size_t)std::max(mi, 1) * sizeof(double));
+    r->reduced_cost = (double *)malloc((size_t)std::max(ni, 1) * sizeof(double));
+    if (ni) {
+        PDAL_HIP(hipMemcpyAsync(r->primal_solution, xo_all_.get() + c0, (size_t)ni * sizeof(double),
+                                hipMemcpyDeviceToHost, stream_));
+        PDAL_HIP(hipMemcpyAsync(r->reduced_cost, rc_all_.get() + c0, (size_t)ni * sizeof(double),
+                                hipMemcpyDeviceToHost, stream_));
+    }
+    if (mi)
+        PDAL_HIP(hipMemcpyAsync(r->dual_solution, yo_all_.get() + r0, (size_t)mi * sizeof(double),
+                                hipMemcpyDeviceToHost, stream_));
+    PDAL_HIP(hipStreamSynchronize(stream_));
+    const InstState &s = final_[i];
+    const lp_problem_t *w = probs_[i];
+    r->num_variables = w->num_variables;
+    r->num_constraints = w->num_constraints;
+    r->num_nonzeros = w->constraint_matrix_num_nonzeros;
+    r->total_count = s.charged;
+    r->rescaling_time_sec = 0.0;
+    r->cumulative_time_sec = solve_s_;
+    r->absolute_primal_residual = s.abs_p;
+    r->relative_primal_residual = s.rel_p;
+    r->absolute_dual_residual = s.abs_d;
+    r->relative_dual_residual = s.rel_d;
+    r->primal_objective_value = s.p_obj;
+    r->dual_objective_value = s.d_obj;
+    r->objective_gap = s.gap;
+    r->relative_objective_gap = s.rel_gap;
+    r->termination_reason = (termination_reason_t)s.reason;
+    return r;
+}
+
+void BatchSolver::stats(double *out) const
+{   // {set-up s, last solve s, graph replays, instances, optimal, total iterations, lanes A, lanes At}
+    int optimal = 0;
+    double iters = 0.0;
+    for (const InstState &s : final_) {
+        optimal += s.reason == TERMINATION_REASON_OPTIMAL;
+        iters += s.charged;
+    }
+    out[0] = setup_s_;
+    out[1] = solve_s_;
+    out[2] = (double)replays_;
+    out[3] = N_;
+    out[4] = optimal;
+    out[5] = iters;
+    out[6] = ta_.G;
+    out[7] = tat_.G;
+}
+
+}  // namespace batch
+
+// ---------------------------------------------------------------------------
+
+using batch::BatchSolver;
+
+void *batch_create(lp_problem_t *const *probs, int count, const cp_al_parameters_t *params)
+{
+    auto *b = new BatchSolver(probs, count, *params);
+    try {
+        b->setup();
+    } catch (...) {
+        delete b;
+        throw;
+    }
+    return b;
+}
+
+int batch_solve(void *h) { return static_cast<BatchSolver *>(h)->solve(); }
+void batch_set_starts(void *h, const double *x, const double *y) { static_cast<BatchSolver *>(h)->set_starts(x, y); }
+cp_al_result_t *batch_result(void *h, int i) { return static_cast<BatchSolver *>(h)->result(i); }
+void batch_stats(void *h, double *out) { static_cast<BatchSolver *>(h)->stats(out); }
+void batch_free(void *h) { delete static_cast<BatchSolver *>(h); }
+
+}  // namespace lp
+}  // namespace pdal

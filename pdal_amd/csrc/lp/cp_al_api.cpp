@@ -6,6 +6,7 @@
 // (dense / CSC / COO -> CSR conversions).
 #include "cp_al.h"
 #include "cp_al_internal.hpp"
+#include "pdal_batch.h"
 #include "pdal_lp.h"
 
 #include <algorithm>
@@ -477,4 +478,74 @@ PDAL_API double pdal_cp_al_session_setup_seconds(void *session)
 PDAL_API void pdal_cp_al_session_free(void *session)
 {
     if (session) session_free(session);
+}
+
+// ---------------------------------------------------------------------------
+// Batched solver (pdal_batch.h)
+
+PDAL_API pdal_cp_al_batch_t *pdal_cp_al_batch_create(lp_problem_t *const *problems, int count,
+                                                     const cp_al_parameters_t *params)
+{
+    if (!problems || count <= 0) return nullptr;
+    for (int i = 0; i < count; ++i)
+        if (!problems[i]) return nullptr;
+    cp_al_parameters_t local;
+    if (params) local = *params;
+    else set_default_parameters(&local);
+    if (local.termination_evaluation_frequency < 3) {
+        fprintf(stderr, "Error: termination_evaluation_frequency must be >= 3 (got %d).\n",
+                local.termination_evaluation_frequency);
+        return nullptr;
+    }
+    try {
+        return static_cast<pdal_cp_al_batch_t *>(batch_create(problems, count, &local));
+    } catch (const std::exception &e) {
+        fprintf(stderr, "[cp_al-mi355x] batch set-up failed: %s\n", e.what());
+        return nullptr;
+    }
+}
+
+PDAL_API int pdal_cp_al_batch_set_starts(pdal_cp_al_batch_t *batch, const double *x, const double *y)
+{
+    if (!batch) return -1;
+    try {
+        batch_set_starts(batch, x, y);
+        return 0;
+    } catch (const std::exception &e) {
+        fprintf(stderr, "[cp_al-mi355x] batch start update failed: %s\n", e.what());
+        return -2;
+    }
+}
+
+PDAL_API int pdal_cp_al_batch_solve(pdal_cp_al_batch_t *batch)
+{
+    if (!batch) return -1;
+    try {
+        return batch_solve(batch);
+    } catch (const std::exception &e) {
+        fprintf(stderr, "[cp_al-mi355x] batch solve failed: %s\n", e.what());
+        return -1;
+    }
+}
+
+PDAL_API cp_al_result_t *pdal_cp_al_batch_result(pdal_cp_al_batch_t *batch, int i)
+{
+    if (!batch) return nullptr;
+    try {
+        return batch_result(batch, i);
+    } catch (const std::exception &e) {
+        fprintf(stderr, "[cp_al-mi355x] batch result failed: %s\n", e.what());
+        return nullptr;
+    }
+}
+
+PDAL_API void pdal_cp_al_batch_stats(const pdal_cp_al_batch_t *batch, double *out)
+{
+    if (!batch || !out) return;
+    batch_stats(const_cast<pdal_cp_al_batch_t *>(batch), out);
+}
+
+PDAL_API void pdal_cp_al_batch_free(pdal_cp_al_batch_t *batch)
+{
+    if (batch) batch_free(batch);
 }

@@ -30,5 +30,13 @@ int session_time_kernels(void *session, int iters, double *out);
 double session_setup_seconds(void *session);
 void session_free(void *session);
 
+// Batched solver (pdal_batch.h).
+void *batch_create(lp_problem_t *const *problems, int count, const cp_al_parameters_t *params);
+int batch_solve(void *batch);
+void batch_set_starts(void *batch, const double *x, const double *y);
+cp_al_result_t *batch_result(void *batch, int i);
+void batch_stats(void *batch, double *out);
+void batch_free(void *batch);
+
 }  // namespace lp
 }  // namespace pdal

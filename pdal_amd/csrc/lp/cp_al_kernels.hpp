@@ -47,7 +47,7 @@ struct PrimalArgs {
 // update.  Reference: compute_next_primal_solution(_major)_kernel,
 // LP/CP_AL/src/solver.cu:2340-2397, launched after the At.y SpMV at :2561-2570.
 template <int G, bool MAJOR>
-__global__ void __launch_bounds__(kBlock) primal_kernel(PrimalArgs a, int k_offset)
+static __global__ void __launch_bounds__(kBlock) primal_kernel(PrimalArgs a, int k_offset)
 {
     const int tid = blockIdx.x * kBlock + threadIdx.x;
     const int row = tid / G;
@@ -91,7 +91,7 @@ struct DualArgs {
 // reflection and Halpern update.  Reference: compute_next_dual_solution(_major)
 // _kernel, solver.cu:2399-2457, after the A.x_bar SpMV at :2617-2626.
 template <int G, bool MAJOR>
-__global__ void __launch_bounds__(kBlock) dual_kernel(DualArgs a, int k_offset)
+static __global__ void __launch_bounds__(kBlock) dual_kernel(DualArgs a, int k_offset)
 {
     const int tid = blockIdx.x * kBlock + threadIdx.x;
     const int row = tid / G;
@@ -129,7 +129,7 @@ struct PredictorArgs {
 // A.x SpMV fused with the box-AL dual predictor (only launched when beta > 0).
 // Reference: compute_box_al_predictor_kernel, solver.cu:2459-2479, after :2533-2542.
 template <int G>
-__global__ void __launch_bounds__(kBlock) predictor_kernel(PredictorArgs a)
+static __global__ void __launch_bounds__(kBlock) predictor_kernel(PredictorArgs a)
 {
     const int tid = blockIdx.x * kBlock + threadIdx.x;
     const int row = tid / G;
@@ -173,7 +173,7 @@ struct EvalAtArgs {
 
 // Rows of At: (At dy)_j with dy = yR - yT and (At yT)_j in one pass.
 template <int G>
-__global__ void __launch_bounds__(kBlock) eval_at_kernel(EvalAtArgs a)
+static __global__ void __launch_bounds__(kBlock) eval_at_kernel(EvalAtArgs a)
 {
     constexpr int RPB = kBlock / G;
     double q[QAT_N] = {0, 0, 0, 0, 0, 0, 0};
@@ -216,7 +216,7 @@ struct EvalAArgs {
 
 // Rows of A: (A dx)_i with dx = xR - xT and (A xT)_i in one pass.
 template <int G>
-__global__ void __launch_bounds__(kBlock) eval_a_kernel(EvalAArgs a)
+static __global__ void __launch_bounds__(kBlock) eval_a_kernel(EvalAArgs a)
 {
     constexpr int RPB = kBlock / G;
     double q[QA_N] = {0, 0, 0, 0, 0, 0};
@@ -248,7 +248,7 @@ __global__ void __launch_bounds__(kBlock) eval_a_kernel(EvalAArgs a)
 }
 
 // Restart to the T-output: anchor = current = T (solver.cu:3056-3073).
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     restart_copy_kernel(const double *__restrict__ xT, double *__restrict__ x0, double *__restrict__ x, int n,
                         const double *__restrict__ yT, double *__restrict__ y0, double *__restrict__ y, int m)
 {
@@ -283,7 +283,7 @@ __device__ __forceinline__ bool power_stopped(const PowerState *ps, int max_iter
     return ps->done != 0 || ps->iter >= max_iter;
 }
 
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     power_normalize_kernel(const double *__restrict__ v, double *__restrict__ vh, int m,
                            const PowerState *__restrict__ ps, int max_iter)
 {
@@ -294,7 +294,7 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 template <int G>
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     spmv_gated_kernel(CsrView M, const double *__restrict__ x, double *__restrict__ y,
                       const PowerState *__restrict__ ps, int max_iter)
 {
@@ -309,7 +309,7 @@ __global__ void __launch_bounds__(kBlock)
 
 // v = A w, partial <v_hat, v> per block (grid-stride, fixed grid).
 template <int G>
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     power_av_dot_kernel(CsrView A, const double *__restrict__ w, const double *__restrict__ vh,
                         double *__restrict__ v, double *__restrict__ parts, const PowerState *__restrict__ ps,
                         int max_iter)
@@ -332,7 +332,7 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // r = -sigma_sq * v_hat + v; partials of ||r||^2 and ||v||^2.
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     power_resid_kernel(const double *__restrict__ v, const double *__restrict__ vh, int m,
                        double *__restrict__ parts, const PowerState *__restrict__ ps, int max_iter)
 {
@@ -349,7 +349,7 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // Single block: finish sigma_sq.
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     power_finish_dot_kernel(const double *__restrict__ parts, int nparts, PowerState *__restrict__ ps,
                             int max_iter)
 {
@@ -368,7 +368,7 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // Single block: finish ||r|| and ||v||, test convergence, count the iteration.
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     power_finish_resid_kernel(const double *__restrict__ parts, int nparts, PowerState *__restrict__ ps,
                               int max_iter, double tol)
 {
@@ -399,7 +399,7 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     sumsq_partials_kernel(const double *__restrict__ v, int len, double *__restrict__ parts)
 {
     double q[1] = {0.0};
@@ -410,7 +410,7 @@ __global__ void __launch_bounds__(kBlock)
 // ---------------------------------------------------------------------------
 // Rescaling (LP/CP_AL/src/preconditioner.cu:477-736), one thread per row.
 
-__global__ void __launch_bounds__(kBlock) row_absmax_kernel(CsrView M, double *__restrict__ out)
+static __global__ void __launch_bounds__(kBlock) row_absmax_kernel(CsrView M, double *__restrict__ out)
 {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= M.rows) return;
@@ -431,7 +431,7 @@ __device__ __forceinline__ double pow_fast(double v, double p)
     return pow(v, p);
 }
 
-__global__ void __launch_bounds__(kBlock) row_powsum_kernel(CsrView M, double degree, double *__restrict__ out)
+static __global__ void __launch_bounds__(kBlock) row_powsum_kernel(CsrView M, double degree, double *__restrict__ out)
 {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= M.rows) return;
@@ -444,7 +444,7 @@ __global__ void __launch_bounds__(kBlock) row_powsum_kernel(CsrView M, double de
     out[i] = acc;
 }
 
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     clamp_sqrt_accum_kernel(double *__restrict__ sf, double *__restrict__ inv, double *__restrict__ cum, int len)
 {
     const int t = blockIdx.x * kBlock + threadIdx.x;
@@ -456,7 +456,7 @@ __global__ void __launch_bounds__(kBlock)
     inv[t] = 1.0 / s;
 }
 
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     cr_log_update_kernel(CsrView M, const double *__restrict__ other, double *__restrict__ log_scale)
 {
     const int r = blockIdx.x * kBlock + threadIdx.x;
@@ -475,7 +475,7 @@ __global__ void __launch_bounds__(kBlock)
     log_scale[r] = acc / (double)(e - b);
 }
 
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     cr_exp_kernel(double *__restrict__ ls, double *__restrict__ inv, double *__restrict__ cum, int len)
 {
     const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -489,7 +489,7 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // c *= 1/d, bounds *= d, x0 *= d  (scale_variables_kernel)
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     scale_vars_kernel(double *__restrict__ c, double *__restrict__ lb, double *__restrict__ ub,
                       double *__restrict__ x0, const double *__restrict__ d, const double *__restrict__ inv_d, int n)
 {
@@ -503,7 +503,7 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // bounds *= 1/e, y0 *= e  (scale_constraints_kernel)
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     scale_cons_kernel(double *__restrict__ l, double *__restrict__ u, double *__restrict__ y0,
                       const double *__restrict__ e, const double *__restrict__ inv_e, int m)
 {
@@ -518,7 +518,7 @@ __global__ void __launch_bounds__(kBlock)
 // v[k] *= inv_row[row] * inv_col[col]: the product of the two inverse factors is
 // commutative, so A and At receive bit-identical scale factors without a
 // transpose map (scale_csr_nnz_kernel, preconditioner.cu:513-531).
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     scale_rows_kernel(const int *__restrict__ rp, const int *__restrict__ ci, double *__restrict__ v, int rows,
                       const double *__restrict__ inv_row, const double *__restrict__ inv_col)
 {
@@ -528,7 +528,7 @@ __global__ void __launch_bounds__(kBlock)
     for (int k = rp[r]; k < rp[r + 1]; ++k) v[k] *= inv_col[ci[k]] * ir;
 }
 
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     bound_contrib_partials_kernel(const double *__restrict__ l, const double *__restrict__ u, int m,
                                   const double *__restrict__ c, int n, double *__restrict__ parts)
 {   // compute_bound_contrib_kernel (:699-729) and ||c||^2 in one pass
@@ -545,7 +545,7 @@ __global__ void __launch_bounds__(kBlock)
     block_reduce_store<2, 0>(q, parts + (size_t)blockIdx.x * 2);
 }
 
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     bound_obj_scale_kernel(double *__restrict__ l, double *__restrict__ u, double *__restrict__ y0, int m,
                            double *__restrict__ lb, double *__restrict__ ub, double *__restrict__ c,
                            double *__restrict__ x0, int n, double cs, double os)
@@ -564,7 +564,7 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
-__global__ void __launch_bounds__(kBlock) fill_kernel(double *__restrict__ x, double v, int len)
+static __global__ void __launch_bounds__(kBlock) fill_kernel(double *__restrict__ x, double v, int len)
 {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i < len) x[i] = v;
@@ -573,7 +573,7 @@ __global__ void __launch_bounds__(kBlock) fill_kernel(double *__restrict__ x, do
 // Result: reduced cost with infinite-bound clamps, then unscale x and y
 // (compute_and_rescale_reduced_cost_kernel :1768-1793, rescale_solution_kernel :2481-2500).
 template <int G>
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     reduced_cost_kernel(CsrView At, const double *__restrict__ yT, const double *__restrict__ c,
                         const double *__restrict__ D, const double *__restrict__ lb, const double *__restrict__ ub,
                         double c_scale, double *__restrict__ rc)
@@ -590,7 +590,7 @@ __global__ void __launch_bounds__(kBlock)
     rc[row] = r;
 }
 
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     unscale_kernel(const double *__restrict__ xT, const double *__restrict__ D, double b_scale,
                    double *__restrict__ xo, int n, const double *__restrict__ yT, const double *__restrict__ E,
                    double c_scale, double *__restrict__ yo, int m)

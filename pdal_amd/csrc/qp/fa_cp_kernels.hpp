@@ -49,7 +49,7 @@ struct SimplePrimalArgs {
 };
 
 template <int G, bool MAJOR, bool DIAG, bool HALPERN>
-__global__ void __launch_bounds__(kBlock) simple_primal_kernel(SimplePrimalArgs a)
+static __global__ void __launch_bounds__(kBlock) simple_primal_kernel(SimplePrimalArgs a)
 {
     const int tid = blockIdx.x * kBlock + threadIdx.x;
     const int row = tid / G;
@@ -88,7 +88,7 @@ struct DualArgs {
 };
 
 template <int G, bool MAJOR, bool AL, bool HALPERN>
-__global__ void __launch_bounds__(kBlock) dual_kernel(DualArgs a)
+static __global__ void __launch_bounds__(kBlock) dual_kernel(DualArgs a)
 {
     const int tid = blockIdx.x * kBlock + threadIdx.x;
     const int row = tid / G;
@@ -126,7 +126,7 @@ __global__ void __launch_bounds__(kBlock) dual_kernel(DualArgs a)
 }
 
 // Halpern update for iterations with a fixed-point evaluation (:225-242).
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     halpern_kernel(double *__restrict__ x, const double *__restrict__ xR, const double *__restrict__ x0, int n,
                    double *__restrict__ y, const double *__restrict__ yR, const double *__restrict__ y0, int m,
                    StepScalars sc)
@@ -158,7 +158,7 @@ __device__ __forceinline__ double qrow(const CsrView &Qm, const CsrView &Rt, int
 }
 
 template <int G, int MODE>
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     qx_kernel(CsrView Qm, CsrView Rt, const double *__restrict__ xv, const double *__restrict__ rx,
               double *__restrict__ qx, int n, const BBState *__restrict__ bb)
 {
@@ -173,7 +173,7 @@ __global__ void __launch_bounds__(kBlock)
 
 // y = M x gated by the BB loop flag (R.x, A.x, At.u inside the inner loop).
 template <int G>
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     gated_spmv_kernel(CsrView M, const double *__restrict__ x, double *__restrict__ y,
                       const BBState *__restrict__ bb)
 {
@@ -188,7 +188,7 @@ __global__ void __launch_bounds__(kBlock)
 
 // AL effective dual u = y - s(Ax - proj(Ax - y/s)) from A.x (:181-192, :275-303).
 template <int G>
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     al_effective_dual_kernel(CsrView A, const double *__restrict__ xv, const double *__restrict__ y,
                              const double *__restrict__ l, const double *__restrict__ u,
                              double *__restrict__ pp_out, double *__restrict__ uo,
@@ -224,7 +224,7 @@ struct BBArgs {
 };
 
 // bb_init (:381-399): gradient at x, first projected step with alpha = 1/(1/tau).
-__global__ void __launch_bounds__(kBlock) bb_init_kernel(BBArgs a)
+static __global__ void __launch_bounds__(kBlock) bb_init_kernel(BBArgs a)
 {
     const double alpha = 1.0 / (1.0 / a.sc.tau);
     double q[1] = {0.0};
@@ -245,7 +245,7 @@ __global__ void __launch_bounds__(kBlock) bb_init_kernel(BBArgs a)
 // Gradient refresh (:401-415) fused with the product row: cur = Q.xT + c - dp +
 // (xT - x)/tau; grad = cur; partial <dir, cur - grad_old>.
 template <int G, int MODE>
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     bb_grad_kernel(BBArgs a, CsrView Qm, CsrView Rt, const double *__restrict__ rx, const double *__restrict__ qdiag)
 {
     if (a.bb->done) return;
@@ -273,7 +273,7 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // alpha = ||d||^2 / <d, dg> (single block, :412-413).
-__global__ void __launch_bounds__(kBlock) bb_alpha_kernel(const double *__restrict__ parts, int nparts, BBState *bb)
+static __global__ void __launch_bounds__(kBlock) bb_alpha_kernel(const double *__restrict__ parts, int nparts, BBState *bb)
 {
     if (bb->done) return;
     __shared__ double red[kWavesPerBlock];
@@ -290,7 +290,7 @@ __global__ void __launch_bounds__(kBlock) bb_alpha_kernel(const double *__restri
 }
 
 // Direction update (:417-430) with partial ||d||^2.
-__global__ void __launch_bounds__(kBlock) bb_dir_kernel(BBArgs a)
+static __global__ void __launch_bounds__(kBlock) bb_dir_kernel(BBArgs a)
 {
     if (a.bb->done) return;
     const double alpha = a.bb->alpha;
@@ -309,7 +309,7 @@ __global__ void __launch_bounds__(kBlock) bb_dir_kernel(BBArgs a)
 
 // ||d|| and the loop test.  FIRST: the norm after bb_init (it stays 1).
 template <bool FIRST>
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     bb_check_kernel(const double *__restrict__ parts, int nparts, BBState *bb, double tol, int limit)
 {
     if (!FIRST && bb->done) return;
@@ -336,7 +336,7 @@ __global__ void __launch_bounds__(kBlock)
 // Final reflection (:432-443) and the inner-count accumulation; optionally the
 // x Halpern step.
 template <bool HALPERN>
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     bb_final_kernel(double *__restrict__ x, const double *__restrict__ xT, double *__restrict__ xR,
                     const double *__restrict__ x0, int n, BBState *bb, StepScalars sc)
 {
@@ -363,7 +363,7 @@ constexpr int kEvalStride = 12;
 // residual (sum sq, max abs) and the dual-objective contribution.
 enum ResAQ { RA_PRES2, RA_CONTRIB, RA_PRESMAX, RA_N };
 template <int G>
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     res_a_kernel(CsrView A, const double *__restrict__ xT, const double *__restrict__ yT,
                  const double *__restrict__ l, const double *__restrict__ u, const double *__restrict__ E,
                  double *__restrict__ parts)
@@ -393,7 +393,7 @@ __global__ void __launch_bounds__(kBlock)
 // objective dots and the QP dual-residual denominators.
 enum ResAtQ { RT_DRES2, RT_CX, RT_SX, RT_XQX, RT_QXD2, RT_DPD2, RT_DRESMAX, RT_QXDMAX, RT_DPDMAX, RT_N };
 template <int G, bool QP>
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     res_at_kernel(CsrView At, const double *__restrict__ yT, const double *__restrict__ xT,
                   const double *__restrict__ qx, const double *__restrict__ c, const double *__restrict__ lb,
                   const double *__restrict__ ub, const double *__restrict__ D, double *__restrict__ dslack,
@@ -441,7 +441,7 @@ __global__ void __launch_bounds__(kBlock)
 // infeasibility test), At.dy cross term and the two norms.
 enum FpQ { FP_CROSS, FP_PN2, FP_N };
 template <int G>
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     fp_at_kernel(CsrView At, const double *__restrict__ yR, const double *__restrict__ y,
                  const double *__restrict__ xR, const double *__restrict__ x, double *__restrict__ dx,
                  double *__restrict__ dual_prod, double *__restrict__ parts)
@@ -467,7 +467,7 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // dy = a - b with partial ||dy||^2 (fixed-point and restart displacements).
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     diff_norm_kernel(const double *__restrict__ a, const double *__restrict__ b, double *__restrict__ d, int len,
                      double *__restrict__ parts)
 {
@@ -482,7 +482,7 @@ __global__ void __launch_bounds__(kBlock)
 
 // ||M x||^2 partials (normal motion at restart).
 template <int G>
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     spmv_norm_kernel(CsrView M, const double *__restrict__ x, double *__restrict__ y, double *__restrict__ parts)
 {
     constexpr int RPB = kBlock / G;
@@ -502,7 +502,7 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // Restart copy: anchor = current = T-output (:998-1010).
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     restart_copy_kernel(const double *__restrict__ xT, double *__restrict__ x0, double *__restrict__ x, int n,
                         const double *__restrict__ yT, double *__restrict__ y0, double *__restrict__ y, int m)
 {
@@ -520,7 +520,7 @@ __global__ void __launch_bounds__(kBlock)
 // ---------------------------------------------------------------------------
 // Infeasibility information (utils.cu:1028-1132), run at most three times.
 // Step 1: project the rays by the bounds, partial max |.|.
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     ray_project_kernel(double *__restrict__ dx, const double *__restrict__ lb, const double *__restrict__ ub, int n,
                        double *__restrict__ dy, const double *__restrict__ l, const double *__restrict__ u, int m,
                        double *__restrict__ parts)
@@ -543,7 +543,7 @@ __global__ void __launch_bounds__(kBlock)
     block_reduce_store<0, 2>(q, parts + (size_t)blockIdx.x * kEvalStride);
 }
 
-__global__ void __launch_bounds__(kBlock) scale_kernel(double *__restrict__ v, double s, int len)
+static __global__ void __launch_bounds__(kBlock) scale_kernel(double *__restrict__ v, double s, int len)
 {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i < len) v[i] *= s;
@@ -554,7 +554,7 @@ __global__ void __launch_bounds__(kBlock) scale_kernel(double *__restrict__ v, d
 // reference.  Sums: c.dx, sum pslack_obj, sum dslack_obj; maxima: primal and
 // dual infeasibility.
 enum RayQ { RY_CDX, RY_PSUM, RY_DSUM, RY_PMAX, RY_DMAX, RY_N };
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     ray_terms_kernel(const double *__restrict__ c, const double *__restrict__ dx, const double *__restrict__ dp,
                      const double *__restrict__ lb, const double *__restrict__ ub, const double *__restrict__ D,
                      double *__restrict__ dslack, int n, const double *__restrict__ dy,
@@ -582,7 +582,7 @@ __global__ void __launch_bounds__(kBlock)
     block_reduce_store<3, 2>(q, parts + (size_t)blockIdx.x * kEvalStride);
 }
 
-__global__ void __launch_bounds__(kBlock) absmax_kernel(const double *__restrict__ v, int len, double *__restrict__ parts)
+static __global__ void __launch_bounds__(kBlock) absmax_kernel(const double *__restrict__ v, int len, double *__restrict__ parts)
 {
     double q[1] = {0.0};
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < len; i += gridDim.x * kBlock) q[0] = fmax(q[0], fabs(v[i]));
@@ -591,7 +591,7 @@ __global__ void __launch_bounds__(kBlock) absmax_kernel(const double *__restrict
 
 // ---------------------------------------------------------------------------
 // Power iterations (utils.cu:65-315).
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     dot_sumsq_kernel(const double *__restrict__ a, const double *__restrict__ b, int len, double *__restrict__ parts)
 {   // partials of <a,b> and ||b||^2
     double q[2] = {0.0, 0.0};
@@ -606,7 +606,7 @@ __global__ void __launch_bounds__(kBlock)
 // Device rescaling (QP/FA_CP/src/preconditioner.c:92-289 moved to the GPU).
 // Column maxima / power sums come from the transposed copies so every column
 // is reduced by one thread in the reference's row order.
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     ruiz_factors_kernel(CsrView A, CsrView At, CsrView Qt, double *__restrict__ e, double *__restrict__ d)
 {
     const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -643,7 +643,7 @@ __device__ __forceinline__ double abs_pow(double v, double p)
     return pow(v, p);
 }
 
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     pc_factors_kernel(CsrView A, CsrView At, CsrView Qt, double alpha, double *__restrict__ e, double *__restrict__ d)
 {
     const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -665,7 +665,7 @@ __global__ void __launch_bounds__(kBlock)
 // (the two factors of an entry multiply commutatively, so A and At, Q and Qt
 // receive identical bits), MODE 1 by col_f[col], MODE 2 by row_f[row].
 template <int MODE>
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     divide_rows_kernel(CsrView M, double *__restrict__ v, const double *__restrict__ row_f,
                        const double *__restrict__ col_f)
 {
@@ -678,7 +678,7 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     scale_vectors_kernel(double *__restrict__ c, double *__restrict__ lb, double *__restrict__ ub,
                          const double *__restrict__ d, double *__restrict__ cumD, int n, double *__restrict__ l,
                          double *__restrict__ u, const double *__restrict__ e, double *__restrict__ cumE, int m)
@@ -697,7 +697,7 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     bound_obj_sums_kernel(const double *__restrict__ l, const double *__restrict__ u, int m,
                           const double *__restrict__ c, int n, double *__restrict__ parts)
 {
@@ -711,7 +711,7 @@ __global__ void __launch_bounds__(kBlock)
     block_reduce_store<2, 0>(q, parts + (size_t)blockIdx.x * kEvalStride);
 }
 
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     bound_obj_apply_kernel(double *__restrict__ l, double *__restrict__ u, int m, double *__restrict__ lb,
                            double *__restrict__ ub, double *__restrict__ c, int n, double bs, double cs)
 {
@@ -727,20 +727,20 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
-__global__ void __launch_bounds__(kBlock) fill_kernel(double *__restrict__ v, double val, int len)
+static __global__ void __launch_bounds__(kBlock) fill_kernel(double *__restrict__ v, double val, int len)
 {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i < len) v[i] = val;
 }
 
-__global__ void __launch_bounds__(kBlock) mul_all_kernel(double *__restrict__ v, long len, double f)
+static __global__ void __launch_bounds__(kBlock) mul_all_kernel(double *__restrict__ v, long len, double f)
 {
     const long i = (long)blockIdx.x * kBlock + threadIdx.x;
     if (i < len) v[i] *= f;
 }
 
 // Diagonal of a diagonal-only Q plus 1e-12 (solver_state.cu:462-492).
-__global__ void __launch_bounds__(kBlock) qdiag_kernel(CsrView Qm, int n, double *__restrict__ qdiag)
+static __global__ void __launch_bounds__(kBlock) qdiag_kernel(CsrView Qm, int n, double *__restrict__ qdiag)
 {
     const int r = blockIdx.x * kBlock + threadIdx.x;
     if (r >= Qm.rows) return;
@@ -748,7 +748,7 @@ __global__ void __launch_bounds__(kBlock) qdiag_kernel(CsrView Qm, int n, double
         if (Qm.ci[k] < n) qdiag[Qm.ci[k]] = Qm.v[k] + 1e-12;
 }
 
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     diag_mul_kernel(const double *__restrict__ d, const double *__restrict__ x, double *__restrict__ y, int len)
 {
     const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -757,7 +757,7 @@ __global__ void __launch_bounds__(kBlock)
 
 // Shifted power step of the minimum-eigenvalue estimate (utils.cu:240-315):
 // sh = lmax*v - Qv, partial <v, sh>.
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     shift_dot_kernel(const double *__restrict__ v, const double *__restrict__ av, double lmax,
                      double *__restrict__ sh, int len, double *__restrict__ parts)
 {
@@ -772,7 +772,7 @@ __global__ void __launch_bounds__(kBlock)
 
 // Power-iteration normalisation with the norm read from the device:
 // dst = src * (1/sqrt(*sumsq)).
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     normalize_dev_kernel(const double *__restrict__ src, double *__restrict__ dst, int len,
                          const double *__restrict__ sumsq)
 {
@@ -782,7 +782,7 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // r = -(*sig)*a + b in place into a, partial ||r||^2 (utils.cu:65-170).
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     sv_resid_kernel(double *__restrict__ a, const double *__restrict__ b, int len, const double *__restrict__ sig,
                     double *__restrict__ parts)
 {
@@ -796,7 +796,7 @@ __global__ void __launch_bounds__(kBlock)
     block_reduce_store<1, 0>(q, parts + (size_t)blockIdx.x * kEvalStride);
 }
 
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     start_scale_kernel(double *__restrict__ x, const double *__restrict__ D, double bs, int n, double *__restrict__ y,
                        const double *__restrict__ E, double cs, int m)
 {   // warm start into the scaled space (solver_state.cu:630-657)
@@ -805,7 +805,7 @@ __global__ void __launch_bounds__(kBlock)
     else if (i < n + m) y[i - n] = y[i - n] * E[i - n] * cs;
 }
 
-__global__ void __launch_bounds__(kBlock)
+static __global__ void __launch_bounds__(kBlock)
     result_kernel(const double *__restrict__ xT, const double *__restrict__ D, double bs, double *__restrict__ xo,
                   const double *__restrict__ c, const double *__restrict__ dp, double cs, double *__restrict__ rc,
                   int n, const double *__restrict__ yT, const double *__restrict__ E, double *__restrict__ yo, int m)

@@ -226,6 +226,20 @@ class CpAl:
             L.pdal_cp_al_session_setup_seconds.argtypes = [C.c_void_p]
             L.pdal_cp_al_session_free.restype = None
             L.pdal_cp_al_session_free.argtypes = [C.c_void_p]
+        if hasattr(L, "pdal_cp_al_batch_create"):
+            L.pdal_cp_al_batch_create.restype = C.c_void_p
+            L.pdal_cp_al_batch_create.argtypes = [C.POINTER(C.POINTER(LpProblem)), C.c_int,
+                                                  C.POINTER(CpAlParameters)]
+            L.pdal_cp_al_batch_set_starts.restype = C.c_int
+            L.pdal_cp_al_batch_set_starts.argtypes = [C.c_void_p, _dp, _dp]
+            L.pdal_cp_al_batch_solve.restype = C.c_int
+            L.pdal_cp_al_batch_solve.argtypes = [C.c_void_p]
+            L.pdal_cp_al_batch_result.restype = C.POINTER(CpAlResult)
+            L.pdal_cp_al_batch_result.argtypes = [C.c_void_p, C.c_int]
+            L.pdal_cp_al_batch_stats.restype = None
+            L.pdal_cp_al_batch_stats.argtypes = [C.c_void_p, _dp]
+            L.pdal_cp_al_batch_free.restype = None
+            L.pdal_cp_al_batch_free.argtypes = [C.c_void_p]
 
     # ------------------------------------------------------------------
     def default_params(self, **overrides) -> CpAlParameters:
@@ -387,6 +401,65 @@ class LpSession:
             self.lib.lib.pdal_cp_al_session_free(self.h)
             self.h = None
             self.lib.free_problem(self.prob)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class LpBatch:
+    """Batched device-resident solve of many LPs (pdal_batch.h).  `instances`
+    is a list of keyword dicts accepted by CpAl.make_problem."""
+
+    def __init__(self, lib: CpAl, instances: list, params: Optional[CpAlParameters] = None):
+        if not hasattr(lib.lib, "pdal_cp_al_batch_create"):
+            raise AttributeError(f"{lib.path.name} has no batch entry points")
+        self.lib = lib
+        self.probs = [lib.make_problem(**kw) for kw in instances]
+        arr = (C.POINTER(LpProblem) * len(self.probs))(*self.probs)
+        self.params = params if params is not None else lib.default_params()
+        self.h = lib.lib.pdal_cp_al_batch_create(arr, len(self.probs), C.byref(self.params))
+        if not self.h:
+            self._free_probs()
+            raise RuntimeError("pdal_cp_al_batch_create failed")
+        self.sizes = [(p.contents.num_constraints, p.contents.num_variables) for p in self.probs]
+
+    def _free_probs(self) -> None:
+        for p in self.probs:
+            self.lib.free_problem(p)
+        self.probs = []
+
+    def set_starts(self, x=None, y=None) -> None:
+        xs, ys = _f64(x), _f64(y)
+        rc = self.lib.lib.pdal_cp_al_batch_set_starts(self.h, _ptr(xs, C.c_double), _ptr(ys, C.c_double))
+        if rc != 0:
+            raise RuntimeError(f"pdal_cp_al_batch_set_starts failed ({rc})")
+
+    def solve(self) -> int:
+        n_opt = self.lib.lib.pdal_cp_al_batch_solve(self.h)
+        if n_opt < 0:
+            raise RuntimeError("pdal_cp_al_batch_solve failed")
+        return n_opt
+
+    def result(self, i: int) -> LpResult:
+        res = self.lib.lib.pdal_cp_al_batch_result(self.h, i)
+        if not res:
+            raise IndexError(i)
+        return self.lib._take_result(res, self.probs[i])
+
+    def stats(self) -> dict:
+        out = (C.c_double * 8)()
+        self.lib.lib.pdal_cp_al_batch_stats(self.h, out)
+        return dict(setup_s=out[0], solve_s=out[1], replays=int(out[2]), instances=int(out[3]),
+                    optimal=int(out[4]), iterations=int(out[5]), lanes_a=int(out[6]), lanes_at=int(out[7]))
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.lib.pdal_cp_al_batch_free(self.h)
+            self.h = None
+        self._free_probs()
 
     def __enter__(self):
         return self

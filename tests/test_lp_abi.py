@@ -32,7 +32,7 @@ def _exported(lib: Path) -> set[str]:
     return {ln.split()[-1] for ln in out.stdout.splitlines() if " T " in ln}
 
 
-@pytest.mark.parametrize("header", ["cp_al.h", "mps_parser.h", "pdal_lp.h"])
+@pytest.mark.parametrize("header", ["cp_al.h", "mps_parser.h", "pdal_lp.h", "pdal_batch.h"])
 def test_library_exports_every_declared_symbol(gpu_lp, header):
     declared = _declared_functions(INCLUDE / header)
     assert declared, header
