@@ -46,10 +46,21 @@ int pdal_cp_al_batch_solve(pdal_cp_al_batch_t *batch);
  * NULL before the first solve or for an invalid index. */
 cp_al_result_t *pdal_cp_al_batch_result(pdal_cp_al_batch_t *batch, int i);
 
-/* out[0..7] = {set-up seconds, last solve seconds (wall clock, device
+/* out[0..8] = {set-up seconds, last solve seconds (wall clock, device
  * resident), hipGraph replays, instances, OPTIMAL instances, sum of
- * iteration counts, lanes per row of A, lanes per row of At}. */
+ * iteration counts, lanes per row of A, lanes per row of At, block-table
+ * compactions (stopped instances are dropped from the launch tables and the
+ * graph re-captured whenever the active count halves)}. */
 void pdal_cp_al_batch_stats(const pdal_cp_al_batch_t *batch, double *out);
+
+/* Launch `iters` plain iterations over all instances kernel by kernel
+ * (instances reset to their starts first) and time the two fused SpMV kernels
+ * with hipEvents on the batch's stream, the first iteration excluded.
+ * out[0..7] = {avg ms primal kernel (At.y over all instances), avg ms dual
+ * kernel (A.x_bar), lanes per row of At, lanes per row of A, total nonzeros,
+ * total columns, total rows, workgroups of the primal kernel}.  Returns 0 on
+ * success.  A following solve starts from the starts again. */
+int pdal_cp_al_batch_time_kernels(pdal_cp_al_batch_t *batch, int iters, double *out);
 
 void pdal_cp_al_batch_free(pdal_cp_al_batch_t *batch);
 

@@ -43,6 +43,14 @@ void pdal_cp_al_session_free(void *session);
 /* reference: LP/CP_AL/src/utils.cu:230-253 (exported there through utils.h) */
 const char *termination_reason_to_string(termination_reason_t reason);
 
+/* Counter calibration: copy `bytes` bytes (rounded down to whole doubles)
+ * between two fresh device buffers with a plain grid-stride copy kernel,
+ * `reps` times; returns the average kernel time in ms (hipEvents), or a
+ * negative value on error.  The FETCH_SIZE / WRITE_SIZE counters of these
+ * launches read `bytes` each when the counters are exact, which is how the
+ * HBM traffic of the solver kernels is scaled. */
+double pdal_copy_calibration(long long bytes, int reps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -62,10 +62,13 @@ class BatchSolver {
     int solve();
     cp_al_result_t *result(int i);
     void stats(double *out) const;
+    void time_kernels(int iters, double *out);
     int count() const { return N_; }
 
   private:
-    void build_table(Table &t, const std::vector<long> &off, int G);
+    void build_table(Table &t, const std::vector<long> &off, int G, const std::vector<char> *keep = nullptr);
+    int compact();
+    void restore_full_tables();
     void rescale();
     void power_iteration();
     void autotune();
@@ -101,6 +104,8 @@ class BatchSolver {
     hipEvent_t ev_[2] = {nullptr, nullptr};
     bool res_ready_ = false;
     DBuf<double> rc_all_, xo_all_, yo_all_;
+    bool full_tables_ = true, compact_ = true, use_graph_ = true;
+    int table_instances_ = 0, compactions_ = 0;
 
     void copy_d(double *dst, const double *src, long len)
     {
@@ -122,14 +127,15 @@ BatchSolver::~BatchSolver()
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
-void BatchSolver::build_table(Table &t, const std::vector<long> &off, int G)
-{
-    const int rpb = kBlock / G;
+void BatchSolver::build_table(Table &t, const std::vector<long> &off, int G, const std::vector<char> *keep)
+{   // workgroups of the instances in `keep` (all when null), in instance order
+    const int rpb = (kBlock / G) * kGroupRows;
     std::vector<int> inst, row0;
     t.first.assign(N_, 0);
     t.count.assign(N_, 0);
     for (int i = 0; i < N_; ++i) {
         t.first[i] = (int)inst.size();
+        if (keep && !(*keep)[i]) continue;
         for (long r = off[i]; r < off[i + 1]; r += rpb) {
             inst.push_back(i);
             row0.push_back((int)r);
@@ -138,12 +144,16 @@ void BatchSolver::build_table(Table &t, const std::vector<long> &off, int G)
     }
     t.blocks = (int)inst.size();
     t.G = G;
-    t.inst.alloc(std::max<size_t>(inst.size(), 1));
-    t.row0.alloc(std::max<size_t>(row0.size(), 1));
+    if (t.inst.size() < std::max<size_t>(inst.size(), 1)) {  // tables only shrink after the first build
+        t.inst.alloc(std::max<size_t>(inst.size(), 1));
+        t.row0.alloc(std::max<size_t>(row0.size(), 1));
+    }
     t.inst.upload(inst.data(), inst.size(), stream_);
     t.row0.upload(row0.data(), row0.size(), stream_);
-    t.d_first.alloc(N_);
-    t.d_count.alloc(N_);
+    if (t.d_first.size() < (size_t)N_) {
+        t.d_first.alloc(N_);
+        t.d_count.alloc(N_);
+    }
     t.d_first.upload(t.first.data(), N_, stream_);
     t.d_count.upload(t.count.data(), N_, stream_);
     PDAL_HIP(hipStreamSynchronize(stream_));
@@ -152,6 +162,8 @@ void BatchSolver::build_table(Table &t, const std::vector<long> &off, int G)
 void BatchSolver::setup()
 {
     const auto t0 = std::chrono::steady_clock::now();
+    use_graph_ = !(getenv("PDAL_NO_GRAPH") && atoi(getenv("PDAL_NO_GRAPH")) != 0);
+    compact_ = !(getenv("PDAL_BATCH_NO_COMPACT") && atoi(getenv("PDAL_BATCH_NO_COMPACT")) != 0);
     PDAL_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     for (auto &e : ev_) PDAL_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     // --- concatenate (block diagonal) ---
@@ -444,10 +456,14 @@ void BatchSolver::power_iteration()
     ps.upload(ps0.data(), N_, stream_);
     power_init_kernel<<<N_, kBlock, 0, stream_>>>(d_row_off_.get(), v.get(), ps.get());
     PDAL_LAUNCHED();
-    const BlockTable bta{ta_.inst.get(), ta_.row0.get(), row_end_a_.get()};
-    const BlockTable btat{tat_.inst.get(), tat_.row0.get(), row_end_at_.get()};
     const double tol = P_.sv_tol;
+    if (ta_.blocks == 0 || tat_.blocks == 0) {  // no instance has a nonzero: nothing to estimate
+        for (int i = 0; i < N_; ++i) init_[i].lambda_est = 1.0;
+        return;
+    }
     auto one = [&]() {
+        const BlockTable bta{ta_.inst.get(), ta_.row0.get(), row_end_a_.get()};
+        const BlockTable btat{tat_.inst.get(), tat_.row0.get(), row_end_at_.get()};
         power_normalize_kernel<<<blocks_for_l(m_), kBlock, 0, stream_>>>(row_inst_.get(), v.get(), vh.get(), m_,
                                                                         ps.get(), max_iter);
         dispatch_group(tat_.G, [&](auto g) {
@@ -466,23 +482,44 @@ void BatchSolver::power_iteration()
         PDAL_LAUNCHED();
     };
     constexpr int kChunk = 32;
-    hipGraph_t graph;
     hipGraphExec_t exec = nullptr;
-    PDAL_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeGlobal));
-    for (int i = 0; i < kChunk; ++i) one();
-    PDAL_HIP(hipStreamEndCapture(stream_, &graph));
-    PDAL_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-    PDAL_HIP(hipGraphDestroy(graph));
+    auto capture = [&]() {
+        if (exec) PDAL_HIP(hipGraphExecDestroy(exec));
+        exec = nullptr;
+        if (!use_graph_) return;
+        hipGraph_t graph;
+        PDAL_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeGlobal));
+        for (int i = 0; i < kChunk; ++i) one();
+        PDAL_HIP(hipStreamEndCapture(stream_, &graph));
+        PDAL_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        PDAL_HIP(hipGraphDestroy(graph));
+    };
+    capture();
     std::vector<PowerState> h(N_);
+    int in_tables = N_;
     for (int done_iters = 0; done_iters < max_iter; done_iters += kChunk) {
-        PDAL_HIP(hipGraphLaunch(exec, stream_));
+        if (exec) PDAL_HIP(hipGraphLaunch(exec, stream_));
+        else
+            for (int i = 0; i < kChunk; ++i) one();
         ps.download(h.data(), N_, stream_);
         PDAL_HIP(hipStreamSynchronize(stream_));
-        bool all = true;
-        for (const PowerState &p : h) all = all && (p.done || p.iter >= max_iter);
-        if (all) break;
+        std::vector<char> keep(N_, 0);
+        int running = 0;
+        for (int i = 0; i < N_; ++i) {
+            keep[i] = !(h[i].done || h[i].iter >= max_iter);
+            running += keep[i];
+        }
+        if (running == 0) break;
+        if (compact_ && running <= in_tables / 2) {  // drop converged instances from the launch tables
+            build_table(ta_, row_off_, ta_.G, &keep);
+            build_table(tat_, col_off_, tat_.G, &keep);
+            full_tables_ = false;
+            in_tables = running;
+            capture();
+        }
     }
-    (void)hipGraphExecDestroy(exec);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    restore_full_tables();
     for (int i = 0; i < N_; ++i) {
         const bool empty = probs_[i]->constraint_matrix_num_nonzeros == 0;
         const double sv = std::sqrt(h[i].sigma_sq);
@@ -504,9 +541,11 @@ void BatchSolver::launch_block()
     const int K = cp_.K;
     const BlockTable bta{ta_.inst.get(), ta_.row0.get(), row_end_a_.get()};
     const BlockTable btat{tat_.inst.get(), tat_.row0.get(), row_end_at_.get()};
+    // a table can be empty (instances without rows or columns): its kernels are skipped
+    const bool has_a = ta_.blocks > 0, has_at = tat_.blocks > 0;
     for (int i = 1; i <= K; ++i) {
         const bool major = i == 1 || i == K;
-        if (pred_) {
+        if (pred_ && has_a) {
             PredictorArgs pa{A_.view(), bta, x_.get(), y_.get(), l_.get(), u_.get(), yR_.get(), st_.get()};
             dispatch_group(ta_.G, [&](auto g) {
                 constexpr int G = decltype(g)::value;
@@ -515,14 +554,14 @@ void BatchSolver::launch_block()
         }
         PrimalArgs pa{At_.view(), btat, pred_ ? yR_.get() : y_.get(), c_.get(), lb_.get(), ub_.get(), x0_.get(),
                       x_.get(), xR_.get(), xT_.get(), xH_.get(), dslack_.get(), st_.get(), rho_};
-        dispatch_group(tat_.G, [&](auto g) {
+        if (has_at) dispatch_group(tat_.G, [&](auto g) {
             constexpr int G = decltype(g)::value;
             if (major) primal_kernel<G, true><<<tat_.blocks, kBlock, 0, stream_>>>(pa, i);
             else primal_kernel<G, false><<<tat_.blocks, kBlock, 0, stream_>>>(pa, i);
         });
         DualArgs da{A_.view(), bta, xR_.get(), l_.get(), u_.get(), y0_.get(), y_.get(), yT_.get(), yH_.get(),
                     yR_.get(), st_.get(), rho_};
-        dispatch_group(ta_.G, [&](auto g) {
+        if (has_a) dispatch_group(ta_.G, [&](auto g) {
             constexpr int G = decltype(g)::value;
             if (major) dual_kernel<G, true><<<ta_.blocks, kBlock, 0, stream_>>>(da, i);
             else dual_kernel<G, false><<<ta_.blocks, kBlock, 0, stream_>>>(da, i);
@@ -531,13 +570,13 @@ void BatchSolver::launch_block()
     }
     EvalAtArgs ea{At_.view(), btat, yR_.get(), yT_.get(), xR_.get(), xT_.get(), x0_.get(), c_.get(), dslack_.get(),
                   D_.get(), st_.get(), parts_at_.get()};
-    dispatch_group(tat_.G, [&](auto g) {
+    if (has_at) dispatch_group(tat_.G, [&](auto g) {
         constexpr int G = decltype(g)::value;
         eval_at_kernel<G><<<tat_.blocks, kBlock, 0, stream_>>>(ea);
     });
     EvalAArgs eb{A_.view(), bta, xR_.get(), xT_.get(), yR_.get(), yT_.get(), y0_.get(), l_.get(), u_.get(), E_.get(),
                  st_.get(), parts_a_.get()};
-    dispatch_group(ta_.G, [&](auto g) {
+    if (has_a) dispatch_group(ta_.G, [&](auto g) {
         constexpr int G = decltype(g)::value;
         eval_a_kernel<G><<<ta_.blocks, kBlock, 0, stream_>>>(eb);
     });
@@ -546,15 +585,23 @@ void BatchSolver::launch_block()
     seg_reduce_kernel<<<N_, kBlock, 0, stream_>>>(parts_a_.get(), ta_.d_first.get(), ta_.d_count.get(), kASums, kAQ,
                                                   8, evalq_.get(), st_.get());
     control_kernel<<<blocks_for_l(N_), kBlock, 0, stream_>>>(st_.get(), evalq_.get(), N_, cp_);
-    restart_copy_kernel<<<blocks_for_l(n_ + m_), kBlock, 0, stream_>>>(col_inst_.get(), xT_.get(), x0_.get(),
-                                                                      x_.get(), n_, row_inst_.get(), yT_.get(),
-                                                                      y0_.get(), y_.get(), m_, st_.get());
+    if (has_at) dispatch_group(tat_.G, [&](auto g) {
+        constexpr int G = decltype(g)::value;
+        restart_copy_table_kernel<G><<<tat_.blocks, kBlock, 0, stream_>>>(btat, xT_.get(), x0_.get(), x_.get(),
+                                                                          st_.get());
+    });
+    if (has_a) dispatch_group(ta_.G, [&](auto g) {
+        constexpr int G = decltype(g)::value;
+        restart_copy_table_kernel<G><<<ta_.blocks, kBlock, 0, stream_>>>(bta, yT_.get(), y0_.get(), y_.get(),
+                                                                         st_.get());
+    });
     active_count_kernel<<<1, kBlock, 0, stream_>>>(st_.get(), N_, active_.get());
     PDAL_LAUNCHED();
 }
 
 void BatchSolver::record_graph()
 {
+    if (!use_graph_) return;
     hipGraph_t graph;
     PDAL_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeGlobal));
     launch_block();
@@ -563,10 +610,49 @@ void BatchSolver::record_graph()
     PDAL_HIP(hipGraphDestroy(graph));
 }
 
+void BatchSolver::restore_full_tables()
+{   // every instance back in the tables; the graph captured the old block counts
+    if (full_tables_) return;
+    PDAL_HIP(hipStreamSynchronize(stream_));
+    build_table(ta_, row_off_, ta_.G);
+    build_table(tat_, col_off_, tat_.G);
+    if (graph_) PDAL_HIP(hipGraphExecDestroy(graph_));
+    graph_ = nullptr;
+    full_tables_ = true;
+}
+
+int BatchSolver::compact()
+{   // drop stopped instances from the block tables and re-capture the graph;
+    // returns the number of instances still running (the replays that were in
+    // flight may have stopped the last ones)
+    PDAL_HIP(hipStreamSynchronize(stream_));
+    std::vector<InstState> h(N_);
+    st_.download(h.data(), N_, stream_);
+    PDAL_HIP(hipStreamSynchronize(stream_));
+    std::vector<char> keep(N_, 0);
+    int active = 0;
+    for (int i = 0; i < N_; ++i) {
+        keep[i] = h[i].active ? 1 : 0;
+        active += keep[i];
+    }
+    if (active == 0) return 0;
+    build_table(ta_, row_off_, ta_.G, &keep);
+    build_table(tat_, col_off_, tat_.G, &keep);
+    if (graph_) PDAL_HIP(hipGraphExecDestroy(graph_));
+    graph_ = nullptr;
+    record_graph();
+    table_instances_ = active;
+    full_tables_ = false;
+    return active;
+}
+
 int BatchSolver::solve()
 {
     reset();
     res_ready_ = false;
+    restore_full_tables();  // a previous solve may have compacted the tables
+    table_instances_ = N_;
+    compactions_ = 0;
     if (!graph_) record_graph();
     PDAL_HIP(hipStreamSynchronize(stream_));
     const auto t0 = std::chrono::steady_clock::now();
@@ -577,7 +663,8 @@ int BatchSolver::solve()
     bool timed_out = false;
     while (true) {
         while (launched < checked + 2) {
-            PDAL_HIP(hipGraphLaunch(graph_, stream_));
+            if (graph_) PDAL_HIP(hipGraphLaunch(graph_, stream_));
+            else launch_block();
             PDAL_HIP(hipMemcpyAsync(h_active_.get() + (launched & 1), active_.get(), sizeof(int),
                                     hipMemcpyDeviceToHost, stream_));
             PDAL_HIP(hipEventRecord(ev_[launched & 1], stream_));
@@ -590,6 +677,12 @@ int BatchSolver::solve()
         if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() >= tlim) {
             timed_out = true;
             break;
+        }
+        if (compact_ && active <= table_instances_ / 2) {  // the replays in flight finish first
+            const int still = compact();
+            checked = launched;
+            if (still == 0) break;
+            ++compactions_;
         }
     }
     PDAL_HIP(hipStreamSynchronize(stream_));
@@ -637,11 +730,12 @@ cp_al_result_t *BatchSolver::result(int i)
     const int mi = (int)(row_off_[i + 1] - r0), ni = (int)(col_off_[i + 1] - c0);
     // results of all instances are computed on the first request
     if (!res_ready_) {
+        restore_full_tables();  // results of the instances dropped by compaction too
         rc_all_.alloc(n_);
         xo_all_.alloc(n_);
         yo_all_.alloc(m_);
         const BlockTable btat{tat_.inst.get(), tat_.row0.get(), row_end_at_.get()};
-        dispatch_group(tat_.G, [&](auto g) {
+        if (tat_.blocks > 0) dispatch_group(tat_.G, [&](auto g) {
             constexpr int G = decltype(g)::value;
             result_x_kernel<G><<<tat_.blocks, kBlock, 0, stream_>>>(At_.view(), btat, yT_.get(), xT_.get(), c_.get(),
                                                                    D_.get(), lb_.get(), ub_.get(), st_.get(),
@@ -686,8 +780,57 @@ cp_al_result_t *BatchSolver::result(int i)
     return r;
 }
 
+void BatchSolver::time_kernels(int iters, double *out)
+{   // every instance active; hipEvents around each fused SpMV kernel on the batch stream
+    reset();
+    restore_full_tables();
+    hipEvent_t e[4];
+    for (auto &ev : e) PDAL_HIP(hipEventCreate(&ev));
+    const BlockTable bta{ta_.inst.get(), ta_.row0.get(), row_end_a_.get()};
+    const BlockTable btat{tat_.inst.get(), tat_.row0.get(), row_end_at_.get()};
+    double primal = 0.0, dual = 0.0;
+    if (ta_.blocks == 0 || tat_.blocks == 0) iters = 0;
+    for (int it = 1; it <= iters; ++it) {
+        PrimalArgs pa{At_.view(), btat, y_.get(), c_.get(), lb_.get(), ub_.get(), x0_.get(), x_.get(), xR_.get(),
+                      xT_.get(), xH_.get(), dslack_.get(), st_.get(), rho_};
+        DualArgs da{A_.view(), bta, xR_.get(), l_.get(), u_.get(), y0_.get(), y_.get(), yT_.get(), yH_.get(),
+                    yR_.get(), st_.get(), rho_};
+        PDAL_HIP(hipEventRecord(e[0], stream_));
+        dispatch_group(tat_.G, [&](auto g) {
+            constexpr int G = decltype(g)::value;
+            primal_kernel<G, false><<<tat_.blocks, kBlock, 0, stream_>>>(pa, it);
+        });
+        PDAL_HIP(hipEventRecord(e[1], stream_));
+        dispatch_group(ta_.G, [&](auto g) {
+            constexpr int G = decltype(g)::value;
+            dual_kernel<G, false><<<ta_.blocks, kBlock, 0, stream_>>>(da, it);
+        });
+        PDAL_HIP(hipEventRecord(e[2], stream_));
+        PDAL_LAUNCHED();
+        PDAL_HIP(hipEventSynchronize(e[2]));
+        float a = 0.0f, b = 0.0f;
+        PDAL_HIP(hipEventElapsedTime(&a, e[0], e[1]));
+        PDAL_HIP(hipEventElapsedTime(&b, e[1], e[2]));
+        if (it > 1) {  // the first iteration warms the caches and is not counted
+            primal += a;
+            dual += b;
+        }
+    }
+    for (auto &ev : e) (void)hipEventDestroy(ev);
+    const int n = std::max(1, iters - 1);
+    out[0] = primal / n;
+    out[1] = dual / n;
+    out[2] = tat_.G;
+    out[3] = ta_.G;
+    out[4] = (double)nnz_;
+    out[5] = (double)n_;
+    out[6] = (double)m_;
+    out[7] = (double)tat_.blocks;
+}
+
 void BatchSolver::stats(double *out) const
-{   // {set-up s, last solve s, graph replays, instances, optimal, total iterations, lanes A, lanes At}
+{   // {set-up s, last solve s, graph replays, instances, optimal, total iterations, lanes A, lanes At,
+    //  table compactions}
     int optimal = 0;
     double iters = 0.0;
     for (const InstState &s : final_) {
@@ -702,6 +845,7 @@ void BatchSolver::stats(double *out) const
     out[5] = iters;
     out[6] = ta_.G;
     out[7] = tat_.G;
+    out[8] = compactions_;
 }
 
 }  // namespace batch
@@ -727,6 +871,7 @@ void batch_set_starts(void *h, const double *x, const double *y) { static_cast<B
 cp_al_result_t *batch_result(void *h, int i) { return static_cast<BatchSolver *>(h)->result(i); }
 void batch_stats(void *h, double *out) { static_cast<BatchSolver *>(h)->stats(out); }
 void batch_free(void *h) { delete static_cast<BatchSolver *>(h); }
+void batch_time_kernels(void *h, int iters, double *out) { static_cast<BatchSolver *>(h)->time_kernels(iters, out); }
 
 }  // namespace lp
 }  // namespace pdal

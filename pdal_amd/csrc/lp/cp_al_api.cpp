@@ -480,6 +480,16 @@ PDAL_API void pdal_cp_al_session_free(void *session)
     if (session) session_free(session);
 }
 
+PDAL_API double pdal_copy_calibration(long long bytes, int reps)
+{
+    try {
+        return copy_calibration(bytes, reps);
+    } catch (const std::exception &e) {
+        fprintf(stderr, "[cp_al-mi355x] copy calibration failed: %s\n", e.what());
+        return -2.0;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Batched solver (pdal_batch.h)
 
@@ -543,6 +553,18 @@ PDAL_API void pdal_cp_al_batch_stats(const pdal_cp_al_batch_t *batch, double *ou
 {
     if (!batch || !out) return;
     batch_stats(const_cast<pdal_cp_al_batch_t *>(batch), out);
+}
+
+PDAL_API int pdal_cp_al_batch_time_kernels(pdal_cp_al_batch_t *batch, int iters, double *out)
+{
+    if (!batch || !out || iters < 2) return -1;
+    try {
+        batch_time_kernels(batch, iters, out);
+        return 0;
+    } catch (const std::exception &e) {
+        fprintf(stderr, "[cp_al-mi355x] batch kernel timing failed: %s\n", e.what());
+        return -2;
+    }
 }
 
 PDAL_API void pdal_cp_al_batch_free(pdal_cp_al_batch_t *batch)

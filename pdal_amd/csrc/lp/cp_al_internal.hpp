@@ -30,6 +30,9 @@ int session_time_kernels(void *session, int iters, double *out);
 double session_setup_seconds(void *session);
 void session_free(void *session);
 
+// Plain copy kernel timing for counter calibration (pdal_lp.h).
+double copy_calibration(long long bytes, int reps);
+
 // Batched solver (pdal_batch.h).
 void *batch_create(lp_problem_t *const *problems, int count, const cp_al_parameters_t *params);
 int batch_solve(void *batch);
@@ -37,6 +40,7 @@ void batch_set_starts(void *batch, const double *x, const double *y);
 cp_al_result_t *batch_result(void *batch, int i);
 void batch_stats(void *batch, double *out);
 void batch_free(void *batch);
+void batch_time_kernels(void *batch, int iters, double *out);
 
 }  // namespace lp
 }  // namespace pdal

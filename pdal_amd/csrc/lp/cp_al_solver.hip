@@ -880,5 +880,34 @@ double session_setup_seconds(void *h) { return static_cast<Session *>(h)->solver
 
 void session_free(void *h) { delete static_cast<Session *>(h); }
 
+__global__ void __launch_bounds__(kBlock) copy_calibration_kernel(const double *__restrict__ src,
+                                                                  double *__restrict__ dst, long n)
+{
+    for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) dst[i] = src[i];
+}
+
+double copy_calibration(long long bytes, int reps)
+{
+    const long n = (long)(bytes / (long long)sizeof(double));
+    if (n <= 0 || reps <= 0) return -1.0;
+    DBuf<double> a(n), b(n);
+    PDAL_HIP(hipMemset(a.get(), 0, (size_t)n * sizeof(double)));
+    hipEvent_t e0, e1;
+    PDAL_HIP(hipEventCreate(&e0));
+    PDAL_HIP(hipEventCreate(&e1));
+    const int grid = 256 * 16;
+    copy_calibration_kernel<<<grid, kBlock>>>(a.get(), b.get(), n);
+    PDAL_HIP(hipEventRecord(e0, nullptr));
+    for (int r = 0; r < reps; ++r) copy_calibration_kernel<<<grid, kBlock>>>(a.get(), b.get(), n);
+    PDAL_HIP(hipEventRecord(e1, nullptr));
+    PDAL_LAUNCHED();
+    PDAL_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    PDAL_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ms / reps;
+}
+
 }  // namespace lp
 }  // namespace pdal

@@ -226,6 +226,9 @@ class CpAl:
             L.pdal_cp_al_session_setup_seconds.argtypes = [C.c_void_p]
             L.pdal_cp_al_session_free.restype = None
             L.pdal_cp_al_session_free.argtypes = [C.c_void_p]
+        if hasattr(L, "pdal_copy_calibration"):
+            L.pdal_copy_calibration.restype = C.c_double
+            L.pdal_copy_calibration.argtypes = [C.c_longlong, C.c_int]
         if hasattr(L, "pdal_cp_al_batch_create"):
             L.pdal_cp_al_batch_create.restype = C.c_void_p
             L.pdal_cp_al_batch_create.argtypes = [C.POINTER(C.POINTER(LpProblem)), C.c_int,
@@ -240,6 +243,8 @@ class CpAl:
             L.pdal_cp_al_batch_stats.argtypes = [C.c_void_p, _dp]
             L.pdal_cp_al_batch_free.restype = None
             L.pdal_cp_al_batch_free.argtypes = [C.c_void_p]
+            L.pdal_cp_al_batch_time_kernels.restype = C.c_int
+            L.pdal_cp_al_batch_time_kernels.argtypes = [C.c_void_p, C.c_int, _dp]
 
     # ------------------------------------------------------------------
     def default_params(self, **overrides) -> CpAlParameters:
@@ -450,10 +455,19 @@ class LpBatch:
         return self.lib._take_result(res, self.probs[i])
 
     def stats(self) -> dict:
-        out = (C.c_double * 8)()
+        out = (C.c_double * 9)()
         self.lib.lib.pdal_cp_al_batch_stats(self.h, out)
         return dict(setup_s=out[0], solve_s=out[1], replays=int(out[2]), instances=int(out[3]),
-                    optimal=int(out[4]), iterations=int(out[5]), lanes_a=int(out[6]), lanes_at=int(out[7]))
+                    optimal=int(out[4]), iterations=int(out[5]), lanes_a=int(out[6]), lanes_at=int(out[7]),
+                    compactions=int(out[8]))
+
+    def time_kernels(self, iters: int = 20) -> dict:
+        out = (C.c_double * 8)()
+        rc = self.lib.lib.pdal_cp_al_batch_time_kernels(self.h, iters, out)
+        if rc != 0:
+            raise RuntimeError(f"pdal_cp_al_batch_time_kernels failed ({rc})")
+        return dict(primal_ms=out[0], dual_ms=out[1], lanes_at=int(out[2]), lanes_a=int(out[3]), nnz=int(out[4]),
+                    n=int(out[5]), m=int(out[6]), primal_blocks=int(out[7]))
 
     def close(self) -> None:
         if self.h:

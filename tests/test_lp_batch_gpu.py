@@ -172,3 +172,45 @@ def test_batch_of_config3_family(oracle_lp, gpu_lp, monkeypatch):
             g = b.result(k)
             assert g.iterations == o.iterations
             assert abs(g.primal_objective - inst.optimal_value) <= 1e-5 * abs(inst.optimal_value)
+
+
+def test_compaction_does_not_change_results(gpu_lp, monkeypatch):
+    """Dropping stopped instances from the launch tables (and re-capturing the
+    graph) is invisible in the results: identical bits with it switched off.
+    The lane counts are pinned: the autotuner picks them by timing, and the
+    lane count sets the summation order of a row."""
+    monkeypatch.setenv("CP_AL_LAMBDA", "0")
+    monkeypatch.setenv("PDAL_BATCH_G_A", "8")
+    monkeypatch.setenv("PDAL_BATCH_G_AT", "4")
+    insts = mixed_instances(count=12, seed=6)
+    params = gpu_lp.default_params(termination_evaluation_frequency=K, eps_optimal_relative=1e-6,
+                                   eps_feasible_relative=1e-6)
+    runs = []
+    for flag in ("0", "1"):
+        monkeypatch.setenv("PDAL_BATCH_NO_COMPACT", flag)
+        with LpBatch(gpu_lp, [i.lp_args() for i in insts], params) as b:
+            b.solve()
+            runs.append(([b.result(k) for k in range(len(insts))], b.stats()))
+    (with_c, st_c), (without, st_n) = runs
+    assert st_c["compactions"] >= 1 and st_n["compactions"] == 0
+    for a, b_ in zip(with_c, without):
+        assert a.iterations == b_.iterations and np.array_equal(a.x, b_.x) and np.array_equal(a.y, b_.y)
+
+
+def test_direct_launches_match_graph_replays(gpu_lp, monkeypatch):
+    """PDAL_NO_GRAPH=1 launches each block's kernels directly instead of
+    replaying the captured graph: same kernels, same bits."""
+    monkeypatch.setenv("CP_AL_LAMBDA", "0")
+    monkeypatch.setenv("PDAL_BATCH_G_A", "8")
+    monkeypatch.setenv("PDAL_BATCH_G_AT", "4")
+    insts = mixed_instances(count=6, seed=7)
+    params = gpu_lp.default_params(termination_evaluation_frequency=K, eps_optimal_relative=1e-6,
+                                   eps_feasible_relative=1e-6)
+    runs = []
+    for flag in ("0", "1"):
+        monkeypatch.setenv("PDAL_NO_GRAPH", flag)
+        with LpBatch(gpu_lp, [i.lp_args() for i in insts], params) as b:
+            b.solve()
+            runs.append([b.result(k) for k in range(len(insts))])
+    for a, b_ in zip(*runs):
+        assert a.iterations == b_.iterations and np.array_equal(a.x, b_.x) and np.array_equal(a.y, b_.y)
