@@ -4,6 +4,7 @@ Products (HIP, gfx950):
   pdal_amd/lib/libcp_al.so   -- CP_AL LP solver behind include/cp_al.h
   pdal_amd/lib/cp_al         -- the LP runner command line
   pdal_amd/lib/libfa_cp.so   -- FA_CP QP solver behind include/fa_cp.h
+  pdal_amd/lib/libpdal_ws.so -- learned warm start (MLP on MFMA, POD expansion), include/pdal_warmstart.h
 Checkers (CPU, gcc), built by oracle/build_oracle.py, are separate.
 
 Run `python -m pdal_amd.build` (or __graft_entry__.build()).
@@ -100,9 +101,21 @@ def build_fa_cp() -> Path:
     return out
 
 
+def build_ws() -> Path:
+    objdir = ROOT / "pdal_amd" / "_build" / "ws"
+    objs = _compile_objects([CSRC / "ws" / "warmstart.hip"], objdir)
+    LIB.mkdir(parents=True, exist_ok=True)
+    out = LIB / "libpdal_ws.so"
+    tmp = LIB / "libpdal_ws.so.tmp"
+    _run([_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-Bsymbolic", *objs, "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
 def build_all() -> None:
     build_cp_al()
     build_fa_cp()
+    build_ws()
 
 
 if __name__ == "__main__":

@@ -98,6 +98,7 @@ class Solver {
     Solver(const fa_cp_parameters_t &p, const qp_problem_t *w) : P_(p), W_(w) {}
     ~Solver()
     {
+        if (bb_graph_) (void)hipGraphExecDestroy(bb_graph_);
         if (stream_) (void)hipStreamDestroy(stream_);
     }
     fa_cp_result_t *run();
@@ -186,6 +187,15 @@ class Solver {
     int inner_limit_ = 0;
     double inner_tol_ = 0.0;
     int bb_chunk_ = 4;
+    // hipGraph of kGraphSteps gated inner steps, re-captured when its baked
+    // arguments (tau, AL penalty, tolerance, limit, AL branch) change
+    static constexpr int kGraphSteps = 8;
+    bool use_graph_ = true;
+    hipGraphExec_t bb_graph_ = nullptr;
+    double g_tau_ = NAN, g_al_ = NAN, g_tol_ = NAN;
+    int g_limit_ = -1;
+    bool g_use_al_ = false;
+    void launch_inner_steps(const StepScalars &sc, int steps);
 
     // AL sigma state (fa_cp_core_op.cu:642-906)
     bool al_available_ = false, use_al_ = false;
@@ -214,6 +224,7 @@ void Solver::upload()
     r_ = w->num_rank_lowrank_obj;
     obj_const_ = w->objective_constant;
     PDAL_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    use_graph_ = !(getenv("PDAL_NO_GRAPH") && atoi(getenv("PDAL_NO_GRAPH")) != 0);
 
     HostCsr hA = host_component(w->constraint_matrix, m_, n_, w->constraint_matrix_num_nonzeros);
     hQ_ = host_component(w->objective_sparse_matrix, n_, n_, w->objective_sparse_matrix_num_nonzeros);
@@ -606,6 +617,31 @@ void Solver::bb_inner_step(const StepScalars &sc)
     PDAL_LAUNCHED();
 }
 
+void Solver::launch_inner_steps(const StepScalars &sc, int steps)
+{
+    if (!use_graph_) {
+        for (int k = 0; k < steps; ++k) bb_inner_step(sc);
+        return;
+    }
+    if (!bb_graph_ || g_tau_ != sc.tau || g_al_ != sc.al_sigma || g_tol_ != inner_tol_ || g_limit_ != inner_limit_ ||
+        g_use_al_ != use_al_) {
+        if (bb_graph_) PDAL_HIP(hipGraphExecDestroy(bb_graph_));
+        bb_graph_ = nullptr;
+        hipGraph_t graph;
+        PDAL_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeGlobal));
+        for (int k = 0; k < kGraphSteps; ++k) bb_inner_step(sc);
+        PDAL_HIP(hipStreamEndCapture(stream_, &graph));
+        PDAL_HIP(hipGraphInstantiate(&bb_graph_, graph, nullptr, nullptr, 0));
+        PDAL_HIP(hipGraphDestroy(graph));
+        g_tau_ = sc.tau;
+        g_al_ = sc.al_sigma;
+        g_tol_ = inner_tol_;
+        g_limit_ = inner_limit_;
+        g_use_al_ = use_al_;
+    }
+    for (int k = 0; k < steps; k += kGraphSteps) PDAL_HIP(hipGraphLaunch(bb_graph_, stream_));
+}
+
 void Solver::primal_bb(const StepScalars &sc, bool fuse_halpern)
 {   // primal_BB_step_size_update, fa_cp_core_op.cu:445-503
     if (use_al_) {  // compute_al_effective_dual_product at x (:275-303)
@@ -628,7 +664,7 @@ void Solver::primal_bb(const StepScalars &sc, bool fuse_halpern)
     // launched after the loop has stopped return at once (gated kernels).
     int chunk = bb_chunk_;
     while (true) {
-        for (int k = 0; k < chunk; ++k) bb_inner_step(sc);
+        launch_inner_steps(sc, chunk);
         PDAL_HIP(hipMemcpyAsync(hbb_.get(), dbb_.get(), sizeof(BBState), hipMemcpyDeviceToHost, stream_));
         PDAL_HIP(hipStreamSynchronize(stream_));
         if (hbb_[0].done) break;
