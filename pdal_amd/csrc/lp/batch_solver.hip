@@ -661,6 +661,7 @@ int BatchSolver::solve()
     // replay r is read.  Replays after the last instance stopped are no-ops.
     long launched = 0, checked = 0;
     bool timed_out = false;
+    double next_report = 20.0;
     while (true) {
         while (launched < checked + 2) {
             if (graph_) PDAL_HIP(hipGraphLaunch(graph_, stream_));
@@ -673,6 +674,14 @@ int BatchSolver::solve()
         PDAL_HIP(hipEventSynchronize(ev_[checked & 1]));
         const int active = h_active_[checked & 1];
         ++checked;
+        if (P_.verbose) {  // progress of the solve itself: at most one line per 20 s
+            const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (el >= next_report) {
+                fprintf(stderr, "[batch] %.1f s: %ld blocks of %d iterations, %d of %d instances running\n", el,
+                        checked, cp_.K, active, N_);
+                next_report = el + 20.0;
+            }
+        }
         if (active == 0) break;
         if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() >= tlim) {
             timed_out = true;

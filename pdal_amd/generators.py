@@ -217,3 +217,49 @@ def config2(seed: int = 2, scale: float = 1.0) -> PlantedQP:
     """n=1e5, m=5e4, M 2e4 x n (5 nnz/col), A 8 nnz/col; `scale` shrinks the sizes."""
     n = int(round(100_000 * scale))
     return planted_qp(n, n // 2, max(n // 5, 1), seed)
+
+
+@dataclass
+class LpFamily:
+    """Fixed-A LP family: b = b0 + U theta_b, c = c0 + V theta_c (x >= 0, A x = b),
+    theta = [theta_b, theta_c] of length 2p; base (A, b0, c0) planted as planted_lp."""
+    base: PlantedLP
+    U: np.ndarray   # m x p
+    V: np.ndarray   # n x p
+
+    @property
+    def m(self) -> int:
+        return self.base.m
+
+    @property
+    def n(self) -> int:
+        return self.base.n
+
+    @property
+    def p(self) -> int:
+        return self.U.shape[1]
+
+    def instance(self, theta) -> dict:
+        """Keyword arguments of CpAl.make_problem / LpBatch for parameters theta."""
+        theta = np.asarray(theta, dtype=np.float64)
+        b = self.base.b + self.U @ theta[: self.p]
+        c = self.base.c + self.V @ theta[self.p:]
+        n = self.n
+        return dict(A=self.base.A, c=c, con_lb=b, con_ub=b, var_lb=np.zeros(n), var_ub=np.full(n, np.inf))
+
+
+def lp_family(m: int, n: int, nnz_per_col: int, seed: int, p: int = 32, scale: float = 0.1) -> LpFamily:
+    base = planted_lp(m, n, nnz_per_col, seed)
+    rng = np.random.default_rng(seed + 1_000_003)
+    return LpFamily(base=base, U=scale * rng.standard_normal((m, p)), V=scale * rng.standard_normal((n, p)))
+
+
+def config4_family(seed: int = 4, scale: float = 1.0) -> LpFamily:
+    """m=5e4, n=1e5, 8 nonzeros per column; U (m x 32), V (n x 32) Gaussian x 0.1."""
+    m = int(round(50_000 * scale))
+    return lp_family(m, 2 * m, 8, seed)
+
+
+def config4_thetas(count: int, seed: int, p: int = 32) -> np.ndarray:
+    """theta ~ N(0, I_2p), one row per instance."""
+    return np.random.default_rng(seed).standard_normal((count, 2 * p))
