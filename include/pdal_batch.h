@@ -62,6 +62,16 @@ void pdal_cp_al_batch_stats(const pdal_cp_al_batch_t *batch, double *out);
  * success.  A following solve starts from the starts again. */
 int pdal_cp_al_batch_time_kernels(pdal_cp_al_batch_t *batch, int iters, double *out);
 
+/* Primal weight of every instance (count doubles): after a solve, the value
+ * the PID controller ended with; before the first solve, the start value. */
+int pdal_cp_al_batch_primal_weights(const pdal_cp_al_batch_t *batch, double *out);
+
+/* Start value of the primal weight per instance (entries that are not finite
+ * and positive keep the default); the step schedule follows it.  A warm start
+ * near the solution converges in the first blocks only with a primal weight
+ * near the one a cold solve ends with. */
+int pdal_cp_al_batch_set_primal_weights(pdal_cp_al_batch_t *batch, const double *pw);
+
 void pdal_cp_al_batch_free(pdal_cp_al_batch_t *batch);
 
 #ifdef __cplusplus

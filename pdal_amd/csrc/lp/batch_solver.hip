@@ -63,6 +63,8 @@ class BatchSolver {
     cp_al_result_t *result(int i);
     void stats(double *out) const;
     void time_kernels(int iters, double *out);
+    void primal_weights(double *out) const;
+    void set_primal_weights(const double *pw);
     int count() const { return N_; }
 
   private:
@@ -837,6 +839,26 @@ void BatchSolver::time_kernels(int iters, double *out)
     out[7] = (double)tat_.blocks;
 }
 
+void BatchSolver::primal_weights(double *out) const
+{   // primal weight of every instance at the end of the last solve (its start value before one)
+    for (int i = 0; i < N_; ++i) out[i] = final_.empty() ? init_[i].pw : final_[i].pw;
+}
+
+void BatchSolver::set_primal_weights(const double *pw)
+{   // initial primal weight per instance; the step schedule follows it
+    // (refresh_step_schedule, solver.cu:3103-3126)
+    for (int i = 0; i < N_; ++i) {
+        if (!(std::isfinite(pw[i]) && pw[i] > 0.0)) continue;
+        InstState &s = init_[i];
+        s.pw = pw[i];
+        s.best_pw = pw[i];
+        const double dual_step = s.step * std::fmax(s.pw, 1e-12);
+        s.beta = dual_step * s.al_lambda / std::fmax(1.0 - s.al_lambda, 1e-12);
+        s.tau = s.step / s.pw;
+        s.sigma = s.step * s.pw;
+    }
+}
+
 void BatchSolver::stats(double *out) const
 {   // {set-up s, last solve s, graph replays, instances, optimal, total iterations, lanes A, lanes At,
     //  table compactions}
@@ -881,6 +903,8 @@ cp_al_result_t *batch_result(void *h, int i) { return static_cast<BatchSolver *>
 void batch_stats(void *h, double *out) { static_cast<BatchSolver *>(h)->stats(out); }
 void batch_free(void *h) { delete static_cast<BatchSolver *>(h); }
 void batch_time_kernels(void *h, int iters, double *out) { static_cast<BatchSolver *>(h)->time_kernels(iters, out); }
+void batch_primal_weights(void *h, double *out) { static_cast<BatchSolver *>(h)->primal_weights(out); }
+void batch_set_primal_weights(void *h, const double *pw) { static_cast<BatchSolver *>(h)->set_primal_weights(pw); }
 
 }  // namespace lp
 }  // namespace pdal

@@ -567,6 +567,20 @@ PDAL_API int pdal_cp_al_batch_time_kernels(pdal_cp_al_batch_t *batch, int iters,
     }
 }
 
+PDAL_API int pdal_cp_al_batch_primal_weights(const pdal_cp_al_batch_t *batch, double *out)
+{
+    if (!batch || !out) return -1;
+    batch_primal_weights(const_cast<pdal_cp_al_batch_t *>(batch), out);
+    return 0;
+}
+
+PDAL_API int pdal_cp_al_batch_set_primal_weights(pdal_cp_al_batch_t *batch, const double *pw)
+{
+    if (!batch || !pw) return -1;
+    batch_set_primal_weights(batch, pw);
+    return 0;
+}
+
 PDAL_API void pdal_cp_al_batch_free(pdal_cp_al_batch_t *batch)
 {
     if (batch) batch_free(batch);

@@ -41,6 +41,8 @@ cp_al_result_t *batch_result(void *batch, int i);
 void batch_stats(void *batch, double *out);
 void batch_free(void *batch);
 void batch_time_kernels(void *batch, int iters, double *out);
+void batch_primal_weights(void *batch, double *out);
+void batch_set_primal_weights(void *batch, const double *pw);
 
 }  // namespace lp
 }  // namespace pdal

@@ -243,6 +243,10 @@ class CpAl:
             L.pdal_cp_al_batch_stats.argtypes = [C.c_void_p, _dp]
             L.pdal_cp_al_batch_free.restype = None
             L.pdal_cp_al_batch_free.argtypes = [C.c_void_p]
+            L.pdal_cp_al_batch_primal_weights.restype = C.c_int
+            L.pdal_cp_al_batch_primal_weights.argtypes = [C.c_void_p, _dp]
+            L.pdal_cp_al_batch_set_primal_weights.restype = C.c_int
+            L.pdal_cp_al_batch_set_primal_weights.argtypes = [C.c_void_p, _dp]
             L.pdal_cp_al_batch_time_kernels.restype = C.c_int
             L.pdal_cp_al_batch_time_kernels.argtypes = [C.c_void_p, C.c_int, _dp]
 
@@ -460,6 +464,17 @@ class LpBatch:
         return dict(setup_s=out[0], solve_s=out[1], replays=int(out[2]), instances=int(out[3]),
                     optimal=int(out[4]), iterations=int(out[5]), lanes_a=int(out[6]), lanes_at=int(out[7]),
                     compactions=int(out[8]))
+
+    def primal_weights(self) -> np.ndarray:
+        out = np.empty(len(self.sizes), dtype=np.float64)
+        self.lib.lib.pdal_cp_al_batch_primal_weights(self.h, out.ctypes.data_as(_dp))
+        return out
+
+    def set_primal_weights(self, pw) -> None:
+        pw = np.ascontiguousarray(pw, dtype=np.float64)
+        if pw.size != len(self.sizes) or self.lib.lib.pdal_cp_al_batch_set_primal_weights(
+                self.h, pw.ctypes.data_as(_dp)) != 0:
+            raise RuntimeError("pdal_cp_al_batch_set_primal_weights failed")
 
     def time_kernels(self, iters: int = 20) -> dict:
         out = (C.c_double * 8)()

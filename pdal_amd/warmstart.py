@@ -114,3 +114,117 @@ class Mlp:
 def load_ws() -> WarmStartLib:
     """The MI355X warm-start library; raises FileNotFoundError if it is not built."""
     return WarmStartLib(LIB_DIR / "libpdal_ws.so")
+
+
+class LearnedWarmStart:
+    """Warm start for a fixed-A LP family: theta -> MLP -> standardized POD
+    coefficients (+ log primal weight) -> (x_hat, y_hat, pw_hat).
+
+    fit(): POD of the training solutions (mean + the leading kx modes of x and
+    ky modes of y, from the eigendecomposition of the snapshot Gram matrix in
+    FP64 on the host), then the MLP (widths[0] = len(theta), widths[-1] outputs)
+    trained on the device to predict the coefficients divided by their
+    per-mode standard deviation.  When the training solves' final primal
+    weights are given, the last output predicts their standardized logarithm:
+    a PDHG start near the solution pays off only together with a primal weight
+    near the one the solve ends with.  predict(): MLP forward + POD expansion on
+    the device; x_hat is clipped to the variable bounds given to fit()."""
+
+    def __init__(self, lib: WarmStartLib, widths=(64, 512, 512, 512, 256), kx: int = 128, seed: int = 0):
+        if kx >= widths[-1] - 1:
+            raise ValueError("kx must leave room for the dual modes")
+        self.lib = lib
+        self.widths = list(widths)
+        self.kx = kx
+        self.seed = seed
+        self.mlp = None
+
+    @staticmethod
+    def _pod(S: np.ndarray, k: int):
+        mean = S.mean(axis=0)
+        C0 = S - mean
+        gram = C0 @ C0.T
+        w, V = np.linalg.eigh(gram)
+        order = np.argsort(w)[::-1][:k]
+        w, V = np.maximum(w[order], 1e-300), V[:, order]
+        modes = (C0.T @ V) / np.sqrt(w)  # dim x k, orthonormal columns
+        energy = float(w.sum() / max(np.trace(gram), 1e-300))
+        return mean, modes.T.copy(), energy  # basis k x dim
+
+    def fit(self, thetas, X, Y, pw=None, lb=None, ub=None, epochs: int = 300, batch: int = 256,
+            lr: float = 1e-3) -> dict:
+        thetas = np.asarray(thetas, dtype=np.float64)
+        X = np.asarray(X, dtype=np.float64)
+        Y = np.asarray(Y, dtype=np.float64)
+        out = self.widths[-1]
+        self.has_pw = pw is not None
+        ky_room = out - self.kx - (1 if self.has_pw else 0)
+        kx, ky = min(self.kx, X.shape[0] - 1), min(ky_room, Y.shape[0] - 1)
+        self.mx, self.Bx, ex = self._pod(X, kx)
+        self.my, self.By, ey = self._pod(Y, ky)
+        coef = np.zeros((X.shape[0], out))
+        coef[:, :kx] = (X - self.mx) @ self.Bx.T
+        coef[:, self.kx: self.kx + ky] = (Y - self.my) @ self.By.T  # fewer snapshots than modes: the rest learn zeros
+        if self.has_pw:
+            coef[:, -1] = np.log(np.asarray(pw, dtype=np.float64))
+        self.kx_eff, self.ky_eff = kx, ky
+        self.cmu = coef.mean(axis=0)
+        self.csd = coef.std(axis=0) + 1e-12
+        target = (coef - self.cmu) / self.csd
+        self.lb = None if lb is None else np.asarray(lb, dtype=np.float64)
+        self.ub = None if ub is None else np.asarray(ub, dtype=np.float64)
+        if self.mlp is not None:
+            self.mlp.close()
+        self.mlp = self.lib.create(self.widths, seed=self.seed)
+        hist = self.mlp.train(thetas, target, batch=min(batch, thetas.shape[0]), epochs=epochs, lr=lr,
+                              seed=self.seed)
+        # reconstruction error of the POD itself on the training set (the floor of any prediction)
+        recon_x = self.mx + ((X - self.mx) @ self.Bx.T) @ self.Bx
+        return {"pod_energy_x": ex, "pod_energy_y": ey, "loss_first": float(hist[0]), "loss_last": float(hist[-1]),
+                "pod_rel_err_x": float(np.linalg.norm(recon_x - X) / np.linalg.norm(X)), "epochs": epochs}
+
+    def predict(self, thetas):
+        """(x_hat, y_hat, pw_hat): pw_hat is None when fit() had no primal weights."""
+        coef = self.mlp.forward(np.asarray(thetas, dtype=np.float32)).astype(np.float64) * self.csd + self.cmu
+        cx = coef[:, : self.kx_eff].astype(np.float32)
+        cy = coef[:, self.kx: self.kx + self.ky_eff].astype(np.float32)
+        xh = self.lib.pod_expand(self.Bx.astype(np.float32), self.mx.astype(np.float32), cx)
+        yh = self.lib.pod_expand(self.By.astype(np.float32), self.my.astype(np.float32), cy)
+        if self.lb is not None:
+            xh = np.maximum(xh, self.lb)
+        if self.ub is not None:
+            xh = np.minimum(xh, self.ub)
+        pw = np.exp(coef[:, -1]) if self.has_pw else None
+        return xh, yh, pw
+
+    def close(self) -> None:
+        if self.mlp is not None:
+            self.mlp.close()
+            self.mlp = None
+
+
+def solve_warm(lp_lib, instances: list, params, x_hat, y_hat, pw_hat=None):
+    """Batched solve of `instances` from predicted starts (rows of x_hat /
+    y_hat, one per instance) and, when given, predicted primal weights.  The
+    FP64 KKT test of the solver judges every warm answer; instances the warm
+    pass leaves unsolved are solved again from zero (the cold fallback), and
+    their reported iterations charge both passes.  Returns (results, info)."""
+    from .lp import LpBatch
+    with LpBatch(lp_lib, instances, params) as b:
+        b.set_starts(np.asarray(x_hat, dtype=np.float64).ravel(), np.asarray(y_hat, dtype=np.float64).ravel())
+        if pw_hat is not None:
+            b.set_primal_weights(pw_hat)
+        b.solve()
+        res = [b.result(k) for k in range(len(instances))]
+        warm_s = b.stats()["solve_s"]
+    failed = [k for k, r in enumerate(res) if r.status != "OPTIMAL"]
+    cold_s = 0.0
+    if failed:
+        with LpBatch(lp_lib, [instances[k] for k in failed], params) as b:
+            b.solve()
+            cold_s = b.stats()["solve_s"]
+            for j, k in enumerate(failed):
+                r = b.result(j)
+                r.iterations += res[k].iterations
+                res[k] = r
+    return res, {"fallback": len(failed), "warm_solve_s": warm_s, "fallback_solve_s": cold_s}

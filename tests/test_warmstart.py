@@ -132,3 +132,105 @@ def test_pod_expand_matches_numpy(ws):
     ref = mean.astype(np.float64) + coef.astype(np.float64) @ basis.astype(np.float64)
     assert out.dtype == np.float64 and out.shape == (batch, dim)
     assert np.max(np.abs(out - ref)) <= 1e-5 * np.max(np.abs(ref))
+
+
+def test_pod_modes_are_orthonormal_and_capture_a_low_rank_family():
+    """Snapshots of rank 5 plus a mean: 5 modes reconstruct them exactly."""
+    from pdal_amd.warmstart import LearnedWarmStart
+    rng = np.random.default_rng(8)
+    basis = rng.standard_normal((5, 300))
+    S = rng.standard_normal((40, 5)) @ basis + rng.standard_normal(300)
+    mean, B, energy = LearnedWarmStart._pod(S, 5)
+    np.testing.assert_allclose(B @ B.T, np.eye(5), atol=1e-10)
+    recon = mean + ((S - mean) @ B.T) @ B
+    np.testing.assert_allclose(recon, S, atol=1e-9)
+    assert energy == pytest.approx(1.0, abs=1e-12)
+
+
+# Objective agreement between a cold and a warm solve of one instance, both
+# stopped by the relative KKT test at 1e-6: the primal residual (~1e-7
+# relative) times the dual magnitude moves the objective of these
+# ill-conditioned random instances by up to ~1e-5 relative.
+OBJ_AGREEMENT = 5e-5
+
+
+@pytest.mark.gpu
+def test_learned_warm_start_pipeline(ws, monkeypatch):
+    """The fixed-A family (b = b0 + U theta_b, c = c0 + V theta_c, U and V
+    Gaussian x 0.1) at m=300: 512 training instances solved cold in one batch,
+    POD + MLP fitted on their solutions and final primal weights, then 32
+    held-out instances solved cold and from the network's starts.
+
+    What the trained network gains over the method without it: its held-out
+    prediction error is below that of the training-mean solution (the best
+    start without a network).  Every warm answer passes the same FP64 KKT test
+    (or falls back to a cold solve) and matches the cold objective.  The
+    iteration counts are printed, not asserted: on this family the solution
+    moves across active sets over theta and the network's starts are tens of
+    percent off, where a PDHG warm start saves little."""
+    monkeypatch.setenv("CP_AL_LAMBDA", "0")
+    from pdal_amd.generators import config4_thetas, lp_family
+    from pdal_amd.lp import LpBatch, load_cp_al
+    from pdal_amd.warmstart import LearnedWarmStart, solve_warm
+    lib = load_cp_al()
+    fam = lp_family(300, 600, 8, seed=21)
+    tr, te = config4_thetas(512, 1), config4_thetas(32, 2)
+    params = lib.default_params(termination_evaluation_frequency=64, eps_optimal_relative=1e-6,
+                                eps_feasible_relative=1e-6, iteration_limit=200000)
+    with LpBatch(lib, [fam.instance(t) for t in tr], params) as b:
+        assert b.solve() == len(tr)
+        sols = [b.result(k) for k in range(len(tr))]
+        pw = b.primal_weights()
+    assert np.all(np.isfinite(pw)) and np.all(pw > 0)
+    X, Y = np.stack([s.x for s in sols]), np.stack([s.y for s in sols])
+    lws = LearnedWarmStart(ws, kx=128, seed=3)
+    try:
+        info = lws.fit(tr, X, Y, pw=pw, lb=np.zeros(fam.n), epochs=150)
+        assert info["loss_last"] < 0.5 * info["loss_first"]
+        xh, yh, pwh = lws.predict(te)
+    finally:
+        lws.close()
+    test_inst = [fam.instance(t) for t in te]
+    with LpBatch(lib, test_inst, params) as b:
+        assert b.solve() == len(te)
+        cold = [b.result(k) for k in range(len(te))]
+    Zc = np.hstack([np.stack([c.x for c in cold]), np.stack([c.y for c in cold])])
+    err_net = np.linalg.norm(np.hstack([xh, yh]) - Zc) / np.linalg.norm(Zc)
+    err_mean = np.linalg.norm(np.hstack([X.mean(0), Y.mean(0)]) - Zc) / np.linalg.norm(Zc)
+    assert err_net < err_mean
+    assert np.all(np.isfinite(pwh)) and np.all(pwh > 0)
+    warm, winfo = solve_warm(lib, test_inst, params, xh, yh, pwh)
+    assert all(w.status == "OPTIMAL" for w in warm)
+    for c, w in zip(cold, warm):  # two points that pass the 1e-6 KKT test from different starts
+        assert abs(w.primal_objective - c.primal_objective) <= OBJ_AGREEMENT * (1 + abs(c.primal_objective))
+    print(f"\nprediction error net {err_net:.3f} / mean {err_mean:.3f}; iterations cold "
+          f"{sum(c.iterations for c in cold)} warm {sum(w.iterations for w in warm)} "
+          f"(fallbacks {winfo['fallback']})")
+
+
+@pytest.mark.gpu
+def test_batch_primal_weights_roundtrip_and_exact_restart(monkeypatch):
+    """set_primal_weights is read back before a solve; a batch restarted from
+    its own solutions with the primal weights it ended with converges within a
+    few evaluation blocks, far below the cold count."""
+    monkeypatch.setenv("CP_AL_LAMBDA", "0")
+    from pdal_amd.generators import config4_thetas, lp_family
+    from pdal_amd.lp import LpBatch, load_cp_al
+    lib = load_cp_al()
+    fam = lp_family(300, 600, 8, seed=21)
+    inst = [fam.instance(t) for t in config4_thetas(8, 5)]
+    params = lib.default_params(termination_evaluation_frequency=64, eps_optimal_relative=1e-6,
+                                eps_feasible_relative=1e-6, iteration_limit=200000)
+    with LpBatch(lib, inst, params) as b:
+        assert b.solve() == len(inst)
+        cold = [b.result(k) for k in range(len(inst))]
+        pw = b.primal_weights()
+    with LpBatch(lib, inst, params) as b:
+        b.set_primal_weights(pw)
+        np.testing.assert_array_equal(b.primal_weights(), pw)
+        b.set_starts(np.concatenate([c.x for c in cold]), np.concatenate([c.y for c in cold]))
+        assert b.solve() == len(inst)
+        warm = [b.result(k) for k in range(len(inst))]
+    assert sum(w.iterations for w in warm) < 0.2 * sum(c.iterations for c in cold)
+    for c, w in zip(cold, warm):  # two points that pass the 1e-6 KKT test from different starts
+        assert abs(w.primal_objective - c.primal_objective) <= OBJ_AGREEMENT * (1 + abs(c.primal_objective))
