@@ -189,6 +189,8 @@ def run_config3(args, lib) -> dict:
         "iterations": {"mean": float(its.mean()), "p50": float(np.median(its)), "p99": float(np.percentile(its, 99)),
                        "max": int(its.max())},
         "replays": st["replays"], "compactions": st["compactions"],
+        "seconds_until_half_stopped": round(st["half_stopped_s"], 2),
+        "seconds_until_99pct_stopped": round(st["p99_stopped_s"], 2),
         "plain_solves_per_s": round(plain, 3), "speedup_vs_plain": round(value / plain, 2),
         "plain_sample": int(len(sample)), "objective_rel_err_vs_planted_max3": obj_err,
     })

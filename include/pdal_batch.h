@@ -46,11 +46,15 @@ int pdal_cp_al_batch_solve(pdal_cp_al_batch_t *batch);
  * NULL before the first solve or for an invalid index. */
 cp_al_result_t *pdal_cp_al_batch_result(pdal_cp_al_batch_t *batch, int i);
 
-/* out[0..8] = {set-up seconds, last solve seconds (wall clock, device
+/* out[0..13] = {set-up seconds, last solve seconds (wall clock, device
  * resident), hipGraph replays, instances, OPTIMAL instances, sum of
  * iteration counts, lanes per row of A, lanes per row of At, block-table
  * compactions (stopped instances are dropped from the launch tables and the
- * graph re-captured whenever the active count halves)}. */
+ * graph re-captured whenever the active count halves), seconds of the last
+ * solve until half of the instances had stopped, seconds until 99 % had,
+ * instances whose At.y / A.x_bar runs LDS-staged (one workgroup per instance,
+ * gathered segment in LDS) while every instance runs: At.y staged, A.x_bar
+ * staged whole, A.x_bar staged in two column halves}. */
 void pdal_cp_al_batch_stats(const pdal_cp_al_batch_t *batch, double *out);
 
 /* Launch `iters` plain iterations over all instances kernel by kernel

@@ -43,10 +43,26 @@ constexpr int kGroupRows = 1;
 
 // Workgroup -> (instance, first row) for a row-parallel kernel over A or At;
 // the workgroup covers (256/G) * kGroupRows rows from there.
+//
+// With xcd set, workgroup b takes table entry block(): the dispatcher deals
+// workgroups to the 8 XCDs in turn (b mod 8), so the entries are cut into 8
+// contiguous ranges and XCD x walks range x.  An instance's rows then run on
+// one or two XCDs and its gathered vector is filled into those L2 caches
+// only, not into all eight.  Entry ranges and partial-sum slots stay in table
+// order, so per-instance reductions are unchanged.
 struct BlockTable {
     const int *__restrict__ inst;
     const int *__restrict__ row0;
     const int *__restrict__ row_end;  // per instance: one past its last row
+    int xcd;
+
+    __device__ __forceinline__ int block() const
+    {
+        if (!xcd) return blockIdx.x;
+        const int nb = gridDim.x, per = nb >> 3, rem = nb & 7;
+        const int x = blockIdx.x & 7, k = blockIdx.x >> 3;  // range x holds per + (x < rem) entries
+        return (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + k;
+    }
 };
 
 // Device-resident state of one instance.
@@ -83,12 +99,13 @@ struct PrimalArgs {
 template <int G, bool MAJOR>
 static __global__ void __launch_bounds__(kBlock) primal_kernel(PrimalArgs a, int k_offset)
 {
-    const int inst = a.bt.inst[blockIdx.x];
+    const int tb = a.bt.block();
+    const int inst = a.bt.inst[tb];
     const InstState &s = a.st[inst];
     if (!s.active) return;
     const int lane = threadIdx.x & (G - 1);
     const int end = a.bt.row_end[inst];
-    const int base = a.bt.row0[blockIdx.x] + threadIdx.x / G;
+    const int base = a.bt.row0[tb] + threadIdx.x / G;
     const double tau = s.tau;
     const int k = s.inner + k_offset;
     const double w = (double)k / (double)(k + 1);
@@ -127,12 +144,13 @@ struct DualArgs {
 template <int G, bool MAJOR>
 static __global__ void __launch_bounds__(kBlock) dual_kernel(DualArgs a, int k_offset)
 {
-    const int inst = a.bt.inst[blockIdx.x];
+    const int tb = a.bt.block();
+    const int inst = a.bt.inst[tb];
     const InstState &s = a.st[inst];
     if (!s.active) return;
     const int lane = threadIdx.x & (G - 1);
     const int end = a.bt.row_end[inst];
-    const int base = a.bt.row0[blockIdx.x] + threadIdx.x / G;
+    const int base = a.bt.row0[tb] + threadIdx.x / G;
     const double sigma = s.sigma;
     const double prox = sigma + s.beta;
     const int k = s.inner + k_offset;
@@ -156,6 +174,196 @@ static __global__ void __launch_bounds__(kBlock) dual_kernel(DualArgs a, int k_o
     }
 }
 
+// ---------------------------------------------------------------------------
+// LDS-staged iteration kernels: one 1024-thread workgroup per instance.  The
+// instance's segment of the gathered vector (y for At rows, x_bar for A rows)
+// is copied into LDS with coalesced loads, then every row of the instance
+// gathers from LDS instead of issuing one random global load per nonzero.
+// gfx950 gives a workgroup up to 160 KB of LDS: 20480 doubles.  The matrix,
+// the row-wise vectors and the epilogue are those of the table kernels.
+
+constexpr int kStagedBlock = 1024;
+
+struct StagedList {
+    const int *__restrict__ inst;      // instance per workgroup
+    const long *__restrict__ row_off;  // N+1: the instance's rows of the walked matrix
+    const long *__restrict__ seg_off;  // N+1: its segment of the gathered vector
+};
+
+// Two rows of one lane group at a time (their loads interleave: more memory
+// requests in flight per wave).  Each row is still summed in lane-strided
+// order over [b, e) and folded by the G-lane butterfly, as in row_dot.
+template <int G>
+__device__ __forceinline__ void range_dot2(const CsrView &M, int b1, int e1, int b2, int e2, int lane,
+                                           const double *seg, int base1, int base2, double &s1, double &s2)
+{
+    double a = 0.0, c = 0.0;
+    int k1 = b1 + lane, k2 = b2 + lane;
+    for (; k1 < e1 && k2 < e2; k1 += G, k2 += G) {
+        const double v1 = M.v[k1], v2 = M.v[k2];
+        const int c1 = M.ci[k1], c2 = M.ci[k2];
+        a += v1 * seg[c1 - base1];
+        c += v2 * seg[c2 - base2];
+    }
+    for (; k1 < e1; k1 += G) a += M.v[k1] * seg[M.ci[k1] - base1];
+    for (; k2 < e2; k2 += G) c += M.v[k2] * seg[M.ci[k2] - base2];
+    s1 = group_sum<G>(a);
+    s2 = group_sum<G>(c);
+}
+
+__device__ __forceinline__ int stage_segment(const StagedList &sl, int inst, const double *__restrict__ src,
+                                             double *seg)
+{
+    const int g0 = (int)sl.seg_off[inst], glen = (int)(sl.seg_off[inst + 1] - g0);
+    for (int t = threadIdx.x; t < glen; t += kStagedBlock) seg[t] = src[g0 + t];
+    __syncthreads();
+    return g0;
+}
+
+// The fused epilogues of primal_kernel and dual_kernel for one row.
+template <bool MAJOR>
+__device__ __forceinline__ void primal_row_update(const PrimalArgs &a, int row, double sdot, double tau, double w)
+{
+    const double xj = a.x[row];
+    const double temp = xj - tau * (a.c[row] - sdot);
+    const double proj = fmax(a.lb[row], fmin(temp, a.ub[row]));
+    const double x0 = a.x0[row];
+    if (MAJOR) {
+        a.xT[row] = proj;
+        a.dslack[row] = (proj - temp) / tau;
+        a.xH[row] = w * proj + (1.0 - w) * x0;
+    }
+    const double xr = 2.0 * proj - xj;
+    a.xR[row] = xr;
+    const double refl = a.rho * xr + (1.0 - a.rho) * xj;
+    a.x[row] = w * refl + (1.0 - w) * x0;
+}
+
+template <bool MAJOR>
+__device__ __forceinline__ void dual_row_update(const DualArgs &a, int row, double pp, double sigma, double prox,
+                                                double w)
+{
+    const double yi = a.y[row];
+    const double zb = fmax(a.l[row], fmin(pp - yi / prox, a.u[row]));
+    const double yn = yi - sigma * (pp - zb);
+    const double y0 = a.y0[row];
+    if (MAJOR) {
+        a.yT[row] = yn;
+        a.yH[row] = w * yn + (1.0 - w) * y0;
+        a.yR[row] = 2.0 * yn - yi;
+    }
+    const double refl = a.rho * (2.0 * yn - yi) + (1.0 - a.rho) * yi;
+    a.y[row] = w * refl + (1.0 - w) * y0;
+}
+
+template <int G, bool MAJOR>
+static __global__ void __launch_bounds__(kStagedBlock) primal_staged_kernel(PrimalArgs a, StagedList sl, int k_offset)
+{
+    extern __shared__ double seg[];
+    const int inst = sl.inst[blockIdx.x];
+    const InstState &s = a.st[inst];
+    if (!s.active) return;
+    const int g0 = stage_segment(sl, inst, a.yv, seg);
+    const int lane = threadIdx.x & (G - 1);
+    const int end = (int)sl.row_off[inst + 1];
+    const double tau = s.tau;
+    const int k = s.inner + k_offset;
+    const double w = (double)k / (double)(k + 1);
+    constexpr int kStride = kStagedBlock / G;
+    for (int row = (int)sl.row_off[inst] + threadIdx.x / G; row < end; row += 2 * kStride) {
+        const int row2 = row + kStride;
+        const bool two = row2 < end;
+        double d1, d2;
+        range_dot2<G>(a.At, a.At.rp[row], a.At.rp[row + 1], two ? a.At.rp[row2] : 0, two ? a.At.rp[row2 + 1] : 0,
+                      lane, seg, g0, g0, d1, d2);
+        if (lane == 0) {
+            primal_row_update<MAJOR>(a, row, d1, tau, w);
+            if (two) primal_row_update<MAJOR>(a, row2, d2, tau, w);
+        }
+    }
+}
+
+template <int G, bool MAJOR>
+static __global__ void __launch_bounds__(kStagedBlock) dual_staged_kernel(DualArgs a, StagedList sl, int k_offset)
+{
+    extern __shared__ double seg[];
+    const int inst = sl.inst[blockIdx.x];
+    const InstState &s = a.st[inst];
+    if (!s.active) return;
+    const int g0 = stage_segment(sl, inst, a.xR, seg);
+    const int lane = threadIdx.x & (G - 1);
+    const int end = (int)sl.row_off[inst + 1];
+    const double sigma = s.sigma;
+    const double prox = sigma + s.beta;
+    const int k = s.inner + k_offset;
+    const double w = (double)k / (double)(k + 1);
+    constexpr int kStride = kStagedBlock / G;
+    for (int row = (int)sl.row_off[inst] + threadIdx.x / G; row < end; row += 2 * kStride) {
+        const int row2 = row + kStride;
+        const bool two = row2 < end;
+        double d1, d2;
+        range_dot2<G>(a.A, a.A.rp[row], a.A.rp[row + 1], two ? a.A.rp[row2] : 0, two ? a.A.rp[row2 + 1] : 0, lane,
+                      seg, g0, g0, d1, d2);
+        if (lane == 0) {
+            dual_row_update<MAJOR>(a, row, d1, sigma, prox, w);
+            if (two) dual_row_update<MAJOR>(a, row2, d2, sigma, prox, w);
+        }
+    }
+}
+
+// A.x_bar for an instance whose x_bar segment fills up to two LDS loads
+// (gfx950: n up to 40960).  Its rows hold ascending column indices, and
+// split[row] is the first nonzero of the second column half.  Pass 1 stages
+// the first half and stores every row's partial sum in part[row]; pass 2
+// stages the second half, adds its partial and runs the dual epilogue.  A row
+// is summed as (first-half sum) + (second-half sum), each in lane-strided
+// order: the same terms as the table kernel in a different association.
+template <int G, bool MAJOR>
+static __global__ void __launch_bounds__(kStagedBlock)
+    dual_split_kernel(DualArgs a, StagedList sl, const int *__restrict__ split, double *__restrict__ part,
+                      int k_offset)
+{
+    extern __shared__ double seg[];
+    const int inst = sl.inst[blockIdx.x];
+    const InstState &s = a.st[inst];
+    if (!s.active) return;
+    const int c0 = (int)sl.seg_off[inst], n = (int)(sl.seg_off[inst + 1] - c0), h = (n + 1) / 2;
+    const int lane = threadIdx.x & (G - 1);
+    const int r0 = (int)sl.row_off[inst], end = (int)sl.row_off[inst + 1];
+    constexpr int kStride = kStagedBlock / G;
+    for (int t = threadIdx.x; t < h; t += kStagedBlock) seg[t] = a.xR[c0 + t];
+    __syncthreads();
+    for (int row = r0 + threadIdx.x / G; row < end; row += 2 * kStride) {
+        const int row2 = row + kStride;
+        const bool two = row2 < end;
+        double p1, p2;
+        range_dot2<G>(a.A, a.A.rp[row], split[row], two ? a.A.rp[row2] : 0, two ? split[row2] : 0, lane, seg, c0,
+                      c0, p1, p2);
+        if (lane == 0) {
+            part[row] = p1;
+            if (two) part[row2] = p2;
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < n - h; t += kStagedBlock) seg[t] = a.xR[c0 + h + t];
+    __syncthreads();
+    const double sigma = s.sigma;
+    const double prox = sigma + s.beta;
+    const int k = s.inner + k_offset;
+    const double w = (double)k / (double)(k + 1);
+    for (int row = r0 + threadIdx.x / G; row < end; row += 2 * kStride) {
+        const int row2 = row + kStride;
+        const bool two = row2 < end;
+        double q1, q2;
+        range_dot2<G>(a.A, split[row], a.A.rp[row + 1], two ? split[row2] : 0, two ? a.A.rp[row2 + 1] : 0, lane,
+                      seg, c0 + h, c0 + h, q1, q2);
+        if (lane == 0) {
+            dual_row_update<MAJOR>(a, row, part[row] + q1, sigma, prox, w);
+            if (two) dual_row_update<MAJOR>(a, row2, part[row2] + q2, sigma, prox, w);
+        }
+    }
+}
+
 struct PredictorArgs {
     CsrView A;
     BlockTable bt;
@@ -168,12 +376,13 @@ struct PredictorArgs {
 template <int G>
 static __global__ void __launch_bounds__(kBlock) predictor_kernel(PredictorArgs a)
 {
-    const int inst = a.bt.inst[blockIdx.x];
+    const int tb = a.bt.block();
+    const int inst = a.bt.inst[tb];
     const InstState &s = a.st[inst];
     if (!s.active) return;
     const int lane = threadIdx.x & (G - 1);
     const int end = a.bt.row_end[inst];
-    const int base = a.bt.row0[blockIdx.x] + threadIdx.x / G;
+    const int base = a.bt.row0[tb] + threadIdx.x / G;
     const double beta = s.beta;
     for (int u = 0; u < kGroupRows; ++u) {
         const int row = base + u * (kBlock / G);
@@ -205,12 +414,13 @@ struct EvalAtArgs {
 template <int G>
 static __global__ void __launch_bounds__(kBlock) eval_at_kernel(EvalAtArgs a)
 {
-    const int inst = a.bt.inst[blockIdx.x];
+    const int tb = a.bt.block();
+    const int inst = a.bt.inst[tb];
     if (!a.st[inst].active) return;
     double q[kAtQ] = {0, 0, 0, 0, 0, 0, 0};
     const int lane = threadIdx.x & (G - 1);
     const int end = a.bt.row_end[inst];
-    const int base = a.bt.row0[blockIdx.x] + threadIdx.x / G;
+    const int base = a.bt.row0[tb] + threadIdx.x / G;
     for (int u = 0; u < kGroupRows; ++u) {
         const int row = base + u * (kBlock / G);
         if (row >= end) break;
@@ -230,7 +440,7 @@ static __global__ void __launch_bounds__(kBlock) eval_at_kernel(EvalAtArgs a)
             q[E_RX2] += rx * rx;
         }
     }
-    block_reduce_store<kAtSums, 1>(q, a.parts + (size_t)blockIdx.x * kPartStride);
+    block_reduce_store<kAtSums, 1>(q, a.parts + (size_t)tb * kPartStride);
 }
 
 struct EvalAArgs {
@@ -245,12 +455,13 @@ struct EvalAArgs {
 template <int G>
 static __global__ void __launch_bounds__(kBlock) eval_a_kernel(EvalAArgs a)
 {
-    const int inst = a.bt.inst[blockIdx.x];
+    const int tb = a.bt.block();
+    const int inst = a.bt.inst[tb];
     if (!a.st[inst].active) return;
     double q[kAQ] = {0, 0, 0, 0, 0, 0};
     const int lane = threadIdx.x & (G - 1);
     const int end = a.bt.row_end[inst];
-    const int base = a.bt.row0[blockIdx.x] + threadIdx.x / G;
+    const int base = a.bt.row0[tb] + threadIdx.x / G;
     for (int u = 0; u < kGroupRows; ++u) {
         const int row = base + u * (kBlock / G);
         if (row >= end) break;
@@ -273,7 +484,7 @@ static __global__ void __launch_bounds__(kBlock) eval_a_kernel(EvalAArgs a)
             q[4] += ry * ry;
         }
     }
-    block_reduce_store<kASums, 1>(q, a.parts + (size_t)blockIdx.x * kPartStride);
+    block_reduce_store<kASums, 1>(q, a.parts + (size_t)tb * kPartStride);
 }
 
 // One workgroup per instance: combine its workgroups' partial rows in index
@@ -441,11 +652,12 @@ static __global__ void __launch_bounds__(kBlock)
     restart_copy_table_kernel(BlockTable bt, const double *__restrict__ src, double *__restrict__ anchor,
                               double *__restrict__ cur, const InstState *__restrict__ st)
 {
-    const int inst = bt.inst[blockIdx.x];
+    const int tb = bt.block();
+    const int inst = bt.inst[tb];
     if (!st[inst].restart) return;
     const int end = bt.row_end[inst];
     for (int r = threadIdx.x; r < (kBlock / G) * kGroupRows; r += kBlock) {
-        const int row = bt.row0[blockIdx.x] + r;
+        const int row = bt.row0[tb] + r;
         if (row >= end) break;
         const double v = src[row];
         anchor[row] = v;
@@ -473,10 +685,11 @@ template <int G>
 static __global__ void __launch_bounds__(kBlock)
     table_spmv_kernel(CsrView M, BlockTable bt, const double *__restrict__ x, double *__restrict__ y)
 {
-    const int inst = bt.inst[blockIdx.x];
+    const int tb = bt.block();
+    const int inst = bt.inst[tb];
     const int lane = threadIdx.x & (G - 1);
     const int end = bt.row_end[inst];
-    const int base = bt.row0[blockIdx.x] + threadIdx.x / G;
+    const int base = bt.row0[tb] + threadIdx.x / G;
     for (int u = 0; u < kGroupRows; ++u) {
         const int row = base + u * (kBlock / G);
         if (row >= end) break;
@@ -528,11 +741,12 @@ static __global__ void __launch_bounds__(kBlock)
     power_at_kernel(CsrView At, BlockTable bt, const double *__restrict__ vh, double *__restrict__ w,
                     const PowerState *__restrict__ ps, int max_iter)
 {
-    const int inst = bt.inst[blockIdx.x];
+    const int tb = bt.block();
+    const int inst = bt.inst[tb];
     if (ps[inst].done || ps[inst].iter >= max_iter) return;
     const int lane = threadIdx.x & (G - 1);
     const int end = bt.row_end[inst];
-    const int base = bt.row0[blockIdx.x] + threadIdx.x / G;
+    const int base = bt.row0[tb] + threadIdx.x / G;
     for (int u = 0; u < kGroupRows; ++u) {
         const int row = base + u * (kBlock / G);
         if (row >= end) break;
@@ -547,12 +761,13 @@ static __global__ void __launch_bounds__(kBlock)
     power_a_kernel(CsrView A, BlockTable bt, const double *__restrict__ w, const double *__restrict__ vh,
                    double *__restrict__ v, double *__restrict__ parts, const PowerState *__restrict__ ps, int max_iter)
 {
-    const int inst = bt.inst[blockIdx.x];
+    const int tb = bt.block();
+    const int inst = bt.inst[tb];
     if (ps[inst].done || ps[inst].iter >= max_iter) return;
     double q[1] = {0.0};
     const int lane = threadIdx.x & (G - 1);
     const int end = bt.row_end[inst];
-    const int base = bt.row0[blockIdx.x] + threadIdx.x / G;
+    const int base = bt.row0[tb] + threadIdx.x / G;
     for (int u = 0; u < kGroupRows; ++u) {
         const int row = base + u * (kBlock / G);
         if (row >= end) break;
@@ -562,7 +777,7 @@ static __global__ void __launch_bounds__(kBlock)
             q[0] += vh[row] * s;
         }
     }
-    block_reduce_store<1, 0>(q, parts + (size_t)blockIdx.x * kPartStride);
+    block_reduce_store<1, 0>(q, parts + (size_t)tb * kPartStride);
 }
 
 // Per instance: sigma_sq = sum of its partials (one workgroup per instance).
@@ -709,10 +924,11 @@ static __global__ void __launch_bounds__(kBlock)
                     const double *__restrict__ ub, const InstState *__restrict__ st, double *__restrict__ rc,
                     double *__restrict__ xo)
 {
-    const int inst = bt.inst[blockIdx.x];
+    const int tb = bt.block();
+    const int inst = bt.inst[tb];
     const int lane = threadIdx.x & (G - 1);
     const int end = bt.row_end[inst];
-    const int base = bt.row0[blockIdx.x] + threadIdx.x / G;
+    const int base = bt.row0[tb] + threadIdx.x / G;
     const double cs = st[inst].c_scale, bs = st[inst].b_scale;
     for (int u = 0; u < kGroupRows; ++u) {
         const int row = base + u * (kBlock / G);

@@ -28,6 +28,7 @@
 #include "cp_al_kernels.hpp"
 
 #include <algorithm>
+#include <functional>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -47,6 +48,15 @@ struct Table {
     std::vector<int> first, count;  // per instance: first block, number of blocks
     DBuf<int> d_first, d_count;
     int blocks = 0, G = 1;
+};
+
+// Instances run by the LDS-staged kernels, one workgroup each: first those
+// whose segment fits in half the LDS (two workgroups per CU), then the rest;
+// largest first within each class so the long workgroups start early.
+struct Staged {
+    DBuf<int> inst;
+    int n_half = 0, n_full = 0;
+    size_t b_half = 0, b_full = 0;  // dynamic LDS bytes per workgroup of each class
 };
 
 inline int blocks_for_l(long work) { return work > 0 ? (int)((work + kBlock - 1) / kBlock) : 1; }
@@ -69,6 +79,13 @@ class BatchSolver {
 
   private:
     void build_table(Table &t, const std::vector<long> &off, int G, const std::vector<char> *keep = nullptr);
+    void build_staged(Staged &S, const std::vector<long> &rows, const std::vector<long> &seg,
+                      const std::vector<char> &take, const std::vector<char> *keep, int parts = 1);
+    void build_iteration_tables(const std::vector<char> *keep);
+    void setup_staging();
+    void prepare_split(const std::vector<int> &rp, const std::vector<int> &ci);
+    void launch_primal(bool major, int i, const double *yv);
+    void launch_dual(bool major, int i);
     int compact();
     void restore_full_tables();
     void rescale();
@@ -87,7 +104,14 @@ class BatchSolver {
     DevCsr A_, At_;
     DBuf<int> row_inst_, col_inst_, row_end_a_, row_end_at_;
     DBuf<long> d_row_off_, d_col_off_;
-    Table ta_, tat_;
+    Table ta_, tat_;        // every instance: evaluation, restart copies, results
+    Table ta_it_, tat_it_;  // iteration kernels: the instances the staged kernels do not take
+    Staged sp_, sd_;        // LDS-staged primal (At rows, y segment) / dual (A rows, x_bar segment)
+    Staged sx_;             // dual staged in two column halves (dual_split_kernel)
+    std::vector<char> in_sp_, in_sd_, in_sx_;
+    DBuf<int> split_;       // per row of A (split instances): first nonzero of the second column half
+    DBuf<double> dpart_;    // per row of A (split instances): first-half partial sum
+    size_t lds_cap_ = 0;
     DBuf<double> c_, lb_, ub_, l_, u_, E_, D_;
     DBuf<double> x0_, x_, xT_, xH_, xR_, dslack_, y0_, y_, yT_, yH_, yR_;
     DBuf<double> xs_, ys_;  // scaled starts (kept for repeated solves)
@@ -99,15 +123,27 @@ class BatchSolver {
     hipGraphExec_t graph_ = nullptr;
     ControlParams cp_{};
     bool pred_ = false;
+    int xcd_ = 1;  // XCD-contiguous table entries (PDAL_BATCH_NO_XCD=1: dispatch order)
+    bool staged_ = true;  // LDS-staged iteration kernels (PDAL_BATCH_NO_STAGE=1: table kernels only)
+    int min_staged_ = 512;  // fewest instances worth staging: two workgroups per CU on 256 CUs
+                            // (PDAL_BATCH_MIN_STAGED overrides)
     double rho_ = 1.0;
     double setup_s_ = 0.0, solve_s_ = 0.0;
     long replays_ = 0;
     std::vector<InstState> final_;
     hipEvent_t ev_[2] = {nullptr, nullptr};
+    // Independent kernels of one SpMV pass (staged classes, table kernel) run
+    // on side streams forked from and joined back to stream_, so one kernel's
+    // tail overlaps the others (PDAL_BATCH_NO_OVERLAP=1: one stream).
+    hipStream_t side_[2] = {nullptr, nullptr};
+    hipEvent_t fork_ev_ = nullptr, join_ev_[2] = {nullptr, nullptr};
+    bool overlap_ = true;
+    void run_forked(std::vector<std::function<void(hipStream_t)>> &work, bool fork);
     bool res_ready_ = false;
     DBuf<double> rc_all_, xo_all_, yo_all_;
     bool full_tables_ = true, compact_ = true, use_graph_ = true;
     int table_instances_ = 0, compactions_ = 0;
+    double t_half_ = 0.0, t_p99_ = 0.0;  // seconds until half / 99 % of the instances stopped
 
     void copy_d(double *dst, const double *src, long len)
     {
@@ -126,6 +162,11 @@ BatchSolver::~BatchSolver()
     if (graph_) (void)hipGraphExecDestroy(graph_);
     for (auto &e : ev_)
         if (e) (void)hipEventDestroy(e);
+    for (auto &e : join_ev_)
+        if (e) (void)hipEventDestroy(e);
+    if (fork_ev_) (void)hipEventDestroy(fork_ev_);
+    for (auto &q : side_)
+        if (q) (void)hipStreamDestroy(q);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -161,13 +202,236 @@ void BatchSolver::build_table(Table &t, const std::vector<long> &off, int G, con
     PDAL_HIP(hipStreamSynchronize(stream_));
 }
 
+void BatchSolver::build_staged(Staged &S, const std::vector<long> &rows, const std::vector<long> &seg,
+                               const std::vector<char> &take, const std::vector<char> *keep, int parts)
+{   // parts: the segment is staged in that many pieces (LDS holds the largest)
+    auto seg_bytes = [&](int i) { return (size_t)((seg[i + 1] - seg[i] + parts - 1) / parts) * sizeof(double); };
+    std::vector<int> half, full;
+    for (int i = 0; i < N_; ++i) {
+        if (!take[i] || (keep && !(*keep)[i])) continue;
+        const size_t bytes = seg_bytes(i);
+        (bytes <= lds_cap_ / 2 ? half : full).push_back(i);
+    }
+    auto by_rows = [&](int a, int b) {
+        const long ra = rows[a + 1] - rows[a], rb = rows[b + 1] - rows[b];
+        return ra != rb ? ra > rb : a < b;
+    };
+    std::sort(half.begin(), half.end(), by_rows);
+    std::sort(full.begin(), full.end(), by_rows);
+    auto max_bytes = [&](const std::vector<int> &v) {
+        size_t b = 0;
+        for (int i : v) b = std::max(b, seg_bytes(i));
+        return b;
+    };
+    S.n_half = (int)half.size();
+    S.n_full = (int)full.size();
+    S.b_half = max_bytes(half);
+    S.b_full = max_bytes(full);
+    half.insert(half.end(), full.begin(), full.end());
+    if (S.inst.size() < (size_t)std::max(N_, 1)) S.inst.alloc(std::max(N_, 1));
+    if (!half.empty()) S.inst.upload(half.data(), half.size(), stream_);
+    PDAL_HIP(hipStreamSynchronize(stream_));
+}
+
+void BatchSolver::build_iteration_tables(const std::vector<char> *keep)
+{   // A staged workgroup walks all rows of its instance on one CU.  With few
+    // running instances that leaves most CUs idle while the table kernels
+    // spread every instance over the whole chip, so staging is used only
+    // while at least kMinStaged instances take it.
+    std::vector<char> sp(N_), sd(N_), sx(N_);
+    int nsp = 0, nsd = 0;
+    for (int i = 0; i < N_; ++i) {
+        const bool k = !keep || (*keep)[i];
+        sp[i] = k && in_sp_[i];
+        sd[i] = k && in_sd_[i];
+        sx[i] = k && in_sx_[i];
+        nsp += sp[i];
+        nsd += sd[i] + sx[i];
+    }
+    if (nsp < min_staged_) std::fill(sp.begin(), sp.end(), 0);
+    if (nsd < min_staged_) {
+        std::fill(sd.begin(), sd.end(), 0);
+        std::fill(sx.begin(), sx.end(), 0);
+    }
+    std::vector<char> ka(N_), kat(N_);
+    for (int i = 0; i < N_; ++i) {
+        const bool k = !keep || (*keep)[i];
+        ka[i] = k && !sd[i] && !sx[i];
+        kat[i] = k && !sp[i];
+    }
+    build_table(ta_it_, row_off_, ta_.G, &ka);
+    build_table(tat_it_, col_off_, tat_.G, &kat);
+    build_staged(sp_, col_off_, row_off_, sp, nullptr);
+    build_staged(sd_, row_off_, col_off_, sd, nullptr);
+    build_staged(sx_, row_off_, col_off_, sx, nullptr, 2);
+}
+
+void BatchSolver::prepare_split(const std::vector<int> &rp, const std::vector<int> &ci)
+{   // instances whose x_bar segment fits in two LDS loads and whose rows of A
+    // hold ascending column indices: split index per row for dual_split_kernel
+    int dev = 0, cap = 0;
+    PDAL_HIP(hipGetDevice(&dev));
+    PDAL_HIP(hipDeviceGetAttribute(&cap, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    lds_cap_ = staged_ ? (size_t)cap : 0;
+    in_sx_.assign(N_, 0);
+    // Two-half staging of A.x_bar is opt-in (PDAL_BATCH_SPLIT=1): on the
+    // 4096-instance family it ran 2.04 ms per pass against 1.86 ms with those
+    // instances left to the table kernel.
+    if (!(getenv("PDAL_BATCH_SPLIT") && atoi(getenv("PDAL_BATCH_SPLIT")) != 0)) return;
+    std::vector<int> split;
+    for (int i = 0; i < N_; ++i) {
+        const long mi = row_off_[i + 1] - row_off_[i], ni = col_off_[i + 1] - col_off_[i];
+        const size_t half = (size_t)((ni + 1) / 2) * sizeof(double);
+        if (mi == 0 || (size_t)ni * sizeof(double) <= lds_cap_ || half > lds_cap_) continue;
+        if (split.empty()) split.assign((size_t)m_, 0);
+        const int mid = (int)(col_off_[i] + (ni + 1) / 2);
+        bool sorted = true;
+        for (long r = row_off_[i]; r < row_off_[i + 1] && sorted; ++r) {
+            int k = rp[r];
+            for (int q = rp[r] + 1; q < rp[r + 1]; ++q) sorted = sorted && ci[q - 1] < ci[q];
+            while (k < rp[r + 1] && ci[k] < mid) ++k;
+            split[r] = k;
+        }
+        in_sx_[i] = sorted;
+    }
+    if (!split.empty()) {
+        split_.alloc((size_t)m_);
+        split_.upload(split.data(), (size_t)m_, stream_);
+        dpart_.alloc((size_t)m_);
+        PDAL_HIP(hipStreamSynchronize(stream_));
+    }
+}
+
+void BatchSolver::setup_staging()
+{   // an instance is staged when its gathered segment fits in one workgroup's LDS
+    const int cap = (int)lds_cap_;
+    in_sp_.assign(N_, 0);
+    in_sd_.assign(N_, 0);
+    for (int i = 0; i < N_; ++i) {
+        const long mi = row_off_[i + 1] - row_off_[i], ni = col_off_[i + 1] - col_off_[i];
+        if (mi == 0 || ni == 0) continue;
+        in_sp_[i] = (size_t)mi * sizeof(double) <= lds_cap_;
+        in_sd_[i] = (size_t)ni * sizeof(double) <= lds_cap_;
+    }
+    if (lds_cap_ > 0) {
+        dispatch_group(tat_.G, [&](auto g) {
+            constexpr int G = decltype(g)::value;
+            PDAL_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&primal_staged_kernel<G, true>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, cap));
+            PDAL_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&primal_staged_kernel<G, false>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, cap));
+        });
+        dispatch_group(ta_.G, [&](auto g) {
+            constexpr int G = decltype(g)::value;
+            PDAL_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&dual_staged_kernel<G, true>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, cap));
+            PDAL_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&dual_staged_kernel<G, false>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, cap));
+            PDAL_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&dual_split_kernel<G, true>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, cap));
+            PDAL_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&dual_split_kernel<G, false>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, cap));
+        });
+    }
+    build_iteration_tables(nullptr);
+}
+
+void BatchSolver::run_forked(std::vector<std::function<void(hipStream_t)>> &work, bool fork)
+{   // work[0..n-2] on the side streams, the last on stream_; stream_ then waits for all.
+    // Measured on the 4096-instance family: the A.x_bar pass (two staged classes
+    // + the table kernel) 1.86 -> 1.75 ms forked, the At.y pass (two staged
+    // classes only) 1.38 -> 1.43 ms, so only passes with a table kernel fork.
+    const int nside = overlap_ && fork ? std::min<int>((int)work.size() - 1, 2) : 0;
+    if (nside > 0) {
+        PDAL_HIP(hipEventRecord(fork_ev_, stream_));
+        for (int j = 0; j < nside; ++j) PDAL_HIP(hipStreamWaitEvent(side_[j], fork_ev_, 0));
+    }
+    for (size_t j = 0; j < work.size(); ++j) work[j]((int)j < nside ? side_[j] : stream_);
+    for (int j = 0; j < nside; ++j) {
+        PDAL_HIP(hipEventRecord(join_ev_[j], side_[j]));
+        PDAL_HIP(hipStreamWaitEvent(stream_, join_ev_[j], 0));
+    }
+}
+
+void BatchSolver::launch_primal(bool major, int i, const double *yv)
+{   // At.y + primal update over every running instance: table kernel for the
+    // instances too large to stage, staged kernels for the rest; the
+    // longest workgroups (full-LDS class) are queued first
+    const BlockTable bt{tat_it_.inst.get(), tat_it_.row0.get(), row_end_at_.get(), xcd_};
+    const PrimalArgs pa{At_.view(), bt, yv, c_.get(), lb_.get(), ub_.get(), x0_.get(), x_.get(), xR_.get(),
+                        xT_.get(), xH_.get(), dslack_.get(), st_.get(), rho_};
+    const StagedList sl{sp_.inst.get(), d_col_off_.get(), d_row_off_.get()};
+    dispatch_group(tat_.G, [&](auto g) {
+        constexpr int G = decltype(g)::value;
+        std::vector<std::function<void(hipStream_t)>> work;
+        if (sp_.n_full > 0) work.push_back([&](hipStream_t q) {
+            StagedList sf = sl;
+            sf.inst += sp_.n_half;
+            if (major) primal_staged_kernel<G, true><<<sp_.n_full, kStagedBlock, sp_.b_full, q>>>(pa, sf, i);
+            else primal_staged_kernel<G, false><<<sp_.n_full, kStagedBlock, sp_.b_full, q>>>(pa, sf, i);
+        });
+        if (tat_it_.blocks > 0) work.push_back([&](hipStream_t q) {
+            if (major) primal_kernel<G, true><<<tat_it_.blocks, kBlock, 0, q>>>(pa, i);
+            else primal_kernel<G, false><<<tat_it_.blocks, kBlock, 0, q>>>(pa, i);
+        });
+        if (sp_.n_half > 0) work.push_back([&](hipStream_t q) {
+            if (major) primal_staged_kernel<G, true><<<sp_.n_half, kStagedBlock, sp_.b_half, q>>>(pa, sl, i);
+            else primal_staged_kernel<G, false><<<sp_.n_half, kStagedBlock, sp_.b_half, q>>>(pa, sl, i);
+        });
+        run_forked(work, tat_it_.blocks > 0);
+    });
+}
+
+void BatchSolver::launch_dual(bool major, int i)
+{   // A.x_bar + dual update, split like launch_primal
+    const BlockTable bt{ta_it_.inst.get(), ta_it_.row0.get(), row_end_a_.get(), xcd_};
+    const DualArgs da{A_.view(), bt, xR_.get(), l_.get(), u_.get(), y0_.get(), y_.get(), yT_.get(), yH_.get(),
+                      yR_.get(), st_.get(), rho_};
+    const StagedList sl{sd_.inst.get(), d_row_off_.get(), d_col_off_.get()};
+    dispatch_group(ta_.G, [&](auto g) {
+        constexpr int G = decltype(g)::value;
+        std::vector<std::function<void(hipStream_t)>> work;
+        const int nx = sx_.n_half + sx_.n_full;
+        if (nx > 0) work.push_back([&](hipStream_t q) {  // every member needs more than half the LDS
+            const StagedList sx{sx_.inst.get(), d_row_off_.get(), d_col_off_.get()};
+            const size_t bytes = std::max(sx_.b_half, sx_.b_full);
+            if (major)
+                dual_split_kernel<G, true><<<nx, kStagedBlock, bytes, q>>>(da, sx, split_.get(), dpart_.get(), i);
+            else
+                dual_split_kernel<G, false><<<nx, kStagedBlock, bytes, q>>>(da, sx, split_.get(), dpart_.get(), i);
+        });
+        if (sd_.n_full > 0) work.push_back([&](hipStream_t q) {
+            StagedList sf = sl;
+            sf.inst += sd_.n_half;
+            if (major) dual_staged_kernel<G, true><<<sd_.n_full, kStagedBlock, sd_.b_full, q>>>(da, sf, i);
+            else dual_staged_kernel<G, false><<<sd_.n_full, kStagedBlock, sd_.b_full, q>>>(da, sf, i);
+        });
+        if (ta_it_.blocks > 0) work.push_back([&](hipStream_t q) {
+            if (major) dual_kernel<G, true><<<ta_it_.blocks, kBlock, 0, q>>>(da, i);
+            else dual_kernel<G, false><<<ta_it_.blocks, kBlock, 0, q>>>(da, i);
+        });
+        if (sd_.n_half > 0) work.push_back([&](hipStream_t q) {
+            if (major) dual_staged_kernel<G, true><<<sd_.n_half, kStagedBlock, sd_.b_half, q>>>(da, sl, i);
+            else dual_staged_kernel<G, false><<<sd_.n_half, kStagedBlock, sd_.b_half, q>>>(da, sl, i);
+        });
+        run_forked(work, ta_it_.blocks > 0);
+    });
+}
+
 void BatchSolver::setup()
 {
     const auto t0 = std::chrono::steady_clock::now();
     use_graph_ = !(getenv("PDAL_NO_GRAPH") && atoi(getenv("PDAL_NO_GRAPH")) != 0);
     compact_ = !(getenv("PDAL_BATCH_NO_COMPACT") && atoi(getenv("PDAL_BATCH_NO_COMPACT")) != 0);
+    xcd_ = (getenv("PDAL_BATCH_NO_XCD") && atoi(getenv("PDAL_BATCH_NO_XCD")) != 0) ? 0 : 1;
+    staged_ = !(getenv("PDAL_BATCH_NO_STAGE") && atoi(getenv("PDAL_BATCH_NO_STAGE")) != 0);
+    if (const char *raw = getenv("PDAL_BATCH_MIN_STAGED")) min_staged_ = std::max(1, atoi(raw));
     PDAL_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     for (auto &e : ev_) PDAL_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    overlap_ = !(getenv("PDAL_BATCH_NO_OVERLAP") && atoi(getenv("PDAL_BATCH_NO_OVERLAP")) != 0);
+    for (auto &q : side_) PDAL_HIP(hipStreamCreateWithFlags(&q, hipStreamNonBlocking));
+    PDAL_HIP(hipEventCreateWithFlags(&fork_ev_, hipEventDisableTiming));
+    for (auto &e : join_ev_) PDAL_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     // --- concatenate (block diagonal) ---
     row_off_.assign(N_ + 1, 0);
     col_off_.assign(N_ + 1, 0);
@@ -208,6 +472,7 @@ void BatchSolver::setup()
             if (w->primal_start) xs[c0 + j] = w->primal_start[j];
         }
     }
+    prepare_split(rp, ci);
     HostCsr T = transpose_csr((int)m_, (int)n_, rp.data(), ci.data(), v.data());
     auto put = [&](DevCsr &Dm, int rows, const int *hrp, const int *hci, const double *hv) {
         Dm.rows = rows;
@@ -291,6 +556,7 @@ void BatchSolver::setup()
     PDAL_HIP(hipMemsetAsync(xR_.get(), 0, (size_t)n_ * sizeof(double), stream_));
     PDAL_HIP(hipMemsetAsync(yR_.get(), 0, (size_t)m_ * sizeof(double), stream_));
     autotune();
+    setup_staging();
     parts_at_.alloc((size_t)tat_.blocks * kPartStride);
     parts_a_.alloc((size_t)ta_.blocks * kPartStride);
     power_iteration();
@@ -404,7 +670,7 @@ void BatchSolver::autotune()
             if (cands.size() == 1) break;
             Table tt;
             build_table(tt, off, G);
-            const BlockTable bt{tt.inst.get(), tt.row0.get(), row_end};
+            const BlockTable bt{tt.inst.get(), tt.row0.get(), row_end, xcd_};
             auto launch = [&]() {
                 dispatch_group(G, [&](auto g) {
                     constexpr int GG = decltype(g)::value;
@@ -464,8 +730,8 @@ void BatchSolver::power_iteration()
         return;
     }
     auto one = [&]() {
-        const BlockTable bta{ta_.inst.get(), ta_.row0.get(), row_end_a_.get()};
-        const BlockTable btat{tat_.inst.get(), tat_.row0.get(), row_end_at_.get()};
+        const BlockTable bta{ta_.inst.get(), ta_.row0.get(), row_end_a_.get(), xcd_};
+        const BlockTable btat{tat_.inst.get(), tat_.row0.get(), row_end_at_.get(), xcd_};
         power_normalize_kernel<<<blocks_for_l(m_), kBlock, 0, stream_>>>(row_inst_.get(), v.get(), vh.get(), m_,
                                                                         ps.get(), max_iter);
         dispatch_group(tat_.G, [&](auto g) {
@@ -541,8 +807,8 @@ void BatchSolver::reset()
 void BatchSolver::launch_block()
 {
     const int K = cp_.K;
-    const BlockTable bta{ta_.inst.get(), ta_.row0.get(), row_end_a_.get()};
-    const BlockTable btat{tat_.inst.get(), tat_.row0.get(), row_end_at_.get()};
+    const BlockTable bta{ta_.inst.get(), ta_.row0.get(), row_end_a_.get(), xcd_};
+    const BlockTable btat{tat_.inst.get(), tat_.row0.get(), row_end_at_.get(), xcd_};
     // a table can be empty (instances without rows or columns): its kernels are skipped
     const bool has_a = ta_.blocks > 0, has_at = tat_.blocks > 0;
     for (int i = 1; i <= K; ++i) {
@@ -554,20 +820,8 @@ void BatchSolver::launch_block()
                 predictor_kernel<G><<<ta_.blocks, kBlock, 0, stream_>>>(pa);
             });
         }
-        PrimalArgs pa{At_.view(), btat, pred_ ? yR_.get() : y_.get(), c_.get(), lb_.get(), ub_.get(), x0_.get(),
-                      x_.get(), xR_.get(), xT_.get(), xH_.get(), dslack_.get(), st_.get(), rho_};
-        if (has_at) dispatch_group(tat_.G, [&](auto g) {
-            constexpr int G = decltype(g)::value;
-            if (major) primal_kernel<G, true><<<tat_.blocks, kBlock, 0, stream_>>>(pa, i);
-            else primal_kernel<G, false><<<tat_.blocks, kBlock, 0, stream_>>>(pa, i);
-        });
-        DualArgs da{A_.view(), bta, xR_.get(), l_.get(), u_.get(), y0_.get(), y_.get(), yT_.get(), yH_.get(),
-                    yR_.get(), st_.get(), rho_};
-        if (has_a) dispatch_group(ta_.G, [&](auto g) {
-            constexpr int G = decltype(g)::value;
-            if (major) dual_kernel<G, true><<<ta_.blocks, kBlock, 0, stream_>>>(da, i);
-            else dual_kernel<G, false><<<ta_.blocks, kBlock, 0, stream_>>>(da, i);
-        });
+        if (has_at) launch_primal(major, i, pred_ ? yR_.get() : y_.get());
+        if (has_a) launch_dual(major, i);
         PDAL_LAUNCHED();
     }
     EvalAtArgs ea{At_.view(), btat, yR_.get(), yT_.get(), xR_.get(), xT_.get(), x0_.get(), c_.get(), dslack_.get(),
@@ -618,6 +872,7 @@ void BatchSolver::restore_full_tables()
     PDAL_HIP(hipStreamSynchronize(stream_));
     build_table(ta_, row_off_, ta_.G);
     build_table(tat_, col_off_, tat_.G);
+    build_iteration_tables(nullptr);
     if (graph_) PDAL_HIP(hipGraphExecDestroy(graph_));
     graph_ = nullptr;
     full_tables_ = true;
@@ -640,6 +895,7 @@ int BatchSolver::compact()
     if (active == 0) return 0;
     build_table(ta_, row_off_, ta_.G, &keep);
     build_table(tat_, col_off_, tat_.G, &keep);
+    build_iteration_tables(&keep);
     if (graph_) PDAL_HIP(hipGraphExecDestroy(graph_));
     graph_ = nullptr;
     record_graph();
@@ -655,6 +911,7 @@ int BatchSolver::solve()
     restore_full_tables();  // a previous solve may have compacted the tables
     table_instances_ = N_;
     compactions_ = 0;
+    t_half_ = t_p99_ = 0.0;
     if (!graph_) record_graph();
     PDAL_HIP(hipStreamSynchronize(stream_));
     const auto t0 = std::chrono::steady_clock::now();
@@ -684,6 +941,10 @@ int BatchSolver::solve()
                 next_report = el + 20.0;
             }
         }
+        if (t_half_ == 0.0 && active <= N_ / 2)
+            t_half_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (t_p99_ == 0.0 && active <= N_ / 100)
+            t_p99_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         if (active == 0) break;
         if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() >= tlim) {
             timed_out = true;
@@ -745,7 +1006,7 @@ cp_al_result_t *BatchSolver::result(int i)
         rc_all_.alloc(n_);
         xo_all_.alloc(n_);
         yo_all_.alloc(m_);
-        const BlockTable btat{tat_.inst.get(), tat_.row0.get(), row_end_at_.get()};
+        const BlockTable btat{tat_.inst.get(), tat_.row0.get(), row_end_at_.get(), xcd_};
         if (tat_.blocks > 0) dispatch_group(tat_.G, [&](auto g) {
             constexpr int G = decltype(g)::value;
             result_x_kernel<G><<<tat_.blocks, kBlock, 0, stream_>>>(At_.view(), btat, yT_.get(), xT_.get(), c_.get(),
@@ -797,25 +1058,13 @@ void BatchSolver::time_kernels(int iters, double *out)
     restore_full_tables();
     hipEvent_t e[4];
     for (auto &ev : e) PDAL_HIP(hipEventCreate(&ev));
-    const BlockTable bta{ta_.inst.get(), ta_.row0.get(), row_end_a_.get()};
-    const BlockTable btat{tat_.inst.get(), tat_.row0.get(), row_end_at_.get()};
     double primal = 0.0, dual = 0.0;
     if (ta_.blocks == 0 || tat_.blocks == 0) iters = 0;
     for (int it = 1; it <= iters; ++it) {
-        PrimalArgs pa{At_.view(), btat, y_.get(), c_.get(), lb_.get(), ub_.get(), x0_.get(), x_.get(), xR_.get(),
-                      xT_.get(), xH_.get(), dslack_.get(), st_.get(), rho_};
-        DualArgs da{A_.view(), bta, xR_.get(), l_.get(), u_.get(), y0_.get(), y_.get(), yT_.get(), yH_.get(),
-                    yR_.get(), st_.get(), rho_};
         PDAL_HIP(hipEventRecord(e[0], stream_));
-        dispatch_group(tat_.G, [&](auto g) {
-            constexpr int G = decltype(g)::value;
-            primal_kernel<G, false><<<tat_.blocks, kBlock, 0, stream_>>>(pa, it);
-        });
+        launch_primal(false, it, y_.get());
         PDAL_HIP(hipEventRecord(e[1], stream_));
-        dispatch_group(ta_.G, [&](auto g) {
-            constexpr int G = decltype(g)::value;
-            dual_kernel<G, false><<<ta_.blocks, kBlock, 0, stream_>>>(da, it);
-        });
+        launch_dual(false, it);
         PDAL_HIP(hipEventRecord(e[2], stream_));
         PDAL_LAUNCHED();
         PDAL_HIP(hipEventSynchronize(e[2]));
@@ -861,7 +1110,8 @@ void BatchSolver::set_primal_weights(const double *pw)
 
 void BatchSolver::stats(double *out) const
 {   // {set-up s, last solve s, graph replays, instances, optimal, total iterations, lanes A, lanes At,
-    //  table compactions}
+    //  table compactions, s until half stopped, s until 99 % stopped, instances staged by
+    //  primal_staged_kernel, dual_staged_kernel, dual_split_kernel when all run}
     int optimal = 0;
     double iters = 0.0;
     for (const InstState &s : final_) {
@@ -877,6 +1127,19 @@ void BatchSolver::stats(double *out) const
     out[6] = ta_.G;
     out[7] = tat_.G;
     out[8] = compactions_;
+    out[9] = t_half_;
+    out[10] = t_p99_;
+    int sp = 0, sd = 0, sx = 0;  // instances the staged kernels take with every instance running
+    for (int i = 0; i < N_ && !in_sx_.empty(); ++i) {
+        sp += in_sp_[i];
+        sd += in_sd_[i];
+        sx += in_sx_[i];
+    }
+    if (sp < min_staged_) sp = 0;
+    if (sd + sx < min_staged_) sd = sx = 0;
+    out[11] = sp;
+    out[12] = sd;
+    out[13] = sx;
 }
 
 }  // namespace batch

@@ -214,3 +214,105 @@ def test_direct_launches_match_graph_replays(gpu_lp, monkeypatch):
             runs.append([b.result(k) for k in range(len(insts))])
     for a, b_ in zip(*runs):
         assert a.iterations == b_.iterations and np.array_equal(a.x, b_.x) and np.array_equal(a.y, b_.y)
+
+
+def test_staged_kernels_equal_table_kernels(gpu_lp, monkeypatch):
+    """The LDS-staged iteration kernels (one workgroup per instance, gathered
+    segment in LDS) sum every row in the table kernels' order: identical bits,
+    iterations and statuses with staging forced on for every instance and
+    switched off.  Lanes are pinned (they set the summation order)."""
+    monkeypatch.setenv("CP_AL_LAMBDA", "0")
+    monkeypatch.setenv("PDAL_BATCH_G_A", "8")
+    monkeypatch.setenv("PDAL_BATCH_G_AT", "4")
+    monkeypatch.setenv("PDAL_BATCH_MIN_STAGED", "1")
+    # small instances (two workgroups per CU) and one whose segments need more
+    # than half the LDS (82400 B of y, 163200 B of x_bar)
+    insts = mixed_instances(count=9, seed=11) + [planted_lp(10300, 20400, 8, seed=77)]
+    params = gpu_lp.default_params(termination_evaluation_frequency=K, eps_optimal_relative=1e-6,
+                                   eps_feasible_relative=1e-6, iteration_limit=20 * K)
+    runs = []
+    for flag in ("0", "1"):
+        monkeypatch.setenv("PDAL_BATCH_NO_STAGE", flag)
+        print(f"[case] staged={flag == '0'} sizes={[i.m for i in insts]}", flush=True)
+        with LpBatch(gpu_lp, [i.lp_args() for i in insts], params) as b:
+            b.solve()
+            runs.append([b.result(k) for k in range(len(insts))])
+            st = b.stats()
+            if flag == "0":
+                assert st["staged_at"] == len(insts) and st["staged_a"] == len(insts), st
+            else:
+                assert st["staged_at"] == st["staged_a"] == st["split_a"] == 0, st
+    for a, b_ in zip(*runs):
+        assert a.status == b_.status and a.iterations == b_.iterations
+        assert np.array_equal(a.x, b_.x) and np.array_equal(a.y, b_.y)
+
+
+@pytest.mark.parametrize("lam", ["0", "0.05"])
+def test_staged_iterates_track_oracle(oracle_lp, gpu_lp, monkeypatch, lam):
+    """Staging forced on: 5 evaluation blocks of every instance within 1e-10
+    of the oracle (the predictor runs as a table kernel when lambda > 0)."""
+    monkeypatch.setenv("CP_AL_LAMBDA", lam)
+    monkeypatch.setenv("PDAL_BATCH_MIN_STAGED", "1")
+    insts = mixed_instances(count=6, seed=12)
+    params = dict(iteration_limit=5 * K, termination_evaluation_frequency=K, eps_optimal_relative=1e-14,
+                  eps_feasible_relative=1e-14)
+    with LpBatch(gpu_lp, [i.lp_args() for i in insts], gpu_lp.default_params(**params)) as b:
+        assert b.solve() == 0
+        for k, inst in enumerate(insts):
+            o = oracle_solve(oracle_lp, inst, oracle_lp.default_params(**params))
+            g = b.result(k)
+            assert g.iterations == o.iterations == 5 * K
+            assert rel_err(g.x, o.x) <= ITER_RTOL, k
+            assert rel_err(g.y, o.y) <= ITER_RTOL, k
+
+
+@pytest.mark.parametrize("lam", ["0", "0.05"])
+def test_split_dual_tracks_oracle(oracle_lp, gpu_lp, monkeypatch, lam):
+    """A.x_bar staged in two column halves (x_bar of 20600 doubles exceeds one
+    160 KB LDS load): 5 evaluation blocks within 1e-10 of the oracle, next to
+    small fully staged instances.  The streams fork: the split kernel and the
+    two staged classes run concurrently."""
+    monkeypatch.setenv("CP_AL_LAMBDA", lam)
+    monkeypatch.setenv("PDAL_BATCH_MIN_STAGED", "1")
+    monkeypatch.setenv("PDAL_BATCH_SPLIT", "1")  # opt-in path
+    insts = [planted_lp(10300, 20600, 8, seed=78)] + mixed_instances(count=3, seed=13)
+    params = dict(iteration_limit=5 * K, termination_evaluation_frequency=K, eps_optimal_relative=1e-14,
+                  eps_feasible_relative=1e-14)
+    print(f"[case] split dual lam={lam} sizes={[i.m for i in insts]}", flush=True)
+    with LpBatch(gpu_lp, [i.lp_args() for i in insts], gpu_lp.default_params(**params)) as b:
+        st = b.stats()
+        assert st["split_a"] == 1 and st["staged_a"] == len(insts) - 1, st
+        assert b.solve() == 0
+        for k, inst in enumerate(insts):
+            o = oracle_solve(oracle_lp, inst, oracle_lp.default_params(**params))
+            g = b.result(k)
+            assert g.iterations == o.iterations == 5 * K
+            assert rel_err(g.x, o.x) <= ITER_RTOL, k
+            assert rel_err(g.y, o.y) <= ITER_RTOL, k
+
+
+def test_overlapped_streams_match_one_stream(gpu_lp, monkeypatch):
+    """The staged classes and the table kernel of one pass run on forked
+    streams; the results are the bits of the one-stream run, with graphs and
+    with direct launches."""
+    monkeypatch.setenv("CP_AL_LAMBDA", "0")
+    monkeypatch.setenv("PDAL_BATCH_G_A", "8")
+    monkeypatch.setenv("PDAL_BATCH_G_AT", "4")
+    monkeypatch.setenv("PDAL_BATCH_MIN_STAGED", "1")
+    # small (half-LDS class), full-LDS class, and too large to stage for A.x_bar
+    insts = mixed_instances(count=6, seed=14) + [planted_lp(10300, 20400, 8, seed=79),
+                                                 planted_lp(10400, 20800, 8, seed=80)]
+    params = gpu_lp.default_params(termination_evaluation_frequency=K, eps_optimal_relative=1e-6,
+                                   eps_feasible_relative=1e-6, iteration_limit=12 * K)
+    runs = []
+    for overlap, graph in (("1", "0"), ("0", "0"), ("0", "1")):
+        monkeypatch.setenv("PDAL_BATCH_NO_OVERLAP", overlap)
+        monkeypatch.setenv("PDAL_NO_GRAPH", graph)
+        print(f"[case] no_overlap={overlap} no_graph={graph}", flush=True)
+        with LpBatch(gpu_lp, [i.lp_args() for i in insts], params) as b:
+            b.solve()
+            runs.append([b.result(k) for k in range(len(insts))])
+    for other in runs[1:]:
+        for a, b_ in zip(runs[0], other):
+            assert a.status == b_.status and a.iterations == b_.iterations
+            assert np.array_equal(a.x, b_.x) and np.array_equal(a.y, b_.y)
