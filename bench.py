@@ -61,7 +61,7 @@ def measured_traffic(tag: str):
     """HBM bytes per launch of a kernel from a committed rocprofv3 summary
     (FETCH_SIZE and WRITE_SIZE scaled by a copy kernel of known size); the
     summary states the commands that produced it.  None when absent."""
-    f = ROOT / "profiles" / "batch_primal_kernel_config3.md"
+    f = ROOT / "profiles" / "batch_staged_spmv_4096_instances.md"
     if not f.exists():
         return None
     for line in f.read_text().splitlines():
@@ -141,9 +141,9 @@ def run_config3(args, lib) -> dict:
     kt = batch.time_kernels(args.kernel_iters)
     nnz, n, m = kt["nnz"], kt["n"], kt["m"]
     bp, bd = fused_primal_bytes(nnz, n, m, count), fused_dual_bytes(nnz, n, m, count)
-    traffic = measured_traffic("batch_primal_kernel") if count == 4096 else None
-    roof = roofline(bp, kt["primal_ms"], "batch primal_kernel (At.y SpMV over all instances + fused primal update)",
-                    traffic)
+    traffic = measured_traffic("batch_at_pass") if count == 4096 else None
+    roof = roofline(bp, kt["primal_ms"], "batch At.y pass: primal_staged_kernel (LDS-staged At.y + fused primal "
+                    "update), one launch per LDS class, over all instances", traffic)
     log(f"batch kernels: primal {kt['primal_ms']:.3f} ms ({roof['achieved']:.0f} GB/s), dual {kt['dual_ms']:.3f} ms "
         f"({bd / (kt['dual_ms'] * 1e-3) / 1e9:.0f} GB/s), lanes At={kt['lanes_at']} A={kt['lanes_a']}")
     result = {"roofline": roof, "setup_s": round(setup_s, 3),
