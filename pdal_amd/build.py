@@ -3,7 +3,8 @@
 Products (HIP, gfx950):
   pdal_amd/lib/libcp_al.so   -- CP_AL LP solver behind include/cp_al.h
   pdal_amd/lib/cp_al         -- the LP runner command line
-  pdal_amd/lib/libfa_cp.so   -- FA_CP QP solver behind include/fa_cp.h
+  pdal_amd/lib/libfa_cp.so   -- FA_CP QP solver behind include/fa_cp.h (+ the QPS reader, fa_cp_mps_parser.h)
+  pdal_amd/lib/fa_cp         -- the QP runner command line
   pdal_amd/lib/libpdal_ws.so -- learned warm start (MLP on MFMA, POD expansion), include/pdal_warmstart.h
 Checkers (CPU, gcc), built by oracle/build_oracle.py, are separate.
 
@@ -87,6 +88,7 @@ def build_cp_al() -> Path:
 FA_CP_SOURCES = [
     CSRC / "qp" / "fa_cp_api.cpp",
     CSRC / "qp" / "fa_cp_solver.hip",
+    CSRC / "qp" / "qps_parser.cpp",
 ]
 
 
@@ -96,8 +98,11 @@ def build_fa_cp() -> Path:
     LIB.mkdir(parents=True, exist_ok=True)
     out = LIB / "libfa_cp.so"
     tmp = LIB / "libfa_cp.so.tmp"
-    _run([_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-Bsymbolic", *objs, "-o", tmp])
+    _run([_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-Bsymbolic", *objs, "-lz", "-o", tmp])
     os.replace(tmp, out)
+    # QP runner CLI linked against the shared library
+    _run(["g++", "-O2", "-std=c++17", f"-I{INCLUDE}", CSRC / "qp" / "cli.cpp", f"-L{LIB}", "-lfa_cp",
+          f"-Wl,-rpath,$ORIGIN", "-o", LIB / "fa_cp"])
     return out
 
 

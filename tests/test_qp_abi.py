@@ -30,7 +30,7 @@ def _exported(lib: Path) -> set[str]:
     return {ln.split()[-1] for ln in out.stdout.splitlines() if " T " in ln}
 
 
-@pytest.mark.parametrize("header", ["fa_cp.h", "pdal_qp.h"])
+@pytest.mark.parametrize("header", ["fa_cp.h", "pdal_qp.h", "fa_cp_mps_parser.h"])
 def test_library_exports_every_declared_symbol(gpu_qp, header):
     declared = _declared_functions(INCLUDE / header)
     assert declared, header
