@@ -16,8 +16,10 @@
  *   initialize_solver_state .. solver.cu:1795-2273 (norms :1999-2067)
  *   iteration kernels ........ solver.cu:2340-2459, launch order :2523-2661
  *   fixed-point error ........ solver.cu:3611-3686
- *   perform_restart .......... solver.cu:2821-3101 (target "current", PID branch)
- *   refresh_step_schedule .... solver.cu:3103-3126 (alpha mode)
+ *   perform_restart .......... solver.cu:2821-3101 (target "current"; PID on the
+ *                              primal weight, or on sigma in sigma mode :2857-2966)
+ *   refresh_step_schedule .... solver.cu:3103-3126 (alpha and sigma mode)
+ *   sigma parameters ......... solver.cu:3141-3164
  *   step size / primal weight  solver.cu:3538-3609
  *   create_result_from_state . solver.cu:3828-3909
  *   power iteration .......... LP/CP_AL/src/utils.cu:87-228 (global mt19937(1), :27-28)
@@ -31,9 +33,12 @@
  * Supported environment knobs (read where the reference reads them):
  *   CP_AL_LAMBDA, CP_AL_STEP_MARGIN, CP_AL_REFLECTION_COEFFICIENT,
  *   CP_AL_RESTART_SUFFICIENT / _NECESSARY / _ARTIFICIAL, CP_AL_CR_SCALING,
- *   CP_AL_CR_ITERS.  The sigma-mode, adaptive-lambda, alpha-trial,
- *   direct-pulse, restart-select and dynamic-eval controllers are not part of
- *   this oracle (nor of the HIP path).
+ *   CP_AL_CR_ITERS, and sigma mode: CP_AL_SIGMA_MODE, CP_AL_SIGMA,
+ *   CP_AL_SIGMA_ADAPTIVE, CP_AL_SIGMA_INIT/MIN/MAX_FACTOR, CP_AL_LAMBDA_FACTOR,
+ *   CP_AL_LAMBDA_PARAM, CP_AL_SIGMA_BALANCE_GAIN, CP_AL_SIGMA_I_SMOOTH,
+ *   CP_AL_SIGMA_KP/KI/KD, CP_AL_SIGMA_UPDATE_MIN/MAX.  The adaptive-lambda,
+ *   alpha-trial, direct-pulse, restart-select and dynamic-eval controllers are
+ *   not part of this oracle (nor of the HIP path).
  *
  * Deliberate differences from the reference, shared with the HIP path:
  *   - timing uses steady_clock wall time, not clock() CPU time;
@@ -152,6 +157,10 @@ struct State {
 
     double step_size = 0.0, primal_weight = 1.0, al_lambda = 0.0, al_penalty = 0.0;
     double lambda_est = 1.0;
+    // sigma mode (solver.cu:3141-3164)
+    bool sigma_mode = false, sigma_adaptive = false;
+    double al_sigma = 0.0, sigma_base = 0.0, sigma_min = 0.0, sigma_max = 0.0, sigma_best = 0.0;
+    double sigma_err_sum = 0.0, sigma_last_err = 0.0, lambda_param = 0.0;
     double pw_error_sum = 0.0, pw_last_error = 0.0, best_pw = 1.0, best_gap = INFINITY;
     int total_count = 0, inner_count = 0;
     double fp = 0.0, fp0 = 0.0, last_trial_fp = INFINITY;
@@ -383,9 +392,30 @@ void compute_residual(State &s, norm_type_t norm)  // utils.cu:812-917
 
 /* ---------------- solver.cu ---------------- */
 
-void refresh_step_schedule(State &s)  // solver.cu:3103-3126, alpha mode
+void refresh_step_schedule(State &s)  // solver.cu:3103-3126
 {
     const double margin = env_double("CP_AL_STEP_MARGIN", 0.998 * 0.998);
+    if (s.sigma_mode) {  // :3106-3126: tau = 1/(sigma lambda_param), dual step = penalty = sigma
+        const double sigma = std::fmax(s.al_sigma, 0.0);
+        if (sigma <= 0.0) {
+            s.al_penalty = 0.0;
+            s.lambda_param = 0.0;
+            s.step_size = std::sqrt(std::fmax(margin / std::fmax(s.lambda_est, 1e-12), 1e-24));
+            s.al_lambda = 0.0;
+            return;
+        }
+        if (!std::isfinite(s.lambda_param) || s.lambda_param <= 0.0)
+            s.lambda_param = env_double("CP_AL_LAMBDA_PARAM",
+                                        env_double("CP_AL_LAMBDA_FACTOR", 2.01) * std::fmax(s.lambda_est, 1e-12));
+        s.lambda_param = std::fmax(s.lambda_param, (2.0 + 1e-12) * std::fmax(s.lambda_est, 1e-12));
+        const double tau = 1.0 / (sigma * s.lambda_param);
+        const double rho = sigma;
+        s.al_penalty = sigma;
+        s.step_size = std::sqrt(std::fmax(tau * rho, 1e-24));
+        s.primal_weight = std::sqrt(std::fmax(rho / tau, 1e-24));
+        s.al_lambda = 0.0;
+        return;
+    }
     s.al_lambda = std::fmin(std::fmax(s.al_lambda, 0.0), 0.95);
     const double eta_sq = margin * (1.0 - s.al_lambda) / std::fmax(s.lambda_est, 1e-12);
     s.step_size = std::sqrt(std::fmax(eta_sq, 1e-24));
@@ -444,13 +474,18 @@ void iteration(State &s, int k_offset, double rho, bool major)
     }
 }
 
-void compute_fixed_point_error(State &s)  // solver.cu:3611-3686 (non-sigma branch)
+void compute_fixed_point_error(State &s)  // solver.cu:3611-3686
 {
     for (int j = 0; j < s.n; ++j) s.dx[j] = s.xR[j] - s.xT[j];
     for (int i = 0; i < s.m; ++i) s.dy[i] = s.yR[i] - s.yT[i];
     spmv(s.At, s.dy.data(), s.dual_prod.data());
     spmv(s.A, s.dx.data(), s.prim_prod.data());
     const double dn = nrm2(s.dy), pn = nrm2(s.dx), an = nrm2(s.prim_prod);
+    if (s.sigma_mode && s.al_sigma > 0.0 && s.lambda_param > 0.0) {  // :3664-3671
+        const double val = s.al_sigma * (s.lambda_param * pn * pn - an * an) + dn * dn / std::fmax(s.al_sigma, 1e-16);
+        s.fp = std::sqrt(std::fmax(val, 0.0));
+        return;
+    }
     const double movement = pn * pn * s.primal_weight + dn * dn / s.primal_weight;
     const double cross = dot(s.dual_prod, s.dx);
     const double interaction = 2 * s.step_size * cross;
@@ -489,7 +524,53 @@ void perform_restart(State &s, const cp_al_parameters_t *p)  // solver.cu:2821-3
     pd = std::sqrt(pd);
     dd = std::sqrt(dd);
     const double ratio = s.rel_d / std::fmax(s.rel_p, 1e-300);
-    if (pd > 1e-16 && dd > 1e-16 && pd < 1e12 && dd < 1e12 && ratio > 1e-8 && ratio < 1e8) {
+    if (s.sigma_mode) {  // solver.cu:2857-2966
+        if (s.sigma_adaptive) {
+            for (int j = 0; j < s.n; ++j) s.dx[j] = s.xT[j] - s.x0[j];
+            spmv(s.A, s.dx.data(), s.prim_prod.data());
+            const double ax = nrm2(s.prim_prod);
+            const double dx_metric_sq = s.lambda_param * pd * pd - ax * ax;
+            if (dd > 1e-16 && dd < 1e12 && dx_metric_sq > 1e-24 && dx_metric_sq < 1e24 && std::isfinite(s.al_sigma) &&
+                s.al_sigma > 1e-16 && ratio > 1e-8 && ratio < 1e8) {
+                const restart_parameters_t &rp = p->restart_params;
+                const double gain = env_double("CP_AL_SIGMA_BALANCE_GAIN", 0.15);
+                const double smooth = env_double("CP_AL_SIGMA_I_SMOOTH", rp.i_smooth);
+                const double kp = env_double("CP_AL_SIGMA_KP", rp.k_p);
+                const double ki = env_double("CP_AL_SIGMA_KI", rp.k_i);
+                const double kd = env_double("CP_AL_SIGMA_KD", rp.k_d);
+                const double rmin = env_double("CP_AL_SIGMA_UPDATE_MIN", 0.60);
+                const double rmax = env_double("CP_AL_SIGMA_UPDATE_MAX", 1.70);
+                double err = std::log(std::fmax(dd, 1e-300)) - 0.5 * std::log(std::fmax(dx_metric_sq, 1e-300)) -
+                             std::log(std::fmax(s.al_sigma, 1e-300));
+                double bal = 0.5 * std::log(std::fmax(ratio, 1e-300));
+                bal = std::fmax(-2.0, std::fmin(bal, 2.0));
+                err += gain * bal;
+                s.sigma_err_sum = smooth * s.sigma_err_sum + err;
+                const double de = err - s.sigma_last_err;
+                double r = std::exp(kp * err + ki * s.sigma_err_sum + kd * de);
+                r = std::fmin(std::fmax(r, rmin), rmax);
+                s.al_sigma *= r;
+                s.sigma_last_err = err;
+            } else {
+                s.al_sigma = s.sigma_best;
+                s.sigma_err_sum = 0.0;
+                s.sigma_last_err = 0.0;
+            }
+        }
+        if (!std::isfinite(s.al_sigma) || s.al_sigma <= 1e-16) {
+            s.al_sigma = s.sigma_best;
+            s.sigma_err_sum = 0.0;
+            s.sigma_last_err = 0.0;
+        }
+        s.al_sigma = std::fmin(std::fmax(s.al_sigma, s.sigma_min), s.sigma_max);
+        const double g = std::fabs(std::log10(std::fmax(s.rel_d, 1e-300) / std::fmax(s.rel_p, 1e-300)));
+        if (std::isfinite(g) && g < s.best_gap) {
+            s.best_gap = g;
+            s.sigma_best = s.al_sigma;
+        }
+        s.pw_error_sum = 0.0;
+        s.pw_last_error = 0.0;
+    } else if (pd > 1e-16 && dd > 1e-16 && pd < 1e12 && dd < 1e12 && ratio > 1e-8 && ratio < 1e8) {
         const double err = std::log(dd) - std::log(pd) - std::log(s.primal_weight);
         s.pw_error_sum *= p->restart_params.i_smooth;
         s.pw_error_sum += err;
@@ -502,10 +583,12 @@ void perform_restart(State &s, const cp_al_parameters_t *p)  // solver.cu:2821-3
         s.pw_error_sum = 0.0;
         s.pw_last_error = 0.0;
     }
-    const double g = std::fabs(std::log10(std::fmax(s.rel_d, 1e-300) / std::fmax(s.rel_p, 1e-300)));
-    if (g < s.best_gap) {
-        s.best_gap = g;
-        s.best_pw = s.primal_weight;
+    if (!s.sigma_mode) {
+        const double g = std::fabs(std::log10(std::fmax(s.rel_d, 1e-300) / std::fmax(s.rel_p, 1e-300)));
+        if (g < s.best_gap) {
+            s.best_gap = g;
+            s.best_pw = s.primal_weight;
+        }
     }
     refresh_step_schedule(s);
     s.x0 = s.xT;
@@ -821,6 +904,20 @@ ORACLE_API cp_al_result_t *optimize(const cp_al_parameters_t *params, lp_problem
         s.lambda_est = sv * sv;
     }
     s.primal_weight = params->bound_objective_rescaling ? 1.0 : (s.c_norm + 1.0) / (s.b_norm + 1.0);
+    const bool sigma_env = getenv("CP_AL_SIGMA") && getenv("CP_AL_SIGMA")[0] != '\0';
+    s.sigma_mode = sigma_env || env_int("CP_AL_SIGMA_MODE", 0) != 0;
+    if (s.sigma_mode) {  // initialize_sigma_parameters, solver.cu:3141-3164
+        s.lambda_param = env_double("CP_AL_LAMBDA_PARAM",
+                                    env_double("CP_AL_LAMBDA_FACTOR", 2.01) * std::fmax(s.lambda_est, 1e-12));
+        s.lambda_param = std::fmax(s.lambda_param, (2.0 + 1e-12) * std::fmax(s.lambda_est, 1e-12));
+        s.sigma_base = 1.0 / std::sqrt(std::fmax(s.lambda_param, 1e-16));
+        s.sigma_min = env_double("CP_AL_SIGMA_MIN_FACTOR", 1e-4) * s.sigma_base;
+        s.sigma_max = env_double("CP_AL_SIGMA_MAX_FACTOR", 100.0) * s.sigma_base;
+        s.al_sigma = env_double("CP_AL_SIGMA", env_double("CP_AL_SIGMA_INIT_FACTOR", 6.0) * s.sigma_base);
+        s.al_sigma = std::fmin(std::fmax(s.al_sigma, s.sigma_min), s.sigma_max);
+        s.sigma_best = s.al_sigma;
+        s.sigma_adaptive = env_int("CP_AL_SIGMA_ADAPTIVE", sigma_env ? 0 : 1) != 0;
+    }
     s.al_lambda = env_double("CP_AL_LAMBDA", 0.05);
     refresh_step_schedule(s);
     s.best_pw = s.primal_weight;

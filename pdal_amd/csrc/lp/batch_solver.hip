@@ -597,6 +597,8 @@ void BatchSolver::setup()
     warn_unsupported_controllers();
     if (env_set("CP_AL_CR_SCALING"))
         fprintf(stderr, "[box-cpal] CP_AL_CR_SCALING is not applied by the batched solver.\n");
+    if (env_set("CP_AL_SIGMA") || env_int("CP_AL_SIGMA_MODE", 0) != 0)
+        fprintf(stderr, "[box-cpal] sigma mode is not applied by the batched solver (alpha mode runs).\n");
     setup_s_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 

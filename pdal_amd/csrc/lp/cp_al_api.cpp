@@ -56,10 +56,9 @@ bool env_set(const char *name)
 void warn_unsupported_controllers()
 {
     // Controllers of the reference that are switched on only through these
-    // variables (solver.cu:189-1057, 3166-3536; utils.cu:324-332 are honoured).
-    static const char *const knobs[] = {"CP_AL_SIGMA",
-                                        "CP_AL_SIGMA_MODE",
-                                        "CP_AL_SIGMA_TRIGGER",
+    // variables (solver.cu:189-1057, 3166-3536); the restart knobs
+    // (utils.cu:324-332) and sigma mode (solver.cu:3141-3164) are honoured.
+    static const char *const knobs[] = {"CP_AL_SIGMA_TRIGGER",
                                         "CP_AL_ADAPTIVE_LAMBDA",
                                         "CP_AL_LAMBDA_PROFILE",
                                         "CP_AL_ALPHA_ROLLBACK_TRIAL",

@@ -265,6 +265,28 @@ static __global__ void __launch_bounds__(kBlock)
     }
 }
 
+// Partials of ||A (xa - xb)||^2 (grid-stride, fixed grid): the sigma-mode
+// restart's normal-motion term (solver.cu:2867-2884), the difference rounded
+// first as the reference's materialised delta vector is.
+template <int G>
+static __global__ void __launch_bounds__(kBlock)
+    delta_spmv_sumsq_kernel(CsrView A, const double *__restrict__ xa, const double *__restrict__ xb,
+                            double *__restrict__ parts)
+{
+    constexpr int RPB = kBlock / G;
+    double q[1] = {0.0};
+    const int lane = threadIdx.x & (G - 1);
+    for (int base = blockIdx.x * RPB; base < A.rows; base += gridDim.x * RPB) {
+        const int row = base + threadIdx.x / G;
+        if (row < A.rows) {
+            double s_diff, s_b;
+            row_dot2<G>(A, row, lane, xa, xb, s_diff, s_b);
+            if (lane == 0) q[0] += s_diff * s_diff;
+        }
+    }
+    block_reduce_store<1, 0>(q, parts + blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // Power iteration for sigma_max(A)^2 on A A^T (utils.cu:87-228), device
 // resident: every kernel returns at once after convergence or max_iter, so a

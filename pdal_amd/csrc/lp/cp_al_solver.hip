@@ -79,6 +79,8 @@ class Solver {
     bool should_restart(int K);
     void perform_restart(const Eval &e);
     void refresh_step_schedule();
+    void init_sigma();
+    double delta_normal_motion();
     cp_al_result_t *make_result(const lp_problem_t *original, int charged);
     void display() const;
 
@@ -105,6 +107,10 @@ class Solver {
 
     double b_scale_ = 1.0, c_scale_ = 1.0, c_norm_ = 0.0, b_norm_ = 0.0;
     double step_size_ = 0.0, pw_ = 1.0, al_lambda_ = 0.0, al_penalty_ = 0.0, lambda_est_ = 1.0;
+    // sigma mode (solver.cu:3141-3164): steps from sigma and lambda_param, PID on sigma at restarts
+    bool sigma_mode_ = false, sigma_adaptive_ = false;
+    double al_sigma_ = 0.0, sigma_base_ = 0.0, sigma_min_ = 0.0, sigma_max_ = 0.0, sigma_best_ = 0.0;
+    double sigma_err_sum_ = 0.0, sigma_last_err_ = 0.0, lambda_param_ = 0.0;
     double pw_err_sum_ = 0.0, pw_last_err_ = 0.0, best_pw_ = 1.0, best_gap_ = INFINITY;
     int total_ = 0, inner_ = 0;
     double fp_ = 0.0, fp0_ = 0.0, last_trial_fp_ = INFINITY;
@@ -361,9 +367,50 @@ void Solver::autotune_lanes()
     parts_a_.alloc((size_t)eval_grid_a_ * kEvalStride);
 }
 
+void Solver::init_sigma()
+{   // sigma mode on CP_AL_SIGMA or CP_AL_SIGMA_MODE (solver.cu:3563-3578), parameters :3141-3164
+    const bool sigma_env = env_set("CP_AL_SIGMA");
+    sigma_mode_ = sigma_env || env_int("CP_AL_SIGMA_MODE", 0) != 0;
+    al_sigma_ = sigma_base_ = sigma_min_ = sigma_max_ = sigma_best_ = 0.0;
+    sigma_err_sum_ = sigma_last_err_ = lambda_param_ = 0.0;
+    sigma_adaptive_ = false;
+    if (!sigma_mode_) return;
+    lambda_param_ = env_double("CP_AL_LAMBDA_PARAM",
+                               env_double("CP_AL_LAMBDA_FACTOR", 2.01) * std::fmax(lambda_est_, 1e-12));
+    lambda_param_ = std::fmax(lambda_param_, (2.0 + 1e-12) * std::fmax(lambda_est_, 1e-12));
+    sigma_base_ = 1.0 / std::sqrt(std::fmax(lambda_param_, 1e-16));
+    sigma_min_ = env_double("CP_AL_SIGMA_MIN_FACTOR", 1e-4) * sigma_base_;
+    sigma_max_ = env_double("CP_AL_SIGMA_MAX_FACTOR", 100.0) * sigma_base_;
+    al_sigma_ = env_double("CP_AL_SIGMA", env_double("CP_AL_SIGMA_INIT_FACTOR", 6.0) * sigma_base_);
+    al_sigma_ = std::fmin(std::fmax(al_sigma_, sigma_min_), sigma_max_);
+    sigma_best_ = al_sigma_;
+    sigma_adaptive_ = env_int("CP_AL_SIGMA_ADAPTIVE", sigma_env ? 0 : 1) != 0;
+}
+
 void Solver::refresh_step_schedule()
-{   // solver.cu:3103-3126 (alpha mode)
+{   // solver.cu:3103-3126
     const double margin = env_double("CP_AL_STEP_MARGIN", 0.998 * 0.998);
+    if (sigma_mode_) {  // tau = 1/(sigma lambda_param), dual step = penalty = sigma, no alpha
+        const double sigma = std::fmax(al_sigma_, 0.0);
+        if (sigma <= 0.0) {
+            al_penalty_ = 0.0;
+            lambda_param_ = 0.0;
+            step_size_ = std::sqrt(std::fmax(margin / std::fmax(lambda_est_, 1e-12), 1e-24));
+            al_lambda_ = 0.0;
+            return;
+        }
+        if (!std::isfinite(lambda_param_) || lambda_param_ <= 0.0)
+            lambda_param_ = env_double("CP_AL_LAMBDA_PARAM",
+                                       env_double("CP_AL_LAMBDA_FACTOR", 2.01) * std::fmax(lambda_est_, 1e-12));
+        lambda_param_ = std::fmax(lambda_param_, (2.0 + 1e-12) * std::fmax(lambda_est_, 1e-12));
+        const double tau = 1.0 / (sigma * lambda_param_);
+        const double rho = sigma;
+        al_penalty_ = sigma;
+        step_size_ = std::sqrt(std::fmax(tau * rho, 1e-24));
+        pw_ = std::sqrt(std::fmax(rho / tau, 1e-24));
+        al_lambda_ = 0.0;
+        return;
+    }
     al_lambda_ = std::fmin(std::fmax(al_lambda_, 0.0), 0.95);
     const double eta_sq = margin * (1.0 - al_lambda_) / std::fmax(lambda_est_, 1e-12);
     step_size_ = std::sqrt(std::fmax(eta_sq, 1e-24));
@@ -379,13 +426,17 @@ void Solver::init_steps()
         lambda_est_ = sv * sv;
     }
     pw_ = P_.bound_objective_rescaling ? 1.0 : (c_norm_ + 1.0) / (b_norm_ + 1.0);
+    init_sigma();
     al_lambda_ = env_double("CP_AL_LAMBDA", 0.05);
     refresh_step_schedule();
     best_pw_ = pw_;
     warn_unsupported_controllers();
     if (P_.verbose)
-        fprintf(stderr, "[box-cpal] pw_init=%.3e al_lambda=%.3e eta=%.3e refl=%.3e al_penalty=%.3e lambdaA=%.3e\n",
-                pw_, al_lambda_, step_size_, P_.reflection_coefficient, al_penalty_, lambda_est_);
+        fprintf(stderr,
+                "[box-cpal] pw_init=%.3e al_lambda=%.3e sigma_mode=%d sigma_adapt=%d al_sigma=%.3e sigma_base=%.3e "
+                "lambda_param=%.3e eta=%.3e refl=%.3e al_penalty=%.3e lambdaA=%.3e\n",
+                pw_, al_lambda_, (int)sigma_mode_, (int)sigma_adaptive_, al_sigma_, sigma_base_, lambda_param_,
+                step_size_, P_.reflection_coefficient, al_penalty_, lambda_est_);
 }
 
 void Solver::setup()
@@ -443,6 +494,7 @@ void Solver::reset_iterates()
     for (auto *b : {&xR_, &dslack_}) PDAL_HIP(hipMemsetAsync(b->get(), 0, (size_t)n_ * sizeof(double), stream_));
     PDAL_HIP(hipMemsetAsync(yR_.get(), 0, (size_t)m_ * sizeof(double), stream_));
     pw_ = pw_init_;
+    init_sigma();
     al_lambda_ = env_double("CP_AL_LAMBDA", 0.05);
     refresh_step_schedule();
     best_pw_ = pw_;
@@ -571,8 +623,13 @@ Eval Solver::read_eval()
 }
 
 void Solver::fixed_point_error(const Eval &e)
-{   // solver.cu:3658-3686 (alpha-mode branch)
+{   // solver.cu:3658-3686
     const double dn = std::sqrt(e.dn2), pn = std::sqrt(e.pn2), an = std::sqrt(e.ax2);
+    if (sigma_mode_ && al_sigma_ > 0.0 && lambda_param_ > 0.0) {  // :3664-3671
+        const double val = al_sigma_ * (lambda_param_ * pn * pn - an * an) + dn * dn / std::fmax(al_sigma_, 1e-16);
+        fp_ = std::sqrt(std::fmax(val, 0.0));
+        return;
+    }
     const double movement = pn * pn * pw_ + dn * dn / pw_;
     const double interaction = 2 * step_size_ * e.cross;
     const double aug = step_size_ * al_penalty_ * an * an;
@@ -616,12 +673,68 @@ bool Solver::should_restart(int K)
     return r;
 }
 
+double Solver::delta_normal_motion()
+{   // ||A (xT - x0)|| before the restart copies (solver.cu:2867-2884)
+    const int grid = eval_grid_a_;
+    dispatch_group(A_.G, [&](auto g) {
+        constexpr int G = decltype(g)::value;
+        delta_spmv_sumsq_kernel<G><<<grid, kBlock, 0, stream_>>>(A_.view(), xT_.get(), x0_.get(), parts_a_.get());
+    });
+    reduce_partials_kernel<<<1, kBlock, 0, stream_>>>(parts_a_.get(), grid, 1, 1, 1, evalq_.get());
+    PDAL_LAUNCHED();
+    double ss = 0.0;
+    evalq_.download(&ss, 1, stream_);
+    PDAL_HIP(hipStreamSynchronize(stream_));
+    return std::sqrt(ss);
+}
+
 void Solver::perform_restart(const Eval &e)
-{   // solver.cu:2821-3101, restart target "current", PID primal weight
+{   // solver.cu:2821-3101, restart target "current"; PID on the primal weight, or on sigma in sigma mode
     const double pd = std::sqrt(e.rx2), dd = std::sqrt(e.ry2);
     const double ratio = rel_d_ / std::fmax(rel_p_, 1e-300);
     const restart_parameters_t &rp = P_.restart_params;
-    if (pd > 1e-16 && dd > 1e-16 && pd < 1e12 && dd < 1e12 && ratio > 1e-8 && ratio < 1e8) {
+    if (sigma_mode_) {  // :2857-2966
+        if (sigma_adaptive_) {
+            const double ax = delta_normal_motion();
+            const double dx_metric_sq = lambda_param_ * pd * pd - ax * ax;
+            if (dd > 1e-16 && dd < 1e12 && dx_metric_sq > 1e-24 && dx_metric_sq < 1e24 && std::isfinite(al_sigma_) &&
+                al_sigma_ > 1e-16 && ratio > 1e-8 && ratio < 1e8) {
+                const double gain = env_double("CP_AL_SIGMA_BALANCE_GAIN", 0.15);
+                const double smooth = env_double("CP_AL_SIGMA_I_SMOOTH", rp.i_smooth);
+                const double kp = env_double("CP_AL_SIGMA_KP", rp.k_p);
+                const double ki = env_double("CP_AL_SIGMA_KI", rp.k_i);
+                const double kd = env_double("CP_AL_SIGMA_KD", rp.k_d);
+                const double rmin = env_double("CP_AL_SIGMA_UPDATE_MIN", 0.60);
+                const double rmax = env_double("CP_AL_SIGMA_UPDATE_MAX", 1.70);
+                double err = std::log(std::fmax(dd, 1e-300)) - 0.5 * std::log(std::fmax(dx_metric_sq, 1e-300)) -
+                             std::log(std::fmax(al_sigma_, 1e-300));
+                const double bal = std::fmax(-2.0, std::fmin(0.5 * std::log(std::fmax(ratio, 1e-300)), 2.0));
+                err += gain * bal;
+                sigma_err_sum_ = smooth * sigma_err_sum_ + err;
+                const double de = err - sigma_last_err_;
+                const double r = std::fmin(std::fmax(std::exp(kp * err + ki * sigma_err_sum_ + kd * de), rmin), rmax);
+                al_sigma_ *= r;
+                sigma_last_err_ = err;
+            } else {
+                al_sigma_ = sigma_best_;
+                sigma_err_sum_ = 0.0;
+                sigma_last_err_ = 0.0;
+            }
+        }
+        if (!std::isfinite(al_sigma_) || al_sigma_ <= 1e-16) {
+            al_sigma_ = sigma_best_;
+            sigma_err_sum_ = 0.0;
+            sigma_last_err_ = 0.0;
+        }
+        al_sigma_ = std::fmin(std::fmax(al_sigma_, sigma_min_), sigma_max_);
+        const double g = std::fabs(std::log10(std::fmax(rel_d_, 1e-300) / std::fmax(rel_p_, 1e-300)));
+        if (std::isfinite(g) && g < best_gap_) {
+            best_gap_ = g;
+            sigma_best_ = al_sigma_;
+        }
+        pw_err_sum_ = 0.0;
+        pw_last_err_ = 0.0;
+    } else if (pd > 1e-16 && dd > 1e-16 && pd < 1e12 && dd < 1e12 && ratio > 1e-8 && ratio < 1e8) {
         const double err = std::log(dd) - std::log(pd) - std::log(pw_);
         pw_err_sum_ *= rp.i_smooth;
         pw_err_sum_ += err;
@@ -633,10 +746,12 @@ void Solver::perform_restart(const Eval &e)
         pw_err_sum_ = 0.0;
         pw_last_err_ = 0.0;
     }
-    const double g = std::fabs(std::log10(std::fmax(rel_d_, 1e-300) / std::fmax(rel_p_, 1e-300)));
-    if (g < best_gap_) {
-        best_gap_ = g;
-        best_pw_ = pw_;
+    if (!sigma_mode_) {
+        const double g = std::fabs(std::log10(std::fmax(rel_d_, 1e-300) / std::fmax(rel_p_, 1e-300)));
+        if (g < best_gap_) {
+            best_gap_ = g;
+            best_pw_ = pw_;
+        }
     }
     refresh_step_schedule();
     restart_copy_kernel<<<blocks_for((long)n_ + m_), kBlock, 0, stream_>>>(xT_.get(), x0_.get(), x_.get(), n_,
