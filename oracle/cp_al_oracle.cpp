@@ -20,6 +20,7 @@
  *                              primal weight, or on sigma in sigma mode :2857-2966)
  *   refresh_step_schedule .... solver.cu:3103-3126 (alpha and sigma mode)
  *   sigma parameters ......... solver.cu:3141-3164
+ *   feasibility polishing .... solver.cu:3912-4456, utils.cu:1194-1440
  *   step size / primal weight  solver.cu:3538-3609
  *   create_result_from_state . solver.cu:3828-3909
  *   power iteration .......... LP/CP_AL/src/utils.cu:87-228 (global mt19937(1), :27-28)
@@ -168,6 +169,8 @@ struct State {
     double abs_p = 0, rel_p = 0, abs_d = 0, rel_d = 0, p_obj = 0, d_obj = 0, gap = 0, rel_gap = 0;
     termination_reason_t reason = TERMINATION_REASON_UNSPECIFIED;
     double rescaling_time = 0.0, solve_time = 0.0;
+    double polish_time = 0.0;
+    int polish_iters = 0;
 };
 
 /* ---------------- preconditioner.cu ---------------- */
@@ -599,6 +602,194 @@ void perform_restart(State &s, const cp_al_parameters_t *p)  // solver.cu:2821-3
     s.last_trial_fp = INFINITY;
 }
 
+/* ---------------- feasibility polishing (solver.cu:3912-4456) ---------------- */
+
+const char *reason_name(termination_reason_t r);
+
+struct PolishOut {
+    termination_reason_t reason;
+    int total;
+    double time;
+};
+
+// primal_feasibility_polish / dual_feasibility_polish (solver.cu:3980-4127) on
+// the phase state ps; ori is the main state.
+PolishOut polish_loop(const cp_al_parameters_t *p, State &ps, const State &ori, bool primal,
+                      std::chrono::steady_clock::time_point t_start)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    const int K = p->termination_evaluation_frequency;
+    const double rho = p->reflection_coefficient;
+    const termination_criteria_t &tc = p->termination_criteria;
+    const bool linf = p->optimality_norm == NORM_TYPE_L_INF;
+    auto phase_fp = [&]() {  // compute_primal/dual_fixed_point_error, solver.cu:4421-4456
+        if (primal) {
+            for (int j = 0; j < ps.n; ++j) ps.dx[j] = ps.xR[j] - ps.xT[j];
+            const double pn = nrm2(ps.dx);
+            ps.fp = pn * pn * ps.primal_weight;
+        } else {
+            for (int i = 0; i < ps.m; ++i) ps.dy[i] = ps.yR[i] - ps.yT[i];
+            const double dn = nrm2(ps.dy);
+            ps.fp = dn * dn / ps.primal_weight;
+        }
+    };
+    PolishOut out{TERMINATION_REASON_UNSPECIFIED, 0, 0.0};
+    bool do_restart = false;
+    while (ps.reason == TERMINATION_REASON_UNSPECIFIED) {
+        iteration(ps, 1, rho, true);
+        if (do_restart) {
+            phase_fp();
+            ps.fp0 = ps.fp;
+            do_restart = false;
+        }
+        for (int i = 2; i <= K; ++i) iteration(ps, i, rho, i == K);
+        phase_fp();
+        const double bc = ps.b_scale * ps.c_scale;
+        if (primal) {  // compute_primal_feas_polish_residual, utils.cu:1341-1392
+            spmv(ps.A, ps.xT.data(), ps.prim_prod.data());
+            for (int i = 0; i < ps.m; ++i) {
+                const double cl = std::fmax(ps.l[i], std::fmin(ps.prim_prod[i], ps.u[i]));
+                ps.primal_res[i] = (ps.prim_prod[i] - cl) * ps.E[i];
+            }
+            ps.abs_p = (linf ? inf_norm(ps.primal_res) : nrm2(ps.primal_res)) / ps.b_scale;
+            ps.rel_p = ps.abs_p / (1.0 + ps.b_norm);
+            ps.p_obj = dot(ori.c, ps.xT) / bc + ps.obj_const;
+        } else {  // compute_dual_feas_polish_residual, utils.cu:1394-1440
+            spmv(ps.At, ps.yT.data(), ps.dual_prod.data());
+            for (int j = 0; j < ps.n; ++j) ps.dual_res[j] = (ps.c[j] - ps.dual_prod[j] - ps.dual_slack[j]) * ps.D[j];
+            double contrib = 0.0;
+            for (int i = 0; i < ps.m; ++i)
+                contrib += std::fmax(ps.yT[i], 0.0) * ori.l_fin[i] + std::fmin(ps.yT[i], 0.0) * ori.u_fin[i];
+            ps.abs_d = (linf ? inf_norm(ps.dual_res) : nrm2(ps.dual_res)) / ps.c_scale;
+            ps.rel_d = ps.abs_d / (1.0 + ps.c_norm);
+            ps.d_obj = (dot(ps.dual_slack, ori.xT) + contrib) / bc + ps.obj_const;
+        }
+        ps.inner_count += K;
+        ps.total_count += K;
+        out.time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        // check_feas_polishing_termination_criteria, utils.cu:1194-1227
+        if ((primal ? ps.rel_p : ps.rel_d) <= tc.eps_feas_polish_relative)
+            ps.reason = TERMINATION_REASON_FEAS_POLISH_SUCCESS;
+        else if (ps.total_count >= tc.iteration_limit)
+            ps.reason = TERMINATION_REASON_ITERATION_LIMIT;
+        else if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count() >=
+                 tc.time_sec_limit)
+            ps.reason = TERMINATION_REASON_TIME_LIMIT;
+        do_restart = should_restart(ps, p->restart_params, K);
+        if (do_restart) {  // perform_primal_restart / perform_dual_restart, solver.cu:4391-4419
+            if (primal) {
+                ps.x0 = ps.xT;
+                ps.x = ps.xT;
+            } else {
+                ps.y0 = ps.yT;
+                ps.y = ps.yT;
+            }
+            ps.inner_count = 0;
+            ps.last_trial_fp = INFINITY;
+        }
+    }
+    out.reason = ps.reason;
+    out.total = ps.total_count;
+    return out;
+}
+
+void reset_phase(State &ps, double pw0)
+{   // scalar resets of initialize_primal/dual_feas_polish_state (solver.cu:4172-4190, 4318-4336)
+    ps.pw_error_sum = 0.0;
+    ps.pw_last_error = 0.0;
+    ps.primal_weight = pw0;
+    ps.best_pw = pw0;
+    ps.fp = ps.fp0 = ps.last_trial_fp = INFINITY;
+    ps.total_count = 0;
+    ps.inner_count = 0;
+    ps.reason = TERMINATION_REASON_UNSPECIFIED;
+    ps.best_gap = INFINITY;
+}
+
+void feasibility_polish(const cp_al_parameters_t *p, State &s, std::chrono::steady_clock::time_point t_start)
+{   // solver.cu:3912-3978
+    const auto t0 = std::chrono::steady_clock::now();
+    const termination_criteria_t &tc = p->termination_criteria;
+    if (s.rel_p < tc.eps_feas_polish_relative && s.rel_d < tc.eps_feas_polish_relative) {
+        printf("Skipping feasibility polishing as the solution is already sufficiently feasible.\n");
+        fflush(stdout);
+        return;
+    }
+    const double pw0 = p->bound_objective_rescaling ? 1.0 : (s.c_norm + 1.0) / (s.b_norm + 1.0);
+
+    // primal phase (initialize_primal_feas_polish_state, solver.cu:4129-4190): c = 0, constant 0,
+    // dual iterates and slacks zero, primal iterates from the main state
+    State ps = s;
+    std::fill(ps.c.begin(), ps.c.end(), 0.0);
+    ps.obj_const = 0.0;
+    for (auto *v : {&ps.y0, &ps.y, &ps.yT, &ps.yH, &ps.yR, &ps.primal_slack, &ps.primal_res, &ps.dy})
+        std::fill(v->begin(), v->end(), 0.0);
+    for (auto *v : {&ps.dual_prod, &ps.dual_slack, &ps.dual_res, &ps.dx}) std::fill(v->begin(), v->end(), 0.0);
+    reset_phase(ps, pw0);
+    ps.rel_d = ps.abs_d = ps.rel_gap = ps.gap = 0.0;
+    const PolishOut pr = polish_loop(p, ps, s, true, t_start);
+    if (pr.reason == TERMINATION_REASON_FEAS_POLISH_SUCCESS) {
+        s.xT = ps.xT;
+        s.abs_p = ps.abs_p;
+        s.rel_p = ps.rel_p;
+        s.p_obj = ps.p_obj;
+    }
+    s.polish_iters += ps.total_count - 1;
+
+    // dual phase (initialize_dual_feas_polish_state, solver.cu:4209-4337): finite bounds 0,
+    // primal iterates zero, dual iterates and slack from the main state
+    State ds = s;
+    for (auto *v : {&ds.l, &ds.u, &ds.lb, &ds.ub})
+        for (double &b : *v)
+            if (std::isfinite(b)) b = 0.0;
+    std::fill(ds.l_fin.begin(), ds.l_fin.end(), 0.0);
+    std::fill(ds.u_fin.begin(), ds.u_fin.end(), 0.0);
+    for (auto *v : {&ds.x0, &ds.x, &ds.xT, &ds.xH, &ds.xR, &ds.dual_res, &ds.dx}) std::fill(v->begin(), v->end(), 0.0);
+    for (auto *v : {&ds.prim_prod, &ds.primal_slack, &ds.primal_res, &ds.dy}) std::fill(v->begin(), v->end(), 0.0);
+    reset_phase(ds, pw0);
+    ds.rel_p = ds.abs_p = ds.rel_gap = ds.gap = 0.0;
+    const PolishOut du = polish_loop(p, ds, s, false, t_start);
+    if (du.reason == TERMINATION_REASON_FEAS_POLISH_SUCCESS) {
+        s.yT = ds.yT;
+        s.abs_d = ds.abs_d;
+        s.rel_d = ds.rel_d;
+        s.d_obj = ds.d_obj;
+    }
+    s.polish_iters += ds.total_count - 1;
+    s.gap = std::fabs(s.p_obj - s.d_obj);
+    s.rel_gap = s.gap / (1.0 + std::fabs(s.p_obj) + std::fabs(s.d_obj));
+    // cp_al_feas_polish_final_log, utils.cu:1247-1268
+    if (p->verbose) printf("---------------------------------------------------------------------------------------\n");
+    printf("Feasibility Polishing Summary\n");
+    printf("  Primal Status        : %s\n", reason_name(pr.reason));
+    printf("  Primal Iterations    : %d\n", pr.total);
+    printf("  Primal Time Usage    : %.3g sec\n", pr.time);
+    printf("  Dual Status          : %s\n", reason_name(du.reason));
+    printf("  Dual Iterations      : %d\n", du.total);
+    printf("  Dual Time Usage      : %.3g sec\n", du.time);
+    printf("  Primal Residual      : %.3e\n", ps.rel_p);
+    printf("  Dual Residual        : %.3e\n", ds.rel_d);
+    printf("  Primal Dual Gap      : %.3e\n",
+           std::fabs(ps.p_obj - ds.d_obj) / (1.0 + std::fabs(ps.p_obj) + std::fabs(ds.d_obj)));
+    fflush(stdout);
+    s.polish_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+const char *reason_name(termination_reason_t r)
+{
+    switch (r) {
+    case TERMINATION_REASON_OPTIMAL: return "OPTIMAL";
+    case TERMINATION_REASON_PRIMAL_INFEASIBLE: return "PRIMAL_INFEASIBLE";
+    case TERMINATION_REASON_DUAL_INFEASIBLE: return "DUAL_INFEASIBLE";
+    case TERMINATION_REASON_INFEASIBLE_OR_UNBOUNDED: return "INFEASIBLE_OR_UNBOUNDED";
+    case TERMINATION_REASON_TIME_LIMIT: return "TIME_LIMIT";
+    case TERMINATION_REASON_ITERATION_LIMIT: return "ITERATION_LIMIT";
+    case TERMINATION_REASON_UNSPECIFIED: return "UNSPECIFIED";
+    case TERMINATION_REASON_FEAS_POLISH_SUCCESS: return "FEAS_POLISH_SUCCESS";
+    default: return "UNKNOWN";
+    }
+}
+
 void filter_matrix(lp_problem_t *pr, const cp_al_parameters_t *p)  // utils.cu:396-478
 {
     const int rows = pr->num_constraints, nnz = pr->constraint_matrix_num_nonzeros;
@@ -961,6 +1152,9 @@ ORACLE_API cp_al_result_t *optimize(const cp_al_parameters_t *params, lp_problem
         s.reason = TERMINATION_REASON_ITERATION_LIMIT;
         compute_residual(s, params->optimality_norm);
     }
+    if (params->feasibility_polishing && s.reason != TERMINATION_REASON_DUAL_INFEASIBLE &&
+        s.reason != TERMINATION_REASON_PRIMAL_INFEASIBLE)  // solver.cu:1715-1719
+        feasibility_polish(params, s, t_start);
 
     // create_result_from_state, solver.cu:3828-3909
     cp_al_result_t *r = (cp_al_result_t *)calloc(1, sizeof(cp_al_result_t));
@@ -991,6 +1185,8 @@ ORACLE_API cp_al_result_t *optimize(const cp_al_parameters_t *params, lp_problem
     r->objective_gap = s.gap;
     r->relative_objective_gap = s.rel_gap;
     r->termination_reason = s.reason;
+    r->feasibility_polishing_time = s.polish_time;
+    r->feasibility_iteration = s.polish_iters;
     return r;
 }
 

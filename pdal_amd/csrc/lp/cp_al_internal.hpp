@@ -3,6 +3,8 @@
 
 #include "cp_al_types.h"
 
+extern "C" const char *termination_reason_to_string(termination_reason_t reason);
+
 namespace pdal {
 namespace lp {
 

@@ -586,6 +586,14 @@ static __global__ void __launch_bounds__(kBlock)
     }
 }
 
+// Finite entries -> 0, infinite ones kept: the dual polishing problem's
+// bounds (zero_finite_value_vectors_kernel, solver.cu:4199-4207).
+static __global__ void __launch_bounds__(kBlock) zero_finite_kernel(double *__restrict__ v, int len)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < len && isfinite(v[i])) v[i] = 0.0;
+}
+
 static __global__ void __launch_bounds__(kBlock) fill_kernel(double *__restrict__ x, double v, int len)
 {
     const int i = blockIdx.x * kBlock + threadIdx.x;

@@ -49,6 +49,7 @@ class Solver {
     ~Solver()
     {
         if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+        if (polish_graph_) (void)hipGraphExecDestroy(polish_graph_);
         if (stream_) (void)hipStreamDestroy(stream_);
     }
     cp_al_result_t *run(const lp_problem_t *original);
@@ -83,6 +84,15 @@ class Solver {
     double delta_normal_motion();
     cp_al_result_t *make_result(const lp_problem_t *original, int charged);
     void display() const;
+    // feasibility polishing (solver.cu:3912-4456)
+    struct PolishOut {
+        termination_reason_t reason;
+        int total;
+        double abs, rel, obj, time;
+    };
+    void feasibility_polish(std::chrono::steady_clock::time_point t_start);
+    PolishOut polish_phase(bool primal, double pw0, std::chrono::steady_clock::time_point t_start);
+    void launch_polish_block(int K);
 
     cp_al_parameters_t P_;
     const lp_problem_t *W_;
@@ -119,6 +129,16 @@ class Solver {
     double rescale_time_ = 0.0, solve_time_ = 0.0, setup_time_ = 0.0;
     double pw_init_ = 1.0;
     Eval last_eval_{};
+    // Problem data the iteration kernels read and the vectors the evaluation
+    // reads; the feasibility-polishing phases point them at their own copies.
+    struct View {
+        const double *c, *lb, *ub, *l, *u;
+    };
+    View iter_view_{}, eval_view_{};
+    const double *eval_x_ = nullptr;  // x_T of the objective products (the polished x in the dual phase)
+    hipGraphExec_t polish_graph_ = nullptr;
+    double polish_time_ = 0.0;
+    int polish_iters_ = 0;
 };
 
 void Solver::upload()
@@ -478,6 +498,8 @@ void Solver::setup()
     }
     b_norm_ = linf ? mx : std::sqrt(sq);
     PDAL_HIP(hipStreamSynchronize(stream_));
+    iter_view_ = eval_view_ = View{c_.get(), lb_.get(), ub_.get(), l_.get(), u_.get()};
+    eval_x_ = xT_.get();
     init_steps();
     pw_init_ = pw_;
     setup_time_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -527,21 +549,22 @@ void Solver::launch_iteration(int k_offset, bool major)
 {
     const double rho = P_.reflection_coefficient;
     const bool pred = al_penalty_ > 0.0;
+    const View &v = iter_view_;
     if (pred) {
-        PredictorArgs a{A_.view(), x_.get(), y_.get(), l_.get(), u_.get(), yR_.get(), sc_.get()};
+        PredictorArgs a{A_.view(), x_.get(), y_.get(), v.l, v.u, yR_.get(), sc_.get()};
         dispatch_group(A_.G, [&](auto g) {
             constexpr int G = decltype(g)::value;
             predictor_kernel<G><<<blocks_for((long)m_ * G), kBlock, 0, stream_>>>(a);
         });
     }
-    PrimalArgs pa{At_.view(), pred ? yR_.get() : y_.get(), c_.get(), lb_.get(), ub_.get(), x0_.get(), x_.get(),
+    PrimalArgs pa{At_.view(), pred ? yR_.get() : y_.get(), v.c, v.lb, v.ub, x0_.get(), x_.get(),
                   xR_.get(), xT_.get(), xH_.get(), dslack_.get(), sc_.get(), rho};
     dispatch_group(At_.G, [&](auto g) {
         constexpr int G = decltype(g)::value;
         if (major) primal_kernel<G, true><<<blocks_for((long)n_ * G), kBlock, 0, stream_>>>(pa, k_offset);
         else primal_kernel<G, false><<<blocks_for((long)n_ * G), kBlock, 0, stream_>>>(pa, k_offset);
     });
-    DualArgs da{A_.view(), xR_.get(), l_.get(), u_.get(), y0_.get(), y_.get(), yT_.get(), yH_.get(), yR_.get(),
+    DualArgs da{A_.view(), xR_.get(), v.l, v.u, y0_.get(), y_.get(), yT_.get(), yH_.get(), yR_.get(),
                 sc_.get(), rho};
     dispatch_group(A_.G, [&](auto g) {
         constexpr int G = decltype(g)::value;
@@ -553,13 +576,14 @@ void Solver::launch_iteration(int k_offset, bool major)
 
 void Solver::launch_eval()
 {
-    EvalAtArgs ea{At_.view(), yR_.get(), yT_.get(), xR_.get(), xT_.get(), x0_.get(), c_.get(), dslack_.get(),
+    const View &v = eval_view_;
+    EvalAtArgs ea{At_.view(), yR_.get(), yT_.get(), xR_.get(), eval_x_, x0_.get(), v.c, dslack_.get(),
                   D_.get(), parts_at_.get()};
     dispatch_group(At_.G, [&](auto g) {
         constexpr int G = decltype(g)::value;
         eval_at_kernel<G><<<eval_grid_at_, kBlock, 0, stream_>>>(ea);
     });
-    EvalAArgs eb{A_.view(), xR_.get(), xT_.get(), yR_.get(), yT_.get(), y0_.get(), l_.get(), u_.get(), E_.get(),
+    EvalAArgs eb{A_.view(), xR_.get(), xT_.get(), yR_.get(), yT_.get(), y0_.get(), v.l, v.u, E_.get(),
                  parts_a_.get()};
     dispatch_group(A_.G, [&](auto g) {
         constexpr int G = decltype(g)::value;
@@ -814,6 +838,8 @@ cp_al_result_t *Solver::make_result(const lp_problem_t *original, int charged)
     r->objective_gap = gap_;
     r->relative_objective_gap = rel_gap_;
     r->termination_reason = reason_;
+    r->feasibility_polishing_time = polish_time_;
+    r->feasibility_iteration = polish_iters_;
     return r;
 }
 
@@ -876,6 +902,11 @@ cp_al_result_t *Solver::solve_once(const lp_problem_t *original)
         residual(last_eval_);
         display();
     }
+    polish_time_ = 0.0;
+    polish_iters_ = 0;
+    if (P_.feasibility_polishing && reason_ != TERMINATION_REASON_DUAL_INFEASIBLE &&
+        reason_ != TERMINATION_REASON_PRIMAL_INFEASIBLE)  // :1715-1719
+        feasibility_polish(t_start);
     return make_result(original, charged);
 }
 
@@ -946,6 +977,217 @@ void reset_rng()
 {
     g_gen.seed(1);
     g_dist.reset();
+}
+
+// ---------------------------------------------------------------------------
+// Feasibility polishing (solver.cu:3912-4456, utils.cu:1194-1440).  Two
+// phases after the main solve, each a restarted Halpern run of the same
+// kernels on a feasibility problem: the primal phase with c = 0 (dual iterates
+// from zero), the dual phase with every finite bound set to 0 (primal iterates
+// from zero); each keeps the primal weight fixed, restarts only its own side
+// and stops when its relative residual reaches eps_feas_polish_relative.
+
+void Solver::launch_polish_block(int K)
+{   // iterations 2..K (K major) + the evaluation pass, captured once per phase
+    auto body = [&]() {
+        for (int i = 2; i <= K; ++i) launch_iteration(i, i == K);
+        launch_eval();
+    };
+    if (!use_graph_) {
+        body();
+        return;
+    }
+    if (!polish_graph_) {
+        hipGraph_t graph;
+        PDAL_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeGlobal));
+        body();
+        PDAL_HIP(hipStreamEndCapture(stream_, &graph));
+        PDAL_HIP(hipGraphInstantiate(&polish_graph_, graph, nullptr, nullptr, 0));
+        PDAL_HIP(hipGraphDestroy(graph));
+    }
+    PDAL_HIP(hipGraphLaunch(polish_graph_, stream_));
+}
+
+Solver::PolishOut Solver::polish_phase(bool primal, double pw0, std::chrono::steady_clock::time_point t_start)
+{   // primal_feasibility_polish / dual_feasibility_polish, solver.cu:3980-4127
+    const auto t0 = std::chrono::steady_clock::now();
+    const termination_criteria_t &tc = P_.termination_criteria;
+    const int K = P_.termination_evaluation_frequency;
+    const bool linf = P_.optimality_norm == NORM_TYPE_L_INF;
+    pw_ = pw0;
+    best_pw_ = pw0;
+    total_ = 0;
+    inner_ = 0;
+    fp_ = fp0_ = last_trial_fp_ = INFINITY;
+    if (polish_graph_) {
+        PDAL_HIP(hipGraphExecDestroy(polish_graph_));
+        polish_graph_ = nullptr;
+    }
+    if (P_.verbose) {  // print_initial_feas_polish_info, utils.cu:1229-1245
+        printf("---------------------------------------------------------------------------------------\n");
+        printf("Starting %s Feasibility Polishing Phase with relative tolerance %.2e\n", primal ? "Primal" : "Dual",
+               tc.eps_feas_polish_relative);
+        printf("---------------------------------------------------------------------------------------\n");
+        printf("%s %s |  %s  | %s | %s \n", "  iter", "  time ", primal ? "pr obj" : "du obj",
+               primal ? " abs pr res " : " abs du res ", primal ? " rel pr res " : " rel du res ");
+        printf("---------------------------------------------------------------------------------------\n");
+    }
+    PolishOut out{TERMINATION_REASON_UNSPECIFIED, 0, 0.0, 0.0, 0.0, 0.0};
+    auto phase_fp = [&](const Eval &e) { return primal ? e.pn2 * pw_ : e.dn2 / pw_; };
+    bool do_restart = false;
+    while (out.reason == TERMINATION_REASON_UNSPECIFIED) {
+        sync_scalars();
+        launch_iteration(1, true);
+        if (do_restart) {  // fixed-point error after the block's first iteration
+            launch_eval();
+            fp_ = phase_fp(read_eval());
+            fp0_ = fp_;
+            do_restart = false;
+        }
+        launch_polish_block(K);
+        const Eval e = read_eval();
+        fp_ = phase_fp(e);
+        const double bc = b_scale_ * c_scale_;
+        if (primal) {  // compute_primal_feas_polish_residual, utils.cu:1341-1392
+            out.abs = (linf ? e.presmax : std::sqrt(e.pres2)) / b_scale_;
+            out.rel = out.abs / (1.0 + b_norm_);
+            out.obj = e.cx / bc;  // the phase's objective constant is 0
+        } else {  // compute_dual_feas_polish_residual, utils.cu:1394-1440
+            out.abs = (linf ? e.dresmax : std::sqrt(e.dres2)) / c_scale_;
+            out.rel = out.abs / (1.0 + c_norm_);
+            out.obj = (e.lx + e.contrib) / bc + obj_const_;
+        }
+        inner_ += K;
+        total_ += K;
+        out.time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        // check_feas_polishing_termination_criteria, utils.cu:1194-1227
+        if (out.rel <= tc.eps_feas_polish_relative)
+            out.reason = TERMINATION_REASON_FEAS_POLISH_SUCCESS;
+        else if (total_ >= tc.iteration_limit)
+            out.reason = TERMINATION_REASON_ITERATION_LIMIT;
+        else if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count() >=
+                 tc.time_sec_limit)
+            out.reason = TERMINATION_REASON_TIME_LIMIT;
+        if (P_.verbose && total_ % print_frequency(total_) == 0)
+            printf("%6d %.1e | %8.1e |    %.1e   |   %.1e   \n", total_, out.time, out.obj, out.abs, out.rel);
+        do_restart = should_restart(K);
+        if (do_restart) {  // perform_primal_restart / perform_dual_restart: anchor = current = T, own side only
+            const size_t bytes = (size_t)(primal ? n_ : m_) * sizeof(double);
+            double *T = primal ? xT_.get() : yT_.get();
+            if (bytes) {
+                PDAL_HIP(hipMemcpyAsync(primal ? x0_.get() : y0_.get(), T, bytes, hipMemcpyDeviceToDevice, stream_));
+                PDAL_HIP(hipMemcpyAsync(primal ? x_.get() : y_.get(), T, bytes, hipMemcpyDeviceToDevice, stream_));
+            }
+            inner_ = 0;
+            last_trial_fp_ = INFINITY;
+        }
+    }
+    out.total = total_;
+    if (polish_graph_) {
+        PDAL_HIP(hipGraphExecDestroy(polish_graph_));
+        polish_graph_ = nullptr;
+    }
+    return out;
+}
+
+void Solver::feasibility_polish(std::chrono::steady_clock::time_point t_start)
+{   // feasibility_polish, solver.cu:3912-3978
+    const auto t0 = std::chrono::steady_clock::now();
+    const termination_criteria_t &tc = P_.termination_criteria;
+    if (rel_p_ < tc.eps_feas_polish_relative && rel_d_ < tc.eps_feas_polish_relative) {
+        printf("Skipping feasibility polishing as the solution is already sufficiently feasible.\n");
+        fflush(stdout);
+        return;
+    }
+    const double pw0 = P_.bound_objective_rescaling ? 1.0 : (c_norm_ + 1.0) / (b_norm_ + 1.0);
+    if (graph_exec_) {  // the main block's graph holds the main views
+        PDAL_HIP(hipGraphExecDestroy(graph_exec_));
+        graph_exec_ = nullptr;
+    }
+    auto copy = [&](double *dst, const double *src, int len) {
+        if (len > 0)
+            PDAL_HIP(hipMemcpyAsync(dst, src, (size_t)len * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    };
+    auto zero = [&](double *v, int len) {
+        if (len > 0) fill_kernel<<<blocks_for(len), kBlock, 0, stream_>>>(v, 0.0, len);
+    };
+    // the main state's vectors the two phases start from or fall back to
+    DBuf<double> sx(n_), sdslack(n_), sy0(m_), sy(m_), syT(m_), syH(m_), syR(m_);
+    copy(sx.get(), xT_.get(), n_);
+    copy(sdslack.get(), dslack_.get(), n_);
+    copy(sy0.get(), y0_.get(), m_);
+    copy(sy.get(), y_.get(), m_);
+    copy(syT.get(), yT_.get(), m_);
+    copy(syH.get(), yH_.get(), m_);
+    copy(syR.get(), yR_.get(), m_);
+    // the phases' problem data: c = 0; finite bounds = 0
+    DBuf<double> zc(n_), zlb(n_), zub(n_), zl(m_), zu(m_);
+    zero(zc.get(), n_);
+    copy(zlb.get(), lb_.get(), n_);
+    copy(zub.get(), ub_.get(), n_);
+    copy(zl.get(), l_.get(), m_);
+    copy(zu.get(), u_.get(), m_);
+    for (auto [p, len] : {std::pair<double *, int>{zlb.get(), n_}, {zub.get(), n_}, {zl.get(), m_}, {zu.get(), m_}})
+        if (len > 0) zero_finite_kernel<<<blocks_for(len), kBlock, 0, stream_>>>(p, len);
+    PDAL_LAUNCHED();
+
+    // primal phase: c = 0, dual iterates from zero, primal iterates continue
+    for (double *v : {y0_.get(), y_.get(), yT_.get(), yH_.get(), yR_.get()}) zero(v, m_);
+    zero(dslack_.get(), n_);
+    iter_view_ = View{zc.get(), lb_.get(), ub_.get(), l_.get(), u_.get()};
+    eval_view_ = View{c_.get(), lb_.get(), ub_.get(), l_.get(), u_.get()};
+    eval_x_ = xT_.get();
+    const PolishOut pr = polish_phase(true, pw0, t_start);
+    if (pr.reason == TERMINATION_REASON_FEAS_POLISH_SUCCESS) {
+        copy(sx.get(), xT_.get(), n_);
+        abs_p_ = pr.abs;
+        rel_p_ = pr.rel;
+        p_obj_ = pr.obj;
+    }
+    polish_iters_ += pr.total - 1;
+
+    // dual phase: finite bounds = 0, dual iterates from the main state's, primal from zero
+    copy(y0_.get(), sy0.get(), m_);
+    copy(y_.get(), sy.get(), m_);
+    copy(yT_.get(), syT.get(), m_);
+    copy(yH_.get(), syH.get(), m_);
+    copy(yR_.get(), syR.get(), m_);
+    copy(dslack_.get(), sdslack.get(), n_);
+    for (double *v : {x0_.get(), x_.get(), xT_.get(), xH_.get(), xR_.get()}) zero(v, n_);
+    iter_view_ = View{c_.get(), zlb.get(), zub.get(), zl.get(), zu.get()};
+    eval_view_ = View{c_.get(), lb_.get(), ub_.get(), l_.get(), u_.get()};
+    eval_x_ = sx.get();  // the dual objective pairs the slack with the (polished) primal solution
+    const PolishOut du = polish_phase(false, pw0, t_start);
+    if (du.reason == TERMINATION_REASON_FEAS_POLISH_SUCCESS) {
+        abs_d_ = du.abs;
+        rel_d_ = du.rel;
+        d_obj_ = du.obj;
+    } else {
+        copy(yT_.get(), syT.get(), m_);
+    }
+    polish_iters_ += du.total - 1;
+    copy(xT_.get(), sx.get(), n_);  // the result's x
+    gap_ = std::fabs(p_obj_ - d_obj_);
+    rel_gap_ = gap_ / (1.0 + std::fabs(p_obj_) + std::fabs(d_obj_));
+
+    iter_view_ = eval_view_ = View{c_.get(), lb_.get(), ub_.get(), l_.get(), u_.get()};
+    eval_x_ = xT_.get();
+    PDAL_HIP(hipStreamSynchronize(stream_));
+    // cp_al_feas_polish_final_log, utils.cu:1247-1268
+    if (P_.verbose) printf("---------------------------------------------------------------------------------------\n");
+    printf("Feasibility Polishing Summary\n");
+    printf("  Primal Status        : %s\n", termination_reason_to_string(pr.reason));
+    printf("  Primal Iterations    : %d\n", pr.total);
+    printf("  Primal Time Usage    : %.3g sec\n", pr.time);
+    printf("  Dual Status          : %s\n", termination_reason_to_string(du.reason));
+    printf("  Dual Iterations      : %d\n", du.total);
+    printf("  Dual Time Usage      : %.3g sec\n", du.time);
+    printf("  Primal Residual      : %.3e\n", pr.rel);
+    printf("  Dual Residual        : %.3e\n", du.rel);
+    printf("  Primal Dual Gap      : %.3e\n",
+           std::fabs(pr.obj - du.obj) / (1.0 + std::fabs(pr.obj) + std::fabs(du.obj)));
+    fflush(stdout);
+    polish_time_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
 cp_al_result_t *optimize_gpu(const cp_al_parameters_t *params, lp_problem_t *problem)

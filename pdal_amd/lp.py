@@ -343,6 +343,8 @@ class CpAl:
                 objective_gap=r.objective_gap,
                 rescaling_time_sec=r.rescaling_time_sec,
                 solve_time_sec=r.cumulative_time_sec,
+                extra={"feasibility_polishing_time": r.feasibility_polishing_time,
+                       "feasibility_iteration": r.feasibility_iteration},
             )
         finally:
             self.lib.cp_al_result_free(res)
