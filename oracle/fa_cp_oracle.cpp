@@ -19,10 +19,11 @@
  *   Halpern, restart, sigma rules,
  *   fixed-point error, result ........... src/fa_cp_core_op.cu:93-1239
  *   problem construction, conversions ... src/fa_cp.c:28-376, src/utils.cu:1135-1358
+ *   feasibility polishing ............... src/fa_cp_core_op.cu:1242-1735, src/utils.cu:1360-1575
  *
  * Deliberate differences shared with the HIP path: wall-clock timing, no
- * trajectory CSV, no feasibility polishing, restart-select env knobs ignored,
- * errors return NULL instead of exit().
+ * trajectory CSV, restart-select env knobs ignored, errors return NULL
+ * instead of exit().
  *
  * Built a second time with -DFA_CP_ORACLE_REVERSE_SUMS, every SpMV row, dot
  * product and norm sums its terms in reverse order.  That variant is the same
@@ -816,6 +817,170 @@ void perform_restart(S &s)
     s.last_trial_fp = INFINITY;
 }
 
+/* ---------------- feasibility polishing (fa_cp_core_op.cu:1242-1735) ---------------- */
+
+struct PolishOut {
+    termination_reason_t reason;
+    int total;
+    double time;
+};
+
+void polish_residual(S &ps, const S &ori, bool primal)
+{   // compute_primal_feas_polish_residual / compute_dual_feas_polish_residual, utils.cu:1491-1575
+    const bool linf = ps.P.optimality_norm == NORM_TYPE_L_INF;
+    const double bc = ps.b_scale * ps.c_scale;
+    if (primal) {
+        spmv(ps.A, ps.xT.data(), ps.prim_prod.data());
+        for (int i = 0; i < ps.m; ++i) {
+            const double cl = std::fmax(ps.l[i], std::fmin(ps.prim_prod[i], ps.u[i]));
+            ps.pres[i] = (ps.prim_prod[i] - cl) * ps.E[i];
+        }
+        ps.abs_p = (linf ? inf_norm(ps.pres) : nrm2(ps.pres)) / ps.b_scale;
+        ps.rel_p = ps.abs_p / (1.0 + ps.b_norm);
+        ps.p_obj = dot(ori.c, ps.xT) / bc + ps.obj_const;
+    } else {  // no Q x term: the reference's dual polish residual is c - A^T y - slack
+        spmv(ps.At, ps.yT.data(), ps.dual_prod.data());
+        for (int j = 0; j < ps.n; ++j) ps.dres[j] = (ps.c[j] - ps.dual_prod[j] - ps.dslack[j]) * ps.D[j];
+        for (int i = 0; i < ps.m; ++i)
+            ps.pslack[i] = std::fmax(ps.yT[i], 0.0) * ori.lfin[i] + std::fmin(ps.yT[i], 0.0) * ori.ufin[i];
+        ps.abs_d = (linf ? inf_norm(ps.dres) : nrm2(ps.dres)) / ps.c_scale;
+        ps.rel_d = ps.abs_d / (1.0 + ps.c_norm);
+        ps.d_obj = (dot(ps.dslack, ori.xT) + sum(ps.pslack)) / bc + ps.obj_const;
+    }
+}
+
+PolishOut polish_loop(S &ps, const S &ori, bool primal)
+{   // primal_feasibility_polish / dual_feasibility_polish, fa_cp_core_op.cu:1312-1404
+    const auto t0 = std::chrono::steady_clock::now();
+    const int K = ps.P.termination_evaluation_frequency;
+    const auto &tc = ps.P.termination_criteria;
+    PolishOut out{TERMINATION_REASON_UNSPECIFIED, 0, 0.0};
+    bool do_restart = false;
+    while (ps.reason == TERMINATION_REASON_UNSPECIFIED) {
+        const bool consider = ps.major || ps.total == 0;
+        if (consider || ps.total % print_frequency(ps.total) == 0) {
+            polish_residual(ps, ori, primal);
+            out.time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            // check_feas_polishing_termination_criteria, utils.cu:1360-1384 (the phase's own clock)
+            if ((primal ? ps.rel_p : ps.rel_d) <= tc.eps_feas_polish_relative)
+                ps.reason = TERMINATION_REASON_FEAS_POLISH_SUCCESS;
+            else if (ps.total >= tc.iteration_limit)
+                ps.reason = TERMINATION_REASON_ITERATION_LIMIT;
+            else if (out.time >= tc.time_sec_limit)
+                ps.reason = TERMINATION_REASON_TIME_LIMIT;
+        }
+        if (consider) {  // also after a stop, as the reference loop runs it
+            do_restart = should_restart(ps);
+            if (do_restart) {  // perform_primal_restart / perform_dual_restart, :1668-1690
+                if (primal) {
+                    ps.x0 = ps.xT;
+                    ps.x = ps.xT;
+                } else {
+                    ps.y0 = ps.yT;
+                    ps.y = ps.yT;
+                }
+                ps.inner = 0;
+                ps.last_trial_fp = INFINITY;
+            }
+        }
+        ps.major = ((ps.total + 1) % K) == 0;
+        fa_cp_update(ps);
+        if (ps.major || do_restart) {  // compute_primal/dual_fixed_point_error, :1714-1735
+            if (primal) {
+                for (int i = 0; i < ps.n; ++i) ps.dx[i] = ps.xR[i] - ps.x[i];
+                const double pn = nrm2(ps.dx);
+                ps.fp = pn * pn * ps.pw;
+            } else {
+                for (int i = 0; i < ps.m; ++i) ps.dy[i] = ps.yR[i] - ps.y[i];
+                const double dn = nrm2(ps.dy);
+                ps.fp = dn * dn / ps.pw;
+            }
+            if (do_restart) {
+                ps.fp0 = ps.fp;
+                do_restart = false;
+            }
+        }
+        halpern(ps);
+        ps.inner++;
+        ps.total++;
+    }
+    out.reason = ps.reason;
+    out.total = ps.total;
+    return out;
+}
+
+void reset_phase(S &ps, double pw0)
+{   // scalar resets of initialize_primal/dual_feas_polish_state, :1474-1494, 1626-1646
+    ps.pw_err_sum = ps.pw_last_err = 0.0;
+    ps.pw = ps.best_pw = pw0;
+    ps.fp = ps.fp0 = ps.last_trial_fp = INFINITY;
+    ps.major = false;
+    ps.total = ps.inner = 0;
+    ps.reason = TERMINATION_REASON_UNSPECIFIED;
+    ps.best_gap = INFINITY;
+}
+
+void feasibility_polish(S &s, double &polish_time, int &polish_iters)
+{   // feasibility_polish, fa_cp_core_op.cu:1242-1310
+    const auto t0 = std::chrono::steady_clock::now();
+    const auto &tc = s.P.termination_criteria;
+    if (s.rel_p < tc.eps_feas_polish_relative && s.rel_d < tc.eps_feas_polish_relative) {
+        printf("Skipping feasibility polishing as the solution is already sufficiently feasible.\n");
+        fflush(stdout);
+        return;
+    }
+    const double pw0 = s.P.bound_objective_rescaling ? 1.0 : (s.c_norm + 1.0) / (s.b_norm + 1.0);
+    auto zero = [](Vec &v) { std::fill(v.begin(), v.end(), 0.0); };
+
+    // primal phase: c = 0, constant 0, dual side zero, primal side from the main state
+    S ps = s;
+    zero(ps.c);
+    ps.obj_const = 0.0;
+    for (Vec *v : {&ps.y0, &ps.y, &ps.yT, &ps.yR, &ps.al_u, &ps.pslack, &ps.pres, &ps.dy, &ps.dual_prod, &ps.dslack,
+                   &ps.dres, &ps.dx})
+        zero(*v);
+    reset_phase(ps, pw0);
+    ps.rel_d = ps.abs_d = ps.rel_gap = ps.gap = 0.0;
+    const PolishOut pr = polish_loop(ps, s, true);
+    s.inner_total = ps.inner_total;  // the inner solver is shared with the phases
+    if (pr.reason == TERMINATION_REASON_FEAS_POLISH_SUCCESS) {
+        s.xT = ps.xT;
+        s.abs_p = ps.abs_p;
+        s.rel_p = ps.rel_p;
+        s.p_obj = ps.p_obj;
+    }
+    polish_iters += ps.total - 1;
+
+    // dual phase: finite bounds 0, primal side zero, dual side from the main state
+    S ds = s;
+    for (Vec *v : {&ds.l, &ds.u, &ds.lb, &ds.ub})
+        for (double &b : *v)
+            if (std::isfinite(b)) b = 0.0;
+    zero(ds.lfin);
+    zero(ds.ufin);
+    for (Vec *v : {&ds.x0, &ds.x, &ds.xT, &ds.xR, &ds.dx, &ds.dres, &ds.prim_prod, &ds.al_u, &ds.pslack, &ds.pres,
+                   &ds.dy})
+        zero(*v);
+    reset_phase(ds, pw0);
+    ds.rel_p = ds.abs_p = ds.rel_gap = ds.gap = 0.0;
+    const PolishOut du = polish_loop(ds, s, false);
+    s.inner_total = ds.inner_total;
+    if (du.reason == TERMINATION_REASON_FEAS_POLISH_SUCCESS) {
+        s.yT = ds.yT;
+        s.abs_d = ds.abs_d;
+        s.rel_d = ds.rel_d;
+        s.d_obj = ds.d_obj;
+    }
+    polish_iters += ds.total - 1;
+    s.gap = std::fabs(s.p_obj - s.d_obj);
+    s.rel_gap = s.gap / (1.0 + std::fabs(s.p_obj) + std::fabs(s.d_obj));
+    printf("Feasibility Polishing Summary\n");
+    printf("  Primal Status        : %d\n  Primal Iterations    : %d\n", (int)pr.reason, pr.total);
+    printf("  Dual Status          : %d\n  Dual Iterations      : %d\n", (int)du.reason, du.total);
+    fflush(stdout);
+    polish_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
 void init_state(S &s, const qp_problem_t *w)
 {   // solver_state.cu:494-877 after rescale_problem (preconditioner.c)
     s.n = w->num_variables;
@@ -1190,6 +1355,11 @@ ORACLE_API fa_cp_result_t *optimize(const fa_cp_parameters_t *params, const qp_p
         s.reason = TERMINATION_REASON_ITERATION_LIMIT;
         compute_residual(s, s.P.optimality_norm);
     }
+    double polish_time = 0.0;
+    int polish_iters = 0;
+    if (s.P.feasibility_polishing && s.reason != TERMINATION_REASON_DUAL_INFEASIBLE &&
+        s.reason != TERMINATION_REASON_PRIMAL_INFEASIBLE)  // solver.cu:156-160
+        feasibility_polish(s, polish_time, polish_iters);
     // create_result_from_state, :1171-1239
     fa_cp_result_t *r = (fa_cp_result_t *)calloc(1, sizeof(fa_cp_result_t));
     spmv(s.At, s.yT.data(), s.dual_prod.data());
@@ -1224,6 +1394,8 @@ ORACLE_API fa_cp_result_t *optimize(const fa_cp_parameters_t *params, const qp_p
     r->sigma_update_count = s.sigma_update_count;
     r->sigma_update_rule = s.al_available ? s.P.sigma_update_rule : FA_CP_SIGMA_UPDATE_FIXED;
     r->termination_reason = s.reason;
+    r->feasibility_polishing_time = polish_time;
+    r->feasibility_iteration = polish_iters;
     return r;
 }
 

@@ -321,8 +321,6 @@ PDAL_API void fa_cp_result_free(fa_cp_result_t *r)
 PDAL_API fa_cp_result_t *optimize(const fa_cp_parameters_t *params, const qp_problem_t *prob)
 {   // solver.cu:74-168.  Errors return NULL instead of exiting the process.
     if (!params || !prob) return nullptr;
-    if (params->feasibility_polishing)
-        fprintf(stderr, "[fa_cp] feasibility polishing is not available in this build; solving without it.\n");
     if (params->termination_evaluation_frequency <= 0) {
         fprintf(stderr, "[fa_cp] termination_evaluation_frequency must be positive.\n");
         return nullptr;

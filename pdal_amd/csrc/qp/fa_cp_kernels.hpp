@@ -727,6 +727,14 @@ static __global__ void __launch_bounds__(kBlock)
     }
 }
 
+// Finite entries -> 0, infinite ones kept: the dual polishing problem's
+// bounds (zero_finite_value_vectors_kernel, fa_cp_core_op.cu:1530-1537).
+static __global__ void __launch_bounds__(kBlock) zero_finite_kernel(double *__restrict__ v, int len)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < len && isfinite(v[i])) v[i] = 0.0;
+}
+
 static __global__ void __launch_bounds__(kBlock) fill_kernel(double *__restrict__ v, double val, int len)
 {
     const int i = blockIdx.x * kBlock + threadIdx.x;

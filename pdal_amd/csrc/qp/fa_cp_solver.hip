@@ -161,6 +161,28 @@ class Solver {
     fa_cp_result_t *make_result();
     void print_initial_info() const;
     void display_stats() const;
+    // feasibility polishing (fa_cp_core_op.cu:1242-1735)
+    struct PolishOut {
+        termination_reason_t reason;
+        int total;
+        double abs, rel, obj, time;
+    };
+    struct PolishRefs {  // the main problem's data the phases' residuals read
+        const double *c, *l, *u, *x;
+    };
+    void feasibility_polish();
+    PolishOut polish_phase(bool primal, double pw0, const PolishRefs &ref);
+    void polish_residual(bool primal, const PolishRefs &ref, PolishOut &out);
+    double polish_fixed_point_error(bool primal);
+    void drop_bb_graph()
+    {
+        if (bb_graph_) PDAL_HIP(hipGraphExecDestroy(bb_graph_));
+        bb_graph_ = nullptr;
+        g_tau_ = g_al_ = g_tol_ = NAN;
+        g_limit_ = -1;
+    }
+    double polish_time_ = 0.0;
+    int polish_iters_ = 0;
 
     fa_cp_parameters_t P_;
     const qp_problem_t *W_;
@@ -1088,6 +1110,8 @@ fa_cp_result_t *Solver::make_result()
     r->sigma_update_count = sigma_update_count_;
     r->sigma_update_rule = al_available_ ? P_.sigma_update_rule : FA_CP_SIGMA_UPDATE_FIXED;
     r->termination_reason = reason_;
+    r->feasibility_polishing_time = polish_time_;
+    r->feasibility_iteration = polish_iters_;
     return r;
 }
 
@@ -1168,7 +1192,257 @@ fa_cp_result_t *Solver::run()
         compute_residual();
         display_stats();
     }
+    if (P_.feasibility_polishing && reason_ != TERMINATION_REASON_DUAL_INFEASIBLE &&
+        reason_ != TERMINATION_REASON_PRIMAL_INFEASIBLE)  // solver.cu:156-160
+        feasibility_polish();
     return make_result();
+}
+
+// ---------------------------------------------------------------------------
+// Feasibility polishing (fa_cp_core_op.cu:1242-1735, utils.cu:1360-1575): a
+// primal phase with c = 0 (Q kept) and a dual phase with every finite bound 0,
+// each the main loop's iteration with the primal weight fixed, its own
+// fixed-point error (squared distance of the reflected from the current point,
+// one side) and residual, restarts of its own side only, until its relative
+// residual reaches eps_feas_polish_relative.  The phases run in place: the
+// problem vectors are swapped for the phase's copies and the main iterates
+// saved around them; the inner solver's state and counter carry on, as the
+// reference's phase states share them.
+
+void Solver::polish_residual(bool primal, const PolishRefs &ref, PolishOut &out)
+{   // compute_primal_feas_polish_residual / compute_dual_feas_polish_residual, utils.cu:1491-1575
+    int nba = 1, nbt = 1;
+    const double *xa = primal ? xT_.get() : ref.x;
+    dispatch_group(A_.G, [&](auto g) {
+        constexpr int G = decltype(g)::value;
+        nba = row_grid(m_, G);  // primal residual with the phase's (= original) bounds; dual contributions with the original ones
+        res_a_kernel<G><<<nba, kBlock, 0, stream_>>>(A_.view(), xa, yT_.get(), primal ? l_.get() : ref.l,
+                                                     primal ? u_.get() : ref.u, E_.get(), parts_.get());
+    });
+    PDAL_LAUNCHED();
+    reduce(nba, 2, RA_N, S_RA);
+    const StepScalars sc = scalars();
+    dispatch_group(At_.G, [&](auto g) {
+        constexpr int G = decltype(g)::value;
+        nbt = row_grid(n_, G);  // (c - At.y - slack) D, c.x and slack.x against the original c / polished x
+        res_at_kernel<G, false><<<nbt, kBlock, 0, stream_>>>(At_.view(), yT_.get(), primal ? xT_.get() : ref.x,
+                                                             qx_.get(), primal ? ref.c : c_.get(), lb_.get(),
+                                                             ub_.get(), D_.get(), dslack_.get(), sc, parts_.get());
+    });
+    PDAL_LAUNCHED();
+    reduce(nbt, 6, RT_N, S_RT);
+    fetch(S_RA, RA_N + RT_N);
+    const bool linf = P_.optimality_norm == NORM_TYPE_L_INF;
+    const double *ra = hscal_.get() + S_RA, *rt = hscal_.get() + S_RT;
+    const double bc = b_scale_ * c_scale_;
+    if (primal) {
+        out.abs = (linf ? ra[RA_PRESMAX] : std::sqrt(ra[RA_PRES2])) / b_scale_;
+        out.rel = out.abs / (1.0 + b_norm_);
+        out.obj = rt[RT_CX] / bc;  // the phase's objective constant is 0
+    } else {
+        out.abs = (linf ? rt[RT_DRESMAX] : std::sqrt(rt[RT_DRES2])) / c_scale_;
+        out.rel = out.abs / (1.0 + c_norm_);
+        out.obj = (rt[RT_SX] + ra[RA_CONTRIB]) / bc + obj_const_;
+    }
+    abs_p_ = primal ? out.abs : 0.0;
+    rel_p_ = primal ? out.rel : 0.0;
+    abs_d_ = primal ? 0.0 : out.abs;
+    rel_d_ = primal ? 0.0 : out.rel;
+}
+
+double Solver::polish_fixed_point_error(bool primal)
+{   // compute_primal/dual_fixed_point_error, fa_cp_core_op.cu:1714-1735
+    const int len = primal ? n_ : m_;
+    const int nb = stride_grid(len);
+    diff_norm_kernel<<<nb, kBlock, 0, stream_>>>(primal ? xR_.get() : yR_.get(), primal ? x_.get() : y_.get(),
+                                                 primal ? dx_.get() : dy_.get(), len, parts_.get());
+    reduce(nb, 1, 1, S_DN2);
+    fetch(S_DN2, 1);
+    const double ss = hs(S_DN2);
+    return primal ? ss * pw_ : ss / pw_;
+}
+
+Solver::PolishOut Solver::polish_phase(bool primal, double pw0, const PolishRefs &ref)
+{   // primal_feasibility_polish / dual_feasibility_polish, fa_cp_core_op.cu:1312-1404
+    const termination_criteria_t &tc = P_.termination_criteria;
+    const int K = P_.termination_evaluation_frequency;
+    // scalar resets of initialize_primal/dual_feas_polish_state (:1474-1494, 1626-1646)
+    pw_err_sum_ = pw_last_err_ = 0.0;
+    pw_ = best_pw_ = pw0;
+    fp_ = fp0_ = last_trial_fp_ = INFINITY;
+    major_ = false;
+    total_ = inner_ = 0;
+    best_gap_ = INFINITY;
+    drop_bb_graph();
+    if (P_.verbose) {  // print_initial_feas_polish_info, utils.cu:1386-1407
+        printf("---------------------------------------------------------------------------------------\n");
+        printf("Starting %s Feasibility Polishing Phase with relative tolerance %.2e\n", primal ? "Primal" : "Dual",
+               tc.eps_feas_polish_relative);
+        printf("---------------------------------------------------------------------------------------\n");
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    PolishOut out{TERMINATION_REASON_UNSPECIFIED, 0, 0.0, 0.0, 0.0, 0.0};
+    bool do_restart = false;
+    while (out.reason == TERMINATION_REASON_UNSPECIFIED) {
+        const bool consider = major_ || total_ == 0;
+        if (consider || total_ % print_frequency(total_) == 0) {
+            polish_residual(primal, ref, out);
+            out.time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            // check_feas_polishing_termination_criteria, utils.cu:1360-1384 (the phase's own clock)
+            if (out.rel <= tc.eps_feas_polish_relative) out.reason = TERMINATION_REASON_FEAS_POLISH_SUCCESS;
+            else if (total_ >= tc.iteration_limit) out.reason = TERMINATION_REASON_ITERATION_LIMIT;
+            else if (out.time >= tc.time_sec_limit) out.reason = TERMINATION_REASON_TIME_LIMIT;
+            if (P_.verbose >= 2 && total_ % print_frequency(total_) == 0)
+                printf("%6d %.1e | %8.1e |    %.1e   |   %.1e   \n", total_, out.time, out.obj, out.abs, out.rel);
+        }
+        if (consider) {  // the restart test runs after a stop as well, as in the reference loop
+            do_restart = should_restart();
+            if (do_restart) {  // perform_primal_restart / perform_dual_restart, :1668-1690
+                copy(primal ? x0_.get() : y0_.get(), primal ? xT_.get() : yT_.get(), primal ? n_ : m_);
+                copy(primal ? x_.get() : y_.get(), primal ? xT_.get() : yT_.get(), primal ? n_ : m_);
+                inner_ = 0;
+                last_trial_fp_ = INFINITY;
+            }
+        }
+        major_ = ((total_ + 1) % K) == 0;
+        const bool need_fp = major_ || do_restart;
+        update(!need_fp);
+        if (need_fp) {
+            fp_ = polish_fixed_point_error(primal);
+            if (do_restart) {
+                fp0_ = fp_;
+                do_restart = false;
+            }
+            halpern();
+        }
+        inner_++;
+        total_++;
+    }
+    out.total = total_;
+    drop_bb_graph();
+    return out;
+}
+
+void Solver::feasibility_polish()
+{   // feasibility_polish, fa_cp_core_op.cu:1242-1310
+    const auto t_start = std::chrono::steady_clock::now();
+    const termination_criteria_t &tc = P_.termination_criteria;
+    if (rel_p_ < tc.eps_feas_polish_relative && rel_d_ < tc.eps_feas_polish_relative) {
+        printf("Skipping feasibility polishing as the solution is already sufficiently feasible.\n");
+        fflush(stdout);
+        return;
+    }
+    const double pw0 = P_.bound_objective_rescaling ? 1.0 : (c_norm_ + 1.0) / (b_norm_ + 1.0);
+    // the main state's scalars and vectors the phases overwrite
+    const int m_total = total_, m_inner = inner_;
+    const bool m_major = major_;
+    const double m_fp = fp_, m_fp0 = fp0_, m_last = last_trial_fp_, m_pw = pw_, m_best_pw = best_pw_;
+    const double m_err = pw_err_sum_, m_last_err = pw_last_err_, m_best_gap = best_gap_;
+    double abs_p = abs_p_, rel_p = rel_p_, p_obj = p_obj_, abs_d = abs_d_, rel_d = rel_d_, d_obj = d_obj_;
+    DBuf<double> sx(n_), sy0(m_), sy(m_), syT(m_), syR(m_), sdp(n_), sds(n_);
+    copy(sx.get(), xT_.get(), n_);
+    copy(sy0.get(), y0_.get(), m_);
+    copy(sy.get(), y_.get(), m_);
+    copy(syT.get(), yT_.get(), m_);
+    copy(syR.get(), yR_.get(), m_);
+    copy(sdp.get(), dual_prod_.get(), n_);
+    copy(sds.get(), dslack_.get(), n_);
+    auto zero = [&](double *v, int len) {
+        if (len > 0) fill_kernel<<<blocks_for(len), kBlock, 0, stream_>>>(v, 0.0, len);
+    };
+
+    // primal phase (initialize_primal_feas_polish_state, :1412-1494): c = 0, constant 0,
+    // dual iterates, products and slacks zero, primal iterates from the main state
+    DBuf<double> zc(n_);
+    zero(zc.get(), n_);
+    std::swap(c_, zc);  // zc: the original c
+    const double oc = obj_const_;
+    obj_const_ = 0.0;
+    for (double *v : {y0_.get(), y_.get(), yT_.get(), yR_.get(), al_u_.get(), pslack_.get(), dy_.get()}) zero(v, m_);
+    for (double *v : {dual_prod_.get(), dslack_.get(), dx_.get()}) zero(v, n_);
+    PDAL_LAUNCHED();
+    const PolishOut pr = polish_phase(true, pw0, PolishRefs{zc.get(), l_.get(), u_.get(), nullptr});
+    std::swap(c_, zc);
+    obj_const_ = oc;
+    if (pr.reason == TERMINATION_REASON_FEAS_POLISH_SUCCESS) {
+        copy(sx.get(), xT_.get(), n_);
+        abs_p = pr.abs;
+        rel_p = pr.rel;
+        p_obj = pr.obj;
+    }
+    polish_iters_ += pr.total - 1;
+
+    // dual phase (initialize_dual_feas_polish_state, :1539-1646): finite bounds 0, primal
+    // iterates and products zero, dual iterates, product and slack from the main state
+    DBuf<double> zlb(n_), zub(n_), zl(m_), zu(m_);
+    copy(zlb.get(), lb_.get(), n_);
+    copy(zub.get(), ub_.get(), n_);
+    copy(zl.get(), l_.get(), m_);
+    copy(zu.get(), u_.get(), m_);
+    for (auto [v, len] : {std::pair<double *, int>{zlb.get(), n_}, {zub.get(), n_}, {zl.get(), m_}, {zu.get(), m_}})
+        if (len > 0) zero_finite_kernel<<<blocks_for(len), kBlock, 0, stream_>>>(v, len);
+    std::swap(lb_, zlb);  // z*: the original bounds
+    std::swap(ub_, zub);
+    std::swap(l_, zl);
+    std::swap(u_, zu);
+    copy(y0_.get(), sy0.get(), m_);
+    copy(y_.get(), sy.get(), m_);
+    copy(yT_.get(), syT.get(), m_);
+    copy(yR_.get(), syR.get(), m_);
+    copy(dual_prod_.get(), sdp.get(), n_);
+    copy(dslack_.get(), sds.get(), n_);
+    for (double *v : {x0_.get(), x_.get(), xT_.get(), xR_.get(), dx_.get()}) zero(v, n_);
+    for (double *v : {prim_prod_.get(), al_u_.get(), pslack_.get(), dy_.get()}) zero(v, m_);
+    PDAL_LAUNCHED();
+    const PolishOut du = polish_phase(false, pw0, PolishRefs{c_.get(), zl.get(), zu.get(), sx.get()});
+    std::swap(lb_, zlb);
+    std::swap(ub_, zub);
+    std::swap(l_, zl);
+    std::swap(u_, zu);
+    if (du.reason == TERMINATION_REASON_FEAS_POLISH_SUCCESS) {
+        abs_d = du.abs;
+        rel_d = du.rel;
+        d_obj = du.obj;
+    } else {
+        copy(yT_.get(), syT.get(), m_);
+    }
+    polish_iters_ += du.total - 1;
+    copy(xT_.get(), sx.get(), n_);  // the result's x
+    PDAL_HIP(hipStreamSynchronize(stream_));
+
+    total_ = m_total;
+    inner_ = m_inner;
+    major_ = m_major;
+    fp_ = m_fp;
+    fp0_ = m_fp0;
+    last_trial_fp_ = m_last;
+    pw_ = m_pw;
+    best_pw_ = m_best_pw;
+    pw_err_sum_ = m_err;
+    pw_last_err_ = m_last_err;
+    best_gap_ = m_best_gap;
+    abs_p_ = abs_p;
+    rel_p_ = rel_p;
+    p_obj_ = p_obj;
+    abs_d_ = abs_d;
+    rel_d_ = rel_d;
+    d_obj_ = d_obj;
+    gap_ = std::fabs(p_obj_ - d_obj_);
+    rel_gap_ = gap_ / (1.0 + std::fabs(p_obj_) + std::fabs(d_obj_));
+    // fa_cp_feas_polish_final_log, utils.cu:1409-1440
+    if (P_.verbose) printf("---------------------------------------------------------------------------------------\n");
+    printf("Feasibility Polishing Summary\n");
+    printf("  Primal Status        : %s\n", termination_reason_name(pr.reason));
+    printf("  Primal Iterations    : %d\n", pr.total);
+    printf("  Primal Time Usage    : %.3g sec\n", pr.time);
+    printf("  Dual Status          : %s\n", termination_reason_name(du.reason));
+    printf("  Dual Iterations      : %d\n", du.total);
+    printf("  Dual Time Usage      : %.3g sec\n", du.time);
+    printf("  Primal Residual      : %.3e\n", pr.rel);
+    printf("  Dual Residual        : %.3e\n", du.rel);
+    printf("  Primal Dual Gap      : %.3e\n", std::fabs(pr.obj - du.obj) / (1.0 + std::fabs(pr.obj) + std::fabs(du.obj)));
+    fflush(stdout);
+    polish_time_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
 }
 
 }  // namespace

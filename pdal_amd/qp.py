@@ -238,6 +238,8 @@ class QpResult:
     sigma_update_count: int
     rescaling_time_sec: float
     solve_time_sec: float
+    feasibility_polishing_time: float = 0.0
+    feasibility_iteration: int = 0
 
     @property
     def kkt(self) -> float:
@@ -322,7 +324,9 @@ class FaCp:
                 max_dual_ray_infeasibility=r.max_dual_ray_infeasibility,
                 primal_ray_linear_objective=r.primal_ray_linear_objective, dual_ray_objective=r.dual_ray_objective,
                 final_al_sigma=r.final_al_sigma, sigma_update_count=r.sigma_update_count,
-                rescaling_time_sec=r.rescaling_time_sec, solve_time_sec=r.cumulative_time_sec)
+                rescaling_time_sec=r.rescaling_time_sec, solve_time_sec=r.cumulative_time_sec,
+                feasibility_polishing_time=r.feasibility_polishing_time,
+                feasibility_iteration=r.feasibility_iteration)
         finally:
             self.lib.fa_cp_result_free(res)
 
