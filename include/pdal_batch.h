@@ -46,7 +46,7 @@ int pdal_cp_al_batch_solve(pdal_cp_al_batch_t *batch);
  * NULL before the first solve or for an invalid index. */
 cp_al_result_t *pdal_cp_al_batch_result(pdal_cp_al_batch_t *batch, int i);
 
-/* out[0..13] = {set-up seconds, last solve seconds (wall clock, device
+/* out[0..14] = {set-up seconds, last solve seconds (wall clock, device
  * resident), hipGraph replays, instances, OPTIMAL instances, sum of
  * iteration counts, lanes per row of A, lanes per row of At, block-table
  * compactions (stopped instances are dropped from the launch tables and the
@@ -54,7 +54,10 @@ cp_al_result_t *pdal_cp_al_batch_result(pdal_cp_al_batch_t *batch, int i);
  * solve until half of the instances had stopped, seconds until 99 % had,
  * instances whose At.y / A.x_bar runs LDS-staged (one workgroup per instance,
  * gathered segment in LDS) while every instance runs: At.y staged, A.x_bar
- * staged whole, A.x_bar staged in two column halves}. */
+ * staged whole, A.x_bar staged in two column halves, 1 when the batch runs
+ * the shared-A engine (every instance has the same constraint matrix: the
+ * instances' vectors side by side, one pass over A serves them all; up to 512
+ * instances)}. */
 void pdal_cp_al_batch_stats(const pdal_cp_al_batch_t *batch, double *out);
 
 /* Launch `iters` plain iterations over all instances kernel by kernel

@@ -1,0 +1,91 @@
+"""Shared-A engine of the batched CP_AL (pdal_batch.h): when every instance
+has the same constraint matrix, the batch keeps the instances' vectors side by
+side and one pass over A serves all of them.  Every instance must still follow
+the single-instance algorithm: iterates within 1e-10 of the oracle solving it
+alone, and the same solves as the block-diagonal engine.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from pdal_amd.generators import config4_thetas, lp_family
+from pdal_amd.lp import LpBatch
+
+pytestmark = pytest.mark.gpu
+
+ITER_RTOL = 1e-10
+K = 64
+
+
+def rel_err(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300))
+
+
+def family_instances(count, seed=3, m=200):
+    fam = lp_family(m, 2 * m, 8, seed=seed)
+    return [fam.instance(t) for t in config4_thetas(count, seed)]
+
+
+@pytest.mark.parametrize("lam", ["0", "0.05"])
+@pytest.mark.parametrize("count,blocks", [(10, 1), (10, 6), (70, 3)])
+def test_shared_engine_tracks_oracle(oracle_lp, gpu_lp, monkeypatch, lam, count, blocks):
+    monkeypatch.setenv("CP_AL_LAMBDA", lam)
+    insts = family_instances(count)
+    params = dict(iteration_limit=blocks * K, termination_evaluation_frequency=K, eps_optimal_relative=1e-14,
+                  eps_feasible_relative=1e-14)
+    print(f"[case] shared-A batch lam={lam} count={count} blocks={blocks}", flush=True)
+    with LpBatch(gpu_lp, insts, gpu_lp.default_params(**params)) as b:
+        assert b.stats()["shared_a"]
+        assert b.solve() == 0
+        for k in range(0, count, max(1, count // 8)):
+            oracle_lp.reset_rng()
+            o = oracle_lp.solve(**insts[k], params=oracle_lp.default_params(**params))
+            g = b.result(k)
+            assert g.iterations == o.iterations == blocks * K
+            assert rel_err(g.x, o.x) <= ITER_RTOL, k
+            assert rel_err(g.y, o.y) <= ITER_RTOL, k
+            assert rel_err(g.reduced_cost, o.reduced_cost) <= ITER_RTOL, k
+
+
+def test_shared_engine_equals_block_diagonal(gpu_lp, monkeypatch):
+    monkeypatch.setenv("CP_AL_LAMBDA", "0")
+    insts = family_instances(40, seed=4)
+    params = gpu_lp.default_params(termination_evaluation_frequency=K, eps_optimal_relative=1e-6,
+                                   eps_feasible_relative=1e-6, iteration_limit=100000)
+    runs = []
+    for flag in ("0", "1"):
+        monkeypatch.setenv("PDAL_BATCH_NO_SHARED", flag)
+        print(f"[case] fixed-A solve, no_shared={flag}", flush=True)
+        with LpBatch(gpu_lp, insts, params) as b:
+            assert b.stats()["shared_a"] == (flag == "0")
+            assert b.solve() == len(insts)
+            runs.append([b.result(k) for k in range(len(insts))])
+    for a, c in zip(*runs):
+        assert a.status == c.status == "OPTIMAL"
+        assert a.iterations == c.iterations
+        assert rel_err(a.x, c.x) <= 1e-8 and rel_err(a.y, c.y) <= 1e-8
+        assert abs(a.primal_objective - c.primal_objective) <= 1e-9 * (1 + abs(c.primal_objective))
+
+
+def test_shared_engine_repeated_solves_and_starts(gpu_lp, monkeypatch):
+    """Bit-identical repeated solves; warm starts and primal weights reach the
+    shared layout (a restart from the own solutions converges fast)."""
+    monkeypatch.setenv("CP_AL_LAMBDA", "0")
+    insts = family_instances(12, seed=5)
+    params = gpu_lp.default_params(termination_evaluation_frequency=K, eps_optimal_relative=1e-6,
+                                   eps_feasible_relative=1e-6, iteration_limit=100000)
+    with LpBatch(gpu_lp, insts, params) as b:
+        assert b.stats()["shared_a"]
+        b.solve()
+        first = [b.result(k) for k in range(len(insts))]
+        pw = b.primal_weights()
+        b.solve()
+        for k, r in enumerate(first):
+            s = b.result(k)
+            assert np.array_equal(s.x, r.x) and s.iterations == r.iterations
+        b.set_primal_weights(pw)
+        b.set_starts(np.concatenate([r.x for r in first]), np.concatenate([r.y for r in first]))
+        assert b.solve() == len(insts)
+        warm = [b.result(k) for k in range(len(insts))]
+    assert sum(w.iterations for w in warm) < 0.3 * sum(r.iterations for r in first)
