@@ -1,27 +1,36 @@
 """Benchmark of the MI355X CP_AL solvers (one GPU).
 
-Default workload: the batched family of 4096 planted LPs (m_i log-uniform on
-[200, 20000] with seed 12345, n_i = 2 m_i, 8 nonzeros per column at uniform
-rows, N(0,1) values, instance i seeded with i; pdal_amd.generators
-.config3_sizes / config3_instance), FP64, every instance solved to relative
-KKT 1e-6 (alpha = 0, blocks of 64 iterations) in one batched device-resident
-run.  A step is one solve of all 4096 instances with the batch already
-resident in HBM (upload, transpose, rescaling and the batched power iteration
-happen once before the timed region and are reported as setup_s).
-`value` = instances solved per second over the K timed steps.
+Default workload: batched + learned-warm-start solves of the fixed-A LP family
+(pdal_amd.generators.config4_family: one A with m = 5e4, n = 1e5, 8 nonzeros
+per column, N(0,1); b = b0 + U theta_b, c = c0 + V theta_c with 32 + 32
+parameters around a planted optimum).  Before the timed region: 512 training
+members solved in one batch to 1e-8 (iteration cap 40,000), the affine head +
+MLP (64 -> 512 -> 512 -> 512 -> 256, BF16 MFMA, trained on the device) fitted
+on their POD coefficients and final primal weights, and the batch of 256
+held-out members set up in HBM (shared-A engine).  A step = predict the starts
+and primal weights of the 256 held-out members (MLP forward + POD expansion),
+hand them to the batch, solve all 256 to relative KKT 1e-6 (alpha = 0, blocks
+of 64 iterations), cold re-solve of any member the warm pass leaves unsolved.
+`value` = held-out solves per second over the K timed steps.
 
 The same JSON line carries:
   * plain_solves_per_s: the single-instance GPU path (device session per
-    instance, set-up untimed) on a random sample of the same instances;
-  * roofline of the dominant kernel (the batched At.y SpMV fused with the
-    primal update), timed live with hipEvents on the batch's stream:
-    algorithmic bytes per launch / average launch time, against 8 TB/s;
+    instance, set-up untimed, cold) on held-out members; speedup_vs_plain;
+  * cold_batched_solves_per_s: the same batch solved from zero;
+  * roofline of the dominant kernel (the shared-A At.y pass fused with the
+    primal update over the 256 members), timed live with hipEvents on the
+    batch's stream: algorithmic bytes per launch / average launch time,
+    against 8 TB/s, with the HBM traffic measured by rocprofv3;
+  * batch_4096: the batched family of 4096 uneven planted LPs (m_i
+    log-uniform on [200, 20000], n_i = 2 m_i, 8 nonzeros per column; cold),
+    solves/s and its own roofline;
   * spmv: the plain path's fused SpMV kernels on the large planted LP
-    (m = 2e6, n = 4e6, Zipf(1.5) column counts; config1) with their HBM GB/s;
-  * cpu_baseline: the FP64 oracle (single thread) on a bounded sample of the
-    same instances, scaled to solves/s with the GPU iteration counts.
+    (m = 2e6, n = 4e6, Zipf(1.5) column counts) with their HBM GB/s;
+  * cpu_baseline: the FP64 oracle (single thread, cold) on a bounded sample of
+    held-out members, scaled to solves/s with the GPU cold iteration counts.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+       python bench.py --workload config3   (the 4096-instance batch alone)
        python bench.py --workload config1   (plain single-instance solve of the large LP)
        python bench.py --kernels-only       (set-up and kernel timing only, for rocprofv3)
 """
@@ -57,11 +66,18 @@ def fused_dual_bytes(nnz: int, n: int, m: int, segments: int = 1) -> int:
     return 12 * nnz + 4 * (m + segments) + 8 * n + 40 * m
 
 
-def measured_traffic(tag: str):
+def shared_primal_bytes(nnz: int, n: int, m: int, count: int) -> int:
+    # shared-A At.y pass: the one At streamed once (12 B per nonzero, 4 B per
+    # row pointer), every instance's y gathered once (8 m per instance), and the
+    # epilogue per instance: read c, lb, ub, x0, x, write xR, x (7 x 8 B per row)
+    return 12 * nnz + 4 * (n + 1) + 8 * m * count + 56 * n * count
+
+
+def measured_traffic(tag: str, profile: str = "batch_staged_spmv_4096_instances.md"):
     """HBM bytes per launch of a kernel from a committed rocprofv3 summary
     (FETCH_SIZE and WRITE_SIZE scaled by a copy kernel of known size); the
     summary states the commands that produced it.  None when absent."""
-    f = ROOT / "profiles" / "batch_staged_spmv_4096_instances.md"
+    f = ROOT / "profiles" / profile
     if not f.exists():
         return None
     for line in f.read_text().splitlines():
@@ -201,6 +217,150 @@ def run_config3(args, lib) -> dict:
     return result
 
 
+def cpu_baseline_family(insts, gpu_iters: list, sample: int, iters_cap: int) -> dict:
+    """Oracle (single-threaded FP64 port of the reference's CP_AL, cold start):
+    `iters_cap` iterations on each of a few held-out instances, scaled to full
+    solves with the GPU's cold iteration counts of the same instances."""
+    from pdal_amd.lp import CpAl
+    oracle = CpAl(ROOT / "oracle" / "_build" / "libcp_al_oracle.so")
+    est, done_its = 0.0, 0
+    t_all = time.perf_counter()
+    for k in range(sample):
+        oracle.reset_rng()
+        p = oracle.default_params(iteration_limit=iters_cap, termination_evaluation_frequency=K_BATCH,
+                                  eps_optimal_relative=1e-6, eps_feasible_relative=1e-6)
+        r = oracle.solve(**insts[k], params=p)
+        est += gpu_iters[k] / (r.iterations / max(r.solve_time_sec, 1e-9))
+        done_its += r.iterations
+        log(f"CPU baseline instance {k}: {r.iterations} iterations in {r.solve_time_sec:.1f} s")
+    wall = time.perf_counter() - t_all
+    return {"value": round(sample / est, 5), "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": (f"{sample} held-out instances, {iters_cap} cold oracle iterations each ({done_its} iterations, "
+                       f"{wall:.1f} s wall incl. set-up); each instance's full-solve time = its GPU cold iteration "
+                       f"count / the oracle's iterations per second on it")}
+
+
+def run_config4(args, lib) -> dict:
+    """Batched + learned-warm-start solves of held-out instances of the fixed-A
+    family (m=5e4, n=1e5) against the plain single-instance GPU path."""
+    import numpy as np
+    from pdal_amd.generators import config4_family, config4_thetas
+    from pdal_amd.lp import LpBatch, LpSession
+    from pdal_amd.warmstart import LearnedWarmStart, load_ws
+    count = args.heldout
+    t0 = time.perf_counter()
+    fam = config4_family(seed=4)
+    te = config4_thetas(count, 2)
+    test_inst = [fam.instance(t) for t in te]
+    log(f"family m={fam.m} n={fam.n} nnz={fam.base.A.nnz}; {count} held-out instances in {time.perf_counter() - t0:.1f} s")
+    params = lib.default_params(termination_evaluation_frequency=K_BATCH, eps_optimal_relative=args.tol,
+                                eps_feasible_relative=args.tol, iteration_limit=1000000, time_sec_limit=300.0)
+    t0 = time.perf_counter()
+    batch = LpBatch(lib, test_inst, params)
+    st = batch.stats()
+    if not st["shared_a"]:
+        raise RuntimeError("the held-out batch did not take the shared-A engine")
+    setup_s = st["setup_s"]
+    log(f"held-out batch set-up {setup_s:.2f} s (wall {time.perf_counter() - t0:.2f} s)")
+    kt = batch.time_kernels(args.kernel_iters)
+    nnz1, n1, m1 = kt["nnz"] // count, kt["n"] // count, kt["m"] // count
+    bp = shared_primal_bytes(nnz1, n1, m1, count)
+    traffic = measured_traffic("shared_at_pass", "shared_spmv_fixed_a_family_256_instances.md") \
+        if count == 256 else None
+    roof = roofline(bp, kt["primal_ms"], "shared-A At.y pass: sh_primal_kernel<4> (one At, 256 instances side by "
+                    "side, fused primal update)", traffic)
+    log(f"shared kernels: primal {kt['primal_ms']:.3f} ms ({roof['achieved']:.0f} GB/s), dual {kt['dual_ms']:.3f} ms")
+    result = {"roofline": roof, "setup_s": round(setup_s, 3),
+              "shared_kernel_ms": {"primal": kt["primal_ms"], "dual": kt["dual_ms"]}}
+    if args.kernels_only:
+        batch.close()
+        return result
+    # training: the solver's own solutions of other family members (outside the timed region)
+    tr = config4_thetas(args.train, 1)
+    p_train = lib.default_params(termination_evaluation_frequency=K_BATCH, eps_optimal_relative=1e-8,
+                                 eps_feasible_relative=1e-8, iteration_limit=args.train_iters, time_sec_limit=300.0)
+    t0 = time.perf_counter()
+    with LpBatch(lib, [fam.instance(t) for t in tr], p_train) as b:
+        b.solve()
+        tst = b.stats()
+        sols = [b.result(k) for k in range(len(tr))]
+        pw = b.primal_weights()
+    X, Y = np.stack([r.x for r in sols]), np.stack([r.y for r in sols])
+    del sols
+    log(f"training: {len(tr)} solves to 1e-8 (cap {args.train_iters} iterations) in {tst['solve_s']:.1f} s "
+        f"(wall {time.perf_counter() - t0:.1f} s incl. set-up)")
+    ws = load_ws()
+    lws = LearnedWarmStart(ws, kx=128, seed=3)
+    t0 = time.perf_counter()
+    info = lws.fit(tr, X, Y, pw=pw, lb=np.zeros(fam.n), epochs=args.epochs)
+    fit_s = time.perf_counter() - t0
+    log(f"fit {fit_s:.1f} s: {info}")
+    # cold batched solve of the held-out set (reference point, same batch)
+    t1 = time.perf_counter()
+    batch.solve()
+    cold_s = time.perf_counter() - t1
+    cold_its = np.array([batch.result(k).iterations for k in range(count)])
+    log(f"cold batched: {count} in {cold_s:.2f} s, iterations mean {cold_its.mean():.0f} max {cold_its.max()}")
+
+    def warm_step():
+        xh, yh, pwh = lws.predict(te)
+        batch.set_starts(xh.ravel(), yh.ravel())
+        batch.set_primal_weights(pwh)
+        n_opt = batch.solve()
+        if n_opt < count:  # the cold fallback for what the warm pass left unsolved
+            bad = [k for k in range(count) if batch.result(k).status != "OPTIMAL"]
+            with LpBatch(lib, [test_inst[k] for k in bad], params) as fb:
+                fb.solve()
+            return len(bad)
+        return 0
+
+    for w in range(args.warmup):
+        warm_step()
+    times, fallbacks = [], 0
+    for s in range(args.steps):
+        t1 = time.perf_counter()
+        fallbacks += warm_step()
+        times.append(time.perf_counter() - t1)
+        log(f"warm step {s}: {count} held-out solves in {times[-1]:.3f} s")
+    res = [batch.result(k) for k in range(count)]
+    warm_its = np.array([r.iterations for r in res])
+    obj_err = max(abs(r.primal_objective - fam.optimal_value(t)) / (1.0 + abs(fam.optimal_value(t)))
+                  for r, t in zip(res, te))
+    batch.close()
+    lws.close()
+    total = sum(times)
+    value = count * args.steps / total
+    # plain single-instance GPU path, cold, on held-out instances (set-up untimed)
+    plain_t, plain_its = 0.0, []
+    for k in range(args.plain_sample):
+        with LpSession(lib, **test_inst[k], params=params) as sess:
+            lib.reset_rng()
+            t1 = time.perf_counter()
+            r = sess.solve()
+            plain_t += time.perf_counter() - t1
+            plain_its.append(r.iterations)
+    plain = args.plain_sample / plain_t
+    log(f"plain single-instance GPU: {args.plain_sample} solves in {plain_t:.2f} s -> {plain:.2f} solves/s")
+    result.update({
+        "value": value, "ms_per_step": 1e3 * total / args.steps, "optimal": int(sum(r.status == "OPTIMAL" for r in res)),
+        "fallbacks": fallbacks, "warm_iterations": {"mean": float(warm_its.mean()), "max": int(warm_its.max())},
+        "cold_batched_solves_per_s": round(count / cold_s, 3),
+        "cold_iterations": {"mean": float(cold_its.mean()), "max": int(cold_its.max())},
+        "plain_solves_per_s": round(plain, 3), "plain_sample": args.plain_sample,
+        "plain_iterations_mean": float(np.mean(plain_its)),
+        "speedup_vs_plain": round(value / plain, 2), "speedup_cold_batched_vs_plain": round(count / cold_s / plain, 2),
+        "objective_rel_err_vs_planted_max": obj_err,
+        "training": {"instances": args.train, "tolerance": 1e-8, "iteration_cap": args.train_iters,
+                     "solve_s": round(tst["solve_s"], 2), "fit_s": round(fit_s, 2), "epochs": args.epochs,
+                     "head_rel_resid": info.get("head_rel_resid")},
+    })
+    if not args.no_cpu:
+        log("CPU baseline (oracle, 1 thread) ...")
+        result["cpu_baseline"] = cpu_baseline_family(test_inst, [int(i) for i in cold_its], args.cpu_family_sample,
+                                                     args.cpu_family_iters)
+    return result
+
+
 def run_config1(args, lib) -> dict:
     from pdal_amd.generators import config1
     from pdal_amd.lp import CpAl, LpSession
@@ -243,17 +403,25 @@ def run_config1(args, lib) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
-    ap.add_argument("--workload", choices=["config3", "config1"], default="config3")
-    ap.add_argument("--instances", type=int, default=4096)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["config4", "config3", "config1"], default="config4")
+    ap.add_argument("--heldout", type=int, default=256, help="config4: held-out instances per batch")
+    ap.add_argument("--train", type=int, default=512, help="config4: training solves")
+    ap.add_argument("--train-iters", type=int, default=40000, help="config4: iteration cap of the training solves")
+    ap.add_argument("--epochs", type=int, default=300, help="config4: MLP training epochs")
+    ap.add_argument("--instances", type=int, default=4096, help="config3: batch size")
+    ap.add_argument("--batch-steps", type=int, default=1, help="config3 leg of the default run: timed solves")
     ap.add_argument("--scale", type=float, default=1.0, help="config1 only: shrink m and n")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--plain-sample", type=int, default=12)
     ap.add_argument("--cpu-sample", type=int, default=8)
     ap.add_argument("--cpu-iters", type=int, default=6400)
+    ap.add_argument("--cpu-family-sample", type=int, default=2)
+    ap.add_argument("--cpu-family-iters", type=int, default=6400)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-spmv", action="store_true", help="skip the large-LP SpMV leg")
+    ap.add_argument("--no-batch-leg", action="store_true", help="config4: skip the 4096-instance batch leg")
     ap.add_argument("--kernels-only", action="store_true", help="set up and time kernels only (profiling)")
     ap.add_argument("--kernel-iters", type=int, default=20)
     args = ap.parse_args()
@@ -268,7 +436,15 @@ def main() -> int:
 
     from pdal_amd.lp import load_cp_al
     lib = load_cp_al()
-    if args.workload == "config3":
+    if args.workload == "config4":
+        res = run_config4(args, lib)
+        metric = ("solves/sec to rel-KKT 1e-6, batched + learned warm start (fixed-A LP family, 256 held-out "
+                  "instances per batch, one GPU)")
+        config = {"workload": "fixed-A LP family m=5e4 n=1e5 8 nnz/col, b=b0+U theta_b, c=c0+V theta_c (32+32 "
+                              "parameters, planted optimum); 256 held-out instances per batch, learned warm start",
+                  "heldout": args.heldout, "train": args.train, "tolerance": args.tol, "alpha": 0.0,
+                  "eval_freq": K_BATCH}
+    elif args.workload == "config3":
         res = run_config3(args, lib)
         metric = "solves/sec to rel-KKT 1e-6 (batched CP_AL, 4096 planted LPs, one GPU)"
         config = {"workload": "batched CP_AL: 4096 planted LPs, m log-uniform [200,20000], n=2m, 8 nnz/col",
@@ -290,7 +466,14 @@ def main() -> int:
     roof = res.pop("roofline")
     cpu = res.pop("cpu_baseline", None)
     out.update(res)
-    if args.workload == "config3" and not args.no_spmv:
+    if args.workload == "config4" and not args.no_batch_leg:
+        leg_args = argparse.Namespace(**{**vars(args), "steps": args.batch_steps, "warmup": 0, "no_cpu": True})
+        leg = run_config3(leg_args, lib)
+        leg.pop("ms_per_step")
+        leg["solves_per_s"] = round(leg.pop("value"), 4)
+        leg["workload"] = "4096 planted LPs, m log-uniform [200,20000], n=2m, 8 nnz/col (uneven sizes, cold)"
+        out["batch_4096"] = leg
+    if args.workload in ("config4", "config3") and not args.no_spmv:
         out["spmv"] = spmv_leg(lib, args.kernel_iters)
     out["roofline"] = roof
     out["cpu_baseline"] = cpu

@@ -254,10 +254,85 @@ def lp_family(m: int, n: int, nnz_per_col: int, seed: int, p: int = 32, scale: f
     return LpFamily(base=base, U=scale * rng.standard_normal((m, p)), V=scale * rng.standard_normal((n, p)))
 
 
-def config4_family(seed: int = 4, scale: float = 1.0) -> LpFamily:
-    """m=5e4, n=1e5, 8 nonzeros per column; U (m x 32), V (n x 32) Gaussian x 0.1."""
+@dataclass
+class PlantedFamily:
+    """Fixed-A LP family with a planted optimum for every parameter vector.
+
+    b(theta) = b0 + U theta_b and c(theta) = c0 + V theta_c as in LpFamily, with
+    U and V dense but built from the planted optimum of the base LP:
+
+      x(theta) = x* . (1 + W theta_b)        U = A diag(x*) W
+      y(theta) = y* + G theta_c              V = A^T G + diag(s*) H
+      s(theta) = s* . (1 + H theta_c)
+
+    W (n x p), G (m x p), H (n x p) have N(0, scale^2 / p) entries, so x, y and
+    s move by `scale` relative per entry.  x(theta) >= 0, s(theta) >= 0 keep
+    their supports (disjoint, so s^T x = 0), A x(theta) = b(theta) and
+    c(theta) - A^T y(theta) = s(theta): (x, y, s)(theta) is optimal for every
+    theta with 1 + W theta_b > 0 and 1 + H theta_c > 0 (a 10-sigma event to
+    fail at scale 0.1; `instance` checks it).  Independent Gaussian U, V cannot
+    be used at this size: rows of A whose few entries share one sign force a
+    sign on b_i, and an additive shift of b flips it (infeasible LPs).
+    """
+    base: PlantedLP
+    W: np.ndarray   # n x p
+    G: np.ndarray   # m x p
+    H: np.ndarray   # n x p
+    U: np.ndarray   # m x p
+    V: np.ndarray   # n x p
+
+    @property
+    def m(self) -> int:
+        return self.base.m
+
+    @property
+    def n(self) -> int:
+        return self.base.n
+
+    @property
+    def p(self) -> int:
+        return self.U.shape[1]
+
+    def solution(self, theta):
+        """The planted (x, y, s) of parameters theta."""
+        theta = np.asarray(theta, dtype=np.float64)
+        tb, tc = theta[: self.p], theta[self.p:]
+        fx, fs = 1.0 + self.W @ tb, 1.0 + self.H @ tc
+        if fx.min() <= 0.0 or fs.min() <= 0.0:
+            raise ValueError("theta outside the family's planted region")
+        return self.base.x_star * fx, self.base.y_star + self.G @ tc, self.base.s_star * fs
+
+    def optimal_value(self, theta) -> float:
+        x, _, _ = self.solution(theta)
+        return float(self.instance(theta)["c"] @ x)
+
+    def instance(self, theta) -> dict:
+        """Keyword arguments of CpAl.make_problem / LpBatch for parameters theta."""
+        theta = np.asarray(theta, dtype=np.float64)
+        self.solution(theta)  # the region check
+        b = self.base.b + self.U @ theta[: self.p]
+        c = self.base.c + self.V @ theta[self.p:]
+        n = self.n
+        return dict(A=self.base.A, c=c, con_lb=b, con_ub=b, var_lb=np.zeros(n), var_ub=np.full(n, np.inf))
+
+
+def planted_family(m: int, n: int, nnz_per_col: int, seed: int, p: int = 32, scale: float = 0.1) -> PlantedFamily:
+    base = planted_lp(m, n, nnz_per_col, seed)
+    rng = np.random.default_rng(seed + 2_000_003)
+    sd = scale / np.sqrt(p)
+    W = sd * rng.standard_normal((n, p))
+    G = sd * rng.standard_normal((m, p))
+    H = sd * rng.standard_normal((n, p))
+    U = base.A @ (base.x_star[:, None] * W)
+    V = base.A.T @ G + base.s_star[:, None] * H
+    return PlantedFamily(base=base, W=W, G=G, H=H, U=np.ascontiguousarray(U), V=np.ascontiguousarray(V))
+
+
+def config4_family(seed: int = 4, scale: float = 1.0) -> PlantedFamily:
+    """m=5e4, n=1e5, 8 nonzeros per column; b = b0 + U theta_b, c = c0 + V theta_c
+    with dense U (m x 32), V (n x 32) around a planted optimum, 0.1 relative."""
     m = int(round(50_000 * scale))
-    return lp_family(m, 2 * m, 8, seed)
+    return planted_family(m, 2 * m, 8, seed)
 
 
 def config4_thetas(count: int, seed: int, p: int = 32) -> np.ndarray:

@@ -117,26 +117,38 @@ def load_ws() -> WarmStartLib:
 
 
 class LearnedWarmStart:
-    """Warm start for a fixed-A LP family: theta -> MLP -> standardized POD
-    coefficients (+ log primal weight) -> (x_hat, y_hat, pw_hat).
+    """Warm start for a fixed-A LP family: theta -> POD coefficients (+ log
+    primal weight) -> (x_hat, y_hat, pw_hat).
 
     fit(): POD of the training solutions (mean + the leading kx modes of x and
     ky modes of y, from the eigendecomposition of the snapshot Gram matrix in
-    FP64 on the host), then the MLP (widths[0] = len(theta), widths[-1] outputs)
-    trained on the device to predict the coefficients divided by their
-    per-mode standard deviation.  When the training solves' final primal
-    weights are given, the last output predicts their standardized logarithm:
-    a PDHG start near the solution pays off only together with a primal weight
-    near the one the solve ends with.  predict(): MLP forward + POD expansion on
-    the device; x_hat is clipped to the variable bounds given to fit()."""
+    FP64 on the host).  The coefficients are predicted by an affine head
+    (least squares in FP64 on [theta, 1]) plus the MLP (widths[0] = len(theta),
+    widths[-1] outputs), trained on the device on what the head leaves.  While
+    a family's optimal basis does not change, its solution is an affine
+    function of theta (x_B = A_B^-1 b(theta), y = A_B^-T c_B(theta) for
+    b, c affine in theta), which the head fits exactly; the network carries
+    the part that changes of active set make nonlinear.  The network's
+    targets are the head's residuals divided by one scale per block (x modes,
+    y modes, primal weight): the POD modes are orthonormal, so its loss is the
+    solution-space error of each block, and modes that only hold the training
+    solves' own KKT noise weigh as little as they are worth.  `head=False`
+    gives the network alone, on per-mode standardized coefficients.  When the
+    training solves' final primal weights are given, the last output predicts
+    their logarithm: a PDHG start near the solution pays off only together with
+    a primal weight near the one the solve ends with.  predict(): head + MLP
+    forward + POD expansion on the device; x_hat is clipped to the variable
+    bounds given to fit()."""
 
-    def __init__(self, lib: WarmStartLib, widths=(64, 512, 512, 512, 256), kx: int = 128, seed: int = 0):
+    def __init__(self, lib: WarmStartLib, widths=(64, 512, 512, 512, 256), kx: int = 128, seed: int = 0,
+                 head: bool = True):
         if kx >= widths[-1] - 1:
             raise ValueError("kx must leave room for the dual modes")
         self.lib = lib
         self.widths = list(widths)
         self.kx = kx
         self.seed = seed
+        self.head = head
         self.mlp = None
 
     @staticmethod
@@ -150,6 +162,10 @@ class LearnedWarmStart:
         modes = (C0.T @ V) / np.sqrt(w)  # dim x k, orthonormal columns
         energy = float(w.sum() / max(np.trace(gram), 1e-300))
         return mean, modes.T.copy(), energy  # basis k x dim
+
+    @staticmethod
+    def _design(thetas: np.ndarray) -> np.ndarray:
+        return np.hstack([thetas, np.ones((thetas.shape[0], 1))])
 
     def fit(self, thetas, X, Y, pw=None, lb=None, ub=None, epochs: int = 300, batch: int = 256,
             lr: float = 1e-3) -> dict:
@@ -168,9 +184,24 @@ class LearnedWarmStart:
         if self.has_pw:
             coef[:, -1] = np.log(np.asarray(pw, dtype=np.float64))
         self.kx_eff, self.ky_eff = kx, ky
-        self.cmu = coef.mean(axis=0)
-        self.csd = coef.std(axis=0) + 1e-12
-        target = (coef - self.cmu) / self.csd
+        if self.head:
+            D = self._design(thetas)
+            self.Wlin = np.linalg.lstsq(D, coef, rcond=None)[0]
+            resid = coef - D @ self.Wlin
+            self.cmu = np.zeros(out)
+            scale = np.ones(out)
+            blocks = [slice(0, self.kx), slice(self.kx, out - 1 if self.has_pw else out)]
+            if self.has_pw:
+                blocks.append(slice(out - 1, out))
+            for blk in blocks:
+                scale[blk] = np.sqrt(np.mean(resid[:, blk] ** 2)) + 1e-300
+            self.csd = scale
+        else:
+            self.Wlin = None
+            resid = coef
+            self.cmu = coef.mean(axis=0)
+            self.csd = coef.std(axis=0) + 1e-12
+        target = (resid - self.cmu) / self.csd
         self.lb = None if lb is None else np.asarray(lb, dtype=np.float64)
         self.ub = None if ub is None else np.asarray(ub, dtype=np.float64)
         if self.mlp is not None:
@@ -180,12 +211,27 @@ class LearnedWarmStart:
                               seed=self.seed)
         # reconstruction error of the POD itself on the training set (the floor of any prediction)
         recon_x = self.mx + ((X - self.mx) @ self.Bx.T) @ self.Bx
-        return {"pod_energy_x": ex, "pod_energy_y": ey, "loss_first": float(hist[0]), "loss_last": float(hist[-1]),
+        info = {"pod_energy_x": ex, "pod_energy_y": ey, "loss_first": float(hist[0]), "loss_last": float(hist[-1]),
                 "pod_rel_err_x": float(np.linalg.norm(recon_x - X) / np.linalg.norm(X)), "epochs": epochs}
+        if self.head:
+            info["head_rel_resid"] = float(np.linalg.norm(resid) / max(np.linalg.norm(coef), 1e-300))
+        return info
 
-    def predict(self, thetas):
+    def predict_coef(self, thetas, mlp: bool = True) -> np.ndarray:
+        """Predicted coefficients (FP64); mlp=False: the affine head alone."""
+        thetas = np.asarray(thetas, dtype=np.float64)
+        coef = np.zeros((thetas.shape[0], self.widths[-1]))
+        if self.head:
+            coef += self._design(thetas) @ self.Wlin
+        elif not mlp:
+            raise ValueError("no affine head to predict with")
+        if mlp:
+            coef += self.mlp.forward(thetas.astype(np.float32)).astype(np.float64) * self.csd + self.cmu
+        return coef
+
+    def predict(self, thetas, mlp: bool = True):
         """(x_hat, y_hat, pw_hat): pw_hat is None when fit() had no primal weights."""
-        coef = self.mlp.forward(np.asarray(thetas, dtype=np.float32)).astype(np.float64) * self.csd + self.cmu
+        coef = self.predict_coef(thetas, mlp)
         cx = coef[:, : self.kx_eff].astype(np.float32)
         cy = coef[:, self.kx: self.kx + self.ky_eff].astype(np.float32)
         xh = self.lib.pod_expand(self.Bx.astype(np.float32), self.mx.astype(np.float32), cx)
