@@ -89,3 +89,32 @@ def test_shared_engine_repeated_solves_and_starts(gpu_lp, monkeypatch):
         assert b.solve() == len(insts)
         warm = [b.result(k) for k in range(len(insts))]
     assert sum(w.iterations for w in warm) < 0.3 * sum(r.iterations for r in first)
+
+
+@pytest.mark.parametrize("count", [150, 300])
+def test_shared_packing_is_bitwise_neutral(gpu_lp, monkeypatch, count):
+    """Packing the running instances into narrower layouts as others stop
+    (256 -> 128 -> 64 slots, or 512 -> ...) leaves every instance's arithmetic
+    as it was: the same bits as the solve that never packs."""
+    monkeypatch.setenv("CP_AL_LAMBDA", "0")
+    from pdal_amd.generators import planted_family
+    fam = planted_family(150, 300, 8, seed=6)
+    insts = [fam.instance(t) for t in config4_thetas(count, 8)]
+    params = gpu_lp.default_params(termination_evaluation_frequency=K, eps_optimal_relative=1e-6,
+                                   eps_feasible_relative=1e-6, iteration_limit=100000)
+    runs = []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("PDAL_BATCH_NO_COMPACT", flag)
+        print(f"[case] shared-A packing count={count} no_compact={flag}", flush=True)
+        with LpBatch(gpu_lp, insts, params) as b:
+            assert b.stats()["shared_a"]
+            assert b.solve() == count
+            st = b.stats()
+            assert (st["compactions"] > 0) == (flag == "0")
+            runs.append([b.result(k) for k in range(count)])
+    for a, c in zip(*runs):
+        assert a.status == c.status == "OPTIMAL" and a.iterations == c.iterations
+        assert np.array_equal(a.x, c.x) and np.array_equal(a.y, c.y)
+        assert np.array_equal(a.reduced_cost, c.reduced_cost)
+    its = [r.iterations for r in runs[0]]
+    assert max(its) > min(its)  # the instances stop at different blocks
