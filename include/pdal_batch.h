@@ -46,7 +46,7 @@ int pdal_cp_al_batch_solve(pdal_cp_al_batch_t *batch);
  * NULL before the first solve or for an invalid index. */
 cp_al_result_t *pdal_cp_al_batch_result(pdal_cp_al_batch_t *batch, int i);
 
-/* out[0..14] = {set-up seconds, last solve seconds (wall clock, device
+/* out[0..15] = {set-up seconds, last solve seconds (wall clock, device
  * resident), hipGraph replays, instances, OPTIMAL instances, sum of
  * iteration counts, lanes per row of A, lanes per row of At, block-table
  * compactions (stopped instances are dropped from the launch tables and the
@@ -57,7 +57,10 @@ cp_al_result_t *pdal_cp_al_batch_result(pdal_cp_al_batch_t *batch, int i);
  * staged whole, A.x_bar staged in two column halves, 1 when the batch runs
  * the shared-A engine (every instance has the same constraint matrix: the
  * instances' vectors side by side, one pass over A serves them all; up to 512
- * instances)}. */
+ * instances; its compactions pack the running instances into narrower
+ * layouts), 1 when the last shared-A solve handed its last few running
+ * instances (8 or fewer) to the block-diagonal engine}.  `out` holds 16
+ * doubles. */
 void pdal_cp_al_batch_stats(const pdal_cp_al_batch_t *batch, double *out);
 
 /* Launch `iters` plain iterations over all instances kernel by kernel

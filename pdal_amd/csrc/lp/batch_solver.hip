@@ -149,6 +149,7 @@ class BatchSolver {
     // at the smallest width that holds them (a new block graph is captured).
     bool shared_ = false;
     int Bp_ = 0, NK_ = 0, Bp_full_ = 0, NK_full_ = 0, cur_ = 0;
+
     struct SharedBufs {
         DBuf<double> c, lb, ub, l, u, x0, x, xT, xH, xR, ds, y0, y, yT, yH, yR;
     } sh_[2];
@@ -169,6 +170,14 @@ class BatchSolver {
     void sh_end();
     void sh_drop_compact_graph();
     int sh_compact();
+    int sh_handoff();
+    // The last few running instances go to the block-diagonal engine: a
+    // shared-A pass costs about the same for 2 running slots as for 64 (it is
+    // latency-bound), while a block-diagonal pass costs in proportion to the
+    // instances it holds (PDAL_BATCH_NO_HANDOFF=1: never).
+    static constexpr int kShHandoff = 8;
+    bool handoff_ = true;
+    int handoffs_ = 0;
     bool res_ready_ = false;
     DBuf<double> rc_all_, xo_all_, yo_all_;
     bool full_tables_ = true, compact_ = true, use_graph_ = true;
@@ -623,6 +632,32 @@ void BatchSolver::sh_end()
     sh_drop_compact_graph();
 }
 
+int BatchSolver::sh_handoff()
+{   // every iterate of the current layout back into the concatenated vectors,
+    // then the block-diagonal tables and graph over the running instances
+    PDAL_HIP(hipStreamSynchronize(stream_));
+    const long m = m_ / N_, n = n_ / N_;
+    const int *ids = sh_ids_[cur_].get();
+    SharedBufs &b = sh_[cur_];
+    auto put = [&](const DBuf<double> &in, DBuf<double> &cat, long rows) {
+        sh_scatter_kernel<<<blocks_for_l(rows * Bp_), kBlock, 0, stream_>>>(in.get(), cat.get(), rows, Bp_, ids);
+    };
+    put(b.x0, x0_, n);
+    put(b.x, x_, n);
+    put(b.xT, xT_, n);
+    put(b.xH, xH_, n);
+    put(b.xR, xR_, n);
+    put(b.ds, dslack_, n);
+    put(b.y0, y0_, m);
+    put(b.y, y_, m);
+    put(b.yT, yT_, m);
+    put(b.yH, yH_, m);
+    put(b.yR, yR_, m);
+    PDAL_LAUNCHED();
+    sh_drop_compact_graph();
+    return compact();
+}
+
 void BatchSolver::sh_drop_compact_graph()
 {
     if (sh_graph_c_) (void)hipGraphExecDestroy(sh_graph_c_);
@@ -687,6 +722,7 @@ void BatchSolver::setup()
     const auto t0 = std::chrono::steady_clock::now();
     use_graph_ = !(getenv("PDAL_NO_GRAPH") && atoi(getenv("PDAL_NO_GRAPH")) != 0);
     compact_ = !(getenv("PDAL_BATCH_NO_COMPACT") && atoi(getenv("PDAL_BATCH_NO_COMPACT")) != 0);
+    handoff_ = !(getenv("PDAL_BATCH_NO_HANDOFF") && atoi(getenv("PDAL_BATCH_NO_HANDOFF")) != 0);
     xcd_ = (getenv("PDAL_BATCH_NO_XCD") && atoi(getenv("PDAL_BATCH_NO_XCD")) != 0) ? 0 : 1;
     staged_ = !(getenv("PDAL_BATCH_NO_STAGE") && atoi(getenv("PDAL_BATCH_NO_STAGE")) != 0);
     if (const char *raw = getenv("PDAL_BATCH_MIN_STAGED")) min_staged_ = std::max(1, atoi(raw));
@@ -1187,11 +1223,12 @@ int BatchSolver::solve()
     // Two replays in flight: replay r+1 is queued before the active count of
     // replay r is read.  Replays after the last instance stopped are no-ops.
     long launched = 0, checked = 0;
-    bool timed_out = false;
+    bool timed_out = false, handed = false;  // handed: the shared-A solve went on block-diagonally
     double next_report = 20.0;
+    handoffs_ = 0;
     while (true) {
         while (launched < checked + 2) {
-            if (shared_) {
+            if (shared_ && !handed) {
                 hipGraphExec_t g = cur_ == 0 && Bp_ == Bp_full_ ? sh_graph_ : sh_graph_c_;
                 if (g) PDAL_HIP(hipGraphLaunch(g, stream_));
                 else sh_launch_block();
@@ -1225,12 +1262,18 @@ int BatchSolver::solve()
             timed_out = true;
             break;
         }
-        if (!shared_ && compact_ && active <= table_instances_ / 2) {  // the replays in flight finish first
+        if ((!shared_ || handed) && compact_ && active <= table_instances_ / 2) {  // the replays in flight finish first
             const int still = compact();
             checked = launched;
             if (still == 0) break;
             ++compactions_;
-        } else if (shared_ && compact_ && NK_ > 1 && active <= Bp_ / 2) {
+        } else if (shared_ && !handed && handoff_ && active <= kShHandoff) {
+            const int still = sh_handoff();
+            handed = true;
+            ++handoffs_;
+            checked = launched;
+            if (still == 0) break;
+        } else if (shared_ && !handed && compact_ && NK_ > 1 && active <= Bp_ / 2) {
             const int still = sh_compact();
             checked = launched;
             if (still == 0) break;
@@ -1240,7 +1283,7 @@ int BatchSolver::solve()
     PDAL_HIP(hipStreamSynchronize(stream_));
     solve_s_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     replays_ = launched;
-    if (shared_) sh_end();
+    if (shared_ && !handed) sh_end();
     final_.resize(N_);
     st_.download(final_.data(), N_, stream_);
     PDAL_HIP(hipStreamSynchronize(stream_));
@@ -1337,7 +1380,7 @@ void BatchSolver::time_kernels(int iters, double *out)
 {   // every instance active; hipEvents around each fused SpMV kernel on the batch stream
     reset();
     if (shared_) {  // the shared-A kernels: non-major iterations, events on the batch stream
-        sh_begin();
+            sh_begin();
         hipEvent_t e[3];
         for (auto &ev : e) PDAL_HIP(hipEventCreate(&ev));
         const int m = (int)(m_ / N_), n = (int)(n_ / N_);
@@ -1347,21 +1390,18 @@ void BatchSolver::time_kernels(int iters, double *out)
         const ShVecs v = sh_vecs();
         const int ga = std::max(1, std::min(4096, (m + kShWaves - 1) / kShWaves));
         const int gat = std::max(1, std::min(4096, (n + kShWaves - 1) / kShWaves));
+        const int *ids = sh_ids_[0].get();
         double primal = 0.0, dual = 0.0;
         for (int it = 1; it <= iters; ++it) {
             PDAL_HIP(hipEventRecord(e[0], stream_));
-            auto run = [&](auto nk, int which) {
-                constexpr int NK = decltype(nk)::value;
-                if (which == 0) sh_primal_kernel<NK, false><<<gat, kBlock, 0, stream_>>>(At, v.y, v, st_.get(), sh_ids_[0].get(), Bp_, rho_, it);
-                else sh_dual_kernel<NK, false><<<ga, kBlock, 0, stream_>>>(A, v, st_.get(), sh_ids_[0].get(), Bp_, rho_, it);
-            };
             for (int which = 0; which < 2; ++which) {
-                switch (NK_) {
-                case 1: run(std::integral_constant<int, 1>{}, which); break;
-                case 2: run(std::integral_constant<int, 2>{}, which); break;
-                case 4: run(std::integral_constant<int, 4>{}, which); break;
-                default: run(std::integral_constant<int, 8>{}, which); break;
-                }
+                sh_dispatch(NK_, [&](auto nk) {
+                    constexpr int NK = decltype(nk)::value;
+                    if (which == 0)
+                        sh_primal_kernel<NK, false><<<gat, kBlock, 0, stream_>>>(At, v.y, v, st_.get(), ids, Bp_, rho_, it);
+                    else
+                        sh_dual_kernel<NK, false><<<ga, kBlock, 0, stream_>>>(A, v, st_.get(), ids, Bp_, rho_, it);
+                });
                 PDAL_HIP(hipEventRecord(e[which + 1], stream_));
             }
             PDAL_LAUNCHED();
@@ -1442,7 +1482,8 @@ void BatchSolver::set_primal_weights(const double *pw)
 void BatchSolver::stats(double *out) const
 {   // {set-up s, last solve s, graph replays, instances, optimal, total iterations, lanes A, lanes At,
     //  table compactions, s until half stopped, s until 99 % stopped, instances staged by
-    //  primal_staged_kernel, dual_staged_kernel, dual_split_kernel when all run, shared-A engine}
+    //  primal_staged_kernel, dual_staged_kernel, dual_split_kernel when all run, shared-A engine,
+    //  hand-offs of a shared-A solve to the block-diagonal engine}
     int optimal = 0;
     double iters = 0.0;
     for (const InstState &s : final_) {
@@ -1472,6 +1513,7 @@ void BatchSolver::stats(double *out) const
     out[12] = sd;
     out[13] = sx;
     out[14] = shared_ ? 1.0 : 0.0;
+    out[15] = (double)handoffs_;
 }
 
 }  // namespace batch

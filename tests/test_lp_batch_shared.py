@@ -102,6 +102,7 @@ def test_shared_packing_is_bitwise_neutral(gpu_lp, monkeypatch, count):
     insts = [fam.instance(t) for t in config4_thetas(count, 8)]
     params = gpu_lp.default_params(termination_evaluation_frequency=K, eps_optimal_relative=1e-6,
                                    eps_feasible_relative=1e-6, iteration_limit=100000)
+    monkeypatch.setenv("PDAL_BATCH_NO_HANDOFF", "1")  # the block-diagonal engine sums in another order
     runs = []
     for flag in ("1", "0"):
         monkeypatch.setenv("PDAL_BATCH_NO_COMPACT", flag)
@@ -118,3 +119,33 @@ def test_shared_packing_is_bitwise_neutral(gpu_lp, monkeypatch, count):
         assert np.array_equal(a.reduced_cost, c.reduced_cost)
     its = [r.iterations for r in runs[0]]
     assert max(its) > min(its)  # the instances stop at different blocks
+
+
+def test_shared_handoff_tracks_oracle(oracle_lp, gpu_lp, monkeypatch):
+    """The last running instances (8 or fewer) finish on the block-diagonal
+    engine: each instance still follows the oracle's solve (same iteration
+    count, objective within 1e-6), those that finished on either side."""
+    monkeypatch.setenv("CP_AL_LAMBDA", "0")
+    from pdal_amd.generators import planted_family
+    fam = planted_family(150, 300, 8, seed=6)
+    count = 100
+    insts = [fam.instance(t) for t in config4_thetas(count, 10)]
+    params = dict(termination_evaluation_frequency=K, eps_optimal_relative=1e-6, eps_feasible_relative=1e-6,
+                  iteration_limit=100000)
+    print("[case] shared-A hand-off", flush=True)
+    with LpBatch(gpu_lp, insts, gpu_lp.default_params(**params)) as b:
+        assert b.solve() == count
+        st = b.stats()
+        assert st["shared_a"] and st["handoffs"] == 1
+        res = [b.result(k) for k in range(count)]
+    its = np.array([r.iterations for r in res])
+    late = list(np.argsort(its)[-10:])  # the last to stop: 8 of them finished block-diagonally
+    early = list(np.argsort(its)[:4])
+    for k in late + early:
+        oracle_lp.reset_rng()
+        o = oracle_lp.solve(**insts[k], params=oracle_lp.default_params(**params))
+        g = res[k]
+        assert g.status == o.status == "OPTIMAL"
+        assert g.iterations == o.iterations, k
+        assert abs(g.primal_objective - o.primal_objective) <= 1e-6 * abs(o.primal_objective)
+        assert rel_err(g.x, o.x) <= 1e-6
