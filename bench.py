@@ -8,9 +8,10 @@ members solved in one batch to 1e-8 (iteration cap 40,000), the affine head +
 MLP (64 -> 512 -> 512 -> 512 -> 256, BF16 MFMA, trained on the device) fitted
 on their POD coefficients and final primal weights, and the batch of 256
 held-out members set up in HBM (shared-A engine).  A step = predict the starts
-and primal weights of the 256 held-out members (MLP forward + POD expansion),
-hand them to the batch, solve all 256 to relative KKT 1e-6 (alpha = 0, blocks
-of 64 iterations), cold re-solve of any member the warm pass leaves unsolved.
+and primal weights of the 256 held-out members (MLP forward + affine head,
+POD expansion written into the batch's start buffers on the device), solve all
+256 to relative KKT 1e-6 (alpha = 0, blocks of 64 iterations), cold re-solve
+of any member the warm pass leaves unsolved.
 `value` = held-out solves per second over the K timed steps.
 
 The same JSON line carries:
@@ -303,9 +304,7 @@ def run_config4(args, lib) -> dict:
     log(f"cold batched: {count} in {cold_s:.2f} s, iterations mean {cold_its.mean():.0f} max {cold_its.max()}")
 
     def warm_step():
-        xh, yh, pwh = lws.predict(te)
-        batch.set_starts(xh.ravel(), yh.ravel())
-        batch.set_primal_weights(pwh)
+        lws.predict_into(batch, te)  # MLP + head, POD expansion into the batch's start buffers on the device
         n_opt = batch.solve()
         if n_opt < count:  # the cold fallback for what the warm pass left unsolved
             bad = [k for k in range(count) if batch.result(k).status != "OPTIMAL"]

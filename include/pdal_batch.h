@@ -39,6 +39,15 @@ pdal_cp_al_batch_t *pdal_cp_al_batch_create(lp_problem_t *const *problems, int c
  * starts; either may be NULL to keep the current one.  Returns 0 on success. */
 int pdal_cp_al_batch_set_starts(pdal_cp_al_batch_t *batch, const double *x, const double *y);
 
+/* Device addresses (current device) of the batch's unscaled start buffers:
+ * *x holds sum n_i doubles and *y sum m_i, concatenated in instance order as
+ * set_starts takes them.  A producer on the device (pdal_pod_expand_device)
+ * writes the starts there, then pdal_cp_al_batch_set_starts_from_buffers
+ * takes them for the next solves as set_starts would; no host copy.  The
+ * buffers live as long as the batch.  Returns 0 on success. */
+int pdal_cp_al_batch_start_buffers(pdal_cp_al_batch_t *batch, double **x, double **y);
+int pdal_cp_al_batch_set_starts_from_buffers(pdal_cp_al_batch_t *batch);
+
 /* Solve every instance; returns the number that reached OPTIMAL, or -1. */
 int pdal_cp_al_batch_solve(pdal_cp_al_batch_t *batch);
 

@@ -12,8 +12,9 @@
  * dense, ReLU between hidden layers, none after the last; the loss is the mean
  * squared error over the batch and the outputs.
  *
- * All arrays are host memory, row-major, float32 unless stated; every entry
- * point returns 0 on success and a negative value on error.
+ * All arrays are host memory, row-major, float32 unless stated (the one
+ * device address is pdal_pod_expand_device's output); every entry point
+ * returns 0 on success and a negative value on error.
  */
 #pragma once
 
@@ -51,6 +52,20 @@ int pdal_mlp_train(pdal_mlp_t *mlp, const float *x, const float *y, int n, int b
  * FP32 products accumulated in FP32, stored as FP64 (a start point). */
 int pdal_pod_expand(const float *basis, const float *mean, int dim, int k, const float *coef, int batch,
                     double *out);
+
+/* A POD basis kept on the device: basis (k x dim) and mean (dim) uploaded
+ * once, with optional bounds lo, hi (dim float64 each, may be NULL) that clip
+ * every expanded entry.  NULL on error. */
+typedef struct pdal_pod pdal_pod_t;
+pdal_pod_t *pdal_pod_create(const float *basis, const float *mean, int dim, int k, const double *lo,
+                            const double *hi);
+void pdal_pod_free(pdal_pod_t *pod);
+
+/* out (batch x dim float64, a DEVICE address on the current device, e.g. the
+ * start buffer of a batch: pdal_cp_al_batch_start_buffers) = clip(mean +
+ * coef . basis), coef (batch x k float32) in host memory.  Returns when the
+ * expansion has finished. */
+int pdal_pod_expand_device(pdal_pod_t *pod, const float *coef, int batch, double *out);
 
 #ifdef __cplusplus
 }

@@ -70,6 +70,9 @@ class BatchSolver {
     ~BatchSolver();
     void setup();
     void set_starts(const double *x, const double *y);  // concatenated, unscaled; nullptr = keep
+    void start_buffers(double **x, double **y);          // device buffers set_starts_from_buffers reads
+    void scale_starts(bool x, bool y);
+    DBuf<double> sx_in_, sy_in_;                         // unscaled starts, concatenated
     int solve();
     cp_al_result_t *result(int i);
     void stats(double *out) const;
@@ -1296,25 +1299,37 @@ int BatchSolver::solve()
 }
 
 void BatchSolver::set_starts(const double *x, const double *y)
-{   // unscaled starts -> scaled space: x * D * b_scale, y * E * c_scale
-    DBuf<double> hx, hy;
+{   // host arrays into the start buffers, then as set_starts_from_buffers
     if (x) {
-        hx.alloc(n_);
-        hx.upload(x, n_, stream_);
+        if (!sx_in_.get()) sx_in_.alloc(n_);
+        sx_in_.upload(x, n_, stream_);
     }
     if (y) {
-        hy.alloc(m_);
-        hy.upload(y, m_, stream_);
+        if (!sy_in_.get()) sy_in_.alloc(m_);
+        sy_in_.upload(y, m_, stream_);
     }
+    scale_starts(x != nullptr, y != nullptr);
+}
+
+void BatchSolver::start_buffers(double **x, double **y)
+{
+    if (!sx_in_.get()) sx_in_.alloc(n_);
+    if (!sy_in_.get()) sy_in_.alloc(m_);
+    *x = sx_in_.get();
+    *y = sy_in_.get();
+}
+
+void BatchSolver::scale_starts(bool x, bool y)
+{   // unscaled starts in sx_in_ / sy_in_ -> scaled space: x * D * b_scale, y * E * c_scale
     st_.upload(init_.data(), N_, stream_);
     if (x)
-        scale_start_kernel<<<blocks_for_l(n_), kBlock, 0, stream_>>>(col_inst_.get(), hx.get(), D_.get(), xs_.get(),
-                                                                    n_, row_inst_.get(), nullptr, E_.get(), nullptr,
-                                                                    0, st_.get());
+        scale_start_kernel<<<blocks_for_l(n_), kBlock, 0, stream_>>>(col_inst_.get(), sx_in_.get(), D_.get(),
+                                                                    xs_.get(), n_, row_inst_.get(), nullptr,
+                                                                    E_.get(), nullptr, 0, st_.get());
     if (y)
         scale_start_kernel<<<blocks_for_l(m_), kBlock, 0, stream_>>>(col_inst_.get(), nullptr, D_.get(), nullptr, 0,
-                                                                    row_inst_.get(), hy.get(), E_.get(), ys_.get(),
-                                                                    m_, st_.get());
+                                                                    row_inst_.get(), sy_in_.get(), E_.get(),
+                                                                    ys_.get(), m_, st_.get());
     PDAL_LAUNCHED();
     PDAL_HIP(hipStreamSynchronize(stream_));
 }
@@ -1536,6 +1551,8 @@ void *batch_create(lp_problem_t *const *probs, int count, const cp_al_parameters
 
 int batch_solve(void *h) { return static_cast<BatchSolver *>(h)->solve(); }
 void batch_set_starts(void *h, const double *x, const double *y) { static_cast<BatchSolver *>(h)->set_starts(x, y); }
+void batch_start_buffers(void *h, double **x, double **y) { static_cast<BatchSolver *>(h)->start_buffers(x, y); }
+void batch_set_starts_from_buffers(void *h) { static_cast<BatchSolver *>(h)->scale_starts(true, true); }
 cp_al_result_t *batch_result(void *h, int i) { return static_cast<BatchSolver *>(h)->result(i); }
 void batch_stats(void *h, double *out) { static_cast<BatchSolver *>(h)->stats(out); }
 void batch_free(void *h) { delete static_cast<BatchSolver *>(h); }

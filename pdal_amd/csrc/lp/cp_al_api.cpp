@@ -526,6 +526,30 @@ PDAL_API int pdal_cp_al_batch_set_starts(pdal_cp_al_batch_t *batch, const double
     }
 }
 
+PDAL_API int pdal_cp_al_batch_start_buffers(pdal_cp_al_batch_t *batch, double **x, double **y)
+{
+    if (!batch || !x || !y) return -1;
+    try {
+        batch_start_buffers(batch, x, y);
+        return 0;
+    } catch (const std::exception &e) {
+        fprintf(stderr, "[cp_al-mi355x] batch start buffers failed: %s\n", e.what());
+        return -2;
+    }
+}
+
+PDAL_API int pdal_cp_al_batch_set_starts_from_buffers(pdal_cp_al_batch_t *batch)
+{
+    if (!batch) return -1;
+    try {
+        batch_set_starts_from_buffers(batch);
+        return 0;
+    } catch (const std::exception &e) {
+        fprintf(stderr, "[cp_al-mi355x] batch start update failed: %s\n", e.what());
+        return -2;
+    }
+}
+
 PDAL_API int pdal_cp_al_batch_solve(pdal_cp_al_batch_t *batch)
 {
     if (!batch) return -1;

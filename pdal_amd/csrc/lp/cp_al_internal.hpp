@@ -39,6 +39,8 @@ double copy_calibration(long long bytes, int reps);
 void *batch_create(lp_problem_t *const *problems, int count, const cp_al_parameters_t *params);
 int batch_solve(void *batch);
 void batch_set_starts(void *batch, const double *x, const double *y);
+void batch_start_buffers(void *batch, double **x, double **y);
+void batch_set_starts_from_buffers(void *batch);
 cp_al_result_t *batch_result(void *batch, int i);
 void batch_stats(void *batch, double *out);
 void batch_free(void *batch);

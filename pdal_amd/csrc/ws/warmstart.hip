@@ -188,7 +188,8 @@ __global__ void __launch_bounds__(256) gather_rows_kernel(const float *__restric
 constexpr int kExpandRows = 16;
 __global__ void __launch_bounds__(256)
     pod_expand_kernel(const float *__restrict__ basis, const float *__restrict__ mean, int dim, int k,
-                      const float *__restrict__ coef, int batch, double *__restrict__ out)
+                      const float *__restrict__ coef, int batch, double *__restrict__ out,
+                      const double *__restrict__ lo = nullptr, const double *__restrict__ hi = nullptr)
 {
     extern __shared__ float cs[];  // kExpandRows x k
     const int b0 = blockIdx.y * kExpandRows;
@@ -206,8 +207,9 @@ __global__ void __launch_bounds__(256)
         for (int r = 0; r < kExpandRows; ++r) acc[r] += cs[r * k + j] * w;
     }
     const float mu = mean[d];
+    const double l = lo ? lo[d] : -INFINITY, h = hi ? hi[d] : INFINITY;
     for (int r = 0; r < kExpandRows; ++r)
-        if (b0 + r < batch) out[(size_t)(b0 + r) * dim + d] = (double)(acc[r] + mu);
+        if (b0 + r < batch) out[(size_t)(b0 + r) * dim + d] = fmin(fmax((double)(acc[r] + mu), l), h);
 }
 
 // ---------------------------------------------------------------------------
@@ -500,5 +502,77 @@ PDAL_API int pdal_pod_expand(const float *basis, const float *mean, int dim, int
         O.download(out, (size_t)batch * dim, s);
         PDAL_HIP(hipStreamSynchronize(s));
         (void)hipStreamDestroy(s);
+    });
+}
+
+// ---------------------------------------------------------------------------
+// POD model resident on the device: basis, mean and bounds uploaded once; the
+// expansion writes into a device buffer of the caller (a batch's start buffer).
+
+struct pdal_pod {
+    hipStream_t s = nullptr;
+    int dim = 0, k = 0;
+    pdal::DBuf<float> B, mu;
+    pdal::DBuf<double> lo, hi;
+    pdal::DBuf<float> C;
+    bool has_lo = false, has_hi = false;
+};
+
+PDAL_API pdal_pod_t *pdal_pod_create(const float *basis, const float *mean, int dim, int k, const double *lo,
+                                     const double *hi)
+{
+    if (!basis || !mean || dim <= 0 || k <= 0) return nullptr;
+    if ((size_t)pdal::ws::kExpandRows * k * sizeof(float) > 64 * 1024) return nullptr;  // LDS staging limit
+    pdal_pod *p = new pdal_pod;
+    const int rc = guarded("pod_create", [&] {
+        PDAL_HIP(hipStreamCreateWithFlags(&p->s, hipStreamNonBlocking));
+        p->dim = dim;
+        p->k = k;
+        p->B.alloc((size_t)k * dim);
+        p->mu.alloc(dim);
+        p->B.upload(basis, (size_t)k * dim, p->s);
+        p->mu.upload(mean, dim, p->s);
+        if (lo) {
+            p->lo.alloc(dim);
+            p->lo.upload(lo, dim, p->s);
+            p->has_lo = true;
+        }
+        if (hi) {
+            p->hi.alloc(dim);
+            p->hi.upload(hi, dim, p->s);
+            p->has_hi = true;
+        }
+        PDAL_HIP(hipStreamSynchronize(p->s));
+    });
+    if (rc != 0) {
+        if (p->s) (void)hipStreamDestroy(p->s);
+        delete p;
+        return nullptr;
+    }
+    return p;
+}
+
+PDAL_API void pdal_pod_free(pdal_pod_t *p)
+{
+    if (!p) return;
+    if (p->s) {
+        (void)hipStreamSynchronize(p->s);
+        (void)hipStreamDestroy(p->s);
+    }
+    delete p;
+}
+
+PDAL_API int pdal_pod_expand_device(pdal_pod_t *p, const float *coef, int batch, double *out)
+{
+    if (!p || !coef || !out || batch <= 0) return -1;
+    return guarded("pod_expand_device", [&] {
+        if (p->C.size() < (size_t)batch * p->k) p->C.alloc((size_t)batch * p->k);
+        p->C.upload(coef, (size_t)batch * p->k, p->s);
+        const dim3 grid((p->dim + 255) / 256, (batch + pdal::ws::kExpandRows - 1) / pdal::ws::kExpandRows);
+        pdal::ws::pod_expand_kernel<<<grid, 256, pdal::ws::kExpandRows * p->k * sizeof(float), p->s>>>(
+            p->B.get(), p->mu.get(), p->dim, p->k, p->C.get(), batch, out, p->has_lo ? p->lo.get() : nullptr,
+            p->has_hi ? p->hi.get() : nullptr);
+        PDAL_LAUNCHED();
+        PDAL_HIP(hipStreamSynchronize(p->s));
     });
 }

@@ -249,6 +249,10 @@ class CpAl:
             L.pdal_cp_al_batch_set_primal_weights.argtypes = [C.c_void_p, _dp]
             L.pdal_cp_al_batch_time_kernels.restype = C.c_int
             L.pdal_cp_al_batch_time_kernels.argtypes = [C.c_void_p, C.c_int, _dp]
+            L.pdal_cp_al_batch_start_buffers.restype = C.c_int
+            L.pdal_cp_al_batch_start_buffers.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]
+            L.pdal_cp_al_batch_set_starts_from_buffers.restype = C.c_int
+            L.pdal_cp_al_batch_set_starts_from_buffers.argtypes = [C.c_void_p]
 
     # ------------------------------------------------------------------
     def default_params(self, **overrides) -> CpAlParameters:
@@ -447,6 +451,18 @@ class LpBatch:
         rc = self.lib.lib.pdal_cp_al_batch_set_starts(self.h, _ptr(xs, C.c_double), _ptr(ys, C.c_double))
         if rc != 0:
             raise RuntimeError(f"pdal_cp_al_batch_set_starts failed ({rc})")
+
+    def start_buffers(self):
+        """Device addresses (ints) of the unscaled start buffers (x: sum n_i,
+        y: sum m_i doubles, concatenated) a device-side producer writes."""
+        px, py = C.c_void_p(), C.c_void_p()
+        if self.lib.lib.pdal_cp_al_batch_start_buffers(self.h, C.byref(px), C.byref(py)) != 0:
+            raise RuntimeError("pdal_cp_al_batch_start_buffers failed")
+        return px.value, py.value
+
+    def set_starts_from_buffers(self) -> None:
+        if self.lib.lib.pdal_cp_al_batch_set_starts_from_buffers(self.h) != 0:
+            raise RuntimeError("pdal_cp_al_batch_set_starts_from_buffers failed")
 
     def solve(self) -> int:
         n_opt = self.lib.lib.pdal_cp_al_batch_solve(self.h)

@@ -42,6 +42,12 @@ class WarmStartLib:
         L.pdal_mlp_train.argtypes = [C.c_void_p, _fp, _fp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_ulonglong, _fp]
         L.pdal_pod_expand.restype = C.c_int
         L.pdal_pod_expand.argtypes = [_fp, _fp, C.c_int, C.c_int, _fp, C.c_int, _dp]
+        L.pdal_pod_create.restype = C.c_void_p
+        L.pdal_pod_create.argtypes = [_fp, _fp, C.c_int, C.c_int, _dp, _dp]
+        L.pdal_pod_free.restype = None
+        L.pdal_pod_free.argtypes = [C.c_void_p]
+        L.pdal_pod_expand_device.restype = C.c_int
+        L.pdal_pod_expand_device.argtypes = [C.c_void_p, _fp, C.c_int, C.c_void_p]
 
     def create(self, widths, seed: int = 0) -> "Mlp":
         return Mlp(self, widths, seed)
@@ -56,6 +62,36 @@ class WarmStartLib:
         if rc != 0:
             raise RuntimeError(f"pdal_pod_expand failed ({rc})")
         return out
+
+
+class DevicePod:
+    """A POD basis kept on the device (pdal_pod_create); expand() writes
+    clip(mean + coef . basis) into a device address."""
+
+    def __init__(self, lib: WarmStartLib, basis, mean, lo=None, hi=None):
+        self.lib = lib
+        basis, mean = _f32(basis), _f32(mean)
+        self.k, self.dim = basis.shape
+        lo = None if lo is None else np.ascontiguousarray(lo, dtype=np.float64)
+        hi = None if hi is None else np.ascontiguousarray(hi, dtype=np.float64)
+        self.h = lib.lib.pdal_pod_create(basis.ctypes.data_as(_fp), mean.ctypes.data_as(_fp), self.dim, self.k,
+                                         None if lo is None else lo.ctypes.data_as(_dp),
+                                         None if hi is None else hi.ctypes.data_as(_dp))
+        if not self.h:
+            raise RuntimeError("pdal_pod_create failed")
+
+    def expand(self, coef, out_device: int) -> None:
+        coef = _f32(coef)
+        if coef.shape[1] != self.k:
+            raise ValueError("coefficient count does not match the basis")
+        if self.lib.lib.pdal_pod_expand_device(self.h, coef.ctypes.data_as(_fp), coef.shape[0],
+                                               C.c_void_p(out_device)) != 0:
+            raise RuntimeError("pdal_pod_expand_device failed")
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.lib.pdal_pod_free(self.h)
+            self.h = None
 
 
 class Mlp:
@@ -150,6 +186,7 @@ class LearnedWarmStart:
         self.seed = seed
         self.head = head
         self.mlp = None
+        self._dev = None  # device-resident bases of predict_into
 
     @staticmethod
     def _pod(S: np.ndarray, k: int):
@@ -206,6 +243,10 @@ class LearnedWarmStart:
         self.ub = None if ub is None else np.asarray(ub, dtype=np.float64)
         if self.mlp is not None:
             self.mlp.close()
+        if self._dev is not None:
+            for d in self._dev:
+                d.close()
+            self._dev = None
         self.mlp = self.lib.create(self.widths, seed=self.seed)
         hist = self.mlp.train(thetas, target, batch=min(batch, thetas.shape[0]), epochs=epochs, lr=lr,
                               seed=self.seed)
@@ -243,10 +284,31 @@ class LearnedWarmStart:
         pw = np.exp(coef[:, -1]) if self.has_pw else None
         return xh, yh, pw
 
+    def predict_into(self, batch, thetas, mlp: bool = True):
+        """predict() with the expansion written straight into the start
+        buffers of `batch` (an LpBatch of len(thetas) members, in order) and
+        handed to it on the device, with the predicted primal weights.  The
+        bases stay on the device between calls.  Returns pw_hat."""
+        coef = self.predict_coef(thetas, mlp)
+        if self._dev is None:
+            self._dev = (DevicePod(self.lib, self.Bx, self.mx, self.lb, self.ub), DevicePod(self.lib, self.By, self.my))
+        px, py = batch.start_buffers()
+        self._dev[0].expand(coef[:, : self.kx_eff], px)
+        self._dev[1].expand(coef[:, self.kx: self.kx + self.ky_eff], py)
+        batch.set_starts_from_buffers()
+        pw = np.exp(coef[:, -1]) if self.has_pw else None
+        if pw is not None:
+            batch.set_primal_weights(pw)
+        return pw
+
     def close(self) -> None:
         if self.mlp is not None:
             self.mlp.close()
             self.mlp = None
+        if self._dev is not None:
+            for d in self._dev:
+                d.close()
+            self._dev = None
 
 
 def solve_warm(lp_lib, instances: list, params, x_hat, y_hat, pw_hat=None):

@@ -103,6 +103,9 @@ def test_shared_packing_is_bitwise_neutral(gpu_lp, monkeypatch, count):
     params = gpu_lp.default_params(termination_evaluation_frequency=K, eps_optimal_relative=1e-6,
                                    eps_feasible_relative=1e-6, iteration_limit=100000)
     monkeypatch.setenv("PDAL_BATCH_NO_HANDOFF", "1")  # the block-diagonal engine sums in another order
+    # lane counts pinned: the set-up's power iteration of two batches must not autotune apart
+    monkeypatch.setenv("PDAL_BATCH_G_A", "16")
+    monkeypatch.setenv("PDAL_BATCH_G_AT", "8")
     runs = []
     for flag in ("1", "0"):
         monkeypatch.setenv("PDAL_BATCH_NO_COMPACT", flag)

@@ -222,6 +222,9 @@ def test_learned_warm_start_on_planted_family(ws, monkeypatch):
     is printed: on a family whose optimal basis does not move the solution
     is affine in theta and the head already fits it."""
     monkeypatch.setenv("CP_AL_LAMBDA", "0")
+    # lane counts pinned: two batches compared bit for bit must not autotune apart
+    monkeypatch.setenv("PDAL_BATCH_G_A", "16")
+    monkeypatch.setenv("PDAL_BATCH_G_AT", "8")
     from pdal_amd.generators import config4_thetas, planted_family
     from pdal_amd.lp import LpBatch, load_cp_al
     from pdal_amd.warmstart import LearnedWarmStart, solve_warm
@@ -257,9 +260,17 @@ def test_learned_warm_start_on_planted_family(ws, monkeypatch):
         info = lws.fit(tr, X, Y, pw=pw, lb=np.zeros(fam.n), epochs=150)
         xh, yh, pwh = lws.predict(te)
         xa, ya, _ = lws.predict(te, mlp=False)
+        # the same starts expanded on the device straight into the batch's start buffers
+        with LpBatch(lib, test_inst, params) as b:
+            pw_dev = lws.predict_into(b, te)
+            b.solve()
+            dev = [b.result(k) for k in range(len(te))]
     finally:
         lws.close()
+    np.testing.assert_array_equal(pw_dev, pwh)
     warm, winfo = solve_warm(lib, test_inst, params, xh, yh, pwh)
+    for d, w in zip(dev, warm):
+        assert d.iterations == w.iterations and np.array_equal(d.x, w.x) and np.array_equal(d.y, w.y)
     assert winfo["fallback"] == 0 and all(w.status == "OPTIMAL" for w in warm)
     for w, t in zip(warm, te):
         opt = fam.optimal_value(t)
