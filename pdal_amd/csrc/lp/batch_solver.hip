@@ -162,6 +162,8 @@ class BatchSolver {
     std::vector<int> h_ids_;    // the current set's, on the host
     hipGraphExec_t sh_graph_ = nullptr;    // full width, set 0
     hipGraphExec_t sh_graph_c_ = nullptr;  // the current packed layout
+    bool same_a_ = false;  // every instance has the first's constraint matrix
+    bool same_matrices() const;
     void detect_shared();
     void sh_alloc(int set, int width);
     ShVecs sh_vecs();
@@ -465,21 +467,31 @@ void BatchSolver::launch_dual(bool major, int i)
 // ---------------------------------------------------------------------------
 // Shared-A engine
 
-void BatchSolver::detect_shared()
+bool BatchSolver::same_matrices() const
 {   // every instance's CSR arrays equal the first's (sizes, pattern, values)
-    shared_ = false;
-    if (N_ < 2 || (getenv("PDAL_BATCH_NO_SHARED") && atoi(getenv("PDAL_BATCH_NO_SHARED")) != 0)) return;
+    if (N_ < 2) return false;
     const lp_problem_t *p0 = probs_[0];
     const int m = p0->num_constraints, n = p0->num_variables, nnz = p0->constraint_matrix_num_nonzeros;
-    if (m == 0 || n == 0 || nnz == 0) return;
+    if (m == 0 || n == 0 || nnz == 0) return false;
     for (int i = 1; i < N_; ++i) {
         const lp_problem_t *p = probs_[i];
-        if (p->num_constraints != m || p->num_variables != n || p->constraint_matrix_num_nonzeros != nnz) return;
+        if (p->num_constraints != m || p->num_variables != n || p->constraint_matrix_num_nonzeros != nnz) return false;
+        if (p->constraint_matrix_row_pointers == p0->constraint_matrix_row_pointers &&
+            p->constraint_matrix_col_indices == p0->constraint_matrix_col_indices &&
+            p->constraint_matrix_values == p0->constraint_matrix_values)
+            continue;
         if (memcmp(p->constraint_matrix_row_pointers, p0->constraint_matrix_row_pointers, (size_t)(m + 1) * sizeof(int)) ||
             memcmp(p->constraint_matrix_col_indices, p0->constraint_matrix_col_indices, (size_t)nnz * sizeof(int)) ||
             memcmp(p->constraint_matrix_values, p0->constraint_matrix_values, (size_t)nnz * sizeof(double)))
-            return;
+            return false;
     }
+    return true;
+}
+
+void BatchSolver::detect_shared()
+{
+    shared_ = false;
+    if (!same_a_ || (getenv("PDAL_BATCH_NO_SHARED") && atoi(getenv("PDAL_BATCH_NO_SHARED")) != 0)) return;
     int nk = 1;
     while (nk * kWave < N_) nk *= 2;
     if (nk > 8) return;  // up to 512 instances side by side
@@ -746,6 +758,7 @@ void BatchSolver::setup()
     }
     m_ = row_off_[N_];
     n_ = col_off_[N_];
+    same_a_ = same_matrices();
     if (m_ > INT32_MAX - 1 || n_ > INT32_MAX - 1 || nnz_ > INT32_MAX - 1)
         throw HipError("batch exceeds int32 indexing (rows, columns or nonzeros)");
     std::vector<int> rp((size_t)m_ + 1), ci(nnz_), rinst(m_), cinst(n_);
@@ -776,7 +789,28 @@ void BatchSolver::setup()
         }
     }
     prepare_split(rp, ci);
-    HostCsr T = transpose_csr((int)m_, (int)n_, rp.data(), ci.data(), v.data());
+    HostCsr T;
+    if (same_a_) {  // block diagonal of one matrix: transpose it once and repeat it with offsets
+        const lp_problem_t *p0 = probs_[0];
+        const int m0 = p0->num_constraints, n0 = p0->num_variables, z0 = p0->constraint_matrix_num_nonzeros;
+        const HostCsr T0 = transpose_csr(m0, n0, p0->constraint_matrix_row_pointers,
+                                         p0->constraint_matrix_col_indices, p0->constraint_matrix_values);
+        T.rows = (int)n_;
+        T.cols = (int)m_;
+        T.rp.resize((size_t)n_ + 1);
+        T.ci.resize(nnz_);
+        T.v.resize(nnz_);
+        for (int i = 0; i < N_; ++i) {
+            const long c0 = col_off_[i], z = (long)i * z0;
+            const int r0 = (int)row_off_[i];
+            for (int r = 0; r < n0; ++r) T.rp[c0 + r] = (int)(z + T0.rp[r]);
+            for (int k = 0; k < z0; ++k) T.ci[z + k] = r0 + T0.ci[k];
+            std::copy(T0.v.begin(), T0.v.end(), T.v.begin() + z);
+        }
+        T.rp[n_] = (int)nnz_;
+    } else {
+        T = transpose_csr((int)m_, (int)n_, rp.data(), ci.data(), v.data());
+    }
     auto put = [&](DevCsr &Dm, int rows, const int *hrp, const int *hci, const double *hv) {
         Dm.rows = rows;
         Dm.nnz = (int)nnz_;
@@ -1018,11 +1052,14 @@ void BatchSolver::power_iteration()
     }
     std::vector<double> v0(m_);
     std::vector<PowerState> ps0(N_);
+    // With one matrix for all instances every instance starts from the same
+    // prefix of the stream and follows the same iteration: only the first runs
+    // and its estimate is everyone's.
     for (int i = 0; i < N_; ++i) {
         const long mi = row_off_[i + 1] - row_off_[i];
         std::copy(stream.begin(), stream.begin() + mi, v0.begin() + row_off_[i]);
         const long nnz_i = probs_[i]->constraint_matrix_num_nonzeros;
-        ps0[i] = PowerState{0.0, 1.0, 0, nnz_i == 0 ? 1 : 0};  // no power iteration for an empty A
+        ps0[i] = PowerState{0.0, 1.0, 0, (nnz_i == 0 || (same_a_ && i > 0)) ? 1 : 0};  // no power iteration for an empty A
     }
     DBuf<double> v(m_), vh(m_), w(n_);
     DBuf<PowerState> ps(N_);
@@ -1068,9 +1105,17 @@ void BatchSolver::power_iteration()
         PDAL_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
         PDAL_HIP(hipGraphDestroy(graph));
     };
+    int in_tables = N_;
+    if (same_a_) {
+        std::vector<char> first(N_, 0);
+        first[0] = 1;
+        build_table(ta_, row_off_, ta_.G, &first);
+        build_table(tat_, col_off_, tat_.G, &first);
+        full_tables_ = false;
+        in_tables = 1;
+    }
     capture();
     std::vector<PowerState> h(N_);
-    int in_tables = N_;
     for (int done_iters = 0; done_iters < max_iter; done_iters += kChunk) {
         if (exec) PDAL_HIP(hipGraphLaunch(exec, stream_));
         else
@@ -1096,7 +1141,7 @@ void BatchSolver::power_iteration()
     restore_full_tables();
     for (int i = 0; i < N_; ++i) {
         const bool empty = probs_[i]->constraint_matrix_num_nonzeros == 0;
-        const double sv = std::sqrt(h[i].sigma_sq);
+        const double sv = std::sqrt(h[same_a_ ? 0 : i].sigma_sq);
         init_[i].lambda_est = empty ? 1.0 : sv * sv;
     }
 }
