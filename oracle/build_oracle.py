@@ -1,8 +1,9 @@
 """Build the CPU oracle shared objects (test infrastructure, gcc only).
 
-Output: oracle/_build/libcp_al_oracle.so, libfa_cp_oracle.so and
+Output: oracle/_build/libcp_al_oracle.so, libfa_cp_oracle.so,
 libfa_cp_oracle_rev.so (the FA_CP oracle summing in reverse order, the
-summation-order noise floor of a trajectory).  The oracle exports the same C names
+summation-order noise floor of a trajectory) and liblin_cp_al_oracle.so (the
+linearized CP-AL QP solver, the FA_CP source built with -DORACLE_LIN_CP_AL).  The oracle exports the same C names
 as the product library; it is built with hidden visibility and -Bsymbolic so a
 process can load both without the two binding to each other's symbols.
 
@@ -38,6 +39,7 @@ def _one(src: str, name: str, defines=()) -> Path:
 def build() -> Path:
     _one("fa_cp_oracle.cpp", "libfa_cp_oracle.so")
     _one("fa_cp_oracle.cpp", "libfa_cp_oracle_rev.so", ["-DFA_CP_ORACLE_REVERSE_SUMS"])
+    _one("fa_cp_oracle.cpp", "liblin_cp_al_oracle.so", ["-DORACLE_LIN_CP_AL"])
     return _one("cp_al_oracle.cpp", "libcp_al_oracle.so")
 
 

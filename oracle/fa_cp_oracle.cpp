@@ -29,8 +29,33 @@
  * product and norm sums its terms in reverse order.  That variant is the same
  * algorithm with a different (equally valid) rounding; tests use the distance
  * between the two builds as the summation-order noise floor of a trajectory.
+ *
+ * Built with -DORACLE_LIN_CP_AL it is the oracle of the linearized CP-AL
+ * solver instead (QP/LIN_CP_AL, types of lin_cp_al_types.h): the same
+ * scaling, power iterations, residual, infeasibility and termination code
+ * (LIN_CP_AL's utils.cu and solver_state.cu differ from FA_CP's only where
+ * marked), with LIN_CP_AL's own step sizes (cp_al_core_op.cu:609-691), update
+ * (:496-575: AL effective dual, one projected gradient step, equality dual),
+ * relaxed Halpern step (:187-206, 577-585), fixed-point error every iteration
+ * (:693-770), restart test (utils.cu:451-463) and restart (:595-607), main loop
+ * (solver.cu:34-114).  Its feasibility polishing is not restated (a request
+ * for it prints a notice and is ignored).
  */
+#ifdef ORACLE_LIN_CP_AL
+#include "lin_cp_al.h"
+typedef cp_al_parameters_t params_t;
+typedef cp_al_result_t result_t;
+// the shared code names the Q types by their FA_CP names
+constexpr quad_obj_type_t FA_CP_SPARSE_Q = CP_AL_SPARSE_Q;
+constexpr quad_obj_type_t FA_CP_DIAG_Q = CP_AL_DIAG_Q;
+constexpr quad_obj_type_t FA_CP_LOW_RANK_PLUS_SPARSE_Q = CP_AL_LOW_RANK_PLUS_SPARSE_Q;
+constexpr quad_obj_type_t FA_CP_LOW_RANK_Q = CP_AL_LOW_RANK_Q;
+constexpr quad_obj_type_t FA_CP_NON_Q = CP_AL_NON_Q;
+#else
 #include "fa_cp.h"
+typedef fa_cp_parameters_t params_t;
+typedef fa_cp_result_t result_t;
+#endif
 
 #include <algorithm>
 #include <chrono>
@@ -158,7 +183,7 @@ int print_frequency(int iter)  // utils.cu:690-703
 }
 
 struct S {
-    fa_cp_parameters_t P{};
+    params_t P{};
     int n = 0, m = 0, r = 0;
     double obj_const = 0.0;
     Csr A, At, Q, R, Rt;
@@ -187,6 +212,8 @@ struct S {
     int sigma_update_count = 0, sigma_dir = 1;
 
     double step = 0.0, pw = 1.0, best_pw = 1.0, pw_err_sum = 0.0, pw_last_err = 0.0, best_gap = INFINITY;
+    // linearized CP-AL: the fixed penalty sigma and its interval, lambda of tau = 1 / (sigma lambda)
+    double sigma = 0.0, sigma_base = 1.0, sigma_min = 0.0, sigma_max = 0.0, lambda_est = 1.0;
     int total = 0, inner = 0;
     bool major = false;
     double fp = INFINITY, fp0 = INFINITY, last_trial_fp = INFINITY;
@@ -543,7 +570,12 @@ void compute_residual(S &s, norm_type_t norm)
     if (s.is_lp) {  // compute_lp_residual_kernel :705-733
         for (int j = 0; j < s.n; ++j) s.dres[j] = (s.c[j] - s.dual_prod[j] - s.dslack[j]) * s.D[j];
     } else {  // compute_qp_residual_kernel :735-770
+#ifdef ORACLE_LIN_CP_AL
+        // LIN_CP_AL utils.cu:855-858: the AL step 1 / (sigma lambda)
+        const double tr = s.use_al ? 1.0 / std::fmax(s.sigma * s.lambda_est, 1e-16) : s.step / s.pw;
+#else
         const double tr = s.step / s.pw;
+#endif
         for (int j = 0; j < s.n; ++j) {
             const double g = s.qx[j] + s.c[j] - s.dual_prod[j];
             const double t = s.xT[j] - tr * g;
@@ -667,6 +699,7 @@ bool should_restart(S &s)
     return r;
 }
 
+#ifndef ORACLE_LIN_CP_AL
 void set_sigma_active(S &s, double sigma, double zero_thr)
 {   // :667-676
     if (!std::isfinite(sigma) || sigma <= zero_thr) {
@@ -981,6 +1014,148 @@ void feasibility_polish(S &s, double &polish_time, int &polish_iters)
     polish_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
+#else  // ORACLE_LIN_CP_AL
+
+/* ---------------- linearized CP-AL (QP/LIN_CP_AL/src/cp_al_core_op.cu) ---------------- */
+
+double lin_primal_step(const S &s) { return 1.0 / std::fmax(s.sigma * s.lambda_est, 1e-16); }  // :253-255
+double lin_dual_step(const S &s) { return std::fmax(s.sigma, 1e-16); }                          // :257-259
+
+void lin_init_steps(S &s)
+{   // initialize_step_size_and_primal_weight, :609-691
+    const double max_sv = s.A.nnz() > 0 ? max_singular_value(s) : 0.0;
+    const double a_sq = max_sv * max_sv;
+    const double half_lf = 0.5 * std::fmax(s.q_norm, 0.0);
+    s.pw = s.P.bound_objective_rescaling ? 1.0 : (s.c_norm + 1.0) / (s.b_norm + 1.0);
+    s.best_pw = s.pw;
+    if (s.use_al) {
+        if (a_sq <= 1e-16 && half_lf <= 1e-16) {
+            s.lambda_est = 1.0;
+            s.sigma_base = 1.0;
+        } else {  // t = sqrt(lambda) solves t^2 - (L_f / 2) t - 2 ||A||^2 = 0 (with 8.0004 for 8)
+            const double t = std::fmax(0.5 * (half_lf + std::sqrt(half_lf * half_lf + 8.0004 * a_sq)), 1e-16);
+            s.lambda_est = t * t;
+            s.sigma_base = 1.0 / t;
+        }
+        s.sigma_min = s.sigma_base;
+        s.sigma_max = 100.0 * s.sigma_base;
+        s.sigma = s.P.sigma_init > 0.0 ? s.P.sigma_init : 6.0 * s.sigma_base;
+        s.sigma = std::fmin(std::fmax(s.sigma, s.sigma_min), s.sigma_max);
+        s.step = lin_primal_step(s);
+    } else {
+        const double wgt = std::fmax(s.pw, 1e-16);
+        if (a_sq <= 1e-16 && half_lf <= 1e-16) {
+            s.step = 1.0;
+        } else if (a_sq <= 1e-16) {
+            s.step = 0.99 * wgt / std::fmax(half_lf, 1e-16);
+        } else {
+            const double lc = half_lf / wgt;
+            s.step = 0.99 * (-lc + std::sqrt(lc * lc + 4.0 * a_sq)) / (2.0 * a_sq);
+        }
+    }
+}
+
+void lin_update(S &s)
+{   // cp_al_update, :496-575: the T-outputs (xT, yT) and reflections every iteration
+    double tau, dstep, al_sigma = 0.0;
+    if (s.use_al) {
+        tau = lin_primal_step(s);
+        dstep = lin_dual_step(s);
+        al_sigma = s.sigma;
+        s.step = tau;
+    } else {
+        tau = s.step / s.pw;
+        dstep = s.step * s.pw;
+    }
+    if (s.nonconvexity < 0) {
+        tau = std::fmax(tau, -1.01 * std::fmin(0.0, s.nonconvexity));
+        tau /= 2;
+    }
+    if (s.use_al) {  // compute_al_effective_dual_product :275-302, kernel :149-158 (rhs = l)
+        spmv(s.A, s.x.data(), s.prim_prod.data());
+        for (int i = 0; i < s.m; ++i) s.al_u[i] = s.y[i] - al_sigma * (s.prim_prod[i] - s.l[i]);
+        spmv(s.At, s.al_u.data(), s.dual_prod.data());
+    } else {
+        spmv(s.At, s.y.data(), s.dual_prod.data());
+    }
+    if (s.qtype == FA_CP_NON_Q) {  // lp_primal_update :304-312, kernel :73-87
+        for (int j = 0; j < s.n; ++j) {
+            const double xj = s.x[j];
+            const double t = xj - tau * (s.c[j] - s.dual_prod[j]);
+            const double p = std::fmax(s.lb[j], std::fmin(t, s.ub[j]));
+            s.xR[j] = 2.0 * p - xj;
+            s.xT[j] = p;
+            s.dslack[j] = (p - t) / tau;
+        }
+    } else {  // primal_gradient_update :484-494, kernel :353-369 (one projected gradient step)
+        obj_product(s, s.x);
+        for (int j = 0; j < s.n; ++j) {
+            const double xj = s.x[j];
+            const double g = s.qx[j] + s.c[j] - s.dual_prod[j];
+            const double p = std::fmax(s.lb[j], std::fmin(xj - tau * g, s.ub[j]));
+            s.xT[j] = p;
+            s.xR[j] = 2.0 * p - xj;
+        }
+    }
+    spmv(s.A, s.xR.data(), s.prim_prod.data());
+    for (int i = 0; i < s.m; ++i) {  // kernels :135-147 and :173-185: rhs = l
+        const double nd = s.y[i] + dstep * (s.l[i] - s.prim_prod[i]);
+        s.yT[i] = nd;
+        s.yR[i] = 2.0 * nd - s.y[i];
+    }
+}
+
+void lin_halpern(S &s)
+{   // halpern_update :577-585, kernel :187-206: relaxed (1 + rho) T - rho z
+    const double w = (double)(s.inner + 1) / (s.inner + 2);
+    const double rho = s.P.reflection_coefficient;
+    for (int i = 0; i < s.n; ++i) {
+        const double rl = (1.0 + rho) * s.xT[i] - rho * s.x[i];
+        s.x[i] = w * rl + (1.0 - w) * s.x0[i];
+    }
+    for (int i = 0; i < s.m; ++i) {
+        const double rl = (1.0 + rho) * s.yT[i] - rho * s.y[i];
+        s.y[i] = w * rl + (1.0 - w) * s.y0[i];
+    }
+}
+
+void lin_fixed_point_error(S &s)
+{   // compute_fixed_point_error, :693-770 (delta = T-output - current)
+    for (int i = 0; i < s.n; ++i) s.dx[i] = s.xT[i] - s.x[i];
+    for (int i = 0; i < s.m; ++i) s.dy[i] = s.yT[i] - s.y[i];
+    const double dn = nrm2(s.dy), pn = nrm2(s.dx);
+    double fp2;
+    if (s.use_al) {
+        spmv(s.A, s.dx.data(), s.prim_prod.data());
+        const double an = nrm2(s.prim_prod);
+        const double inter = dot(s.prim_prod, s.dy);
+        fp2 = s.sigma * (s.lambda_est * pn * pn - an * an) + 2.0 * inter + dn * dn / s.sigma;
+    } else {
+        spmv(s.At, s.dy.data(), s.dual_prod.data());
+        const double cross = dot(s.dual_prod, s.dx);
+        const double tau = s.step / s.pw, rho = s.step * s.pw;
+        fp2 = pn * pn / std::fmax(tau, 1e-16) + dn * dn / std::fmax(rho, 1e-16) + 2.0 * cross;
+    }
+    s.fp = std::sqrt(std::fmax(0.0, fp2));
+}
+
+bool lin_should_restart(const S &s)
+{   // should_do_adaptive_restart, utils.cu:451-463
+    if (!std::isfinite(s.fp0) || !std::isfinite(s.fp)) return false;
+    if (s.total == s.inner) return s.inner >= s.P.termination_evaluation_frequency;
+    return s.fp <= s.P.restart_params.sufficient_reduction_for_restart * s.fp0;
+}
+
+void lin_restart(S &s)
+{   // perform_restart, :595-607: the anchors move to the current iterates
+    s.x0 = s.x;
+    s.y0 = s.y;
+    s.inner = 0;
+    s.last_trial_fp = INFINITY;
+}
+
+#endif  // ORACLE_LIN_CP_AL
+
 void init_state(S &s, const qp_problem_t *w)
 {   // solver_state.cu:494-877 after rescale_problem (preconditioner.c)
     s.n = w->num_variables;
@@ -1002,8 +1177,10 @@ void init_state(S &s, const qp_problem_t *w)
     s.Rt = transpose(s.R);
     s.al_available = s.P.use_al_qp;
     s.use_al = s.P.use_al_qp;
+#ifndef ORACLE_LIN_CP_AL
     s.al_sigma = s.P.al_sigma;
     s.best_al_sigma = s.P.al_sigma;
+#endif
     for (Vec *v : {&s.x0, &s.x, &s.xT, &s.xR, &s.dual_prod, &s.dslack, &s.dres, &s.dx, &s.qx, &s.grad, &s.dir, &s.gbuf})
         v->assign(s.n, 0.0);
     for (Vec *v : {&s.y0, &s.y, &s.yT, &s.yR, &s.prim_prod, &s.al_u, &s.pslack, &s.pres, &s.dy}) v->assign(s.m, 0.0);
@@ -1057,6 +1234,16 @@ void init_state(S &s, const qp_problem_t *w)
         s.q_norm = mxe;
         s.nonconvexity = mne;
     }
+#ifdef ORACLE_LIN_CP_AL
+    // LIN_CP_AL solver_state.cu:305-340: an inner loop only for the non-diagonal Q types
+    // (the linearized update never runs it)
+    const bool simple = s.qtype == FA_CP_NON_Q || s.qtype == FA_CP_DIAG_Q;
+    s.has_inner = !simple;
+    if (!simple) {
+        s.inner_limit = 1000;
+        s.inner_tol = 1e-3;
+    }
+#else
     if (s.al_available && s.P.al_sigma <= s.P.sigma_zero_threshold) {
         s.use_al = false;
         s.al_sigma = 0.0;
@@ -1069,6 +1256,7 @@ void init_state(S &s, const qp_problem_t *w)
         s.inner_limit = 1000;
         s.inner_tol = 1e-3;
     }
+#endif
     s.is_lp = s.qtype == FA_CP_NON_Q;
 }
 
@@ -1076,13 +1264,49 @@ void init_state(S &s, const qp_problem_t *w)
 
 /* ---------------- public C API ---------------- */
 
+#ifdef ORACLE_LIN_CP_AL
+ORACLE_API void lin_cp_al_oracle_reset_rng(void)
+#else
 ORACLE_API void fa_cp_oracle_reset_rng(void)
+#endif
 {
     g_gen.seed(1);
     g_dist.reset();
     srand(1);
 }
 
+#ifdef ORACLE_LIN_CP_AL
+ORACLE_API void set_default_parameters(cp_al_parameters_t *p)  // LIN_CP_AL utils.cu:466-497
+{
+    memset(p, 0, sizeof(*p));
+    p->l_inf_ruiz_iterations = 10;
+    p->has_pock_chambolle_alpha = true;
+    p->pock_chambolle_alpha = 1.0;
+    p->bound_objective_rescaling = true;
+    p->use_al_qp = true;
+    p->sigma_init = 0.0;
+    p->verbose = 1;
+    p->termination_evaluation_frequency = 200;
+    p->feasibility_polishing = false;
+    p->reflection_coefficient = 0.0;
+    p->sv_max_iter = 5000;
+    p->sv_tol = 1e-4;
+    p->termination_criteria.eps_optimal_relative = 1e-4;
+    p->termination_criteria.eps_feasible_relative = 1e-4;
+    p->termination_criteria.eps_infeasible = 1e-10;
+    p->termination_criteria.time_sec_limit = 3600.0;
+    p->termination_criteria.iteration_limit = INT32_MAX;
+    p->termination_criteria.eps_feas_polish_relative = 1e-6;
+    p->restart_params.artificial_restart_threshold = 0.36;
+    p->restart_params.sufficient_reduction_for_restart = 0.2;
+    p->restart_params.necessary_reduction_for_restart = 0.8;
+    p->restart_params.k_p = 0.99;
+    p->restart_params.k_i = 0.01;
+    p->restart_params.k_d = 0.0;
+    p->restart_params.i_smooth = 0.3;
+    p->optimality_norm = NORM_TYPE_L_INF;
+}
+#else
 ORACLE_API void set_default_parameters(fa_cp_parameters_t *p)  // utils.cu:494-537
 {
     memset(p, 0, sizeof(*p));
@@ -1125,6 +1349,7 @@ ORACLE_API void set_default_parameters(fa_cp_parameters_t *p)  // utils.cu:494-5
     p->restart_params.i_smooth = 0.3;
     p->optimality_norm = NORM_TYPE_L_INF;
 }
+#endif
 
 static bool to_csr(const matrix_desc_t *d, int rows_expected, CsrComponent *out, int *nnz_out)
 {   // fa_cp.c:60-270 with utils.cu:1135-1358 (zero_tolerance rules)
@@ -1298,7 +1523,11 @@ ORACLE_API void set_start_values(qp_problem_t *p, const double *primal, const do
     }
 }
 
+#ifdef ORACLE_LIN_CP_AL
+ORACLE_API void cp_al_result_free(cp_al_result_t *r)
+#else
 ORACLE_API void fa_cp_result_free(fa_cp_result_t *r)
+#endif
 {
     if (!r) return;
     free(r->primal_solution);
@@ -1307,6 +1536,50 @@ ORACLE_API void fa_cp_result_free(fa_cp_result_t *r)
     free(r);
 }
 
+static result_t *result_from_state(S &s, const qp_problem_t *prob, double polish_time, int polish_iters);
+
+#ifdef ORACLE_LIN_CP_AL
+ORACLE_API cp_al_result_t *optimize(const cp_al_parameters_t *params, const qp_problem_t *prob)
+{   // LIN_CP_AL solver.cu:34-114
+    if (!params || !prob) return nullptr;
+    S s;
+    s.P = *params;
+    init_state(s, prob);
+    if (s.nonconvexity < 0) s.inner_limit = 1;
+    lin_init_steps(s);
+    const auto t0 = std::chrono::steady_clock::now();
+    const int K = s.P.termination_evaluation_frequency;
+    while (s.total < s.P.termination_criteria.iteration_limit) {
+        const bool eval = s.total == 0 || s.total % K == 0;
+        s.major = eval;
+        if (eval || s.total % print_frequency(s.total) == 0) {
+            compute_residual(s, s.P.optimality_norm);
+            if (eval && s.total < 3 * K) infeasibility_info(s);
+            s.solve_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            check_termination(s);
+            if (s.reason != TERMINATION_REASON_UNSPECIFIED) break;
+        }
+        lin_update(s);
+        lin_fixed_point_error(s);
+        if (!std::isfinite(s.fp0)) s.fp0 = s.fp;
+        if (eval && lin_should_restart(s)) {
+            lin_restart(s);
+            s.fp0 = s.fp;
+        }
+        lin_halpern(s);
+        s.inner++;
+        s.total++;
+    }
+    if (s.reason == TERMINATION_REASON_UNSPECIFIED) {
+        s.reason = TERMINATION_REASON_ITERATION_LIMIT;
+        compute_residual(s, s.P.optimality_norm);
+    }
+    if (s.P.feasibility_polishing && s.reason != TERMINATION_REASON_DUAL_INFEASIBLE &&
+        s.reason != TERMINATION_REASON_PRIMAL_INFEASIBLE)
+        fprintf(stderr, "[lin_cp_al oracle] feasibility polishing is not restated; skipped.\n");
+    return result_from_state(s, prob, 0.0, 0);
+}
+#else
 ORACLE_API fa_cp_result_t *optimize(const fa_cp_parameters_t *params, const qp_problem_t *prob)
 {   // solver.cu:74-168
     if (!params || !prob) return nullptr;
@@ -1360,8 +1633,13 @@ ORACLE_API fa_cp_result_t *optimize(const fa_cp_parameters_t *params, const qp_p
     if (s.P.feasibility_polishing && s.reason != TERMINATION_REASON_DUAL_INFEASIBLE &&
         s.reason != TERMINATION_REASON_PRIMAL_INFEASIBLE)  // solver.cu:156-160
         feasibility_polish(s, polish_time, polish_iters);
-    // create_result_from_state, :1171-1239
-    fa_cp_result_t *r = (fa_cp_result_t *)calloc(1, sizeof(fa_cp_result_t));
+    return result_from_state(s, prob, polish_time, polish_iters);
+}
+#endif
+
+static result_t *result_from_state(S &s, const qp_problem_t *prob, double polish_time, int polish_iters)
+{   // create_result_from_state, fa_cp_core_op.cu:1171-1239 (LIN_CP_AL cp_al_core_op.cu:772-837)
+    result_t *r = (result_t *)calloc(1, sizeof(result_t));
     spmv(s.At, s.yT.data(), s.dual_prod.data());
     r->primal_solution = (double *)malloc((size_t)std::max(s.n, 1) * sizeof(double));
     r->dual_solution = (double *)malloc((size_t)std::max(s.m, 1) * sizeof(double));
@@ -1390,22 +1668,24 @@ ORACLE_API fa_cp_result_t *optimize(const fa_cp_parameters_t *params, const qp_p
     r->max_dual_ray_infeasibility = s.max_d_ray;
     r->primal_ray_linear_objective = s.p_ray_obj;
     r->dual_ray_objective = s.d_ray_obj;
+#ifndef ORACLE_LIN_CP_AL
     r->final_al_sigma = s.al_sigma;
     r->sigma_update_count = s.sigma_update_count;
     r->sigma_update_rule = s.al_available ? s.P.sigma_update_rule : FA_CP_SIGMA_UPDATE_FIXED;
+#endif
     r->termination_reason = s.reason;
     r->feasibility_polishing_time = polish_time;
     r->feasibility_iteration = polish_iters;
     return r;
 }
 
-ORACLE_API fa_cp_result_t *solve_qp_problem(const qp_problem_t *prob, const fa_cp_parameters_t *params)
+ORACLE_API result_t *solve_qp_problem(const qp_problem_t *prob, const params_t *params)
 {   // fa_cp.c:352-376
     if (!prob) {
         fprintf(stderr, "[interface] solve_qp_problem: invalid arguments.\n");
         return nullptr;
     }
-    fa_cp_parameters_t local;
+    params_t local;
     if (params) local = *params;
     else set_default_parameters(&local);
     return optimize(&local, prob);

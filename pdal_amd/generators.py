@@ -163,10 +163,12 @@ class PlantedQP:
         x = self.x_star
         return float(0.5 * (np.sum((self.M @ x) ** 2) + self.reg * x @ x) + self.c @ x)
 
-    def qp_args(self, form: str = "explicit") -> dict:
+    def qp_args(self, form: str = "explicit", equality: bool = False) -> dict:
         """Keyword arguments of FaCp.solve: "explicit" passes Q = M^T M + reg I;
-        "lowrank" passes Q = reg I and R = M."""
-        base = dict(c=self.c, A=self.A, con_lb=self.l, con_ub=self.u, var_lb=self.lb, var_ub=self.ub)
+        "lowrank" passes Q = reg I and R = M.  equality=True gives the equality-
+        constrained problem A x = A x* (the same planted KKT point: y is free)."""
+        l, u = (self.A @ self.x_star,) * 2 if equality else (self.l, self.u)
+        base = dict(c=self.c, A=self.A, con_lb=l, con_ub=u, var_lb=self.lb, var_ub=self.ub)
         if form == "explicit":
             return dict(base, Q=self.q_explicit())
         return dict(base, Q=self.q_diag_part(), R=self.M)

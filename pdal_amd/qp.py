@@ -137,6 +137,33 @@ class FaCpResult(C.Structure):
     ]
 
 
+class LinCpAlParameters(C.Structure):
+    """cp_al_parameters_t of the linearized CP-AL QP solver (lin_cp_al_types.h)."""
+    _fields_ = [
+        ("l_inf_ruiz_iterations", C.c_int),
+        ("has_pock_chambolle_alpha", C.c_bool),
+        ("pock_chambolle_alpha", C.c_double),
+        ("bound_objective_rescaling", C.c_bool),
+        ("use_al_qp", C.c_bool),
+        ("sigma_init", C.c_double),
+        ("verbose", C.c_int),
+        ("termination_evaluation_frequency", C.c_int),
+        ("sv_max_iter", C.c_int),
+        ("sv_tol", C.c_double),
+        ("termination_criteria", TerminationCriteria),
+        ("restart_params", RestartParameters),
+        ("reflection_coefficient", C.c_double),
+        ("feasibility_polishing", C.c_bool),
+        ("optimality_norm", C.c_int),
+    ]
+
+
+class LinCpAlResult(C.Structure):
+    """cp_al_result_t of the linearized CP-AL QP solver: FaCpResult without the sigma fields."""
+    _fields_ = [f for f in FaCpResult._fields_
+                if f[0] not in ("final_al_sigma", "sigma_update_count", "sigma_update_rule")]
+
+
 class _Dense(C.Structure):
     _fields_ = [("A", _dp)]
 
@@ -248,6 +275,10 @@ class QpResult:
 
 class FaCp:
     """A library exporting the fa_cp.h entry points."""
+    Params = FaCpParameters
+    Result = FaCpResult
+    result_free = "fa_cp_result_free"
+    reset_names = ("pdal_fa_cp_reset_rng", "fa_cp_oracle_reset_rng")
 
     def __init__(self, path):
         path = Path(path)
@@ -261,24 +292,25 @@ class FaCp:
                                         _dp, _dp, _dp, _dp, _dp]
         L.set_start_values.restype = None
         L.set_start_values.argtypes = [C.POINTER(QpProblem), _dp, _dp]
-        L.solve_qp_problem.restype = C.POINTER(FaCpResult)
-        L.solve_qp_problem.argtypes = [C.POINTER(QpProblem), C.POINTER(FaCpParameters)]
+        L.solve_qp_problem.restype = C.POINTER(self.Result)
+        L.solve_qp_problem.argtypes = [C.POINTER(QpProblem), C.POINTER(self.Params)]
         L.set_default_parameters.restype = None
-        L.set_default_parameters.argtypes = [C.POINTER(FaCpParameters)]
-        L.fa_cp_result_free.restype = None
-        L.fa_cp_result_free.argtypes = [C.POINTER(FaCpResult)]
+        L.set_default_parameters.argtypes = [C.POINTER(self.Params)]
+        self._free = getattr(L, self.result_free)
+        self._free.restype = None
+        self._free.argtypes = [C.POINTER(self.Result)]
         L.qp_problem_free.restype = None
         L.qp_problem_free.argtypes = [C.POINTER(QpProblem)]
         self._reset = None
-        for name in ("pdal_fa_cp_reset_rng", "fa_cp_oracle_reset_rng"):
+        for name in self.reset_names:
             if hasattr(L, name):
                 self._reset = getattr(L, name)
                 self._reset.restype = None
                 self._reset.argtypes = []
                 break
 
-    def default_params(self, **overrides) -> FaCpParameters:
-        p = FaCpParameters()
+    def default_params(self, **overrides):
+        p = self.Params()
         self.lib.set_default_parameters(C.byref(p))
         p.verbose = 0
         for k, v in overrides.items():
@@ -323,15 +355,16 @@ class FaCp:
                 max_primal_ray_infeasibility=r.max_primal_ray_infeasibility,
                 max_dual_ray_infeasibility=r.max_dual_ray_infeasibility,
                 primal_ray_linear_objective=r.primal_ray_linear_objective, dual_ray_objective=r.dual_ray_objective,
-                final_al_sigma=r.final_al_sigma, sigma_update_count=r.sigma_update_count,
+                final_al_sigma=getattr(r, "final_al_sigma", 0.0),
+                sigma_update_count=getattr(r, "sigma_update_count", 0),
                 rescaling_time_sec=r.rescaling_time_sec, solve_time_sec=r.cumulative_time_sec,
                 feasibility_polishing_time=r.feasibility_polishing_time,
                 feasibility_iteration=r.feasibility_iteration)
         finally:
-            self.lib.fa_cp_result_free(res)
+            self._free(res)
 
     def solve(self, c, Q=None, R=None, A=None, con_lb=None, con_ub=None, var_lb=None, var_ub=None,
-              obj_const=None, params: Optional[FaCpParameters] = None, primal_start=None, dual_start=None,
+              obj_const=None, params=None, primal_start=None, dual_start=None,
               fmt: str = "csr", zero_tolerance: float = 0.0) -> QpResult:
         prob = self.make_problem(c, Q, R, A, con_lb, con_ub, var_lb, var_ub, obj_const, fmt, zero_tolerance)
         try:
@@ -347,7 +380,16 @@ class FaCp:
             self.free_problem(prob)
 
 
-def set_param(p: FaCpParameters, key: str, value) -> None:
+class LinCpAl(FaCp):
+    """A library exporting the linearized CP-AL QP entry points (lin_cp_al.h):
+    the same problem constructors as FA_CP, its own parameter and result types."""
+    Params = LinCpAlParameters
+    Result = LinCpAlResult
+    result_free = "cp_al_result_free"
+    reset_names = ("pdal_lin_cp_al_reset_rng", "lin_cp_al_oracle_reset_rng")
+
+
+def set_param(p, key: str, value) -> None:
     if key in ("iteration_limit", "eps_optimal_relative", "eps_feasible_relative", "time_sec_limit",
                "eps_feas_polish_relative", "eps_infeasible"):
         setattr(p.termination_criteria, key, value)
@@ -365,3 +407,8 @@ def set_param(p: FaCpParameters, key: str, value) -> None:
 def load_fa_cp() -> FaCp:
     """The MI355X libfa_cp.so; raises FileNotFoundError if it is not built."""
     return FaCp(LIB_DIR / "libfa_cp.so")
+
+
+def load_lin_cp_al() -> LinCpAl:
+    """The MI355X librhr_lin_cp_al.so; raises FileNotFoundError if it is not built."""
+    return LinCpAl(LIB_DIR / "librhr_lin_cp_al.so")
