@@ -42,6 +42,12 @@ void qp_problem_free(qp_problem_t *prob);
 /* reference: QP/LIN_CP_AL/src/solver.cu:34-114 */
 cp_al_result_t *optimize(const cp_al_parameters_t *params, const qp_problem_t *original_problem);
 
+/* Extras of this build, no counterpart in the reference (as pdal_qp.h for
+ * FA_CP): reseed the power-iteration generators (std::mt19937(1) and srand(1),
+ * process-global there), and the name of a termination reason. */
+void pdal_lin_cp_al_reset_rng(void);
+const char *pdal_lin_cp_al_termination_reason_to_string(termination_reason_t reason);
+
 #ifdef __cplusplus
 }
 #endif

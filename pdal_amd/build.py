@@ -106,6 +106,18 @@ def build_fa_cp() -> Path:
     return out
 
 
+def build_lin_cp_al() -> Path:
+    """The linearized CP-AL QP solver: the FA_CP sources built with -DPDAL_LIN_CP_AL."""
+    objdir = ROOT / "pdal_amd" / "_build" / "lin_cp_al"
+    objs = _compile_objects(FA_CP_SOURCES, objdir, extra=["-DPDAL_LIN_CP_AL"])
+    LIB.mkdir(parents=True, exist_ok=True)
+    out = LIB / "librhr_lin_cp_al.so"
+    tmp = LIB / "librhr_lin_cp_al.so.tmp"
+    _run([_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-Bsymbolic", *objs, "-lz", "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
 def build_ws() -> Path:
     objdir = ROOT / "pdal_amd" / "_build" / "ws"
     objs = _compile_objects([CSRC / "ws" / "warmstart.hip"], objdir)
@@ -120,6 +132,7 @@ def build_ws() -> Path:
 def build_all() -> None:
     build_cp_al()
     build_fa_cp()
+    build_lin_cp_al()
     build_ws()
 
 

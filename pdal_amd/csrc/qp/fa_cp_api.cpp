@@ -1,11 +1,15 @@
-// Public C entry points of libfa_cp.so.
+// Public C entry points of libfa_cp.so, and with -DPDAL_LIN_CP_AL of
+// librhr_lin_cp_al.so (the same problem construction, QP/LIN_CP_AL/src/cp_al.c,
+// its own defaults and result type).
 //
 // Reference: QP/FA_CP/src/fa_cp.c:28-376 (problem construction, start values,
 // solve, frees), QP/FA_CP/src/utils.cu:494-537 (defaults), :661-678 (final
-// summary), :1135-1358 (dense / CSC / COO -> CSR with zero_tolerance).
-#include "fa_cp.h"
+// summary), :1135-1358 (dense / CSC / COO -> CSR with zero_tolerance);
+// QP/LIN_CP_AL/src/utils.cu:466-497 (its defaults).
 #include "fa_cp_internal.hpp"
+#ifndef PDAL_LIN_CP_AL
 #include "pdal_qp.h"
+#endif
 
 #include <algorithm>
 #include <cmath>
@@ -140,7 +144,10 @@ void free_component(CsrComponent *c)
     free(c);
 }
 
-void final_log(const fa_cp_result_t *r, const fa_cp_parameters_t *p)
+using pdal::qp::params_t;
+using pdal::qp::result_t;
+
+void final_log(const result_t *r, const params_t *p)
 {   // fa_cp_final_log, utils.cu:661-678
     const char *quiet = getenv("PDAL_QUIET");
     if (quiet && quiet[0] == '1') return;
@@ -160,6 +167,38 @@ void final_log(const fa_cp_result_t *r, const fa_cp_parameters_t *p)
 
 }  // namespace
 
+#ifdef PDAL_LIN_CP_AL
+PDAL_API void set_default_parameters(cp_al_parameters_t *p)
+{   // QP/LIN_CP_AL/src/utils.cu:466-497
+    memset(p, 0, sizeof(*p));
+    p->l_inf_ruiz_iterations = 10;
+    p->has_pock_chambolle_alpha = true;
+    p->pock_chambolle_alpha = 1.0;
+    p->bound_objective_rescaling = true;
+    p->use_al_qp = true;
+    p->sigma_init = 0.0;
+    p->verbose = 1;
+    p->termination_evaluation_frequency = 200;
+    p->feasibility_polishing = false;
+    p->reflection_coefficient = 0.0;
+    p->sv_max_iter = 5000;
+    p->sv_tol = 1e-4;
+    p->termination_criteria.eps_optimal_relative = 1e-4;
+    p->termination_criteria.eps_feasible_relative = 1e-4;
+    p->termination_criteria.eps_infeasible = 1e-10;
+    p->termination_criteria.time_sec_limit = 3600.0;
+    p->termination_criteria.iteration_limit = INT32_MAX;
+    p->termination_criteria.eps_feas_polish_relative = 1e-6;
+    p->restart_params.artificial_restart_threshold = 0.36;
+    p->restart_params.sufficient_reduction_for_restart = 0.2;
+    p->restart_params.necessary_reduction_for_restart = 0.8;
+    p->restart_params.k_p = 0.99;
+    p->restart_params.k_i = 0.01;
+    p->restart_params.k_d = 0.0;
+    p->restart_params.i_smooth = 0.3;
+    p->optimality_norm = NORM_TYPE_L_INF;
+}
+#else
 PDAL_API void set_default_parameters(fa_cp_parameters_t *p)
 {   // utils.cu:494-537
     memset(p, 0, sizeof(*p));
@@ -202,6 +241,7 @@ PDAL_API void set_default_parameters(fa_cp_parameters_t *p)
     p->restart_params.i_smooth = 0.3;
     p->optimality_norm = NORM_TYPE_L_INF;
 }
+#endif
 
 PDAL_API void qp_problem_free(qp_problem_t *p)
 {   // fa_cp.c:300-318
@@ -309,7 +349,11 @@ PDAL_API void set_start_values(qp_problem_t *p, const double *primal, const doub
     }
 }
 
+#ifdef PDAL_LIN_CP_AL
+PDAL_API void cp_al_result_free(cp_al_result_t *r)
+#else
 PDAL_API void fa_cp_result_free(fa_cp_result_t *r)
+#endif
 {   // fa_cp.c:270-277; reduced_cost is freed too (the reference leaks it)
     if (!r) return;
     free(r->primal_solution);
@@ -318,7 +362,7 @@ PDAL_API void fa_cp_result_free(fa_cp_result_t *r)
     free(r);
 }
 
-PDAL_API fa_cp_result_t *optimize(const fa_cp_parameters_t *params, const qp_problem_t *prob)
+PDAL_API result_t *optimize(const params_t *params, const qp_problem_t *prob)
 {   // solver.cu:74-168.  Errors return NULL instead of exiting the process.
     if (!params || !prob) return nullptr;
     if (params->termination_evaluation_frequency <= 0) {
@@ -326,7 +370,7 @@ PDAL_API fa_cp_result_t *optimize(const fa_cp_parameters_t *params, const qp_pro
         return nullptr;
     }
     try {
-        fa_cp_result_t *r = pdal::qp::optimize_gpu(params, prob);
+        result_t *r = pdal::qp::optimize_gpu(params, prob);
         if (r) final_log(r, params);
         return r;
     } catch (const std::exception &e) {
@@ -335,23 +379,32 @@ PDAL_API fa_cp_result_t *optimize(const fa_cp_parameters_t *params, const qp_pro
     }
 }
 
-PDAL_API fa_cp_result_t *solve_qp_problem(const qp_problem_t *prob, const fa_cp_parameters_t *params)
+PDAL_API result_t *solve_qp_problem(const qp_problem_t *prob, const params_t *params)
 {   // fa_cp.c:352-376
     if (!prob) {
         fprintf(stderr, "[interface] solve_qp_problem: invalid arguments.\n");
         return nullptr;
     }
-    fa_cp_parameters_t local;
+    params_t local;
     if (params) local = *params;
     else set_default_parameters(&local);
-    fa_cp_result_t *res = optimize(&local, prob);
+    result_t *res = optimize(&local, prob);
     if (!res) fprintf(stderr, "[interface] optimize returned NULL.\n");
     return res;
 }
 
+#ifdef PDAL_LIN_CP_AL
+PDAL_API void pdal_lin_cp_al_reset_rng(void) { pdal::qp::reset_rng(); }
+
+PDAL_API const char *pdal_lin_cp_al_termination_reason_to_string(termination_reason_t r)
+{
+    return pdal::qp::termination_reason_name(r);
+}
+#else
 PDAL_API void pdal_fa_cp_reset_rng(void) { pdal::qp::reset_rng(); }
 
 PDAL_API const char *pdal_fa_cp_termination_reason_to_string(termination_reason_t r)
 {
     return pdal::qp::termination_reason_name(r);
 }
+#endif

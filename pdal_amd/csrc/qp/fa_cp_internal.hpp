@@ -2,13 +2,13 @@
 // and the HIP solver (fa_cp_solver.hip).
 #pragma once
 
-#include "fa_cp.h"
+#include "qp_variant.hpp"
 
 namespace pdal {
 namespace qp {
 
 // Solve on the current HIP device; throws pdal::HipError / std::exception on failure.
-fa_cp_result_t *optimize_gpu(const fa_cp_parameters_t *params, const qp_problem_t *prob);
+result_t *optimize_gpu(const params_t *params, const qp_problem_t *prob);
 
 // Re-seed the power-iteration generators (std::mt19937(1) + normal cache, srand(1)).
 void reset_rng();

@@ -17,12 +17,20 @@
 // chunks and reads one flag per chunk.  Checkpoints (residual, infeasibility,
 // restart, fixed-point error) are SpMVs with fused deterministic reductions;
 // the host reads a few doubles and runs the scalar logic.
-#include "fa_cp.h"
+//
+// Built with -DPDAL_LIN_CP_AL the same driver is the linearized CP-AL solver
+// RHR-Lin-CP-AL (QP/LIN_CP_AL): the shared upload, scaling, power iterations,
+// residual, infeasibility and termination; its own step sizes
+// (cp_al_core_op.cu:609-691), update (:496-575, one projected gradient step),
+// relaxed Halpern step (:577-585), per-iteration fixed-point error (:693-770),
+// restart test and restart (utils.cu:451-463, :595-607), main loop
+// (solver.cu:34-114).  Its feasibility polishing is not built.
 #include "fa_cp_internal.hpp"
 
 #include "../common/csr_host.hpp"
 #include "../common/hip_util.hpp"
 #include "fa_cp_kernels.hpp"
+#include "lin_cp_al_kernels.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -95,13 +103,13 @@ enum Slot {
 
 class Solver {
   public:
-    Solver(const fa_cp_parameters_t &p, const qp_problem_t *w) : P_(p), W_(w) {}
+    Solver(const params_t &p, const qp_problem_t *w) : P_(p), W_(w) {}
     ~Solver()
     {
         if (bb_graph_) (void)hipGraphExecDestroy(bb_graph_);
         if (stream_) (void)hipStreamDestroy(stream_);
     }
-    fa_cp_result_t *run();
+    result_t *run();
 
   private:
     // ---- set-up ----
@@ -158,7 +166,17 @@ class Solver {
     double clipped_update(double cur, double ratio) const;
     void update_al_sigma(double pd, double dd, double normal);
     void perform_restart();
-    fa_cp_result_t *make_result();
+    // linearized CP-AL (PDAL_LIN_CP_AL builds)
+    double sigma_ = 0.0, sigma_base_ = 1.0, sigma_min_ = 0.0, sigma_max_ = 0.0, lambda_est_ = 1.0;
+    double lin_tau() const { return 1.0 / std::fmax(sigma_ * lambda_est_, 1e-16); }
+    void lin_init_steps();
+    void lin_update();
+    void lin_fixed_point_error();
+    bool lin_should_restart() const;
+    void lin_restart();
+    void lin_halpern();
+    result_t *run_linearized();
+    result_t *make_result();
     void print_initial_info() const;
     void display_stats() const;
     // feasibility polishing (fa_cp_core_op.cu:1242-1735)
@@ -184,7 +202,7 @@ class Solver {
     double polish_time_ = 0.0;
     int polish_iters_ = 0;
 
-    fa_cp_parameters_t P_;
+    params_t P_;
     const qp_problem_t *W_;
     hipStream_t stream_ = nullptr;
 
@@ -450,8 +468,10 @@ void Solver::init_state()
     const qp_problem_t *w = W_;
     al_available_ = P_.use_al_qp;
     use_al_ = P_.use_al_qp;
+#ifndef PDAL_LIN_CP_AL
     al_sigma_ = P_.al_sigma;
     best_al_sigma_ = P_.al_sigma;
+#endif
     if (w->primal_start || w->dual_start) {  // solver_state.cu:630-657
         if (w->primal_start) x_.upload(w->primal_start, n_, stream_);
         if (w->dual_start) y_.upload(w->dual_start, m_, stream_);
@@ -517,6 +537,16 @@ void Solver::init_state()
         q_norm_ = q_max_eig();
         nonconvexity_ = q_min_eig(q_norm_);
     }
+#ifdef PDAL_LIN_CP_AL
+    // LIN_CP_AL solver_state.cu:305-340: an inner loop only for the non-diagonal Q types
+    // (its linearized update never runs it)
+    const bool simple = qtype_ == FA_CP_NON_Q || qtype_ == FA_CP_DIAG_Q;
+    has_inner_ = !simple;
+    if (!simple) {
+        inner_limit_ = 1000;
+        inner_tol_ = 1e-3;
+    }
+#else
     if (al_available_ && P_.al_sigma <= P_.sigma_zero_threshold) {
         use_al_ = false;
         al_sigma_ = 0.0;
@@ -529,6 +559,7 @@ void Solver::init_state()
         inner_limit_ = 1000;
         inner_tol_ = 1e-3;
     }
+#endif
     is_lp_ = qtype_ == FA_CP_NON_Q;
 }
 
@@ -587,7 +618,11 @@ StepScalars Solver::scalars() const
     sc.al_sigma = use_al_ ? al_sigma_ : 0.0;
     sc.weight = (double)(inner_ + 1) / (inner_ + 2);
     sc.rho = P_.reflection_coefficient;
+#ifdef PDAL_LIN_CP_AL
+    sc.tau_res = use_al_ ? lin_tau() : step_ / pw_;  // LIN_CP_AL utils.cu:855-858
+#else
     sc.tau_res = step_ / pw_;
+#endif
     return sc;
 }
 
@@ -907,6 +942,7 @@ bool Solver::should_restart()
     return r;
 }
 
+#ifndef PDAL_LIN_CP_AL
 void Solver::set_sigma_active(double sigma, double zero_thr)
 {   // fa_cp_core_op.cu:667-676
     if (!std::isfinite(sigma) || sigma <= zero_thr) {
@@ -1069,7 +1105,9 @@ void Solver::perform_restart()
     last_trial_fp_ = INFINITY;
 }
 
-fa_cp_result_t *Solver::make_result()
+#endif  // !PDAL_LIN_CP_AL
+
+result_t *Solver::make_result()
 {   // create_result_from_state, fa_cp_core_op.cu:1171-1239
     spmv(At_, yT_.get(), dual_prod_.get());
     // reuse iterate buffers that are no longer needed as outputs
@@ -1079,7 +1117,7 @@ fa_cp_result_t *Solver::make_result()
                                                                      E_.get(), yo, m_);
     PDAL_LAUNCHED();
     PDAL_HIP(hipMemcpyAsync(hbb_.get(), dbb_.get(), sizeof(BBState), hipMemcpyDeviceToHost, stream_));
-    fa_cp_result_t *r = (fa_cp_result_t *)calloc(1, sizeof(fa_cp_result_t));
+    result_t *r = (result_t *)calloc(1, sizeof(result_t));
     r->primal_solution = (double *)malloc((size_t)std::max(n_, 1) * sizeof(double));
     r->dual_solution = (double *)malloc((size_t)std::max(m_, 1) * sizeof(double));
     r->reduced_cost = (double *)malloc((size_t)std::max(n_, 1) * sizeof(double));
@@ -1106,9 +1144,11 @@ fa_cp_result_t *Solver::make_result()
     r->max_dual_ray_infeasibility = max_d_ray_;
     r->primal_ray_linear_objective = p_ray_obj_;
     r->dual_ray_objective = d_ray_obj_;
+#ifndef PDAL_LIN_CP_AL
     r->final_al_sigma = al_sigma_;
     r->sigma_update_count = sigma_update_count_;
     r->sigma_update_rule = al_available_ ? P_.sigma_update_rule : FA_CP_SIGMA_UPDATE_FIXED;
+#endif
     r->termination_reason = reason_;
     r->feasibility_polishing_time = polish_time_;
     r->feasibility_iteration = polish_iters_;
@@ -1119,7 +1159,7 @@ void Solver::print_initial_info() const
 {   // print_initial_info, utils.cu:560-659 (the settings that differ from the defaults are omitted)
     if (P_.verbose < 2) return;
     printf("---------------------------------------------------------------------------------------\n");
-    printf("                          FA_CP (MI355X / gfx950 HIP build)\n");
+    printf("                          %s (MI355X / gfx950 HIP build)\n", kLinearized ? "RHR-Lin-CP-AL" : "FA_CP");
     printf("---------------------------------------------------------------------------------------\n");
     printf("problem: %d rows, %d columns, %d nonzeros\n", W_->num_constraints, W_->num_variables,
            W_->constraint_matrix_num_nonzeros);
@@ -1139,8 +1179,196 @@ void Solver::display_stats() const
                abs_p_, abs_d_, gap_, rel_p_, rel_d_, rel_gap_);
 }
 
-fa_cp_result_t *Solver::run()
-{   // optimize, QP/FA_CP/src/solver.cu:74-168
+// ---------------------------------------------------------------------------
+// Linearized CP-AL (PDAL_LIN_CP_AL builds; QP/LIN_CP_AL/src/cp_al_core_op.cu)
+
+void Solver::lin_init_steps()
+{   // initialize_step_size_and_primal_weight, :609-691
+    const double max_sv = A_.nnz > 0 ? max_singular_value() : 0.0;
+    const double a_sq = max_sv * max_sv;
+    const double half_lf = 0.5 * std::fmax(q_norm_, 0.0);
+    pw_ = P_.bound_objective_rescaling ? 1.0 : (c_norm_ + 1.0) / (b_norm_ + 1.0);
+    best_pw_ = pw_;
+    if (use_al_) {
+        if (a_sq <= 1e-16 && half_lf <= 1e-16) {
+            lambda_est_ = 1.0;
+            sigma_base_ = 1.0;
+        } else {  // t = sqrt(lambda) solves t^2 - (L_f / 2) t - 2 ||A||^2 = 0 (with 8.0004 for 8)
+            const double t = std::fmax(0.5 * (half_lf + std::sqrt(half_lf * half_lf + 8.0004 * a_sq)), 1e-16);
+            lambda_est_ = t * t;
+            sigma_base_ = 1.0 / t;
+        }
+        sigma_min_ = sigma_base_;
+        sigma_max_ = 100.0 * sigma_base_;
+#ifdef PDAL_LIN_CP_AL
+        const double init = P_.sigma_init;
+#else
+        const double init = 0.0;
+#endif
+        sigma_ = init > 0.0 ? init : 6.0 * sigma_base_;
+        sigma_ = std::fmin(std::fmax(sigma_, sigma_min_), sigma_max_);
+        step_ = lin_tau();
+    } else {
+        const double wgt = std::fmax(pw_, 1e-16);
+        if (a_sq <= 1e-16 && half_lf <= 1e-16) {
+            step_ = 1.0;
+        } else if (a_sq <= 1e-16) {
+            step_ = 0.99 * wgt / std::fmax(half_lf, 1e-16);
+        } else {
+            const double lc = half_lf / wgt;
+            step_ = 0.99 * (-lc + std::sqrt(lc * lc + 4.0 * a_sq)) / (2.0 * a_sq);
+        }
+    }
+}
+
+void Solver::lin_update()
+{   // cp_al_update, :496-575: the T-outputs and reflections every iteration
+    double tau, dstep, al_sigma = 0.0;
+    if (use_al_) {
+        tau = lin_tau();
+        dstep = std::fmax(sigma_, 1e-16);
+        al_sigma = sigma_;
+        step_ = tau;
+    } else {
+        tau = step_ / pw_;
+        dstep = step_ * pw_;
+    }
+    if (nonconvexity_ < 0) {
+        tau = std::fmax(tau, -1.01 * std::fmin(0.0, nonconvexity_));
+        tau /= 2;
+    }
+    if (use_al_) {  // compute_al_effective_dual_product, :275-302
+        spmv(A_, x_.get(), prim_prod_.get());
+        lin_al_dual_kernel<<<blocks_for(m_), kBlock, 0, stream_>>>(prim_prod_.get(), y_.get(), l_.get(), al_sigma,
+                                                                   al_u_.get(), m_);
+        PDAL_LAUNCHED();
+        spmv(At_, al_u_.get(), dual_prod_.get());
+    } else {
+        spmv(At_, y_.get(), dual_prod_.get());
+    }
+    if (qtype_ == FA_CP_NON_Q) {  // lp_primal_update, :304-312
+        lin_lp_primal_kernel<<<blocks_for(n_), kBlock, 0, stream_>>>(x_.get(), dual_prod_.get(), c_.get(), lb_.get(),
+                                                                     ub_.get(), tau, xT_.get(), xR_.get(),
+                                                                     dslack_.get(), n_);
+    } else {  // primal_gradient_update, :484-494
+        obj_product(x_.get());
+        lin_grad_primal_kernel<<<blocks_for(n_), kBlock, 0, stream_>>>(x_.get(), qx_.get(), c_.get(),
+                                                                       dual_prod_.get(), lb_.get(), ub_.get(), tau,
+                                                                       xT_.get(), xR_.get(), n_);
+    }
+    PDAL_LAUNCHED();
+    spmv(A_, xR_.get(), prim_prod_.get());
+    lin_dual_kernel<<<blocks_for(m_), kBlock, 0, stream_>>>(y_.get(), prim_prod_.get(), l_.get(), dstep, yT_.get(),
+                                                            yR_.get(), m_);
+    PDAL_LAUNCHED();
+}
+
+void Solver::lin_fixed_point_error()
+{   // compute_fixed_point_error, :693-770 (delta = T-output - current)
+    const int nbn = stride_grid(n_), nbm = stride_grid(m_);
+    diff_norm_kernel<<<nbm, kBlock, 0, stream_>>>(yT_.get(), y_.get(), dy_.get(), m_, parts_.get());
+    reduce(nbm, 1, 1, S_DN2);
+    diff_norm_kernel<<<nbn, kBlock, 0, stream_>>>(xT_.get(), x_.get(), dx_.get(), n_, parts_.get());
+    reduce(nbn, 1, 1, S_PD2);
+    PDAL_LAUNCHED();
+    if (use_al_) {  // <A dx, dy> and ||A dx||^2
+        spmv(A_, dx_.get(), prim_prod_.get());
+        dot_sumsq_kernel<<<nbm, kBlock, 0, stream_>>>(dy_.get(), prim_prod_.get(), m_, parts_.get());
+        reduce(nbm, 2, 2, S_FP);
+    } else {  // <At dy, dx>
+        spmv(At_, dy_.get(), dual_prod_.get());
+        dot_sumsq_kernel<<<nbn, kBlock, 0, stream_>>>(dx_.get(), dual_prod_.get(), n_, parts_.get());
+        reduce(nbn, 2, 2, S_FP);
+    }
+    PDAL_LAUNCHED();
+    fetch(S_FP, S_PD2 - S_FP + 1);
+    const double pn2 = hs(S_PD2), dn2 = hs(S_DN2), cross = hs(S_FP);
+    double fp2;
+    if (use_al_) {
+        const double an2 = hs(S_FP + 1);
+        fp2 = sigma_ * (lambda_est_ * pn2 - an2) + 2.0 * cross + dn2 / sigma_;
+    } else {
+        const double tau = step_ / pw_, rho = step_ * pw_;
+        fp2 = pn2 / std::fmax(tau, 1e-16) + dn2 / std::fmax(rho, 1e-16) + 2.0 * cross;
+    }
+    fp_ = std::sqrt(std::fmax(0.0, fp2));
+}
+
+bool Solver::lin_should_restart() const
+{   // should_do_adaptive_restart, LIN_CP_AL utils.cu:451-463
+    if (!std::isfinite(fp0_) || !std::isfinite(fp_)) return false;
+    if (total_ == inner_) return inner_ >= P_.termination_evaluation_frequency;
+    return fp_ <= P_.restart_params.sufficient_reduction_for_restart * fp0_;
+}
+
+void Solver::lin_restart()
+{   // perform_restart, :595-607: the anchors move to the current iterates
+    copy(x0_.get(), x_.get(), n_);
+    copy(y0_.get(), y_.get(), m_);
+    inner_ = 0;
+    last_trial_fp_ = INFINITY;
+}
+
+void Solver::lin_halpern()
+{   // halpern_update, :577-585
+    const double w = (double)(inner_ + 1) / (inner_ + 2);
+    lin_halpern_kernel<<<blocks_for((long)n_ + m_), kBlock, 0, stream_>>>(x_.get(), xT_.get(), x0_.get(), n_,
+                                                                          y_.get(), yT_.get(), y0_.get(), m_, w,
+                                                                          P_.reflection_coefficient);
+    PDAL_LAUNCHED();
+}
+
+result_t *Solver::run_linearized()
+{   // optimize, QP/LIN_CP_AL/src/solver.cu:34-114
+    print_initial_info();
+    const auto t_setup = std::chrono::steady_clock::now();
+    upload();
+    rescale();
+    rescale_time_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_setup).count();
+    init_state();
+    if (nonconvexity_ < 0) inner_limit_ = 1;
+    lin_init_steps();
+    PDAL_HIP(hipStreamSynchronize(stream_));
+    const auto t0 = std::chrono::steady_clock::now();
+    const int K = P_.termination_evaluation_frequency;
+    while (total_ < P_.termination_criteria.iteration_limit) {
+        const bool eval = total_ == 0 || total_ % K == 0;
+        major_ = eval;
+        if (eval || total_ % print_frequency(total_) == 0) {
+            compute_residual();
+            if (eval && total_ < 3 * K) infeasibility_info();
+            solve_time_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            check_termination();
+            display_stats();
+            if (reason_ != TERMINATION_REASON_UNSPECIFIED) break;
+        }
+        lin_update();
+        lin_fixed_point_error();
+        if (!std::isfinite(fp0_)) fp0_ = fp_;
+        if (eval && lin_should_restart()) {
+            lin_restart();
+            fp0_ = fp_;
+        }
+        lin_halpern();
+        inner_++;
+        total_++;
+    }
+    if (reason_ == TERMINATION_REASON_UNSPECIFIED) {
+        reason_ = TERMINATION_REASON_ITERATION_LIMIT;
+        compute_residual();
+        display_stats();
+    }
+    if (P_.feasibility_polishing && reason_ != TERMINATION_REASON_DUAL_INFEASIBLE &&
+        reason_ != TERMINATION_REASON_PRIMAL_INFEASIBLE)
+        fprintf(stderr, "[rhr_lin_cp_al] feasibility polishing is not built for the linearized solver; skipped.\n");
+    return make_result();
+}
+
+result_t *Solver::run()
+{
+#ifdef PDAL_LIN_CP_AL
+    return run_linearized();
+#else   // optimize, QP/FA_CP/src/solver.cu:74-168
     print_initial_info();
     const auto t_setup = std::chrono::steady_clock::now();
     upload();
@@ -1196,8 +1424,10 @@ fa_cp_result_t *Solver::run()
         reason_ != TERMINATION_REASON_PRIMAL_INFEASIBLE)  // solver.cu:156-160
         feasibility_polish();
     return make_result();
+#endif
 }
 
+#ifndef PDAL_LIN_CP_AL
 // ---------------------------------------------------------------------------
 // Feasibility polishing (fa_cp_core_op.cu:1242-1735, utils.cu:1360-1575): a
 // primal phase with c = 0 (Q kept) and a dual phase with every finite bound 0,
@@ -1444,10 +1674,11 @@ void Solver::feasibility_polish()
     fflush(stdout);
     polish_time_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
 }
+#endif  // !PDAL_LIN_CP_AL
 
 }  // namespace
 
-fa_cp_result_t *optimize_gpu(const fa_cp_parameters_t *params, const qp_problem_t *prob)
+result_t *optimize_gpu(const params_t *params, const qp_problem_t *prob)
 {
     Solver s(*params, prob);
     return s.run();

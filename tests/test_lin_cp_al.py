@@ -17,11 +17,20 @@ and objective on full solves.
 """
 from __future__ import annotations
 
+import ctypes as C
+import re
+import subprocess
+from pathlib import Path
+
 import numpy as np
 import pytest
 
+from pdal_amd import qp as Q
 from pdal_amd.generators import planted_lp, planted_qp
 from pdal_amd.qp import LinCpAl
+
+ROOT = Path(__file__).resolve().parent.parent
+INCLUDE = ROOT / "include"
 
 ROOT_ORACLE = "oracle/_build/liblin_cp_al_oracle.so"
 TOL = dict(eps_optimal_relative=1e-6, eps_feasible_relative=1e-6, iteration_limit=300000)
@@ -96,3 +105,105 @@ def test_oracle_defaults_and_sigma_init(oracle_lin):
     big2 = solve(oracle_lin, args, iteration_limit=300, sigma_init=1e9)
     assert np.array_equal(big1.x, big2.x)
     assert not np.allclose(base.x, big1.x)
+
+
+# ---------------------------------------------------------------------------
+# Boundary of librhr_lin_cp_al.so (no GPU call)
+
+@pytest.fixture(scope="module")
+def gpu_lin():
+    """The MI355X build; the test fails (no fallback) if it is missing."""
+    return Q.load_lin_cp_al()
+
+
+def _declared(header: Path) -> list:
+    text = re.sub(r"/\*.*?\*/", "", header.read_text(), flags=re.S)
+    return [n for n in re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\([^;{]*\)\s*;", text)
+            if n not in ("if", "while", "return", "sizeof")]
+
+
+def _exported(lib: Path) -> set:
+    out = subprocess.run(["nm", "-D", "--defined-only", str(lib)], check=True, capture_output=True, text=True)
+    return {ln.split()[-1] for ln in out.stdout.splitlines() if " T " in ln}
+
+
+def test_library_exports_every_declared_symbol(gpu_lin):
+    declared = _declared(INCLUDE / "lin_cp_al.h")
+    assert "solve_qp_problem" in declared and "cp_al_result_free" in declared
+    exported = _exported(gpu_lin.path)
+    assert not set(declared) - exported
+    assert not [n for n in exported if "pdal" in n and not n.startswith("pdal_lin_cp_al_")]
+    assert "fa_cp_result_free" not in exported
+
+
+def test_struct_layout_matches_header(tmp_path):
+    """ctypes mirror == C layout of include/lin_cp_al_types.h (the field order
+    and types of QP/LIN_CP_AL/include/cp_al_types.h)."""
+    structs = {"cp_al_parameters_t": Q.LinCpAlParameters, "cp_al_result_t": Q.LinCpAlResult,
+               "qp_problem_t": Q.QpProblem, "matrix_desc_t": Q.MatrixDesc}
+    lines = []
+    for cname, cls in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'printf("{cname}.{fname} %zu\\n", offsetof({cname}, {fname}));')
+    src = tmp_path / "layout.c"
+    src.write_text("#include <stddef.h>\n#include <stdio.h>\n#include \"lin_cp_al_types.h\"\nint main(void){\n"
+                   + "\n".join(lines) + "\nreturn 0;}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", f"-I{INCLUDE}", str(src), "-o", str(exe)], check=True)
+    got = dict(ln.rsplit(" ", 1) for ln in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                          text=True).stdout.splitlines())
+    for cname, cls in structs.items():
+        assert int(got[f"{cname} size"]) == C.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert int(got[f"{cname}.{fname}"]) == getattr(cls, fname).offset, (cname, fname)
+
+
+def test_default_parameters_match_oracle(gpu_lin, oracle_lin):
+    a, b = gpu_lin.default_params(), oracle_lin.default_params()
+    assert bytes(a) == bytes(b)
+
+
+# ---------------------------------------------------------------------------
+# GPU path against the oracle
+
+ITER_RTOL = 1e-10
+
+
+def rel_err(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300))
+
+
+CASES = [("lp", True), ("lp", False), ("explicit", True), ("explicit", False), ("lowrank", True)]
+
+
+def case_args(kind):
+    return lp_case()[1] if kind == "lp" else qp_case(kind)[1]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("iters", [50, 450, 1200])
+@pytest.mark.parametrize("kind,al", CASES)
+def test_gpu_iterates_track_oracle(oracle_lin, gpu_lin, kind, al, iters):
+    """Fixed iteration budgets through the first restarts (the evaluation and
+    restart points are every 200 iterations)."""
+    args = case_args(kind)
+    p = dict(iteration_limit=iters, eps_optimal_relative=1e-14, eps_feasible_relative=1e-14, use_al_qp=al)
+    print(f"[case] lin_cp_al {kind} al={al} iters={iters}", flush=True)
+    o = solve(oracle_lin, args, **p)
+    g = solve(gpu_lin, args, **p)
+    assert o.iterations == g.iterations == iters
+    assert rel_err(g.x, o.x) <= ITER_RTOL and rel_err(g.y, o.y) <= ITER_RTOL
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,al", CASES)
+def test_gpu_solves_match_oracle(oracle_lin, gpu_lin, kind, al):
+    args = case_args(kind)
+    print(f"[case] lin_cp_al solve {kind} al={al}", flush=True)
+    o = solve(oracle_lin, args, use_al_qp=al, **TOL)
+    g = solve(gpu_lin, args, use_al_qp=al, **TOL)
+    assert g.status == o.status == "OPTIMAL"
+    assert g.iterations == o.iterations
+    assert abs(g.primal_objective - o.primal_objective) <= 1e-9 * (1 + abs(o.primal_objective))
+    assert rel_err(g.x, o.x) <= 1e-7
