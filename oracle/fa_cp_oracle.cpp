@@ -40,8 +40,21 @@
  * (:693-770), restart test (utils.cu:451-463) and restart (:595-607), main loop
  * (solver.cu:34-114).  Its feasibility polishing is not restated (a request
  * for it prints a notice and is ignored).
+ *
+ * Built with -DORACLE_PLANTED_FA_CP it is the oracle of RHR-FA-CP, the
+ * planted-KKT FA_CP solver (QP/PLANTED_FA_CP): FA_CP's iteration and restart
+ * test (its utils.cu differs only in defaults and printing), with equality
+ * AL kernels (fa_cp_core_op.cu:147-183: right-hand side = lower bound), AL on
+ * when al_sigma != 0 (solver_state.cu:498-504), its inner-loop rule
+ * (:305-345), a fixed-point error without the clamp at zero (:893-940), and
+ * its AL sigma update at restarts (:610-747, off unless adaptive_al_sigma)
+ * in place of FA_CP's sigma rules; feasibility polishing not restated.
  */
-#ifdef ORACLE_LIN_CP_AL
+#if defined(ORACLE_PLANTED_FA_CP)
+#include "planted_fa_cp.h"
+typedef fa_cp_parameters_t params_t;
+typedef fa_cp_result_t result_t;
+#elif defined(ORACLE_LIN_CP_AL)
 #include "lin_cp_al.h"
 typedef cp_al_parameters_t params_t;
 typedef cp_al_result_t result_t;
@@ -214,6 +227,9 @@ struct S {
     double step = 0.0, pw = 1.0, best_pw = 1.0, pw_err_sum = 0.0, pw_last_err = 0.0, best_gap = INFINITY;
     // linearized CP-AL: the fixed penalty sigma and its interval, lambda of tau = 1 / (sigma lambda)
     double sigma = 0.0, sigma_base = 1.0, sigma_min = 0.0, sigma_max = 0.0, lambda_est = 1.0;
+    // planted FA_CP: iteration of the last AL sigma change, the inner tolerance a change restores
+    int last_sigma_update_iter = 0;
+    double inner_tol0 = 0.0;
     int total = 0, inner = 0;
     bool major = false;
     double fp = INFINITY, fp0 = INFINITY, last_trial_fp = INFINITY;
@@ -420,9 +436,13 @@ void al_effective_dual_product(S &s, const Vec &xv, double sigma)
 {   // compute_al_effective_dual_product, :275-303 and kernel :181-192
     spmv(s.A, xv.data(), s.prim_prod.data());
     for (int i = 0; i < s.m; ++i) {
+#ifdef ORACLE_PLANTED_FA_CP
+        s.al_u[i] = s.y[i] - sigma * (s.prim_prod[i] - s.l[i]);  // planted: equality rows (:147-156)
+#else
         const double sh = s.prim_prod[i] - s.y[i] / sigma;
         const double pr = std::fmax(s.l[i], std::fmin(sh, s.u[i]));
         s.al_u[i] = s.y[i] - sigma * (s.prim_prod[i] - pr);
+#endif
     }
     spmv(s.At, s.al_u.data(), s.dual_prod.data());
 }
@@ -510,10 +530,15 @@ void fa_cp_update(S &s)
     spmv(s.A, s.xR.data(), s.prim_prod.data());
     if (s.use_al) {  // kernels :194-223
         for (int i = 0; i < s.m; ++i) {
+#ifdef ORACLE_PLANTED_FA_CP
+            (void)al_sigma;
+            const double nd = s.y[i] + sigma * (s.l[i] - s.prim_prod[i]);  // planted: equality rows (:158-183)
+#else
             const double den = al_sigma + sigma;
             const double sh = s.prim_prod[i] - s.y[i] / den;
             const double pr = std::fmax(s.l[i], std::fmin(sh, s.u[i]));
             const double nd = s.y[i] + sigma * (pr - s.prim_prod[i]);
+#endif
             if (mk) s.yT[i] = nd;
             s.yR[i] = 2.0 * nd - s.y[i];
         }
@@ -553,7 +578,11 @@ void fixed_point_error(S &s)
     const double dn = nrm2(s.dy), pn = nrm2(s.dx);
     const double movement = pn * pn * s.pw + dn * dn / s.pw;
     const double cross = dot(s.dual_prod, s.dx);
+#ifdef ORACLE_PLANTED_FA_CP
+    s.fp = std::sqrt(movement + 2 * s.step * cross);  // planted: no clamp at zero (:928-930)
+#else
     s.fp = std::sqrt(std::fmax(movement + 2 * s.step * cross, 0.0));
+#endif
     if (s.has_inner) s.inner_tol = std::fmin(s.inner_tol, std::fmax(0.0005 * pn / s.step * s.pw, 1e-9));
 }
 
@@ -699,7 +728,7 @@ bool should_restart(S &s)
     return r;
 }
 
-#ifndef ORACLE_LIN_CP_AL
+#if !defined(ORACLE_LIN_CP_AL) && !defined(ORACLE_PLANTED_FA_CP)
 void set_sigma_active(S &s, double sigma, double zero_thr)
 {   // :667-676
     if (!std::isfinite(sigma) || sigma <= zero_thr) {
@@ -1014,6 +1043,83 @@ void feasibility_polish(S &s, double &polish_time, int &polish_iters)
     polish_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
+#elif defined(ORACLE_PLANTED_FA_CP)
+
+/* ---------------- planted FA_CP restart (QP/PLANTED_FA_CP/src/fa_cp_core_op.cu) ---------------- */
+
+bool planted_apply_sigma(S &s, double nsig)
+{   // apply_new_al_sigma, :626-658 (the trial bookkeeping it clears is never read)
+    if (std::fabs(nsig - s.al_sigma) <= 1e-12 * std::fmax(1.0, std::fabs(s.al_sigma))) return false;
+    s.al_sigma = nsig;
+    s.last_sigma_update_iter = s.total;
+    if (s.has_inner) s.inner_tol = s.inner_tol0;  // reset_inner_solver_tolerance, :610-614
+    return true;
+}
+
+void planted_sigma_on_restart(S &s)
+{   // maybe_update_al_sigma_on_restart, :670-747
+    const auto &P = s.P;
+    if (!P.adaptive_al_sigma || !s.use_al) return;
+    if (s.last_sigma_update_iter == s.total) return;
+    const double rp = std::fabs(s.rel_p), rd = std::fabs(s.rel_d), rg = std::fabs(s.rel_gap);
+    if (!std::isfinite(rp) || !std::isfinite(rd) || !std::isfinite(rg)) return;
+    const double dg = std::fmax(rd, rg);
+    const double pref = std::fmax(rp, 1e-12), dgref = std::fmax(dg, 1e-12);
+    const double ratio = rp / dgref;
+    const bool up = rp > P.al_sigma_primal_trigger * dgref && P.adaptive_al_sigma_mode != ADAPTIVE_AL_SIGMA_MODE_DOWN_ONLY;
+    const bool down = dg > P.al_sigma_gap_trigger * pref && P.adaptive_al_sigma_mode != ADAPTIVE_AL_SIGMA_MODE_UP_ONLY;
+    double ns = s.al_sigma;
+    switch (P.adaptive_al_sigma_rule) {
+    case ADAPTIVE_AL_SIGMA_RULE_TRIGGER:
+        if (up) ns = std::fmin(P.al_sigma_max, s.al_sigma * P.al_sigma_increase_factor);
+        else if (down) ns = std::fmax(P.al_sigma_min, s.al_sigma * P.al_sigma_decrease_factor);
+        break;
+    case ADAPTIVE_AL_SIGMA_RULE_SQRT:
+        if (up) ns = std::fmin(P.al_sigma_max, s.al_sigma * std::fmin(P.al_sigma_increase_factor, std::sqrt(ratio)));
+        else if (down) ns = std::fmax(P.al_sigma_min, s.al_sigma * std::fmax(P.al_sigma_decrease_factor, std::sqrt(ratio)));
+        break;
+    case ADAPTIVE_AL_SIGMA_RULE_BANGBANG:
+        if (up) ns = P.al_sigma_max;
+        else if (down) ns = P.al_sigma_min;
+        break;
+    default: break;
+    }
+    planted_apply_sigma(s, ns);
+}
+
+void perform_restart(S &s)
+{   // perform_restart, :811-871: PID primal weight as FA_CP, then the AL sigma rule
+    for (int i = 0; i < s.n; ++i) s.dx[i] = s.xT[i] - s.x0[i];
+    for (int i = 0; i < s.m; ++i) s.dy[i] = s.yT[i] - s.y0[i];
+    const double pd = nrm2(s.dx), dd = nrm2(s.dy);
+    const double ratio = s.rel_d / s.rel_p;
+    const auto &rp = s.P.restart_params;
+    if (pd > 1e-16 && dd > 1e-16 && pd < 1e12 && dd < 1e12 && ratio > 1e-8 && ratio < 1e8) {
+        const double err = std::log(dd) - std::log(pd) - std::log(s.pw);
+        s.pw_err_sum *= rp.i_smooth;
+        s.pw_err_sum += err;
+        const double de = err - s.pw_last_err;
+        s.pw *= std::exp(rp.k_p * err + rp.k_i * s.pw_err_sum + rp.k_d * de);
+        s.pw_last_err = err;
+    } else {
+        s.pw = s.best_pw;
+        s.pw_err_sum = 0.0;
+        s.pw_last_err = 0.0;
+    }
+    const double g = std::fabs(std::log10(s.rel_d / s.rel_p));
+    if (g < s.best_gap) {
+        s.best_gap = g;
+        s.best_pw = s.pw;
+    }
+    planted_sigma_on_restart(s);
+    s.x0 = s.xT;
+    s.y0 = s.yT;
+    s.x = s.xT;
+    s.y = s.yT;
+    s.inner = 0;
+    s.last_trial_fp = INFINITY;
+}
+
 #else  // ORACLE_LIN_CP_AL
 
 /* ---------------- linearized CP-AL (QP/LIN_CP_AL/src/cp_al_core_op.cu) ---------------- */
@@ -1177,7 +1283,11 @@ void init_state(S &s, const qp_problem_t *w)
     s.Rt = transpose(s.R);
     s.al_available = s.P.use_al_qp;
     s.use_al = s.P.use_al_qp;
-#ifndef ORACLE_LIN_CP_AL
+#if defined(ORACLE_PLANTED_FA_CP)
+    // planted solver_state.cu:498-504: the AL branch only with a nonzero sigma
+    s.use_al = s.P.use_al_qp && s.P.al_sigma != 0.0;
+    s.al_sigma = s.use_al ? s.P.al_sigma : 0.0;
+#elif !defined(ORACLE_LIN_CP_AL)
     s.al_sigma = s.P.al_sigma;
     s.best_al_sigma = s.P.al_sigma;
 #endif
@@ -1234,7 +1344,15 @@ void init_state(S &s, const qp_problem_t *w)
         s.q_norm = mxe;
         s.nonconvexity = mne;
     }
-#ifdef ORACLE_LIN_CP_AL
+#if defined(ORACLE_PLANTED_FA_CP)
+    // planted FA_CP solver_state.cu:305-345: the inner loop for non-diagonal Q, and for diagonal Q in the AL branch
+    const bool inner_loop = s.qtype != FA_CP_NON_Q && (s.qtype != FA_CP_DIAG_Q || s.use_al);
+    s.has_inner = inner_loop;
+    if (inner_loop) {
+        s.inner_limit = 1000;
+        s.inner_tol = s.inner_tol0 = 1e-3;
+    }
+#elif defined(ORACLE_LIN_CP_AL)
     // LIN_CP_AL solver_state.cu:305-340: an inner loop only for the non-diagonal Q types
     // (the linearized update never runs it)
     const bool simple = s.qtype == FA_CP_NON_Q || s.qtype == FA_CP_DIAG_Q;
@@ -1264,7 +1382,9 @@ void init_state(S &s, const qp_problem_t *w)
 
 /* ---------------- public C API ---------------- */
 
-#ifdef ORACLE_LIN_CP_AL
+#if defined(ORACLE_PLANTED_FA_CP)
+ORACLE_API void planted_fa_cp_oracle_reset_rng(void)
+#elif defined(ORACLE_LIN_CP_AL)
 ORACLE_API void lin_cp_al_oracle_reset_rng(void)
 #else
 ORACLE_API void fa_cp_oracle_reset_rng(void)
@@ -1289,6 +1409,47 @@ ORACLE_API void set_default_parameters(cp_al_parameters_t *p)  // LIN_CP_AL util
     p->termination_evaluation_frequency = 200;
     p->feasibility_polishing = false;
     p->reflection_coefficient = 0.0;
+    p->sv_max_iter = 5000;
+    p->sv_tol = 1e-4;
+    p->termination_criteria.eps_optimal_relative = 1e-4;
+    p->termination_criteria.eps_feasible_relative = 1e-4;
+    p->termination_criteria.eps_infeasible = 1e-10;
+    p->termination_criteria.time_sec_limit = 3600.0;
+    p->termination_criteria.iteration_limit = INT32_MAX;
+    p->termination_criteria.eps_feas_polish_relative = 1e-6;
+    p->restart_params.artificial_restart_threshold = 0.36;
+    p->restart_params.sufficient_reduction_for_restart = 0.2;
+    p->restart_params.necessary_reduction_for_restart = 0.8;
+    p->restart_params.k_p = 0.99;
+    p->restart_params.k_i = 0.01;
+    p->restart_params.k_d = 0.0;
+    p->restart_params.i_smooth = 0.3;
+    p->optimality_norm = NORM_TYPE_L_INF;
+}
+#elif defined(ORACLE_PLANTED_FA_CP)
+ORACLE_API void set_default_parameters(fa_cp_parameters_t *p)  // PLANTED_FA_CP utils.cu:471-512
+{
+    memset(p, 0, sizeof(*p));
+    p->l_inf_ruiz_iterations = 10;
+    p->has_pock_chambolle_alpha = true;
+    p->pock_chambolle_alpha = 1.0;
+    p->bound_objective_rescaling = true;
+    p->use_al_qp = true;
+    p->al_sigma = 0.1;
+    p->adaptive_al_sigma = false;
+    p->adaptive_al_sigma_mode = ADAPTIVE_AL_SIGMA_MODE_BIDIRECTIONAL;
+    p->adaptive_al_sigma_rule = ADAPTIVE_AL_SIGMA_RULE_TRIGGER;
+    p->al_sigma_min = 5e-3;
+    p->al_sigma_max = 3e-1;
+    p->al_sigma_increase_factor = 1.5;
+    p->al_sigma_decrease_factor = 0.7;
+    p->al_sigma_primal_trigger = 3.0;
+    p->al_sigma_gap_trigger = 5.0;
+    p->trajectory_csv = nullptr;
+    p->verbose = 1;
+    p->termination_evaluation_frequency = 200;
+    p->feasibility_polishing = false;
+    p->reflection_coefficient = 1.0;
     p->sv_max_iter = 5000;
     p->sv_tol = 1e-4;
     p->termination_criteria.eps_optimal_relative = 1e-4;
@@ -1632,7 +1793,11 @@ ORACLE_API fa_cp_result_t *optimize(const fa_cp_parameters_t *params, const qp_p
     int polish_iters = 0;
     if (s.P.feasibility_polishing && s.reason != TERMINATION_REASON_DUAL_INFEASIBLE &&
         s.reason != TERMINATION_REASON_PRIMAL_INFEASIBLE)  // solver.cu:156-160
+#ifdef ORACLE_PLANTED_FA_CP
+        fprintf(stderr, "[planted fa_cp oracle] feasibility polishing is not restated; skipped.\n");
+#else
         feasibility_polish(s, polish_time, polish_iters);
+#endif
     return result_from_state(s, prob, polish_time, polish_iters);
 }
 #endif
@@ -1668,7 +1833,7 @@ static result_t *result_from_state(S &s, const qp_problem_t *prob, double polish
     r->max_dual_ray_infeasibility = s.max_d_ray;
     r->primal_ray_linear_objective = s.p_ray_obj;
     r->dual_ray_objective = s.d_ray_obj;
-#ifndef ORACLE_LIN_CP_AL
+#if !defined(ORACLE_LIN_CP_AL) && !defined(ORACLE_PLANTED_FA_CP)
     r->final_al_sigma = s.al_sigma;
     r->sigma_update_count = s.sigma_update_count;
     r->sigma_update_rule = s.al_available ? s.P.sigma_update_rule : FA_CP_SIGMA_UPDATE_FIXED;

@@ -118,6 +118,18 @@ def build_lin_cp_al() -> Path:
     return out
 
 
+def build_planted_fa_cp() -> Path:
+    """The planted-KKT FA_CP (RHR-FA-CP): the FA_CP sources built with -DPDAL_PLANTED_FA_CP."""
+    objdir = ROOT / "pdal_amd" / "_build" / "planted_fa_cp"
+    objs = _compile_objects(FA_CP_SOURCES, objdir, extra=["-DPDAL_PLANTED_FA_CP"])
+    LIB.mkdir(parents=True, exist_ok=True)
+    out = LIB / "librhr_fa_cp.so"
+    tmp = LIB / "librhr_fa_cp.so.tmp"
+    _run([_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-Bsymbolic", *objs, "-lz", "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
 def build_ws() -> Path:
     objdir = ROOT / "pdal_amd" / "_build" / "ws"
     objs = _compile_objects([CSRC / "ws" / "warmstart.hip"], objdir)
@@ -133,6 +145,7 @@ def build_all() -> None:
     build_cp_al()
     build_fa_cp()
     build_lin_cp_al()
+    build_planted_fa_cp()
     build_ws()
 
 

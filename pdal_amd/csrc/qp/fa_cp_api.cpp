@@ -7,7 +7,7 @@
 // summary), :1135-1358 (dense / CSC / COO -> CSR with zero_tolerance);
 // QP/LIN_CP_AL/src/utils.cu:466-497 (its defaults).
 #include "fa_cp_internal.hpp"
-#ifndef PDAL_LIN_CP_AL
+#if !defined(PDAL_LIN_CP_AL) && !defined(PDAL_PLANTED_FA_CP)
 #include "pdal_qp.h"
 #endif
 
@@ -181,6 +181,47 @@ PDAL_API void set_default_parameters(cp_al_parameters_t *p)
     p->termination_evaluation_frequency = 200;
     p->feasibility_polishing = false;
     p->reflection_coefficient = 0.0;
+    p->sv_max_iter = 5000;
+    p->sv_tol = 1e-4;
+    p->termination_criteria.eps_optimal_relative = 1e-4;
+    p->termination_criteria.eps_feasible_relative = 1e-4;
+    p->termination_criteria.eps_infeasible = 1e-10;
+    p->termination_criteria.time_sec_limit = 3600.0;
+    p->termination_criteria.iteration_limit = INT32_MAX;
+    p->termination_criteria.eps_feas_polish_relative = 1e-6;
+    p->restart_params.artificial_restart_threshold = 0.36;
+    p->restart_params.sufficient_reduction_for_restart = 0.2;
+    p->restart_params.necessary_reduction_for_restart = 0.8;
+    p->restart_params.k_p = 0.99;
+    p->restart_params.k_i = 0.01;
+    p->restart_params.k_d = 0.0;
+    p->restart_params.i_smooth = 0.3;
+    p->optimality_norm = NORM_TYPE_L_INF;
+}
+#elif defined(PDAL_PLANTED_FA_CP)
+PDAL_API void set_default_parameters(fa_cp_parameters_t *p)
+{   // QP/PLANTED_FA_CP/src/utils.cu:471-512
+    memset(p, 0, sizeof(*p));
+    p->l_inf_ruiz_iterations = 10;
+    p->has_pock_chambolle_alpha = true;
+    p->pock_chambolle_alpha = 1.0;
+    p->bound_objective_rescaling = true;
+    p->use_al_qp = true;
+    p->al_sigma = 0.1;
+    p->adaptive_al_sigma = false;
+    p->adaptive_al_sigma_mode = ADAPTIVE_AL_SIGMA_MODE_BIDIRECTIONAL;
+    p->adaptive_al_sigma_rule = ADAPTIVE_AL_SIGMA_RULE_TRIGGER;
+    p->al_sigma_min = 5e-3;
+    p->al_sigma_max = 3e-1;
+    p->al_sigma_increase_factor = 1.5;
+    p->al_sigma_decrease_factor = 0.7;
+    p->al_sigma_primal_trigger = 3.0;
+    p->al_sigma_gap_trigger = 5.0;
+    p->trajectory_csv = nullptr;
+    p->verbose = 1;
+    p->termination_evaluation_frequency = 200;
+    p->feasibility_polishing = false;
+    p->reflection_coefficient = 1.0;
     p->sv_max_iter = 5000;
     p->sv_tol = 1e-4;
     p->termination_criteria.eps_optimal_relative = 1e-4;
@@ -393,10 +434,17 @@ PDAL_API result_t *solve_qp_problem(const qp_problem_t *prob, const params_t *pa
     return res;
 }
 
-#ifdef PDAL_LIN_CP_AL
+#if defined(PDAL_LIN_CP_AL)
 PDAL_API void pdal_lin_cp_al_reset_rng(void) { pdal::qp::reset_rng(); }
 
 PDAL_API const char *pdal_lin_cp_al_termination_reason_to_string(termination_reason_t r)
+{
+    return pdal::qp::termination_reason_name(r);
+}
+#elif defined(PDAL_PLANTED_FA_CP)
+PDAL_API void pdal_planted_fa_cp_reset_rng(void) { pdal::qp::reset_rng(); }
+
+PDAL_API const char *pdal_planted_fa_cp_termination_reason_to_string(termination_reason_t r)
 {
     return pdal::qp::termination_reason_name(r);
 }

@@ -8,6 +8,7 @@
 #pragma once
 
 #include "../common/device_reduce.hpp"
+#include "qp_variant.hpp"
 #include "../common/spmv.hpp"
 
 namespace pdal {
@@ -100,10 +101,15 @@ static __global__ void __launch_bounds__(kBlock) dual_kernel(DualArgs a)
     const double yi = a.y[row];
     double yr;
     if (AL) {
-        const double den = a.sc.al_sigma + sigma;
-        const double sh = pp - yi / den;
-        const double pr = fmax(a.l[row], fmin(sh, a.u[row]));
-        const double nd = yi + sigma * (pr - pp);
+        double nd;
+        if (kEqualityAL) {
+            nd = yi + sigma * (a.l[row] - pp);
+        } else {
+            const double den = a.sc.al_sigma + sigma;
+            const double sh = pp - yi / den;
+            const double pr = fmax(a.l[row], fmin(sh, a.u[row]));
+            nd = yi + sigma * (pr - pp);
+        }
         if (MAJOR) a.yT[row] = nd;
         yr = 2.0 * nd - yi;
     } else {
@@ -203,10 +209,14 @@ static __global__ void __launch_bounds__(kBlock)
     if (lane != 0) return;
     const double s = sc.al_sigma;
     const double yi = y[row];
-    const double sh = pp - yi / s;
-    const double pr = fmax(l[row], fmin(sh, u[row]));
     pp_out[row] = pp;
-    uo[row] = yi - s * (pp - pr);
+    if (kEqualityAL) {
+        uo[row] = yi - s * (pp - l[row]);
+    } else {
+        const double sh = pp - yi / s;
+        const double pr = fmax(l[row], fmin(sh, u[row]));
+        uo[row] = yi - s * (pp - pr);
+    }
 }
 
 // ---------------------------------------------------------------------------

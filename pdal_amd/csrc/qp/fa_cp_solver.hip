@@ -25,6 +25,12 @@
 // relaxed Halpern step (:577-585), per-iteration fixed-point error (:693-770),
 // restart test and restart (utils.cu:451-463, :595-607), main loop
 // (solver.cu:34-114).  Its feasibility polishing is not built.
+//
+// Built with -DPDAL_PLANTED_FA_CP it is RHR-FA-CP, the planted-KKT FA_CP
+// (QP/PLANTED_FA_CP): this iteration with equality AL kernels, the AL branch
+// only when al_sigma != 0, its inner-loop rule, the fixed-point error without
+// the clamp at zero, and its own AL sigma rule at restarts
+// (fa_cp_core_op.cu:610-747) instead of FA_CP's; feasibility polishing not built.
 #include "fa_cp_internal.hpp"
 
 #include "../common/csr_host.hpp"
@@ -166,6 +172,10 @@ class Solver {
     double clipped_update(double cur, double ratio) const;
     void update_al_sigma(double pd, double dd, double normal);
     void perform_restart();
+    // planted-KKT FA_CP (PDAL_PLANTED_FA_CP builds)
+    int last_sigma_update_iter_ = 0;
+    double inner_tol0_ = 0.0;
+    void planted_sigma_on_restart();
     // linearized CP-AL (PDAL_LIN_CP_AL builds)
     double sigma_ = 0.0, sigma_base_ = 1.0, sigma_min_ = 0.0, sigma_max_ = 0.0, lambda_est_ = 1.0;
     double lin_tau() const { return 1.0 / std::fmax(sigma_ * lambda_est_, 1e-16); }
@@ -468,7 +478,11 @@ void Solver::init_state()
     const qp_problem_t *w = W_;
     al_available_ = P_.use_al_qp;
     use_al_ = P_.use_al_qp;
-#ifndef PDAL_LIN_CP_AL
+#if defined(PDAL_PLANTED_FA_CP)
+    // planted solver_state.cu:498-504: the AL branch only with a nonzero sigma
+    use_al_ = P_.use_al_qp && P_.al_sigma != 0.0;
+    al_sigma_ = use_al_ ? P_.al_sigma : 0.0;
+#elif !defined(PDAL_LIN_CP_AL)
     al_sigma_ = P_.al_sigma;
     best_al_sigma_ = P_.al_sigma;
 #endif
@@ -537,7 +551,15 @@ void Solver::init_state()
         q_norm_ = q_max_eig();
         nonconvexity_ = q_min_eig(q_norm_);
     }
-#ifdef PDAL_LIN_CP_AL
+#if defined(PDAL_PLANTED_FA_CP)
+    // planted solver_state.cu:305-345: the inner loop for non-diagonal Q, and for diagonal Q in the AL branch
+    const bool inner_loop = qtype_ != FA_CP_NON_Q && (qtype_ != FA_CP_DIAG_Q || use_al_);
+    has_inner_ = inner_loop;
+    if (inner_loop) {
+        inner_limit_ = 1000;
+        inner_tol_ = inner_tol0_ = 1e-3;
+    }
+#elif defined(PDAL_LIN_CP_AL)
     // LIN_CP_AL solver_state.cu:305-340: an inner loop only for the non-diagonal Q types
     // (its linearized update never runs it)
     const bool simple = qtype_ == FA_CP_NON_Q || qtype_ == FA_CP_DIAG_Q;
@@ -810,7 +832,11 @@ void Solver::fixed_point_error()
     const double dn = std::sqrt(hs(S_DN2)), pn = std::sqrt(hs(S_FP + FP_PN2));
     const double movement = pn * pn * pw_ + dn * dn / pw_;
     const double cross = hs(S_FP + FP_CROSS);
+#ifdef PDAL_PLANTED_FA_CP
+    fp_ = std::sqrt(movement + 2 * step_ * cross);  // planted: no clamp at zero (:928-930)
+#else
     fp_ = std::sqrt(std::fmax(movement + 2 * step_ * cross, 0.0));
+#endif
     if (has_inner_) inner_tol_ = std::fmin(inner_tol_, std::fmax(0.0005 * pn / step_ * pw_, 1e-9));
 }
 
@@ -942,7 +968,7 @@ bool Solver::should_restart()
     return r;
 }
 
-#ifndef PDAL_LIN_CP_AL
+#if !defined(PDAL_LIN_CP_AL) && !defined(PDAL_PLANTED_FA_CP)
 void Solver::set_sigma_active(double sigma, double zero_thr)
 {   // fa_cp_core_op.cu:667-676
     if (!std::isfinite(sigma) || sigma <= zero_thr) {
@@ -1055,6 +1081,45 @@ void Solver::update_al_sigma(double pd, double dd, double normal_motion)
     sigma_update_count_++;
 }
 
+#endif
+
+#ifdef PDAL_PLANTED_FA_CP
+void Solver::planted_sigma_on_restart()
+{   // maybe_update_al_sigma_on_restart + apply_new_al_sigma, QP/PLANTED_FA_CP fa_cp_core_op.cu:610-747
+    const auto &P = P_;
+    if (!P.adaptive_al_sigma || !use_al_) return;
+    if (last_sigma_update_iter_ == total_) return;
+    const double rp = std::fabs(rel_p_), rd = std::fabs(rel_d_), rg = std::fabs(rel_gap_);
+    if (!std::isfinite(rp) || !std::isfinite(rd) || !std::isfinite(rg)) return;
+    const double dg = std::fmax(rd, rg);
+    const double pref = std::fmax(rp, 1e-12), dgref = std::fmax(dg, 1e-12);
+    const double ratio = rp / dgref;
+    const bool up = rp > P.al_sigma_primal_trigger * dgref && P.adaptive_al_sigma_mode != ADAPTIVE_AL_SIGMA_MODE_DOWN_ONLY;
+    const bool down = dg > P.al_sigma_gap_trigger * pref && P.adaptive_al_sigma_mode != ADAPTIVE_AL_SIGMA_MODE_UP_ONLY;
+    double ns = al_sigma_;
+    switch (P.adaptive_al_sigma_rule) {
+    case ADAPTIVE_AL_SIGMA_RULE_TRIGGER:
+        if (up) ns = std::fmin(P.al_sigma_max, al_sigma_ * P.al_sigma_increase_factor);
+        else if (down) ns = std::fmax(P.al_sigma_min, al_sigma_ * P.al_sigma_decrease_factor);
+        break;
+    case ADAPTIVE_AL_SIGMA_RULE_SQRT:
+        if (up) ns = std::fmin(P.al_sigma_max, al_sigma_ * std::fmin(P.al_sigma_increase_factor, std::sqrt(ratio)));
+        else if (down) ns = std::fmax(P.al_sigma_min, al_sigma_ * std::fmax(P.al_sigma_decrease_factor, std::sqrt(ratio)));
+        break;
+    case ADAPTIVE_AL_SIGMA_RULE_BANGBANG:
+        if (up) ns = P.al_sigma_max;
+        else if (down) ns = P.al_sigma_min;
+        break;
+    default: break;
+    }
+    if (std::fabs(ns - al_sigma_) <= 1e-12 * std::fmax(1.0, std::fabs(al_sigma_))) return;
+    al_sigma_ = ns;
+    last_sigma_update_iter_ = total_;
+    if (has_inner_) inner_tol_ = inner_tol0_;  // reset_inner_solver_tolerance, :610-614
+}
+#endif
+
+#ifndef PDAL_LIN_CP_AL
 void Solver::perform_restart()
 {   // perform_restart, fa_cp_core_op.cu:908-1020 (restart target: the T-output)
     const int nbn = stride_grid(n_), nbm = stride_grid(m_);
@@ -1097,7 +1162,12 @@ void Solver::perform_restart()
         best_gap_ = g;
         best_pw_ = pw_;
     }
+#ifdef PDAL_PLANTED_FA_CP
+    (void)normal;
+    planted_sigma_on_restart();
+#else
     update_al_sigma(pd, dd, normal);
+#endif
     restart_copy_kernel<<<blocks_for((long)n_ + m_), kBlock, 0, stream_>>>(xT_.get(), x0_.get(), x_.get(), n_,
                                                                            yT_.get(), y0_.get(), y_.get(), m_);
     PDAL_LAUNCHED();
@@ -1144,7 +1214,7 @@ result_t *Solver::make_result()
     r->max_dual_ray_infeasibility = max_d_ray_;
     r->primal_ray_linear_objective = p_ray_obj_;
     r->dual_ray_objective = d_ray_obj_;
-#ifndef PDAL_LIN_CP_AL
+#if !defined(PDAL_LIN_CP_AL) && !defined(PDAL_PLANTED_FA_CP)
     r->final_al_sigma = al_sigma_;
     r->sigma_update_count = sigma_update_count_;
     r->sigma_update_rule = al_available_ ? P_.sigma_update_rule : FA_CP_SIGMA_UPDATE_FIXED;
@@ -1159,7 +1229,7 @@ void Solver::print_initial_info() const
 {   // print_initial_info, utils.cu:560-659 (the settings that differ from the defaults are omitted)
     if (P_.verbose < 2) return;
     printf("---------------------------------------------------------------------------------------\n");
-    printf("                          %s (MI355X / gfx950 HIP build)\n", kLinearized ? "RHR-Lin-CP-AL" : "FA_CP");
+    printf("                          %s (MI355X / gfx950 HIP build)\n", kLinearized ? "RHR-Lin-CP-AL" : kEqualityAL ? "RHR-FA-CP" : "FA_CP");
     printf("---------------------------------------------------------------------------------------\n");
     printf("problem: %d rows, %d columns, %d nonzeros\n", W_->num_constraints, W_->num_variables,
            W_->constraint_matrix_num_nonzeros);
@@ -1422,12 +1492,16 @@ result_t *Solver::run()
     }
     if (P_.feasibility_polishing && reason_ != TERMINATION_REASON_DUAL_INFEASIBLE &&
         reason_ != TERMINATION_REASON_PRIMAL_INFEASIBLE)  // solver.cu:156-160
+#ifdef PDAL_PLANTED_FA_CP
+        fprintf(stderr, "[rhr_fa_cp] feasibility polishing is not built for the planted-KKT solver; skipped.\n");
+#else
         feasibility_polish();
+#endif
     return make_result();
 #endif
 }
 
-#ifndef PDAL_LIN_CP_AL
+#if !defined(PDAL_LIN_CP_AL) && !defined(PDAL_PLANTED_FA_CP)
 // ---------------------------------------------------------------------------
 // Feasibility polishing (fa_cp_core_op.cu:1242-1735, utils.cu:1360-1575): a
 // primal phase with c = 0 (Q kept) and a dual phase with every finite bound 0,
@@ -1674,7 +1748,7 @@ void Solver::feasibility_polish()
     fflush(stdout);
     polish_time_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
 }
-#endif  // !PDAL_LIN_CP_AL
+#endif  // polishing
 
 }  // namespace
 

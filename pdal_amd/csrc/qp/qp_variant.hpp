@@ -1,12 +1,15 @@
-// Which QP solver a translation unit builds: FA_CP (libfa_cp.so, the default)
-// or, with -DPDAL_LIN_CP_AL, the linearized CP-AL solver RHR-Lin-CP-AL
-// (librhr_lin_cp_al.so).  The two share problem construction, scaling, power
+// Which QP solver a translation unit builds: FA_CP (libfa_cp.so, the default);
+// with -DPDAL_LIN_CP_AL the linearized CP-AL solver RHR-Lin-CP-AL
+// (librhr_lin_cp_al.so); with -DPDAL_PLANTED_FA_CP the planted-KKT FA_CP
+// RHR-FA-CP (librhr_fa_cp.so).  The two share problem construction, scaling, power
 // iterations, residuals, infeasibility detection and termination; they differ
 // in their parameter and result types and in the iteration itself.
 #pragma once
 
-#ifdef PDAL_LIN_CP_AL
+#if defined(PDAL_LIN_CP_AL)
 #include "lin_cp_al.h"
+#elif defined(PDAL_PLANTED_FA_CP)
+#include "planted_fa_cp.h"
 #else
 #include "fa_cp.h"
 #endif
@@ -28,6 +31,14 @@ constexpr bool kLinearized = true;
 using params_t = fa_cp_parameters_t;
 using result_t = fa_cp_result_t;
 constexpr bool kLinearized = false;
+#endif
+
+// Planted-KKT FA_CP: the AL kernels treat every row as an equality with the
+// lower bound as right-hand side (QP/PLANTED_FA_CP/src/fa_cp_core_op.cu:147-183).
+#ifdef PDAL_PLANTED_FA_CP
+constexpr bool kEqualityAL = true;
+#else
+constexpr bool kEqualityAL = false;
 #endif
 
 }  // namespace qp

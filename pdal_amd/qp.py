@@ -164,6 +164,37 @@ class LinCpAlResult(C.Structure):
                 if f[0] not in ("final_al_sigma", "sigma_update_count", "sigma_update_rule")]
 
 
+class PlantedFaCpParameters(C.Structure):
+    """fa_cp_parameters_t of the planted-KKT FA_CP (planted_fa_cp_types.h)."""
+    _fields_ = [
+        ("l_inf_ruiz_iterations", C.c_int),
+        ("has_pock_chambolle_alpha", C.c_bool),
+        ("pock_chambolle_alpha", C.c_double),
+        ("bound_objective_rescaling", C.c_bool),
+        ("use_al_qp", C.c_bool),
+        ("al_sigma", C.c_double),
+        ("adaptive_al_sigma", C.c_bool),
+        ("adaptive_al_sigma_mode", C.c_int),
+        ("adaptive_al_sigma_rule", C.c_int),
+        ("al_sigma_min", C.c_double),
+        ("al_sigma_max", C.c_double),
+        ("al_sigma_increase_factor", C.c_double),
+        ("al_sigma_decrease_factor", C.c_double),
+        ("al_sigma_primal_trigger", C.c_double),
+        ("al_sigma_gap_trigger", C.c_double),
+        ("trajectory_csv", C.c_char_p),
+        ("verbose", C.c_int),
+        ("termination_evaluation_frequency", C.c_int),
+        ("sv_max_iter", C.c_int),
+        ("sv_tol", C.c_double),
+        ("termination_criteria", TerminationCriteria),
+        ("restart_params", RestartParameters),
+        ("reflection_coefficient", C.c_double),
+        ("feasibility_polishing", C.c_bool),
+        ("optimality_norm", C.c_int),
+    ]
+
+
 class _Dense(C.Structure):
     _fields_ = [("A", _dp)]
 
@@ -389,6 +420,15 @@ class LinCpAl(FaCp):
     reset_names = ("pdal_lin_cp_al_reset_rng", "lin_cp_al_oracle_reset_rng")
 
 
+class PlantedFaCp(FaCp):
+    """A library exporting the planted-KKT FA_CP entry points (planted_fa_cp.h):
+    FA_CP's names, its own parameter type, the result without the sigma fields."""
+    Params = PlantedFaCpParameters
+    Result = LinCpAlResult  # the same layout: FA_CP's result without the sigma fields
+    result_free = "fa_cp_result_free"
+    reset_names = ("pdal_planted_fa_cp_reset_rng", "planted_fa_cp_oracle_reset_rng")
+
+
 def set_param(p, key: str, value) -> None:
     if key in ("iteration_limit", "eps_optimal_relative", "eps_feasible_relative", "time_sec_limit",
                "eps_feas_polish_relative", "eps_infeasible"):
@@ -407,6 +447,11 @@ def set_param(p, key: str, value) -> None:
 def load_fa_cp() -> FaCp:
     """The MI355X libfa_cp.so; raises FileNotFoundError if it is not built."""
     return FaCp(LIB_DIR / "libfa_cp.so")
+
+
+def load_planted_fa_cp() -> PlantedFaCp:
+    """The MI355X librhr_fa_cp.so; raises FileNotFoundError if it is not built."""
+    return PlantedFaCp(LIB_DIR / "librhr_fa_cp.so")
 
 
 def load_lin_cp_al() -> LinCpAl:
