@@ -38,8 +38,8 @@
  * (:496-575: AL effective dual, one projected gradient step, equality dual),
  * relaxed Halpern step (:187-206, 577-585), fixed-point error every iteration
  * (:693-770), restart test (utils.cu:451-463) and restart (:595-607), main loop
- * (solver.cu:34-114).  Its feasibility polishing is not restated (a request
- * for it prints a notice and is ignored).
+ * (solver.cu:34-114); feasibility polishing as FA_CP's with the linearized
+ * iteration and fixed-point errors (cp_al_core_op.cu:839-1375).
  *
  * Built with -DORACLE_PLANTED_FA_CP it is the oracle of RHR-FA-CP, the
  * planted-KKT FA_CP solver (QP/PLANTED_FA_CP): FA_CP's iteration and restart
@@ -48,7 +48,7 @@
  * when al_sigma != 0 (solver_state.cu:498-504), its inner-loop rule
  * (:305-345), a fixed-point error without the clamp at zero (:893-940), and
  * its AL sigma update at restarts (:610-747, off unless adaptive_al_sigma)
- * in place of FA_CP's sigma rules; feasibility polishing not restated.
+ * in place of FA_CP's sigma rules; its feasibility polishing is FA_CP's code.
  */
 #if defined(ORACLE_PLANTED_FA_CP)
 #include "planted_fa_cp.h"
@@ -878,8 +878,18 @@ void perform_restart(S &s)
     s.inner = 0;
     s.last_trial_fp = INFINITY;
 }
+#endif  // FA_CP sigma rules and restart
 
-/* ---------------- feasibility polishing (fa_cp_core_op.cu:1242-1735) ---------------- */
+/* ---------------- feasibility polishing (fa_cp_core_op.cu:1242-1735; the planted
+   solver's QP/PLANTED_FA_CP/src/fa_cp_core_op.cu:1009-1502 is the same code; the
+   linearized solver's QP/LIN_CP_AL/src/cp_al_core_op.cu:839-1375 runs its own
+   iteration, restart test and fixed-point errors in the same loop) ---------------- */
+
+#ifdef ORACLE_LIN_CP_AL
+void lin_update(S &s);
+void lin_halpern(S &s);
+bool lin_should_restart(const S &s);
+#endif
 
 struct PolishOut {
     termination_reason_t reason;
@@ -932,7 +942,11 @@ PolishOut polish_loop(S &ps, const S &ori, bool primal)
                 ps.reason = TERMINATION_REASON_TIME_LIMIT;
         }
         if (consider) {  // also after a stop, as the reference loop runs it
+#ifdef ORACLE_LIN_CP_AL
+            do_restart = lin_should_restart(ps);  // never true here: fp0 starts at infinity
+#else
             do_restart = should_restart(ps);
+#endif
             if (do_restart) {  // perform_primal_restart / perform_dual_restart, :1668-1690
                 if (primal) {
                     ps.x0 = ps.xT;
@@ -946,6 +960,31 @@ PolishOut polish_loop(S &ps, const S &ori, bool primal)
             }
         }
         ps.major = ((ps.total + 1) % K) == 0;
+#ifdef ORACLE_LIN_CP_AL
+        lin_update(ps);
+        if (ps.major || do_restart) {  // compute_primal/dual_fixed_point_error, LIN_CP_AL :1331-1375
+            if (primal) {
+                for (int i = 0; i < ps.n; ++i) ps.dx[i] = ps.xR[i] - ps.x[i];
+                const double pn = nrm2(ps.dx);
+                if (ps.use_al) {
+                    spmv(ps.A, ps.dx.data(), ps.prim_prod.data());
+                    const double an = nrm2(ps.prim_prod);
+                    ps.fp = std::fmax(0.0, ps.sigma * (ps.lambda_est * pn * pn - an * an));
+                } else {
+                    ps.fp = pn * pn * ps.pw;
+                }
+            } else {
+                for (int i = 0; i < ps.m; ++i) ps.dy[i] = ps.yR[i] - ps.y[i];
+                const double dn = nrm2(ps.dy);
+                ps.fp = dn * dn / (ps.use_al ? ps.sigma : ps.pw);
+            }
+            if (do_restart) {
+                ps.fp0 = ps.fp;
+                do_restart = false;
+            }
+        }
+        lin_halpern(ps);
+#else
         fa_cp_update(ps);
         if (ps.major || do_restart) {  // compute_primal/dual_fixed_point_error, :1714-1735
             if (primal) {
@@ -963,6 +1002,7 @@ PolishOut polish_loop(S &ps, const S &ori, bool primal)
             }
         }
         halpern(ps);
+#endif
         ps.inner++;
         ps.total++;
     }
@@ -1043,7 +1083,7 @@ void feasibility_polish(S &s, double &polish_time, int &polish_iters)
     polish_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
-#elif defined(ORACLE_PLANTED_FA_CP)
+#if defined(ORACLE_PLANTED_FA_CP)
 
 /* ---------------- planted FA_CP restart (QP/PLANTED_FA_CP/src/fa_cp_core_op.cu) ---------------- */
 
@@ -1120,7 +1160,7 @@ void perform_restart(S &s)
     s.last_trial_fp = INFINITY;
 }
 
-#else  // ORACLE_LIN_CP_AL
+#elif defined(ORACLE_LIN_CP_AL)
 
 /* ---------------- linearized CP-AL (QP/LIN_CP_AL/src/cp_al_core_op.cu) ---------------- */
 
@@ -1735,10 +1775,12 @@ ORACLE_API cp_al_result_t *optimize(const cp_al_parameters_t *params, const qp_p
         s.reason = TERMINATION_REASON_ITERATION_LIMIT;
         compute_residual(s, s.P.optimality_norm);
     }
+    double polish_time = 0.0;
+    int polish_iters = 0;
     if (s.P.feasibility_polishing && s.reason != TERMINATION_REASON_DUAL_INFEASIBLE &&
-        s.reason != TERMINATION_REASON_PRIMAL_INFEASIBLE)
-        fprintf(stderr, "[lin_cp_al oracle] feasibility polishing is not restated; skipped.\n");
-    return result_from_state(s, prob, 0.0, 0);
+        s.reason != TERMINATION_REASON_PRIMAL_INFEASIBLE)  // LIN_CP_AL solver.cu:102-106
+        feasibility_polish(s, polish_time, polish_iters);
+    return result_from_state(s, prob, polish_time, polish_iters);
 }
 #else
 ORACLE_API fa_cp_result_t *optimize(const fa_cp_parameters_t *params, const qp_problem_t *prob)
@@ -1792,12 +1834,8 @@ ORACLE_API fa_cp_result_t *optimize(const fa_cp_parameters_t *params, const qp_p
     double polish_time = 0.0;
     int polish_iters = 0;
     if (s.P.feasibility_polishing && s.reason != TERMINATION_REASON_DUAL_INFEASIBLE &&
-        s.reason != TERMINATION_REASON_PRIMAL_INFEASIBLE)  // solver.cu:156-160
-#ifdef ORACLE_PLANTED_FA_CP
-        fprintf(stderr, "[planted fa_cp oracle] feasibility polishing is not restated; skipped.\n");
-#else
+        s.reason != TERMINATION_REASON_PRIMAL_INFEASIBLE)  // solver.cu:156-160 (planted :187-190)
         feasibility_polish(s, polish_time, polish_iters);
-#endif
     return result_from_state(s, prob, polish_time, polish_iters);
 }
 #endif

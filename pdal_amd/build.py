@@ -115,6 +115,9 @@ def build_lin_cp_al() -> Path:
     tmp = LIB / "librhr_lin_cp_al.so.tmp"
     _run([_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-Bsymbolic", *objs, "-lz", "-o", tmp])
     os.replace(tmp, out)
+    # its runner (QP/LIN_CP_AL's rhr_lin_cp_al) linked against the shared library
+    _run(["g++", "-O2", "-std=c++17", "-DPDAL_LIN_CP_AL", f"-I{INCLUDE}", CSRC / "qp" / "variant_cli.cpp",
+          f"-L{LIB}", "-l:librhr_lin_cp_al.so", f"-Wl,-rpath,$ORIGIN", "-o", LIB / "rhr_lin_cp_al"])
     return out
 
 
@@ -127,6 +130,9 @@ def build_planted_fa_cp() -> Path:
     tmp = LIB / "librhr_fa_cp.so.tmp"
     _run([_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-Bsymbolic", *objs, "-lz", "-o", tmp])
     os.replace(tmp, out)
+    # its runner (QP/PLANTED_FA_CP's rhr_fa_cp) linked against the shared library
+    _run(["g++", "-O2", "-std=c++17", "-DPDAL_PLANTED_FA_CP", f"-I{INCLUDE}", CSRC / "qp" / "variant_cli.cpp",
+          f"-L{LIB}", "-l:librhr_fa_cp.so", f"-Wl,-rpath,$ORIGIN", "-o", LIB / "rhr_fa_cp"])
     return out
 
 

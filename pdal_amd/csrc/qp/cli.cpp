@@ -1,14 +1,14 @@
 // fa_cp: the QP runner command line, linked against libfa_cp.so.
 //
 // Reference: QP/FA_CP/src/cli.c:260-455 (getopt flags, defaults from
-// set_default_parameters, read_mps_file -> optimize), :120-166 (the
-// `_summary.txt` keys), :101-118 (`%.10g` solution files), :38-78 (sigma-rule
-// names), :80-99 (instance name = basename up to the first dot).  The
+// set_default_parameters, read_mps_file -> optimize), :38-78 (sigma-rule
+// names); the output files are written by cli_io.hpp.  The
 // trajectory CSV is not written by this build (the flag is accepted and
 // reported).
 #include "fa_cp.h"
 #include "fa_cp_mps_parser.h"
 #include "pdal_qp.h"
+#include "cli_io.hpp"
 
 #include <getopt.h>
 
@@ -59,65 +59,6 @@ const char *rule_name(sigma_update_rule_t rule)
     for (const RuleName &r : kRules)
         if (r.rule == rule) return r.name;
     return "unknown";
-}
-
-std::string instance_name(const char *path)
-{
-    std::string s(path);
-    const size_t slash = s.find_last_of('/');
-    if (slash != std::string::npos) s = s.substr(slash + 1);
-    const size_t dot = s.find('.');
-    if (dot != std::string::npos) s = s.substr(0, dot);
-    return s;
-}
-
-std::string out_path(const char *dir, const std::string &name, const char *suffix)
-{
-    return std::string(dir) + "/" + name + suffix;
-}
-
-void save_solution(const double *v, int n, const char *dir, const std::string &name, const char *suffix)
-{
-    if (!v) return;
-    const std::string path = out_path(dir, name, suffix);
-    FILE *f = fopen(path.c_str(), "w");
-    if (!f) {
-        perror("Error opening solution file");
-        return;
-    }
-    for (int i = 0; i < n; ++i) fprintf(f, "%.10g\n", v[i]);
-    fclose(f);
-}
-
-void save_summary(const fa_cp_result_t *r, const char *dir, const std::string &name)
-{
-    const std::string path = out_path(dir, name, "_summary.txt");
-    FILE *f = fopen(path.c_str(), "w");
-    if (!f) {
-        perror("Error opening summary file");
-        return;
-    }
-    fprintf(f, "Termination Reason: %s\n", pdal_fa_cp_termination_reason_to_string(r->termination_reason));
-    fprintf(f, "Runtime (sec): %e\n", r->cumulative_time_sec);
-    fprintf(f, "Iterations Count: %d\n", r->total_count);
-    if (r->total_inner_count > 0) fprintf(f, "Inner Iterations Count: %d\n", r->total_inner_count);
-    fprintf(f, "Primal Objective Value: %e\n", r->primal_objective_value);
-    fprintf(f, "Dual Objective Value: %e\n", r->dual_objective_value);
-    fprintf(f, "Relative Primal Residual: %e\n", r->relative_primal_residual);
-    fprintf(f, "Relative Dual Residual: %e\n", r->relative_dual_residual);
-    fprintf(f, "Absolute Objective Gap: %e\n", r->objective_gap);
-    fprintf(f, "Relative Objective Gap: %e\n", r->relative_objective_gap);
-    fprintf(f, "Final AL Sigma: %e\n", r->final_al_sigma);
-    fprintf(f, "Sigma Update Rule: %s\n", rule_name(r->sigma_update_rule));
-    fprintf(f, "Sigma Update Count: %d\n", r->sigma_update_count);
-    fprintf(f, "Rows: %d\n", r->num_constraints);
-    fprintf(f, "Columns: %d\n", r->num_variables);
-    fprintf(f, "Nonzeros: %d\n", r->num_nonzeros);
-    if (r->feasibility_polishing_time > 0.0) {
-        fprintf(f, "Feasibility Polishing Time (sec): %e\n", r->feasibility_polishing_time);
-        fprintf(f, "Feasibility Polishing Iteration Count: %d\n", r->feasibility_iteration);
-    }
-    fclose(f);
 }
 
 void usage(const char *prog)
@@ -262,7 +203,7 @@ int main(int argc, char *argv[])
     }
     const char *file = argv[optind];
     const char *outdir = argv[optind + 1];
-    const std::string name = instance_name(file);
+    const std::string name = pdal::qpcli::instance_name(file);
     qp_problem_t *problem = read_mps_file(file);
     if (!problem) {
         fprintf(stderr, "Failed to read or parse the file.\n");
@@ -272,9 +213,14 @@ int main(int argc, char *argv[])
     if (!r) {
         fprintf(stderr, "Solver failed.\n");
     } else {
-        save_summary(r, outdir, name);
-        save_solution(r->primal_solution, problem->num_variables, outdir, name, "_primal_solution.txt");
-        save_solution(r->dual_solution, problem->num_constraints, outdir, name, "_dual_solution.txt");
+        pdal::qpcli::save_summary(r, pdal_fa_cp_termination_reason_to_string(r->termination_reason), outdir, name,
+                                  [r](FILE *f) {
+                                      fprintf(f, "Final AL Sigma: %e\n", r->final_al_sigma);
+                                      fprintf(f, "Sigma Update Rule: %s\n", rule_name(r->sigma_update_rule));
+                                      fprintf(f, "Sigma Update Count: %d\n", r->sigma_update_count);
+                                  });
+        pdal::qpcli::save_solution(r->primal_solution, problem->num_variables, outdir, name, "_primal_solution.txt");
+        pdal::qpcli::save_solution(r->dual_solution, problem->num_constraints, outdir, name, "_dual_solution.txt");
         fa_cp_result_free(r);
     }
     qp_problem_free(problem);

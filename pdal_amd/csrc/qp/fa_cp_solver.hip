@@ -24,13 +24,15 @@
 // (cp_al_core_op.cu:609-691), update (:496-575, one projected gradient step),
 // relaxed Halpern step (:577-585), per-iteration fixed-point error (:693-770),
 // restart test and restart (utils.cu:451-463, :595-607), main loop
-// (solver.cu:34-114).  Its feasibility polishing is not built.
+// (solver.cu:34-114); feasibility polishing runs FA_CP's phases with this
+// iteration (cp_al_core_op.cu:839-1375).
 //
 // Built with -DPDAL_PLANTED_FA_CP it is RHR-FA-CP, the planted-KKT FA_CP
 // (QP/PLANTED_FA_CP): this iteration with equality AL kernels, the AL branch
 // only when al_sigma != 0, its inner-loop rule, the fixed-point error without
 // the clamp at zero, and its own AL sigma rule at restarts
-// (fa_cp_core_op.cu:610-747) instead of FA_CP's; feasibility polishing not built.
+// (fa_cp_core_op.cu:610-747) instead of FA_CP's; its feasibility polishing is
+// FA_CP's code (fa_cp_core_op.cu:1009-1502).
 #include "fa_cp_internal.hpp"
 
 #include "../common/csr_host.hpp"
@@ -1429,8 +1431,8 @@ result_t *Solver::run_linearized()
         display_stats();
     }
     if (P_.feasibility_polishing && reason_ != TERMINATION_REASON_DUAL_INFEASIBLE &&
-        reason_ != TERMINATION_REASON_PRIMAL_INFEASIBLE)
-        fprintf(stderr, "[rhr_lin_cp_al] feasibility polishing is not built for the linearized solver; skipped.\n");
+        reason_ != TERMINATION_REASON_PRIMAL_INFEASIBLE)  // QP/LIN_CP_AL/src/solver.cu:102-106
+        feasibility_polish();
     return make_result();
 }
 
@@ -1491,19 +1493,17 @@ result_t *Solver::run()
         display_stats();
     }
     if (P_.feasibility_polishing && reason_ != TERMINATION_REASON_DUAL_INFEASIBLE &&
-        reason_ != TERMINATION_REASON_PRIMAL_INFEASIBLE)  // solver.cu:156-160
-#ifdef PDAL_PLANTED_FA_CP
-        fprintf(stderr, "[rhr_fa_cp] feasibility polishing is not built for the planted-KKT solver; skipped.\n");
-#else
+        reason_ != TERMINATION_REASON_PRIMAL_INFEASIBLE)  // solver.cu:156-160 (planted :187-190)
         feasibility_polish();
-#endif
     return make_result();
 #endif
 }
 
-#if !defined(PDAL_LIN_CP_AL) && !defined(PDAL_PLANTED_FA_CP)
 // ---------------------------------------------------------------------------
-// Feasibility polishing (fa_cp_core_op.cu:1242-1735, utils.cu:1360-1575): a
+// Feasibility polishing (fa_cp_core_op.cu:1242-1735, utils.cu:1360-1575; the
+// planted solver's QP/PLANTED_FA_CP/src/fa_cp_core_op.cu:1009-1502 is the same
+// code, and the linearized solver's QP/LIN_CP_AL/src/cp_al_core_op.cu:839-1375
+// differs only in the iteration it runs, see polish_phase): a
 // primal phase with c = 0 (Q kept) and a dual phase with every finite bound 0,
 // each the main loop's iteration with the primal weight fixed, its own
 // fixed-point error (squared distance of the reflected from the current point,
@@ -1600,7 +1600,11 @@ Solver::PolishOut Solver::polish_phase(bool primal, double pw0, const PolishRefs
                 printf("%6d %.1e | %8.1e |    %.1e   |   %.1e   \n", total_, out.time, out.obj, out.abs, out.rel);
         }
         if (consider) {  // the restart test runs after a stop as well, as in the reference loop
+#ifdef PDAL_LIN_CP_AL
+            do_restart = lin_should_restart();
+#else
             do_restart = should_restart();
+#endif
             if (do_restart) {  // perform_primal_restart / perform_dual_restart, :1668-1690
                 copy(primal ? x0_.get() : y0_.get(), primal ? xT_.get() : yT_.get(), primal ? n_ : m_);
                 copy(primal ? x_.get() : y_.get(), primal ? xT_.get() : yT_.get(), primal ? n_ : m_);
@@ -1609,6 +1613,16 @@ Solver::PolishOut Solver::polish_phase(bool primal, double pw0, const PolishRefs
             }
         }
         major_ = ((total_ + 1) % K) == 0;
+#ifdef PDAL_LIN_CP_AL
+        // cp_al_update, then halpern_update with the reflection coefficient.  The
+        // phase's restart test (should_do_adaptive_restart, LIN_CP_AL utils.cu:451-463)
+        // needs a finite reference error, which a phase starts without
+        // (initialize_*_feas_polish_state set it to infinity) and only a restart sets:
+        // it never fires, so the phase's fixed-point error (:1331-1375), read by
+        // nothing else, is not computed.
+        lin_update();
+        lin_halpern();
+#else
         const bool need_fp = major_ || do_restart;
         update(!need_fp);
         if (need_fp) {
@@ -1619,6 +1633,7 @@ Solver::PolishOut Solver::polish_phase(bool primal, double pw0, const PolishRefs
             }
             halpern();
         }
+#endif
         inner_++;
         total_++;
     }
@@ -1748,7 +1763,6 @@ void Solver::feasibility_polish()
     fflush(stdout);
     polish_time_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
 }
-#endif  // polishing
 
 }  // namespace
 
