@@ -4,8 +4,9 @@ Output: oracle/_build/libcp_al_oracle.so, libfa_cp_oracle.so,
 libfa_cp_oracle_rev.so (the FA_CP oracle summing in reverse order, the
 summation-order noise floor of a trajectory) and liblin_cp_al_oracle.so (the
 linearized CP-AL QP solver, the FA_CP source built with -DORACLE_LIN_CP_AL) and
-libplanted_fa_cp_oracle.so (the planted-KKT FA_CP, -DORACLE_PLANTED_FA_CP; a
-reverse-summing build of it too).  The oracle exports the same C names
+libplanted_fa_cp_oracle.so (the planted-KKT FA_CP, -DORACLE_PLANTED_FA_CP) and
+libplanted_cp_al_oracle.so (the planted-KKT CP-AL, -DORACLE_PLANTED_CP_AL), each
+with a reverse-summing build.  The oracle exports the same C names
 as the product library; it is built with hidden visibility and -Bsymbolic so a
 process can load both without the two binding to each other's symbols.
 
@@ -44,6 +45,8 @@ def build() -> Path:
     _one("fa_cp_oracle.cpp", "liblin_cp_al_oracle.so", ["-DORACLE_LIN_CP_AL"])
     _one("fa_cp_oracle.cpp", "libplanted_fa_cp_oracle.so", ["-DORACLE_PLANTED_FA_CP"])
     _one("fa_cp_oracle.cpp", "libplanted_fa_cp_oracle_rev.so", ["-DORACLE_PLANTED_FA_CP", "-DFA_CP_ORACLE_REVERSE_SUMS"])
+    _one("fa_cp_oracle.cpp", "libplanted_cp_al_oracle.so", ["-DORACLE_PLANTED_CP_AL"])
+    _one("fa_cp_oracle.cpp", "libplanted_cp_al_oracle_rev.so", ["-DORACLE_PLANTED_CP_AL", "-DFA_CP_ORACLE_REVERSE_SUMS"])
     return _one("cp_al_oracle.cpp", "libcp_al_oracle.so")
 
 

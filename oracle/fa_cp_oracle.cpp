@@ -49,13 +49,29 @@
  * (:305-345), a fixed-point error without the clamp at zero (:893-940), and
  * its AL sigma update at restarts (:610-747, off unless adaptive_al_sigma)
  * in place of FA_CP's sigma rules; its feasibility polishing is FA_CP's code.
+ *
+ * Built with -DORACLE_PLANTED_CP_AL it is the oracle of RHR-CP-AL, the
+ * planted-KKT CP-AL solver (QP/PLANTED_CP_AL): FA_CP's main loop and restart
+ * test (solver.cu:111-204, utils.cu:451-480) over the linearized solver's
+ * sigma / lambda steps with its own set-up (lambda = 2.001 ||A||^2, sigma =
+ * sigma_base / 2; cp_al_core_op.cu:805-858), update (equality rows; LP, diagonal
+ * Q in closed form, FA_CP's BB inner solve for the other Q types; T-outputs in
+ * major-kernel iterations only; :531-635), FA_CP's Halpern step, fixed-point
+ * error on the reflection (:860-935), and at restarts a sigma controller (AL
+ * branch) or the primal-weight PID (:655-803); its polishing is the linearized
+ * solver's code around this iteration.
  */
 #if defined(ORACLE_PLANTED_FA_CP)
 #include "planted_fa_cp.h"
 typedef fa_cp_parameters_t params_t;
 typedef fa_cp_result_t result_t;
-#elif defined(ORACLE_LIN_CP_AL)
+#elif defined(ORACLE_LIN_CP_AL) || defined(ORACLE_PLANTED_CP_AL)
+#define ORACLE_CP_AL_TYPES 1
+#if defined(ORACLE_LIN_CP_AL)
 #include "lin_cp_al.h"
+#else
+#include "planted_cp_al.h"
+#endif
 typedef cp_al_parameters_t params_t;
 typedef cp_al_result_t result_t;
 // the shared code names the Q types by their FA_CP names
@@ -227,6 +243,8 @@ struct S {
     double step = 0.0, pw = 1.0, best_pw = 1.0, pw_err_sum = 0.0, pw_last_err = 0.0, best_gap = INFINITY;
     // linearized CP-AL: the fixed penalty sigma and its interval, lambda of tau = 1 / (sigma lambda)
     double sigma = 0.0, sigma_base = 1.0, sigma_min = 0.0, sigma_max = 0.0, lambda_est = 1.0;
+    // planted CP-AL: the sigma controller's best sigma and its restart metric and residual gap
+    double best_sigma = 0.0, best_term_metric = INFINITY, best_sigma_gap = INFINITY;
     // planted FA_CP: iteration of the last AL sigma change, the inner tolerance a change restores
     int last_sigma_update_iter = 0;
     double inner_tol0 = 0.0;
@@ -599,8 +617,8 @@ void compute_residual(S &s, norm_type_t norm)
     if (s.is_lp) {  // compute_lp_residual_kernel :705-733
         for (int j = 0; j < s.n; ++j) s.dres[j] = (s.c[j] - s.dual_prod[j] - s.dslack[j]) * s.D[j];
     } else {  // compute_qp_residual_kernel :735-770
-#ifdef ORACLE_LIN_CP_AL
-        // LIN_CP_AL utils.cu:855-858: the AL step 1 / (sigma lambda)
+#ifdef ORACLE_CP_AL_TYPES
+        // LIN_CP_AL utils.cu:855-858 (PLANTED_CP_AL :871-874): the AL step 1 / (sigma lambda)
         const double tr = s.use_al ? 1.0 / std::fmax(s.sigma * s.lambda_est, 1e-16) : s.step / s.pw;
 #else
         const double tr = s.step / s.pw;
@@ -728,7 +746,7 @@ bool should_restart(S &s)
     return r;
 }
 
-#if !defined(ORACLE_LIN_CP_AL) && !defined(ORACLE_PLANTED_FA_CP)
+#if !defined(ORACLE_CP_AL_TYPES) && !defined(ORACLE_PLANTED_FA_CP)
 void set_sigma_active(S &s, double sigma, double zero_thr)
 {   // :667-676
     if (!std::isfinite(sigma) || sigma <= zero_thr) {
@@ -885,10 +903,12 @@ void perform_restart(S &s)
    linearized solver's QP/LIN_CP_AL/src/cp_al_core_op.cu:839-1375 runs its own
    iteration, restart test and fixed-point errors in the same loop) ---------------- */
 
-#ifdef ORACLE_LIN_CP_AL
+#if defined(ORACLE_LIN_CP_AL)
 void lin_update(S &s);
 void lin_halpern(S &s);
 bool lin_should_restart(const S &s);
+#elif defined(ORACLE_PLANTED_CP_AL)
+void pcal_update(S &s);
 #endif
 
 struct PolishOut {
@@ -960,9 +980,14 @@ PolishOut polish_loop(S &ps, const S &ori, bool primal)
             }
         }
         ps.major = ((ps.total + 1) % K) == 0;
+#ifdef ORACLE_CP_AL_TYPES
 #ifdef ORACLE_LIN_CP_AL
         lin_update(ps);
-        if (ps.major || do_restart) {  // compute_primal/dual_fixed_point_error, LIN_CP_AL :1331-1375
+#else
+        pcal_update(ps);
+#endif
+        // compute_primal/dual_fixed_point_error, LIN_CP_AL :1331-1375 (PLANTED_CP_AL :1495-1539)
+        if (ps.major || do_restart) {
             if (primal) {
                 for (int i = 0; i < ps.n; ++i) ps.dx[i] = ps.xR[i] - ps.x[i];
                 const double pn = nrm2(ps.dx);
@@ -983,7 +1008,11 @@ PolishOut polish_loop(S &ps, const S &ori, bool primal)
                 do_restart = false;
             }
         }
+#ifdef ORACLE_LIN_CP_AL
         lin_halpern(ps);
+#else
+        halpern(ps);
+#endif
 #else
         fa_cp_update(ps);
         if (ps.major || do_restart) {  // compute_primal/dual_fixed_point_error, :1714-1735
@@ -1300,7 +1329,226 @@ void lin_restart(S &s)
     s.last_trial_fp = INFINITY;
 }
 
-#endif  // ORACLE_LIN_CP_AL
+#elif defined(ORACLE_PLANTED_CP_AL)
+
+/* ---------------- planted-KKT CP-AL (QP/PLANTED_CP_AL/src/cp_al_core_op.cu) ---------------- */
+
+double pcal_tau(const S &s)
+{   // cpal_effective_primal_step_size, :255-265: 1 / (sigma lambda), nonconvexity-adjusted
+    double t = 1.0 / std::fmax(s.sigma * s.lambda_est, 1e-16);
+    if (s.nonconvexity < 0) {
+        t = std::fmax(t, -1.01 * std::fmin(0.0, s.nonconvexity));
+        t /= 2;
+    }
+    return t;
+}
+
+void pcal_init_steps(S &s)
+{   // initialize_step_size_and_primal_weight, :805-858
+    if (s.use_al) {
+        if (s.A.nnz() == 0) {
+            s.lambda_est = 1.0;
+        } else {
+            const double sv = max_singular_value(s);
+            s.lambda_est = 2.001 * sv * sv;
+        }
+        s.sigma_base = 1.0 / std::sqrt(std::fmax(s.lambda_est, 1e-16));
+        s.sigma_min = 1e-4 * s.sigma_base;
+        s.sigma_max = 100.0 * s.sigma_base;
+        s.sigma = s.P.sigma_init > 0.0 ? s.P.sigma_init : 0.5 * s.sigma_base;
+        s.sigma = std::fmin(std::fmax(s.sigma, s.sigma_min), s.sigma_max);
+        s.best_sigma = s.sigma;
+        s.best_sigma_gap = INFINITY;
+        s.best_term_metric = INFINITY;
+        s.step = pcal_tau(s);
+    } else {
+        s.step = s.A.nnz() == 0 ? 1.0 : 0.998 / max_singular_value(s);
+    }
+    s.pw = s.P.bound_objective_rescaling ? 1.0 : (s.c_norm + 1.0) / (s.b_norm + 1.0);
+    s.best_pw = s.pw;
+}
+
+void pcal_update(S &s)
+{   // cp_al_update, :531-635: the T-outputs (and the LP dual slack) only in major-kernel
+    // iterations; equality rows (right-hand side l) in both branches
+    double tau, dstep;
+    if (s.use_al) {
+        tau = pcal_tau(s);
+        dstep = std::fmax(s.sigma, 1e-16);
+        s.step = tau;
+    } else {
+        tau = s.step / s.pw;
+        dstep = s.step * s.pw;
+        if (s.nonconvexity < 0) {
+            tau = std::fmax(tau, -1.01 * std::fmin(0.0, s.nonconvexity));
+            tau /= 2;
+        }
+    }
+    const bool mk = major_kernel(s);
+    if (s.use_al) {  // compute_al_effective_dual_product, :285-312 (kernel :149-158)
+        spmv(s.A, s.x.data(), s.prim_prod.data());
+        for (int i = 0; i < s.m; ++i) s.al_u[i] = s.y[i] - s.sigma * (s.prim_prod[i] - s.l[i]);
+        spmv(s.At, s.al_u.data(), s.dual_prod.data());
+    } else {
+        spmv(s.At, s.y.data(), s.dual_prod.data());
+    }
+    switch (s.qtype) {
+    case FA_CP_NON_Q:  // lp_primal_update, :314-333 (kernels :59-87)
+        for (int j = 0; j < s.n; ++j) {
+            const double xj = s.x[j];
+            const double t = xj - tau * (s.c[j] - s.dual_prod[j]);
+            const double p = std::fmax(s.lb[j], std::fmin(t, s.ub[j]));
+            s.xR[j] = 2.0 * p - xj;
+            if (mk) {
+                s.xT[j] = p;
+                s.dslack[j] = (p - t) / tau;
+            }
+        }
+        break;
+    case FA_CP_DIAG_Q:  // diag_q_primal_update, :335-356 (kernels :89-120)
+        for (int j = 0; j < s.n; ++j) {
+            const double xj = s.x[j];
+            const double t = (xj - tau * (s.c[j] - s.dual_prod[j])) / (1.0 + tau * s.qdiag[j]);
+            const double p = std::fmax(s.lb[j], std::fmin(t, s.ub[j]));
+            s.xR[j] = 2.0 * p - xj;
+            if (mk) s.xT[j] = p;
+        }
+        break;
+    default: {  // primal_BB_step_size_update, :454-503: FA_CP's BB with the product fixed
+        const bool al = s.use_al;
+        s.use_al = false;
+        primal_bb(s, tau, 0.0);
+        s.use_al = al;
+        break;
+    }
+    }
+    spmv(s.A, s.xR.data(), s.prim_prod.data());
+    for (int i = 0; i < s.m; ++i) {  // kernels :122-147, :160-185
+        const double nd = s.y[i] + dstep * (s.l[i] - s.prim_prod[i]);
+        if (mk) s.yT[i] = nd;
+        s.yR[i] = 2.0 * nd - s.y[i];
+    }
+}
+
+void pcal_fixed_point_error(S &s)
+{   // compute_fixed_point_error, :860-935 (delta = reflected - current)
+    for (int i = 0; i < s.n; ++i) s.dx[i] = s.xR[i] - s.x[i];
+    for (int i = 0; i < s.m; ++i) s.dy[i] = s.yR[i] - s.y[i];
+    const double dn = nrm2(s.dy), pn = nrm2(s.dx);
+    if (s.use_al) {
+        spmv(s.A, s.dx.data(), s.prim_prod.data());
+        const double an = nrm2(s.prim_prod);
+        const double inter = dot(s.prim_prod, s.dy);
+        const double fp2 = s.sigma * (s.lambda_est * pn * pn - an * an) + 2.0 * inter + dn * dn / s.sigma;
+        s.fp = std::sqrt(std::fmax(0.0, fp2));
+    } else {
+        spmv(s.At, s.dy.data(), s.dual_prod.data());
+        const double movement = pn * pn * s.pw + dn * dn / s.pw;
+        const double cross = dot(s.dual_prod, s.dx);
+        s.fp = std::sqrt(movement + 2 * s.step * cross);  // no clamp at zero (:921)
+    }
+    if (s.has_inner) {
+        const double st = s.use_al ? s.step : s.step / s.pw;
+        s.inner_tol = std::fmin(s.inner_tol, std::fmax(0.0005 * pn / st, 1e-9));
+    }
+}
+
+void pcal_reset_sigma(S &s)
+{   // reset_sigma_controller, :271-283
+    double sg = std::fmax(s.best_sigma, 1e-16);
+    if (s.sigma_min > 0.0) sg = std::fmax(sg, s.sigma_min);
+    if (s.sigma_max > 0.0) sg = std::fmin(sg, s.sigma_max);
+    s.sigma = sg;
+    s.step = pcal_tau(s);
+}
+
+void pcal_restart(S &s)
+{   // perform_restart, :655-803: the AL branch's sigma controller (log-damped step toward
+    // ||dy|| / sqrt(lambda ||dx||^2 - ||A dx||^2), clipped to [0.6, 1.7] per restart, rolled
+    // back to the best sigma when the restart metric worsens), else the primal-weight PID;
+    // then the anchors and iterates move to the T-outputs
+    for (int i = 0; i < s.n; ++i) s.dx[i] = s.xT[i] - s.x0[i];
+    for (int i = 0; i < s.m; ++i) s.dy[i] = s.yT[i] - s.y0[i];
+    const double pd = nrm2(s.dx), dd = nrm2(s.dy);
+    if (s.use_al) {
+        spmv(s.A, s.dx.data(), s.prim_prod.data());
+        const double ax = nrm2(s.prim_prod);
+        double cur = s.fp;
+        const double dxm = s.lambda_est * pd * pd - ax * ax;
+        const double ratio = s.rel_d / std::fmax(s.rel_p, 1e-300);
+        const double old = std::fmax(s.sigma, 1e-16);
+        bool rolled = false;
+        if (std::isfinite(cur) && cur < s.best_term_metric) {
+            s.best_term_metric = cur;
+            s.best_sigma = old;
+            s.best_sigma_gap = std::fabs(std::log10(std::fmax(s.rel_d, 1e-300) / std::fmax(s.rel_p, 1e-300)));
+        }
+        if (!std::isfinite(cur)) {
+            pcal_reset_sigma(s);
+            rolled = true;
+            cur = INFINITY;
+        } else if (std::isfinite(s.best_term_metric) && cur > 1.05 * s.best_term_metric) {
+            pcal_reset_sigma(s);
+            rolled = true;
+        } else {
+            bool valid = false;
+            double est = old;
+            if (dd > 1e-16 && dd < 1e12 && dxm > 1e-16 && dxm < 1e24 && std::isfinite(s.sigma) && ratio > 1e-8 &&
+                ratio < 1e8) {
+                est = dd / std::sqrt(dxm);
+                valid = std::isfinite(est) && est > 1e-16;
+            }
+            if (valid) {
+                const double target = old * std::exp(0.5 * std::log(est / old));
+                if (std::isfinite(target) && target > 1e-16) {
+                    const double r = std::fmax(0.60, std::fmin(target / old, 1.70));
+                    s.sigma = old * r;
+                }
+            }
+        }
+        if (!std::isfinite(s.sigma) || s.sigma <= 1e-16 || s.sigma >= 1e16) {
+            pcal_reset_sigma(s);
+            rolled = true;
+        } else {
+            s.sigma = std::fmin(std::fmax(s.sigma, s.sigma_min), s.sigma_max);
+            s.step = pcal_tau(s);
+        }
+        const double g = std::fabs(std::log10(std::fmax(s.rel_d, 1e-300) / std::fmax(s.rel_p, 1e-300)));
+        if (g < s.best_sigma_gap) s.best_sigma_gap = g;
+        if (rolled) {
+            s.sigma = s.best_sigma;
+            s.step = pcal_tau(s);
+        }
+    } else {
+        const double ratio = s.rel_d / s.rel_p;
+        const auto &rp = s.P.restart_params;
+        if (pd > 1e-16 && dd > 1e-16 && pd < 1e12 && dd < 1e12 && ratio > 1e-8 && ratio < 1e8) {
+            const double err = std::log(dd) - std::log(pd) - std::log(s.pw);
+            s.pw_err_sum *= rp.i_smooth;
+            s.pw_err_sum += err;
+            const double de = err - s.pw_last_err;
+            s.pw *= std::exp(rp.k_p * err + rp.k_i * s.pw_err_sum + rp.k_d * de);
+            s.pw_last_err = err;
+        } else {
+            s.pw = s.best_pw;
+            s.pw_err_sum = 0.0;
+            s.pw_last_err = 0.0;
+        }
+        const double g = std::fabs(std::log10(s.rel_d / s.rel_p));
+        if (g < s.best_gap) {
+            s.best_gap = g;
+            s.best_pw = s.pw;
+        }
+    }
+    s.x0 = s.xT;
+    s.x = s.xT;
+    s.y0 = s.yT;
+    s.y = s.yT;
+    s.inner = 0;
+    s.last_trial_fp = INFINITY;
+}
+
+#endif  // ORACLE_LIN_CP_AL / ORACLE_PLANTED_CP_AL
 
 void init_state(S &s, const qp_problem_t *w)
 {   // solver_state.cu:494-877 after rescale_problem (preconditioner.c)
@@ -1327,7 +1575,7 @@ void init_state(S &s, const qp_problem_t *w)
     // planted solver_state.cu:498-504: the AL branch only with a nonzero sigma
     s.use_al = s.P.use_al_qp && s.P.al_sigma != 0.0;
     s.al_sigma = s.use_al ? s.P.al_sigma : 0.0;
-#elif !defined(ORACLE_LIN_CP_AL)
+#elif !defined(ORACLE_CP_AL_TYPES)
     s.al_sigma = s.P.al_sigma;
     s.best_al_sigma = s.P.al_sigma;
 #endif
@@ -1392,9 +1640,9 @@ void init_state(S &s, const qp_problem_t *w)
         s.inner_limit = 1000;
         s.inner_tol = s.inner_tol0 = 1e-3;
     }
-#elif defined(ORACLE_LIN_CP_AL)
-    // LIN_CP_AL solver_state.cu:305-340: an inner loop only for the non-diagonal Q types
-    // (the linearized update never runs it)
+#elif defined(ORACLE_CP_AL_TYPES)
+    // LIN_CP_AL / PLANTED_CP_AL solver_state.cu:305-340: an inner loop only for the
+    // non-diagonal Q types (the linearized update never runs it; the planted one's BB does)
     const bool simple = s.qtype == FA_CP_NON_Q || s.qtype == FA_CP_DIAG_Q;
     s.has_inner = !simple;
     if (!simple) {
@@ -1426,6 +1674,8 @@ void init_state(S &s, const qp_problem_t *w)
 ORACLE_API void planted_fa_cp_oracle_reset_rng(void)
 #elif defined(ORACLE_LIN_CP_AL)
 ORACLE_API void lin_cp_al_oracle_reset_rng(void)
+#elif defined(ORACLE_PLANTED_CP_AL)
+ORACLE_API void planted_cp_al_oracle_reset_rng(void)
 #else
 ORACLE_API void fa_cp_oracle_reset_rng(void)
 #endif
@@ -1449,6 +1699,38 @@ ORACLE_API void set_default_parameters(cp_al_parameters_t *p)  // LIN_CP_AL util
     p->termination_evaluation_frequency = 200;
     p->feasibility_polishing = false;
     p->reflection_coefficient = 0.0;
+    p->sv_max_iter = 5000;
+    p->sv_tol = 1e-4;
+    p->termination_criteria.eps_optimal_relative = 1e-4;
+    p->termination_criteria.eps_feasible_relative = 1e-4;
+    p->termination_criteria.eps_infeasible = 1e-10;
+    p->termination_criteria.time_sec_limit = 3600.0;
+    p->termination_criteria.iteration_limit = INT32_MAX;
+    p->termination_criteria.eps_feas_polish_relative = 1e-6;
+    p->restart_params.artificial_restart_threshold = 0.36;
+    p->restart_params.sufficient_reduction_for_restart = 0.2;
+    p->restart_params.necessary_reduction_for_restart = 0.8;
+    p->restart_params.k_p = 0.99;
+    p->restart_params.k_i = 0.01;
+    p->restart_params.k_d = 0.0;
+    p->restart_params.i_smooth = 0.3;
+    p->optimality_norm = NORM_TYPE_L_INF;
+}
+#elif defined(ORACLE_PLANTED_CP_AL)
+ORACLE_API void set_default_parameters(cp_al_parameters_t *p)  // PLANTED_CP_AL utils.cu:481-513
+{
+    memset(p, 0, sizeof(*p));
+    p->l_inf_ruiz_iterations = 10;
+    p->has_pock_chambolle_alpha = true;
+    p->pock_chambolle_alpha = 1.0;
+    p->bound_objective_rescaling = true;
+    p->use_al_qp = true;
+    p->sigma_init = 0.0;
+    p->trajectory_csv = nullptr;
+    p->verbose = 1;
+    p->termination_evaluation_frequency = 200;
+    p->feasibility_polishing = false;
+    p->reflection_coefficient = 1.0;
     p->sv_max_iter = 5000;
     p->sv_tol = 1e-4;
     p->termination_criteria.eps_optimal_relative = 1e-4;
@@ -1724,7 +2006,7 @@ ORACLE_API void set_start_values(qp_problem_t *p, const double *primal, const do
     }
 }
 
-#ifdef ORACLE_LIN_CP_AL
+#ifdef ORACLE_CP_AL_TYPES
 ORACLE_API void cp_al_result_free(cp_al_result_t *r)
 #else
 ORACLE_API void fa_cp_result_free(fa_cp_result_t *r)
@@ -1783,18 +2065,23 @@ ORACLE_API cp_al_result_t *optimize(const cp_al_parameters_t *params, const qp_p
     return result_from_state(s, prob, polish_time, polish_iters);
 }
 #else
-ORACLE_API fa_cp_result_t *optimize(const fa_cp_parameters_t *params, const qp_problem_t *prob)
-{   // solver.cu:74-168
+ORACLE_API result_t *optimize(const params_t *params, const qp_problem_t *prob)
+{   // solver.cu:74-168 (PLANTED_CP_AL solver.cu:111-204: the same loop with its own update,
+    // fixed-point error, restart and step set-up)
     if (!params || !prob) return nullptr;
     S s;
     s.P = *params;
     init_state(s, prob);
     if (s.nonconvexity < 0) s.inner_limit = 1;
+#ifdef ORACLE_PLANTED_CP_AL
+    pcal_init_steps(s);
+#else
     // initialize_step_size_and_primal_weight, :1022-1040
     if (s.A.nnz() == 0) s.step = 1.0;
     else s.step = 0.998 / max_singular_value(s);
     s.pw = s.P.bound_objective_rescaling ? 1.0 : (s.c_norm + 1.0) / (s.b_norm + 1.0);
     s.best_pw = s.pw;
+#endif
 
     const auto t0 = std::chrono::steady_clock::now();
     const int K = s.P.termination_evaluation_frequency;
@@ -1812,12 +2099,24 @@ ORACLE_API fa_cp_result_t *optimize(const fa_cp_parameters_t *params, const qp_p
         }
         if (consider) {
             do_restart = planned;
+#ifdef ORACLE_PLANTED_CP_AL
+            if (do_restart) pcal_restart(s);
+#else
             if (do_restart) perform_restart(s);
+#endif
         }
         s.major = ((s.total + 1) % K) == 0;
+#ifdef ORACLE_PLANTED_CP_AL
+        pcal_update(s);
+#else
         fa_cp_update(s);
+#endif
         if (s.major || do_restart) {
+#ifdef ORACLE_PLANTED_CP_AL
+            pcal_fixed_point_error(s);
+#else
             fixed_point_error(s);
+#endif
             if (do_restart) {
                 s.fp0 = s.fp;
                 do_restart = false;
@@ -1871,7 +2170,7 @@ static result_t *result_from_state(S &s, const qp_problem_t *prob, double polish
     r->max_dual_ray_infeasibility = s.max_d_ray;
     r->primal_ray_linear_objective = s.p_ray_obj;
     r->dual_ray_objective = s.d_ray_obj;
-#if !defined(ORACLE_LIN_CP_AL) && !defined(ORACLE_PLANTED_FA_CP)
+#if !defined(ORACLE_CP_AL_TYPES) && !defined(ORACLE_PLANTED_FA_CP)
     r->final_al_sigma = s.al_sigma;
     r->sigma_update_count = s.sigma_update_count;
     r->sigma_update_rule = s.al_available ? s.P.sigma_update_rule : FA_CP_SIGMA_UPDATE_FIXED;

@@ -136,6 +136,21 @@ def build_planted_fa_cp() -> Path:
     return out
 
 
+def build_planted_cp_al() -> Path:
+    """The planted-KKT CP-AL (RHR-CP-AL): the FA_CP sources built with -DPDAL_PLANTED_CP_AL."""
+    objdir = ROOT / "pdal_amd" / "_build" / "planted_cp_al"
+    objs = _compile_objects(FA_CP_SOURCES, objdir, extra=["-DPDAL_PLANTED_CP_AL"])
+    LIB.mkdir(parents=True, exist_ok=True)
+    out = LIB / "librhr_cp_al.so"
+    tmp = LIB / "librhr_cp_al.so.tmp"
+    _run([_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-Bsymbolic", *objs, "-lz", "-o", tmp])
+    os.replace(tmp, out)
+    # its runner (QP/PLANTED_CP_AL's rhr_cp_al) linked against the shared library
+    _run(["g++", "-O2", "-std=c++17", "-DPDAL_PLANTED_CP_AL", f"-I{INCLUDE}", CSRC / "qp" / "variant_cli.cpp",
+          f"-L{LIB}", "-l:librhr_cp_al.so", f"-Wl,-rpath,$ORIGIN", "-o", LIB / "rhr_cp_al"])
+    return out
+
+
 def build_ws() -> Path:
     objdir = ROOT / "pdal_amd" / "_build" / "ws"
     objs = _compile_objects([CSRC / "ws" / "warmstart.hip"], objdir)
@@ -152,6 +167,7 @@ def build_all() -> None:
     build_fa_cp()
     build_lin_cp_al()
     build_planted_fa_cp()
+    build_planted_cp_al()
     build_ws()
 
 

@@ -1,13 +1,14 @@
 // Public C entry points of libfa_cp.so, and with -DPDAL_LIN_CP_AL of
-// librhr_lin_cp_al.so (the same problem construction, QP/LIN_CP_AL/src/cp_al.c,
-// its own defaults and result type).
+// librhr_lin_cp_al.so, -DPDAL_PLANTED_FA_CP of librhr_fa_cp.so and
+// -DPDAL_PLANTED_CP_AL of librhr_cp_al.so (the same problem construction,
+// e.g. QP/LIN_CP_AL/src/cp_al.c; each its own defaults and result type).
 //
 // Reference: QP/FA_CP/src/fa_cp.c:28-376 (problem construction, start values,
 // solve, frees), QP/FA_CP/src/utils.cu:494-537 (defaults), :661-678 (final
 // summary), :1135-1358 (dense / CSC / COO -> CSR with zero_tolerance);
 // QP/LIN_CP_AL/src/utils.cu:466-497 (its defaults).
 #include "fa_cp_internal.hpp"
-#if !defined(PDAL_LIN_CP_AL) && !defined(PDAL_PLANTED_FA_CP)
+#if !defined(PDAL_CP_AL_TYPES) && !defined(PDAL_PLANTED_FA_CP)
 #include "pdal_qp.h"
 #endif
 
@@ -181,6 +182,38 @@ PDAL_API void set_default_parameters(cp_al_parameters_t *p)
     p->termination_evaluation_frequency = 200;
     p->feasibility_polishing = false;
     p->reflection_coefficient = 0.0;
+    p->sv_max_iter = 5000;
+    p->sv_tol = 1e-4;
+    p->termination_criteria.eps_optimal_relative = 1e-4;
+    p->termination_criteria.eps_feasible_relative = 1e-4;
+    p->termination_criteria.eps_infeasible = 1e-10;
+    p->termination_criteria.time_sec_limit = 3600.0;
+    p->termination_criteria.iteration_limit = INT32_MAX;
+    p->termination_criteria.eps_feas_polish_relative = 1e-6;
+    p->restart_params.artificial_restart_threshold = 0.36;
+    p->restart_params.sufficient_reduction_for_restart = 0.2;
+    p->restart_params.necessary_reduction_for_restart = 0.8;
+    p->restart_params.k_p = 0.99;
+    p->restart_params.k_i = 0.01;
+    p->restart_params.k_d = 0.0;
+    p->restart_params.i_smooth = 0.3;
+    p->optimality_norm = NORM_TYPE_L_INF;
+}
+#elif defined(PDAL_PLANTED_CP_AL)
+PDAL_API void set_default_parameters(cp_al_parameters_t *p)
+{   // QP/PLANTED_CP_AL/src/utils.cu:481-513
+    memset(p, 0, sizeof(*p));
+    p->l_inf_ruiz_iterations = 10;
+    p->has_pock_chambolle_alpha = true;
+    p->pock_chambolle_alpha = 1.0;
+    p->bound_objective_rescaling = true;
+    p->use_al_qp = true;
+    p->sigma_init = 0.0;
+    p->trajectory_csv = nullptr;
+    p->verbose = 1;
+    p->termination_evaluation_frequency = 200;
+    p->feasibility_polishing = false;
+    p->reflection_coefficient = 1.0;
     p->sv_max_iter = 5000;
     p->sv_tol = 1e-4;
     p->termination_criteria.eps_optimal_relative = 1e-4;
@@ -390,7 +423,7 @@ PDAL_API void set_start_values(qp_problem_t *p, const double *primal, const doub
     }
 }
 
-#ifdef PDAL_LIN_CP_AL
+#ifdef PDAL_CP_AL_TYPES
 PDAL_API void cp_al_result_free(cp_al_result_t *r)
 #else
 PDAL_API void fa_cp_result_free(fa_cp_result_t *r)
@@ -438,6 +471,13 @@ PDAL_API result_t *solve_qp_problem(const qp_problem_t *prob, const params_t *pa
 PDAL_API void pdal_lin_cp_al_reset_rng(void) { pdal::qp::reset_rng(); }
 
 PDAL_API const char *pdal_lin_cp_al_termination_reason_to_string(termination_reason_t r)
+{
+    return pdal::qp::termination_reason_name(r);
+}
+#elif defined(PDAL_PLANTED_CP_AL)
+PDAL_API void pdal_planted_cp_al_reset_rng(void) { pdal::qp::reset_rng(); }
+
+PDAL_API const char *pdal_planted_cp_al_termination_reason_to_string(termination_reason_t r)
 {
     return pdal::qp::termination_reason_name(r);
 }

@@ -33,6 +33,15 @@
 // the clamp at zero, and its own AL sigma rule at restarts
 // (fa_cp_core_op.cu:610-747) instead of FA_CP's; its feasibility polishing is
 // FA_CP's code (fa_cp_core_op.cu:1009-1502).
+//
+// Built with -DPDAL_PLANTED_CP_AL it is RHR-CP-AL, the planted-KKT CP-AL
+// (QP/PLANTED_CP_AL): FA_CP's main loop, restart test and Halpern step over the
+// linearized solver's sigma / lambda steps, with its own set-up
+// (cp_al_core_op.cu:805-858), update (:531-635: equality rows, LP and diagonal Q
+// in closed form, FA_CP's BB inner solve with the effective dual fixed for the
+// other Q types, T-outputs in major-kernel iterations only), fixed-point error
+// on the reflection (:860-935), and at restarts a sigma controller in the AL
+// branch or the primal-weight PID (:655-803).
 #include "fa_cp_internal.hpp"
 
 #include "../common/csr_host.hpp"
@@ -188,6 +197,31 @@ class Solver {
     void lin_restart();
     void lin_halpern();
     result_t *run_linearized();
+    // planted-KKT CP-AL (PDAL_PLANTED_CP_AL builds)
+    double best_sigma_ = 0.0, best_term_metric_ = INFINITY, best_sigma_gap_ = INFINITY;
+    double pcal_tau() const
+    {   // cpal_effective_primal_step_size, PLANTED_CP_AL cp_al_core_op.cu:255-265
+        double t = lin_tau();
+        if (nonconvexity_ < 0) {
+            t = std::fmax(t, -1.01 * std::fmin(0.0, nonconvexity_));
+            t /= 2;
+        }
+        return t;
+    }
+    void pcal_init_steps();
+    void pcal_update(bool fuse_halpern);
+    void pcal_fixed_point_error();
+    void pcal_reset_sigma();
+    void pcal_sigma_controller(double pd, double dd, double ax);
+    void pid_primal_weight(double pd, double dd);
+    // the BB inner solve recomputes the AL effective dual at each inner step
+    // (FA_CP's full AL subproblem) except in the planted CP-AL, whose BB keeps
+    // the product fixed at the outer iterate
+#ifdef PDAL_PLANTED_CP_AL
+    bool bb_al() const { return false; }
+#else
+    bool bb_al() const { return use_al_; }
+#endif
     result_t *make_result();
     void print_initial_info() const;
     void display_stats() const;
@@ -484,7 +518,7 @@ void Solver::init_state()
     // planted solver_state.cu:498-504: the AL branch only with a nonzero sigma
     use_al_ = P_.use_al_qp && P_.al_sigma != 0.0;
     al_sigma_ = use_al_ ? P_.al_sigma : 0.0;
-#elif !defined(PDAL_LIN_CP_AL)
+#elif !defined(PDAL_CP_AL_TYPES)
     al_sigma_ = P_.al_sigma;
     best_al_sigma_ = P_.al_sigma;
 #endif
@@ -561,9 +595,9 @@ void Solver::init_state()
         inner_limit_ = 1000;
         inner_tol_ = inner_tol0_ = 1e-3;
     }
-#elif defined(PDAL_LIN_CP_AL)
-    // LIN_CP_AL solver_state.cu:305-340: an inner loop only for the non-diagonal Q types
-    // (its linearized update never runs it)
+#elif defined(PDAL_CP_AL_TYPES)
+    // LIN_CP_AL / PLANTED_CP_AL solver_state.cu:305-340: an inner loop only for the
+    // non-diagonal Q types (the linearized update never runs it; the planted one's BB does)
     const bool simple = qtype_ == FA_CP_NON_Q || qtype_ == FA_CP_DIAG_Q;
     has_inner_ = !simple;
     if (!simple) {
@@ -642,8 +676,8 @@ StepScalars Solver::scalars() const
     sc.al_sigma = use_al_ ? al_sigma_ : 0.0;
     sc.weight = (double)(inner_ + 1) / (inner_ + 2);
     sc.rho = P_.reflection_coefficient;
-#ifdef PDAL_LIN_CP_AL
-    sc.tau_res = use_al_ ? lin_tau() : step_ / pw_;  // LIN_CP_AL utils.cu:855-858
+#ifdef PDAL_CP_AL_TYPES
+    sc.tau_res = use_al_ ? lin_tau() : step_ / pw_;  // LIN_CP_AL utils.cu:855-858 (PLANTED_CP_AL :871-874)
 #else
     sc.tau_res = step_ / pw_;
 #endif
@@ -653,7 +687,7 @@ StepScalars Solver::scalars() const
 void Solver::bb_inner_step(const StepScalars &sc)
 {   // one pass of the loop body in primal_BB_step_size_update, :452-500
     BBState *bb = dbb_.get();
-    if (use_al_) {
+    if (bb_al()) {
         dispatch_group(A_.G, [&](auto g) {
             constexpr int G = decltype(g)::value;
             al_effective_dual_kernel<G><<<blocks_for((long)m_ * G), kBlock, 0, stream_>>>(
@@ -705,7 +739,7 @@ void Solver::launch_inner_steps(const StepScalars &sc, int steps)
         return;
     }
     if (!bb_graph_ || g_tau_ != sc.tau || g_al_ != sc.al_sigma || g_tol_ != inner_tol_ || g_limit_ != inner_limit_ ||
-        g_use_al_ != use_al_) {
+        g_use_al_ != bb_al()) {
         if (bb_graph_) PDAL_HIP(hipGraphExecDestroy(bb_graph_));
         bb_graph_ = nullptr;
         hipGraph_t graph;
@@ -718,14 +752,14 @@ void Solver::launch_inner_steps(const StepScalars &sc, int steps)
         g_al_ = sc.al_sigma;
         g_tol_ = inner_tol_;
         g_limit_ = inner_limit_;
-        g_use_al_ = use_al_;
+        g_use_al_ = bb_al();
     }
     for (int k = 0; k < steps; k += kGraphSteps) PDAL_HIP(hipGraphLaunch(bb_graph_, stream_));
 }
 
 void Solver::primal_bb(const StepScalars &sc, bool fuse_halpern)
 {   // primal_BB_step_size_update, fa_cp_core_op.cu:445-503
-    if (use_al_) {  // compute_al_effective_dual_product at x (:275-303)
+    if (bb_al()) {  // compute_al_effective_dual_product at x (:275-303)
         dispatch_group(A_.G, [&](auto g) {
             constexpr int G = decltype(g)::value;
             al_effective_dual_kernel<G><<<blocks_for((long)m_ * G), kBlock, 0, stream_>>>(
@@ -763,6 +797,10 @@ void Solver::primal_bb(const StepScalars &sc, bool fuse_halpern)
 
 void Solver::update(bool fuse)
 {   // fa_cp_update, fa_cp_core_op.cu:531-630 (+ the Halpern step when fused)
+#ifdef PDAL_PLANTED_CP_AL
+    pcal_update(fuse);
+    return;
+#endif
     const StepScalars sc = scalars();
     const bool mk = major_kernel();
     const bool simple = !use_al_ && (qtype_ == FA_CP_NON_Q || qtype_ == FA_CP_DIAG_Q);
@@ -818,6 +856,10 @@ void Solver::halpern()
 
 void Solver::fixed_point_error()
 {   // compute_fixed_point_error, fa_cp_core_op.cu:1042-1089
+#ifdef PDAL_PLANTED_CP_AL
+    pcal_fixed_point_error();
+    return;
+#endif
     const int nbm = stride_grid(m_);
     diff_norm_kernel<<<nbm, kBlock, 0, stream_>>>(yR_.get(), y_.get(), dy_.get(), m_, parts_.get());
     reduce(nbm, 1, 1, S_DN2);
@@ -970,7 +1012,7 @@ bool Solver::should_restart()
     return r;
 }
 
-#if !defined(PDAL_LIN_CP_AL) && !defined(PDAL_PLANTED_FA_CP)
+#if !defined(PDAL_CP_AL_TYPES) && !defined(PDAL_PLANTED_FA_CP)
 void Solver::set_sigma_active(double sigma, double zero_thr)
 {   // fa_cp_core_op.cu:667-676
     if (!std::isfinite(sigma) || sigma <= zero_thr) {
@@ -1122,29 +1164,8 @@ void Solver::planted_sigma_on_restart()
 #endif
 
 #ifndef PDAL_LIN_CP_AL
-void Solver::perform_restart()
-{   // perform_restart, fa_cp_core_op.cu:908-1020 (restart target: the T-output)
-    const int nbn = stride_grid(n_), nbm = stride_grid(m_);
-    diff_norm_kernel<<<nbn, kBlock, 0, stream_>>>(xT_.get(), x0_.get(), dx_.get(), n_, parts_.get());
-    reduce(nbn, 1, 1, S_PD2);
-    diff_norm_kernel<<<nbm, kBlock, 0, stream_>>>(yT_.get(), y0_.get(), dy_.get(), m_, parts_.get());
-    reduce(nbm, 1, 1, S_DD2);
-    PDAL_LAUNCHED();
-    fetch(S_PD2, 2);
-    const double pd = std::sqrt(hs(S_PD2)), dd = std::sqrt(hs(S_DD2));
-    double normal = INFINITY;
-    if (pd > 1e-16 && std::isfinite(pd)) {
-        int nb = 1;
-        dispatch_group(A_.G, [&](auto g) {
-            constexpr int G = decltype(g)::value;
-            nb = row_grid(m_, G);
-            spmv_norm_kernel<G><<<nb, kBlock, 0, stream_>>>(A_.view(), dx_.get(), prim_prod_.get(), parts_.get());
-        });
-        PDAL_LAUNCHED();
-        reduce(nb, 1, 1, S_NORMAL);
-        fetch(S_NORMAL, 1);
-        normal = std::sqrt(hs(S_NORMAL)) / pd;
-    }
+void Solver::pid_primal_weight(double pd, double dd)
+{   // the primal-weight PID of perform_restart, fa_cp_core_op.cu:908-1020 (PLANTED_CP_AL :758-786)
     const double ratio = rel_d_ / rel_p_;
     const auto &rp = P_.restart_params;
     if (pd > 1e-16 && dd > 1e-16 && pd < 1e12 && dd < 1e12 && ratio > 1e-8 && ratio < 1e8) {
@@ -1164,12 +1185,55 @@ void Solver::perform_restart()
         best_gap_ = g;
         best_pw_ = pw_;
     }
+}
+
+void Solver::perform_restart()
+{   // perform_restart, fa_cp_core_op.cu:908-1020 (restart target: the T-output)
+    const int nbn = stride_grid(n_), nbm = stride_grid(m_);
+    diff_norm_kernel<<<nbn, kBlock, 0, stream_>>>(xT_.get(), x0_.get(), dx_.get(), n_, parts_.get());
+    reduce(nbn, 1, 1, S_PD2);
+    diff_norm_kernel<<<nbm, kBlock, 0, stream_>>>(yT_.get(), y0_.get(), dy_.get(), m_, parts_.get());
+    reduce(nbm, 1, 1, S_DD2);
+    PDAL_LAUNCHED();
+    fetch(S_PD2, 2);
+    const double pd = std::sqrt(hs(S_PD2)), dd = std::sqrt(hs(S_DD2));
+#ifdef PDAL_PLANTED_CP_AL
+    if (use_al_) {  // ||A (xT - x0)|| for the sigma controller (PLANTED_CP_AL :670-686)
+        int nb = 1;
+        dispatch_group(A_.G, [&](auto g) {
+            constexpr int G = decltype(g)::value;
+            nb = row_grid(m_, G);
+            spmv_norm_kernel<G><<<nb, kBlock, 0, stream_>>>(A_.view(), dx_.get(), prim_prod_.get(), parts_.get());
+        });
+        PDAL_LAUNCHED();
+        reduce(nb, 1, 1, S_NORMAL);
+        fetch(S_NORMAL, 1);
+        pcal_sigma_controller(pd, dd, std::sqrt(hs(S_NORMAL)));
+    } else {
+        pid_primal_weight(pd, dd);
+    }
+#else
+    double normal = INFINITY;
+    if (pd > 1e-16 && std::isfinite(pd)) {
+        int nb = 1;
+        dispatch_group(A_.G, [&](auto g) {
+            constexpr int G = decltype(g)::value;
+            nb = row_grid(m_, G);
+            spmv_norm_kernel<G><<<nb, kBlock, 0, stream_>>>(A_.view(), dx_.get(), prim_prod_.get(), parts_.get());
+        });
+        PDAL_LAUNCHED();
+        reduce(nb, 1, 1, S_NORMAL);
+        fetch(S_NORMAL, 1);
+        normal = std::sqrt(hs(S_NORMAL)) / pd;
+    }
+    pid_primal_weight(pd, dd);
 #ifdef PDAL_PLANTED_FA_CP
     (void)normal;
     planted_sigma_on_restart();
 #else
     update_al_sigma(pd, dd, normal);
 #endif
+#endif  // PDAL_PLANTED_CP_AL
     restart_copy_kernel<<<blocks_for((long)n_ + m_), kBlock, 0, stream_>>>(xT_.get(), x0_.get(), x_.get(), n_,
                                                                            yT_.get(), y0_.get(), y_.get(), m_);
     PDAL_LAUNCHED();
@@ -1216,7 +1280,7 @@ result_t *Solver::make_result()
     r->max_dual_ray_infeasibility = max_d_ray_;
     r->primal_ray_linear_objective = p_ray_obj_;
     r->dual_ray_objective = d_ray_obj_;
-#if !defined(PDAL_LIN_CP_AL) && !defined(PDAL_PLANTED_FA_CP)
+#if !defined(PDAL_CP_AL_TYPES) && !defined(PDAL_PLANTED_FA_CP)
     r->final_al_sigma = al_sigma_;
     r->sigma_update_count = sigma_update_count_;
     r->sigma_update_rule = al_available_ ? P_.sigma_update_rule : FA_CP_SIGMA_UPDATE_FIXED;
@@ -1231,7 +1295,7 @@ void Solver::print_initial_info() const
 {   // print_initial_info, utils.cu:560-659 (the settings that differ from the defaults are omitted)
     if (P_.verbose < 2) return;
     printf("---------------------------------------------------------------------------------------\n");
-    printf("                          %s (MI355X / gfx950 HIP build)\n", kLinearized ? "RHR-Lin-CP-AL" : kEqualityAL ? "RHR-FA-CP" : "FA_CP");
+    printf("                          %s (MI355X / gfx950 HIP build)\n", kPlantedCpAl ? "RHR-CP-AL" : kLinearized ? "RHR-Lin-CP-AL" : kEqualityAL ? "RHR-FA-CP" : "FA_CP");
     printf("---------------------------------------------------------------------------------------\n");
     printf("problem: %d rows, %d columns, %d nonzeros\n", W_->num_constraints, W_->num_variables,
            W_->constraint_matrix_num_nonzeros);
@@ -1272,7 +1336,7 @@ void Solver::lin_init_steps()
         }
         sigma_min_ = sigma_base_;
         sigma_max_ = 100.0 * sigma_base_;
-#ifdef PDAL_LIN_CP_AL
+#ifdef PDAL_CP_AL_TYPES
         const double init = P_.sigma_init;
 #else
         const double init = 0.0;
@@ -1330,7 +1394,7 @@ void Solver::lin_update()
     }
     PDAL_LAUNCHED();
     spmv(A_, xR_.get(), prim_prod_.get());
-    lin_dual_kernel<<<blocks_for(m_), kBlock, 0, stream_>>>(y_.get(), prim_prod_.get(), l_.get(), dstep, yT_.get(),
+    lin_dual_kernel<true><<<blocks_for(m_), kBlock, 0, stream_>>>(y_.get(), prim_prod_.get(), l_.get(), dstep, yT_.get(),
                                                             yR_.get(), m_);
     PDAL_LAUNCHED();
 }
@@ -1436,6 +1500,188 @@ result_t *Solver::run_linearized()
     return make_result();
 }
 
+#ifdef PDAL_PLANTED_CP_AL
+// ---------------------------------------------------------------------------
+// Planted-KKT CP-AL (PDAL_PLANTED_CP_AL builds; QP/PLANTED_CP_AL/src/cp_al_core_op.cu)
+
+void Solver::pcal_init_steps()
+{   // initialize_step_size_and_primal_weight, :805-858
+    if (use_al_) {
+        if (A_.nnz == 0) {
+            lambda_est_ = 1.0;
+        } else {
+            const double sv = max_singular_value();
+            lambda_est_ = 2.001 * sv * sv;
+        }
+        sigma_base_ = 1.0 / std::sqrt(std::fmax(lambda_est_, 1e-16));
+        sigma_min_ = 1e-4 * sigma_base_;
+        sigma_max_ = 100.0 * sigma_base_;
+        sigma_ = P_.sigma_init > 0.0 ? P_.sigma_init : 0.5 * sigma_base_;
+        sigma_ = std::fmin(std::fmax(sigma_, sigma_min_), sigma_max_);
+        best_sigma_ = sigma_;
+        best_sigma_gap_ = INFINITY;
+        best_term_metric_ = INFINITY;
+        step_ = pcal_tau();
+    } else {
+        step_ = A_.nnz == 0 ? 1.0 : 0.998 / max_singular_value();
+    }
+    pw_ = P_.bound_objective_rescaling ? 1.0 : (c_norm_ + 1.0) / (b_norm_ + 1.0);
+    best_pw_ = pw_;
+}
+
+void Solver::pcal_update(bool fuse)
+{   // cp_al_update, :531-635: equality rows; the T-outputs (and the LP dual slack)
+    // only in major-kernel iterations
+    double tau, dstep;
+    if (use_al_) {
+        tau = pcal_tau();
+        dstep = std::fmax(sigma_, 1e-16);
+        step_ = tau;
+    } else {
+        tau = step_ / pw_;
+        dstep = step_ * pw_;
+        if (nonconvexity_ < 0) {
+            tau = std::fmax(tau, -1.01 * std::fmin(0.0, nonconvexity_));
+            tau /= 2;
+        }
+    }
+    const bool mk = major_kernel();
+    if (use_al_) {  // compute_al_effective_dual_product, :285-312
+        spmv(A_, x_.get(), prim_prod_.get());
+        lin_al_dual_kernel<<<blocks_for(m_), kBlock, 0, stream_>>>(prim_prod_.get(), y_.get(), l_.get(), sigma_,
+                                                                   al_u_.get(), m_);
+        PDAL_LAUNCHED();
+        spmv(At_, al_u_.get(), dual_prod_.get());
+    } else {
+        spmv(At_, y_.get(), dual_prod_.get());
+    }
+    if (qtype_ == FA_CP_NON_Q || qtype_ == FA_CP_DIAG_Q) {  // lp_primal_update / diag_q_primal_update, :314-356
+        const bool diag = qtype_ == FA_CP_DIAG_Q;
+        auto launch = [&](auto dtag, auto mtag) {
+            pcal_primal_kernel<decltype(dtag)::value, decltype(mtag)::value><<<blocks_for(n_), kBlock, 0, stream_>>>(
+                x_.get(), dual_prod_.get(), c_.get(), lb_.get(), ub_.get(), qdiag_.get(), tau, xT_.get(), xR_.get(),
+                dslack_.get(), n_);
+        };
+        using T = std::true_type;
+        using F = std::false_type;
+        if (diag) {
+            if (mk) launch(T{}, T{});
+            else launch(T{}, F{});
+        } else {
+            if (mk) launch(F{}, T{});
+            else launch(F{}, F{});
+        }
+        PDAL_LAUNCHED();
+    } else {  // primal_BB_step_size_update, :454-503: the BB with At.u fixed (bb_al() is false)
+        StepScalars sc = scalars();
+        sc.tau = tau;
+        sc.al_sigma = 0.0;
+        primal_bb(sc, false);
+    }
+    spmv(A_, xR_.get(), prim_prod_.get());
+    if (mk)
+        lin_dual_kernel<true><<<blocks_for(m_), kBlock, 0, stream_>>>(y_.get(), prim_prod_.get(), l_.get(), dstep,
+                                                                      yT_.get(), yR_.get(), m_);
+    else
+        lin_dual_kernel<false><<<blocks_for(m_), kBlock, 0, stream_>>>(y_.get(), prim_prod_.get(), l_.get(), dstep,
+                                                                       yT_.get(), yR_.get(), m_);
+    PDAL_LAUNCHED();
+    if (fuse) halpern();
+}
+
+void Solver::pcal_fixed_point_error()
+{   // compute_fixed_point_error, :860-935 (delta = reflected - current)
+    const int nbn = stride_grid(n_), nbm = stride_grid(m_);
+    diff_norm_kernel<<<nbm, kBlock, 0, stream_>>>(yR_.get(), y_.get(), dy_.get(), m_, parts_.get());
+    reduce(nbm, 1, 1, S_DN2);
+    diff_norm_kernel<<<nbn, kBlock, 0, stream_>>>(xR_.get(), x_.get(), dx_.get(), n_, parts_.get());
+    reduce(nbn, 1, 1, S_PD2);
+    PDAL_LAUNCHED();
+    if (use_al_) {  // <A dx, dy> and ||A dx||^2
+        spmv(A_, dx_.get(), prim_prod_.get());
+        dot_sumsq_kernel<<<nbm, kBlock, 0, stream_>>>(dy_.get(), prim_prod_.get(), m_, parts_.get());
+        reduce(nbm, 2, 2, S_FP);
+    } else {  // <At dy, dx>
+        spmv(At_, dy_.get(), dual_prod_.get());
+        dot_sumsq_kernel<<<nbn, kBlock, 0, stream_>>>(dx_.get(), dual_prod_.get(), n_, parts_.get());
+        reduce(nbn, 2, 2, S_FP);
+    }
+    PDAL_LAUNCHED();
+    fetch(S_FP, S_PD2 - S_FP + 1);
+    const double pn = std::sqrt(hs(S_PD2)), dn = std::sqrt(hs(S_DN2)), cross = hs(S_FP);
+    if (use_al_) {
+        const double an = std::sqrt(hs(S_FP + 1));
+        const double fp2 = sigma_ * (lambda_est_ * pn * pn - an * an) + 2.0 * cross + dn * dn / sigma_;
+        fp_ = std::sqrt(std::fmax(0.0, fp2));
+    } else {
+        const double movement = pn * pn * pw_ + dn * dn / pw_;
+        fp_ = std::sqrt(movement + 2 * step_ * cross);  // no clamp at zero (:921)
+    }
+    if (has_inner_) {
+        const double st = use_al_ ? step_ : step_ / pw_;
+        inner_tol_ = std::fmin(inner_tol_, std::fmax(0.0005 * pn / st, 1e-9));
+    }
+}
+
+void Solver::pcal_reset_sigma()
+{   // reset_sigma_controller, :271-283
+    double sg = std::fmax(best_sigma_, 1e-16);
+    if (sigma_min_ > 0.0) sg = std::fmax(sg, sigma_min_);
+    if (sigma_max_ > 0.0) sg = std::fmin(sg, sigma_max_);
+    sigma_ = sg;
+    step_ = pcal_tau();
+}
+
+void Solver::pcal_sigma_controller(double pd, double dd, double ax)
+{   // the AL branch of perform_restart, :687-756: a log-damped step toward
+    // ||dy|| / sqrt(lambda ||dx||^2 - ||A dx||^2), clipped to [0.6, 1.7] per restart,
+    // rolled back to the best sigma when the restart metric worsens by 5 %
+    double cur = fp_;
+    const double dxm = lambda_est_ * pd * pd - ax * ax;
+    const double ratio = rel_d_ / std::fmax(rel_p_, 1e-300);
+    const double old = std::fmax(sigma_, 1e-16);
+    bool rolled = false;
+    if (std::isfinite(cur) && cur < best_term_metric_) {
+        best_term_metric_ = cur;
+        best_sigma_ = old;
+        best_sigma_gap_ = std::fabs(std::log10(std::fmax(rel_d_, 1e-300) / std::fmax(rel_p_, 1e-300)));
+    }
+    if (!std::isfinite(cur)) {
+        pcal_reset_sigma();
+        rolled = true;
+        cur = INFINITY;
+    } else if (std::isfinite(best_term_metric_) && cur > 1.05 * best_term_metric_) {
+        pcal_reset_sigma();
+        rolled = true;
+    } else {
+        bool valid = false;
+        double est = old;
+        if (dd > 1e-16 && dd < 1e12 && dxm > 1e-16 && dxm < 1e24 && std::isfinite(sigma_) && ratio > 1e-8 &&
+            ratio < 1e8) {
+            est = dd / std::sqrt(dxm);
+            valid = std::isfinite(est) && est > 1e-16;
+        }
+        if (valid) {
+            const double target = old * std::exp(0.5 * std::log(est / old));
+            if (std::isfinite(target) && target > 1e-16) sigma_ = old * std::fmax(0.60, std::fmin(target / old, 1.70));
+        }
+    }
+    if (!std::isfinite(sigma_) || sigma_ <= 1e-16 || sigma_ >= 1e16) {
+        pcal_reset_sigma();
+        rolled = true;
+    } else {
+        sigma_ = std::fmin(std::fmax(sigma_, sigma_min_), sigma_max_);
+        step_ = pcal_tau();
+    }
+    const double g = std::fabs(std::log10(std::fmax(rel_d_, 1e-300) / std::fmax(rel_p_, 1e-300)));
+    if (g < best_sigma_gap_) best_sigma_gap_ = g;
+    if (rolled) {
+        sigma_ = best_sigma_;
+        step_ = pcal_tau();
+    }
+}
+#endif  // PDAL_PLANTED_CP_AL
+
 result_t *Solver::run()
 {
 #ifdef PDAL_LIN_CP_AL
@@ -1448,10 +1694,14 @@ result_t *Solver::run()
     rescale_time_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_setup).count();
     init_state();
     if (nonconvexity_ < 0) inner_limit_ = 1;
+#ifdef PDAL_PLANTED_CP_AL
+    pcal_init_steps();
+#else
     // initialize_step_size_and_primal_weight, fa_cp_core_op.cu:1022-1040
     step_ = A_.nnz == 0 ? 1.0 : 0.998 / max_singular_value();
     pw_ = P_.bound_objective_rescaling ? 1.0 : (c_norm_ + 1.0) / (b_norm_ + 1.0);
     best_pw_ = pw_;
+#endif
     PDAL_HIP(hipStreamSynchronize(stream_));
 
     const auto t0 = std::chrono::steady_clock::now();
@@ -1563,7 +1813,24 @@ double Solver::polish_fixed_point_error(bool primal)
     reduce(nb, 1, 1, S_DN2);
     fetch(S_DN2, 1);
     const double ss = hs(S_DN2);
+#ifdef PDAL_PLANTED_CP_AL
+    // compute_primal/dual_fixed_point_error, PLANTED_CP_AL cp_al_core_op.cu:1495-1539
+    if (!primal) return ss / (use_al_ ? sigma_ : pw_);
+    if (!use_al_) return ss * pw_;
+    int nba = 1;
+    dispatch_group(A_.G, [&](auto g) {
+        constexpr int G = decltype(g)::value;
+        nba = row_grid(m_, G);
+        spmv_norm_kernel<G><<<nba, kBlock, 0, stream_>>>(A_.view(), dx_.get(), prim_prod_.get(), parts_.get());
+    });
+    PDAL_LAUNCHED();
+    reduce(nba, 1, 1, S_NORMAL);
+    fetch(S_NORMAL, 1);
+    const double pn = std::sqrt(ss), an = std::sqrt(hs(S_NORMAL));
+    return std::fmax(0.0, sigma_ * (lambda_est_ * pn * pn - an * an));
+#else
     return primal ? ss * pw_ : ss / pw_;
+#endif
 }
 
 Solver::PolishOut Solver::polish_phase(bool primal, double pw0, const PolishRefs &ref)

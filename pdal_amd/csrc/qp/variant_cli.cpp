@@ -1,15 +1,18 @@
-// The runners of the two QP variants, built from this file with a variant flag:
+// The runners of the three QP variants, built from this file with a variant flag:
 //   -DPDAL_LIN_CP_AL     -> rhr_lin_cp_al, linked against librhr_lin_cp_al.so
 //                           (QP/LIN_CP_AL/src/cli.c; CMakeLists.txt:143-158)
 //   -DPDAL_PLANTED_FA_CP -> rhr_fa_cp, linked against librhr_fa_cp.so
 //                           (QP/PLANTED_FA_CP/src/cli.c; CMakeLists.txt:143-158)
+//   -DPDAL_PLANTED_CP_AL -> rhr_cp_al, linked against librhr_cp_al.so
+//                           (QP/PLANTED_CP_AL/src/cli.c; CMakeLists.txt:143-158)
 //
-// Both take FA_CP's common flags (getopt, defaults from set_default_parameters),
+// All take FA_CP's common flags (getopt, defaults from set_default_parameters),
 // accept only equality-constrained problems (validate_eqbox_problem: every row
 // finite with |l - u| <= 1e-9 (1 + max(|l|, |u|))), and write FA_CP's output
 // files without its three sigma lines.  Flags of their own: the linearized
 // solver's --sigma_init (and its deprecated alias --al_lambda) and
-// --reflection_coefficient; the planted solver's --al_sigma, the adaptive AL
+// --reflection_coefficient; the planted CP-AL's --sigma_init,
+// --reflection_coefficient and --trajectory_csv; the planted FA_CP's --al_sigma, the adaptive AL
 // sigma flags, --reflection_coefficient, the restart and primal-weight PID
 // flags and --trajectory_csv (accepted; the file is not written by this
 // build).  As in the reference, whose main discards run_*'s return value,
@@ -21,6 +24,13 @@ using params_t = cp_al_parameters_t;
 #define VARIANT_NAME "rhr_lin_cp_al"
 #define VARIANT_REASON pdal_lin_cp_al_termination_reason_to_string
 #define VARIANT_RESULT_FREE cp_al_result_free
+#elif defined(PDAL_PLANTED_CP_AL)
+#include "planted_cp_al.h"
+#include "planted_cp_al_mps_parser.h"
+using params_t = cp_al_parameters_t;
+#define VARIANT_NAME "rhr_cp_al"
+#define VARIANT_REASON pdal_planted_cp_al_termination_reason_to_string
+#define VARIANT_RESULT_FREE cp_al_result_free
 #elif defined(PDAL_PLANTED_FA_CP)
 #include "planted_fa_cp.h"
 #include "planted_fa_cp_mps_parser.h"
@@ -29,7 +39,7 @@ using params_t = fa_cp_parameters_t;
 #define VARIANT_REASON pdal_planted_fa_cp_termination_reason_to_string
 #define VARIANT_RESULT_FREE fa_cp_result_free
 #else
-#error "build with -DPDAL_LIN_CP_AL or -DPDAL_PLANTED_FA_CP"
+#error "build with -DPDAL_LIN_CP_AL, -DPDAL_PLANTED_CP_AL or -DPDAL_PLANTED_FA_CP"
 #endif
 #include "cli_io.hpp"
 
@@ -45,7 +55,8 @@ using params_t = fa_cp_parameters_t;
 namespace {
 
 bool is_equality_problem(const qp_problem_t *p)
-{   // validate_eqbox_problem (QP/LIN_CP_AL/src/cli.c:59-80, QP/PLANTED_FA_CP/src/cli.c:95-116)
+{   // validate_eqbox_problem (QP/LIN_CP_AL/src/cli.c:59-80, QP/PLANTED_CP_AL/src/cli.c:59-80,
+    // QP/PLANTED_FA_CP/src/cli.c:95-116)
     const double tol = 1e-9;
     for (int i = 0; i < p->num_constraints; ++i) {
         const double lo = p->constraint_lower_bound[i], hi = p->constraint_upper_bound[i];
@@ -116,11 +127,16 @@ void usage(const char *prog)
             "      --no_pock_chambolle             Disable Pock-Chambolle rescaling (default: enabled).\n"
             "      --pock_chambolle_alpha <float>  Value for Pock-Chambolle alpha (default: 1.0).\n"
             "      --no_bound_obj_rescaling        Disable bound objective rescaling (default: enabled).\n"
-#ifdef PDAL_LIN_CP_AL
+#if defined(PDAL_LIN_CP_AL)
             "      --no_al_qp                      Use the non-augmented baseline branch.\n"
             "      --sigma_init <float>            Initial sigma of the linearized CP-AL (0 = automatic, default).\n"
             "      --al_lambda <float>             Deprecated alias for --sigma_init.\n"
             "      --reflection_coefficient <float> Reflection parameter of the Halpern step (default: 0.0).\n"
+#elif defined(PDAL_PLANTED_CP_AL)
+            "      --no_al_qp                      Use the non-augmented baseline branch.\n"
+            "      --sigma_init <float>            Initial sigma of the single-parameter CP-AL (0 = automatic, default).\n"
+            "      --reflection_coefficient <float> Reflection coefficient of the Halpern step (default: 1.0).\n"
+            "      --trajectory_csv <path>         Accepted; not written by this build.\n"
 #else
             "      --no_al_qp                      Use the non-augmented baseline branch.\n"
             "      --al_sigma <float>              AL penalty parameter of the primal subproblem (default: 0.1).\n"
@@ -173,9 +189,13 @@ int run(int argc, char *argv[])
         {"eval_freq", required_argument, nullptr, 1013},
         {"opt_norm", required_argument, nullptr, 1014},
         {"no_al_qp", no_argument, nullptr, 1015},
-#ifdef PDAL_LIN_CP_AL
+#if defined(PDAL_LIN_CP_AL)
         {"sigma_init", required_argument, nullptr, 1016},
         {"al_lambda", required_argument, nullptr, 1017},
+        {"reflection_coefficient", required_argument, nullptr, 1018},
+#elif defined(PDAL_PLANTED_CP_AL)
+        {"sigma_init", required_argument, nullptr, 1016},
+        {"trajectory_csv", required_argument, nullptr, 1017},
         {"reflection_coefficient", required_argument, nullptr, 1018},
 #else
         {"al_sigma", required_argument, nullptr, 1016},
@@ -227,9 +247,16 @@ int run(int argc, char *argv[])
             }
             break;
         case 1015: params.use_al_qp = false; break;
-#ifdef PDAL_LIN_CP_AL
+#if defined(PDAL_LIN_CP_AL)
         case 1016:
         case 1017: params.sigma_init = atof(optarg); break;
+        case 1018: params.reflection_coefficient = atof(optarg); break;
+#elif defined(PDAL_PLANTED_CP_AL)
+        case 1016: params.sigma_init = atof(optarg); break;
+        case 1017:
+            params.trajectory_csv = optarg;
+            fprintf(stderr, "[" VARIANT_NAME "] --trajectory_csv: the trajectory file is not written by this build.\n");
+            break;
         case 1018: params.reflection_coefficient = atof(optarg); break;
 #else
         case 1016: params.al_sigma = atof(optarg); break;

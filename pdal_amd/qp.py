@@ -158,6 +158,13 @@ class LinCpAlParameters(C.Structure):
     ]
 
 
+class PlantedCpAlParameters(C.Structure):
+    """cp_al_parameters_t of the planted-KKT CP-AL (planted_cp_al_types.h): the
+    linearized solver's with trajectory_csv after sigma_init."""
+    _fields_ = (LinCpAlParameters._fields_[:6] + [("trajectory_csv", C.c_char_p)]
+                + LinCpAlParameters._fields_[6:])
+
+
 class LinCpAlResult(C.Structure):
     """cp_al_result_t of the linearized CP-AL QP solver: FaCpResult without the sigma fields."""
     _fields_ = [f for f in FaCpResult._fields_
@@ -420,6 +427,15 @@ class LinCpAl(FaCp):
     reset_names = ("pdal_lin_cp_al_reset_rng", "lin_cp_al_oracle_reset_rng")
 
 
+class PlantedCpAl(FaCp):
+    """A library exporting the planted-KKT CP-AL entry points (planted_cp_al.h):
+    the linearized solver's names and result type, its own parameter type."""
+    Params = PlantedCpAlParameters
+    Result = LinCpAlResult
+    result_free = "cp_al_result_free"
+    reset_names = ("pdal_planted_cp_al_reset_rng", "planted_cp_al_oracle_reset_rng")
+
+
 class PlantedFaCp(FaCp):
     """A library exporting the planted-KKT FA_CP entry points (planted_fa_cp.h):
     FA_CP's names, its own parameter type, the result without the sigma fields."""
@@ -452,6 +468,11 @@ def load_fa_cp() -> FaCp:
 def load_planted_fa_cp() -> PlantedFaCp:
     """The MI355X librhr_fa_cp.so; raises FileNotFoundError if it is not built."""
     return PlantedFaCp(LIB_DIR / "librhr_fa_cp.so")
+
+
+def load_planted_cp_al() -> PlantedCpAl:
+    """The MI355X librhr_cp_al.so; raises FileNotFoundError if it is not built."""
+    return PlantedCpAl(LIB_DIR / "librhr_cp_al.so")
 
 
 def load_lin_cp_al() -> LinCpAl:

@@ -1,6 +1,7 @@
-"""Runners of the two QP variants: rhr_lin_cp_al (QP/LIN_CP_AL/src/cli.c) and
-rhr_fa_cp (QP/PLANTED_FA_CP/src/cli.c), linked against librhr_lin_cp_al.so
-and librhr_fa_cp.so.
+"""Runners of the three QP variants: rhr_lin_cp_al (QP/LIN_CP_AL/src/cli.c),
+rhr_cp_al (QP/PLANTED_CP_AL/src/cli.c) and rhr_fa_cp
+(QP/PLANTED_FA_CP/src/cli.c), linked against librhr_lin_cp_al.so,
+librhr_cp_al.so and librhr_fa_cp.so.
 
 CPU tests: usage and argument errors, the equality-rows check
 (validate_eqbox_problem) that rejects a problem before any solve, and the
@@ -26,8 +27,9 @@ from pdal_amd import qp as Q
 
 ROOT = Path(__file__).resolve().parent.parent
 LIB = ROOT / "pdal_amd" / "lib"
-RUNNERS = {"lin": LIB / "rhr_lin_cp_al", "planted": LIB / "rhr_fa_cp"}
-ORACLES = {"lin": (Q.LinCpAl, "liblin_cp_al_oracle.so"), "planted": (Q.PlantedFaCp, "libplanted_fa_cp_oracle.so")}
+RUNNERS = {"lin": LIB / "rhr_lin_cp_al", "planted_cp": LIB / "rhr_cp_al", "planted": LIB / "rhr_fa_cp"}
+ORACLES = {"lin": (Q.LinCpAl, "liblin_cp_al_oracle.so"), "planted_cp": (Q.PlantedCpAl, "libplanted_cp_al_oracle.so"),
+           "planted": (Q.PlantedFaCp, "libplanted_fa_cp_oracle.so")}
 
 EQTEST = textwrap.dedent("""\
     NAME          EQTEST
@@ -76,7 +78,9 @@ def test_runner_usage_and_argument_errors(which):
     assert RUNNERS[which].exists(), f"{RUNNERS[which].name} not built"
     r = run(which, "--help")
     assert r.returncode == 0 and "Usage:" in r.stderr and "This EQBOX build only accepts problems with Ax = b" in r.stderr
-    assert ("--sigma_init" in r.stderr) == (which == "lin")
+    assert ("--sigma_init" in r.stderr) == (which in ("lin", "planted_cp"))
+    assert ("--al_lambda" in r.stderr) == (which == "lin")
+    assert ("--trajectory_csv" in r.stderr) == (which in ("planted_cp", "planted"))
     assert ("--adaptive_al_sigma_rule" in r.stderr) == (which == "planted")
     assert "--sigma_update" not in r.stderr  # FA_CP's sigma rules are not flags of the variants
     r = run(which, "--opt_norm", "l1", "a", "b")
@@ -147,7 +151,7 @@ def test_runner_solves_eqtest(which, tmp_path):
                      params=oracle.default_params(eps_optimal_relative=1e-8, eps_feasible_relative=1e-8))
     assert o.status == "OPTIMAL"
     assert abs(pobj - o.primal_objective) <= 1e-6 * abs(o.primal_objective)
-    if which == "lin":  # no inner solve: the same iteration count
+    if which == "lin":  # no inner solve: the same iteration count (the others run a BB inner solve on EQTEST's Q)
         assert int(s["Iterations Count"]) == o.iterations
 
 

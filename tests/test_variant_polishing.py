@@ -1,8 +1,10 @@
-"""Feasibility polishing of the two QP variants: the linearized CP-AL
-(QP/LIN_CP_AL/src/cp_al_core_op.cu:839-1375, solver.cu:102-106) and the
-planted-KKT FA_CP (QP/PLANTED_FA_CP/src/fa_cp_core_op.cu:1009-1502, the same
-code as FA_CP's; solver.cu:187-190).  Both run FA_CP's two phases (primal:
-c = 0; dual: finite bounds 0) with their own iteration.  The linearized
+"""Feasibility polishing of the three QP variants: the linearized CP-AL
+(QP/LIN_CP_AL/src/cp_al_core_op.cu:839-1375, solver.cu:102-106), the
+planted-KKT CP-AL (QP/PLANTED_CP_AL/src/cp_al_core_op.cu:1003-1539, the
+linearized solver's code around its own iteration) and the planted-KKT FA_CP
+(QP/PLANTED_FA_CP/src/fa_cp_core_op.cu:1009-1502, the same code as FA_CP's;
+solver.cu:187-190).  All run FA_CP's two phases (primal: c = 0; dual: finite
+bounds 0) with their own iteration.  The linearized
 solver's phases never restart: its restart test (utils.cu:451-463) needs a
 finite reference fixed-point error, which a phase starts without and only a
 restart sets, so a phase is plain Halpern from its start point.
@@ -10,8 +12,8 @@ restart sets, so a phase is plain Halpern from its start point.
 Pins of the oracle (CPU), in numpy on the unscaled problem, on the LP-type
 instance of the FA_CP polishing tests after a 1e-3 main solve that does not
 meet them: the returned x's primal residual, its bounds and the returned y's
-reduced-cost residual, for the linearized solver's AL branch and the planted
-solver's plain branch.  Without the AL branch the linearized solver's
+reduced-cost residual, for the linearized solver's AL branch, both branches
+of the planted CP-AL and the planted FA_CP's plain branch.  Without the AL branch the linearized solver's
 unrestarted phases do not reach 1e-8 within the limit: both run to it and the
 main solution comes back unchanged.  The GPU path is held to the oracle.
 """
@@ -32,6 +34,9 @@ CASES = {  # name: (oracle loader, GPU loader, parameter overrides)
     "lin_al": (lambda: Q.LinCpAl(ORACLE / "liblin_cp_al_oracle.so"), Q.load_lin_cp_al, {}),
     "planted_plain": (lambda: Q.PlantedFaCp(ORACLE / "libplanted_fa_cp_oracle.so"), Q.load_planted_fa_cp,
                       {"use_al_qp": False}),
+    "planted_cp_al_al": (lambda: Q.PlantedCpAl(ORACLE / "libplanted_cp_al_oracle.so"), Q.load_planted_cp_al, {}),
+    "planted_cp_al_plain": (lambda: Q.PlantedCpAl(ORACLE / "libplanted_cp_al_oracle.so"), Q.load_planted_cp_al,
+                            {"use_al_qp": False}),
 }
 LIMIT = 3000
 
