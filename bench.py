@@ -67,10 +67,13 @@ def fused_dual_bytes(nnz: int, n: int, m: int, segments: int = 1) -> int:
     return 12 * nnz + 4 * (m + segments) + 8 * n + 40 * m
 
 
-def shared_primal_bytes(nnz: int, n: int, m: int, count: int) -> int:
+def shared_primal_bytes(nnz: int, n: int, m: int, count: int, shared_bounds: bool) -> int:
     # shared-A At.y pass: the one At streamed once (12 B per nonzero, 4 B per
     # row pointer), every instance's y gathered once (8 m per instance), and the
-    # epilogue per instance: read c, lb, ub, x0, x, write xR, x (7 x 8 B per row)
+    # epilogue per instance: read c, lb, ub, x0, x, write xR, x (7 x 8 B per row);
+    # with the bounds held once for all instances, lb and ub are 2 x 8 B per row
+    if shared_bounds:
+        return 12 * nnz + 4 * (n + 1) + 8 * m * count + 40 * n * count + 16 * n
     return 12 * nnz + 4 * (n + 1) + 8 * m * count + 56 * n * count
 
 
@@ -265,13 +268,13 @@ def run_config4(args, lib) -> dict:
     log(f"held-out batch set-up {setup_s:.2f} s (wall {time.perf_counter() - t0:.2f} s)")
     kt = batch.time_kernels(args.kernel_iters)
     nnz1, n1, m1 = kt["nnz"] // count, kt["n"] // count, kt["m"] // count
-    bp = shared_primal_bytes(nnz1, n1, m1, count)
+    bp = shared_primal_bytes(nnz1, n1, m1, count, st["shared_bounds"])
     traffic = measured_traffic("shared_at_pass", "shared_spmv_fixed_a_family_256_instances.md") \
-        if count == 256 else None
+        if count == 256 and st["shared_bounds"] else None
     roof = roofline(bp, kt["primal_ms"], "shared-A At.y pass: sh_primal_kernel<4> (one At, 256 instances side by "
-                    "side, fused primal update)", traffic)
+                    "side, bounds shared, fused primal update)", traffic)
     log(f"shared kernels: primal {kt['primal_ms']:.3f} ms ({roof['achieved']:.0f} GB/s), dual {kt['dual_ms']:.3f} ms")
-    result = {"roofline": roof, "setup_s": round(setup_s, 3),
+    result = {"roofline": roof, "setup_s": round(setup_s, 3), "shared_bounds": st["shared_bounds"],
               "shared_kernel_ms": {"primal": kt["primal_ms"], "dual": kt["dual_ms"]}}
     if args.kernels_only:
         batch.close()
@@ -315,12 +318,15 @@ def run_config4(args, lib) -> dict:
 
     for w in range(args.warmup):
         warm_step()
-    times, fallbacks = [], 0
+    times, fallbacks, splits = [], 0, []
     for s in range(args.steps):
         t1 = time.perf_counter()
         fallbacks += warm_step()
         times.append(time.perf_counter() - t1)
-        log(f"warm step {s}: {count} held-out solves in {times[-1]:.3f} s")
+        st = batch.stats()
+        splits.append((st["solve_s"], st["half_stopped_s"], st["p99_stopped_s"], st["handoffs"]))
+        log(f"warm step {s}: {count} held-out solves in {times[-1]:.3f} s (solve {st['solve_s']:.3f} s, "
+            f"half stopped {st['half_stopped_s']:.3f} s, 99 % {st['p99_stopped_s']:.3f} s)")
     res = [batch.result(k) for k in range(count)]
     warm_its = np.array([r.iterations for r in res])
     obj_err = max(abs(r.primal_objective - fam.optimal_value(t)) / (1.0 + abs(fam.optimal_value(t)))
@@ -343,6 +349,8 @@ def run_config4(args, lib) -> dict:
     result.update({
         "value": value, "ms_per_step": 1e3 * total / args.steps, "optimal": int(sum(r.status == "OPTIMAL" for r in res)),
         "fallbacks": fallbacks, "warm_iterations": {"mean": float(warm_its.mean()), "max": int(warm_its.max())},
+        "warm_solve_split_s": {k: round(float(np.mean([sp[i] for sp in splits])), 4)
+                               for i, k in enumerate(("solve", "half_stopped", "p99_stopped", "handoffs"))},
         "cold_batched_solves_per_s": round(count / cold_s, 3),
         "cold_iterations": {"mean": float(cold_its.mean()), "max": int(cold_its.max())},
         "plain_solves_per_s": round(plain, 3), "plain_sample": args.plain_sample,

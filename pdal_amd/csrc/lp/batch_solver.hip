@@ -163,6 +163,10 @@ class BatchSolver {
     hipGraphExec_t sh_graph_ = nullptr;    // full width, set 0
     hipGraphExec_t sh_graph_c_ = nullptr;  // the current packed layout
     bool same_a_ = false;  // every instance has the first's constraint matrix
+    // shared-A engine: every instance's scaled variable bounds are the first's,
+    // held once per column (sh_lb_, sh_ub_) instead of side by side
+    bool sb_ = false;
+    DBuf<double> sh_lb_, sh_ub_;
     bool same_matrices() const;
     void detect_shared();
     void sh_alloc(int set, int width);
@@ -497,6 +501,29 @@ void BatchSolver::detect_shared()
     if (nk > 8) return;  // up to 512 instances side by side
     NK_full_ = NK_ = nk;
     Bp_full_ = Bp_ = nk * kWave;
+    // shared bounds: the scaled lb and ub of every instance equal the first's bit for bit
+    sb_ = false;
+    if (!(getenv("PDAL_BATCH_NO_SHARED_BOUNDS") && atoi(getenv("PDAL_BATCH_NO_SHARED_BOUNDS")) != 0)) {
+        const long n = n_ / N_;
+        DBuf<int> flag(1);
+        PinnedBuf<int> hflag{1};
+        PDAL_HIP(hipMemsetAsync(flag.get(), 0, sizeof(int), stream_));
+        const int nb = std::max(1, std::min(4096, blocks_for_l(n_ - n)));
+        for (const double *cat : {lb_.get(), ub_.get()})
+            sh_same_rows_kernel<<<nb, kBlock, 0, stream_>>>(cat, n, n_, flag.get());
+        PDAL_LAUNCHED();
+        PDAL_HIP(hipMemcpyAsync(hflag.get(), flag.get(), sizeof(int), hipMemcpyDeviceToHost, stream_));
+        PDAL_HIP(hipStreamSynchronize(stream_));
+        sb_ = hflag[0] == 0;
+        if (sb_) {
+            sh_lb_.alloc(n);
+            sh_ub_.alloc(n);
+            PDAL_HIP(hipMemcpyAsync(sh_lb_.get(), lb_.get(), (size_t)n * sizeof(double), hipMemcpyDeviceToDevice,
+                                    stream_));
+            PDAL_HIP(hipMemcpyAsync(sh_ub_.get(), ub_.get(), (size_t)n * sizeof(double), hipMemcpyDeviceToDevice,
+                                    stream_));
+        }
+    }
     sh_alloc(0, Bp_);
     sh_parts_.alloc((size_t)kShEvalBlocks * std::max(kAtQ, kAQ) * Bp_);
     for (auto &ids : sh_ids_) ids.alloc(Bp_);
@@ -511,7 +538,9 @@ void BatchSolver::sh_alloc(int set, int width)
 {
     const size_t xn = (size_t)(n_ / N_) * width, ym = (size_t)(m_ / N_) * width;
     SharedBufs &b = sh_[set];
-    for (auto *q : {&b.c, &b.lb, &b.ub, &b.x0, &b.x, &b.xT, &b.xH, &b.xR, &b.ds}) q->alloc(xn);
+    for (auto *q : {&b.c, &b.x0, &b.x, &b.xT, &b.xH, &b.xR, &b.ds}) q->alloc(xn);
+    if (!sb_)
+        for (auto *q : {&b.lb, &b.ub}) q->alloc(xn);
     for (auto *q : {&b.l, &b.u, &b.y0, &b.y, &b.yT, &b.yH, &b.yR}) q->alloc(ym);
     sh_width_[set] = width;
 }
@@ -519,7 +548,8 @@ void BatchSolver::sh_alloc(int set, int width)
 ShVecs BatchSolver::sh_vecs()
 {
     SharedBufs &b = sh_[cur_];
-    return ShVecs{b.c.get(), b.lb.get(), b.ub.get(), b.l.get(),  b.u.get(),  b.x0.get(), b.x.get(),  b.xT.get(),
+    const double *lb = sb_ ? sh_lb_.get() : b.lb.get(), *ub = sb_ ? sh_ub_.get() : b.ub.get();
+    return ShVecs{b.c.get(), lb, ub, b.l.get(),  b.u.get(),  b.x0.get(), b.x.get(),  b.xT.get(),
                   b.xH.get(), b.xR.get(), b.ds.get(), b.y0.get(), b.y.get(), b.yT.get(), b.yH.get(), b.yR.get()};
 }
 
@@ -548,8 +578,13 @@ void BatchSolver::sh_iteration(int i, bool major)
         constexpr int NK = decltype(nk)::value;
         if (pred_) sh_predictor_kernel<NK><<<ga, kBlock, 0, stream_>>>(A, v, st_.get(), ids, Bp_);
         const double *yv = pred_ ? v.yR : v.y;
-        if (major) sh_primal_kernel<NK, true><<<gat, kBlock, 0, stream_>>>(At, yv, v, st_.get(), ids, Bp_, rho_, i);
-        else sh_primal_kernel<NK, false><<<gat, kBlock, 0, stream_>>>(At, yv, v, st_.get(), ids, Bp_, rho_, i);
+        if (sb_) {
+            if (major) sh_primal_kernel<NK, true, true><<<gat, kBlock, 0, stream_>>>(At, yv, v, st_.get(), ids, Bp_, rho_, i);
+            else sh_primal_kernel<NK, false, true><<<gat, kBlock, 0, stream_>>>(At, yv, v, st_.get(), ids, Bp_, rho_, i);
+        } else {
+            if (major) sh_primal_kernel<NK, true, false><<<gat, kBlock, 0, stream_>>>(At, yv, v, st_.get(), ids, Bp_, rho_, i);
+            else sh_primal_kernel<NK, false, false><<<gat, kBlock, 0, stream_>>>(At, yv, v, st_.get(), ids, Bp_, rho_, i);
+        }
         if (major) sh_dual_kernel<NK, true><<<ga, kBlock, 0, stream_>>>(A, v, st_.get(), ids, Bp_, rho_, i);
         else sh_dual_kernel<NK, false><<<ga, kBlock, 0, stream_>>>(A, v, st_.get(), ids, Bp_, rho_, i);
     });
@@ -612,8 +647,10 @@ void BatchSolver::sh_begin()
     };
     SharedBufs &b = sh_[0];
     gather(c_.get(), b.c.get(), n);
-    gather(lb_.get(), b.lb.get(), n);
-    gather(ub_.get(), b.ub.get(), n);
+    if (!sb_) {
+        gather(lb_.get(), b.lb.get(), n);
+        gather(ub_.get(), b.ub.get(), n);
+    }
     gather(l_.get(), b.l.get(), m);
     gather(u_.get(), b.u.get(), m);
     gather(x0_.get(), b.x0.get(), n);
@@ -711,10 +748,12 @@ int BatchSolver::sh_compact()
         sh_compact_kernel<<<blocks_for_l(rows * width), kBlock, 0, stream_>>>(in.get(), out.get(), rows, Bp_, width,
                                                                              d_src.get(), count);
     };
-    for (auto pr : {std::make_pair(&a.c, &b.c), std::make_pair(&a.lb, &b.lb), std::make_pair(&a.ub, &b.ub),
-                    std::make_pair(&a.x0, &b.x0), std::make_pair(&a.x, &b.x), std::make_pair(&a.xT, &b.xT),
-                    std::make_pair(&a.xH, &b.xH), std::make_pair(&a.xR, &b.xR), std::make_pair(&a.ds, &b.ds)})
+    for (auto pr : {std::make_pair(&a.c, &b.c), std::make_pair(&a.x0, &b.x0), std::make_pair(&a.x, &b.x),
+                    std::make_pair(&a.xT, &b.xT), std::make_pair(&a.xH, &b.xH), std::make_pair(&a.xR, &b.xR),
+                    std::make_pair(&a.ds, &b.ds)})
         pack(*pr.first, *pr.second, n);
+    if (!sb_)
+        for (auto pr : {std::make_pair(&a.lb, &b.lb), std::make_pair(&a.ub, &b.ub)}) pack(*pr.first, *pr.second, n);
     for (auto pr : {std::make_pair(&a.l, &b.l), std::make_pair(&a.u, &b.u), std::make_pair(&a.y0, &b.y0),
                     std::make_pair(&a.y, &b.y), std::make_pair(&a.yT, &b.yT), std::make_pair(&a.yH, &b.yH),
                     std::make_pair(&a.yR, &b.yR)})
@@ -1457,8 +1496,10 @@ void BatchSolver::time_kernels(int iters, double *out)
             for (int which = 0; which < 2; ++which) {
                 sh_dispatch(NK_, [&](auto nk) {
                     constexpr int NK = decltype(nk)::value;
-                    if (which == 0)
-                        sh_primal_kernel<NK, false><<<gat, kBlock, 0, stream_>>>(At, v.y, v, st_.get(), ids, Bp_, rho_, it);
+                    if (which == 0 && sb_)
+                        sh_primal_kernel<NK, false, true><<<gat, kBlock, 0, stream_>>>(At, v.y, v, st_.get(), ids, Bp_, rho_, it);
+                    else if (which == 0)
+                        sh_primal_kernel<NK, false, false><<<gat, kBlock, 0, stream_>>>(At, v.y, v, st_.get(), ids, Bp_, rho_, it);
                     else
                         sh_dual_kernel<NK, false><<<ga, kBlock, 0, stream_>>>(A, v, st_.get(), ids, Bp_, rho_, it);
                 });
@@ -1574,6 +1615,7 @@ void BatchSolver::stats(double *out) const
     out[13] = sx;
     out[14] = shared_ ? 1.0 : 0.0;
     out[15] = (double)handoffs_;
+    out[16] = shared_ && sb_ ? 1.0 : 0.0;
 }
 
 }  // namespace batch

@@ -124,6 +124,39 @@ def test_shared_packing_is_bitwise_neutral(gpu_lp, monkeypatch, count):
     assert max(its) > min(its)  # the instances stop at different blocks
 
 
+@pytest.mark.parametrize("mixed", [False, True])
+def test_shared_bounds_are_bitwise_neutral(gpu_lp, monkeypatch, mixed):
+    """Variable bounds held once for all instances (their scaled bounds equal
+    bit for bit) give the same bits as the side-by-side layout.  With one
+    instance's upper bound changed (mixed) the batch must not share them, and
+    both runs take the side-by-side layout."""
+    monkeypatch.setenv("CP_AL_LAMBDA", "0")
+    from pdal_amd.generators import planted_family
+    fam = planted_family(150, 300, 8, seed=6)
+    insts = [fam.instance(t) for t in config4_thetas(100, 9)]
+    if mixed:
+        ub = insts[3]["var_ub"].copy()
+        ub[: 20] = 0.5
+        insts[3] = dict(insts[3], var_ub=ub)
+    params = gpu_lp.default_params(termination_evaluation_frequency=K, eps_optimal_relative=1e-6,
+                                   eps_feasible_relative=1e-6, iteration_limit=20000)
+    monkeypatch.setenv("PDAL_BATCH_G_A", "16")
+    monkeypatch.setenv("PDAL_BATCH_G_AT", "8")
+    runs = []
+    for flag in ("0", "1"):
+        monkeypatch.setenv("PDAL_BATCH_NO_SHARED_BOUNDS", flag)
+        print(f"[case] shared bounds mixed={mixed} no_shared_bounds={flag}", flush=True)
+        with LpBatch(gpu_lp, insts, params) as b:
+            st = b.stats()
+            assert st["shared_a"] and st["shared_bounds"] == (flag == "0" and not mixed)
+            b.solve()
+            runs.append([b.result(k) for k in range(len(insts))])
+    for a, c in zip(*runs):
+        assert a.status == c.status and a.iterations == c.iterations
+        assert np.array_equal(a.x, c.x) and np.array_equal(a.y, c.y)
+    assert sum(r.status == "OPTIMAL" for r in runs[0]) >= len(insts) - 1
+
+
 def test_shared_handoff_tracks_oracle(oracle_lp, gpu_lp, monkeypatch):
     """The last running instances (8 or fewer) finish on the block-diagonal
     engine: each instance still follows the oracle's solve (same iteration
