@@ -55,7 +55,7 @@ int pdal_cp_al_batch_solve(pdal_cp_al_batch_t *batch);
  * NULL before the first solve or for an invalid index. */
 cp_al_result_t *pdal_cp_al_batch_result(pdal_cp_al_batch_t *batch, int i);
 
-/* out[0..16] = {set-up seconds, last solve seconds (wall clock, device
+/* out[0..17] = {set-up seconds, last solve seconds (wall clock, device
  * resident), hipGraph replays, instances, OPTIMAL instances, sum of
  * iteration counts, lanes per row of A, lanes per row of At, block-table
  * compactions (stopped instances are dropped from the launch tables and the
@@ -70,7 +70,9 @@ cp_al_result_t *pdal_cp_al_batch_result(pdal_cp_al_batch_t *batch, int i);
  * layouts), 1 when the last shared-A solve handed its last few running
  * instances (8 or fewer) to the block-diagonal engine, 1 when the shared-A
  * engine holds the variable bounds once for all instances (their scaled
- * bounds are equal bit for bit)}.  `out` holds 17 doubles. */
+ * bounds are equal bit for bit), 1 when its dual pass takes every row as an
+ * equality (l = u bit for bit in every instance: the projection onto [l, u]
+ * is l)}.  `out` holds 18 doubles. */
 void pdal_cp_al_batch_stats(const pdal_cp_al_batch_t *batch, double *out);
 
 /* Launch `iters` plain iterations over all instances kernel by kernel

@@ -167,6 +167,8 @@ class BatchSolver {
     // held once per column (sh_lb_, sh_ub_) instead of side by side
     bool sb_ = false;
     DBuf<double> sh_lb_, sh_ub_;
+    // shared-A engine: every row of every instance has l = u bit for bit
+    bool eq_ = false;
     bool same_matrices() const;
     void detect_shared();
     void sh_alloc(int set, int width);
@@ -501,28 +503,34 @@ void BatchSolver::detect_shared()
     if (nk > 8) return;  // up to 512 instances side by side
     NK_full_ = NK_ = nk;
     Bp_full_ = Bp_ = nk * kWave;
-    // shared bounds: the scaled lb and ub of every instance equal the first's bit for bit
-    sb_ = false;
-    if (!(getenv("PDAL_BATCH_NO_SHARED_BOUNDS") && atoi(getenv("PDAL_BATCH_NO_SHARED_BOUNDS")) != 0)) {
-        const long n = n_ / N_;
-        DBuf<int> flag(1);
-        PinnedBuf<int> hflag{1};
+    // shared bounds: the scaled lb and ub of every instance equal the first's bit
+    // for bit; equality rows: l = u bit for bit in every row of every instance
+    const auto env_off = [](const char *k) { return getenv(k) && atoi(getenv(k)) != 0; };
+    const long n = n_ / N_;
+    DBuf<int> flag(1);
+    PinnedBuf<int> hflag{1};
+    auto flagged = [&](auto &&launch) {
         PDAL_HIP(hipMemsetAsync(flag.get(), 0, sizeof(int), stream_));
-        const int nb = std::max(1, std::min(4096, blocks_for_l(n_ - n)));
-        for (const double *cat : {lb_.get(), ub_.get()})
-            sh_same_rows_kernel<<<nb, kBlock, 0, stream_>>>(cat, n, n_, flag.get());
+        launch();
         PDAL_LAUNCHED();
         PDAL_HIP(hipMemcpyAsync(hflag.get(), flag.get(), sizeof(int), hipMemcpyDeviceToHost, stream_));
         PDAL_HIP(hipStreamSynchronize(stream_));
-        sb_ = hflag[0] == 0;
-        if (sb_) {
-            sh_lb_.alloc(n);
-            sh_ub_.alloc(n);
-            PDAL_HIP(hipMemcpyAsync(sh_lb_.get(), lb_.get(), (size_t)n * sizeof(double), hipMemcpyDeviceToDevice,
-                                    stream_));
-            PDAL_HIP(hipMemcpyAsync(sh_ub_.get(), ub_.get(), (size_t)n * sizeof(double), hipMemcpyDeviceToDevice,
-                                    stream_));
-        }
+        return hflag[0] != 0;
+    };
+    sb_ = !env_off("PDAL_BATCH_NO_SHARED_BOUNDS") && !flagged([&] {
+        const int nb = std::max(1, std::min(4096, blocks_for_l(n_ - n)));
+        for (const double *cat : {lb_.get(), ub_.get()})
+            sh_same_rows_kernel<<<nb, kBlock, 0, stream_>>>(cat, n, n_, flag.get());
+    });
+    eq_ = !env_off("PDAL_BATCH_NO_EQ_ROWS") && !flagged([&] {
+        sh_differ_kernel<<<std::max(1, std::min(4096, blocks_for_l(m_))), kBlock, 0, stream_>>>(l_.get(), u_.get(),
+                                                                                                   m_, flag.get());
+    });
+    if (sb_) {
+        sh_lb_.alloc(n);
+        sh_ub_.alloc(n);
+        PDAL_HIP(hipMemcpyAsync(sh_lb_.get(), lb_.get(), (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+        PDAL_HIP(hipMemcpyAsync(sh_ub_.get(), ub_.get(), (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, stream_));
     }
     sh_alloc(0, Bp_);
     sh_parts_.alloc((size_t)kShEvalBlocks * std::max(kAtQ, kAQ) * Bp_);
@@ -585,8 +593,13 @@ void BatchSolver::sh_iteration(int i, bool major)
             if (major) sh_primal_kernel<NK, true, false><<<gat, kBlock, 0, stream_>>>(At, yv, v, st_.get(), ids, Bp_, rho_, i);
             else sh_primal_kernel<NK, false, false><<<gat, kBlock, 0, stream_>>>(At, yv, v, st_.get(), ids, Bp_, rho_, i);
         }
-        if (major) sh_dual_kernel<NK, true><<<ga, kBlock, 0, stream_>>>(A, v, st_.get(), ids, Bp_, rho_, i);
-        else sh_dual_kernel<NK, false><<<ga, kBlock, 0, stream_>>>(A, v, st_.get(), ids, Bp_, rho_, i);
+        if (eq_) {
+            if (major) sh_dual_kernel<NK, true, true><<<ga, kBlock, 0, stream_>>>(A, v, st_.get(), ids, Bp_, rho_, i);
+            else sh_dual_kernel<NK, false, true><<<ga, kBlock, 0, stream_>>>(A, v, st_.get(), ids, Bp_, rho_, i);
+        } else {
+            if (major) sh_dual_kernel<NK, true, false><<<ga, kBlock, 0, stream_>>>(A, v, st_.get(), ids, Bp_, rho_, i);
+            else sh_dual_kernel<NK, false, false><<<ga, kBlock, 0, stream_>>>(A, v, st_.get(), ids, Bp_, rho_, i);
+        }
     });
     PDAL_LAUNCHED();
 }
@@ -1500,8 +1513,10 @@ void BatchSolver::time_kernels(int iters, double *out)
                         sh_primal_kernel<NK, false, true><<<gat, kBlock, 0, stream_>>>(At, v.y, v, st_.get(), ids, Bp_, rho_, it);
                     else if (which == 0)
                         sh_primal_kernel<NK, false, false><<<gat, kBlock, 0, stream_>>>(At, v.y, v, st_.get(), ids, Bp_, rho_, it);
+                    else if (eq_)
+                        sh_dual_kernel<NK, false, true><<<ga, kBlock, 0, stream_>>>(A, v, st_.get(), ids, Bp_, rho_, it);
                     else
-                        sh_dual_kernel<NK, false><<<ga, kBlock, 0, stream_>>>(A, v, st_.get(), ids, Bp_, rho_, it);
+                        sh_dual_kernel<NK, false, false><<<ga, kBlock, 0, stream_>>>(A, v, st_.get(), ids, Bp_, rho_, it);
                 });
                 PDAL_HIP(hipEventRecord(e[which + 1], stream_));
             }
@@ -1616,6 +1631,7 @@ void BatchSolver::stats(double *out) const
     out[14] = shared_ ? 1.0 : 0.0;
     out[15] = (double)handoffs_;
     out[16] = shared_ && sb_ ? 1.0 : 0.0;
+    out[17] = shared_ && eq_ ? 1.0 : 0.0;
 }
 
 }  // namespace batch

@@ -477,13 +477,14 @@ class LpBatch:
         return self.lib._take_result(res, self.probs[i])
 
     def stats(self) -> dict:
-        out = (C.c_double * 17)()
+        out = (C.c_double * 18)()
         self.lib.lib.pdal_cp_al_batch_stats(self.h, out)
         return dict(setup_s=out[0], solve_s=out[1], replays=int(out[2]), instances=int(out[3]),
                     optimal=int(out[4]), iterations=int(out[5]), lanes_a=int(out[6]), lanes_at=int(out[7]),
                     compactions=int(out[8]), half_stopped_s=out[9], p99_stopped_s=out[10],
                     staged_at=int(out[11]), staged_a=int(out[12]), split_a=int(out[13]),
-                    shared_a=bool(out[14]), handoffs=int(out[15]), shared_bounds=bool(out[16]))
+                    shared_a=bool(out[14]), handoffs=int(out[15]), shared_bounds=bool(out[16]),
+                    equality_rows=bool(out[17]))
 
     def primal_weights(self) -> np.ndarray:
         out = np.empty(len(self.sizes), dtype=np.float64)

@@ -125,6 +125,40 @@ def test_shared_packing_is_bitwise_neutral(gpu_lp, monkeypatch, count):
 
 
 @pytest.mark.parametrize("mixed", [False, True])
+def test_equality_rows_are_bitwise_neutral(gpu_lp, monkeypatch, mixed):
+    """Rows with l = u in every instance: the dual pass takes the projection
+    onto [l, u] as l itself, with the same bits as the general projection.
+    With one instance's row made a range (mixed) the batch must not, and both
+    runs take the general projection."""
+    monkeypatch.setenv("CP_AL_LAMBDA", "0")
+    from pdal_amd.generators import planted_family
+    fam = planted_family(150, 300, 8, seed=6)
+    insts = [fam.instance(t) for t in config4_thetas(100, 10)]
+    assert all(np.array_equal(i["con_lb"], i["con_ub"]) for i in insts)
+    if mixed:
+        u = insts[5]["con_ub"].copy()
+        u[7] += 1.0
+        insts[5] = dict(insts[5], con_ub=u)
+    params = gpu_lp.default_params(termination_evaluation_frequency=K, eps_optimal_relative=1e-6,
+                                   eps_feasible_relative=1e-6, iteration_limit=20000)
+    monkeypatch.setenv("PDAL_BATCH_G_A", "16")
+    monkeypatch.setenv("PDAL_BATCH_G_AT", "8")
+    runs = []
+    for flag in ("0", "1"):
+        monkeypatch.setenv("PDAL_BATCH_NO_EQ_ROWS", flag)
+        print(f"[case] equality rows mixed={mixed} no_eq_rows={flag}", flush=True)
+        with LpBatch(gpu_lp, insts, params) as b:
+            st = b.stats()
+            assert st["shared_a"] and st["equality_rows"] == (flag == "0" and not mixed)
+            b.solve()
+            runs.append([b.result(k) for k in range(len(insts))])
+    for a, c in zip(*runs):
+        assert a.status == c.status and a.iterations == c.iterations
+        assert np.array_equal(a.x, c.x) and np.array_equal(a.y, c.y)
+    assert sum(r.status == "OPTIMAL" for r in runs[0]) >= len(insts) - 1
+
+
+@pytest.mark.parametrize("mixed", [False, True])
 def test_shared_bounds_are_bitwise_neutral(gpu_lp, monkeypatch, mixed):
     """Variable bounds held once for all instances (their scaled bounds equal
     bit for bit) give the same bits as the side-by-side layout.  With one
