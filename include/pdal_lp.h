@@ -51,6 +51,14 @@ const char *termination_reason_to_string(termination_reason_t reason);
  * HBM traffic of the solver kernels is scaled. */
 double pdal_copy_calibration(long long bytes, int reps);
 
+/* Gather calibration: `count` FP64 loads at uniformly random indices of a
+ * `len`-entry device vector (4-byte indices streamed, one running sum per
+ * thread), `reps` times; returns the average kernel time in ms (hipEvents),
+ * or a negative value on error.  It is the floor of an SpMV pass whose column
+ * indices are uniform: that pass makes the same loads and streams the values
+ * and the result besides. */
+double pdal_gather_calibration(long long count, long long len, int reps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -479,6 +479,16 @@ PDAL_API void pdal_cp_al_session_free(void *session)
     if (session) session_free(session);
 }
 
+PDAL_API double pdal_gather_calibration(long long count, long long len, int reps)
+{
+    try {
+        return gather_calibration(count, len, reps);
+    } catch (const std::exception &e) {
+        fprintf(stderr, "[cp_al-mi355x] gather calibration failed: %s\n", e.what());
+        return -2.0;
+    }
+}
+
 PDAL_API double pdal_copy_calibration(long long bytes, int reps)
 {
     try {

@@ -34,6 +34,7 @@ void session_free(void *session);
 
 // Plain copy kernel timing for counter calibration (pdal_lp.h).
 double copy_calibration(long long bytes, int reps);
+double gather_calibration(long long count, long long len, int reps);
 
 // Batched solver (pdal_batch.h).
 void *batch_create(lp_problem_t *const *problems, int count, const cp_al_parameters_t *params);

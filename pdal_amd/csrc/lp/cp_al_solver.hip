@@ -1266,5 +1266,49 @@ double copy_calibration(long long bytes, int reps)
     return ms / reps;
 }
 
+__global__ void __launch_bounds__(kBlock) gather_index_kernel(int *__restrict__ idx, long count, long len)
+{   // uniform indices in [0, len) from a 64-bit mix of the position (splitmix64 finalizer)
+    for (long k = (long)blockIdx.x * kBlock + threadIdx.x; k < count; k += (long)gridDim.x * kBlock) {
+        unsigned long long z = (unsigned long long)k + 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        idx[k] = (int)(z % (unsigned long long)len);
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) gather_calibration_kernel(const int *__restrict__ idx,
+                                                                    const double *__restrict__ v, long count,
+                                                                    double *__restrict__ out)
+{
+    double acc = 0.0;
+    for (long k = (long)blockIdx.x * kBlock + threadIdx.x; k < count; k += (long)gridDim.x * kBlock) acc += v[idx[k]];
+    out[(long)blockIdx.x * kBlock + threadIdx.x] = acc;
+}
+
+double gather_calibration(long long count, long long len, int reps)
+{
+    if (count <= 0 || len <= 0 || len > INT32_MAX || reps <= 0) return -1.0;
+    const int grid = 256 * 16;
+    DBuf<int> idx(count);
+    DBuf<double> v(len), out((size_t)grid * kBlock);
+    PDAL_HIP(hipMemset(v.get(), 0, (size_t)len * sizeof(double)));
+    gather_index_kernel<<<grid, kBlock>>>(idx.get(), count, len);
+    hipEvent_t e0, e1;
+    PDAL_HIP(hipEventCreate(&e0));
+    PDAL_HIP(hipEventCreate(&e1));
+    gather_calibration_kernel<<<grid, kBlock>>>(idx.get(), v.get(), count, out.get());
+    PDAL_HIP(hipEventRecord(e0, nullptr));
+    for (int r = 0; r < reps; ++r) gather_calibration_kernel<<<grid, kBlock>>>(idx.get(), v.get(), count, out.get());
+    PDAL_HIP(hipEventRecord(e1, nullptr));
+    PDAL_LAUNCHED();
+    PDAL_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    PDAL_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ms / reps;
+}
+
 }  // namespace lp
 }  // namespace pdal

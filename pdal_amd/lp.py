@@ -226,6 +226,9 @@ class CpAl:
             L.pdal_cp_al_session_setup_seconds.argtypes = [C.c_void_p]
             L.pdal_cp_al_session_free.restype = None
             L.pdal_cp_al_session_free.argtypes = [C.c_void_p]
+        if hasattr(L, "pdal_gather_calibration"):
+            L.pdal_gather_calibration.restype = C.c_double
+            L.pdal_gather_calibration.argtypes = [C.c_longlong, C.c_longlong, C.c_int]
         if hasattr(L, "pdal_copy_calibration"):
             L.pdal_copy_calibration.restype = C.c_double
             L.pdal_copy_calibration.argtypes = [C.c_longlong, C.c_int]
