@@ -23,7 +23,10 @@ void pdal_cp_al_reset_rng(void);
  * fused SpMV kernel with hipEvents on the solver's stream.  out[0..6] =
  * {avg ms primal (At.y + primal update), avg ms dual (A.x_bar + dual update),
  * avg ms predictor (0 if beta == 0), avg ms per iteration, ms of one
- * evaluation pass, lanes per row for At, lanes per row for A}.
+ * evaluation pass, lanes per row for At, lanes per row for A}; a negative
+ * lane count -S means the pass runs over the column-sliced copy of the matrix
+ * with S column slices (one workgroup per row range; taken when the gathered
+ * vector is far larger than an XCD's L2 and that pass is faster).
  * Returns 0 on success, negative on error. */
 int pdal_cp_al_time_kernels(const lp_problem_t *prob, const cp_al_parameters_t *params, int iters, double *out);
 

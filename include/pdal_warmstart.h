@@ -63,9 +63,10 @@ void pdal_pod_free(pdal_pod_t *pod);
 
 /* out (batch x dim float64, a DEVICE address on the current device, e.g. the
  * start buffer of a batch: pdal_cp_al_batch_start_buffers) = clip(mean +
- * coef . basis), coef (batch x k float32) in host memory.  Returns when the
- * expansion has finished. */
-int pdal_pod_expand_device(pdal_pod_t *pod, const float *coef, int batch, double *out);
+ * coef . basis), coef (batch x k float32) in host memory.  out_capacity is
+ * the number of doubles `out` holds; a call whose batch x dim exceeds it
+ * writes nothing and returns -1.  Returns when the expansion has finished. */
+int pdal_pod_expand_device(pdal_pod_t *pod, const float *coef, int batch, double *out, long long out_capacity);
 
 #ifdef __cplusplus
 }

@@ -14,6 +14,7 @@
 #pragma once
 
 #include "../common/device_reduce.hpp"
+#include "../common/sliced_csr.hpp"
 #include "../common/spmv.hpp"
 
 namespace pdal {
@@ -46,15 +47,9 @@ struct PrimalArgs {
 // At.y SpMV fused with the primal prox / projection / reflection / Halpern
 // update.  Reference: compute_next_primal_solution(_major)_kernel,
 // LP/CP_AL/src/solver.cu:2340-2397, launched after the At.y SpMV at :2561-2570.
-template <int G, bool MAJOR>
-static __global__ void __launch_bounds__(kBlock) primal_kernel(PrimalArgs a, int k_offset)
+template <bool MAJOR>
+__device__ __forceinline__ void primal_update(const PrimalArgs &a, int row, double s, int k_offset)
 {
-    const int tid = blockIdx.x * kBlock + threadIdx.x;
-    const int row = tid / G;
-    const int lane = tid & (G - 1);
-    if (row >= a.At.rows) return;  // whole groups exit together (kBlock % G == 0)
-    const double s = row_dot<G>(a.At, row, lane, a.yv);
-    if (lane != 0) return;
     const double tau = a.sc->tau;
     const int k = a.sc->inner + k_offset;
     const double w = (double)k / (double)(k + 1);
@@ -71,6 +66,32 @@ static __global__ void __launch_bounds__(kBlock) primal_kernel(PrimalArgs a, int
     a.xR[row] = xr;
     const double refl = a.rho * xr + (1.0 - a.rho) * xj;
     a.x[row] = w * refl + (1.0 - w) * x0;
+}
+
+template <int G, bool MAJOR>
+static __global__ void __launch_bounds__(kBlock) primal_kernel(PrimalArgs a, int k_offset)
+{
+    const int tid = blockIdx.x * kBlock + threadIdx.x;
+    const int row = tid / G;
+    const int lane = tid & (G - 1);
+    if (row >= a.At.rows) return;  // whole groups exit together (kBlock % G == 0)
+    const double s = row_dot<G>(a.At, row, lane, a.yv);
+    if (lane != 0) return;
+    primal_update<MAJOR>(a, row, s, k_offset);
+}
+
+// The same pass over the column-sliced copy of At (one workgroup per row
+// range, the gathered y slice resident in the XCD's L2; row sums in LDS).
+template <int U, bool MAJOR>
+static __global__ void __launch_bounds__(kSliceBlock) primal_sliced_kernel(PrimalArgs a, SlicedView S, int k_offset)
+{
+    extern __shared__ double acc[];
+    sliced_row_sums<U>(S, blockIdx.x, a.yv, acc);
+    const int rw = kSliceWavesPerGroup * S.rpw;
+    for (int j = threadIdx.x; j < rw; j += kSliceBlock) {
+        const long row = (long)blockIdx.x * rw + j;
+        if (row < S.rows) primal_update<MAJOR>(a, (int)row, acc[j], k_offset);
+    }
 }
 
 struct DualArgs {
@@ -90,15 +111,9 @@ struct DualArgs {
 // A.x_bar SpMV fused with the dual prox on [l,u] with denominator sigma+beta,
 // reflection and Halpern update.  Reference: compute_next_dual_solution(_major)
 // _kernel, solver.cu:2399-2457, after the A.x_bar SpMV at :2617-2626.
-template <int G, bool MAJOR>
-static __global__ void __launch_bounds__(kBlock) dual_kernel(DualArgs a, int k_offset)
+template <bool MAJOR>
+__device__ __forceinline__ void dual_update(const DualArgs &a, int row, double pp, int k_offset)
 {
-    const int tid = blockIdx.x * kBlock + threadIdx.x;
-    const int row = tid / G;
-    const int lane = tid & (G - 1);
-    if (row >= a.A.rows) return;
-    const double pp = row_dot<G>(a.A, row, lane, a.xR);
-    if (lane != 0) return;
     const double sigma = a.sc->sigma;
     const double prox = sigma + a.sc->beta;
     const int k = a.sc->inner + k_offset;
@@ -116,6 +131,31 @@ static __global__ void __launch_bounds__(kBlock) dual_kernel(DualArgs a, int k_o
     a.y[row] = w * refl + (1.0 - w) * y0;
 }
 
+template <int G, bool MAJOR>
+static __global__ void __launch_bounds__(kBlock) dual_kernel(DualArgs a, int k_offset)
+{
+    const int tid = blockIdx.x * kBlock + threadIdx.x;
+    const int row = tid / G;
+    const int lane = tid & (G - 1);
+    if (row >= a.A.rows) return;
+    const double pp = row_dot<G>(a.A, row, lane, a.xR);
+    if (lane != 0) return;
+    dual_update<MAJOR>(a, row, pp, k_offset);
+}
+
+// The same pass over the column-sliced copy of A (x_bar slice in L2).
+template <int U, bool MAJOR>
+static __global__ void __launch_bounds__(kSliceBlock) dual_sliced_kernel(DualArgs a, SlicedView S, int k_offset)
+{
+    extern __shared__ double acc[];
+    sliced_row_sums<U>(S, blockIdx.x, a.xR, acc);
+    const int rw = kSliceWavesPerGroup * S.rpw;
+    for (int j = threadIdx.x; j < rw; j += kSliceBlock) {
+        const long row = (long)blockIdx.x * rw + j;
+        if (row < S.rows) dual_update<MAJOR>(a, (int)row, acc[j], k_offset);
+    }
+}
+
 struct PredictorArgs {
     CsrView A;
     const double *__restrict__ x;
@@ -128,6 +168,18 @@ struct PredictorArgs {
 
 // A.x SpMV fused with the box-AL dual predictor (only launched when beta > 0).
 // Reference: compute_box_al_predictor_kernel, solver.cu:2459-2479, after :2533-2542.
+__device__ __forceinline__ void predictor_update(const PredictorArgs &a, int row, double pp)
+{
+    const double beta = a.sc->beta;
+    const double yi = a.y[row];
+    if (beta <= 0.0) {
+        a.out[row] = yi;
+        return;
+    }
+    const double z = fmax(a.l[row], fmin(pp - yi / beta, a.u[row]));
+    a.out[row] = yi - beta * (pp - z);
+}
+
 template <int G>
 static __global__ void __launch_bounds__(kBlock) predictor_kernel(PredictorArgs a)
 {
@@ -137,14 +189,19 @@ static __global__ void __launch_bounds__(kBlock) predictor_kernel(PredictorArgs 
     if (row >= a.A.rows) return;
     const double pp = row_dot<G>(a.A, row, lane, a.x);
     if (lane != 0) return;
-    const double beta = a.sc->beta;
-    const double yi = a.y[row];
-    if (beta <= 0.0) {
-        a.out[row] = yi;
-        return;
+    predictor_update(a, row, pp);
+}
+
+template <int U>
+static __global__ void __launch_bounds__(kSliceBlock) predictor_sliced_kernel(PredictorArgs a, SlicedView S)
+{
+    extern __shared__ double acc[];
+    sliced_row_sums<U>(S, blockIdx.x, a.x, acc);
+    const int rw = kSliceWavesPerGroup * S.rpw;
+    for (int j = threadIdx.x; j < rw; j += kSliceBlock) {
+        const long row = (long)blockIdx.x * rw + j;
+        if (row < S.rows) predictor_update(a, (int)row, acc[j]);
     }
-    const double z = fmax(a.l[row], fmin(pp - yi / beta, a.u[row]));
-    a.out[row] = yi - beta * (pp - z);
 }
 
 // ---------------------------------------------------------------------------

@@ -66,10 +66,14 @@ class Solver {
     void upload();
     void rescale();
     void autotune_lanes();
+    void setup_sliced();
     double power_iteration();
     void init_steps();
     // iteration
     void launch_iteration(int k_offset, bool major);
+    void launch_primal(const PrimalArgs &pa, int k_offset, bool major);
+    void launch_dual(const DualArgs &da, int k_offset, bool major);
+    void launch_predictor(const PredictorArgs &a);
     void launch_eval();
     void launch_block(int K);
     void record_graph(int K);
@@ -101,6 +105,12 @@ class Solver {
     long nnz_ = 0;
     double obj_const_ = 0.0;
     DevCsr A_, At_;
+    // column-sliced copies of A and At for the iteration passes when their
+    // gathered vector is far larger than an XCD's L2 (sliced_csr.hpp)
+    DevSliced sA_, sAt_;
+    SlicedPlan planA_, planAt_;
+    int sliced_mode_ = -1;  // PDAL_SPMV_SLICED: 0 never, 1 always, unset: chosen by timing
+    int slice_unroll_ = 4;  // PDAL_SPMV_SLICE_UNROLL: chunks of 64 nonzeros in flight (2, 4, 8)
     DBuf<double> c_, lb_, ub_, l_, u_, E_, D_;
     DBuf<double> x0_, x_, xT_, xH_, xR_, dslack_;
     DBuf<double> y0_, y_, yT_, yH_, yR_;
@@ -167,6 +177,25 @@ void Solver::upload()
     put_csr(A_, m_, w->constraint_matrix_row_pointers, w->constraint_matrix_col_indices,
             w->constraint_matrix_values, "PDAL_SPMV_G_A");
     put_csr(At_, n_, T.rp.data(), T.ci.data(), T.v.data(), "PDAL_SPMV_G_AT");
+
+    // Plan the column-sliced copies while the host CSR is at hand; the device
+    // copies are made after rescaling (setup_sliced).  Automatic only when a
+    // gathered vector exceeds 8 MB (twice an XCD's L2) on a large matrix.
+    sliced_mode_ = getenv("PDAL_SPMV_SLICED") ? (atoi(getenv("PDAL_SPMV_SLICED")) != 0 ? 1 : 0) : -1;
+    const long slice_cols = std::max(1, env_int("PDAL_SPMV_SLICE_COLS", (int)kSliceDefaultCols));
+    const int rpw_env = env_int("PDAL_SPMV_SLICE_RPW", 0);
+    slice_unroll_ = env_int("PDAL_SPMV_SLICE_UNROLL", 4);
+    auto want = [&](int cols) {
+        if (sliced_mode_ == 0 || nnz_ == 0) return false;
+        if (sliced_mode_ == 1) return true;
+        return (long)cols * (long)sizeof(double) > (8L << 20) && nnz_ >= (1L << 20);
+    };
+    if (want(m_) && !build_sliced_plan(n_, m_, T.rp.data(), T.ci.data(), slice_cols,
+                                       sliced_rows_per_wave(n_, m_, rpw_env), planAt_))
+        planAt_ = SlicedPlan{};
+    if (want(n_) && !build_sliced_plan(m_, n_, w->constraint_matrix_row_pointers, w->constraint_matrix_col_indices,
+                                       slice_cols, sliced_rows_per_wave(m_, n_, rpw_env), planA_))
+        planA_ = SlicedPlan{};
 
     for (auto *b : {&c_, &lb_, &ub_, &D_, &x0_, &x_, &xT_, &xH_, &xR_, &dslack_}) b->alloc(n_);
     for (auto *b : {&l_, &u_, &E_, &y0_, &y_, &yT_, &yH_, &yR_}) b->alloc(m_);
@@ -387,6 +416,75 @@ void Solver::autotune_lanes()
     parts_a_.alloc((size_t)eval_grid_a_ * kEvalStride);
 }
 
+void Solver::setup_sliced()
+{   // device copies of the planned sliced layouts (values after rescaling);
+    // in automatic mode each is kept only if its plain SpMV beats the best
+    // lane count of the CSR kernel on this matrix
+    auto make = [&](DevSliced &S, SlicedPlan &P, const DevCsr &M) {
+        S = DevSliced{};
+        if (P.groups == 0) return;
+        S.rows = P.rows;
+        S.slices = P.slices;
+        S.rpw = P.rpw;
+        S.groups = P.groups;
+        S.colbits = P.colbits;
+        S.slice_cols = P.slice_cols;
+        const long nnz = (long)P.perm.size();
+        DBuf<int> perm(nnz);
+        perm.upload(P.perm.data(), nnz, stream_);
+        S.ci.alloc(nnz);
+        S.v.alloc(nnz);
+        S.seg.alloc(P.seg.size());
+        S.ci.upload(P.ci.data(), nnz, stream_);
+        S.seg.upload(P.seg.data(), P.seg.size(), stream_);
+        sliced_permute_kernel<<<std::max(1, std::min(8192, blocks_for(nnz))), kBlock, 0, stream_>>>(
+            perm.get(), nnz, M.v.get(), S.v.get());
+        PDAL_LAUNCHED();
+        PDAL_HIP(hipStreamSynchronize(stream_));
+        P = SlicedPlan{};
+        S.ready = true;
+    };
+    make(sAt_, planAt_, At_);
+    make(sA_, planA_, A_);
+    if (sliced_mode_ == 1) return;
+    hipEvent_t e0, e1;
+    PDAL_HIP(hipEventCreate(&e0));
+    PDAL_HIP(hipEventCreate(&e1));
+    auto keep_if_faster = [&](DevSliced &S, const DevCsr &M, int cols) {
+        if (!S.ready) return;
+        DBuf<double> xa(cols), ya(M.rows);
+        PDAL_HIP(hipMemsetAsync(xa.get(), 0, (size_t)cols * sizeof(double), stream_));
+        auto time3 = [&](auto &&launch) {
+            launch();
+            PDAL_HIP(hipEventRecord(e0, stream_));
+            for (int r = 0; r < 3; ++r) launch();
+            PDAL_HIP(hipEventRecord(e1, stream_));
+            PDAL_LAUNCHED();
+            PDAL_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            PDAL_HIP(hipEventElapsedTime(&ms, e0, e1));
+            return ms;
+        };
+        const float t_csr = time3([&] {
+            dispatch_group(M.G, [&](auto g) {
+                constexpr int G = decltype(g)::value;
+                spmv_kernel<G><<<blocks_for((long)M.rows * G), kBlock, 0, stream_>>>(M.view(), xa.get(), ya.get());
+            });
+        });
+        const float t_sl = time3([&] {
+            dispatch_unroll(slice_unroll_, [&](auto u) {
+                constexpr int U = decltype(u)::value;
+                sliced_spmv_kernel<U><<<S.groups, kSliceBlock, S.lds_bytes(), stream_>>>(S.view(), xa.get(), ya.get());
+            });
+        });
+        if (!(t_sl < t_csr)) S = DevSliced{};
+    };
+    keep_if_faster(sAt_, At_, m_);
+    keep_if_faster(sA_, A_, n_);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+}
+
 void Solver::init_sigma()
 {   // sigma mode on CP_AL_SIGMA or CP_AL_SIGMA_MODE (solver.cu:3563-3578), parameters :3141-3164
     const bool sigma_env = env_set("CP_AL_SIGMA");
@@ -465,6 +563,7 @@ void Solver::setup()
     upload();
     rescale();
     autotune_lanes();
+    setup_sliced();
     x_start_.alloc(n_);
     y_start_.alloc(m_);
     // anchor = current = T-output = Halpern point = start (solver.cu:1954-1965)
@@ -552,26 +651,66 @@ void Solver::launch_iteration(int k_offset, bool major)
     const View &v = iter_view_;
     if (pred) {
         PredictorArgs a{A_.view(), x_.get(), y_.get(), v.l, v.u, yR_.get(), sc_.get()};
-        dispatch_group(A_.G, [&](auto g) {
-            constexpr int G = decltype(g)::value;
-            predictor_kernel<G><<<blocks_for((long)m_ * G), kBlock, 0, stream_>>>(a);
-        });
+        launch_predictor(a);
     }
     PrimalArgs pa{At_.view(), pred ? yR_.get() : y_.get(), v.c, v.lb, v.ub, x0_.get(), x_.get(),
                   xR_.get(), xT_.get(), xH_.get(), dslack_.get(), sc_.get(), rho};
+    launch_primal(pa, k_offset, major);
+    DualArgs da{A_.view(), xR_.get(), v.l, v.u, y0_.get(), y_.get(), yT_.get(), yH_.get(), yR_.get(),
+                sc_.get(), rho};
+    launch_dual(da, k_offset, major);
+    PDAL_LAUNCHED();
+}
+
+void Solver::launch_primal(const PrimalArgs &pa, int k_offset, bool major)
+{   // At.y + primal update: the sliced pass when At has a sliced copy
+    if (sAt_.ready) {
+        const size_t lds = sAt_.lds_bytes();
+        dispatch_unroll(slice_unroll_, [&](auto u) {
+            constexpr int U = decltype(u)::value;
+            if (major) primal_sliced_kernel<U, true><<<sAt_.groups, kSliceBlock, lds, stream_>>>(pa, sAt_.view(), k_offset);
+            else primal_sliced_kernel<U, false><<<sAt_.groups, kSliceBlock, lds, stream_>>>(pa, sAt_.view(), k_offset);
+        });
+        return;
+    }
     dispatch_group(At_.G, [&](auto g) {
         constexpr int G = decltype(g)::value;
         if (major) primal_kernel<G, true><<<blocks_for((long)n_ * G), kBlock, 0, stream_>>>(pa, k_offset);
         else primal_kernel<G, false><<<blocks_for((long)n_ * G), kBlock, 0, stream_>>>(pa, k_offset);
     });
-    DualArgs da{A_.view(), xR_.get(), v.l, v.u, y0_.get(), y_.get(), yT_.get(), yH_.get(), yR_.get(),
-                sc_.get(), rho};
+}
+
+void Solver::launch_dual(const DualArgs &da, int k_offset, bool major)
+{   // A.x_bar + dual update: the sliced pass when A has a sliced copy
+    if (sA_.ready) {
+        const size_t lds = sA_.lds_bytes();
+        dispatch_unroll(slice_unroll_, [&](auto u) {
+            constexpr int U = decltype(u)::value;
+            if (major) dual_sliced_kernel<U, true><<<sA_.groups, kSliceBlock, lds, stream_>>>(da, sA_.view(), k_offset);
+            else dual_sliced_kernel<U, false><<<sA_.groups, kSliceBlock, lds, stream_>>>(da, sA_.view(), k_offset);
+        });
+        return;
+    }
     dispatch_group(A_.G, [&](auto g) {
         constexpr int G = decltype(g)::value;
         if (major) dual_kernel<G, true><<<blocks_for((long)m_ * G), kBlock, 0, stream_>>>(da, k_offset);
         else dual_kernel<G, false><<<blocks_for((long)m_ * G), kBlock, 0, stream_>>>(da, k_offset);
     });
-    PDAL_LAUNCHED();
+}
+
+void Solver::launch_predictor(const PredictorArgs &a)
+{
+    if (sA_.ready) {
+        dispatch_unroll(slice_unroll_, [&](auto u) {
+            constexpr int U = decltype(u)::value;
+            predictor_sliced_kernel<U><<<sA_.groups, kSliceBlock, sA_.lds_bytes(), stream_>>>(a, sA_.view());
+        });
+        return;
+    }
+    dispatch_group(A_.G, [&](auto g) {
+        constexpr int G = decltype(g)::value;
+        predictor_kernel<G><<<blocks_for((long)m_ * G), kBlock, 0, stream_>>>(a);
+    });
 }
 
 void Solver::launch_eval()
@@ -925,25 +1064,16 @@ void Solver::time_kernels(int iters, double *out)
         PDAL_HIP(hipEventRecord(ev[0], stream_));
         if (pred) {
             PredictorArgs a{A_.view(), x_.get(), y_.get(), l_.get(), u_.get(), yR_.get(), sc_.get()};
-            dispatch_group(A_.G, [&](auto g) {
-                constexpr int G = decltype(g)::value;
-                predictor_kernel<G><<<blocks_for((long)m_ * G), kBlock, 0, stream_>>>(a);
-            });
+            launch_predictor(a);
         }
         PDAL_HIP(hipEventRecord(ev[1], stream_));
         PrimalArgs pa{At_.view(), pred ? yR_.get() : y_.get(), c_.get(), lb_.get(), ub_.get(), x0_.get(), x_.get(),
                       xR_.get(), xT_.get(), xH_.get(), dslack_.get(), sc_.get(), rho};
-        dispatch_group(At_.G, [&](auto g) {
-            constexpr int G = decltype(g)::value;
-            primal_kernel<G, false><<<blocks_for((long)n_ * G), kBlock, 0, stream_>>>(pa, it + 1);
-        });
+        launch_primal(pa, it + 1, false);
         PDAL_HIP(hipEventRecord(ev[2], stream_));
         DualArgs da{A_.view(), xR_.get(), l_.get(), u_.get(), y0_.get(), y_.get(), yT_.get(), yH_.get(),
                     yR_.get(), sc_.get(), rho};
-        dispatch_group(A_.G, [&](auto g) {
-            constexpr int G = decltype(g)::value;
-            dual_kernel<G, false><<<blocks_for((long)m_ * G), kBlock, 0, stream_>>>(da, it + 1);
-        });
+        launch_dual(da, it + 1, false);
         PDAL_HIP(hipEventRecord(ev[3], stream_));
         PDAL_LAUNCHED();
         PDAL_HIP(hipEventSynchronize(ev[3]));
@@ -966,8 +1096,8 @@ void Solver::time_kernels(int iters, double *out)
     out[2] = pred ? tpr / iters : 0.0;
     out[3] = (tp + td + (pred ? tpr : 0.0)) / iters;
     out[4] = te;
-    out[5] = At_.G;
-    out[6] = A_.G;
+    out[5] = sAt_.ready ? -sAt_.slices : At_.G;  // a negative lane count: the sliced pass with that many slices
+    out[6] = sA_.ready ? -sA_.slices : A_.G;
     for (auto &e : ev) (void)hipEventDestroy(e);
 }
 

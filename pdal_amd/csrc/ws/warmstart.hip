@@ -562,9 +562,11 @@ PDAL_API void pdal_pod_free(pdal_pod_t *p)
     delete p;
 }
 
-PDAL_API int pdal_pod_expand_device(pdal_pod_t *p, const float *coef, int batch, double *out)
+PDAL_API int pdal_pod_expand_device(pdal_pod_t *p, const float *coef, int batch, double *out,
+                                    long long out_capacity)
 {
     if (!p || !coef || !out || batch <= 0) return -1;
+    if ((long long)batch * p->dim > out_capacity) return -1;  // the expansion would overrun `out`
     return guarded("pod_expand_device", [&] {
         if (p->C.size() < (size_t)batch * p->k) p->C.alloc((size_t)batch * p->k);
         p->C.upload(coef, (size_t)batch * p->k, p->s);

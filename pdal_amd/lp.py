@@ -450,7 +450,13 @@ class LpBatch:
         self.probs = []
 
     def set_starts(self, x=None, y=None) -> None:
+        """x: the members' primal starts concatenated (sum n_i doubles), y:
+        their dual starts (sum m_i); None keeps the current one."""
         xs, ys = _f64(x), _f64(y)
+        if xs is not None and xs.size != sum(ni for _, ni in self.sizes):
+            raise ValueError(f"x has {xs.size} entries, the batch has {sum(ni for _, ni in self.sizes)} columns")
+        if ys is not None and ys.size != sum(mi for mi, _ in self.sizes):
+            raise ValueError(f"y has {ys.size} entries, the batch has {sum(mi for mi, _ in self.sizes)} rows")
         rc = self.lib.lib.pdal_cp_al_batch_set_starts(self.h, _ptr(xs, C.c_double), _ptr(ys, C.c_double))
         if rc != 0:
             raise RuntimeError(f"pdal_cp_al_batch_set_starts failed ({rc})")

@@ -47,7 +47,7 @@ class WarmStartLib:
         L.pdal_pod_free.restype = None
         L.pdal_pod_free.argtypes = [C.c_void_p]
         L.pdal_pod_expand_device.restype = C.c_int
-        L.pdal_pod_expand_device.argtypes = [C.c_void_p, _fp, C.c_int, C.c_void_p]
+        L.pdal_pod_expand_device.argtypes = [C.c_void_p, _fp, C.c_int, C.c_void_p, C.c_longlong]
 
     def create(self, widths, seed: int = 0) -> "Mlp":
         return Mlp(self, widths, seed)
@@ -80,12 +80,16 @@ class DevicePod:
         if not self.h:
             raise RuntimeError("pdal_pod_create failed")
 
-    def expand(self, coef, out_device: int) -> None:
+    def expand(self, coef, out_device: int, out_capacity: int) -> None:
+        """out_capacity: doubles available at out_device (the call writes
+        coef.shape[0] * dim of them and refuses more)."""
         coef = _f32(coef)
-        if coef.shape[1] != self.k:
+        if coef.ndim != 2 or coef.shape[1] != self.k:
             raise ValueError("coefficient count does not match the basis")
+        if coef.shape[0] * self.dim > out_capacity:
+            raise ValueError(f"expansion of {coef.shape[0]} x {self.dim} exceeds the {out_capacity}-double buffer")
         if self.lib.lib.pdal_pod_expand_device(self.h, coef.ctypes.data_as(_fp), coef.shape[0],
-                                               C.c_void_p(out_device)) != 0:
+                                               C.c_void_p(out_device), int(out_capacity)) != 0:
             raise RuntimeError("pdal_pod_expand_device failed")
 
     def close(self) -> None:
@@ -289,12 +293,21 @@ class LearnedWarmStart:
         buffers of `batch` (an LpBatch of len(thetas) members, in order) and
         handed to it on the device, with the predicted primal weights.  The
         bases stay on the device between calls.  Returns pw_hat."""
+        thetas = np.asarray(thetas, dtype=np.float64)
+        count = len(batch.sizes)
+        if thetas.ndim != 2 or thetas.shape[0] != count:
+            raise ValueError(f"{thetas.shape[0] if thetas.ndim else 0} parameter rows for a batch of {count}")
+        dim_x, dim_y = self.Bx.shape[1], self.By.shape[1]
+        if any(mi != dim_y or ni != dim_x for mi, ni in batch.sizes):
+            raise ValueError(f"every member must have m = {dim_y}, n = {dim_x} (the POD dimensions)")
         coef = self.predict_coef(thetas, mlp)
         if self._dev is None:
             self._dev = (DevicePod(self.lib, self.Bx, self.mx, self.lb, self.ub), DevicePod(self.lib, self.By, self.my))
         px, py = batch.start_buffers()
-        self._dev[0].expand(coef[:, : self.kx_eff], px)
-        self._dev[1].expand(coef[:, self.kx: self.kx + self.ky_eff], py)
+        cap_x = sum(ni for _, ni in batch.sizes)
+        cap_y = sum(mi for mi, _ in batch.sizes)
+        self._dev[0].expand(coef[:, : self.kx_eff], px, cap_x)
+        self._dev[1].expand(coef[:, self.kx: self.kx + self.ky_eff], py, cap_y)
         batch.set_starts_from_buffers()
         pw = np.exp(coef[:, -1]) if self.has_pw else None
         if pw is not None:
