@@ -55,10 +55,10 @@ def test_zipf_iterates_track_oracle(oracle_lp, gpu_lp, monkeypatch, zipf_mid, sl
 def test_sliced_passes_are_chosen_and_equal_on_large_vectors(gpu_lp, monkeypatch):
     """On the full-size LP the automatic choice takes the sliced passes (the
     gathered vectors are 16 and 32 MB), and forcing them off changes the
-    iterates only by summation order.  The CSR lane counts are pinned: the
-    power iteration and the evaluation passes use them, and a lane count the
-    timing autotuner picks differently moves the power iteration's stopping
-    step and with it the step size by ~1e-8."""
+    iterates only by summation order.  The generator is reseeded before each
+    set-up and the CSR lane counts are pinned: the power iteration and the
+    evaluation passes use them, and another start vector or lane count moves
+    the power iteration's stopping step and with it the step size by ~1e-8."""
     from pdal_amd.lp import LpSession
     monkeypatch.setenv("CP_AL_LAMBDA", "0")
     monkeypatch.setenv("PDAL_SPMV_G_AT", "16")
@@ -70,9 +70,9 @@ def test_sliced_passes_are_chosen_and_equal_on_large_vectors(gpu_lp, monkeypatch
             monkeypatch.setenv("PDAL_SPMV_SLICED", "0")
         print(f"[case] config1 sliced={mode}", flush=True)
         p = gpu_lp.default_params(iteration_limit=400, eps_optimal_relative=1e-14, eps_feasible_relative=1e-14)
+        gpu_lp.reset_rng()  # the session's power iteration draws its start vector at set-up
         with LpSession(gpu_lp, **inst.lp_args(), params=p) as sess:
             kt = sess.time_kernels(3)
-            gpu_lp.reset_rng()
             res[mode] = (kt, sess.solve())
         print(f"[kernels] {mode}: {kt}", flush=True)
     assert res["auto"][0]["lanes_at"] < 0 and res["auto"][0]["lanes_a"] < 0  # sliced passes in use
