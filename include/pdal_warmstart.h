@@ -68,6 +68,17 @@ void pdal_pod_free(pdal_pod_t *pod);
  * writes nothing and returns -1.  Returns when the expansion has finished. */
 int pdal_pod_expand_device(pdal_pod_t *pod, const float *coef, int batch, double *out, long long out_capacity);
 
+/* pdal_pod_expand_device for callers that cannot allocate device memory
+ * (code under HIP graph capture, serving loops that forbid hipMalloc): no
+ * counterpart in the reference.  The same expansion, with the coefficients
+ * staged in a fixed device array of the library (131,072 floats, 512 KiB per
+ * GPU, held until the process ends) instead of a buffer the pod allocates.
+ * batch x k must not exceed 131,072 (else -3 and a message on stderr, nothing
+ * written).  Calls that overlap on one GPU share the array and race.  The
+ * pod, out and coef are the caller's, as for pdal_pod_expand_device. */
+int pdal_pod_expand_device_noalloc(pdal_pod_t *pod, const float *coef, int batch, double *out,
+                                   long long out_capacity);
+
 #ifdef __cplusplus
 }
 #endif

@@ -26,6 +26,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <exception>
+#include <mutex>
 #include <numeric>
 #include <random>
 #include <vector>
@@ -511,6 +512,7 @@ PDAL_API int pdal_pod_expand(const float *basis, const float *mean, int dim, int
 
 struct pdal_pod {
     hipStream_t s = nullptr;
+    int dev = 0;  // device current at create: where B, mu, lo, hi live
     int dim = 0, k = 0;
     pdal::DBuf<float> B, mu;
     pdal::DBuf<double> lo, hi;
@@ -525,6 +527,7 @@ PDAL_API pdal_pod_t *pdal_pod_create(const float *basis, const float *mean, int 
     if ((size_t)pdal::ws::kExpandRows * k * sizeof(float) > 64 * 1024) return nullptr;  // LDS staging limit
     pdal_pod *p = new pdal_pod;
     const int rc = guarded("pod_create", [&] {
+        PDAL_HIP(hipGetDevice(&p->dev));
         PDAL_HIP(hipStreamCreateWithFlags(&p->s, hipStreamNonBlocking));
         p->dim = dim;
         p->k = k;
@@ -573,6 +576,69 @@ PDAL_API int pdal_pod_expand_device(pdal_pod_t *p, const float *coef, int batch,
         const dim3 grid((p->dim + 255) / 256, (batch + pdal::ws::kExpandRows - 1) / pdal::ws::kExpandRows);
         pdal::ws::pod_expand_kernel<<<grid, 256, pdal::ws::kExpandRows * p->k * sizeof(float), p->s>>>(
             p->B.get(), p->mu.get(), p->dim, p->k, p->C.get(), batch, out, p->has_lo ? p->lo.get() : nullptr,
+            p->has_hi ? p->hi.get() : nullptr);
+        PDAL_LAUNCHED();
+        PDAL_HIP(hipStreamSynchronize(p->s));
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Expansion without allocation.  The coefficients of a call are staged in a
+// file-scope device array instead of a buffer the pod grows with hipMalloc:
+// every GPU the library runs on holds one copy of the array (512 KiB) until
+// the process ends.  Host code never uses the array's name as an address: its
+// device address is looked up once per device (hipGetSymbolAddress, with that
+// device current) and kept.  Calls on one GPU share the array, so calls that
+// overlap on one GPU race; nothing in it needs clearing (a call overwrites
+// exactly the coefficients it reads).
+
+namespace {
+
+constexpr size_t kStageFloats = 128 * 1024;  // batch x k coefficients of one call, at most
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) float scratch[kStageFloats];
+
+std::mutex g_scratch_mu;
+float *g_scratch_ptr[kMaxDevices] = {};
+
+float *scratch_on(int dev)
+{
+    if (dev < 0 || dev >= kMaxDevices) throw pdal::HipError("device index beyond the staging table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev]) return g_scratch_ptr[dev];
+    int cur = 0;
+    PDAL_HIP(hipGetDevice(&cur));
+    if (cur != dev) PDAL_HIP(hipSetDevice(dev));
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (cur != dev) PDAL_HIP(hipSetDevice(cur));
+    PDAL_HIP(e);
+    if (!ptr) throw pdal::HipError("hipGetSymbolAddress returned no address for the staging array");
+    g_scratch_ptr[dev] = static_cast<float *>(ptr);
+    return g_scratch_ptr[dev];
+}
+
+}  // namespace
+
+PDAL_API int pdal_pod_expand_device_noalloc(pdal_pod_t *p, const float *coef, int batch, double *out,
+                                            long long out_capacity)
+{
+    if (!p || !coef || !out || batch <= 0) return -1;
+    if ((long long)batch * p->dim > out_capacity) return -1;  // the expansion would overrun `out`
+    if ((size_t)batch * (size_t)p->k > kStageFloats) {
+        fprintf(stderr,
+                "[pdal-ws] pod_expand_device_noalloc: %d x %d coefficients exceed the %zu-float staging area "
+                "(use pdal_pod_expand_device)\n",
+                batch, p->k, kStageFloats);
+        return -3;
+    }
+    return guarded("pod_expand_device_noalloc", [&] {
+        float *C = scratch_on(p->dev);
+        PDAL_HIP(hipMemcpyAsync(C, coef, (size_t)batch * p->k * sizeof(float), hipMemcpyHostToDevice, p->s));
+        const dim3 grid((p->dim + 255) / 256, (batch + pdal::ws::kExpandRows - 1) / pdal::ws::kExpandRows);
+        pdal::ws::pod_expand_kernel<<<grid, 256, pdal::ws::kExpandRows * p->k * sizeof(float), p->s>>>(
+            p->B.get(), p->mu.get(), p->dim, p->k, C, batch, out, p->has_lo ? p->lo.get() : nullptr,
             p->has_hi ? p->hi.get() : nullptr);
         PDAL_LAUNCHED();
         PDAL_HIP(hipStreamSynchronize(p->s));

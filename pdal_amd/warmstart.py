@@ -48,6 +48,8 @@ class WarmStartLib:
         L.pdal_pod_free.argtypes = [C.c_void_p]
         L.pdal_pod_expand_device.restype = C.c_int
         L.pdal_pod_expand_device.argtypes = [C.c_void_p, _fp, C.c_int, C.c_void_p, C.c_longlong]
+        L.pdal_pod_expand_device_noalloc.restype = C.c_int
+        L.pdal_pod_expand_device_noalloc.argtypes = [C.c_void_p, _fp, C.c_int, C.c_void_p, C.c_longlong]
 
     def create(self, widths, seed: int = 0) -> "Mlp":
         return Mlp(self, widths, seed)
@@ -91,6 +93,15 @@ class DevicePod:
         if self.lib.lib.pdal_pod_expand_device(self.h, coef.ctypes.data_as(_fp), coef.shape[0],
                                                C.c_void_p(out_device), int(out_capacity)) != 0:
             raise RuntimeError("pdal_pod_expand_device failed")
+
+    def expand_noalloc(self, coef, out_device: int, out_capacity: int) -> int:
+        """expand() through pdal_pod_expand_device_noalloc (no device
+        allocation; batch x k <= 131072).  Returns the entry point's code."""
+        coef = _f32(coef)
+        if coef.ndim != 2 or coef.shape[1] != self.k:
+            raise ValueError("coefficient count does not match the basis")
+        return int(self.lib.lib.pdal_pod_expand_device_noalloc(self.h, coef.ctypes.data_as(_fp), coef.shape[0],
+                                                               C.c_void_p(out_device), int(out_capacity)))
 
     def close(self) -> None:
         if self.h:
