@@ -1677,6 +1677,28 @@ ORACLE_API void lin_cp_al_oracle_reset_rng(void)
 #elif defined(ORACLE_PLANTED_CP_AL)
 ORACLE_API void planted_cp_al_oracle_reset_rng(void)
 #else
+// Test hook, no counterpart in the reference: the Q eigenvalue estimates of
+// initialize_quadratic_term_information on an n x n CSR matrix -- the power
+// iteration of estimate_maximum_eigenvalue and the shifted one of
+// estimate_minimum_eigenvalue (utils.cu:172-315) -- with the given iteration
+// limit and tolerance, rand() reseeded with `seed` first.  out = {q_norm,
+// minimum eigenvalue estimate}.
+ORACLE_API void fa_cp_oracle_q_eigen(int n, const int *rp, const int *ci, const double *v, int max_iter, double tol,
+                                     unsigned seed, double *out)
+{
+    S s;
+    s.n = n;
+    s.P.sv_max_iter = max_iter;
+    s.P.sv_tol = tol;
+    s.Q.rows = s.Q.cols = n;
+    s.Q.rp.assign(rp, rp + n + 1);
+    s.Q.ci.assign(ci, ci + rp[n]);
+    s.Q.v.assign(v, v + rp[n]);
+    srand(seed);
+    out[0] = q_max_eig(s);
+    out[1] = q_min_eig(s, out[0]);
+}
+
 ORACLE_API void fa_cp_oracle_reset_rng(void)
 #endif
 {

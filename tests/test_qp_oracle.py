@@ -177,3 +177,53 @@ def test_primal_infeasible_is_detected(oracle_qp):
     r = oracle_qp.solve(np.array([1.0, 1.0]), Q=sp.diags([1.0, 1.0]).tocsr(), A=A, con_lb=np.array([-1.0]),
                         con_ub=np.array([-1.0]), var_lb=np.zeros(2), var_ub=np.full(2, np.inf), params=p)
     assert r.status == "PRIMAL_INFEASIBLE"
+
+
+# ---------------------------------------------------------------------------
+# Q eigenvalue estimates (QP/FA_CP/src/utils.cu:172-315) against numpy.
+
+
+def _q_eigen(Q, max_iter=50000, tol=1e-13, seed=3):
+    import ctypes as C
+    from conftest import ORACLE_QP
+    lib = C.CDLL(str(ORACLE_QP))
+    lib.fa_cp_oracle_q_eigen.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double,
+                                         C.c_uint, C.c_void_p]
+    Q = sp.csr_matrix(Q)
+    rp, ci = Q.indptr.astype(np.int32), Q.indices.astype(np.int32)
+    v = Q.data.astype(np.float64)
+    out = np.zeros(2)
+    lib.fa_cp_oracle_q_eigen(Q.shape[0], rp.ctypes.data, ci.ctypes.data, v.ctypes.data, max_iter, tol, seed,
+                             out.ctypes.data)
+    return out
+
+
+def _with_spectrum(d, seed):
+    U, _ = np.linalg.qr(np.random.default_rng(seed).standard_normal((len(d), len(d))))
+    return (U * np.asarray(d)) @ U.T
+
+
+@pytest.mark.parametrize("case", ["psd", "indefinite", "negative_dominant"])
+def test_q_eigen_estimates_match_eigvalsh(case):
+    """estimate_maximum_eigenvalue is a power iteration on Q (Rayleigh quotient
+    of the normalised iterate): it converges to the eigenvalue of largest
+    magnitude, with its sign.  estimate_minimum_eigenvalue runs the same on
+    q_norm I - Q and returns q_norm minus that dominant eigenvalue: the
+    smallest eigenvalue when q_norm is the largest one, and, when a negative
+    eigenvalue dominates, q_norm - (q_norm - lambda_max) = lambda_max -- the
+    reference's formula taken as it stands."""
+    d = {"psd": [10.0, 6.0, 3.0, 2.0, 1.0, 0.7, 0.3, 0.05],
+         "indefinite": [8.0, 3.0, 1.0, 0.2, -0.5, -2.0],
+         "negative_dominant": [3.0, 1.0, 0.5, -6.0]}[case]
+    Q = _with_spectrum(d, seed=len(d))
+    Q[np.abs(Q) < 1e-14] = 0.0
+    ev = np.linalg.eigvalsh(Q)
+    qn, qmin = _q_eigen(Q)
+    dom = ev[np.argmax(np.abs(ev))]
+    assert qn == pytest.approx(dom, abs=1e-9)
+    shifted = qn - ev  # eigenvalues of q_norm I - Q
+    assert qmin == pytest.approx(qn - shifted[np.argmax(np.abs(shifted))], abs=1e-9)
+    if case != "negative_dominant":
+        assert qn == pytest.approx(ev.max(), abs=1e-9) and qmin == pytest.approx(ev.min(), abs=1e-9)
+    else:
+        assert qmin == pytest.approx(ev.max(), abs=1e-9)

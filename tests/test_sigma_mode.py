@@ -139,3 +139,104 @@ def test_gpu_sigma_solves_match_oracle(oracle_lp, gpu_lp, monkeypatch, env):
     assert g.iterations == o.iterations
     assert abs(g.primal_objective - o.primal_objective) <= 1e-6 * abs(o.primal_objective)
     assert abs(g.primal_objective - inst.optimal_value) <= 1e-5 * (1 + abs(inst.optimal_value))
+
+
+# ---------------------------------------------------------------------------
+# The sigma PID at the first restart, through the closed-form iteration map of
+# an equality LP with free variables (every step affine).
+
+
+def _orthogonal_rows(m, n, s, seed):
+    Q, _ = np.linalg.qr(np.random.default_rng(seed).standard_normal((n, n)))
+    return np.sqrt(s) * Q[:m, :]  # A A^T = s I: the power iteration is exact at its first step
+
+
+def _sigma_T(A, b, c, tau, sig):
+    """One sigma-mode step as z -> M z + d on z = (x, y): the box-AL predictor
+    u = y - sig (A x - b) (equality rows, penalty = sig), x+ = x - tau (c - A^T u)
+    (free variables), y+ = y - sig (A (2 x+ - x) - b) (solver.cu:2459-2479,
+    2340-2457 with l = u = b)."""
+    m, n = A.shape
+    I_n, I_m = np.eye(n), np.eye(m)
+    Mx = np.hstack([I_n - tau * sig * A.T @ A, tau * A.T])
+    dx = -tau * c + tau * sig * A.T @ b
+    Mxr = 2 * Mx - np.hstack([I_n, np.zeros((n, m))])  # 2 x+ - x
+    dxr = 2 * dx
+    My = np.hstack([np.zeros((m, n)), I_m]) - sig * A @ Mxr
+    dy = -sig * A @ dxr + sig * b
+    return np.vstack([Mx, My]), np.concatenate([dx, dy])
+
+
+def _halpern_T_output(M, d, z0, K):
+    """T(z_{K-1}) with z_{K-1} = z* + (1/K) sum_{j<K} (2M - I)^j (z0 - z*)."""
+    N = M.shape[0]
+    zs, *_ = np.linalg.lstsq(np.eye(N) - M, d, rcond=None)
+    assert np.linalg.norm((np.eye(N) - M) @ zs - d) < 1e-10
+    R = 2 * M - np.eye(N)
+    v, acc = z0 - zs, np.zeros(N)
+    for _ in range(K):
+        acc += v
+        v = R @ v
+    return M @ (zs + acc / K) + d
+
+
+@pytest.mark.parametrize("case", ["ratio_free_balance_clipped", "balance_free_wide_clip", "ratio_clipped"])
+def test_sigma_pid_first_restart_closed_form(oracle_lp, monkeypatch, case):
+    """Block 1 restarts unconditionally (total == K, utils.cu:336-339) to its
+    T-output; the sigma PID (solver.cu:2857-2966) computes, from the restart
+    displacement (d_x, d_y), ||A d_x||, lambda_param and the residual ratio,
+    err = ln||d_y|| - ln(lambda_param ||d_x||^2 - ||A d_x||^2)/2 - ln sigma
+    + 0.15 clip(ln(r_d/r_p)/2, -2, 2) and sigma <- sigma clip(exp((k_p + k_i) err),
+    lo, hi) (lo, hi = 0.6, 1.7 unless set); block 2 is again a Halpern
+    polynomial from the new anchor with the new steps.  Every quantity here
+    comes from the closed-form map, not from the oracle, except the two
+    residuals of block 1.  The three cases put the PID step and the balance
+    term on each side of their clips."""
+    import scipy.sparse as sp
+    monkeypatch.setenv("CP_AL_SIGMA_MODE", "1")
+    m, n, s, K, seed, f = {"ratio_free_balance_clipped": (9, 15, 1.3, 21, 8, 2.0),
+                           "balance_free_wide_clip": (12, 30, 0.7, 9, 3, 3.0),
+                           "ratio_clipped": (9, 15, 1.3, 21, 8, 6.0)}[case]
+    lo, hi = (1e-6, 1e6) if case == "balance_free_wide_clip" else (0.6, 1.7)
+    monkeypatch.setenv("CP_AL_SIGMA_INIT_FACTOR", repr(f))
+    if case == "balance_free_wide_clip":
+        monkeypatch.setenv("CP_AL_SIGMA_UPDATE_MIN", repr(lo))
+        monkeypatch.setenv("CP_AL_SIGMA_UPDATE_MAX", repr(hi))
+    A = _orthogonal_rows(m, n, s, seed=seed)
+    rng = np.random.default_rng(seed + 1)
+    b = rng.standard_normal(m)
+    c = A.T @ rng.standard_normal(m)
+    lam_param = 2.01 * s
+    sigma = f / np.sqrt(lam_param)  # init factor x sigma_base (solver.cu:3141-3164)
+    common = dict(l_inf_ruiz_iterations=0, has_pock_chambolle_alpha=False, bound_objective_rescaling=False,
+                  termination_evaluation_frequency=K, eps_optimal_relative=1e-15, eps_feasible_relative=1e-15)
+    oracle_lp.reset_rng()
+    r1 = oracle_lp.solve(A=sp.csr_matrix(A), c=c, con_lb=b, con_ub=b,
+                         params=oracle_lp.default_params(iteration_limit=K, **common))
+    M1, d1 = _sigma_T(A, b, c, 1.0 / (sigma * lam_param), sigma)
+    z1 = _halpern_T_output(M1, d1, np.zeros(n + m), K)
+    np.testing.assert_allclose(np.concatenate([r1.x, r1.y]), z1, atol=1e-10 * np.linalg.norm(z1))
+    dxn, dyn = np.linalg.norm(z1[:n]), np.linalg.norm(z1[n:])
+    axn = np.linalg.norm(A @ z1[:n])
+    ratio_infeas = r1.relative_dual_residual / r1.relative_primal_residual
+    assert 1e-8 < ratio_infeas < 1e8
+    balance = 0.5 * np.log(ratio_infeas)
+    err = np.log(dyn) - 0.5 * np.log(lam_param * dxn ** 2 - axn ** 2) - np.log(sigma)
+    err += 0.15 * np.clip(balance, -2.0, 2.0)
+    step = np.exp(0.99 * err + 0.01 * err)
+    if case == "ratio_free_balance_clipped":
+        assert balance < -2.0 and lo < step < hi
+    elif case == "balance_free_wide_clip":
+        assert -2.0 < balance < -1.0 and not 0.6 < step < 1.7
+    else:
+        assert step < lo
+    sigma2 = np.clip(sigma * np.clip(step, lo, hi), 1e-4 / np.sqrt(lam_param), 100.0 / np.sqrt(lam_param))
+    oracle_lp.reset_rng()
+    r2 = oracle_lp.solve(A=sp.csr_matrix(A), c=c, con_lb=b, con_ub=b,
+                         params=oracle_lp.default_params(iteration_limit=2 * K, **common))
+    M2, d2 = _sigma_T(A, b, c, 1.0 / (sigma2 * lam_param), sigma2)
+    z2 = _halpern_T_output(M2, d2, z1, K)
+    np.testing.assert_allclose(np.concatenate([r2.x, r2.y]), z2, atol=1e-9 * np.linalg.norm(z2))
+    # a sigma off by 1 % lands ten times the tolerance away: the check resolves the update
+    z2_off = _halpern_T_output(*_sigma_T(A, b, c, 1.0 / (1.01 * sigma2 * lam_param), 1.01 * sigma2), z1, K)
+    assert np.linalg.norm(z2_off - z2) > 1e-8 * np.linalg.norm(z2)
