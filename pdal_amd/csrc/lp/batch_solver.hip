@@ -652,7 +652,7 @@ hipGraphExec_t BatchSolver::sh_capture()
 {
     hipGraph_t graph;
     hipGraphExec_t exec = nullptr;
-    PDAL_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeGlobal));
+    PDAL_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     sh_launch_block();
     PDAL_HIP(hipStreamEndCapture(stream_, &graph));
     PDAL_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
@@ -1001,8 +1001,6 @@ void BatchSolver::setup()
     pred_ = al_lambda > 0.0 && std::fmin(al_lambda, 0.95) > 0.0;
     detect_shared();
     warn_unsupported_controllers();
-    if (env_set("CP_AL_CR_SCALING"))
-        fprintf(stderr, "[box-cpal] CP_AL_CR_SCALING is not applied by the batched solver.\n");
     if (env_set("CP_AL_SIGMA") || env_int("CP_AL_SIGMA_MODE", 0) != 0)
         fprintf(stderr, "[box-cpal] sigma mode is not applied by the batched solver (alpha mode runs).\n");
     setup_s_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -1025,6 +1023,26 @@ void BatchSolver::rescale()
                                                                    inv_d.get(), inv_e.get());
         PDAL_LAUNCHED();
     };
+    // Curtis-Reid first when CP_AL_CR_SCALING is set (curtis_reid_rescaling,
+    // preconditioner.cu:162-268; iterations from CP_AL_CR_ITERS, default 20):
+    // every row and column update reads only its own instance's block, so the
+    // passes over the block-diagonal system are each instance's own
+    const char *cr_raw = getenv("CP_AL_CR_SCALING");
+    int cr_iters = (cr_raw && cr_raw[0] != '\0' && atoi(cr_raw) != 0) ? 20 : 0;
+    const char *cr_iters_raw = getenv("CP_AL_CR_ITERS");
+    if (cr_iters > 0 && cr_iters_raw && cr_iters_raw[0] != '\0') cr_iters = std::max(atoi(cr_iters_raw), 0);
+    if (cr_iters > 0) {
+        PDAL_HIP(hipMemsetAsync(e.get(), 0, (size_t)m_ * sizeof(double), stream_));
+        PDAL_HIP(hipMemsetAsync(d.get(), 0, (size_t)n_ * sizeof(double), stream_));
+        for (int it = 0; it < cr_iters; ++it) {
+            cr_log_update_kernel<<<blocks_for_l(m_), kBlock, 0, stream_>>>(A_.view(), d.get(), e.get());
+            cr_log_update_kernel<<<blocks_for_l(n_), kBlock, 0, stream_>>>(At_.view(), e.get(), d.get());
+        }
+        cr_exp_kernel<<<blocks_for_l(m_), kBlock, 0, stream_>>>(e.get(), inv_e.get(), E_.get(), mi);
+        cr_exp_kernel<<<blocks_for_l(n_), kBlock, 0, stream_>>>(d.get(), inv_d.get(), D_.get(), ni);
+        PDAL_LAUNCHED();
+        scale_problem();
+    }
     for (int it = 0; it < P_.l_inf_ruiz_iterations; ++it) {
         row_absmax_kernel<<<blocks_for_l(m_), kBlock, 0, stream_>>>(A_.view(), e.get());
         clamp_sqrt_accum_kernel<<<blocks_for_l(m_), kBlock, 0, stream_>>>(e.get(), inv_e.get(), E_.get(), mi);
@@ -1167,7 +1185,7 @@ void BatchSolver::power_iteration()
         exec = nullptr;
         if (!use_graph_) return;
         hipGraph_t graph;
-        PDAL_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeGlobal));
+        PDAL_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
         for (int i = 0; i < kChunk; ++i) one();
         PDAL_HIP(hipStreamEndCapture(stream_, &graph));
         PDAL_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
@@ -1278,7 +1296,7 @@ void BatchSolver::record_graph()
 {
     if (!use_graph_) return;
     hipGraph_t graph;
-    PDAL_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeGlobal));
+    PDAL_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     launch_block();
     PDAL_HIP(hipStreamEndCapture(stream_, &graph));
     PDAL_HIP(hipGraphInstantiate(&graph_, graph, nullptr, nullptr, 0));

@@ -350,7 +350,7 @@ double Solver::power_iteration()
     hipGraphExec_t exec = nullptr;
     if (use_graph_) {
         hipGraph_t graph;
-        PDAL_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeGlobal));
+        PDAL_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
         for (int i = 0; i < kChunk; ++i) one_iter();
         PDAL_HIP(hipStreamEndCapture(stream_, &graph));
         PDAL_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
@@ -742,7 +742,7 @@ void Solver::record_graph(int K)
         graph_exec_ = nullptr;
     }
     hipGraph_t graph;
-    PDAL_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeGlobal));
+    PDAL_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     for (int i = 1; i <= K; ++i) launch_iteration(i, i == 1 || i == K);
     launch_eval();
     PDAL_HIP(hipStreamEndCapture(stream_, &graph));
@@ -1129,7 +1129,7 @@ void Solver::launch_polish_block(int K)
     }
     if (!polish_graph_) {
         hipGraph_t graph;
-        PDAL_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeGlobal));
+        PDAL_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
         body();
         PDAL_HIP(hipStreamEndCapture(stream_, &graph));
         PDAL_HIP(hipGraphInstantiate(&polish_graph_, graph, nullptr, nullptr, 0));

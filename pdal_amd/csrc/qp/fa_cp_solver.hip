@@ -743,7 +743,7 @@ void Solver::launch_inner_steps(const StepScalars &sc, int steps)
         if (bb_graph_) PDAL_HIP(hipGraphExecDestroy(bb_graph_));
         bb_graph_ = nullptr;
         hipGraph_t graph;
-        PDAL_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeGlobal));
+        PDAL_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
         for (int k = 0; k < kGraphSteps; ++k) bb_inner_step(sc);
         PDAL_HIP(hipStreamEndCapture(stream_, &graph));
         PDAL_HIP(hipGraphInstantiate(&bb_graph_, graph, nullptr, nullptr, 0));

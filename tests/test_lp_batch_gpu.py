@@ -60,7 +60,7 @@ def test_batch_iterates_track_oracle(oracle_lp, gpu_lp, monkeypatch, lam, blocks
             assert rel_err(g.reduced_cost, o.reduced_cost) <= ITER_RTOL, k
 
 
-@pytest.mark.parametrize("tol", [1e-4, 1e-6])
+@pytest.mark.parametrize("tol", [1e-4, 1e-7])
 def test_batch_solves_match_oracle_and_planted(oracle_lp, gpu_lp, monkeypatch, tol):
     monkeypatch.setenv("CP_AL_LAMBDA", "0")
     insts = mixed_instances(count=8, seed=1)
@@ -77,8 +77,13 @@ def test_batch_solves_match_oracle_and_planted(oracle_lp, gpu_lp, monkeypatch, t
             assert g.status == o.status == "OPTIMAL"
             assert g.iterations == o.iterations
             assert abs(g.primal_objective - o.primal_objective) <= OBJ_RTOL * abs(o.primal_objective)
-            # the gap test is relative to 1 + |p| + |d|: the planted-optimum bar carries the same 1 +
-            assert abs(g.primal_objective - inst.optimal_value) <= 10 * tol * (1.0 + abs(inst.optimal_value))
+            # the gap test is relative to 1 + |p| + |d|, so a KKT tolerance of tol bounds the
+            # objective error by about 2 tol (1 + |opt|) (plus the residual terms): at 1e-4 that
+            # bound is the bar; at 1e-7 the 1e-6 relative bar of the planted optimum holds
+            if tol >= 1e-4:
+                assert abs(g.primal_objective - inst.optimal_value) <= 10 * tol * (1.0 + abs(inst.optimal_value))
+            else:
+                assert abs(g.primal_objective - inst.optimal_value) <= 1e-6 * abs(inst.optimal_value)
             assert max(g.relative_primal_residual, g.relative_dual_residual, g.relative_objective_gap) < tol
 
 
@@ -159,11 +164,12 @@ def test_batch_norms_and_short_limit(oracle_lp, gpu_lp, monkeypatch, norm):
 
 def test_batch_of_config3_family(oracle_lp, gpu_lp, monkeypatch):
     """A few instances of the 4096-instance family (m log-uniform, n = 2m,
-    8 nonzeros per column, seed = instance index) solved to 1e-6."""
+    8 nonzeros per column, seed = instance index) solved to 1e-7, tight enough
+    for the 1e-6 relative bar on the planted objective."""
     monkeypatch.setenv("CP_AL_LAMBDA", "0")
     ms = [200, 1000, 350]
     insts = [config3_instance(i, m) for i, m in enumerate(ms)]
-    params = dict(termination_evaluation_frequency=K, eps_optimal_relative=1e-6, eps_feasible_relative=1e-6,
+    params = dict(termination_evaluation_frequency=K, eps_optimal_relative=1e-7, eps_feasible_relative=1e-7,
                   iteration_limit=500000)
     with LpBatch(gpu_lp, [i.lp_args() for i in insts], gpu_lp.default_params(**params)) as b:
         assert b.solve() == len(insts)
@@ -171,7 +177,7 @@ def test_batch_of_config3_family(oracle_lp, gpu_lp, monkeypatch):
             o = oracle_solve(oracle_lp, inst, oracle_lp.default_params(**params))
             g = b.result(k)
             assert g.iterations == o.iterations
-            assert abs(g.primal_objective - inst.optimal_value) <= 1e-5 * abs(inst.optimal_value)
+            assert abs(g.primal_objective - inst.optimal_value) <= 1e-6 * abs(inst.optimal_value)
 
 
 def test_compaction_does_not_change_results(gpu_lp, monkeypatch):
@@ -316,3 +322,25 @@ def test_overlapped_streams_match_one_stream(gpu_lp, monkeypatch):
         for a, b_ in zip(runs[0], other):
             assert a.status == b_.status and a.iterations == b_.iterations
             assert np.array_equal(a.x, b_.x) and np.array_equal(a.y, b_.y)
+
+
+@pytest.mark.parametrize("cr_iters", ["5", "20"])
+def test_batch_curtis_reid_tracks_oracle(oracle_lp, gpu_lp, monkeypatch, cr_iters):
+    """Curtis-Reid scaling before Ruiz (CP_AL_CR_SCALING, preconditioner.cu:162-268)
+    over the block-diagonal batch: each instance's iterates equal the oracle's
+    single-instance CR solve."""
+    monkeypatch.setenv("CP_AL_LAMBDA", "0")
+    monkeypatch.setenv("CP_AL_CR_SCALING", "1")
+    monkeypatch.setenv("CP_AL_CR_ITERS", cr_iters)
+    insts = mixed_instances(count=6, seed=4)
+    params = dict(iteration_limit=5 * K, termination_evaluation_frequency=K, eps_optimal_relative=1e-14,
+                  eps_feasible_relative=1e-14)
+    print(f"[case] batch curtis-reid iters={cr_iters}", flush=True)
+    with LpBatch(gpu_lp, [i.lp_args() for i in insts], gpu_lp.default_params(**params)) as b:
+        b.solve()
+        for k, inst in enumerate(insts):
+            o = oracle_solve(oracle_lp, inst, oracle_lp.default_params(**params))
+            g = b.result(k)
+            assert g.iterations == o.iterations == 5 * K
+            assert rel_err(g.x, o.x) <= ITER_RTOL, k
+            assert rel_err(g.y, o.y) <= ITER_RTOL, k

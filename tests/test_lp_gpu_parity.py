@@ -134,6 +134,27 @@ def test_sliced_passes_track_oracle(oracle_lp, gpu_lp, monkeypatch, lam, case):
     assert rel_err(g.reduced_cost, o.reduced_cost) <= ITER_RTOL
 
 
+@pytest.mark.parametrize("cr_iters", ["5", "20"])
+def test_curtis_reid_path_tracks_oracle(oracle_lp, gpu_lp, monkeypatch, cr_iters):
+    """The device Curtis-Reid passes (CP_AL_CR_SCALING, preconditioner.cu:162-268)
+    followed by Ruiz, Pock-Chambolle and bound-objective scaling: iterates
+    within 1e-10 of the oracle through 1000 iterations, and a 1e-8 solve with
+    the same iteration count and objective."""
+    monkeypatch.setenv("CP_AL_LAMBDA", "0")
+    monkeypatch.setenv("CP_AL_CR_SCALING", "1")
+    monkeypatch.setenv("CP_AL_CR_ITERS", cr_iters)
+    args = _edge_instance(13)
+    o, g = run_both(oracle_lp, gpu_lp, args, iteration_limit=1000, eps_optimal_relative=1e-14,
+                    eps_feasible_relative=1e-14)
+    assert o.iterations == g.iterations == 1000
+    assert rel_err(g.x, o.x) <= ITER_RTOL
+    assert rel_err(g.y, o.y) <= ITER_RTOL
+    inst = config0(seed=4)
+    o, g = run_both(oracle_lp, gpu_lp, inst.lp_args(), eps_optimal_relative=1e-8, eps_feasible_relative=1e-8)
+    assert o.status == g.status == "OPTIMAL" and g.iterations == o.iterations
+    assert abs(g.primal_objective - o.primal_objective) <= 1e-9 * abs(o.primal_objective)
+
+
 def test_input_formats_agree(gpu_lp):
     args = _edge_instance(11)
     A = args.pop("A")

@@ -277,3 +277,29 @@ def test_two_blocks_restart_anchor_and_pid(oracle_lp, monkeypatch):
     M2, d2 = _affine_T(A, b, c, eta / pw2, eta * pw2)
     z2 = _halpern_T_output(M2, d2, z1, K, 1.0)
     np.testing.assert_allclose(np.concatenate([r2.x, r2.y]), z2, atol=1e-9 * np.linalg.norm(z2))
+
+
+@pytest.mark.parametrize("cr_iters", ["5", "20"])
+def test_curtis_reid_solve_reaches_planted_optimum(oracle_lp, monkeypatch, cr_iters):
+    """Curtis-Reid scaling inside the full solve: scale -> solve -> unscale must
+    return the planted optimum of the unscaled problem (x*, y*, objective),
+    with coefficients spread over four decades so the scaling does work."""
+    from pdal_amd.generators import planted_lp
+    monkeypatch.setenv("CP_AL_LAMBDA", "0")
+    monkeypatch.setenv("CP_AL_CR_SCALING", "1")
+    monkeypatch.setenv("CP_AL_CR_ITERS", cr_iters)
+    inst = planted_lp(150, 300, 6, seed=17)
+    rng = np.random.default_rng(5)
+    r_scale = 10.0 ** rng.uniform(-2, 2, inst.m)  # badly scaled rows: A' = R A, b' = R b, same x*, y' = y / R
+    A = sp.diags(r_scale) @ inst.A
+    b = r_scale * inst.b
+    oracle_lp.reset_rng()
+    p = oracle_lp.default_params(eps_optimal_relative=1e-9, eps_feasible_relative=1e-9, iteration_limit=400000)
+    r = oracle_lp.solve(A=sp.csr_matrix(A), c=inst.c, con_lb=b, con_ub=b, var_lb=np.zeros(inst.n),
+                        var_ub=np.full(inst.n, np.inf), params=p)
+    assert r.status == "OPTIMAL"
+    assert abs(r.primal_objective - inst.optimal_value) <= 1e-7 * abs(inst.optimal_value)
+    np.testing.assert_allclose(A @ r.x, b, atol=1e-6 * (1 + np.abs(b).max()))
+    assert r.x.min() >= 0.0
+    lam = inst.c - A.T @ r.y
+    assert lam.min() >= -1e-6 * (1 + np.abs(inst.c).max())
