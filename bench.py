@@ -412,12 +412,68 @@ def run_config1(args, lib) -> dict:
     return result
 
 
+def bb_step_bytes(inst, form: str) -> int:
+    """Algorithmic bytes of one projected-BB inner step of FA_CP on the
+    config-2 QP: the Q product (explicit Q: 12 B per nonzero, 4 B per row
+    pointer, the gathered x once; low-rank form: the diagonal part, R.x and
+    R^T.(R x)), the gradient refresh (reads c, A^T u, x_T, x, grad, dir, writes
+    grad: 56 n) and the direction update (reads x_T, grad, lb, ub, writes dir,
+    x_T: 48 n)."""
+    n = inst.n
+    vec = 56 * n + 48 * n
+    if form == "explicit":
+        nnz = int(inst.q_explicit().nnz)
+        return 12 * nnz + 4 * (n + 1) + 8 * n + vec
+    r, nnz_r = inst.M.shape[0], int(inst.M.nnz)
+    diag = 12 * n + 4 * (n + 1) + 8 * n + 8 * n         # Q_diag . x_T into qx
+    rx = 12 * nnz_r + 4 * (r + 1) + 8 * n + 8 * r       # R . x_T
+    rtrx = 12 * nnz_r + 4 * (n + 1) + 8 * r + 8 * n     # R^T . (R x_T), + qx read
+    return diag + rx + rtrx + vec
+
+
+def run_config2_leg(args) -> dict:
+    """FA_CP on the planted QP (n = 1e5, m = 5e4, Q = M^T M + 1e-3 I with M
+    2e4 x n, 5 nonzeros per column; A 8 per column; box [0, 10]), in the
+    explicit-Q and the low-rank (R = M) form: one solve each to rel KKT tol,
+    capped at qp_time seconds; per inner BB step, wall time and algorithmic
+    bytes (a whole-step roofline: every kernel of the step is in the time)."""
+    from pdal_amd.generators import config2
+    from pdal_amd.qp import load_fa_cp
+    qlib = load_fa_cp()
+    inst = config2(seed=2)
+    out = {"workload": "planted QP n=1e5 m=5e4, Q=M^T M+1e-3 I (M 2e4 x n, 5 nnz/col), A 8 nnz/col, box [0,10]",
+           "tolerance": args.tol, "time_cap_s": args.qp_time}
+    objs = {}
+    for form in ("explicit", "lowrank"):
+        qargs = inst.qp_args(form)
+        p = qlib.default_params(eps_optimal_relative=args.tol, eps_feasible_relative=args.tol, verbose=0,
+                                time_sec_limit=args.qp_time, iteration_limit=10_000_000)
+        qlib.reset_rng()
+        t0 = time.perf_counter()
+        r = qlib.solve(**qargs, params=p)
+        wall = time.perf_counter() - t0
+        us_step = 1e6 * r.solve_time_sec / max(r.inner_iterations, 1)
+        bts = bb_step_bytes(inst, form)
+        gbs = bts / (us_step * 1e-6) / 1e9
+        objs[form] = r.primal_objective
+        out[form] = {"status": r.status, "iterations": r.iterations, "inner_steps": r.inner_iterations,
+                     "solve_s": round(r.solve_time_sec, 3), "wall_s": round(wall, 3), "kkt": r.kkt,
+                     "us_per_inner_step": round(us_step, 3), "bytes_per_inner_step": bts,
+                     "step_gbs": round(gbs, 1), "step_frac_of_peak": round(gbs / HBM_PEAK_GBS, 4),
+                     "objective_rel_err_vs_planted": abs(r.primal_objective - inst.optimal_value)
+                     / abs(inst.optimal_value)}
+        log(f"config2 {form}: {out[form]}")
+    if all(k in objs for k in ("explicit", "lowrank")):
+        out["explicit_vs_lowrank_objective_rel"] = abs(objs["explicit"] - objs["lowrank"]) / abs(objs["explicit"])
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["config4", "config3", "config1"], default="config4")
+    ap.add_argument("--workload", choices=["config4", "config3", "config1", "config2"], default="config4")
     ap.add_argument("--heldout", type=int, default=256, help="config4: held-out instances per batch")
     ap.add_argument("--train", type=int, default=512, help="config4: training solves")
     ap.add_argument("--train-iters", type=int, default=40000, help="config4: iteration cap of the training solves")
@@ -435,6 +491,8 @@ def main() -> int:
     ap.add_argument("--no-spmv", action="store_true", help="skip the large-LP SpMV leg")
     ap.add_argument("--no-batch-leg", action="store_true", help="config4: skip the 4096-instance batch leg")
     ap.add_argument("--kernels-only", action="store_true", help="set up and time kernels only (profiling)")
+    ap.add_argument("--no-qp", action="store_true", help="config4: skip the FA_CP config-2 leg")
+    ap.add_argument("--qp-time", type=float, default=25.0, help="config-2 leg: time cap per FA_CP solve (s)")
     ap.add_argument("--kernel-iters", type=int, default=20)
     args = ap.parse_args()
 
@@ -448,6 +506,10 @@ def main() -> int:
 
     from pdal_amd.lp import load_cp_al
     lib = load_cp_al()
+    if args.workload == "config2":
+        res = run_config2_leg(args)
+        print(json.dumps({"metric": "FA_CP planted QP, inner BB steps (see fields)", "config2": res}), flush=True)
+        return 0
     if args.workload == "config4":
         res = run_config4(args, lib)
         metric = ("solves/sec to rel-KKT 1e-6, batched + learned warm start (fixed-A LP family, 256 held-out "
@@ -487,6 +549,8 @@ def main() -> int:
         out["batch_4096"] = leg
     if args.workload in ("config4", "config3") and not args.no_spmv:
         out["spmv"] = spmv_leg(lib, args.kernel_iters)
+    if args.workload == "config4" and not args.no_qp:
+        out["qp_config2"] = run_config2_leg(args)
     out["roofline"] = roof
     out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)

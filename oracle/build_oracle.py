@@ -6,7 +6,8 @@ summation-order noise floor of a trajectory) and liblin_cp_al_oracle.so (the
 linearized CP-AL QP solver, the FA_CP source built with -DORACLE_LIN_CP_AL) and
 libplanted_fa_cp_oracle.so (the planted-KKT FA_CP, -DORACLE_PLANTED_FA_CP) and
 libplanted_cp_al_oracle.so (the planted-KKT CP-AL, -DORACLE_PLANTED_CP_AL), each
-with a reverse-summing build.  The oracle exports the same C names
+with a reverse-summing build, and libcp_al_oracle_omp.so (the LP oracle at -O3
+with OpenMP: the CPU baseline of the benchmark).  The oracle exports the same C names
 as the product library; it is built with hidden visibility and -Bsymbolic so a
 process can load both without the two binding to each other's symbols.
 
@@ -40,6 +41,9 @@ def _one(src: str, name: str, defines=()) -> Path:
 
 
 def build() -> Path:
+    # the CPU baseline's build: -O3 for an x86-64-v3 host and OpenMP over the
+    # host's cores (same source, same results: no FMA contraction, no reassociation)
+    _one("cp_al_oracle.cpp", "libcp_al_oracle_omp.so", ["-O3", "-march=x86-64-v3", "-fopenmp"])
     _one("fa_cp_oracle.cpp", "libfa_cp_oracle.so")
     _one("fa_cp_oracle.cpp", "libfa_cp_oracle_rev.so", ["-DFA_CP_ORACLE_REVERSE_SUMS"])
     _one("fa_cp_oracle.cpp", "liblin_cp_al_oracle.so", ["-DORACLE_LIN_CP_AL"])

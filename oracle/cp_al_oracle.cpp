@@ -58,14 +58,19 @@
 #include <cstdlib>
 #include <cstring>
 #include <random>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <vector>
 
 #define ORACLE_API extern "C" __attribute__((visibility("default")))
 
 namespace {
 
-std::mt19937 g_gen(1);                              // utils.cu:27
-std::normal_distribution<double> g_dist(0.0, 1.0);  // utils.cu:28
+// Per thread: a single-threaded caller sees the reference's process-global
+// stream; the parallel CPU-baseline entry point gives every thread its own.
+thread_local std::mt19937 g_gen(1);                              // utils.cu:27
+thread_local std::normal_distribution<double> g_dist(0.0, 1.0);  // utils.cu:28
 
 double env_double(const char *name, double def)  // solver.cu:377-391, utils.cu:33-46
 {
@@ -1205,6 +1210,35 @@ ORACLE_API cp_al_result_t *solve_lp_problem(lp_problem_t *prob, const cp_al_para
 /* ---------------- test hooks (pin individual oracle functions) ---------------- */
 
 // sigma_max(A) by the oracle's power iteration (utils.cu:87-228), generator reset first.
+// CPU baseline (no counterpart in the reference): solve problems[0..count)
+// with `params` on `nthreads` threads (OpenMP when built with -fopenmp; one at
+// a time otherwise), each solve starting from a freshly seeded power-iteration
+// generator, as a batch of independent reference solves would.  out[i] takes
+// problem i's result (free with cp_al_result_free).
+ORACLE_API void cp_al_oracle_solve_many(lp_problem_t **problems, int count, const cp_al_parameters_t *params,
+                                        int nthreads, cp_al_result_t **out)
+{
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nthreads > 0 ? nthreads : 1) schedule(dynamic, 1)
+#endif
+    for (int i = 0; i < count; ++i) {
+        g_gen.seed(1);
+        g_dist.reset();
+        out[i] = optimize(params, problems[i]);
+    }
+    (void)nthreads;
+}
+
+// Threads the parallel entry point can use (1 without OpenMP).
+ORACLE_API int cp_al_oracle_max_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
 ORACLE_API double cp_al_oracle_sigma_max(int m, int n, const int *rp, const int *ci, const double *v, int max_iter,
                                          double tol)
 {

@@ -227,11 +227,48 @@ struct BBArgs {
     const double *__restrict__ dual_prod;
     const double *__restrict__ qx;
     double *__restrict__ xT, *__restrict__ grad, *__restrict__ dir;
-    double *__restrict__ parts;        // per-block partials
+    double *__restrict__ parts;        // per-block partials of the kernel
+    const double *__restrict__ prev;   // the previous kernel's per-block partials
+    int nprev;                         // how many
     BBState *bb;
     StepScalars sc;
     int n;
+    double tol;                        // inner stopping tolerance on ||direction||
+    int limit;                         // inner iteration limit
 };
+
+// Sum of parts[0..nparts) in the fixed order of the single-block reductions
+// (thread t over b = t, t + 256, ...; lane butterfly; wavefronts in order), so
+// every block that runs it gets the same bits.  The result is in every thread.
+__device__ __forceinline__ double block_sum_parts(const double *parts, int nparts)
+{
+    __shared__ double red[kWavesPerBlock];
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nparts; b += kBlock) acc += parts[b];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = acc;
+    __syncthreads();
+    double r = red[0];
+    for (int w = 1; w < kWavesPerBlock; ++w) r += red[w];
+    __syncthreads();
+    return r;
+}
+
+// The loop test after a direction update (:403-408): every block of the
+// calling kernel reduces the direction partials of the previous kernel itself
+// and reaches the same decision from the same bits (the counter it was
+// advanced by that kernel and is only read here); block 0 records ||d|| and
+// the flag.  Returns true when the loop has ended (the caller returns at once).
+__device__ __forceinline__ bool bb_loop_test(const BBArgs &a)
+{
+    const double dn = sqrt(block_sum_parts(a.prev, a.nprev));
+    const bool done = !(a.bb->it < a.limit) || dn <= a.tol;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.bb->dnorm = dn;
+        if (done) a.bb->done = 1;
+    }
+    return done;
+}
 
 // bb_init (:381-399): gradient at x, first projected step with alpha = 1/(1/tau).
 static __global__ void __launch_bounds__(kBlock) bb_init_kernel(BBArgs a)
@@ -253,12 +290,14 @@ static __global__ void __launch_bounds__(kBlock) bb_init_kernel(BBArgs a)
 }
 
 // Gradient refresh (:401-415) fused with the product row: cur = Q.xT + c - dp +
-// (xT - x)/tau; grad = cur; partial <dir, cur - grad_old>.
-template <int G, int MODE>
+// (xT - x)/tau; grad = cur; partial <dir, cur - grad_old>.  TEST: first run the
+// loop test of the direction update before it (a.prev = its partials).
+template <int G, int MODE, bool TEST>
 static __global__ void __launch_bounds__(kBlock)
     bb_grad_kernel(BBArgs a, CsrView Qm, CsrView Rt, const double *__restrict__ rx, const double *__restrict__ qdiag)
 {
     if (a.bb->done) return;
+    if (TEST && bb_loop_test(a)) return;  // the previous step's loop test, folded in
     constexpr int RPB = kBlock / G;
     const double inv = 1.0 / a.sc.tau;
     double q[1] = {0.0};
@@ -282,28 +321,17 @@ static __global__ void __launch_bounds__(kBlock)
     block_reduce_store<1, 0>(q, a.parts + blockIdx.x);
 }
 
-// alpha = ||d||^2 / <d, dg> (single block, :412-413).
-static __global__ void __launch_bounds__(kBlock) bb_alpha_kernel(const double *__restrict__ parts, int nparts, BBState *bb)
-{
-    if (bb->done) return;
-    __shared__ double red[kWavesPerBlock];
-    double acc = 0.0;
-    for (int b = threadIdx.x; b < nparts; b += kBlock) acc += parts[b];
-    acc = wave_sum(acc);
-    if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double r = red[0];
-        for (int w = 1; w < kWavesPerBlock; ++w) r += red[w];
-        bb->alpha = bb->dnorm * bb->dnorm / r;
-    }
-}
-
-// Direction update (:417-430) with partial ||d||^2.
+// Direction update (:417-430) with partial ||d||^2; the step alpha = ||d||^2 /
+// <d, dg> (:412-413) is computed by every block from the gradient kernel's
+// partials (a.prev), block 0 records it.
 static __global__ void __launch_bounds__(kBlock) bb_dir_kernel(BBArgs a)
 {
     if (a.bb->done) return;
-    const double alpha = a.bb->alpha;
+    const double alpha = a.bb->dnorm * a.bb->dnorm / block_sum_parts(a.prev, a.nprev);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {  // no block of this kernel reads these
+        a.bb->alpha = alpha;
+        a.bb->it += 1;  // inner_solver_iter++ after the direction update (:431)
+    }
     double q[1] = {0.0};
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < a.n; i += gridDim.x * kBlock) {
         const double cs = a.xT[i];
@@ -317,7 +345,9 @@ static __global__ void __launch_bounds__(kBlock) bb_dir_kernel(BBArgs a)
     block_reduce_store<1, 0>(q, a.parts + blockIdx.x);
 }
 
-// ||d|| and the loop test.  FIRST: the norm after bb_init (it stays 1).
+// ||d|| and the loop test as a kernel of its own: FIRST after bb_init (it = 1);
+// otherwise after the last direction update of a chunk (the direction kernel
+// has advanced it).
 template <bool FIRST>
 static __global__ void __launch_bounds__(kBlock)
     bb_check_kernel(const double *__restrict__ parts, int nparts, BBState *bb, double tol, int limit)
@@ -335,8 +365,6 @@ static __global__ void __launch_bounds__(kBlock)
         if (FIRST) {
             bb->it = 1;
             bb->done = 0;
-        } else {
-            bb->it += 1;
         }
         bb->dnorm = sqrt(r);
         if (!(bb->it < limit) || bb->dnorm <= tol) bb->done = 1;

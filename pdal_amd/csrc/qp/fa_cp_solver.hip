@@ -172,7 +172,8 @@ class Solver {
     bool major_kernel() const { return major_ || ((total_ + 2) % print_frequency(total_ + 2)) == 0; }
     void update(bool fuse_halpern);
     void primal_bb(const StepScalars &sc, bool fuse_halpern);
-    void bb_inner_step(const StepScalars &sc);
+    void bb_inner_step(const StepScalars &sc, bool test_first);
+    void bb_loop_test();
     void halpern();
     void fixed_point_error();
     void compute_residual();
@@ -263,7 +264,7 @@ class Solver {
 
     DBuf<double> x0_, x_, xT_, xR_, dual_prod_, dslack_, dx_, qx_, grad_, dir_;
     DBuf<double> y0_, y_, yT_, yR_, prim_prod_, al_u_, pslack_, dy_, rx_;
-    DBuf<double> parts_, bbparts_, dscal_, scratch_e_, scratch_d_;
+    DBuf<double> parts_, bbparts_, bbparts_g_, dscal_, scratch_e_, scratch_d_;
     PinnedBuf<double> hscal_{S_COUNT};
     DBuf<BBState> dbb_;
     PinnedBuf<BBState> hbb_{1};
@@ -352,6 +353,7 @@ void Solver::upload()
     scratch_d_.alloc(n_);
     parts_.alloc((size_t)kMaxPartBlocks * kEvalStride);
     bbparts_.alloc(kMaxPartBlocks);
+    bbparts_g_.alloc(kMaxPartBlocks);
     dscal_.alloc(S_COUNT);
     dbb_.alloc(1);
     PDAL_HIP(hipMemsetAsync(dscal_.get(), 0, S_COUNT * sizeof(double), stream_));
@@ -684,8 +686,12 @@ StepScalars Solver::scalars() const
     return sc;
 }
 
-void Solver::bb_inner_step(const StepScalars &sc)
-{   // one pass of the loop body in primal_BB_step_size_update, :452-500
+void Solver::bb_inner_step(const StepScalars &sc, bool test_first)
+{   // one pass of the loop body in primal_BB_step_size_update, :452-500.  The
+    // loop test of the previous direction update runs at the head of the
+    // gradient kernel when test_first (else a separate kernel ran it); the BB
+    // step runs at the head of the direction kernel.  The gradient partials
+    // go to bbparts_g_, the direction partials to bbparts_.
     BBState *bb = dbb_.get();
     if (bb_al()) {
         dispatch_group(A_.G, [&](auto g) {
@@ -695,14 +701,18 @@ void Solver::bb_inner_step(const StepScalars &sc)
         });
         spmv(At_, al_u_.get(), dual_prod_.get(), bb);
     }
+    const int nbd = stride_grid(n_);
     BBArgs a{c_.get(), lb_.get(), ub_.get(), x_.get(), dual_prod_.get(), qx_.get(), xT_.get(), grad_.get(),
-             dir_.get(), bbparts_.get(), bb, sc, n_};
+             dir_.get(), bbparts_g_.get(), bbparts_.get(), nbd, bb, sc, n_, inner_tol_, inner_limit_};
     int nb = 0;
     auto grad = [&](auto gtag, auto mtag, const DevCsr &M) {
         constexpr int G = decltype(gtag)::value;
         constexpr int MODE = decltype(mtag)::value;
         nb = std::max(1, std::min(kMaxPartBlocks, blocks_for((long)n_ * G)));
-        bb_grad_kernel<G, MODE><<<nb, kBlock, 0, stream_>>>(a, Q_.view(), M.view(), rx_.get(), qdiag_.get());
+        if (test_first)
+            bb_grad_kernel<G, MODE, true><<<nb, kBlock, 0, stream_>>>(a, Q_.view(), M.view(), rx_.get(), qdiag_.get());
+        else
+            bb_grad_kernel<G, MODE, false><<<nb, kBlock, 0, stream_>>>(a, Q_.view(), M.view(), rx_.get(), qdiag_.get());
     };
     switch (qtype_) {
     case FA_CP_SPARSE_Q:
@@ -725,17 +735,27 @@ void Solver::bb_inner_step(const StepScalars &sc)
     default: grad(std::integral_constant<int, 1>{}, std::integral_constant<int, 3>{}, Rt_); break;
     }
     PDAL_LAUNCHED();
-    bb_alpha_kernel<<<1, kBlock, 0, stream_>>>(bbparts_.get(), nb, bb);
-    const int nbd = stride_grid(n_);
-    bb_dir_kernel<<<nbd, kBlock, 0, stream_>>>(a);
-    bb_check_kernel<false><<<1, kBlock, 0, stream_>>>(bbparts_.get(), nbd, bb, inner_tol_, inner_limit_);
+    BBArgs ad = a;
+    ad.parts = bbparts_.get();
+    ad.prev = bbparts_g_.get();
+    ad.nprev = nb;
+    bb_dir_kernel<<<nbd, kBlock, 0, stream_>>>(ad);
+    PDAL_LAUNCHED();
+}
+
+void Solver::bb_loop_test()
+{   // the loop test of the last direction update of a chunk, on its own
+    bb_check_kernel<false><<<1, kBlock, 0, stream_>>>(bbparts_.get(), stride_grid(n_), dbb_.get(), inner_tol_,
+                                                      inner_limit_);
     PDAL_LAUNCHED();
 }
 
 void Solver::launch_inner_steps(const StepScalars &sc, int steps)
-{
+{   // every chunk of steps tests the loop after its last direction update, so
+    // the host's read of the done flag after a chunk is current
     if (!use_graph_) {
-        for (int k = 0; k < steps; ++k) bb_inner_step(sc);
+        for (int k = 0; k < steps; ++k) bb_inner_step(sc, k > 0);
+        bb_loop_test();
         return;
     }
     if (!bb_graph_ || g_tau_ != sc.tau || g_al_ != sc.al_sigma || g_tol_ != inner_tol_ || g_limit_ != inner_limit_ ||
@@ -744,7 +764,8 @@ void Solver::launch_inner_steps(const StepScalars &sc, int steps)
         bb_graph_ = nullptr;
         hipGraph_t graph;
         PDAL_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-        for (int k = 0; k < kGraphSteps; ++k) bb_inner_step(sc);
+        for (int k = 0; k < kGraphSteps; ++k) bb_inner_step(sc, k > 0);
+        bb_loop_test();
         PDAL_HIP(hipStreamEndCapture(stream_, &graph));
         PDAL_HIP(hipGraphInstantiate(&bb_graph_, graph, nullptr, nullptr, 0));
         PDAL_HIP(hipGraphDestroy(graph));
@@ -769,7 +790,7 @@ void Solver::primal_bb(const StepScalars &sc, bool fuse_halpern)
     }
     obj_product(x_.get());
     BBArgs a{c_.get(), lb_.get(), ub_.get(), x_.get(), dual_prod_.get(), qx_.get(), xT_.get(), grad_.get(),
-             dir_.get(), bbparts_.get(), dbb_.get(), sc, n_};
+             dir_.get(), bbparts_.get(), nullptr, 0, dbb_.get(), sc, n_, inner_tol_, inner_limit_};
     const int nbd = stride_grid(n_);
     bb_init_kernel<<<nbd, kBlock, 0, stream_>>>(a);
     bb_check_kernel<true><<<1, kBlock, 0, stream_>>>(bbparts_.get(), nbd, dbb_.get(), inner_tol_, inner_limit_);
