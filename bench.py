@@ -27,8 +27,13 @@ The same JSON line carries:
     solves/s and its own roofline;
   * spmv: the plain path's fused SpMV kernels on the large planted LP
     (m = 2e6, n = 4e6, Zipf(1.5) column counts) with their HBM GB/s;
-  * cpu_baseline: the FP64 oracle (single thread, cold) on a bounded sample of
-    held-out members, scaled to solves/s with the GPU cold iteration counts.
+  * cpu_baseline: the FP64 oracle (-O3, OpenMP over the host's threads, cold)
+    on a bounded sample of held-out members, scaled to solves/s with the GPU
+    cold iteration counts, with the one-thread figure, the thread count and
+    the CPU model; beside it, full oracle and GPU solves of the small planted
+    LP at 1e-4 and 1e-8 (cpu_config0_full_solves);
+  * qp_config2: FA_CP on the planted QP (explicit and low-rank Q), one solve
+    each to 1e-6 within a time cap, time and bytes per inner BB step.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
        python bench.py --workload config3   (the 4096-instance batch alone)
@@ -226,6 +231,87 @@ def run_config3(args, lib) -> dict:
     return result
 
 
+def host_cpu() -> dict:
+    """Threads this process may use and the CPU model of the host."""
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"threads": cores, "model": model}
+
+
+def cpu_baseline_family_parallel(insts, gpu_iters: list, iters_cap: int, threads: int) -> dict:
+    """Oracle (FP64 port, -O3, OpenMP) on the host: the first held-out member
+    on one thread, and `threads` members at once on `threads` threads, each
+    capped at `iters_cap` iterations; each member's full-solve time is its GPU
+    cold iteration count over the measured iterations per second on it."""
+    from pdal_amd.lp import CpAl
+    oracle = CpAl(ROOT / "oracle" / "_build" / "libcp_al_oracle_omp.so")
+    p = oracle.default_params(iteration_limit=iters_cap, termination_evaluation_frequency=K_BATCH,
+                              eps_optimal_relative=1e-6, eps_feasible_relative=1e-6)
+    # iterations per second from each solve's own clock (started after its set-up)
+    t0 = time.perf_counter()
+    r1 = oracle.solve_many([insts[0]], p, 1)[0]
+    one_s = time.perf_counter() - t0
+    single = 1.0 / (gpu_iters[0] * r1.solve_time_sec / r1.iterations)
+    k = min(threads, len(insts))
+    t0 = time.perf_counter()
+    rs = oracle.solve_many(insts[:k], p, k)
+    wall = time.perf_counter() - t0
+    # k members run side by side: the k-thread rate is k over a member's mean full-solve time
+    full = sum(gpu_iters[i] * r.solve_time_sec / r.iterations for i, r in enumerate(rs)) / k
+    host = host_cpu()
+    log(f"CPU baseline: 1 thread {r1.iterations} it in {one_s:.1f} s; {k} threads {sum(r.iterations for r in rs)} it "
+        f"in {wall:.1f} s wall ({host})")
+    return {"value": round(k / full, 5), "unit": "solves/s", "cores": k, "kind": "port",
+            "single_thread_value": round(single, 5), "cpu_model": host["model"], "host_threads": host["threads"],
+            "sample": (f"oracle -O3 with OpenMP: {k} held-out members on {k} threads at once, {iters_cap} cold "
+                       f"iterations each ({wall:.1f} s wall incl. set-up), and member 0 alone on one thread "
+                       f"({one_s:.1f} s); each member's full-solve time = its GPU cold iteration count x the "
+                       f"oracle's measured time per iteration on it (set-up excluded)")}
+
+
+def cpu_config0_full_solves(gpu_lib, threads: int, seeds: int = 8) -> dict:
+    """Full solves of the small planted LP (m = 1000, n = 2000, 10 nonzeros
+    per column) at 1e-4 and 1e-8: the oracle on one thread and on `threads`
+    threads, and the plain single-instance GPU path, on the same instances."""
+    import numpy as np
+    from pdal_amd.generators import config0
+    from pdal_amd.lp import CpAl
+    oracle = CpAl(ROOT / "oracle" / "_build" / "libcp_al_oracle_omp.so")
+    insts = [config0(seed=s).lp_args() for s in range(seeds)]
+    out = {"workload": f"planted LP m=1000 n=2000, 10 nnz/col, seeds 0..{seeds - 1}, full solves"}
+    for tol in (1e-4, 1e-8):
+        p = oracle.default_params(eps_optimal_relative=tol, eps_feasible_relative=tol)
+        t0 = time.perf_counter()
+        one = oracle.solve_many(insts[:2], p, 1)
+        t_one = time.perf_counter() - t0
+        k = min(threads, len(insts))
+        t0 = time.perf_counter()
+        many = oracle.solve_many(insts[:k], p, k)
+        t_many = time.perf_counter() - t0
+        gp = gpu_lib.default_params(eps_optimal_relative=tol, eps_feasible_relative=tol)
+        t0 = time.perf_counter()
+        g = []
+        for kw in insts[:k]:
+            gpu_lib.reset_rng()
+            g.append(gpu_lib.solve(**kw, params=gp))
+        t_gpu = time.perf_counter() - t0
+        same = all(a.iterations == b.iterations and a.status == b.status == "OPTIMAL" for a, b in zip(many, g))
+        out[f"tol_{tol:g}"] = {"cpu_1thread_solves_per_s": round(len(one) / t_one, 3),
+                               f"cpu_{k}threads_solves_per_s": round(k / t_many, 3),
+                               "gpu_plain_solves_per_s": round(k / t_gpu, 3),
+                               "iterations_mean": float(np.mean([r.iterations for r in many])),
+                               "gpu_equals_oracle_iterations": same}
+        log(f"config0 tol {tol:g}: {out[f'tol_{tol:g}']}")
+    return out
+
+
 def cpu_baseline_family(insts, gpu_iters: list, sample: int, iters_cap: int) -> dict:
     """Oracle (single-threaded FP64 port of the reference's CP_AL, cold start):
     `iters_cap` iterations on each of a few held-out instances, scaled to full
@@ -367,9 +453,11 @@ def run_config4(args, lib) -> dict:
                      "head_rel_resid": info.get("head_rel_resid")},
     })
     if not args.no_cpu:
-        log("CPU baseline (oracle, 1 thread) ...")
-        result["cpu_baseline"] = cpu_baseline_family(test_inst, [int(i) for i in cold_its], args.cpu_family_sample,
-                                                     args.cpu_family_iters)
+        threads = min(host_cpu()["threads"], args.cpu_threads)
+        log(f"CPU baseline (oracle, 1 and {threads} threads) ...")
+        result["cpu_baseline"] = cpu_baseline_family_parallel(test_inst, [int(i) for i in cold_its],
+                                                              args.cpu_family_iters, threads)
+        result["cpu_config0_full_solves"] = cpu_config0_full_solves(lib, threads)
     return result
 
 
@@ -486,7 +574,8 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=8)
     ap.add_argument("--cpu-iters", type=int, default=6400)
     ap.add_argument("--cpu-family-sample", type=int, default=2)
-    ap.add_argument("--cpu-family-iters", type=int, default=6400)
+    ap.add_argument("--cpu-family-iters", type=int, default=640)
+    ap.add_argument("--cpu-threads", type=int, default=16, help="CPU baseline: at most this many host threads")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-spmv", action="store_true", help="skip the large-LP SpMV leg")
     ap.add_argument("--no-batch-leg", action="store_true", help="config4: skip the 4096-instance batch leg")

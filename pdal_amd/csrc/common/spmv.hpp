@@ -21,25 +21,12 @@ struct CsrView {
     int rows;
 };
 
-// A lane's nonzeros are taken four at a time with all twelve loads issued
-// before the first product is added, then added in index order: the same sum,
-// bit for bit, as the one-at-a-time loop, with four gathers in flight.
 template <int G>
 __device__ __forceinline__ double row_dot(const CsrView &M, int row, int lane, const double *__restrict__ x)
 {
     double s = 0.0;
     const int b = M.rp[row], e = M.rp[row + 1];
-    int k = b + lane;
-    for (; k + 3 * G < e; k += 4 * G) {
-        const int c0 = M.ci[k], c1 = M.ci[k + G], c2 = M.ci[k + 2 * G], c3 = M.ci[k + 3 * G];
-        const double v0 = M.v[k], v1 = M.v[k + G], v2 = M.v[k + 2 * G], v3 = M.v[k + 3 * G];
-        const double x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
-        s += v0 * x0;
-        s += v1 * x1;
-        s += v2 * x2;
-        s += v3 * x3;
-    }
-    for (; k < e; k += G) s += M.v[k] * x[M.ci[k]];
+    for (int k = b + lane; k < e; k += G) s += M.v[k] * x[M.ci[k]];
     return group_sum<G>(s);
 }
 

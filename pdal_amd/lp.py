@@ -212,6 +212,13 @@ class CpAl:
                 self._reset.restype = None
                 self._reset.argtypes = []
                 break
+        if hasattr(L, "cp_al_oracle_solve_many"):
+            L.cp_al_oracle_solve_many.restype = None
+            L.cp_al_oracle_solve_many.argtypes = [C.POINTER(C.POINTER(LpProblem)), C.c_int,
+                                                  C.POINTER(CpAlParameters), C.c_int,
+                                                  C.POINTER(C.POINTER(CpAlResult))]
+            L.cp_al_oracle_max_threads.restype = C.c_int
+            L.cp_al_oracle_max_threads.argtypes = []
         if hasattr(L, "pdal_cp_al_time_kernels"):
             L.pdal_cp_al_time_kernels.restype = C.c_int
             L.pdal_cp_al_time_kernels.argtypes = [C.POINTER(LpProblem), C.POINTER(CpAlParameters), C.c_int, _dp]
@@ -329,6 +336,27 @@ class CpAl:
         if not res:
             raise RuntimeError("solve_lp_problem returned NULL")
         return self._take_result(res, prob)
+
+    def solve_many(self, instances: list, params: CpAlParameters, threads: int) -> list:
+        """Solve a list of problems (keyword dicts of make_problem) on `threads`
+        host threads through the oracle's parallel entry point (CPU baseline
+        only); every solve starts from a freshly seeded generator."""
+        if not hasattr(self.lib, "cp_al_oracle_solve_many"):
+            raise AttributeError(f"{self.path.name} has no parallel solve entry point")
+        probs = [self.make_problem(**kw) for kw in instances]
+        arr = (C.POINTER(LpProblem) * len(probs))(*probs)
+        out = (C.POINTER(CpAlResult) * len(probs))()
+        try:
+            self.lib.cp_al_oracle_solve_many(arr, len(probs), C.byref(params), int(threads), out)
+            res = []
+            for k, p in enumerate(probs):
+                if not out[k]:
+                    raise RuntimeError(f"oracle solve {k} returned NULL")
+                res.append(self._take_result(out[k], p))
+            return res
+        finally:
+            for p in probs:
+                self.free_problem(p)
 
     def _take_result(self, res, prob) -> LpResult:
         try:
