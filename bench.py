@@ -397,31 +397,63 @@ def run_config4(args, lib) -> dict:
     cold_its = np.array([batch.result(k).iterations for k in range(count)])
     log(f"cold batched: {count} in {cold_s:.2f} s, iterations mean {cold_its.mean():.0f} max {cold_its.max()}")
 
-    def warm_step():
-        lws.predict_into(batch, te)  # MLP + head, POD expansion into the batch's start buffers on the device
+    # Every timed step solves a fresh batch of held-out members: the next
+    # `count` parameter vectors of the pool.  Their b(theta), c(theta) are
+    # input data, made on the host and put in device memory before the timed
+    # region; the step replaces the batch's vectors from there
+    # (pdal_cp_al_batch_set_vectors_device: the scaling passes replayed, A's
+    # scaling and norm estimate kept), predicts the starts and solves.
+    from pdal_amd.devbuf import DeviceArray
+    n_batches = args.warmup + args.steps
+    pool = config4_thetas(count * (1 + n_batches), 2)[count:]  # the first `count` rows are te
+    p = fam.p
+    batches = []
+    t0 = time.perf_counter()
+    for s in range(n_batches):
+        th = pool[s * count:(s + 1) * count]
+        for t in th:
+            fam.solution(t)  # the planted-region check of every member
+        bvec = DeviceArray.from_host((fam.base.b[:, None] + fam.U @ th[:, :p].T).T)
+        cvec = DeviceArray.from_host((fam.base.c[:, None] + fam.V @ th[:, p:].T).T)
+        batches.append((th, bvec, cvec))
+    log(f"{n_batches} fresh batches of {count} members in device memory ({time.perf_counter() - t0:.1f} s)")
+
+    def warm_step(th, bvec, cvec):
+        t_up = time.perf_counter()
+        batch.set_vectors_device(c=cvec, con_lb=bvec, con_ub=bvec)
+        t_up = time.perf_counter() - t_up
+        lws.predict_into(batch, th)  # MLP + head, POD expansion into the batch's start buffers on the device
         n_opt = batch.solve()
         if n_opt < count:  # the cold fallback for what the warm pass left unsolved
             bad = [k for k in range(count) if batch.result(k).status != "OPTIMAL"]
-            with LpBatch(lib, [test_inst[k] for k in bad], params) as fb:
+            with LpBatch(lib, [fam.instance(th[k]) for k in bad], params) as fb:
                 fb.solve()
-            return len(bad)
-        return 0
+            return len(bad), t_up
+        return 0, t_up
 
     for w in range(args.warmup):
-        warm_step()
-    times, fallbacks, splits = [], 0, []
+        warm_step(*batches[w])
+    times, fallbacks, splits, updates, obj_err, warm_its = [], 0, [], [], 0.0, []
     for s in range(args.steps):
+        th, bvec, cvec = batches[args.warmup + s]
         t1 = time.perf_counter()
-        fallbacks += warm_step()
+        fb, t_up = warm_step(th, bvec, cvec)
         times.append(time.perf_counter() - t1)
+        fallbacks += fb
+        updates.append(t_up)
         st = batch.stats()
         splits.append((st["solve_s"], st["half_stopped_s"], st["p99_stopped_s"], st["handoffs"]))
-        log(f"warm step {s}: {count} held-out solves in {times[-1]:.3f} s (solve {st['solve_s']:.3f} s, "
-            f"half stopped {st['half_stopped_s']:.3f} s, 99 % {st['p99_stopped_s']:.3f} s)")
-    res = [batch.result(k) for k in range(count)]
-    warm_its = np.array([r.iterations for r in res])
-    obj_err = max(abs(r.primal_objective - fam.optimal_value(t)) / (1.0 + abs(fam.optimal_value(t)))
-                  for r, t in zip(res, te))
+        res = [batch.result(k) for k in range(count)]
+        warm_its += [r.iterations for r in res]
+        obj_err = max(obj_err, max(abs(r.primal_objective - fam.optimal_value(t)) / (1.0 + abs(fam.optimal_value(t)))
+                                   for r, t in zip(res, th) if r.status == "OPTIMAL"))
+        log(f"warm step {s}: {count} fresh held-out solves in {times[-1]:.3f} s (vector update {t_up * 1e3:.1f} ms, "
+            f"solve {st['solve_s']:.3f} s, half stopped {st['half_stopped_s']:.3f} s, "
+            f"99 % {st['p99_stopped_s']:.3f} s, fallbacks {fb})")
+    warm_its = np.array(warm_its)
+    for _, bvec, cvec in batches:
+        bvec.close()
+        cvec.close()
     batch.close()
     lws.close()
     total = sum(times)
@@ -440,6 +472,8 @@ def run_config4(args, lib) -> dict:
     result.update({
         "value": value, "ms_per_step": 1e3 * total / args.steps, "optimal": int(sum(r.status == "OPTIMAL" for r in res)),
         "fallbacks": fallbacks, "warm_iterations": {"mean": float(warm_its.mean()), "max": int(warm_its.max())},
+        "fresh_batches": {"members": count * args.steps, "vector_update_ms": round(1e3 * float(np.mean(updates)), 2),
+                          "full_setup_s": round(setup_s, 3)},
         "warm_solve_split_s": {k: round(float(np.mean([sp[i] for sp in splits])), 4)
                                for i, k in enumerate(("solve", "half_stopped", "p99_stopped", "handoffs"))},
         "cold_batched_solves_per_s": round(count / cold_s, 3),

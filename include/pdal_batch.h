@@ -94,6 +94,25 @@ int pdal_cp_al_batch_primal_weights(const pdal_cp_al_batch_t *batch, double *out
  * near the one a cold solve ends with. */
 int pdal_cp_al_batch_set_primal_weights(pdal_cp_al_batch_t *batch, const double *pw);
 
+/* New objective and bound vectors for the batch's matrices, each the
+ * concatenation over the instances in instance order (c, var_lb, var_ub: sum
+ * of num_variables entries; con_lb, con_ub: sum of num_constraints); NULL
+ * keeps the current vector.  The matrices keep their scaling and step-size
+ * estimate, which depend on the matrices only: the vectors are scaled as a
+ * fresh pdal_cp_al_batch_create would scale them (the same scaling passes in
+ * the same order, then the bound and objective scales), so the next solve
+ * gives the bits of a batch created with these vectors, at the cost of a few
+ * passes over the vectors instead of a full set-up.  The starts become zero
+ * and the primal weights their defaults (set_starts and set_primal_weights
+ * apply afterwards).  set_vectors reads host arrays, set_vectors_device device
+ * arrays of the current device.  Returns 0 on success.  No counterpart in the
+ * reference (one LP per call); it serves parametric families that share A
+ * (b(theta), c(theta)) without paying the set-up per batch. */
+int pdal_cp_al_batch_set_vectors(pdal_cp_al_batch_t *batch, const double *c, const double *con_lb,
+                                 const double *con_ub, const double *var_lb, const double *var_ub);
+int pdal_cp_al_batch_set_vectors_device(pdal_cp_al_batch_t *batch, const double *c, const double *con_lb,
+                                        const double *con_ub, const double *var_lb, const double *var_ub);
+
 void pdal_cp_al_batch_free(pdal_cp_al_batch_t *batch);
 
 #ifdef __cplusplus

@@ -16,8 +16,9 @@
  *   initialize_solver_state .. solver.cu:1795-2273 (norms :1999-2067)
  *   iteration kernels ........ solver.cu:2340-2459, launch order :2523-2661
  *   fixed-point error ........ solver.cu:3611-3686
- *   perform_restart .......... solver.cu:2821-3101 (target "current"; PID on the
- *                              primal weight, or on sigma in sigma mode :2857-2966)
+ *   perform_restart .......... solver.cu:2821-3101 (restart target of the candidate
+ *                              selection; PID on the primal weight, or on sigma
+ *                              in sigma mode :2857-2966)
  *   refresh_step_schedule .... solver.cu:3103-3126 (alpha and sigma mode)
  *   sigma parameters ......... solver.cu:3141-3164
  *   feasibility polishing .... solver.cu:3912-4456, utils.cu:1194-1440
@@ -37,9 +38,16 @@
  *   CP_AL_CR_ITERS, and sigma mode: CP_AL_SIGMA_MODE, CP_AL_SIGMA,
  *   CP_AL_SIGMA_ADAPTIVE, CP_AL_SIGMA_INIT/MIN/MAX_FACTOR, CP_AL_LAMBDA_FACTOR,
  *   CP_AL_LAMBDA_PARAM, CP_AL_SIGMA_BALANCE_GAIN, CP_AL_SIGMA_I_SMOOTH,
- *   CP_AL_SIGMA_KP/KI/KD, CP_AL_SIGMA_UPDATE_MIN/MAX.  The adaptive-lambda,
- *   alpha-trial, direct-pulse, restart-select and dynamic-eval controllers are
- *   not part of this oracle (nor of the HIP path).
+ *   CP_AL_SIGMA_KP/KI/KD, CP_AL_SIGMA_UPDATE_MIN/MAX; the positive-alpha
+ *   controllers (solver.cu:187-1057, used at :1202-1222, :1372-1494,
+ *   :1631-1695): CP_AL_ALPHA_ROLLBACK_TRIAL / _TRACE / _FORCE_REJECT /
+ *   _TEST_TRAJECTORY_LIMIT, CP_AL_ALPHA_DIRECT_PULSE / _TRACE /
+ *   _TEST_TRAJECTORY_LIMIT, CP_AL_DETERMINISTIC_PULSE_FORCE (plus
+ *   PDAL_ALPHA_TRIAL_MIN_SEC, a test knob for the trial's 2 s floor); restart
+ *   candidate selection (solver.cu:2663-2819): CP_AL_RESTART_SELECT / _MIN_ITERS
+ *   / _MIN_TIME / _THREEPOINT / _MARGIN.  The adaptive-lambda, sigma-trigger
+ *   and dynamic-eval controllers are not part of this oracle (nor of the HIP
+ *   path).
  *
  * Deliberate differences from the reference, shared with the HIP path:
  *   - timing uses steady_clock wall time, not clock() CPU time;
@@ -170,6 +178,10 @@ struct State {
     double pw_error_sum = 0.0, pw_last_error = 0.0, best_pw = 1.0, best_gap = INFINITY;
     int total_count = 0, inner_count = 0;
     double fp = 0.0, fp0 = 0.0, last_trial_fp = INFINITY;
+    // restart candidate (choose_restart_candidate, solver.cu:2749-2819): 0 = T output,
+    // 1 = the current (reflected Halpern) iterate, 2 = the plain Halpern iterate
+    int restart_target = 0;
+    double score_current = 0.0, score_average = INFINITY, score_halpern = INFINITY;
 
     double abs_p = 0, rel_p = 0, abs_d = 0, rel_d = 0, p_obj = 0, d_obj = 0, gap = 0, rel_gap = 0;
     termination_reason_t reason = TERMINATION_REASON_UNSPECIFIED;
@@ -520,13 +532,16 @@ bool should_restart(State &s, const restart_parameters_t &rp, int K)  // utils.c
 
 void perform_restart(State &s, const cp_al_parameters_t *p)  // solver.cu:2821-3101
 {
+    const int target = env_int("CP_AL_RESTART_SELECT", 0) != 0 ? s.restart_target : 0;  // :2823-2834
+    const std::vector<double> &rx = target == 1 ? s.x : target == 2 ? s.xH : s.xT;
+    const std::vector<double> &ry = target == 1 ? s.y : target == 2 ? s.yH : s.yT;
     double pd = 0.0, dd = 0.0;
     for (int j = 0; j < s.n; ++j) {
-        const double t = s.xT[j] - s.x0[j];
+        const double t = rx[j] - s.x0[j];
         pd += t * t;
     }
     for (int i = 0; i < s.m; ++i) {
-        const double t = s.yT[i] - s.y0[i];
+        const double t = ry[i] - s.y0[i];
         dd += t * t;
     }
     pd = std::sqrt(pd);
@@ -534,7 +549,7 @@ void perform_restart(State &s, const cp_al_parameters_t *p)  // solver.cu:2821-3
     const double ratio = s.rel_d / std::fmax(s.rel_p, 1e-300);
     if (s.sigma_mode) {  // solver.cu:2857-2966
         if (s.sigma_adaptive) {
-            for (int j = 0; j < s.n; ++j) s.dx[j] = s.xT[j] - s.x0[j];
+            for (int j = 0; j < s.n; ++j) s.dx[j] = rx[j] - s.x0[j];
             spmv(s.A, s.dx.data(), s.prim_prod.data());
             const double ax = nrm2(s.prim_prod);
             const double dx_metric_sq = s.lambda_param * pd * pd - ax * ax;
@@ -599,12 +614,89 @@ void perform_restart(State &s, const cp_al_parameters_t *p)  // solver.cu:2821-3
         }
     }
     refresh_step_schedule(s);
-    s.x0 = s.xT;
-    s.x = s.xT;
-    s.y0 = s.yT;
-    s.y = s.yT;
+    if (target == 1) {  // :3019-3043: anchor = T output = the current iterate, reflected = old T output
+        s.xR = s.xT;
+        s.yR = s.yT;
+        s.x0 = s.x;
+        s.xT = s.x;
+        s.y0 = s.y;
+        s.yT = s.y;
+    } else if (target == 2) {  // :3044-3076
+        s.xR = s.xT;
+        s.yR = s.yT;
+        s.x0 = s.xH;
+        s.x = s.xH;
+        s.xT = s.xH;
+        s.y0 = s.yH;
+        s.y = s.yH;
+        s.yT = s.yH;
+    } else {
+        s.x0 = s.xT;
+        s.x = s.xT;
+        s.y0 = s.yT;
+        s.y = s.yT;
+    }
     s.inner_count = 0;
     s.last_trial_fp = INFINITY;
+}
+
+/* ---------------- restart candidate (solver.cu:2663-2819) ---------------- */
+
+double restart_delta_score(State &s, const std::vector<double> &cx, const std::vector<double> &rx,
+                           const std::vector<double> &cy, const std::vector<double> &ry)
+{   // compute_restart_delta_score, solver.cu:2663-2747: the fixed-point metric of (candidate - reference)
+    for (int j = 0; j < s.n; ++j) s.dx[j] = cx[j] - rx[j];
+    for (int i = 0; i < s.m; ++i) s.dy[i] = cy[i] - ry[i];
+    spmv(s.At, s.dy.data(), s.dual_prod.data());
+    spmv(s.A, s.dx.data(), s.prim_prod.data());
+    const double pn = nrm2(s.dx), dn = nrm2(s.dy), an = nrm2(s.prim_prod);
+    if (s.sigma_mode && s.al_sigma > 0.0 && s.lambda_param > 0.0) {
+        const double val = s.al_sigma * (s.lambda_param * pn * pn - an * an) + dn * dn / std::fmax(s.al_sigma, 1e-16);
+        return std::sqrt(std::fmax(val, 0.0));
+    }
+    const double cross = dot(s.dual_prod, s.dx);
+    const double movement = pn * pn * s.primal_weight + dn * dn / s.primal_weight;
+    return std::sqrt(std::fmax(movement + 2 * s.step_size * cross - s.step_size * s.al_penalty * an * an, 0.0));
+}
+
+void choose_restart_candidate(State &s)  // solver.cu:2749-2819
+{
+    const int mode = env_int("CP_AL_RESTART_SELECT", 0);
+    s.restart_target = 0;
+    s.score_current = s.fp;
+    s.score_average = INFINITY;
+    s.score_halpern = INFINITY;
+    if (mode == 0) return;
+    if (s.total_count < env_int("CP_AL_RESTART_SELECT_MIN_ITERS", 0) ||
+        s.solve_time < env_double("CP_AL_RESTART_SELECT_MIN_TIME", 0.0))
+        return;
+    s.score_average = restart_delta_score(s, s.xT, s.x, s.yT, s.y);
+    const bool three = mode == 3 || env_int("CP_AL_RESTART_SELECT_THREEPOINT", 0) != 0;
+    if (three) s.score_halpern = restart_delta_score(s, s.xT, s.xH, s.yT, s.yH);
+    const double margin = env_double("CP_AL_RESTART_SELECT_MARGIN", 1.0);
+    if (mode == 2) {
+        s.restart_target = 1;
+        s.fp = s.score_average;
+    } else if (std::isfinite(s.score_current)) {
+        int best = 0;
+        double score = s.score_current;
+        if (std::isfinite(s.score_average) && s.score_average < score) {
+            best = 1;
+            score = s.score_average;
+        }
+        if (three && std::isfinite(s.score_halpern) && s.score_halpern < score) {
+            best = 2;
+            score = s.score_halpern;
+        }
+        if (best != 0 && score <= margin * s.score_current) {
+            s.restart_target = best;
+            s.fp = score;
+        } else {
+            s.fp = s.score_current;
+        }
+    } else {
+        s.fp = s.score_current;
+    }
 }
 
 /* ---------------- feasibility polishing (solver.cu:3912-4456) ---------------- */
@@ -891,6 +983,312 @@ void init_state(State &s, const lp_problem_t *w, const cp_al_parameters_t *p)  /
     s.b_norm = linf ? mx : std::sqrt(sq);
 }
 
+/* ---------------- positive-alpha controllers (solver.cu:187-1057) ---------------- */
+
+double metric_of(double v) { return std::fmax(std::fabs(v), 1e-300); }  // alpha_trial_metric, :409-412
+
+double log_rate(double start, double end, int iters)  // alpha_trial_log_rate, :414-422
+{
+    if (iters <= 0 || !std::isfinite(start) || !std::isfinite(end)) return NAN;
+    const double r = std::log(metric_of(start) / metric_of(end)) / (double)iters;
+    return std::isfinite(r) ? r : NAN;
+}
+
+double predict_ratio(double rate, int iters)  // alpha_trial_predict_ratio, :424-428
+{
+    return std::exp(std::fmax(-700.0, std::fmin(700.0, -rate * (double)iters)));
+}
+
+struct ZeroEpoch {  // alpha_zero_epoch_record_t, :198-208
+    bool valid = false;
+    int iterations = 0;
+    double seconds = 0, fp_rate = 0, p_rate = 0, d_rate = 0, g_rate = 0, kkt_rate = 0;
+};
+
+struct AlphaTrial {  // alpha_trial_controller_t, :210-241; the snapshot is a copy of the whole state
+    bool enabled = false, trace = false, force_reject = false, active = false, epoch_valid = false;
+    int started = 0, accepted = 0, rejected = 0, positive_epochs = 0, positive_charged = 0, count = 0;
+    ZeroEpoch hist[2];
+    int ep_total = 0;
+    double ep_time = 0, ep_p = 0, ep_d = 0, ep_g = 0;
+    int t_total = 0, t_charged = 0, max_iters = 0;
+    double t_time = 0, t_fp = 0, t_p = 0, t_d = 0, t_g = 0, max_sec = 0;
+    bool snap_valid = false;
+    State snap;
+};
+
+enum { PULSE_IDLE = 0, PULSE_POSITIVE = 1, PULSE_DONE = 2 };
+
+struct DirectPulse {  // alpha_direct_pulse_controller_t, :250-274
+    bool enabled = false, trace = false, force = false;
+    int phase = PULSE_IDLE, started = 0, completed = 0, aborted = 0, gate_eval = 0, gate_pass = 0, gate_rej = 0;
+    int pulse_iters = 200, gate_anchor = 0, gate_prev_k = 0, gate_recent_k = 0, anchor_traj = 0, anchor_charged = 0;
+    double gate_dfp = 0, pos_fp = 0, pos_p = 0, pos_d = 0, pos_g = 0;
+    termination_reason_t pos_reason = TERMINATION_REASON_UNSPECIFIED;
+};
+
+double snapshot_restore(AlphaTrial &t, State &s)  // alpha_trial_snapshot_restore, :502-585
+{   // every scalar and the ten iterate vectors of the snapshot; the elapsed time stays charged.
+    // Returns the largest 2-norm change of the ten vectors.
+    if (!t.snap_valid) return INFINITY;
+    auto l2 = [](const std::vector<double> &a, const std::vector<double> &b) {
+        double q = 0.0;
+        for (size_t k = 0; k < a.size(); ++k) q += (a[k] - b[k]) * (a[k] - b[k]);
+        return std::sqrt(q);
+    };
+    const std::vector<double> *now[10] = {&s.x0, &s.x, &s.xT, &s.xH, &s.xR, &s.y0, &s.y, &s.yT, &s.yH, &s.yR};
+    const std::vector<double> *old[10] = {&t.snap.x0, &t.snap.x, &t.snap.xT, &t.snap.xH, &t.snap.xR,
+                                          &t.snap.y0, &t.snap.y, &t.snap.yT, &t.snap.yH, &t.snap.yR};
+    double diff = 0.0;
+    for (int k = 0; k < 10; ++k) diff = std::fmax(diff, l2(*old[k], *now[k]));
+    const double elapsed = s.solve_time;
+    s = t.snap;
+    s.solve_time = elapsed;
+    return diff;
+}
+
+void trial_init(AlphaTrial &t)  // alpha_trial_controller_initialize, :587-603
+{
+    t.enabled = env_int("CP_AL_ALPHA_ROLLBACK_TRIAL", 0) != 0;
+    t.trace = env_int("CP_AL_ALPHA_ROLLBACK_TRACE", 0) != 0;
+    t.force_reject = env_int("CP_AL_ALPHA_ROLLBACK_FORCE_REJECT", 0) != 0;
+    if (t.enabled)
+        fprintf(stderr,
+                "[alpha-trial-config] enabled=1 alpha=2.000000000000e-03 history=2 fp_gain=0.10 kkt_gain=0.05 "
+                "component_slack=0.05 force_reject=%d\n",
+                t.force_reject ? 1 : 0);
+}
+
+void set_epoch_start(AlphaTrial &t, const State &s)  // alpha_trial_set_epoch_start, :605-614
+{
+    t.epoch_valid = true;
+    t.ep_total = s.total_count;
+    t.ep_time = s.solve_time;
+    t.ep_p = metric_of(s.rel_p);
+    t.ep_d = metric_of(s.rel_d);
+    t.ep_g = metric_of(s.rel_gap);
+}
+
+void record_zero_epoch(AlphaTrial &t, const State &s)  // alpha_trial_record_zero_epoch, :616-678
+{
+    if (!t.epoch_valid || t.active || s.al_lambda > 0.0) return;
+    ZeroEpoch r;
+    r.iterations = s.total_count - t.ep_total;
+    r.seconds = s.solve_time - t.ep_time;
+    const double k0 = std::fmax(t.ep_p, std::fmax(t.ep_d, t.ep_g));
+    const double k1 = std::fmax(metric_of(s.rel_p), std::fmax(metric_of(s.rel_d), metric_of(s.rel_gap)));
+    r.fp_rate = log_rate(s.fp0, s.fp, r.iterations);
+    r.p_rate = log_rate(t.ep_p, s.rel_p, r.iterations);
+    r.d_rate = log_rate(t.ep_d, s.rel_d, r.iterations);
+    r.g_rate = log_rate(t.ep_g, s.rel_gap, r.iterations);
+    r.kkt_rate = log_rate(k0, k1, r.iterations);
+    r.valid = r.iterations > 0 && r.seconds >= 0.0 && std::isfinite(r.fp_rate) && std::isfinite(r.p_rate) &&
+              std::isfinite(r.d_rate) && std::isfinite(r.g_rate) && std::isfinite(r.kkt_rate);
+    if (!r.valid) return;
+    if (t.count < 2) {
+        t.hist[t.count++] = r;
+    } else {
+        t.hist[0] = t.hist[1];
+        t.hist[1] = r;
+    }
+    if (t.trace)
+        fprintf(stderr,
+                "[alpha-zero-epoch] iter=%d K=%d sec=%.12e rates(fp=%.12e p=%.12e d=%.12e g=%.12e max=%.12e) "
+                "history=%d\n",
+                s.total_count, r.iterations, r.seconds, r.fp_rate, r.p_rate, r.d_rate, r.g_rate, r.kkt_rate, t.count);
+}
+
+bool trial_eligible(const AlphaTrial &t, const State &s, const cp_al_parameters_t *p, int charged)  // :680-703
+{
+    // The 2 s floor on the elapsed time is the reference's; PDAL_ALPHA_TRIAL_MIN_SEC lowers
+    // it so that a test reaches the trial at a deterministic point.
+    const double min_sec = env_double("PDAL_ALPHA_TRIAL_MIN_SEC", 2.0);
+    if (!t.enabled || t.started != 0 || t.count < 2 || charged < 10000 || s.solve_time < min_sec ||
+        s.reason != TERMINATION_REASON_UNSPECIFIED)
+        return false;
+    const int hmax = std::max(t.hist[0].iterations, t.hist[1].iterations);
+    const double hsec = std::fmax(2.0, std::fmax(t.hist[0].seconds, t.hist[1].seconds));
+    const int rem_it = p->termination_criteria.iteration_limit - charged;
+    const double rem_sec = p->termination_criteria.time_sec_limit - s.solve_time;
+    return hmax > 0 && rem_it >= 3 * hmax && rem_sec >= 3.0 * hsec;
+}
+
+void trial_start(AlphaTrial &t, State &s, int charged)  // alpha_trial_start, :704-761
+{
+    t.snap = s;
+    t.snap_valid = true;
+    t.started = 1;
+    t.active = true;
+    t.t_total = s.total_count;
+    t.t_charged = charged;
+    t.t_time = s.solve_time;
+    t.t_p = metric_of(s.rel_p);
+    t.t_d = metric_of(s.rel_d);
+    t.t_g = metric_of(s.rel_gap);
+    const int hmax = std::max(t.hist[0].iterations, t.hist[1].iterations);
+    t.max_iters = 2 * hmax;
+    t.max_sec = 2.0 * std::fmax(2.0, std::fmax(t.hist[0].seconds, t.hist[1].seconds));
+    s.sigma_mode = false;
+    s.al_lambda = 0.002;
+    refresh_step_schedule(s);
+    compute_fixed_point_error(s);
+    t.t_fp = metric_of(s.fp);
+    if (!std::isfinite(t.t_fp) || t.t_fp <= 0.0) {
+        t.active = false;
+        t.rejected = 1;
+        const double d = snapshot_restore(t, s);
+        fprintf(stderr, "[alpha-trial-restore] reason=invalid-start charged_iter=%d trajectory_iter=%d "
+                        "max_vector_l2=%.12e\n", charged, s.total_count, d);
+        t.snap_valid = false;
+        return;
+    }
+    fprintf(stderr,
+            "[alpha-trial-start] charged_iter=%d trajectory_iter=%d solve_sec=%.12e alpha=2.000000000000e-03 "
+            "fp=%.12e primal=%.12e dual=%.12e gap=%.12e max_K=%d max_sec=%.12e\n",
+            charged, s.total_count, s.solve_time, t.t_fp, t.t_p, t.t_d, t.t_g, t.max_iters, t.max_sec);
+}
+
+bool trial_cap(const AlphaTrial &t, const State &s, int charged)  // alpha_trial_safety_cap_reached, :763-775
+{
+    if (!t.active) return false;
+    return charged - t.t_charged >= t.max_iters || s.solve_time - t.t_time >= t.max_sec;
+}
+
+bool trial_accepts(const AlphaTrial &t, const State &s, int charged, const char **why)  // :777-861
+{
+    const int k = s.total_count - t.t_total;
+    if (k <= 0) {
+        *why = "empty-epoch";
+        return false;
+    }
+    const double fr = 0.5 * (t.hist[0].fp_rate + t.hist[1].fp_rate);
+    const double pr = 0.5 * (t.hist[0].p_rate + t.hist[1].p_rate);
+    const double dr = 0.5 * (t.hist[0].d_rate + t.hist[1].d_rate);
+    const double gr = 0.5 * (t.hist[0].g_rate + t.hist[1].g_rate);
+    const double kr = 0.5 * (t.hist[0].kkt_rate + t.hist[1].kkt_rate);
+    const double fp_pred = predict_ratio(fr, k);
+    const double p_pred = t.t_p * predict_ratio(pr, k);
+    const double d_pred = t.t_d * predict_ratio(dr, k);
+    const double g_pred = t.t_g * predict_ratio(gr, k);
+    const double k_pred = std::fmax(t.t_p, std::fmax(t.t_d, t.t_g)) * predict_ratio(kr, k);
+    const double fp_obs = metric_of(s.fp) / t.t_fp;
+    const double p_obs = metric_of(s.rel_p), d_obs = metric_of(s.rel_d), g_obs = metric_of(s.rel_gap);
+    const double k_obs = std::fmax(p_obs, std::fmax(d_obs, g_obs));
+    const bool finite = std::isfinite(fp_pred) && std::isfinite(p_pred) && std::isfinite(d_pred) &&
+                        std::isfinite(g_pred) && std::isfinite(k_pred) && std::isfinite(fp_obs) &&
+                        std::isfinite(p_obs) && std::isfinite(d_obs) && std::isfinite(g_obs) && std::isfinite(k_obs);
+    bool ok = finite && fp_obs <= 0.90 * fp_pred && k_obs <= 0.95 * k_pred && p_obs <= 1.05 * p_pred &&
+              d_obs <= 1.05 * d_pred && g_obs <= 1.05 * g_pred;
+    if (t.force_reject) {
+        ok = false;
+        *why = "forced-reject";
+    } else {
+        *why = !finite ? "nonfinite" : !ok ? "acceptance-gate" : "accepted";
+    }
+    fprintf(stderr,
+            "[alpha-trial-decision] reason=%s accept=%d charged_iter=%d trajectory_iter=%d K=%d "
+            "fp_ratio_obs=%.12e fp_ratio_pred=%.12e kkt_obs=%.12e kkt_pred=%.12e p_obs=%.12e p_pred=%.12e "
+            "d_obs=%.12e d_pred=%.12e g_obs=%.12e g_pred=%.12e\n",
+            *why, ok ? 1 : 0, charged, s.total_count, k, fp_obs, fp_pred, k_obs, k_pred, p_obs, p_pred, d_obs, d_pred,
+            g_obs, g_pred);
+    return ok;
+}
+
+void pulse_init(DirectPulse &d)  // alpha_direct_pulse_initialize, :884-904
+{
+    d.enabled = env_int("CP_AL_ALPHA_DIRECT_PULSE", 0) != 0;
+    d.trace = env_int("CP_AL_ALPHA_DIRECT_PULSE_TRACE", 0) != 0;
+    d.force = env_int("CP_AL_DETERMINISTIC_PULSE_FORCE", 0) != 0;
+    if (d.enabled)
+        fprintf(stderr,
+                "[alpha-direct-pulse-config] enabled=1 alpha=2.000000000000e-03 pulse_iterations=200 "
+                "history_epochs=2 deterministic_anchor_iter_ge=10000 wall_time_features=0 tail_only=1 "
+                "force_pulse=%d\n",
+                d.force ? 1 : 0);
+}
+
+bool pulse_eligible(DirectPulse &d, const AlphaTrial &h, const State &s, const cp_al_parameters_t *p, int charged,
+                    int K)  // alpha_direct_pulse_is_eligible, :906-958
+{
+    if (!d.enabled || d.started != 0 || d.gate_eval != 0 || d.phase != PULSE_IDLE || h.count < 2 ||
+        charged < 10000 || s.reason != TERMINATION_REASON_UNSPECIFIED || K != 200)
+        return false;
+    for (int i = 0; i < 2; ++i)
+        if (!h.hist[i].valid || h.hist[i].iterations <= 0 || !std::isfinite(h.hist[i].fp_rate)) return false;
+    if (p->termination_criteria.iteration_limit - charged < 400) return false;
+    d.gate_eval = 1;
+    d.gate_anchor = charged;
+    d.gate_prev_k = h.hist[0].iterations;
+    d.gate_recent_k = h.hist[1].iterations;
+    d.gate_dfp = h.hist[1].fp_rate - h.hist[0].fp_rate;
+    const bool pass = d.force;
+    d.gate_pass = pass ? 1 : 0;
+    d.gate_rej = pass ? 0 : 1;
+    fprintf(stderr,
+            "[alpha-deterministic-tail-anchor] charged_iter=%d trajectory_iter=%d previous_K=%d recent_K=%d "
+            "previous_fp_rate=%.12e recent_fp_rate=%.12e delta_fp_rate=%.12e pass=%d force_pulse=%d\n",
+            charged, s.total_count, h.hist[0].iterations, h.hist[1].iterations, h.hist[0].fp_rate,
+            h.hist[1].fp_rate, d.gate_dfp, pass ? 1 : 0, d.force ? 1 : 0);
+    return pass;
+}
+
+void pulse_start(DirectPulse &d, State &s, int charged, int K)  // alpha_direct_pulse_start, :960-982
+{
+    d.started = 1;
+    d.phase = PULSE_POSITIVE;
+    d.pulse_iters = K;
+    d.anchor_traj = s.total_count;
+    d.anchor_charged = charged;
+    s.sigma_mode = false;
+    s.al_lambda = 0.002;
+    refresh_step_schedule(s);
+    fprintf(stderr, "[alpha-direct-pulse-start] charged_iter=%d trajectory_iter=%d alpha=2.000000000000e-03 "
+                    "pulse_iterations=%d\n", charged, s.total_count, d.pulse_iters);
+}
+
+void pulse_finish(DirectPulse &d, State &s, int charged)  // alpha_direct_pulse_finish_positive, :984-1025
+{
+    if (!d.enabled || d.phase != PULSE_POSITIVE) return;
+    d.pos_p = metric_of(s.rel_p);
+    d.pos_d = metric_of(s.rel_d);
+    d.pos_g = metric_of(s.rel_gap);
+    d.pos_reason = s.reason;
+    s.reason = TERMINATION_REASON_UNSPECIFIED;
+    s.sigma_mode = false;
+    s.al_lambda = 0.0;
+    refresh_step_schedule(s);
+    compute_fixed_point_error(s);
+    d.pos_fp = metric_of(s.fp);
+    choose_restart_candidate(s);
+    s.reason = d.pos_reason;
+    d.completed = 1;
+    d.aborted = d.pos_reason != TERMINATION_REASON_UNSPECIFIED && d.pos_reason != TERMINATION_REASON_OPTIMAL;
+    d.phase = PULSE_DONE;
+    fprintf(stderr,
+            "[alpha-direct-pulse-complete] charged_iter=%d trajectory_iter=%d positive_iters=%d fp0map=%.12e "
+            "primal=%.12e dual=%.12e gap=%.12e term=%d\n",
+            charged, s.total_count, d.pulse_iters, d.pos_fp, d.pos_p, d.pos_d, d.pos_g, (int)d.pos_reason);
+}
+
+void alpha_final_logs(const AlphaTrial &t, const DirectPulse &d, const State &s, int charged)  // :863-882, :1027-1057
+{
+    if (t.enabled)
+        fprintf(stderr,
+                "[alpha-trial-final] charged_iter=%d trajectory_iter=%d started=%d accepted=%d rejected=%d "
+                "positive_epochs=%d positive_charged_iters=%d final_alpha=%.12e final_penalty=%.12e\n",
+                charged, s.total_count, t.started, t.accepted, t.rejected, t.positive_epochs, t.positive_charged,
+                s.al_lambda, s.al_penalty);
+    if (d.enabled)
+        fprintf(stderr,
+                "[alpha-direct-pulse-final] charged_iter=%d trajectory_iter=%d started=%d completed=%d aborted=%d "
+                "gate_evaluated=%d gate_passed=%d gate_rejected=%d gate_anchor_charged_iter=%d gate_previous_K=%d "
+                "gate_recent_K=%d gate_delta_fp_rate=%.12e positive_charged_iters=%d final_alpha=%.12e "
+                "final_penalty=%.12e\n",
+                charged, s.total_count, d.started, d.completed, d.aborted, d.gate_eval, d.gate_pass, d.gate_rej,
+                d.gate_anchor, d.gate_prev_k, d.gate_recent_k, d.gate_dfp, d.completed ? d.pulse_iters : 0,
+                s.al_lambda, s.al_penalty);
+}
+
 }  // namespace
 
 /* ---------------- public C API (cp_al.c, utils.cu conversions) ---------------- */
@@ -1118,12 +1516,24 @@ ORACLE_API cp_al_result_t *optimize(const cp_al_parameters_t *params, lp_problem
     refresh_step_schedule(s);
     s.best_pw = s.primal_weight;
 
+    AlphaTrial trial;
+    trial_init(trial);
+    DirectPulse pulse;
+    pulse_init(pulse);
+    if (trial.enabled && pulse.enabled) {  // solver.cu:1216-1222 (an error return instead of exit)
+        fprintf(stderr, "[alpha-controller-error] rollback trial and direct pulse cannot be enabled together\n");
+        return nullptr;
+    }
+    const int trial_test_limit = env_int("CP_AL_ALPHA_ROLLBACK_TEST_TRAJECTORY_LIMIT", 0);
+    const int pulse_test_limit = env_int("CP_AL_ALPHA_DIRECT_PULSE_TEST_TRAJECTORY_LIMIT", 0);
+
     const auto t_start = std::chrono::steady_clock::now();
     int charged = s.total_count;
     bool do_restart = false;
     const int K = params->termination_evaluation_frequency;
     const double rho = params->reflection_coefficient;
-    while (charged < params->termination_criteria.iteration_limit) {
+    const auto &tc = params->termination_criteria;
+    while (charged < tc.iteration_limit) {
         if (do_restart) {
             compute_fixed_point_error(s);
             s.fp0 = s.fp;
@@ -1131,14 +1541,14 @@ ORACLE_API cp_al_result_t *optimize(const cp_al_parameters_t *params, lp_problem
         }
         for (int i = 1; i <= K; ++i) iteration(s, i, rho, i == 1 || i == K);
         compute_fixed_point_error(s);
+        choose_restart_candidate(s);  // solver.cu:1351
         compute_residual(s, params->optimality_norm);
         s.inner_count += K;
         s.total_count += K;
         charged += K;
 
-        // check_termination_criteria, utils.cu:299-317
+        // check_termination_criteria, utils.cu:299-317, and the controllers' test limits (:1372-1391)
         s.solve_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
-        const auto &tc = params->termination_criteria;
         if (s.rel_d < tc.eps_feasible_relative && s.rel_p < tc.eps_feasible_relative &&
             s.rel_gap < tc.eps_optimal_relative)
             s.reason = TERMINATION_REASON_OPTIMAL;
@@ -1148,15 +1558,95 @@ ORACLE_API cp_al_result_t *optimize(const cp_al_parameters_t *params, lp_problem
             s.reason = TERMINATION_REASON_TIME_LIMIT;
         if (s.reason == TERMINATION_REASON_UNSPECIFIED && charged >= tc.iteration_limit)
             s.reason = TERMINATION_REASON_ITERATION_LIMIT;
-        if (s.reason != TERMINATION_REASON_UNSPECIFIED) break;
+        if (s.reason == TERMINATION_REASON_UNSPECIFIED && trial.enabled && trial_test_limit > 0 && !trial.active &&
+            trial.rejected > 0 && s.total_count >= trial_test_limit)
+            s.reason = TERMINATION_REASON_ITERATION_LIMIT;
+        if (s.reason == TERMINATION_REASON_UNSPECIFIED && pulse.enabled && pulse_test_limit > 0 &&
+            pulse.phase == PULSE_DONE && s.total_count >= pulse_test_limit)
+            s.reason = TERMINATION_REASON_ITERATION_LIMIT;
+        if (pulse.phase == PULSE_POSITIVE) {  // :1416-1425
+            pulse_finish(pulse, s, charged);
+            if (s.reason == TERMINATION_REASON_UNSPECIFIED && pulse_test_limit > 0 &&
+                s.total_count >= pulse_test_limit)
+                s.reason = TERMINATION_REASON_ITERATION_LIMIT;
+        }
+        if (s.reason != TERMINATION_REASON_UNSPECIFIED) {  // :1426-1474
+            if (trial.active) {
+                const int pos = charged - trial.t_charged;
+                trial.positive_charged += pos;
+                trial.positive_epochs += pos > 0 ? 1 : 0;
+                if (s.reason == TERMINATION_REASON_OPTIMAL) {
+                    trial.accepted += 1;
+                    trial.active = false;
+                    s.al_lambda = 0.0;
+                    refresh_step_schedule(s);
+                    fprintf(stderr, "[alpha-trial-decision] reason=terminal-optimal accept=1 charged_iter=%d "
+                                    "trajectory_iter=%d K=%d\n", charged, s.total_count, s.total_count - trial.t_total);
+                } else {
+                    const termination_reason_t why = s.reason;
+                    const double d = snapshot_restore(trial, s);
+                    s.reason = why;
+                    trial.rejected += 1;
+                    trial.active = false;
+                    fprintf(stderr, "[alpha-trial-restore] reason=charged-termination charged_iter=%d "
+                                    "trajectory_iter=%d max_vector_l2=%.12e\n", charged, s.total_count, d);
+                }
+                trial.snap_valid = false;
+            }
+            break;
+        }
+        if (trial_cap(trial, s, charged)) {  // :1476-1494
+            const int pos = charged - trial.t_charged;
+            trial.positive_charged += pos;
+            trial.positive_epochs += pos > 0 ? 1 : 0;
+            const double d = snapshot_restore(trial, s);
+            trial.rejected += 1;
+            trial.active = false;
+            set_epoch_start(trial, s);
+            fprintf(stderr, "[alpha-trial-restore] reason=safety-cap charged_iter=%d trajectory_iter=%d "
+                            "positive_iters=%d max_vector_l2=%.12e\n", charged, s.total_count, pos, d);
+            trial.snap_valid = false;
+            do_restart = true;
+            continue;
+        }
 
         do_restart = should_restart(s, params->restart_params, K);
-        if (do_restart) perform_restart(s, params);
+        if (do_restart) {  // :1631-1695
+            if (trial.active) {
+                const int pos = charged - trial.t_charged;
+                trial.positive_charged += pos;
+                trial.positive_epochs += pos > 0 ? 1 : 0;
+                const char *why = nullptr;
+                if (!trial_accepts(trial, s, charged, &why)) {
+                    const double d = snapshot_restore(trial, s);
+                    trial.rejected += 1;
+                    trial.active = false;
+                    set_epoch_start(trial, s);
+                    fprintf(stderr, "[alpha-trial-restore] reason=%s charged_iter=%d trajectory_iter=%d "
+                                    "positive_iters=%d max_vector_l2=%.12e\n", why, charged, s.total_count, pos, d);
+                    trial.snap_valid = false;
+                    do_restart = true;
+                    continue;
+                }
+                trial.accepted += 1;
+                trial.active = false;
+                s.al_lambda = 0.0;
+                refresh_step_schedule(s);
+                trial.snap_valid = false;
+            } else {
+                record_zero_epoch(trial, s);
+            }
+            perform_restart(s, params);
+            set_epoch_start(trial, s);
+            if (trial_eligible(trial, s, params, charged)) trial_start(trial, s, charged);
+            if (pulse_eligible(pulse, trial, s, params, charged, K)) pulse_start(pulse, s, charged, K);
+        }
     }
     if (s.reason == TERMINATION_REASON_UNSPECIFIED) {
         s.reason = TERMINATION_REASON_ITERATION_LIMIT;
         compute_residual(s, params->optimality_norm);
     }
+    alpha_final_logs(trial, pulse, s, charged);
     if (params->feasibility_polishing && s.reason != TERMINATION_REASON_DUAL_INFEASIBLE &&
         s.reason != TERMINATION_REASON_PRIMAL_INFEASIBLE)  // solver.cu:1715-1719
         feasibility_polish(params, s, t_start);

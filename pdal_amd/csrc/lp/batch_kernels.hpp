@@ -901,6 +901,81 @@ static __global__ void __launch_bounds__(kBlock)
     }
 }
 
+// Norms of the unscaled objective and constraint bounds per instance
+// (solver.cu:1999-2067): out[2i] = ||c_i||, out[2i+1] = ||b_i|| over the
+// finite l_r with l_r != u_r and the finite u_r; 2-norm, or max |.| when linf.
+// One workgroup per instance, fixed summation order.
+static __global__ void __launch_bounds__(kBlock)
+    seg_norms_kernel(const long *__restrict__ row_off, const long *__restrict__ col_off, const double *__restrict__ l,
+                     const double *__restrict__ u, const double *__restrict__ c, int linf, double *__restrict__ out)
+{
+    const int inst = blockIdx.x;
+    __shared__ double red[2][kWavesPerBlock];
+    double a = 0.0, b = 0.0;
+    for (long j = col_off[inst] + threadIdx.x; j < col_off[inst + 1]; j += kBlock) {
+        const double v = c[j];
+        a = linf ? fmax(a, fabs(v)) : a + v * v;
+    }
+    for (long i = row_off[inst] + threadIdx.x; i < row_off[inst + 1]; i += kBlock) {
+        const double lo = l[i], hi = u[i];
+        if (isfinite(lo) && lo != hi) b = linf ? fmax(b, fabs(lo)) : b + lo * lo;
+        if (isfinite(hi)) b = linf ? fmax(b, fabs(hi)) : b + hi * hi;
+    }
+    a = linf ? wave_max(a) : wave_sum(a);
+    b = linf ? wave_max(b) : wave_sum(b);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        red[0][threadIdx.x / kWave] = a;
+        red[1][threadIdx.x / kWave] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double ra = red[0][0], rb = red[1][0];
+        for (int k = 1; k < kWavesPerBlock; ++k) {
+            ra = linf ? fmax(ra, red[0][k]) : ra + red[0][k];
+            rb = linf ? fmax(rb, red[1][k]) : rb + red[1][k];
+        }
+        out[2 * inst] = linf ? ra : sqrt(ra);
+        out[2 * inst + 1] = linf ? rb : sqrt(rb);
+    }
+}
+
+// The recorded scaling passes applied again to new vectors of the same
+// matrices: per pass, c *= 1/d, variable bounds *= d (scale_vars_kernel) and
+// constraint bounds *= 1/e (scale_cons_kernel), in the passes' order, so the
+// result has the bits a full rescale would give.  Factor tables are pass-major
+// with `period` entries per pass (one instance's when all share the matrix).
+static __global__ void __launch_bounds__(kBlock)
+    replay_scaling_kernel(double *__restrict__ c, double *__restrict__ lb, double *__restrict__ ub, long n,
+                          const double *__restrict__ fd, const double *__restrict__ finv_d, long pn,
+                          double *__restrict__ l, double *__restrict__ u, long m, const double *__restrict__ finv_e,
+                          long pm, int passes)
+{
+    const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) {
+        const long k = i % pn;
+        double cj = c[i], lo = lb[i], hi = ub[i];
+        for (int p = 0; p < passes; ++p) {
+            const double dj = fd[p * pn + k];
+            cj *= finv_d[p * pn + k];
+            lo *= dj;
+            hi *= dj;
+        }
+        c[i] = cj;
+        lb[i] = lo;
+        ub[i] = hi;
+    } else if (i < n + m) {
+        const long r = i - n, k = r % pm;
+        double lo = l[r], hi = u[r];
+        for (int p = 0; p < passes; ++p) {
+            const double ie = finv_e[p * pm + k];
+            lo *= ie;
+            hi *= ie;
+        }
+        l[r] = lo;
+        u[r] = hi;
+    }
+}
+
 // Warm start into the scaled space: x * D * b_scale, y * E * c_scale.
 static __global__ void __launch_bounds__(kBlock)
     scale_start_kernel(const int *__restrict__ col_inst, const double *__restrict__ xs, const double *__restrict__ D,

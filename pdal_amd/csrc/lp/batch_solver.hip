@@ -79,6 +79,10 @@ class BatchSolver {
     void time_kernels(int iters, double *out);
     void primal_weights(double *out) const;
     void set_primal_weights(const double *pw);
+    // New objective and bound vectors for the same matrices (concatenated,
+    // unscaled; nullptr = keep the current one; device = device addresses)
+    void set_vectors(const double *c, const double *l, const double *u, const double *lb, const double *ub,
+                     bool device);
     int count() const { return N_; }
 
   private:
@@ -93,6 +97,10 @@ class BatchSolver {
     int compact();
     void restore_full_tables();
     void rescale();
+    void bound_objective_scales();
+    void init_norms();
+    void init_steps();
+    bool detect_vector_flags();
     void power_iteration();
     void autotune();
     void reset();
@@ -103,7 +111,7 @@ class BatchSolver {
     cp_al_parameters_t P_;
     int N_ = 0;
     long m_ = 0, n_ = 0, nnz_ = 0;
-    std::vector<long> row_off_, col_off_;  // N+1 prefix offsets
+    std::vector<long> row_off_, col_off_, nz_off_;  // N+1 prefix offsets
     hipStream_t stream_ = nullptr;
     DevCsr A_, At_;
     DBuf<int> row_inst_, col_inst_, row_end_a_, row_end_at_;
@@ -119,6 +127,14 @@ class BatchSolver {
     DBuf<double> c_, lb_, ub_, l_, u_, E_, D_;
     DBuf<double> x0_, x_, xT_, xH_, xR_, dslack_, y0_, y_, yT_, yH_, yR_;
     DBuf<double> xs_, ys_;  // scaled starts (kept for repeated solves)
+    // set_vectors: the unscaled problem vectors, and the column (d, 1/d) and
+    // row (1/e) factors of every scaling pass in order, pass-major with
+    // fac_cols_ / fac_rows_ entries per pass (the first instance's block when
+    // all instances share the matrix, else the whole batch)
+    DBuf<double> raw_c_, raw_lb_, raw_ub_, raw_l_, raw_u_;
+    DBuf<double> fac_d_, fac_inv_d_, fac_inv_e_;
+    long fac_cols_ = 0, fac_rows_ = 0;
+    int fac_passes_ = 0;
     DBuf<double> parts_at_, parts_a_, evalq_;
     DBuf<InstState> st_;
     std::vector<InstState> init_;
@@ -506,8 +522,21 @@ void BatchSolver::detect_shared()
     if (nk > 8) return;  // up to 512 instances side by side
     NK_full_ = NK_ = nk;
     Bp_full_ = Bp_ = nk * kWave;
-    // shared bounds: the scaled lb and ub of every instance equal the first's bit
-    // for bit; equality rows: l = u bit for bit in every row of every instance
+    detect_vector_flags();
+    sh_alloc(0, Bp_);
+    sh_parts_.alloc((size_t)kShEvalBlocks * std::max(kAtQ, kAQ) * Bp_);
+    for (auto &ids : sh_ids_) ids.alloc(Bp_);
+    h_ids_.resize(Bp_);
+    for (int s = 0; s < Bp_; ++s) h_ids_[s] = s < N_ ? s : -1;
+    sh_ids_[0].upload(h_ids_.data(), Bp_, stream_);
+    PDAL_HIP(hipStreamSynchronize(stream_));
+    shared_ = true;
+}
+
+bool BatchSolver::detect_vector_flags()
+{   // shared bounds: the scaled lb and ub of every instance equal the first's bit
+    // for bit; equality rows: l = u bit for bit in every row of every instance.
+    // Returns whether either flag changed.
     const auto env_off = [](const char *k) { return getenv(k) && atoi(getenv(k)) != 0; };
     const long n = n_ / N_;
     DBuf<int> flag(1);
@@ -520,6 +549,7 @@ void BatchSolver::detect_shared()
         PDAL_HIP(hipStreamSynchronize(stream_));
         return hflag[0] != 0;
     };
+    const bool sb0 = sb_, eq0 = eq_;
     sb_ = !env_off("PDAL_BATCH_NO_SHARED_BOUNDS") && !flagged([&] {
         const int nb = std::max(1, std::min(4096, blocks_for_l(n_ - n)));
         for (const double *cat : {lb_.get(), ub_.get()})
@@ -530,19 +560,13 @@ void BatchSolver::detect_shared()
                                                                                                    m_, flag.get());
     });
     if (sb_) {
-        sh_lb_.alloc(n);
-        sh_ub_.alloc(n);
+        if (!sh_lb_.get()) sh_lb_.alloc(n);
+        if (!sh_ub_.get()) sh_ub_.alloc(n);
         PDAL_HIP(hipMemcpyAsync(sh_lb_.get(), lb_.get(), (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, stream_));
         PDAL_HIP(hipMemcpyAsync(sh_ub_.get(), ub_.get(), (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, stream_));
     }
-    sh_alloc(0, Bp_);
-    sh_parts_.alloc((size_t)kShEvalBlocks * std::max(kAtQ, kAQ) * Bp_);
-    for (auto &ids : sh_ids_) ids.alloc(Bp_);
-    h_ids_.resize(Bp_);
-    for (int s = 0; s < Bp_; ++s) h_ids_[s] = s < N_ ? s : -1;
-    sh_ids_[0].upload(h_ids_.data(), Bp_, stream_);
     PDAL_HIP(hipStreamSynchronize(stream_));
-    shared_ = true;
+    return sb_ != sb0 || eq_ != eq0;
 }
 
 void BatchSolver::sh_alloc(int set, int width)
@@ -818,10 +842,12 @@ void BatchSolver::setup()
     // --- concatenate (block diagonal) ---
     row_off_.assign(N_ + 1, 0);
     col_off_.assign(N_ + 1, 0);
+    nz_off_.assign(N_ + 1, 0);
     for (int i = 0; i < N_; ++i) {
         const lp_problem_t *w = probs_[i];
         row_off_[i + 1] = row_off_[i] + w->num_constraints;
         col_off_[i + 1] = col_off_[i] + w->num_variables;
+        nz_off_[i + 1] = nz_off_[i] + w->constraint_matrix_num_nonzeros;
         nnz_ += w->constraint_matrix_num_nonzeros;
     }
     m_ = row_off_[N_];
@@ -925,30 +951,16 @@ void BatchSolver::setup()
 
     // --- per-instance norms of the unscaled problems (solver.cu:1999-2067) ---
     init_.assign(N_, InstState{});
-    const bool linf = P_.optimality_norm == NORM_TYPE_L_INF;
-    for (int i = 0; i < N_; ++i) {
-        const lp_problem_t *w = probs_[i];
-        double sq = 0.0, mx = 0.0;
-        for (int j = 0; j < w->num_variables; ++j) {
-            const double cj = w->objective_vector[j];
-            if (linf) mx = std::max(mx, std::fabs(cj));
-            else sq += cj * cj;
-        }
-        init_[i].c_norm = linf ? mx : std::sqrt(sq);
-        sq = mx = 0.0;
-        for (int r = 0; r < w->num_constraints; ++r) {
-            const double lo = w->constraint_lower_bound[r], hi = w->constraint_upper_bound[r];
-            if (linf) {
-                if (std::isfinite(lo) && lo != hi) mx = std::max(mx, std::fabs(lo));
-                if (std::isfinite(hi)) mx = std::max(mx, std::fabs(hi));
-            } else {
-                if (std::isfinite(lo) && lo != hi) sq += lo * lo;
-                if (std::isfinite(hi)) sq += hi * hi;
-            }
-        }
-        init_[i].b_norm = linf ? mx : std::sqrt(sq);
-        init_[i].obj_const = w->objective_constant;
+    for (int i = 0; i < N_; ++i) init_[i].obj_const = probs_[i]->objective_constant;
+    for (auto pr : {std::make_pair(&raw_c_, &c_), std::make_pair(&raw_lb_, &lb_), std::make_pair(&raw_ub_, &ub_)}) {
+        pr.first->alloc(n_);
+        copy_d(pr.first->get(), pr.second->get(), n_);
     }
+    for (auto pr : {std::make_pair(&raw_l_, &l_), std::make_pair(&raw_u_, &u_)}) {
+        pr.first->alloc(m_);
+        copy_d(pr.first->get(), pr.second->get(), m_);
+    }
+    init_norms();
 
     // the starts are scaled together with the problem (as x0/y0 in the single solver)
     copy_d(x0_.get(), xs_.get(), n_);
@@ -979,10 +991,37 @@ void BatchSolver::setup()
     cp_.eps_opt = P_.termination_criteria.eps_optimal_relative;
     cp_.iteration_limit = P_.termination_criteria.iteration_limit;
     cp_.K = P_.termination_evaluation_frequency;
-    cp_.linf = linf ? 1 : 0;
+    cp_.linf = P_.optimality_norm == NORM_TYPE_L_INF ? 1 : 0;
     rho_ = P_.reflection_coefficient;
     const double al_lambda = env_double("CP_AL_LAMBDA", 0.05);
-    for (int i = 0; i < N_; ++i) {  // initialize_step_size_and_primal_weight, solver.cu:3538-3609
+    init_steps();
+    pred_ = al_lambda > 0.0 && std::fmin(al_lambda, 0.95) > 0.0;
+    detect_shared();
+    warn_unsupported_controllers(true);
+    if (env_set("CP_AL_SIGMA") || env_int("CP_AL_SIGMA_MODE", 0) != 0)
+        fprintf(stderr, "[box-cpal] sigma mode is not applied by the batched solver (alpha mode runs).\n");
+    setup_s_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void BatchSolver::init_norms()
+{   // c_norm, b_norm of every instance from the unscaled vectors in c_, l_, u_
+    DBuf<double> out(2 * (size_t)N_);
+    seg_norms_kernel<<<N_, kBlock, 0, stream_>>>(d_row_off_.get(), d_col_off_.get(), l_.get(), u_.get(), c_.get(),
+                                                  P_.optimality_norm == NORM_TYPE_L_INF ? 1 : 0, out.get());
+    PDAL_LAUNCHED();
+    std::vector<double> h(2 * (size_t)N_);
+    out.download(h.data(), 2 * (size_t)N_, stream_);
+    PDAL_HIP(hipStreamSynchronize(stream_));
+    for (int i = 0; i < N_; ++i) {
+        init_[i].c_norm = h[2 * i];
+        init_[i].b_norm = h[2 * i + 1];
+    }
+}
+
+void BatchSolver::init_steps()
+{   // initialize_step_size_and_primal_weight (solver.cu:3538-3609) for every instance
+    const double al_lambda = env_double("CP_AL_LAMBDA", 0.05);
+    for (int i = 0; i < N_; ++i) {
         InstState &s = init_[i];
         s.pw = P_.bound_objective_rescaling ? 1.0 : (s.c_norm + 1.0) / (s.b_norm + 1.0);
         s.al_lambda = al_lambda;
@@ -998,12 +1037,6 @@ void BatchSolver::setup()
         s.last_trial_fp = INFINITY;
         s.active = 1;
     }
-    pred_ = al_lambda > 0.0 && std::fmin(al_lambda, 0.95) > 0.0;
-    detect_shared();
-    warn_unsupported_controllers();
-    if (env_set("CP_AL_SIGMA") || env_int("CP_AL_SIGMA_MODE", 0) != 0)
-        fprintf(stderr, "[box-cpal] sigma mode is not applied by the batched solver (alpha mode runs).\n");
-    setup_s_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
 void BatchSolver::rescale()
@@ -1022,6 +1055,11 @@ void BatchSolver::rescale()
         scale_rows_kernel<<<blocks_for_l(n_), kBlock, 0, stream_>>>(At_.rp.get(), At_.ci.get(), At_.v.get(), ni,
                                                                    inv_d.get(), inv_e.get());
         PDAL_LAUNCHED();
+        // this pass's factors, for set_vectors
+        const long p = fac_passes_++;
+        copy_d(fac_d_.get() + p * fac_cols_, d.get(), fac_cols_);
+        copy_d(fac_inv_d_.get() + p * fac_cols_, inv_d.get(), fac_cols_);
+        copy_d(fac_inv_e_.get() + p * fac_rows_, inv_e.get(), fac_rows_);
     };
     // Curtis-Reid first when CP_AL_CR_SCALING is set (curtis_reid_rescaling,
     // preconditioner.cu:162-268; iterations from CP_AL_CR_ITERS, default 20):
@@ -1031,6 +1069,13 @@ void BatchSolver::rescale()
     int cr_iters = (cr_raw && cr_raw[0] != '\0' && atoi(cr_raw) != 0) ? 20 : 0;
     const char *cr_iters_raw = getenv("CP_AL_CR_ITERS");
     if (cr_iters > 0 && cr_iters_raw && cr_iters_raw[0] != '\0') cr_iters = std::max(atoi(cr_iters_raw), 0);
+    const int passes = (cr_iters > 0) + std::max(P_.l_inf_ruiz_iterations, 0) + (P_.has_pock_chambolle_alpha ? 1 : 0);
+    fac_cols_ = same_a_ ? n_ / N_ : n_;
+    fac_rows_ = same_a_ ? m_ / N_ : m_;
+    fac_passes_ = 0;
+    fac_d_.alloc((size_t)std::max(passes, 1) * fac_cols_);
+    fac_inv_d_.alloc((size_t)std::max(passes, 1) * fac_cols_);
+    fac_inv_e_.alloc((size_t)std::max(passes, 1) * fac_rows_);
     if (cr_iters > 0) {
         PDAL_HIP(hipMemsetAsync(e.get(), 0, (size_t)m_ * sizeof(double), stream_));
         PDAL_HIP(hipMemsetAsync(d.get(), 0, (size_t)n_ * sizeof(double), stream_));
@@ -1060,6 +1105,13 @@ void BatchSolver::rescale()
         PDAL_LAUNCHED();
         scale_problem();
     }
+    bound_objective_scales();
+}
+
+void BatchSolver::bound_objective_scales()
+{   // per-instance bound and objective scales (bound_objective_rescaling,
+    // preconditioner.cu:329-377, applied at :458-464): l, u,
+    // lb, ub, x0 by b_scale = 1/(||b||+1), c, y0 by c_scale = 1/(||c||+1)
     std::vector<double> scales(2 * (size_t)N_, 1.0);
     if (P_.bound_objective_rescaling) {
         DBuf<double> ds(2 * (size_t)N_);
@@ -1503,10 +1555,9 @@ cp_al_result_t *BatchSolver::result(int i)
                                 hipMemcpyDeviceToHost, stream_));
     PDAL_HIP(hipStreamSynchronize(stream_));
     const InstState &s = final_[i];
-    const lp_problem_t *w = probs_[i];
-    r->num_variables = w->num_variables;
-    r->num_constraints = w->num_constraints;
-    r->num_nonzeros = w->constraint_matrix_num_nonzeros;
+    r->num_variables = (int)(col_off_[i + 1] - col_off_[i]);  // the problems may be freed after set-up
+    r->num_constraints = (int)(row_off_[i + 1] - row_off_[i]);
+    r->num_nonzeros = (int)(nz_off_[i + 1] - nz_off_[i]);
     r->total_count = s.charged;
     r->rescaling_time_sec = 0.0;
     r->cumulative_time_sec = solve_s_;
@@ -1619,6 +1670,54 @@ void BatchSolver::set_primal_weights(const double *pw)
     }
 }
 
+void BatchSolver::set_vectors(const double *c, const double *l, const double *u, const double *lb,
+                              const double *ub, bool device)
+{   // The matrices keep their scaling and step-size estimate (both depend on A
+    // only); the new vectors go through the recorded passes and the bound and
+    // objective scales as a fresh set-up would take them, so the solves that
+    // follow have the bits of a batch created with these vectors.  The starts
+    // become zero and the primal weights their defaults.
+    const hipMemcpyKind kind = device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    auto take = [&](const double *src, DBuf<double> &raw, long len) {
+        if (src && len > 0)
+            PDAL_HIP(hipMemcpyAsync(raw.get(), src, (size_t)len * sizeof(double), kind, stream_));
+    };
+    take(c, raw_c_, n_);
+    take(lb, raw_lb_, n_);
+    take(ub, raw_ub_, n_);
+    take(l, raw_l_, m_);
+    take(u, raw_u_, m_);
+    copy_d(c_.get(), raw_c_.get(), n_);
+    copy_d(lb_.get(), raw_lb_.get(), n_);
+    copy_d(ub_.get(), raw_ub_.get(), n_);
+    copy_d(l_.get(), raw_l_.get(), m_);
+    copy_d(u_.get(), raw_u_.get(), m_);
+    init_norms();  // synchronizes: the host sources may be released after this
+    if (fac_passes_ > 0)
+        replay_scaling_kernel<<<blocks_for_l(n_ + m_), kBlock, 0, stream_>>>(
+            c_.get(), lb_.get(), ub_.get(), n_, fac_d_.get(), fac_inv_d_.get(), fac_cols_, l_.get(), u_.get(), m_,
+            fac_inv_e_.get(), fac_rows_, fac_passes_);
+    PDAL_LAUNCHED();
+    PDAL_HIP(hipMemsetAsync(x0_.get(), 0, (size_t)n_ * sizeof(double), stream_));
+    PDAL_HIP(hipMemsetAsync(y0_.get(), 0, (size_t)m_ * sizeof(double), stream_));
+    bound_objective_scales();
+    PDAL_HIP(hipMemsetAsync(xs_.get(), 0, (size_t)n_ * sizeof(double), stream_));
+    PDAL_HIP(hipMemsetAsync(ys_.get(), 0, (size_t)m_ * sizeof(double), stream_));
+    init_steps();
+    if (shared_ && detect_vector_flags()) {
+        // the shared-A kernels were captured for the other bound / row layout
+        if (sh_graph_) (void)hipGraphExecDestroy(sh_graph_);
+        sh_graph_ = nullptr;
+        sh_drop_compact_graph();
+        sh_alloc(0, sh_width_[0]);
+        sh_width_[1] = 0;  // reallocated (with or without side-by-side bounds) at the next packing
+    }
+    final_.clear();
+    res_ready_ = false;
+    st_.upload(init_.data(), N_, stream_);
+    PDAL_HIP(hipStreamSynchronize(stream_));
+}
+
 void BatchSolver::stats(double *out) const
 {   // {set-up s, last solve s, graph replays, instances, optimal, total iterations, lanes A, lanes At,
     //  table compactions, s until half stopped, s until 99 % stopped, instances staged by
@@ -1686,6 +1785,11 @@ void batch_free(void *h) { delete static_cast<BatchSolver *>(h); }
 void batch_time_kernels(void *h, int iters, double *out) { static_cast<BatchSolver *>(h)->time_kernels(iters, out); }
 void batch_primal_weights(void *h, double *out) { static_cast<BatchSolver *>(h)->primal_weights(out); }
 void batch_set_primal_weights(void *h, const double *pw) { static_cast<BatchSolver *>(h)->set_primal_weights(pw); }
+void batch_set_vectors(void *h, const double *c, const double *l, const double *u, const double *lb, const double *ub,
+                       bool device)
+{
+    static_cast<BatchSolver *>(h)->set_vectors(c, l, u, lb, ub, device);
+}
 
 }  // namespace lp
 }  // namespace pdal

@@ -4,9 +4,8 @@
 // :91-214 (anonymous matrix-structure features), :216-387 (feature auto
 // policy), :408-469 (unified policy), :589-832 (two-phase structural tail),
 // :471-586 (output files).  The policy layer configures the solver through the
-// same CP_AL_* environment variables the library reads; the tail's positive-
-// alpha pulse controller is not part of this build (a request is reported and
-// ignored by the library).
+// same CP_AL_* environment variables the library reads (the tail's positive-
+// alpha pulse, the restart-candidate selection of the feature policy).
 #include "cp_al.h"
 #include "mps_parser.h"
 #include "pdal_lp.h"

@@ -53,26 +53,30 @@ bool env_set(const char *name)
     return raw && raw[0] != '\0';
 }
 
-void warn_unsupported_controllers()
+void warn_unsupported_controllers(bool batch)
 {
     // Controllers of the reference that are switched on only through these
-    // variables (solver.cu:189-1057, 3166-3536); the restart knobs
-    // (utils.cu:324-332) and sigma mode (solver.cu:3141-3164) are honoured.
+    // variables (solver.cu:3166-3536 and the loop's dynamic evaluation); the
+    // restart knobs (utils.cu:324-332) and sigma mode (solver.cu:3141-3164) are
+    // honoured, and the single-instance solver also runs the positive-alpha
+    // controllers (solver.cu:189-1057) and the restart-candidate selection
+    // (:2663-2819), which the batched solver does not.
     static const char *const knobs[] = {"CP_AL_SIGMA_TRIGGER",
                                         "CP_AL_ADAPTIVE_LAMBDA",
                                         "CP_AL_LAMBDA_PROFILE",
-                                        "CP_AL_ALPHA_ROLLBACK_TRIAL",
-                                        "CP_AL_ALPHA_DIRECT_PULSE",
-                                        "CP_AL_RESTART_SELECT",
                                         "CP_AL_DYNAMIC_EVAL",
                                         "CP_AL_DYNAMIC_EVAL_AFTER_SEC",
                                         "CP_AL_DYNAMIC_EVAL_AFTER_ITERS",
-                                        "CP_AL_LOG_FREQ"};
-    for (const char *k : knobs) {
-        const char *raw = getenv(k);
+                                        "CP_AL_LOG_FREQ",
+                                        "CP_AL_ALPHA_ROLLBACK_TRIAL",
+                                        "CP_AL_ALPHA_DIRECT_PULSE",
+                                        "CP_AL_RESTART_SELECT"};
+    const size_t count = batch ? sizeof(knobs) / sizeof(knobs[0]) : 7;
+    for (size_t i = 0; i < count; ++i) {
+        const char *raw = getenv(knobs[i]);
         if (raw && raw[0] != '\0' && strcmp(raw, "0") != 0)
-            fprintf(stderr, "[cp_al-mi355x] %s=%s requests a controller this build does not implement; ignored\n",
-                    k, raw);
+            fprintf(stderr, "[cp_al-mi355x] %s=%s requests a controller this %s does not implement; ignored\n",
+                    knobs[i], raw, batch ? "batched solver" : "build");
     }
 }
 
@@ -612,6 +616,31 @@ PDAL_API int pdal_cp_al_batch_set_primal_weights(pdal_cp_al_batch_t *batch, cons
     if (!batch || !pw) return -1;
     batch_set_primal_weights(batch, pw);
     return 0;
+}
+
+static int batch_vectors(pdal_cp_al_batch_t *batch, const double *c, const double *con_lb, const double *con_ub,
+                         const double *var_lb, const double *var_ub, bool device)
+{
+    if (!batch) return -1;
+    try {
+        batch_set_vectors(batch, c, con_lb, con_ub, var_lb, var_ub, device);
+        return 0;
+    } catch (const std::exception &e) {
+        fprintf(stderr, "[cp_al-mi355x] batch vector update failed: %s\n", e.what());
+        return -2;
+    }
+}
+
+PDAL_API int pdal_cp_al_batch_set_vectors(pdal_cp_al_batch_t *batch, const double *c, const double *con_lb,
+                                          const double *con_ub, const double *var_lb, const double *var_ub)
+{
+    return batch_vectors(batch, c, con_lb, con_ub, var_lb, var_ub, false);
+}
+
+PDAL_API int pdal_cp_al_batch_set_vectors_device(pdal_cp_al_batch_t *batch, const double *c, const double *con_lb,
+                                                 const double *con_ub, const double *var_lb, const double *var_ub)
+{
+    return batch_vectors(batch, c, con_lb, con_ub, var_lb, var_ub, true);
 }
 
 PDAL_API void pdal_cp_al_batch_free(pdal_cp_al_batch_t *batch)

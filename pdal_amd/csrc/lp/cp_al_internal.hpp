@@ -14,7 +14,7 @@ int env_int(const char *name, int def);
 bool env_set(const char *name);
 
 // Warn once per solve about reference controllers this build does not run.
-void warn_unsupported_controllers();
+void warn_unsupported_controllers(bool batch);  // batch: the batched solver (fewer controllers)
 
 // Plain-path solve on the current HIP device (throws pdal::HipError).
 cp_al_result_t *optimize_gpu(const cp_al_parameters_t *params, lp_problem_t *problem);
@@ -48,6 +48,8 @@ void batch_free(void *batch);
 void batch_time_kernels(void *batch, int iters, double *out);
 void batch_primal_weights(void *batch, double *out);
 void batch_set_primal_weights(void *batch, const double *pw);
+void batch_set_vectors(void *batch, const double *c, const double *l, const double *u, const double *lb,
+                       const double *ub, bool device);
 
 }  // namespace lp
 }  // namespace pdal

@@ -344,6 +344,75 @@ static __global__ void __launch_bounds__(kBlock)
     block_reduce_store<1, 0>(q, parts + blockIdx.x);
 }
 
+// Restart-candidate score terms (compute_restart_delta_score, solver.cu:2663-2747)
+// of the difference (candidate - reference) of a primal-dual pair, rounded
+// first as the reference's materialised delta vectors are.  Over the rows of
+// At: ||dx||^2 and <At dy, dx>; over the rows of A: ||A dx||^2 and ||dy||^2.
+// Partials two per block (grid-stride, fixed grid).
+template <int G>
+static __global__ void __launch_bounds__(kBlock)
+    score_at_kernel(CsrView At, const double *__restrict__ cy, const double *__restrict__ ry,
+                    const double *__restrict__ cx, const double *__restrict__ rx, double *__restrict__ parts)
+{
+    constexpr int RPB = kBlock / G;
+    double q[2] = {0.0, 0.0};
+    const int lane = threadIdx.x & (G - 1);
+    for (int base = blockIdx.x * RPB; base < At.rows; base += gridDim.x * RPB) {
+        const int row = base + threadIdx.x / G;
+        if (row < At.rows) {
+            double s_diff, s_b;
+            row_dot2<G>(At, row, lane, cy, ry, s_diff, s_b);
+            if (lane == 0) {
+                const double dx = cx[row] - rx[row];
+                q[0] += dx * dx;
+                q[1] += s_diff * dx;
+            }
+        }
+    }
+    block_reduce_store<2, 0>(q, parts + 2 * blockIdx.x);
+}
+
+template <int G>
+static __global__ void __launch_bounds__(kBlock)
+    score_a_kernel(CsrView A, const double *__restrict__ cx, const double *__restrict__ rx,
+                   const double *__restrict__ cy, const double *__restrict__ ry, double *__restrict__ parts)
+{
+    constexpr int RPB = kBlock / G;
+    double q[2] = {0.0, 0.0};
+    const int lane = threadIdx.x & (G - 1);
+    for (int base = blockIdx.x * RPB; base < A.rows; base += gridDim.x * RPB) {
+        const int row = base + threadIdx.x / G;
+        if (row < A.rows) {
+            double s_diff, s_b;
+            row_dot2<G>(A, row, lane, cx, rx, s_diff, s_b);
+            if (lane == 0) {
+                const double dy = cy[row] - ry[row];
+                q[0] += s_diff * s_diff;
+                q[1] += dy * dy;
+            }
+        }
+    }
+    block_reduce_store<2, 0>(q, parts + 2 * blockIdx.x);
+}
+
+// ||a - b||^2 over n entries and ||c - d||^2 over m entries: the restart
+// distances from the anchor when the restart point is not the T output.
+static __global__ void __launch_bounds__(kBlock)
+    dist2_kernel(const double *__restrict__ a, const double *__restrict__ b, int n, const double *__restrict__ c,
+                 const double *__restrict__ d, int m, double *__restrict__ parts)
+{
+    double q[2] = {0.0, 0.0};
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const double t = a[i] - b[i];
+        q[0] += t * t;
+    }
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < m; i += gridDim.x * kBlock) {
+        const double t = c[i] - d[i];
+        q[1] += t * t;
+    }
+    block_reduce_store<2, 0>(q, parts + 2 * blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // Power iteration for sigma_max(A)^2 on A A^T (utils.cu:87-228), device
 // resident: every kernel returns at once after convergence or max_iter, so a

@@ -85,7 +85,7 @@ class Solver {
     void perform_restart(const Eval &e);
     void refresh_step_schedule();
     void init_sigma();
-    double delta_normal_motion();
+    double delta_normal_motion(const double *rx);
     cp_al_result_t *make_result(const lp_problem_t *original, int charged);
     void display() const;
     // feasibility polishing (solver.cu:3912-4456)
@@ -139,6 +139,63 @@ class Solver {
     double rescale_time_ = 0.0, solve_time_ = 0.0, setup_time_ = 0.0;
     double pw_init_ = 1.0;
     Eval last_eval_{};
+    // Restart candidate (choose_restart_candidate, solver.cu:2749-2819): 0 = the
+    // T output, 1 = the current (reflected Halpern) iterate, 2 = the Halpern point.
+    int restart_target_ = 0;
+    double score_current_ = 0.0, score_average_ = INFINITY, score_halpern_ = INFINITY;
+    bool fp_stale_ = false;  // a restart moved x_R or x_T after the last evaluation
+    DBuf<double> scoreq_;
+    PinnedBuf<double> h_score_{4};
+    double delta_score(const double *cx, const double *rx, const double *cy, const double *ry);
+    void choose_restart_candidate();
+    void refresh_fp();
+    // Positive-alpha controllers (solver.cu:187-1057): the zero-alpha epoch
+    // history, the rollback trial with its snapshot, the deterministic pulse.
+    struct Scalars {
+        double step_size, pw, al_lambda, al_penalty, lambda_param, al_sigma, sigma_best, sigma_err_sum,
+            sigma_last_err, pw_err_sum, pw_last_err, best_pw, best_gap, fp, fp0, last_trial_fp, abs_p, rel_p, abs_d,
+            rel_d, p_obj, d_obj, gap, rel_gap, score_current, score_average, score_halpern;
+        bool sigma_mode, fp_stale;
+        int total, inner, restart_target;
+        termination_reason_t reason;
+        Eval last_eval;
+    };
+    Scalars save_scalars() const;
+    void load_scalars(const Scalars &v);
+    struct ZeroEpoch {
+        bool valid = false;
+        int iterations = 0;
+        double seconds = 0, fp_rate = 0, p_rate = 0, d_rate = 0, g_rate = 0, kkt_rate = 0;
+    };
+    struct Trial {  // alpha_trial_controller_t, solver.cu:210-241
+        bool enabled = false, trace = false, force_reject = false, active = false, epoch_valid = false;
+        int started = 0, accepted = 0, rejected = 0, positive_epochs = 0, positive_charged = 0, count = 0;
+        ZeroEpoch hist[2];
+        int ep_total = 0, t_total = 0, t_charged = 0, max_iters = 0;
+        double ep_time = 0, ep_p = 0, ep_d = 0, ep_g = 0, t_time = 0, t_fp = 0, t_p = 0, t_d = 0, t_g = 0,
+               max_sec = 0;
+        bool snap_valid = false;
+        Scalars snap{};
+    } trial_;
+    enum { kPulseIdle = 0, kPulsePositive = 1, kPulseDone = 2 };
+    struct Pulse {  // alpha_direct_pulse_controller_t, solver.cu:250-274
+        bool enabled = false, trace = false, force = false;
+        int phase = 0, started = 0, completed = 0, aborted = 0, gate_eval = 0, gate_pass = 0, gate_rej = 0;
+        int pulse_iters = 200, gate_anchor = 0, gate_prev_k = 0, gate_recent_k = 0;
+        double gate_dfp = 0, pos_fp = 0, pos_p = 0, pos_d = 0, pos_g = 0;
+        termination_reason_t pos_reason = TERMINATION_REASON_UNSPECIFIED;
+    } pulse_;
+    DBuf<double> snap_x_[5], snap_y_[5];  // x0, x, x_T, x_H, x_R and the y counterparts
+    void trial_snapshot();
+    double trial_restore();
+    void set_epoch_start();
+    void record_zero_epoch();
+    bool trial_eligible(int charged) const;
+    void trial_start(int charged);
+    bool trial_accepts(int charged, const char **why) const;
+    bool pulse_eligible(int charged, int K);
+    void pulse_start(int charged, int K);
+    void pulse_finish(int charged);
     // Problem data the iteration kernels read and the vectors the evaluation
     // reads; the feasibility-polishing phases point them at their own copies.
     struct View {
@@ -217,6 +274,7 @@ void Solver::upload()
     parts_at_.alloc((size_t)eval_grid_at_ * kEvalStride);
     parts_a_.alloc((size_t)eval_grid_a_ * kEvalStride);
     evalq_.alloc(16);
+    scoreq_.alloc(4);
     sc_.alloc(1);
     PDAL_HIP(hipMemsetAsync(sc_.get(), 0, sizeof(IterScalars), stream_));
     PDAL_HIP(hipStreamSynchronize(stream_));  // host CSR copies are released below
@@ -548,7 +606,7 @@ void Solver::init_steps()
     al_lambda_ = env_double("CP_AL_LAMBDA", 0.05);
     refresh_step_schedule();
     best_pw_ = pw_;
-    warn_unsupported_controllers();
+    warn_unsupported_controllers(false);
     if (P_.verbose)
         fprintf(stderr,
                 "[box-cpal] pw_init=%.3e al_lambda=%.3e sigma_mode=%d sigma_adapt=%d al_sigma=%.3e sigma_base=%.3e "
@@ -630,6 +688,10 @@ void Solver::reset_iterates()
     reason_ = TERMINATION_REASON_UNSPECIFIED;
     solve_time_ = 0.0;
     last_eval_ = Eval{};
+    restart_target_ = 0;
+    score_current_ = 0.0;
+    score_average_ = score_halpern_ = INFINITY;
+    fp_stale_ = false;
     sync_scalars();
 }
 
@@ -836,12 +898,12 @@ bool Solver::should_restart(int K)
     return r;
 }
 
-double Solver::delta_normal_motion()
-{   // ||A (xT - x0)|| before the restart copies (solver.cu:2867-2884)
+double Solver::delta_normal_motion(const double *rx)
+{   // ||A (restart point - x0)|| before the restart copies (solver.cu:2867-2884)
     const int grid = eval_grid_a_;
     dispatch_group(A_.G, [&](auto g) {
         constexpr int G = decltype(g)::value;
-        delta_spmv_sumsq_kernel<G><<<grid, kBlock, 0, stream_>>>(A_.view(), xT_.get(), x0_.get(), parts_a_.get());
+        delta_spmv_sumsq_kernel<G><<<grid, kBlock, 0, stream_>>>(A_.view(), rx, x0_.get(), parts_a_.get());
     });
     reduce_partials_kernel<<<1, kBlock, 0, stream_>>>(parts_a_.get(), grid, 1, 1, 1, evalq_.get());
     PDAL_LAUNCHED();
@@ -852,13 +914,26 @@ double Solver::delta_normal_motion()
 }
 
 void Solver::perform_restart(const Eval &e)
-{   // solver.cu:2821-3101, restart target "current"; PID on the primal weight, or on sigma in sigma mode
-    const double pd = std::sqrt(e.rx2), dd = std::sqrt(e.ry2);
+{   // solver.cu:2821-3101: restart to the selected candidate; PID on the primal weight, or on sigma in sigma mode
+    const int target = env_int("CP_AL_RESTART_SELECT", 0) != 0 ? restart_target_ : 0;  // :2823-2834
+    const double *rx = target == 1 ? x_.get() : target == 2 ? xH_.get() : xT_.get();
+    const double *ry = target == 1 ? y_.get() : target == 2 ? yH_.get() : yT_.get();
+    double pd = std::sqrt(e.rx2), dd = std::sqrt(e.ry2);  // the evaluation's ||x_T - x0||, ||y_T - y0||
+    if (target != 0) {
+        const int grid = eval_grid_at_;
+        dist2_kernel<<<grid, kBlock, 0, stream_>>>(rx, x0_.get(), n_, ry, y0_.get(), m_, parts_at_.get());
+        reduce_partials_kernel<<<1, kBlock, 0, stream_>>>(parts_at_.get(), grid, 2, 2, 2, scoreq_.get());
+        PDAL_LAUNCHED();
+        PDAL_HIP(hipMemcpyAsync(h_score_.get(), scoreq_.get(), 2 * sizeof(double), hipMemcpyDeviceToHost, stream_));
+        PDAL_HIP(hipStreamSynchronize(stream_));
+        pd = std::sqrt(h_score_[0]);
+        dd = std::sqrt(h_score_[1]);
+    }
     const double ratio = rel_d_ / std::fmax(rel_p_, 1e-300);
     const restart_parameters_t &rp = P_.restart_params;
     if (sigma_mode_) {  // :2857-2966
         if (sigma_adaptive_) {
-            const double ax = delta_normal_motion();
+            const double ax = delta_normal_motion(rx);
             const double dx_metric_sq = lambda_param_ * pd * pd - ax * ax;
             if (dd > 1e-16 && dd < 1e12 && dx_metric_sq > 1e-24 && dx_metric_sq < 1e24 && std::isfinite(al_sigma_) &&
                 al_sigma_ > 1e-16 && ratio > 1e-8 && ratio < 1e8) {
@@ -917,11 +992,438 @@ void Solver::perform_restart(const Eval &e)
         }
     }
     refresh_step_schedule();
-    restart_copy_kernel<<<blocks_for((long)n_ + m_), kBlock, 0, stream_>>>(xT_.get(), x0_.get(), x_.get(), n_,
-                                                                          yT_.get(), y0_.get(), y_.get(), m_);
-    PDAL_LAUNCHED();
+    auto copy = [&](double *dst, const double *src, int len) {
+        if (len > 0)
+            PDAL_HIP(hipMemcpyAsync(dst, src, (size_t)len * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    };
+    if (target == 1) {  // :3019-3043: reflected = old T output; anchor = T output = the current iterate
+        copy(xR_.get(), xT_.get(), n_);
+        copy(yR_.get(), yT_.get(), m_);
+        copy(x0_.get(), x_.get(), n_);
+        copy(xT_.get(), x_.get(), n_);
+        copy(y0_.get(), y_.get(), m_);
+        copy(yT_.get(), y_.get(), m_);
+        fp_stale_ = true;
+    } else if (target == 2) {  // :3044-3076: reflected = old T output; anchor = current = T output = Halpern point
+        copy(xR_.get(), xT_.get(), n_);
+        copy(yR_.get(), yT_.get(), m_);
+        for (double *d : {x0_.get(), x_.get(), xT_.get()}) copy(d, xH_.get(), n_);
+        for (double *d : {y0_.get(), y_.get(), yT_.get()}) copy(d, yH_.get(), m_);
+        fp_stale_ = true;
+    } else {
+        restart_copy_kernel<<<blocks_for((long)n_ + m_), kBlock, 0, stream_>>>(xT_.get(), x0_.get(), x_.get(), n_,
+                                                                              yT_.get(), y0_.get(), y_.get(), m_);
+        PDAL_LAUNCHED();
+    }
     inner_ = 0;
     last_trial_fp_ = INFINITY;
+}
+
+double Solver::delta_score(const double *cx, const double *rx, const double *cy, const double *ry)
+{   // compute_restart_delta_score (solver.cu:2663-2747), the fixed-point metric of (candidate - reference)
+    dispatch_group(At_.G, [&](auto g) {
+        constexpr int G = decltype(g)::value;
+        score_at_kernel<G><<<eval_grid_at_, kBlock, 0, stream_>>>(At_.view(), cy, ry, cx, rx, parts_at_.get());
+    });
+    dispatch_group(A_.G, [&](auto g) {
+        constexpr int G = decltype(g)::value;
+        score_a_kernel<G><<<eval_grid_a_, kBlock, 0, stream_>>>(A_.view(), cx, rx, cy, ry, parts_a_.get());
+    });
+    reduce_partials_kernel<<<1, kBlock, 0, stream_>>>(parts_at_.get(), eval_grid_at_, 2, 2, 2, scoreq_.get());
+    reduce_partials_kernel<<<1, kBlock, 0, stream_>>>(parts_a_.get(), eval_grid_a_, 2, 2, 2, scoreq_.get() + 2);
+    PDAL_LAUNCHED();
+    PDAL_HIP(hipMemcpyAsync(h_score_.get(), scoreq_.get(), 4 * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    PDAL_HIP(hipStreamSynchronize(stream_));
+    const double pn2 = h_score_[0], cross = h_score_[1], an2 = h_score_[2], dn2 = h_score_[3];
+    if (sigma_mode_ && al_sigma_ > 0.0 && lambda_param_ > 0.0) {
+        const double val = al_sigma_ * (lambda_param_ * pn2 - an2) + dn2 / std::fmax(al_sigma_, 1e-16);
+        return std::sqrt(std::fmax(val, 0.0));
+    }
+    const double movement = pn2 * pw_ + dn2 / pw_;
+    return std::sqrt(std::fmax(movement + 2 * step_size_ * cross - step_size_ * al_penalty_ * an2, 0.0));
+}
+
+void Solver::refresh_fp()
+{   // compute_fixed_point_error (solver.cu:3611-3686) at the current x_R - x_T, y_R - y_T: from the last
+    // evaluation's terms unless a restart has moved those vectors since
+    if (fp_stale_) fp_ = delta_score(xR_.get(), xT_.get(), yR_.get(), yT_.get());
+    else fixed_point_error(last_eval_);
+}
+
+void Solver::choose_restart_candidate()
+{   // solver.cu:2749-2819
+    const int mode = env_int("CP_AL_RESTART_SELECT", 0);
+    restart_target_ = 0;
+    score_current_ = fp_;
+    score_average_ = INFINITY;
+    score_halpern_ = INFINITY;
+    if (mode == 0) return;
+    if (total_ < env_int("CP_AL_RESTART_SELECT_MIN_ITERS", 0) ||
+        solve_time_ < env_double("CP_AL_RESTART_SELECT_MIN_TIME", 0.0))
+        return;
+    score_average_ = delta_score(xT_.get(), x_.get(), yT_.get(), y_.get());
+    const bool three = mode == 3 || env_int("CP_AL_RESTART_SELECT_THREEPOINT", 0) != 0;
+    if (three) score_halpern_ = delta_score(xT_.get(), xH_.get(), yT_.get(), yH_.get());
+    const double margin = env_double("CP_AL_RESTART_SELECT_MARGIN", 1.0);
+    if (mode == 2) {
+        restart_target_ = 1;
+        fp_ = score_average_;
+        return;
+    }
+    if (!std::isfinite(score_current_)) {
+        fp_ = score_current_;
+        return;
+    }
+    int best = 0;
+    double score = score_current_;
+    if (std::isfinite(score_average_) && score_average_ < score) {
+        best = 1;
+        score = score_average_;
+    }
+    if (three && std::isfinite(score_halpern_) && score_halpern_ < score) {
+        best = 2;
+        score = score_halpern_;
+    }
+    if (best != 0 && score <= margin * score_current_) {
+        restart_target_ = best;
+        fp_ = score;
+    } else {
+        fp_ = score_current_;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Positive-alpha controllers (solver.cu:187-1057; used at :1202-1222,
+// :1372-1494, :1631-1695).  Host bookkeeping at the evaluation points; the
+// trial's snapshot holds the scalars and the ten iterate vectors.
+
+double ctl_metric(double v) { return std::fmax(std::fabs(v), 1e-300); }  // alpha_trial_metric, :409-412
+
+double ctl_log_rate(double start, double end, int iters)  // alpha_trial_log_rate, :414-422
+{
+    if (iters <= 0 || !std::isfinite(start) || !std::isfinite(end)) return NAN;
+    const double r = std::log(ctl_metric(start) / ctl_metric(end)) / (double)iters;
+    return std::isfinite(r) ? r : NAN;
+}
+
+double ctl_predict(double rate, int iters)  // alpha_trial_predict_ratio, :424-428
+{
+    return std::exp(std::fmax(-700.0, std::fmin(700.0, -rate * (double)iters)));
+}
+
+Solver::Scalars Solver::save_scalars() const
+{
+    Scalars v;
+    v.step_size = step_size_;
+    v.pw = pw_;
+    v.al_lambda = al_lambda_;
+    v.al_penalty = al_penalty_;
+    v.lambda_param = lambda_param_;
+    v.al_sigma = al_sigma_;
+    v.sigma_best = sigma_best_;
+    v.sigma_err_sum = sigma_err_sum_;
+    v.sigma_last_err = sigma_last_err_;
+    v.pw_err_sum = pw_err_sum_;
+    v.pw_last_err = pw_last_err_;
+    v.best_pw = best_pw_;
+    v.best_gap = best_gap_;
+    v.fp = fp_;
+    v.fp0 = fp0_;
+    v.last_trial_fp = last_trial_fp_;
+    v.abs_p = abs_p_;
+    v.rel_p = rel_p_;
+    v.abs_d = abs_d_;
+    v.rel_d = rel_d_;
+    v.p_obj = p_obj_;
+    v.d_obj = d_obj_;
+    v.gap = gap_;
+    v.rel_gap = rel_gap_;
+    v.score_current = score_current_;
+    v.score_average = score_average_;
+    v.score_halpern = score_halpern_;
+    v.sigma_mode = sigma_mode_;
+    v.fp_stale = fp_stale_;
+    v.total = total_;
+    v.inner = inner_;
+    v.restart_target = restart_target_;
+    v.reason = reason_;
+    v.last_eval = last_eval_;
+    return v;
+}
+
+void Solver::load_scalars(const Scalars &v)
+{
+    step_size_ = v.step_size;
+    pw_ = v.pw;
+    al_lambda_ = v.al_lambda;
+    al_penalty_ = v.al_penalty;
+    lambda_param_ = v.lambda_param;
+    al_sigma_ = v.al_sigma;
+    sigma_best_ = v.sigma_best;
+    sigma_err_sum_ = v.sigma_err_sum;
+    sigma_last_err_ = v.sigma_last_err;
+    pw_err_sum_ = v.pw_err_sum;
+    pw_last_err_ = v.pw_last_err;
+    best_pw_ = v.best_pw;
+    best_gap_ = v.best_gap;
+    fp_ = v.fp;
+    fp0_ = v.fp0;
+    last_trial_fp_ = v.last_trial_fp;
+    abs_p_ = v.abs_p;
+    rel_p_ = v.rel_p;
+    abs_d_ = v.abs_d;
+    rel_d_ = v.rel_d;
+    p_obj_ = v.p_obj;
+    d_obj_ = v.d_obj;
+    gap_ = v.gap;
+    rel_gap_ = v.rel_gap;
+    score_current_ = v.score_current;
+    score_average_ = v.score_average;
+    score_halpern_ = v.score_halpern;
+    sigma_mode_ = v.sigma_mode;
+    fp_stale_ = v.fp_stale;
+    total_ = v.total;
+    inner_ = v.inner;
+    restart_target_ = v.restart_target;
+    reason_ = v.reason;
+    last_eval_ = v.last_eval;
+}
+
+void Solver::trial_snapshot()
+{   // alpha_trial_snapshot_save, :442-475
+    DBuf<double> *xs[5] = {&x0_, &x_, &xT_, &xH_, &xR_}, *ys[5] = {&y0_, &y_, &yT_, &yH_, &yR_};
+    for (int k = 0; k < 5; ++k) {
+        if (!snap_x_[k].get()) snap_x_[k].alloc(n_);
+        if (!snap_y_[k].get()) snap_y_[k].alloc(m_);
+        if (n_ > 0)
+            PDAL_HIP(hipMemcpyAsync(snap_x_[k].get(), xs[k]->get(), (size_t)n_ * sizeof(double),
+                                    hipMemcpyDeviceToDevice, stream_));
+        if (m_ > 0)
+            PDAL_HIP(hipMemcpyAsync(snap_y_[k].get(), ys[k]->get(), (size_t)m_ * sizeof(double),
+                                    hipMemcpyDeviceToDevice, stream_));
+    }
+    trial_.snap = save_scalars();
+    trial_.snap_valid = true;
+}
+
+double Solver::trial_restore()
+{   // alpha_trial_snapshot_restore, :502-585: the snapshot's scalars and vectors, the elapsed time kept;
+    // returns the largest 2-norm change of the ten vectors
+    if (!trial_.snap_valid) return INFINITY;
+    DBuf<double> *xs[5] = {&x0_, &x_, &xT_, &xH_, &xR_}, *ys[5] = {&y0_, &y_, &yT_, &yH_, &yR_};
+    double diff = 0.0;
+    for (int k = 0; k < 5; ++k) {
+        dist2_kernel<<<eval_grid_at_, kBlock, 0, stream_>>>(xs[k]->get(), snap_x_[k].get(), n_, ys[k]->get(),
+                                                             snap_y_[k].get(), m_, parts_at_.get());
+        reduce_partials_kernel<<<1, kBlock, 0, stream_>>>(parts_at_.get(), eval_grid_at_, 2, 2, 2, scoreq_.get());
+        PDAL_LAUNCHED();
+        PDAL_HIP(hipMemcpyAsync(h_score_.get(), scoreq_.get(), 2 * sizeof(double), hipMemcpyDeviceToHost, stream_));
+        PDAL_HIP(hipStreamSynchronize(stream_));
+        diff = std::fmax(diff, std::fmax(std::sqrt(h_score_[0]), std::sqrt(h_score_[1])));
+        if (n_ > 0)
+            PDAL_HIP(hipMemcpyAsync(xs[k]->get(), snap_x_[k].get(), (size_t)n_ * sizeof(double),
+                                    hipMemcpyDeviceToDevice, stream_));
+        if (m_ > 0)
+            PDAL_HIP(hipMemcpyAsync(ys[k]->get(), snap_y_[k].get(), (size_t)m_ * sizeof(double),
+                                    hipMemcpyDeviceToDevice, stream_));
+    }
+    const double elapsed = solve_time_;
+    load_scalars(trial_.snap);
+    solve_time_ = elapsed;
+    return diff;
+}
+
+void Solver::set_epoch_start()
+{   // alpha_trial_set_epoch_start, :605-614
+    trial_.epoch_valid = true;
+    trial_.ep_total = total_;
+    trial_.ep_time = solve_time_;
+    trial_.ep_p = ctl_metric(rel_p_);
+    trial_.ep_d = ctl_metric(rel_d_);
+    trial_.ep_g = ctl_metric(rel_gap_);
+}
+
+void Solver::record_zero_epoch()
+{   // alpha_trial_record_zero_epoch, :616-678: rates of the epoch that ends at this restart
+    Trial &t = trial_;
+    if (!t.epoch_valid || t.active || al_lambda_ > 0.0) return;
+    ZeroEpoch r;
+    r.iterations = total_ - t.ep_total;
+    r.seconds = solve_time_ - t.ep_time;
+    const double k0 = std::fmax(t.ep_p, std::fmax(t.ep_d, t.ep_g));
+    const double k1 = std::fmax(ctl_metric(rel_p_), std::fmax(ctl_metric(rel_d_), ctl_metric(rel_gap_)));
+    r.fp_rate = ctl_log_rate(fp0_, fp_, r.iterations);
+    r.p_rate = ctl_log_rate(t.ep_p, rel_p_, r.iterations);
+    r.d_rate = ctl_log_rate(t.ep_d, rel_d_, r.iterations);
+    r.g_rate = ctl_log_rate(t.ep_g, rel_gap_, r.iterations);
+    r.kkt_rate = ctl_log_rate(k0, k1, r.iterations);
+    r.valid = r.iterations > 0 && r.seconds >= 0.0 && std::isfinite(r.fp_rate) && std::isfinite(r.p_rate) &&
+              std::isfinite(r.d_rate) && std::isfinite(r.g_rate) && std::isfinite(r.kkt_rate);
+    if (!r.valid) return;
+    if (t.count < 2) {
+        t.hist[t.count++] = r;
+    } else {
+        t.hist[0] = t.hist[1];
+        t.hist[1] = r;
+    }
+    if (t.trace)
+        fprintf(stderr,
+                "[alpha-zero-epoch] iter=%d K=%d sec=%.12e rates(fp=%.12e p=%.12e d=%.12e g=%.12e max=%.12e) "
+                "history=%d\n",
+                total_, r.iterations, r.seconds, r.fp_rate, r.p_rate, r.d_rate, r.g_rate, r.kkt_rate, t.count);
+}
+
+bool Solver::trial_eligible(int charged) const
+{   // alpha_trial_is_eligible, :680-703.  PDAL_ALPHA_TRIAL_MIN_SEC lowers the
+    // reference's 2 s floor on the elapsed time, so a test reaches the trial at
+    // a deterministic point.
+    const Trial &t = trial_;
+    const double min_sec = env_double("PDAL_ALPHA_TRIAL_MIN_SEC", 2.0);
+    if (!t.enabled || t.started != 0 || t.count < 2 || charged < 10000 || solve_time_ < min_sec ||
+        reason_ != TERMINATION_REASON_UNSPECIFIED)
+        return false;
+    const int hmax = std::max(t.hist[0].iterations, t.hist[1].iterations);
+    const double hsec = std::fmax(2.0, std::fmax(t.hist[0].seconds, t.hist[1].seconds));
+    const int rem_it = P_.termination_criteria.iteration_limit - charged;
+    const double rem_sec = P_.termination_criteria.time_sec_limit - solve_time_;
+    return hmax > 0 && rem_it >= 3 * hmax && rem_sec >= 3.0 * hsec;
+}
+
+void Solver::trial_start(int charged)
+{   // alpha_trial_start, :704-761
+    Trial &t = trial_;
+    trial_snapshot();
+    t.started = 1;
+    t.active = true;
+    t.t_total = total_;
+    t.t_charged = charged;
+    t.t_time = solve_time_;
+    t.t_p = ctl_metric(rel_p_);
+    t.t_d = ctl_metric(rel_d_);
+    t.t_g = ctl_metric(rel_gap_);
+    t.max_iters = 2 * std::max(t.hist[0].iterations, t.hist[1].iterations);
+    t.max_sec = 2.0 * std::fmax(2.0, std::fmax(t.hist[0].seconds, t.hist[1].seconds));
+    sigma_mode_ = false;
+    al_lambda_ = 0.002;
+    refresh_step_schedule();
+    refresh_fp();
+    t.t_fp = ctl_metric(fp_);
+    if (!std::isfinite(t.t_fp) || t.t_fp <= 0.0) {
+        t.active = false;
+        t.rejected = 1;
+        const double d = trial_restore();
+        fprintf(stderr,
+                "[alpha-trial-restore] reason=invalid-start charged_iter=%d trajectory_iter=%d max_vector_l2=%.12e\n",
+                charged, total_, d);
+        t.snap_valid = false;
+        return;
+    }
+    fprintf(stderr,
+            "[alpha-trial-start] charged_iter=%d trajectory_iter=%d solve_sec=%.12e alpha=2.000000000000e-03 "
+            "fp=%.12e primal=%.12e dual=%.12e gap=%.12e max_K=%d max_sec=%.12e\n",
+            charged, total_, solve_time_, t.t_fp, t.t_p, t.t_d, t.t_g, t.max_iters, t.max_sec);
+}
+
+bool Solver::trial_accepts(int charged, const char **why) const
+{   // alpha_trial_accepts, :777-861: the epoch at alpha > 0 against the rates of the last two zero-alpha epochs
+    const Trial &t = trial_;
+    const int k = total_ - t.t_total;
+    if (k <= 0) {
+        *why = "empty-epoch";
+        return false;
+    }
+    const ZeroEpoch &h0 = t.hist[0], &h1 = t.hist[1];
+    const double fp_pred = ctl_predict(0.5 * (h0.fp_rate + h1.fp_rate), k);
+    const double p_pred = t.t_p * ctl_predict(0.5 * (h0.p_rate + h1.p_rate), k);
+    const double d_pred = t.t_d * ctl_predict(0.5 * (h0.d_rate + h1.d_rate), k);
+    const double g_pred = t.t_g * ctl_predict(0.5 * (h0.g_rate + h1.g_rate), k);
+    const double k_pred = std::fmax(t.t_p, std::fmax(t.t_d, t.t_g)) * ctl_predict(0.5 * (h0.kkt_rate + h1.kkt_rate), k);
+    const double fp_obs = ctl_metric(fp_) / t.t_fp;
+    const double p_obs = ctl_metric(rel_p_), d_obs = ctl_metric(rel_d_), g_obs = ctl_metric(rel_gap_);
+    const double k_obs = std::fmax(p_obs, std::fmax(d_obs, g_obs));
+    const bool finite = std::isfinite(fp_pred) && std::isfinite(p_pred) && std::isfinite(d_pred) &&
+                        std::isfinite(g_pred) && std::isfinite(k_pred) && std::isfinite(fp_obs) &&
+                        std::isfinite(p_obs) && std::isfinite(d_obs) && std::isfinite(g_obs) && std::isfinite(k_obs);
+    bool ok = finite && fp_obs <= 0.90 * fp_pred && k_obs <= 0.95 * k_pred && p_obs <= 1.05 * p_pred &&
+              d_obs <= 1.05 * d_pred && g_obs <= 1.05 * g_pred;
+    if (t.force_reject) {
+        ok = false;
+        *why = "forced-reject";
+    } else {
+        *why = !finite ? "nonfinite" : !ok ? "acceptance-gate" : "accepted";
+    }
+    fprintf(stderr,
+            "[alpha-trial-decision] reason=%s accept=%d charged_iter=%d trajectory_iter=%d K=%d "
+            "fp_ratio_obs=%.12e fp_ratio_pred=%.12e kkt_obs=%.12e kkt_pred=%.12e p_obs=%.12e p_pred=%.12e "
+            "d_obs=%.12e d_pred=%.12e g_obs=%.12e g_pred=%.12e\n",
+            *why, ok ? 1 : 0, charged, total_, k, fp_obs, fp_pred, k_obs, k_pred, p_obs, p_pred, d_obs, d_pred, g_obs,
+            g_pred);
+    return ok;
+}
+
+bool Solver::pulse_eligible(int charged, int K)
+{   // alpha_direct_pulse_is_eligible, :906-958: the gate passes only when forced
+    Pulse &d = pulse_;
+    const Trial &h = trial_;
+    if (!d.enabled || d.started != 0 || d.gate_eval != 0 || d.phase != kPulseIdle || h.count < 2 ||
+        charged < 10000 || reason_ != TERMINATION_REASON_UNSPECIFIED || K != 200)
+        return false;
+    for (const ZeroEpoch &e : h.hist)
+        if (!e.valid || e.iterations <= 0 || !std::isfinite(e.fp_rate)) return false;
+    if (P_.termination_criteria.iteration_limit - charged < 400) return false;
+    d.gate_eval = 1;
+    d.gate_anchor = charged;
+    d.gate_prev_k = h.hist[0].iterations;
+    d.gate_recent_k = h.hist[1].iterations;
+    d.gate_dfp = h.hist[1].fp_rate - h.hist[0].fp_rate;
+    d.gate_pass = d.force ? 1 : 0;
+    d.gate_rej = d.force ? 0 : 1;
+    fprintf(stderr,
+            "[alpha-deterministic-tail-anchor] charged_iter=%d trajectory_iter=%d previous_K=%d recent_K=%d "
+            "previous_fp_rate=%.12e recent_fp_rate=%.12e delta_fp_rate=%.12e pass=%d force_pulse=%d\n",
+            charged, total_, d.gate_prev_k, d.gate_recent_k, h.hist[0].fp_rate, h.hist[1].fp_rate, d.gate_dfp,
+            d.gate_pass, d.force ? 1 : 0);
+    return d.force;
+}
+
+void Solver::pulse_start(int charged, int K)
+{   // alpha_direct_pulse_start, :960-982: one evaluation block at alpha = 0.002
+    Pulse &d = pulse_;
+    d.started = 1;
+    d.phase = kPulsePositive;
+    d.pulse_iters = K;
+    sigma_mode_ = false;
+    al_lambda_ = 0.002;
+    refresh_step_schedule();
+    fprintf(stderr,
+            "[alpha-direct-pulse-start] charged_iter=%d trajectory_iter=%d alpha=2.000000000000e-03 "
+            "pulse_iterations=%d\n",
+            charged, total_, d.pulse_iters);
+}
+
+void Solver::pulse_finish(int charged)
+{   // alpha_direct_pulse_finish_positive, :984-1025
+    Pulse &d = pulse_;
+    if (!d.enabled || d.phase != kPulsePositive) return;
+    d.pos_p = ctl_metric(rel_p_);
+    d.pos_d = ctl_metric(rel_d_);
+    d.pos_g = ctl_metric(rel_gap_);
+    d.pos_reason = reason_;
+    reason_ = TERMINATION_REASON_UNSPECIFIED;
+    sigma_mode_ = false;
+    al_lambda_ = 0.0;
+    refresh_step_schedule();
+    refresh_fp();
+    d.pos_fp = ctl_metric(fp_);
+    choose_restart_candidate();
+    reason_ = d.pos_reason;
+    d.completed = 1;
+    d.aborted = d.pos_reason != TERMINATION_REASON_UNSPECIFIED && d.pos_reason != TERMINATION_REASON_OPTIMAL;
+    d.phase = kPulseDone;
+    fprintf(stderr,
+            "[alpha-direct-pulse-complete] charged_iter=%d trajectory_iter=%d positive_iters=%d fp0map=%.12e "
+            "primal=%.12e dual=%.12e gap=%.12e term=%d\n",
+            charged, total_, d.pulse_iters, d.pos_fp, d.pos_p, d.pos_d, d.pos_g, (int)d.pos_reason);
 }
 
 int print_frequency(int iter)
@@ -992,15 +1494,44 @@ cp_al_result_t *Solver::solve_once(const lp_problem_t *original)
 {   // optimize main loop, solver.cu:1197-1736
     reset_iterates();
     display();
+    trial_ = Trial{};
+    trial_.enabled = env_int("CP_AL_ALPHA_ROLLBACK_TRIAL", 0) != 0;  // :587-603
+    trial_.trace = env_int("CP_AL_ALPHA_ROLLBACK_TRACE", 0) != 0;
+    trial_.force_reject = env_int("CP_AL_ALPHA_ROLLBACK_FORCE_REJECT", 0) != 0;
+    pulse_ = Pulse{};
+    pulse_.enabled = env_int("CP_AL_ALPHA_DIRECT_PULSE", 0) != 0;  // :884-904
+    pulse_.trace = env_int("CP_AL_ALPHA_DIRECT_PULSE_TRACE", 0) != 0;
+    pulse_.force = env_int("CP_AL_DETERMINISTIC_PULSE_FORCE", 0) != 0;
+    if (trial_.enabled && pulse_.enabled)  // :1216-1222 (an error instead of exit)
+        throw HipError("rollback trial and direct pulse cannot be enabled together");
+    if (trial_.enabled)
+        fprintf(stderr,
+                "[alpha-trial-config] enabled=1 alpha=2.000000000000e-03 history=2 fp_gain=0.10 kkt_gain=0.05 "
+                "component_slack=0.05 force_reject=%d\n",
+                trial_.force_reject ? 1 : 0);
+    if (pulse_.enabled)
+        fprintf(stderr,
+                "[alpha-direct-pulse-config] enabled=1 alpha=2.000000000000e-03 pulse_iterations=200 "
+                "history_epochs=2 deterministic_anchor_iter_ge=10000 wall_time_features=0 tail_only=1 "
+                "force_pulse=%d\n",
+                pulse_.force ? 1 : 0);
+    const int trial_test_limit = env_int("CP_AL_ALPHA_ROLLBACK_TEST_TRAJECTORY_LIMIT", 0);
+    const int pulse_test_limit = env_int("CP_AL_ALPHA_DIRECT_PULSE_TEST_TRAJECTORY_LIMIT", 0);
     const auto t_start = std::chrono::steady_clock::now();
     int charged = total_;
     bool do_restart = false;
     const int K = P_.termination_evaluation_frequency;
     const termination_criteria_t &tc = P_.termination_criteria;
     bool have_eval = false;
+    auto positive_charge = [&]() {
+        const int pos = charged - trial_.t_charged;
+        trial_.positive_charged += pos;
+        trial_.positive_epochs += pos > 0 ? 1 : 0;
+        return pos;
+    };
     while (charged < tc.iteration_limit) {
-        if (do_restart) {  // :1279-1284, same vectors as the last evaluation
-            fixed_point_error(last_eval_);
+        if (do_restart) {  // :1279-1284
+            refresh_fp();
             fp0_ = fp_;
             do_restart = false;
         }
@@ -1008,8 +1539,10 @@ cp_al_result_t *Solver::solve_once(const lp_problem_t *original)
         launch_block(K);
         const Eval e = read_eval();
         last_eval_ = e;
+        fp_stale_ = false;
         have_eval = true;
         fixed_point_error(e);
+        choose_restart_candidate();  // :1351
         residual(e);
         inner_ += K;
         total_ += K;
@@ -1025,11 +1558,91 @@ cp_al_result_t *Solver::solve_once(const lp_problem_t *original)
             reason_ = TERMINATION_REASON_TIME_LIMIT;
         if (reason_ == TERMINATION_REASON_UNSPECIFIED && charged >= tc.iteration_limit)
             reason_ = TERMINATION_REASON_ITERATION_LIMIT;
-        if (reason_ != TERMINATION_REASON_UNSPECIFIED) break;
+        // the controllers' test limits on the trajectory count (:1378-1391)
+        if (reason_ == TERMINATION_REASON_UNSPECIFIED && trial_.enabled && trial_test_limit > 0 && !trial_.active &&
+            trial_.rejected > 0 && total_ >= trial_test_limit)
+            reason_ = TERMINATION_REASON_ITERATION_LIMIT;
+        if (reason_ == TERMINATION_REASON_UNSPECIFIED && pulse_.enabled && pulse_test_limit > 0 &&
+            pulse_.phase == kPulseDone && total_ >= pulse_test_limit)
+            reason_ = TERMINATION_REASON_ITERATION_LIMIT;
+        if (pulse_.phase == kPulsePositive) {  // :1416-1425
+            pulse_finish(charged);
+            if (reason_ == TERMINATION_REASON_UNSPECIFIED && pulse_test_limit > 0 && total_ >= pulse_test_limit)
+                reason_ = TERMINATION_REASON_ITERATION_LIMIT;
+        }
+        if (reason_ != TERMINATION_REASON_UNSPECIFIED) {  // :1426-1474
+            if (trial_.active) {
+                positive_charge();
+                if (reason_ == TERMINATION_REASON_OPTIMAL) {
+                    trial_.accepted += 1;
+                    trial_.active = false;
+                    al_lambda_ = 0.0;
+                    refresh_step_schedule();
+                    fprintf(stderr,
+                            "[alpha-trial-decision] reason=terminal-optimal accept=1 charged_iter=%d "
+                            "trajectory_iter=%d K=%d\n",
+                            charged, total_, total_ - trial_.t_total);
+                } else {
+                    const termination_reason_t why = reason_;
+                    const double d = trial_restore();
+                    reason_ = why;
+                    trial_.rejected += 1;
+                    trial_.active = false;
+                    fprintf(stderr,
+                            "[alpha-trial-restore] reason=charged-termination charged_iter=%d trajectory_iter=%d "
+                            "max_vector_l2=%.12e\n",
+                            charged, total_, d);
+                }
+                trial_.snap_valid = false;
+            }
+            break;
+        }
+        if (trial_.active && (charged - trial_.t_charged >= trial_.max_iters ||
+                              solve_time_ - trial_.t_time >= trial_.max_sec)) {  // safety cap, :1476-1494
+            const int pos = positive_charge();
+            const double d = trial_restore();
+            trial_.rejected += 1;
+            trial_.active = false;
+            set_epoch_start();
+            fprintf(stderr,
+                    "[alpha-trial-restore] reason=safety-cap charged_iter=%d trajectory_iter=%d positive_iters=%d "
+                    "max_vector_l2=%.12e\n",
+                    charged, total_, pos, d);
+            trial_.snap_valid = false;
+            do_restart = true;
+            continue;
+        }
         do_restart = should_restart(K);
-        if (do_restart) {
+        if (do_restart) {  // :1631-1695
             if (P_.verbose) printf("restart iter %d fp %.12e\n", total_, fp_);
-            perform_restart(e);
+            if (trial_.active) {
+                const int pos = positive_charge();
+                const char *why = nullptr;
+                if (!trial_accepts(charged, &why)) {
+                    const double d = trial_restore();
+                    trial_.rejected += 1;
+                    trial_.active = false;
+                    set_epoch_start();
+                    fprintf(stderr,
+                            "[alpha-trial-restore] reason=%s charged_iter=%d trajectory_iter=%d positive_iters=%d "
+                            "max_vector_l2=%.12e\n",
+                            why, charged, total_, pos, d);
+                    trial_.snap_valid = false;
+                    do_restart = true;
+                    continue;
+                }
+                trial_.accepted += 1;
+                trial_.active = false;
+                al_lambda_ = 0.0;
+                refresh_step_schedule();
+                trial_.snap_valid = false;
+            } else {
+                record_zero_epoch();
+            }
+            perform_restart(last_eval_);
+            set_epoch_start();
+            if (trial_eligible(charged)) trial_start(charged);
+            if (pulse_eligible(charged, K)) pulse_start(charged, K);
         }
     }
     if (reason_ == TERMINATION_REASON_UNSPECIFIED) {  // :1704-1709
@@ -1041,6 +1654,21 @@ cp_al_result_t *Solver::solve_once(const lp_problem_t *original)
         residual(last_eval_);
         display();
     }
+    if (trial_.enabled)  // alpha_trial_log_final, :863-882
+        fprintf(stderr,
+                "[alpha-trial-final] charged_iter=%d trajectory_iter=%d started=%d accepted=%d rejected=%d "
+                "positive_epochs=%d positive_charged_iters=%d final_alpha=%.12e final_penalty=%.12e\n",
+                charged, total_, trial_.started, trial_.accepted, trial_.rejected, trial_.positive_epochs,
+                trial_.positive_charged, al_lambda_, al_penalty_);
+    if (pulse_.enabled)  // alpha_direct_pulse_log_final, :1027-1057
+        fprintf(stderr,
+                "[alpha-direct-pulse-final] charged_iter=%d trajectory_iter=%d started=%d completed=%d aborted=%d "
+                "gate_evaluated=%d gate_passed=%d gate_rejected=%d gate_anchor_charged_iter=%d gate_previous_K=%d "
+                "gate_recent_K=%d gate_delta_fp_rate=%.12e positive_charged_iters=%d final_alpha=%.12e "
+                "final_penalty=%.12e\n",
+                charged, total_, pulse_.started, pulse_.completed, pulse_.aborted, pulse_.gate_eval,
+                pulse_.gate_pass, pulse_.gate_rej, pulse_.gate_anchor, pulse_.gate_prev_k, pulse_.gate_recent_k,
+                pulse_.gate_dfp, pulse_.completed ? pulse_.pulse_iters : 0, al_lambda_, al_penalty_);
     polish_time_ = 0.0;
     polish_iters_ = 0;
     if (P_.feasibility_polishing && reason_ != TERMINATION_REASON_DUAL_INFEASIBLE &&

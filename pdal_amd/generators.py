@@ -330,6 +330,69 @@ def planted_family(m: int, n: int, nnz_per_col: int, seed: int, p: int = 32, sca
     return PlantedFamily(base=base, W=W, G=G, H=H, U=np.ascontiguousarray(U), V=np.ascontiguousarray(V))
 
 
+@dataclass
+class ActiveSetFamily:
+    """Fixed-A LP family whose optimal active set moves with the parameters.
+
+      z(theta) = (x* - s*) + W theta_b
+      x(theta) = max(z, 0),  s(theta) = max(-z, 0),  y(theta) = y* + G theta_c
+      b(theta) = A x(theta),  c(theta) = A^T y(theta) + s(theta)
+
+    x and s are nonnegative with disjoint supports for every theta, A x = b and
+    c - A^T y = s: (x, y, s)(theta) is a KKT point, so x(theta) is optimal and
+    c^T x(theta) the optimal value, with no region to leave.  An entry of z
+    that crosses zero moves its column between the primal support and the
+    dual slacks: b and c are piecewise linear in theta, the solution map has a
+    kink per column, and an affine predictor of the solution cannot follow it
+    (a ReLU network can).  W (n x p), G (m x p) have N(0, scale^2 / p)
+    entries; the supports of x* and s* have U(0, 1) entries, so about a tenth
+    of the columns change sides at scale 0.3.  When the support of x(theta)
+    is larger than m the optimal x need not be unique; the optimal value is.
+    """
+    base: PlantedLP
+    W: np.ndarray   # n x p
+    G: np.ndarray   # m x p
+
+    @property
+    def m(self) -> int:
+        return self.base.m
+
+    @property
+    def n(self) -> int:
+        return self.base.n
+
+    @property
+    def p(self) -> int:
+        return self.W.shape[1]
+
+    def solution(self, theta):
+        """The planted (x, y, s) of parameters theta."""
+        theta = np.asarray(theta, dtype=np.float64)
+        z = self.base.x_star - self.base.s_star + self.W @ theta[: self.p]
+        return np.maximum(z, 0.0), self.base.y_star + self.G @ theta[self.p:], np.maximum(-z, 0.0)
+
+    def optimal_value(self, theta) -> float:
+        x, y, _ = self.solution(theta)
+        return float(self.instance(theta)["c"] @ x)
+
+    def instance(self, theta) -> dict:
+        """Keyword arguments of CpAl.make_problem / LpBatch for parameters theta."""
+        x, y, s = self.solution(theta)
+        A = self.base.A
+        b = A @ x
+        c = A.T @ y + s
+        n = self.n
+        return dict(A=A, c=c, con_lb=b, con_ub=b, var_lb=np.zeros(n), var_ub=np.full(n, np.inf))
+
+
+def active_set_family(m: int, n: int, nnz_per_col: int, seed: int, p: int = 4,
+                      scale: float = 0.3) -> ActiveSetFamily:
+    base = planted_lp(m, n, nnz_per_col, seed)
+    rng = np.random.default_rng(seed + 3_000_003)
+    sd = scale / np.sqrt(p)
+    return ActiveSetFamily(base=base, W=sd * rng.standard_normal((n, p)), G=sd * rng.standard_normal((m, p)))
+
+
 def config4_family(seed: int = 4, scale: float = 1.0) -> PlantedFamily:
     """m=5e4, n=1e5, 8 nonzeros per column; b = b0 + U theta_b, c = c0 + V theta_c
     with dense U (m x 32), V (n x 32) around a planted optimum, 0.1 relative."""

@@ -263,6 +263,10 @@ class CpAl:
             L.pdal_cp_al_batch_start_buffers.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]
             L.pdal_cp_al_batch_set_starts_from_buffers.restype = C.c_int
             L.pdal_cp_al_batch_set_starts_from_buffers.argtypes = [C.c_void_p]
+            if hasattr(L, "pdal_cp_al_batch_set_vectors"):
+                for name in ("pdal_cp_al_batch_set_vectors", "pdal_cp_al_batch_set_vectors_device"):
+                    getattr(L, name).restype = C.c_int
+                    getattr(L, name).argtypes = [C.c_void_p] + [C.c_void_p] * 5
 
     # ------------------------------------------------------------------
     def default_params(self, **overrides) -> CpAlParameters:
@@ -500,6 +504,57 @@ class LpBatch:
     def set_starts_from_buffers(self) -> None:
         if self.lib.lib.pdal_cp_al_batch_set_starts_from_buffers(self.h) != 0:
             raise RuntimeError("pdal_cp_al_batch_set_starts_from_buffers failed")
+
+    def set_vectors(self, c=None, con_lb=None, con_ub=None, var_lb=None, var_ub=None) -> None:
+        """New objective / bound vectors for the same matrices, each the
+        members' vectors concatenated (c, var_lb, var_ub: sum n_i; con_lb,
+        con_ub: sum m_i); None keeps the current one.  The solves that follow
+        equal those of a batch created with these vectors; starts become zero
+        and primal weights their defaults."""
+        n, m = sum(ni for _, ni in self.sizes), sum(mi for mi, _ in self.sizes)
+        arrs = []
+        for name, v, size in (("c", c, n), ("con_lb", con_lb, m), ("con_ub", con_ub, m), ("var_lb", var_lb, n),
+                              ("var_ub", var_ub, n)):
+            a = _f64(v)
+            if a is not None and a.size != size:
+                raise ValueError(f"{name} has {a.size} entries, the batch needs {size}")
+            arrs.append(a)
+        rc = self.lib.lib.pdal_cp_al_batch_set_vectors(self.h, *[None if a is None else a.ctypes.data for a in arrs])
+        if rc != 0:
+            raise RuntimeError(f"pdal_cp_al_batch_set_vectors failed ({rc})")
+
+    def set_vectors_device(self, c=None, con_lb=None, con_ub=None, var_lb=None, var_ub=None) -> None:
+        """As set_vectors from device memory: each argument a device address
+        (int) or a devbuf.DeviceArray of the right size; None keeps."""
+        n, m = sum(ni for _, ni in self.sizes), sum(mi for mi, _ in self.sizes)
+        ptrs = []
+        for name, v, size in (("c", c, n), ("con_lb", con_lb, m), ("con_ub", con_ub, m), ("var_lb", var_lb, n),
+                              ("var_ub", var_ub, n)):
+            if v is None or isinstance(v, int):
+                ptrs.append(v)
+                continue
+            if v.size != size:
+                raise ValueError(f"{name} has {v.size} entries, the batch needs {size}")
+            ptrs.append(v.ptr)
+        rc = self.lib.lib.pdal_cp_al_batch_set_vectors_device(self.h, *ptrs)
+        if rc != 0:
+            raise RuntimeError(f"pdal_cp_al_batch_set_vectors_device failed ({rc})")
+
+    def set_instances(self, instances: list) -> None:
+        """set_vectors from keyword dicts (as for the constructor) whose
+        matrices are the batch's: only their vectors are read."""
+        if len(instances) != len(self.sizes):
+            raise ValueError(f"{len(instances)} instances for a batch of {len(self.sizes)}")
+        cat = {}
+        defaults = {"c": 0.0, "con_lb": -np.inf, "con_ub": np.inf, "var_lb": -np.inf, "var_ub": np.inf}
+        for key, default in defaults.items():  # create_lp_problem's values for an absent vector
+            parts = []
+            for (mi, ni), kw in zip(self.sizes, instances):
+                size = ni if key in ("c", "var_lb", "var_ub") else mi
+                v = kw.get(key)
+                parts.append(np.full(size, default) if v is None else np.asarray(v, dtype=np.float64).ravel())
+            cat[key] = np.concatenate(parts) if parts else np.zeros(0)
+        self.set_vectors(**cat)
 
     def solve(self) -> int:
         n_opt = self.lib.lib.pdal_cp_al_batch_solve(self.h)

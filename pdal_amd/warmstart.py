@@ -130,8 +130,17 @@ class Mlp:
         if p.size != self.nparam or self.lib.lib.pdal_mlp_set_params(self.h, p.ctypes.data_as(_fp)) != 0:
             raise RuntimeError("pdal_mlp_set_params failed")
 
+    def _check(self, X, Y=None) -> None:
+        """The library reads rows of widths[0] inputs and widths[-1] targets:
+        other shapes would be read past their ends."""
+        if X.ndim != 2 or X.shape[1] != self.widths[0]:
+            raise ValueError(f"inputs of shape {X.shape}: the network takes rows of {self.widths[0]}")
+        if Y is not None and (Y.ndim != 2 or Y.shape != (X.shape[0], self.widths[-1])):
+            raise ValueError(f"targets of shape {Y.shape}: ({X.shape[0]}, {self.widths[-1]}) expected")
+
     def forward(self, X) -> np.ndarray:
         X = _f32(X)
+        self._check(X)
         out = np.empty((X.shape[0], self.widths[-1]), dtype=np.float32)
         if self.lib.lib.pdal_mlp_forward(self.h, X.ctypes.data_as(_fp), X.shape[0], out.ctypes.data_as(_fp)) != 0:
             raise RuntimeError("pdal_mlp_forward failed")
@@ -139,6 +148,7 @@ class Mlp:
 
     def loss_grad(self, X, Y):
         X, Y = _f32(X), _f32(Y)
+        self._check(X, Y)
         loss = C.c_float()
         g = np.empty(self.nparam, dtype=np.float32)
         rc = self.lib.lib.pdal_mlp_loss_grad(self.h, X.ctypes.data_as(_fp), Y.ctypes.data_as(_fp), X.shape[0],
@@ -149,6 +159,7 @@ class Mlp:
 
     def train(self, X, Y, batch: int = 256, epochs: int = 100, lr: float = 1e-3, seed: int = 0) -> np.ndarray:
         X, Y = _f32(X), _f32(Y)
+        self._check(X, Y)
         hist = np.zeros(epochs, dtype=np.float32)
         rc = self.lib.lib.pdal_mlp_train(self.h, X.ctypes.data_as(_fp), Y.ctypes.data_as(_fp), X.shape[0], batch,
                                          epochs, C.c_float(lr), C.c_ulonglong(seed), hist.ctypes.data_as(_fp))

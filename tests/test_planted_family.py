@@ -102,3 +102,52 @@ def test_affine_head_recovers_affine_solutions():
     np.testing.assert_allclose(xm, np.broadcast_to(X.mean(0), xm.shape), rtol=1e-6)
     with pytest.raises(ValueError):
         plain.predict(te, mlp=False)
+
+
+# ---------------------------------------------------------------------------
+# The family whose active set moves with theta (generators.active_set_family)
+
+@pytest.fixture(scope="module")
+def afam():
+    from pdal_amd.generators import active_set_family
+    return active_set_family(200, 400, 8, seed=9)
+
+
+def test_active_set_point_is_kkt_for_every_theta(afam):
+    rng = np.random.default_rng(3)
+    for theta in rng.standard_normal((5, 2 * afam.p)):
+        d = afam.instance(theta)
+        x, y, s = afam.solution(theta)
+        A, b, c = d["A"], d["con_lb"], d["c"]
+        assert np.linalg.norm(A @ x - b) <= 1e-12 * (1 + np.linalg.norm(b))
+        assert np.all(x >= 0) and np.all(s >= 0)
+        np.testing.assert_allclose(c - A.T @ y, s, atol=1e-12 * (1 + np.abs(c).max()))
+        assert np.all((x == 0) | (s == 0))
+
+
+@pytest.mark.parametrize("k", [0, 1])
+def test_active_set_value_is_highs_optimum(afam, k):
+    theta = np.random.default_rng(4).standard_normal((2, 2 * afam.p))[k]
+    d = afam.instance(theta)
+    r = linprog(d["c"], A_eq=d["A"], b_eq=d["con_lb"], bounds=(0, None), method="highs")
+    assert r.status == 0
+    opt = afam.optimal_value(theta)
+    assert abs(r.fun - opt) <= 1e-9 * (1 + abs(opt))
+
+
+def test_active_set_moves_and_data_is_not_affine(afam):
+    """About a tenth of the columns change sides, and a least-squares affine
+    fit of x(theta) over many parameters leaves a large residual, where the
+    affine family's would be zero."""
+    th = np.random.default_rng(5).standard_normal((400, 2 * afam.p))
+    X = np.stack([afam.solution(t)[0] for t in th])
+    moved = np.mean((X > 0) != (afam.base.x_star > 0))
+    assert 0.05 < moved < 0.2
+    D = np.hstack([th, np.ones((len(th), 1))])
+    coef = np.linalg.lstsq(D, X, rcond=None)[0]
+    rel = np.linalg.norm(X - D @ coef) / np.linalg.norm(X - X.mean(0))
+    assert rel > 0.2
+    # the supports of x and s change, the dual y does not leave its affine form
+    Y = np.stack([afam.solution(t)[1] for t in th])
+    cy = np.linalg.lstsq(D, Y, rcond=None)[0]
+    assert np.linalg.norm(Y - D @ cy) <= 1e-10 * np.linalg.norm(Y)

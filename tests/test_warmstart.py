@@ -285,6 +285,65 @@ def test_learned_warm_start_on_planted_family(ws, monkeypatch):
 
 
 @pytest.mark.gpu
+def test_network_beats_affine_head_on_moving_active_set(ws, monkeypatch):
+    """On a family whose optimal active set moves with theta
+    (generators.active_set_family: x = max(0, z(theta)), a kink per column)
+    the affine head cannot follow the solution map.  The network on top of it
+    must cut both the held-out start error against the planted optimum and the
+    held-out iteration count; every warm answer meets the planted optimum."""
+    monkeypatch.setenv("CP_AL_LAMBDA", "0")
+    monkeypatch.setenv("PDAL_BATCH_G_A", "16")
+    monkeypatch.setenv("PDAL_BATCH_G_AT", "8")
+    from pdal_amd.generators import active_set_family
+    from pdal_amd.lp import LpBatch, load_cp_al
+    from pdal_amd.warmstart import LearnedWarmStart
+    lib = load_cp_al()
+    fam = active_set_family(1000, 2000, 8, seed=4)
+    tr = np.random.default_rng(1).standard_normal((512, 2 * fam.p))
+    te = np.random.default_rng(2).standard_normal((64, 2 * fam.p))
+    p_train = lib.default_params(termination_evaluation_frequency=64, eps_optimal_relative=1e-8,
+                                 eps_feasible_relative=1e-8, iteration_limit=60000)
+    params = lib.default_params(termination_evaluation_frequency=64, eps_optimal_relative=1e-6,
+                                eps_feasible_relative=1e-6, iteration_limit=200000)
+    print("[case] active-set family m=1000: training solves", flush=True)
+    with LpBatch(lib, [fam.instance(t) for t in tr], p_train) as b:
+        b.solve()
+        sols = [b.result(k) for k in range(len(tr))]
+        pw = b.primal_weights()
+    X, Y = np.stack([s.x for s in sols]), np.stack([s.y for s in sols])
+    test_inst = [fam.instance(t) for t in te]
+    lws = LearnedWarmStart(ws, widths=(2 * fam.p, 512, 512, 512, 256), kx=128, seed=3)
+    try:
+        lws.fit(tr, X, Y, pw=pw, lb=np.zeros(fam.n), epochs=300)
+        starts = {"head": lws.predict(te, mlp=False), "head+network": lws.predict(te)}
+    finally:
+        lws.close()
+    Xp = np.stack([fam.solution(t)[0] for t in te])
+    err = {k: float(np.linalg.norm(v[0] - Xp) / np.linalg.norm(Xp)) for k, v in starts.items()}
+    print(f"\nx error vs planted {err}; predicted primal weights "
+          f"{ {k: (float(np.min(v[2])), float(np.max(v[2]))) for k, v in starts.items()} }", flush=True)
+    its = {}
+    with LpBatch(lib, test_inst, params) as b:
+        for k, (xh, yh, pwh) in starts.items():
+            assert np.all(np.isfinite(xh)) and np.all(np.isfinite(yh)) and np.all(np.isfinite(pwh)), k
+            b.set_starts(np.ascontiguousarray(xh).ravel(), np.ascontiguousarray(yh).ravel())
+            b.set_primal_weights(pwh)
+            n_opt = b.solve()
+            res = [b.result(i) for i in range(len(te))]
+            assert n_opt == len(te), (k, sorted({r.status for r in res}), [r.iterations for r in res][:8])
+            for r, t in zip(res, te):  # the relative gap test at 1e-6 bounds |p - d| by 1e-6 (1 + |p| + |d|)
+                opt = fam.optimal_value(t)
+                assert abs(r.primal_objective - opt) <= 2.5e-6 * (1 + abs(opt))
+            its[k] = sum(r.iterations for r in res)
+        b.set_vectors()  # the same vectors: zero starts, default primal weights
+        b.solve()
+        its["cold"] = sum(b.result(i).iterations for i in range(len(te)))
+    print(f"\nx error vs planted {err}; iterations {its}")
+    assert err["head+network"] < 0.5 * err["head"]
+    assert its["head+network"] < its["head"] < its["cold"]
+
+
+@pytest.mark.gpu
 def test_batch_primal_weights_roundtrip_and_exact_restart(monkeypatch):
     """set_primal_weights is read back before a solve; a batch restarted from
     its own solutions with the primal weights it ended with converges within a
@@ -310,3 +369,18 @@ def test_batch_primal_weights_roundtrip_and_exact_restart(monkeypatch):
     assert sum(w.iterations for w in warm) < 0.2 * sum(c.iterations for c in cold)
     for c, w in zip(cold, warm):  # two points that pass the 1e-6 KKT test from different starts
         assert abs(w.primal_objective - c.primal_objective) <= OBJ_AGREEMENT * (1 + abs(c.primal_objective))
+
+
+def test_network_rejects_rows_of_the_wrong_width():
+    """Inputs or targets whose rows are not the network's widths would be read
+    past their ends by the library; the wrapper refuses them first."""
+    from pdal_amd.warmstart import Mlp
+    net = object.__new__(Mlp)
+    net.widths = [8, 16, 5]
+    net._check(np.zeros((3, 8), dtype=np.float32), np.zeros((3, 5), dtype=np.float32))
+    with pytest.raises(ValueError):
+        net._check(np.zeros((3, 64), dtype=np.float32))
+    with pytest.raises(ValueError):
+        net._check(np.zeros((3, 8), dtype=np.float32), np.zeros((3, 4), dtype=np.float32))
+    with pytest.raises(ValueError):
+        net._check(np.zeros(8, dtype=np.float32))
