@@ -194,7 +194,9 @@ class LearnedWarmStart:
     targets are the head's residuals divided by one scale per block (x modes,
     y modes, primal weight): the POD modes are orthonormal, so its loss is the
     solution-space error of each block, and modes that only hold the training
-    solves' own KKT noise weigh as little as they are worth.  `head=False`
+    solves' own KKT noise weigh as little as they are worth; a block the head
+    fits to the training solves' own accuracy (residual RMS below 1e-6 of the
+    block's spread) gets no network output at all.  `head=False`
     gives the network alone, on per-mode standardized coefficients.  When the
     training solves' final primal weights are given, the last output predicts
     their logarithm: a PDHG start near the solution pays off only together with
@@ -256,15 +258,22 @@ class LearnedWarmStart:
             blocks = [slice(0, self.kx), slice(self.kx, out - 1 if self.has_pw else out)]
             if self.has_pw:
                 blocks.append(slice(out - 1, out))
-            for blk in blocks:
+            self.exact_blocks = []
+            for name, blk in zip(("x", "y", "pw"), blocks):
                 scale[blk] = np.sqrt(np.mean(resid[:, blk] ** 2)) + 1e-300
+                spread = np.sqrt(np.mean((coef[:, blk] - coef[:, blk].mean(axis=0)) ** 2))
+                if scale[blk][0] <= 1e-6 * spread:
+                    # the head fits this block to the training solves' own accuracy: what is left
+                    # is their KKT noise, which the network would only learn to reproduce
+                    scale[blk] = 0.0
+                    self.exact_blocks.append(name)
             self.csd = scale
         else:
             self.Wlin = None
             resid = coef
             self.cmu = coef.mean(axis=0)
             self.csd = coef.std(axis=0) + 1e-12
-        target = (resid - self.cmu) / self.csd
+        target = np.divide(resid - self.cmu, self.csd, out=np.zeros_like(resid), where=self.csd > 0)
         self.lb = None if lb is None else np.asarray(lb, dtype=np.float64)
         self.ub = None if ub is None else np.asarray(ub, dtype=np.float64)
         if self.mlp is not None:
@@ -282,6 +291,7 @@ class LearnedWarmStart:
                 "pod_rel_err_x": float(np.linalg.norm(recon_x - X) / np.linalg.norm(X)), "epochs": epochs}
         if self.head:
             info["head_rel_resid"] = float(np.linalg.norm(resid) / max(np.linalg.norm(coef), 1e-300))
+            info["head_exact_blocks"] = list(self.exact_blocks)
         return info
 
     def predict_coef(self, thetas, mlp: bool = True) -> np.ndarray:
