@@ -504,8 +504,11 @@ def run_config1(args, lib) -> dict:
     sess = LpSession(lib, **inst.lp_args(), params=params)
     kt = sess.time_kernels(args.kernel_iters)
     bp = fused_primal_bytes(nnz, n, m)
-    result = {"roofline": roofline(bp, kt["primal_ms"], "primal_kernel (At.y SpMV + fused primal update)"),
-              "setup_s": round(sess.setup_seconds, 3)}
+    label = ("primal_sliced_kernel (column-sliced At.y + fused primal update)" if kt["lanes_at"] < 0
+             else f"primal_kernel<{kt['lanes_at']}> (At.y SpMV + fused primal update)")
+    traffic = measured_traffic("primal_sliced_kernel", "cp_al_config1_fused_kernels.md") if kt["lanes_at"] < 0 \
+        else None
+    result = {"roofline": roofline(bp, kt["primal_ms"], label, traffic), "setup_s": round(sess.setup_seconds, 3)}
     if args.kernels_only:
         sess.close()
         return result
@@ -569,7 +572,7 @@ def run_config2_leg(args) -> dict:
     for form in ("explicit", "lowrank"):
         qargs = inst.qp_args(form)
         p = qlib.default_params(eps_optimal_relative=args.tol, eps_feasible_relative=args.tol, verbose=0,
-                                time_sec_limit=args.qp_time, iteration_limit=10_000_000)
+                                time_sec_limit=args.qp_time, iteration_limit=args.qp_iters)
         qlib.reset_rng()
         t0 = time.perf_counter()
         r = qlib.solve(**qargs, params=p)
@@ -616,6 +619,7 @@ def main() -> int:
     ap.add_argument("--kernels-only", action="store_true", help="set up and time kernels only (profiling)")
     ap.add_argument("--no-qp", action="store_true", help="config4: skip the FA_CP config-2 leg")
     ap.add_argument("--qp-time", type=float, default=25.0, help="config-2 leg: time cap per FA_CP solve (s)")
+    ap.add_argument("--qp-iters", type=int, default=10_000_000, help="config-2 leg: iteration cap per FA_CP solve")
     ap.add_argument("--kernel-iters", type=int, default=20)
     args = ap.parse_args()
 

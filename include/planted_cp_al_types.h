@@ -6,7 +6,7 @@
  * field order and types are kept so a caller compiled against that header
  * links against librhr_cp_al.so unchanged.  The types are those of the
  * linearized solver (lin_cp_al_types.h) with one more parameter,
- * trajectory_csv (accepted; the file is not written by this build).  The
+ * trajectory_csv (the per-checkpoint trajectory file).  The
  * problem is
  *   min 1/2 x^T (Q + R^T R) x + c^T x + c0  s.t.  A x = b, lb <= x <= ub
  * (b is the constraint lower bound; the upper bound is not read).  The types

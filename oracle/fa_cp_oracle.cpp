@@ -21,8 +21,8 @@
  *   problem construction, conversions ... src/fa_cp.c:28-376, src/utils.cu:1135-1358
  *   feasibility polishing ............... src/fa_cp_core_op.cu:1242-1735, src/utils.cu:1360-1575
  *
- * Deliberate differences shared with the HIP path: wall-clock timing, no
- * trajectory CSV, restart-select env knobs ignored, errors return NULL
+ * Deliberate differences shared with the HIP path: wall-clock timing (also
+ * in the trajectory CSV's time column, solver.cu:35-71), errors return NULL
  * instead of exit().
  *
  * Built a second time with -DFA_CP_ORACLE_REVERSE_SUMS, every SpMV row, dot
@@ -239,6 +239,9 @@ struct S {
     double al_sigma = 0.0, best_al_sigma = 0.0, best_sigma_score = INFINITY, last_sigma_score = INFINITY;
     double last_sigma_rp = INFINITY, last_sigma_rd = INFINITY, last_sigma_gap = INFINITY;
     int sigma_update_count = 0, sigma_dir = 1;
+    // FA_CP restart candidate (choose_restart_candidate, fa_cp_core_op.cu:1130-1169):
+    // 0 = the T output, 1 = the current (Halpern) iterate
+    int restart_target = 0;
 
     double step = 0.0, pw = 1.0, best_pw = 1.0, pw_err_sum = 0.0, pw_last_err = 0.0, best_gap = INFINITY;
     // linearized CP-AL: the fixed penalty sigma and its interval, lambda of tau = 1 / (sigma lambda)
@@ -859,10 +862,60 @@ void update_al_sigma(S &s, double pd, double dd, double normal_motion)
     s.sigma_update_count++;
 }
 
+const char *restart_env(const char *name, const char *fallback)  // get_env_value, :35-42
+{
+    const char *v = getenv(name);
+    if ((!v || !v[0]) && fallback) v = getenv(fallback);
+    return v;
+}
+
+int restart_env_int(const char *name, const char *fallback, int def)  // :44-51
+{
+    const char *v = restart_env(name, fallback);
+    return (!v || !v[0]) ? def : atoi(v);
+}
+
+double restart_env_double(const char *name, double def)  // :53-66
+{
+    const char *v = restart_env(name, name);
+    if (!v || !v[0]) return def;
+    char *end = nullptr;
+    const double x = strtod(v, &end);
+    return (end == v || !std::isfinite(x)) ? def : x;
+}
+
+void choose_restart_candidate(S &s)
+{   // :1091-1169: the current iterate as restart point when its score beats the T output's
+    // fixed-point error by the margin (mode 2: always)
+    const int mode = restart_env_int("FA_CP_RESTART_SELECT", "FA_CP_RESTART_SELECT_LEGACY", 0);
+    s.restart_target = 0;
+    const double cur = std::isfinite(s.fp) ? s.fp : INFINITY;
+    if (mode == 0 || s.total == 0 || !std::isfinite(cur)) return;
+    if (s.total < restart_env_int("FA_CP_RESTART_SELECT_MIN_ITERS", "FA_CP_RESTART_SELECT_MIN_ITERS", 0) ||
+        s.solve_time < restart_env_double("FA_CP_RESTART_SELECT_MIN_TIME", 0.0))
+        return;
+    // compute_restart_delta_score(x_T vs the current iterate): the fixed-point metric without the AL term
+    for (int i = 0; i < s.n; ++i) s.dx[i] = s.x[i] - s.xT[i];
+    for (int i = 0; i < s.m; ++i) s.dy[i] = s.y[i] - s.yT[i];
+    spmv(s.At, s.dy.data(), s.dual_prod.data());
+    const double pn = nrm2(s.dx), dn = nrm2(s.dy), cross = dot(s.dual_prod, s.dx);
+    const double halpern = std::sqrt(std::fmax(pn * pn * s.pw + dn * dn / s.pw + 2.0 * s.step * cross, 0.0));
+    if (!(cur > 0.0)) return;
+    const double margin = restart_env_double("FA_CP_RESTART_SELECT_MARGIN", 1.0);
+    if (mode == 2 || (std::isfinite(halpern) && halpern < margin * cur)) {
+        s.restart_target = 1;
+        s.fp = halpern;
+    }
+}
+
 void perform_restart(S &s)
-{   // :908-1020 (restart target: the T-output)
-    for (int i = 0; i < s.n; ++i) s.dx[i] = s.xT[i] - s.x0[i];
-    for (int i = 0; i < s.m; ++i) s.dy[i] = s.yT[i] - s.y0[i];
+{   // :908-1020: to the T output, or to the current iterate when the candidate selection chose it
+    const bool to_current =
+        restart_env_int("FA_CP_RESTART_SELECT", "FA_CP_RESTART_SELECT_LEGACY", 0) != 0 && s.restart_target == 1;
+    const Vec &rx = to_current ? s.x : s.xT;
+    const Vec &ry = to_current ? s.y : s.yT;
+    for (int i = 0; i < s.n; ++i) s.dx[i] = rx[i] - s.x0[i];
+    for (int i = 0; i < s.m; ++i) s.dy[i] = ry[i] - s.y0[i];
     const double pd = nrm2(s.dx), dd = nrm2(s.dy);
     double normal = INFINITY;
     if (pd > 1e-16 && std::isfinite(pd)) {
@@ -889,10 +942,17 @@ void perform_restart(S &s)
         s.best_pw = s.pw;
     }
     update_al_sigma(s, pd, dd, normal);
-    s.x0 = s.xT;
-    s.y0 = s.yT;
-    s.x = s.xT;
-    s.y = s.yT;
+    if (to_current) {  // :985-1009: anchor and T output = the current iterate
+        s.x0 = s.x;
+        s.y0 = s.y;
+        s.xT = s.x;
+        s.yT = s.y;
+    } else {
+        s.x0 = s.xT;
+        s.y0 = s.yT;
+        s.x = s.xT;
+        s.y = s.yT;
+    }
     s.inner = 0;
     s.last_trial_fp = INFINITY;
 }
@@ -2087,6 +2147,69 @@ ORACLE_API cp_al_result_t *optimize(const cp_al_parameters_t *params, const qp_p
     return result_from_state(s, prob, polish_time, polish_iters);
 }
 #else
+/* ---------------- trajectory CSV (solver.cu:35-71; the planted variants' :35-109) ---------------- */
+
+bool trajectory_open(const S &s)
+{   // initialize_trajectory_csv: the header row; false when the file cannot be written
+    const char *path = s.P.trajectory_csv;
+    if (!path || !path[0]) return true;
+    FILE *f = fopen(path, "w");
+    if (!f) {
+        perror("Error opening trajectory CSV");
+        return false;
+    }
+#if defined(ORACLE_PLANTED_FA_CP) || defined(ORACLE_PLANTED_CP_AL)
+    fprintf(f, "iteration,time_sec,rel_primal,rel_dual,rel_gap,kkt_residual,fixed_point_residual,restart,sigma,"
+               "primal_weight,al_sigma,lambda_est,primal_step,dual_step\n");
+#else
+    fprintf(f, "iteration,time_sec,rel_primal,rel_dual,rel_gap,kkt_residual,fixed_point_residual,restart,al_sigma,"
+               "sigma_update_count\n");
+#endif
+    fclose(f);
+    return true;
+}
+
+bool trajectory_row(const S &s, bool restart)
+{   // append_trajectory_row: one row per checkpoint, before a planned restart is performed
+    const char *path = s.P.trajectory_csv;
+    if (!path || !path[0]) return true;
+    FILE *f = fopen(path, "a");
+    if (!f) {
+        perror("Error appending trajectory CSV");
+        return false;
+    }
+    const double kkt = std::fmax(s.rel_p, std::fmax(s.rel_d, s.rel_gap));
+    const double fpr = s.total == 0 ? NAN : s.fp;
+#if defined(ORACLE_PLANTED_FA_CP) || defined(ORACLE_PLANTED_CP_AL)
+    double sigma = NAN, al_sigma = NAN, lambda = NAN, pstep = NAN, dstep = NAN;  // compute_trajectory_diagnostics
+#ifdef ORACLE_PLANTED_CP_AL
+    if (s.use_al) {
+        sigma = s.sigma;
+        lambda = s.lambda_est;
+        pstep = s.step;
+        dstep = s.sigma;
+    } else if (s.pw > 0.0) {
+        pstep = s.step / s.pw;
+        dstep = s.step * s.pw;
+    }
+#else
+    al_sigma = s.use_al ? s.al_sigma : NAN;
+    if (s.pw > 0.0) {
+        pstep = s.step / s.pw;
+        dstep = s.step * s.pw;
+    }
+#endif
+    fprintf(f, "%d,%.17e,%.17e,%.17e,%.17e,%.17e,%.17e,%d,%.17e,%.17e,%.17e,%.17e,%.17e,%.17e\n", s.total,
+            s.solve_time, s.rel_p, s.rel_d, s.rel_gap, kkt, fpr, restart ? 1 : 0, sigma, s.pw, al_sigma, lambda, pstep,
+            dstep);
+#else
+    fprintf(f, "%d,%.17e,%.17e,%.17e,%.17e,%.17e,%.17e,%d,%.17e,%d\n", s.total, s.solve_time, s.rel_p, s.rel_d,
+            s.rel_gap, kkt, fpr, restart ? 1 : 0, s.al_sigma, s.sigma_update_count);
+#endif
+    fclose(f);
+    return true;
+}
+
 ORACLE_API result_t *optimize(const params_t *params, const qp_problem_t *prob)
 {   // solver.cu:74-168 (PLANTED_CP_AL solver.cu:111-204: the same loop with its own update,
     // fixed-point error, restart and step set-up)
@@ -2104,6 +2227,7 @@ ORACLE_API result_t *optimize(const params_t *params, const qp_problem_t *prob)
     s.pw = s.P.bound_objective_rescaling ? 1.0 : (s.c_norm + 1.0) / (s.b_norm + 1.0);
     s.best_pw = s.pw;
 #endif
+    if (!trajectory_open(s)) return nullptr;  // the reference exits
 
     const auto t0 = std::chrono::steady_clock::now();
     const int K = s.P.termination_evaluation_frequency;
@@ -2116,7 +2240,13 @@ ORACLE_API result_t *optimize(const params_t *params, const qp_problem_t *prob)
             if (s.major && s.total < 3 * K) infeasibility_info(s);
             s.solve_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
             check_termination(s);
-            if (consider && s.reason == TERMINATION_REASON_UNSPECIFIED) planned = should_restart(s);
+            if (consider && s.reason == TERMINATION_REASON_UNSPECIFIED) {
+#if !defined(ORACLE_CP_AL_TYPES) && !defined(ORACLE_PLANTED_FA_CP)
+                choose_restart_candidate(s);  // solver.cu:112
+#endif
+                planned = should_restart(s);
+            }
+            if (!trajectory_row(s, planned)) return nullptr;
             if (s.reason != TERMINATION_REASON_UNSPECIFIED) break;
         }
         if (consider) {
@@ -2151,6 +2281,7 @@ ORACLE_API result_t *optimize(const params_t *params, const qp_problem_t *prob)
     if (s.reason == TERMINATION_REASON_UNSPECIFIED) {
         s.reason = TERMINATION_REASON_ITERATION_LIMIT;
         compute_residual(s, s.P.optimality_norm);
+        if (!trajectory_row(s, false)) return nullptr;
     }
     double polish_time = 0.0;
     int polish_iters = 0;

@@ -2,9 +2,8 @@
 //
 // Reference: QP/FA_CP/src/cli.c:260-455 (getopt flags, defaults from
 // set_default_parameters, read_mps_file -> optimize), :38-78 (sigma-rule
-// names); the output files are written by cli_io.hpp.  The
-// trajectory CSV is not written by this build (the flag is accepted and
-// reported).
+// names); the output files are written by cli_io.hpp, the trajectory CSV by
+// the solver.
 #include "fa_cp.h"
 #include "fa_cp_mps_parser.h"
 #include "pdal_qp.h"
@@ -97,7 +96,7 @@ void usage(const char *prog)
             "      --sigma_normal_motion_threshold <float>  Minimum ||A dx||/||dx|| (default: 1e-10).\n"
             "      --sigma_activation_value <float>         Sigma on guarded activation (default: 0.1).\n"
             "      --eval_freq <int>               Termination evaluation frequency (default: 200).\n"
-            "      --trajectory_csv <path>         Accepted; not written by this build.\n"
+            "      --trajectory_csv <path>         Write iteration trajectory CSV for plotting.\n"
             "      --sv_max_iter <int>             Max iterations for singular value estimation (default: 5000).\n"
             "      --sv_tol <float>                Tolerance for singular value estimation (default: 1e-4).\n"
             "      --opt_norm <norm_type>          l2 or linf (default: linf).\n");
@@ -173,7 +172,6 @@ int main(int argc, char *argv[])
         case 1016: params.al_sigma = atof(optarg); break;
         case 1017:
             params.trajectory_csv = optarg;
-            fprintf(stderr, "[fa_cp] --trajectory_csv: the trajectory file is not written by this build.\n");
             break;
         case 1018:
             if (!parse_rule(optarg, &params.sigma_update_rule)) {

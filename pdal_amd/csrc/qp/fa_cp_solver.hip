@@ -115,7 +115,8 @@ enum Slot {
     S_NORMAL = S_DD2 + 1,
     S_PW0 = S_NORMAL + 1,     // power iteration: sumsq, <v,Mv>, (unused), resid
     S_BOUND = S_PW0 + 4,      // bound-objective sums
-    S_COUNT = S_BOUND + 2
+    S_SEL = S_BOUND + 2,      // restart candidate: ||dx||^2, ||dy||^2, <At dy, dx>
+    S_COUNT = S_SEL + 3
 };
 
 class Solver {
@@ -180,6 +181,10 @@ class Solver {
     void infeasibility_info();
     void check_termination();
     bool should_restart();
+#if !defined(PDAL_CP_AL_TYPES) && !defined(PDAL_PLANTED_FA_CP)
+    void choose_restart_candidate();  // fa_cp_core_op.cu:1130-1169
+    int restart_target_ = 0;          // 0: the T output, 1: the current iterate
+#endif
     void set_sigma_active(double sigma, double zero_thr);
     double clipped_update(double cur, double ratio) const;
     void update_al_sigma(double pd, double dd, double normal);
@@ -226,6 +231,10 @@ class Solver {
     result_t *make_result();
     void print_initial_info() const;
     void display_stats() const;
+#ifndef PDAL_LIN_CP_AL
+    bool trajectory_open() const;             // initialize_trajectory_csv (solver.cu:35-48)
+    bool trajectory_row(bool restart) const;  // append_trajectory_row (solver.cu:50-71)
+#endif
     // feasibility polishing (fa_cp_core_op.cu:1242-1735)
     struct PolishOut {
         termination_reason_t reason;
@@ -1208,12 +1217,75 @@ void Solver::pid_primal_weight(double pd, double dd)
     }
 }
 
-void Solver::perform_restart()
-{   // perform_restart, fa_cp_core_op.cu:908-1020 (restart target: the T-output)
+#if !defined(PDAL_CP_AL_TYPES) && !defined(PDAL_PLANTED_FA_CP)
+const char *restart_env(const char *name, const char *fallback)  // get_env_value, fa_cp_core_op.cu:35-42
+{
+    const char *v = getenv(name);
+    if ((!v || !v[0]) && fallback) v = getenv(fallback);
+    return v;
+}
+
+int restart_env_int(const char *name, const char *fallback, int def)  // :44-51
+{
+    const char *v = restart_env(name, fallback);
+    return (!v || !v[0]) ? def : atoi(v);
+}
+
+double restart_env_double(const char *name, double def)  // :53-66
+{
+    const char *v = restart_env(name, name);
+    if (!v || !v[0]) return def;
+    char *end = nullptr;
+    const double x = strtod(v, &end);
+    return (end == v || !std::isfinite(x)) ? def : x;
+}
+
+int restart_select_mode() { return restart_env_int("FA_CP_RESTART_SELECT", "FA_CP_RESTART_SELECT_LEGACY", 0); }
+
+void Solver::choose_restart_candidate()
+{   // :1091-1169: the current iterate as restart point when its score (the fixed-point
+    // metric of x_T against it, without the AL term) beats the T output's by the margin
+    const int mode = restart_select_mode();
+    restart_target_ = 0;
+    const double cur = std::isfinite(fp_) ? fp_ : INFINITY;
+    if (mode == 0 || total_ == 0 || !std::isfinite(cur)) return;
+    if (total_ < restart_env_int("FA_CP_RESTART_SELECT_MIN_ITERS", "FA_CP_RESTART_SELECT_MIN_ITERS", 0) ||
+        solve_time_ < restart_env_double("FA_CP_RESTART_SELECT_MIN_TIME", 0.0))
+        return;
     const int nbn = stride_grid(n_), nbm = stride_grid(m_);
-    diff_norm_kernel<<<nbn, kBlock, 0, stream_>>>(xT_.get(), x0_.get(), dx_.get(), n_, parts_.get());
+    diff_norm_kernel<<<nbn, kBlock, 0, stream_>>>(x_.get(), xT_.get(), dx_.get(), n_, parts_.get());
+    reduce(nbn, 1, 1, S_SEL);
+    diff_norm_kernel<<<nbm, kBlock, 0, stream_>>>(y_.get(), yT_.get(), dy_.get(), m_, parts_.get());
+    reduce(nbm, 1, 1, S_SEL + 1);
+    spmv(At_, dy_.get(), dual_prod_.get());
+    dot_sumsq_kernel<<<nbn, kBlock, 0, stream_>>>(dual_prod_.get(), dx_.get(), n_, parts_.get());
+    reduce(nbn, 2, 1, S_SEL + 2);
+    PDAL_LAUNCHED();
+    fetch(S_SEL, 3);
+    const double pn2 = hs(S_SEL), dn2 = hs(S_SEL + 1), cross = hs(S_SEL + 2);
+    const double halpern = std::sqrt(std::fmax(pn2 * pw_ + dn2 / pw_ + 2.0 * step_ * cross, 0.0));
+    if (!(cur > 0.0)) return;
+    const double margin = restart_env_double("FA_CP_RESTART_SELECT_MARGIN", 1.0);
+    if (mode == 2 || (std::isfinite(halpern) && halpern < margin * cur)) {
+        restart_target_ = 1;
+        fp_ = halpern;
+    }
+}
+#endif
+
+void Solver::perform_restart()
+{   // perform_restart, fa_cp_core_op.cu:908-1020: to the T output, or to the current iterate
+    // when FA_CP's candidate selection chose it
+#if !defined(PDAL_CP_AL_TYPES) && !defined(PDAL_PLANTED_FA_CP)
+    const bool to_current = restart_select_mode() != 0 && restart_target_ == 1;
+#else
+    const bool to_current = false;
+#endif
+    const double *rx = to_current ? x_.get() : xT_.get(), *ry = to_current ? y_.get() : yT_.get();
+    const int nbn = stride_grid(n_), nbm = stride_grid(m_);
+    diff_norm_kernel<<<nbn, kBlock, 0, stream_>>>(rx, x0_.get(), dx_.get(), n_, parts_.get());
     reduce(nbn, 1, 1, S_PD2);
-    diff_norm_kernel<<<nbm, kBlock, 0, stream_>>>(yT_.get(), y0_.get(), dy_.get(), m_, parts_.get());
+    diff_norm_kernel<<<nbm, kBlock, 0, stream_>>>(ry, y0_.get(), dy_.get(), m_, parts_.get());
     reduce(nbm, 1, 1, S_DD2);
     PDAL_LAUNCHED();
     fetch(S_PD2, 2);
@@ -1255,8 +1327,13 @@ void Solver::perform_restart()
     update_al_sigma(pd, dd, normal);
 #endif
 #endif  // PDAL_PLANTED_CP_AL
-    restart_copy_kernel<<<blocks_for((long)n_ + m_), kBlock, 0, stream_>>>(xT_.get(), x0_.get(), x_.get(), n_,
-                                                                           yT_.get(), y0_.get(), y_.get(), m_);
+    if (to_current) {  // :985-1009: anchor and T output = the current iterate
+        restart_copy_kernel<<<blocks_for((long)n_ + m_), kBlock, 0, stream_>>>(x_.get(), x0_.get(), xT_.get(), n_,
+                                                                               y_.get(), y0_.get(), yT_.get(), m_);
+    } else {
+        restart_copy_kernel<<<blocks_for((long)n_ + m_), kBlock, 0, stream_>>>(xT_.get(), x0_.get(), x_.get(), n_,
+                                                                               yT_.get(), y0_.get(), y_.get(), m_);
+    }
     PDAL_LAUNCHED();
     inner_ = 0;
     last_trial_fp_ = INFINITY;
@@ -1703,6 +1780,69 @@ void Solver::pcal_sigma_controller(double pd, double dd, double ax)
 }
 #endif  // PDAL_PLANTED_CP_AL
 
+#ifndef PDAL_LIN_CP_AL
+bool Solver::trajectory_open() const
+{   // the header row; false when the file cannot be written (the reference exits)
+    const char *path = P_.trajectory_csv;
+    if (!path || !path[0]) return true;
+    FILE *f = fopen(path, "w");
+    if (!f) {
+        perror("Error opening trajectory CSV");
+        return false;
+    }
+#if defined(PDAL_PLANTED_FA_CP) || defined(PDAL_PLANTED_CP_AL)  // the planted variants' solver.cu:35-49
+    fprintf(f, "iteration,time_sec,rel_primal,rel_dual,rel_gap,kkt_residual,fixed_point_residual,restart,sigma,"
+               "primal_weight,al_sigma,lambda_est,primal_step,dual_step\n");
+#else
+    fprintf(f, "iteration,time_sec,rel_primal,rel_dual,rel_gap,kkt_residual,fixed_point_residual,restart,al_sigma,"
+               "sigma_update_count\n");
+#endif
+    fclose(f);
+    return true;
+}
+
+bool Solver::trajectory_row(bool restart) const
+{   // one row per checkpoint, written before a planned restart is performed
+    const char *path = P_.trajectory_csv;
+    if (!path || !path[0]) return true;
+    FILE *f = fopen(path, "a");
+    if (!f) {
+        perror("Error appending trajectory CSV");
+        return false;
+    }
+    const double kkt = std::fmax(rel_p_, std::fmax(rel_d_, rel_gap_));
+    const double fpr = total_ == 0 ? NAN : fp_;
+#if defined(PDAL_PLANTED_FA_CP) || defined(PDAL_PLANTED_CP_AL)
+    double sigma = NAN, al_sigma = NAN, lambda = NAN, pstep = NAN, dstep = NAN;  // compute_trajectory_diagnostics
+#ifdef PDAL_PLANTED_CP_AL
+    if (use_al_) {
+        sigma = sigma_;
+        lambda = lambda_est_;
+        pstep = step_;
+        dstep = sigma_;
+    } else if (pw_ > 0.0) {
+        pstep = step_ / pw_;
+        dstep = step_ * pw_;
+    }
+#else
+    al_sigma = use_al_ ? al_sigma_ : NAN;
+    if (pw_ > 0.0) {
+        pstep = step_ / pw_;
+        dstep = step_ * pw_;
+    }
+#endif
+    fprintf(f, "%d,%.17e,%.17e,%.17e,%.17e,%.17e,%.17e,%d,%.17e,%.17e,%.17e,%.17e,%.17e,%.17e\n", total_,
+            solve_time_, rel_p_, rel_d_, rel_gap_, kkt, fpr, restart ? 1 : 0, sigma, pw_, al_sigma, lambda, pstep,
+            dstep);
+#else
+    fprintf(f, "%d,%.17e,%.17e,%.17e,%.17e,%.17e,%.17e,%d,%.17e,%d\n", total_, solve_time_, rel_p_, rel_d_,
+            rel_gap_, kkt, fpr, restart ? 1 : 0, al_sigma_, sigma_update_count_);
+#endif
+    fclose(f);
+    return true;
+}
+#endif
+
 result_t *Solver::run()
 {
 #ifdef PDAL_LIN_CP_AL
@@ -1725,6 +1865,7 @@ result_t *Solver::run()
 #endif
     PDAL_HIP(hipStreamSynchronize(stream_));
 
+    if (!trajectory_open()) return nullptr;
     const auto t0 = std::chrono::steady_clock::now();
     const int K = P_.termination_evaluation_frequency;
     bool do_restart = false;
@@ -1736,7 +1877,13 @@ result_t *Solver::run()
             if (major_ && total_ < 3 * K) infeasibility_info();
             solve_time_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
             check_termination();
-            if (consider && reason_ == TERMINATION_REASON_UNSPECIFIED) planned = should_restart();
+            if (consider && reason_ == TERMINATION_REASON_UNSPECIFIED) {
+#if !defined(PDAL_CP_AL_TYPES) && !defined(PDAL_PLANTED_FA_CP)
+                choose_restart_candidate();  // solver.cu:112
+#endif
+                planned = should_restart();
+            }
+            if (!trajectory_row(planned)) return nullptr;
             display_stats();
             if (reason_ != TERMINATION_REASON_UNSPECIFIED) break;
         }
@@ -1761,6 +1908,7 @@ result_t *Solver::run()
     if (reason_ == TERMINATION_REASON_UNSPECIFIED) {
         reason_ = TERMINATION_REASON_ITERATION_LIMIT;
         compute_residual();
+        if (!trajectory_row(false)) return nullptr;
         display_stats();
     }
     if (P_.feasibility_polishing && reason_ != TERMINATION_REASON_DUAL_INFEASIBLE &&

@@ -14,8 +14,7 @@
 // --reflection_coefficient; the planted CP-AL's --sigma_init,
 // --reflection_coefficient and --trajectory_csv; the planted FA_CP's --al_sigma, the adaptive AL
 // sigma flags, --reflection_coefficient, the restart and primal-weight PID
-// flags and --trajectory_csv (accepted; the file is not written by this
-// build).  As in the reference, whose main discards run_*'s return value,
+// flags and --trajectory_csv.  As in the reference, whose main discards run_*'s return value,
 // the process exits with status 0 after an error too.
 #if defined(PDAL_LIN_CP_AL)
 #include "lin_cp_al.h"
@@ -136,7 +135,7 @@ void usage(const char *prog)
             "      --no_al_qp                      Use the non-augmented baseline branch.\n"
             "      --sigma_init <float>            Initial sigma of the single-parameter CP-AL (0 = automatic, default).\n"
             "      --reflection_coefficient <float> Reflection coefficient of the Halpern step (default: 1.0).\n"
-            "      --trajectory_csv <path>         Accepted; not written by this build.\n"
+            "      --trajectory_csv <path>         Write iteration trajectory CSV for plotting.\n"
 #else
             "      --no_al_qp                      Use the non-augmented baseline branch.\n"
             "      --al_sigma <float>              AL penalty parameter of the primal subproblem (default: 0.1).\n"
@@ -150,7 +149,7 @@ void usage(const char *prog)
             "      --al_sigma_primal_trigger <float> Increase when rel_primal dominates by this ratio (default: 3.0).\n"
             "      --al_sigma_gap_trigger <float>  Decrease when rel_gap dominates by this ratio (default: 5.0).\n"
             "      --reflection_coefficient <float> Reflection coefficient of the Halpern step (default: 1.0).\n"
-            "      --trajectory_csv <path>         Accepted; not written by this build.\n"
+            "      --trajectory_csv <path>         Write iteration trajectory CSV for plotting.\n"
             "      --restart_artificial <float>    Artificial restart threshold (default: 0.36).\n"
             "      --restart_sufficient <float>    Sufficient reduction threshold (default: 0.2).\n"
             "      --restart_necessary <float>     Necessary reduction threshold (default: 0.8).\n"
@@ -255,7 +254,6 @@ int run(int argc, char *argv[])
         case 1016: params.sigma_init = atof(optarg); break;
         case 1017:
             params.trajectory_csv = optarg;
-            fprintf(stderr, "[" VARIANT_NAME "] --trajectory_csv: the trajectory file is not written by this build.\n");
             break;
         case 1018: params.reflection_coefficient = atof(optarg); break;
 #else
@@ -293,7 +291,6 @@ int run(int argc, char *argv[])
         } break;
         case 1034:
             params.trajectory_csv = optarg;
-            fprintf(stderr, "[" VARIANT_NAME "] --trajectory_csv: the trajectory file is not written by this build.\n");
             break;
 #endif
         case '?': return 1;

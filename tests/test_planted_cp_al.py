@@ -231,3 +231,83 @@ def test_gpu_solves_match_oracle_and_planted(oracle_pc, gpu_pc, name):
     assert abs(g.primal_objective - opt) <= 1e-6 * (1 + abs(opt))
     if name in NO_INNER:  # no inner solve: the same trajectory
         assert g.iterations == o.iterations
+
+
+# ---------------------------------------------------------------------------
+# The AL branch's sigma controller at restarts (perform_restart,
+# QP/PLANTED_CP_AL/src/cp_al_core_op.cu:655-803) read off the trajectory CSV
+# (rows written at checkpoints before a planned restart is performed: a
+# restart row's fixed-point residual and sigma are the controller's inputs,
+# the next row's sigma its output), and its first update recomputed from
+# solves cut at the restart with every rescaling off.
+
+def _csv_rows(path):
+    lines = Path(path).read_text().splitlines()
+    head = lines[0].split(",")
+    return [dict(zip(head, (float(v) for v in line.split(",")))) for line in lines[1:]]
+
+
+def test_sigma_controller_steps_and_rollbacks(oracle_pc, tmp_path):
+    """Each restart moves sigma by a factor in [0.6, 1.7], unless the restart's
+    fixed-point residual is above 1.05 times the best one so far: then sigma
+    returns to the sigma of that best restart.  The primal step follows
+    sigma (1 / (sigma lambda), or the nonconvexity floor when larger)."""
+    csv = tmp_path / "t.csv"
+    oracle_pc.reset_rng()
+    p = oracle_pc.default_params(trajectory_csv=str(csv).encode(), termination_evaluation_frequency=40,
+                                 iteration_limit=4000, eps_optimal_relative=1e-10, eps_feasible_relative=1e-10)
+    oracle_pc.solve(**qp_inst().qp_args("explicit", True), params=p)
+    rows = _csv_rows(csv)
+    restarts = [k for k, r in enumerate(rows[:-1]) if r["restart"] == 1.0]
+    assert len(restarts) >= 5
+    lam = rows[0]["lambda_est"]
+    base = 1.0 / np.sqrt(lam)
+    best_fp, best_sigma, rolled, stepped = np.inf, None, 0, 0
+    for k in restarts:
+        old, fp, new = rows[k]["sigma"], rows[k]["fixed_point_residual"], rows[k + 1]["sigma"]
+        if fp < best_fp:
+            best_fp, best_sigma = fp, old
+        if fp > 1.05 * best_fp:
+            assert new == best_sigma, rows[k]["iteration"]
+            rolled += 1
+        else:
+            r = new / old
+            clamped = new in (pytest.approx(1e-4 * base, rel=1e-12), pytest.approx(100.0 * base, rel=1e-12))
+            assert 0.6 - 1e-12 <= r <= 1.7 + 1e-12 or clamped, (rows[k]["iteration"], r)
+            stepped += abs(r - 1.0) > 1e-9
+    assert stepped >= 1
+    for r in rows:  # pcal_tau and the step columns
+        assert r["dual_step"] == r["sigma"]
+        assert r["primal_step"] >= 1.0 / (r["sigma"] * r["lambda_est"]) / 2 * (1 - 1e-12)
+
+
+def test_sigma_controller_first_update(oracle_pc, tmp_path):
+    """sigma_1 = sigma_0 * clip(sqrt(est / sigma_0), 0.6, 1.7) with est =
+    ||dy|| / sqrt(lambda ||dx||^2 - ||A dx||^2) over the first restart's
+    displacement (x_T, y_T at iteration K minus the start), every rescaling
+    off so the scaled space is the problem's own."""
+    inst = qp_inst()
+    args = inst.qp_args("explicit", True)
+    K = 40
+    out = []
+    for limit in (0, K, K + 10):
+        csv = tmp_path / f"t{limit}.csv"
+        oracle_pc.reset_rng()
+        p = oracle_pc.default_params(l_inf_ruiz_iterations=0, has_pock_chambolle_alpha=False,
+                                     bound_objective_rescaling=False, trajectory_csv=str(csv).encode(),
+                                     termination_evaluation_frequency=K, iteration_limit=limit,
+                                     eps_optimal_relative=1e-14, eps_feasible_relative=1e-14)
+        out.append((oracle_pc.solve(**args, params=p), _csv_rows(csv)))
+    (r0, _), (rk, rows_k), (_, rows_after) = out
+    restart_row = next(r for r in rows_after if r["iteration"] == K)
+    assert restart_row["restart"] == 1.0
+    sigma0, lam = restart_row["sigma"], restart_row["lambda_est"]
+    sigma1 = next(r for r in rows_after if r["iteration"] > K)["sigma"]
+    dx, dy = rk.x - r0.x, rk.y - r0.y
+    metric = lam * dx @ dx - np.sum((inst.A @ dx) ** 2)
+    est = np.linalg.norm(dy) / np.sqrt(metric)
+    want = sigma0 * min(max(np.sqrt(est / sigma0), 0.6), 1.7)
+    base = 1.0 / np.sqrt(lam)
+    want = min(max(want, 1e-4 * base), 100.0 * base)
+    assert sigma1 == pytest.approx(want, rel=1e-9)
+    assert abs(sigma1 / sigma0 - 1.0) > 1e-3

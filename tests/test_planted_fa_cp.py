@@ -204,3 +204,58 @@ def test_gpu_solves_match_oracle_and_planted(oracle_pl, gpu_pl, name):
     assert abs(g.primal_objective - opt) <= 1e-6 * (1 + abs(opt))
     if name == "lp_plain":  # no inner solve: the same trajectory
         assert g.iterations == o.iterations
+
+
+# ---------------------------------------------------------------------------
+# The adaptive AL sigma rule at restarts (maybe_update_al_sigma_on_restart,
+# QP/PLANTED_FA_CP/src/solver.cu:670-747) read off the trajectory CSV: every
+# row is written at a checkpoint before a planned restart is performed, so a
+# restart row's residuals and al_sigma are the rule's inputs and the next row's
+# al_sigma its output.
+
+def _csv_rows(path):
+    lines = Path(path).read_text().splitlines()
+    head = lines[0].split(",")
+    return [dict(zip(head, (float(v) for v in line.split(",")))) for line in lines[1:]]
+
+
+def _rule_output(row, rule, mode, P):
+    rp, rd, rg, s = abs(row["rel_primal"]), abs(row["rel_dual"]), abs(row["rel_gap"]), row["al_sigma"]
+    dg = max(rd, rg)
+    up = rp > P["trig_p"] * max(dg, 1e-12) and mode != "down"
+    down = dg > P["trig_g"] * max(rp, 1e-12) and mode != "up"
+    ratio = rp / max(dg, 1e-12)
+    if rule == "trigger":
+        return min(P["max"], s * P["inc"]) if up else max(P["min"], s * P["dec"]) if down else s
+    if rule == "sqrt":
+        return (min(P["max"], s * min(P["inc"], np.sqrt(ratio))) if up
+                else max(P["min"], s * max(P["dec"], np.sqrt(ratio))) if down else s)
+    return P["max"] if up else P["min"] if down else s  # bang-bang
+
+
+@pytest.mark.parametrize("rule, mode", [("trigger", "both"), ("sqrt", "both"), ("bangbang", "both"),
+                                        ("trigger", "up"), ("trigger", "down")])
+def test_adaptive_sigma_rule_from_the_trajectory(oracle_pl, tmp_path, rule, mode):
+    csv = tmp_path / "traj.csv"
+    P = dict(trig_p=2.0, trig_g=1.5, inc=1.8, dec=0.6, min=0.01, max=5.0)
+    params = oracle_pl.default_params(
+        adaptive_al_sigma=True, adaptive_al_sigma_rule={"trigger": 0, "sqrt": 1, "bangbang": 2}[rule],
+        adaptive_al_sigma_mode={"both": 0, "up": 1, "down": 2}[mode], al_sigma=0.2, al_sigma_min=P["min"],
+        al_sigma_max=P["max"], al_sigma_increase_factor=P["inc"], al_sigma_decrease_factor=P["dec"],
+        al_sigma_primal_trigger=P["trig_p"], al_sigma_gap_trigger=P["trig_g"], trajectory_csv=str(csv).encode(),
+        termination_evaluation_frequency=50, iteration_limit=3000, eps_optimal_relative=1e-9,
+        eps_feasible_relative=1e-9)
+    oracle_pl.reset_rng()
+    oracle_pl.solve(**qp_inst().qp_args("explicit", True), params=params)
+    rows = _csv_rows(csv)
+    restarts = [k for k, r in enumerate(rows[:-1]) if r["restart"] == 1.0]
+    assert len(restarts) >= 3
+    moved = 0
+    for k in restarts:
+        want = _rule_output(rows[k], rule, mode, P)
+        got = rows[k + 1]["al_sigma"]
+        assert got == pytest.approx(want, rel=1e-15), (rows[k]["iteration"], rows[k])
+        moved += got != rows[k]["al_sigma"]
+        # between restarts the rule does not act
+        assert all(r["al_sigma"] == got for r in rows[k + 1:(restarts + [len(rows)])[restarts.index(k) + 1] + 1])
+    assert moved >= 1
