@@ -4,11 +4,12 @@ QP/FA_CP/src/fa_cp_core_op.cu:1091-1169; restart to the current iterate,
 to the current iterate when its score beats the T output's fixed-point error
 by the margin, mode 2 always.
 
-CPU: with selection on the oracle still reaches the planted optimum, and the
-trajectory changes (so the switch is not a no-op); mode 2 with a margin has
-the margin ignored.  GPU: on an LP solved through FA_CP (no inner solve, so
-the device tracks the oracle through restarts) the device stops in the
-oracle's block with its status and objective.
+CPU: with selection on, the oracle still reaches the planted optimum of a QP
+and the trajectory changes (so the switch is not a no-op); the legacy
+variable name works; a zero margin in mode 1 gives the default trajectory.
+GPU: on an LP solved through FA_CP (no inner solve, so the device tracks the
+oracle through restarts) the device's iterates after 3000 iterations are the
+oracle's.
 """
 from __future__ import annotations
 
@@ -25,8 +26,9 @@ def _lp():
 
 def _solve(lib, args, **kw):
     lib.reset_rng()
-    p = lib.default_params(eps_optimal_relative=1e-7, eps_feasible_relative=1e-7, iteration_limit=200000,
-                           termination_evaluation_frequency=64, **kw)
+    kw.setdefault("iteration_limit", 200000)
+    p = lib.default_params(eps_optimal_relative=1e-7, eps_feasible_relative=1e-7, termination_evaluation_frequency=64,
+                           **kw)
     return lib.solve(**args, params=p)
 
 
@@ -58,13 +60,32 @@ def test_legacy_variable_and_margin(oracle_qp, monkeypatch):
     assert c.iterations == d.iterations and np.array_equal(c.x, d.x)
 
 
+def test_selection_can_stall_an_lp(oracle_qp, monkeypatch):
+    """On this LP (no quadratic term) restarting to the current iterate keeps
+    the solve from converging (rel KKT ~1e-3 after 2e5 iterations), where the
+    default restart converges in about 2.3e4: the option is off by default
+    for good reason, and the oracle takes it as it stands."""
+    lp, args = _lp()
+    base = _solve(oracle_qp, args)
+    monkeypatch.setenv("FA_CP_RESTART_SELECT", "1")
+    r = _solve(oracle_qp, args, iteration_limit=60000)
+    assert base.status == "OPTIMAL" and base.iterations < 60000
+    assert r.status == "ITERATION_LIMIT" and r.kkt > 1e-5
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("mode", ["1", "2"])
 def test_device_selection_matches_oracle(oracle_qp, gpu_qp, monkeypatch, mode):
+    """The LP has no inner solve, so the device tracks the oracle through the
+    restarts: the iterates after 3000 iterations agree, and differ from the
+    default restart's."""
     lp, args = _lp()
     monkeypatch.setenv("FA_CP_RESTART_SELECT", mode)
-    o = _solve(oracle_qp, args)
-    g = _solve(gpu_qp, args)
-    assert o.status == g.status == "OPTIMAL" and o.iterations == g.iterations
-    assert abs(g.primal_objective - o.primal_objective) <= 1e-9 * abs(o.primal_objective)
-    assert abs(g.primal_objective - lp.optimal_value) <= 1e-6 * (1 + abs(lp.optimal_value))
+    o = _solve(oracle_qp, args, iteration_limit=3000)
+    g = _solve(gpu_qp, args, iteration_limit=3000)
+    assert o.status == g.status == "ITERATION_LIMIT" and o.iterations == g.iterations == 3000
+    rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)  # noqa: E731
+    assert rel(g.x, o.x) <= 1e-9 and rel(g.y, o.y) <= 1e-9
+    monkeypatch.setenv("FA_CP_RESTART_SELECT", "0")
+    d = _solve(oracle_qp, args, iteration_limit=3000)
+    assert rel(d.x, o.x) > 1e-6
