@@ -123,6 +123,8 @@ def spmv_leg(lib, kernel_iters: int) -> dict:
     # the floor of a pass with uniform column indices: the same count of random FP64 gathers alone
     g_at = lib.lib.pdal_gather_calibration(nnz, m, 5)  # At.y gathers y (m entries)
     g_a = lib.lib.pdal_gather_calibration(nnz, n, 5)   # A.x_bar gathers x_bar (n entries)
+    if not (g_at > 0 and g_a > 0):
+        raise RuntimeError(f"pdal_gather_calibration failed ({g_at}, {g_a})")
     out["gather_floor_ms"] = {"primal_At": round(g_at, 4), "dual_A": round(g_a, 4)}
     out["spmv_over_gather_floor"] = {"primal_At": round(kt["primal_ms"] / g_at, 3), "dual_A": round(kt["dual_ms"] / g_a, 3)}
     log(f"large LP kernels: {out}")

@@ -56,8 +56,9 @@ double pdal_copy_calibration(long long bytes, int reps);
 
 /* Gather calibration: `count` FP64 loads at uniformly random indices of a
  * `len`-entry device vector (4-byte indices streamed, one running sum per
- * thread), `reps` times; returns the average kernel time in ms (hipEvents),
- * or a negative value on error.  It is the floor of an SpMV pass whose column
+ * thread), `reps` times, with 1, 4 and 8 independent loads in flight per
+ * thread over three grid sizes; returns the fastest average kernel time in ms
+ * (hipEvents), or a negative value on error.  It is the floor of an SpMV pass whose column
  * indices are uniform: that pass makes the same loads and streams the values
  * and the result besides. */
 double pdal_gather_calibration(long long count, long long len, int reps);
