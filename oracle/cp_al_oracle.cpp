@@ -45,9 +45,17 @@
  *   _TEST_TRAJECTORY_LIMIT, CP_AL_DETERMINISTIC_PULSE_FORCE (plus
  *   PDAL_ALPHA_TRIAL_MIN_SEC, a test knob for the trial's 2 s floor); restart
  *   candidate selection (solver.cu:2663-2819): CP_AL_RESTART_SELECT / _MIN_ITERS
- *   / _MIN_TIME / _THREEPOINT / _MARGIN.  The adaptive-lambda, sigma-trigger
- *   and dynamic-eval controllers are not part of this oracle (nor of the HIP
- *   path).
+ *   / _MIN_TIME / _THREEPOINT / _MARGIN; the sigma trigger (solver.cu:1496-1528,
+ *   CP_AL_SIGMA_TRIGGER / _MIN_ITERS / _PRIMAL / _DUAL / _RATIO); the dynamic
+ *   evaluation frequency (:1228-1260, 1533-1584, CP_AL_DYNAMIC_EVAL / _FAST /
+ *   _AFTER_SEC / _AFTER_ITERS / _MIN_ITERS / _SMALL_RES / _LARGE_RES / _RATIO /
+ *   _BOOTSTRAP_ITERS); adaptive lambda (:1059-1145, 3166-3536, CP_AL_ADAPTIVE_LAMBDA,
+ *   CP_AL_LAMBDA_PROFILE, CP_AL_LAMBDA_GRID and the CP_AL_LAMBDA_* thresholds);
+ *   the Curtis-Reid diagnostic and early stop (preconditioner.cu:171-252).
+ *   PDAL_CONTROLLER_SECONDS_PER_ITERATION (a test knob shared with the HIP
+ *   path) replaces the wall clock of the adaptive-lambda epochs and the
+ *   dynamic evaluation's time gate by the iteration count times its value.
+ *   The dense per-iteration trace (CP_AL_LOG_FREQ) is not restated.
  *
  * Deliberate differences from the reference, shared with the HIP path:
  *   - timing uses steady_clock wall time, not clock() CPU time;
@@ -176,6 +184,23 @@ struct State {
     double al_sigma = 0.0, sigma_base = 0.0, sigma_min = 0.0, sigma_max = 0.0, sigma_best = 0.0;
     double sigma_err_sum = 0.0, sigma_last_err = 0.0, lambda_param = 0.0;
     double pw_error_sum = 0.0, pw_last_error = 0.0, best_pw = 1.0, best_gap = INFINITY;
+    // adaptive lambda (solver.cu:3166-3536; internal_types.h:25 CP_AL_MAX_LAMBDA_TIERS = 16)
+    bool adaptive_lambda = false, adaptive_lambda_metrics_ready = false, lambda_race = false;
+    int lambda_num_tiers = 1, lambda_tier = 0, lambda_hold_remaining = 0, lambda_restart_count = 0;
+    double lambda_values[16] = {};
+    int lambda_hold_epochs = 2, lambda_min_restarts = 0, lambda_min_iterations = 0;
+    int lambda_force_restart_iters = 0, lambda_last_restart_iter = 0;
+    int lambda_race_next_tier = 1, lambda_race_best_tier = 0, lambda_race_exploit_remaining = 0;
+    int lambda_race_exploit_epochs = 3, lambda_race_cooldown_remaining = 0, lambda_race_cooldown_epochs = 0;
+    int lambda_race_cycle_best_tier = 0;
+    double lambda_race_best_score = -INFINITY, lambda_race_cycle_best_score = -INFINITY;
+    double lambda_race_bad_ratio = INFINITY, lambda_min_time_sec = 0.0;
+    double lambda_feas_plateau_tol = 0.98, lambda_gap_plateau_tol = 0.95, lambda_fp_plateau_tol = 0.98;
+    double lambda_feas_active_tol = 1e-6, lambda_gap_active_tol = 1e-6, lambda_fp_active_tol = 1e-8;
+    double lambda_gap_dominance = 100.0;
+    double lambda_epoch_feas_start = INFINITY, lambda_epoch_gap_start = INFINITY, lambda_epoch_fp_start = INFINITY;
+    double lambda_epoch_time_start = 0.0;
+    bool cr_early_stopped = false;
     int total_count = 0, inner_count = 0;
     double fp = 0.0, fp0 = 0.0, last_trial_fp = INFINITY;
     // restart candidate (choose_restart_candidate, solver.cu:2749-2819): 0 = T output,
@@ -257,11 +282,13 @@ void clamp_sqrt_accum(std::vector<double> &sf, std::vector<double> &inv, std::ve
     }
 }
 
-void cr_log_update(const Csr &M, const std::vector<double> &other, std::vector<double> &log_scale)
-{   // preconditioner.cu:553-588 (log_delta only feeds diagnostics / early stop)
+void cr_log_update(const Csr &M, const std::vector<double> &other, std::vector<double> &log_scale,
+                   std::vector<double> *log_delta)
+{   // preconditioner.cu:553-588 (log_delta feeds the diagnostic and the early stop)
     for (int r = 0; r < M.rows; ++r) {
         const int b = M.rp[r], e = M.rp[r + 1];
         if (e - b <= 0) {
+            if (log_delta) (*log_delta)[r] = -log_scale[r];
             log_scale[r] = 0.0;
             continue;
         }
@@ -271,8 +298,24 @@ void cr_log_update(const Csr &M, const std::vector<double> &other, std::vector<d
             if (!std::isfinite(v) || v < 1e-300) v = 1e-300;
             acc += -std::log(v) - other[M.ci[k]];
         }
-        log_scale[r] = acc / (double)(e - b);
+        const double updated = acc / (double)(e - b);
+        if (log_delta) (*log_delta)[r] = updated - log_scale[r];
+        log_scale[r] = updated;
     }
+}
+
+double cr_max_edge_change(const Csr &A, const std::vector<double> &row_delta, const std::vector<double> &col_delta)
+{   // curtis_reid_edge_change_kernel (preconditioner.cu:590-609) + cub::DeviceReduce::Max (:222-228)
+    double mx = 0.0;
+    for (int r = 0; r < A.rows; ++r) {
+        double row_max = 0.0;
+        for (int k = A.rp[r]; k < A.rp[r + 1]; ++k) {
+            const double ch = std::fabs(row_delta[r] + col_delta[A.ci[k]]);
+            if (std::isfinite(ch) && ch > row_max) row_max = ch;
+        }
+        mx = r == 0 ? row_max : std::fmax(mx, row_max);
+    }
+    return mx;
 }
 
 void cr_exp_to_scale(std::vector<double> &ls, std::vector<double> &inv, std::vector<double> &cum)
@@ -299,12 +342,31 @@ void rescale_problem(const cp_al_parameters_t *p, State &s)  // preconditioner.c
     const char *cr_iters_raw = getenv("CP_AL_CR_ITERS");
     if (cr_iters > 0 && cr_iters_raw && cr_iters_raw[0] != '\0') cr_iters = atoi(cr_iters_raw);
     cr_iters = cr_iters < 0 ? 0 : cr_iters;
+    s.cr_early_stopped = false;
     if (cr_iters > 0) {  // curtis_reid_rescaling, :162-268
+        const bool emit_diagnostic = getenv("CP_AL_CR_DIAGNOSTIC") != NULL && atoi(getenv("CP_AL_CR_DIAGNOSTIC")) != 0;
+        const bool use_early_stop =
+            getenv("CP_AL_CR_EARLY_STOP") != NULL && atoi(getenv("CP_AL_CR_EARLY_STOP")) != 0 && cr_iters == 20;
+        const char *thr_raw = getenv("CP_AL_CR_EARLY_STOP_THRESHOLD");
+        const double early_stop_threshold = thr_raw != NULL ? atof(thr_raw) : 0.1;
+        const bool measure = emit_diagnostic || use_early_stop;
         std::fill(e.begin(), e.end(), 0.0);
         std::fill(d.begin(), d.end(), 0.0);
         for (int it = 0; it < cr_iters; ++it) {
-            cr_log_update(s.A, d, e);
-            cr_log_update(s.At, e, d);
+            cr_log_update(s.A, d, e, measure ? &inv_e : nullptr);
+            cr_log_update(s.At, e, d, measure ? &inv_d : nullptr);
+            if (measure) {
+                const double max_edge_change = cr_max_edge_change(s.A, inv_e, inv_d);
+                if (emit_diagnostic)
+                    fprintf(stderr, "[cr-diagnostic] iteration=%d max_edge_log_change=%.12e\n", it + 1,
+                            max_edge_change);
+                if (use_early_stop && it + 1 == 5 && max_edge_change <= early_stop_threshold) {
+                    s.cr_early_stopped = true;
+                    printf("[cr-early-stop] selected=%d executed=%d max_edge_log_change=%.12e threshold=%.12e\n",
+                           cr_iters, it + 1, max_edge_change, early_stop_threshold);
+                    break;
+                }
+            }
         }
         cr_exp_to_scale(e, inv_e, s.E);
         cr_exp_to_scale(d, inv_d, s.D);
@@ -441,6 +503,324 @@ void refresh_step_schedule(State &s)  // solver.cu:3103-3126
     s.step_size = std::sqrt(std::fmax(eta_sq, 1e-24));
     const double dual_step = s.step_size * std::fmax(s.primal_weight, 1e-12);
     s.al_penalty = dual_step * s.al_lambda / std::fmax(1.0 - s.al_lambda, 1e-12);
+}
+
+/* adaptive lambda (solver.cu:1059-1145, 3166-3536) */
+
+double controller_seconds(const State &s)
+{   // cumulative_time_sec as the adaptive-lambda epochs and the dynamic evaluation
+    // read it (utils.cu:301); PDAL_CONTROLLER_SECONDS_PER_ITERATION > 0 makes it
+    // total_count times that value (a test knob shared with the HIP path)
+    const double per_iter = env_double("PDAL_CONTROLLER_SECONDS_PER_ITERATION", 0.0);
+    return per_iter > 0.0 ? per_iter * (double)s.total_count : s.solve_time;
+}
+
+int parse_lambda_grid_env(const char *raw, double *values, int max_values)  // :1059-1083
+{
+    if (!raw || !raw[0] || max_values <= 0) return 0;
+    char buffer[512];
+    strncpy(buffer, raw, sizeof(buffer) - 1);
+    buffer[sizeof(buffer) - 1] = '\0';
+    int count = 0;
+    char *token = strtok(buffer, ",;: \t");
+    while (token && count < max_values) {
+        char *end = NULL;
+        const double value = strtod(token, &end);
+        if (end != token && std::isfinite(value)) values[count++] = std::fmin(std::fmax(value, 0.0), 0.95);
+        token = strtok(NULL, ",;: \t");
+    }
+    return count;
+}
+
+void set_lambda_grid(State &s, const double *values, int count)  // :1085-1145
+{
+    double grid[16];
+    int n = 0;
+    if (count <= 0) {
+        grid[n++] = 0.0;
+    } else {
+        bool has_zero = false;
+        for (int i = 0; i < count && n < 16; ++i) {
+            const double value = std::fmin(std::fmax(values[i], 0.0), 0.95);
+            if (value <= 1e-15) has_zero = true;
+            grid[n++] = value;
+        }
+        if (!has_zero && n < 16) grid[n++] = 0.0;
+    }
+    for (int i = 1; i < n; ++i) {
+        const double key = grid[i];
+        int j = i - 1;
+        while (j >= 0 && grid[j] > key) {
+            grid[j + 1] = grid[j];
+            --j;
+        }
+        grid[j + 1] = key;
+    }
+    int unique_count = 0;
+    for (int i = 0; i < n; ++i)
+        if (unique_count == 0 || std::fabs(grid[i] - s.lambda_values[unique_count - 1]) > 1e-15)
+            s.lambda_values[unique_count++] = grid[i];
+    if (unique_count == 0) s.lambda_values[unique_count++] = 0.0;
+    s.lambda_num_tiers = unique_count;
+    for (int i = unique_count; i < 16; ++i) s.lambda_values[i] = s.lambda_values[unique_count - 1];
+}
+
+void initialize_adaptive_lambda(State &s)  // :3166-3262
+{
+    const char *lambda_profile = getenv("CP_AL_LAMBDA_PROFILE");
+    const bool use_grid4_race_profile =
+        lambda_profile && (strcmp(lambda_profile, "grid4_race") == 0 || strcmp(lambda_profile, "race4") == 0);
+    const bool use_mp_race_profile =
+        lambda_profile && (strcmp(lambda_profile, "mp_race") == 0 || strcmp(lambda_profile, "safe_race") == 0);
+    const bool use_mp_auto_profile =
+        lambda_profile && (strcmp(lambda_profile, "mp_auto") == 0 || strcmp(lambda_profile, "safe_auto") == 0);
+    const bool use_small_grid_profile = use_grid4_race_profile || use_mp_race_profile || use_mp_auto_profile;
+
+    s.adaptive_lambda = use_small_grid_profile || env_int("CP_AL_ADAPTIVE_LAMBDA", 0) != 0;
+    s.adaptive_lambda_metrics_ready = false;
+    s.lambda_tier = 0;
+    s.lambda_hold_remaining = 0;
+    s.lambda_restart_count = 0;
+
+    double lambda_grid[16];
+    int lambda_grid_count = parse_lambda_grid_env(getenv("CP_AL_LAMBDA_GRID"), lambda_grid, 16);
+    if (lambda_grid_count == 0 && (use_mp_race_profile || use_mp_auto_profile)) {
+        const double default_grid[] = {0.0, 0.005, 0.01, 0.02, 0.035, 0.05, 0.075, 0.10, 0.15, 0.20, 0.30, 0.40};
+        lambda_grid_count = 12;
+        for (int i = 0; i < lambda_grid_count; ++i) lambda_grid[i] = default_grid[i];
+    }
+    if (lambda_grid_count == 0) {
+        lambda_grid[0] = env_double("CP_AL_LAMBDA_BASE", 0.0);
+        lambda_grid[1] = env_double("CP_AL_LAMBDA_LOW", use_small_grid_profile ? 0.02 : 0.10);
+        lambda_grid[2] = env_double("CP_AL_LAMBDA_MID", use_small_grid_profile ? 0.10 : 0.10);
+        lambda_grid[3] = env_double("CP_AL_LAMBDA_HIGH", use_small_grid_profile ? 0.20 : 0.10);
+        lambda_grid_count = 4;
+    }
+    set_lambda_grid(s, lambda_grid, lambda_grid_count);
+
+    s.lambda_hold_epochs = env_int("CP_AL_LAMBDA_HOLD_EPOCHS", use_mp_auto_profile ? 1 : 2);
+    s.lambda_min_restarts = env_int("CP_AL_LAMBDA_MIN_RESTARTS", use_mp_race_profile || use_mp_auto_profile
+                                                                      ? 2
+                                                                      : (use_grid4_race_profile ? 1 : 0));
+    s.lambda_min_iterations =
+        env_int("CP_AL_LAMBDA_MIN_ITERS", use_mp_race_profile || use_mp_auto_profile ? 10000 : 0);
+    s.lambda_force_restart_iters = env_int("CP_AL_LAMBDA_FORCE_RESTART_ITERS", use_mp_race_profile ? 20000 : 0);
+    s.lambda_last_restart_iter = 0;
+    s.lambda_race = use_grid4_race_profile || use_mp_race_profile || env_int("CP_AL_LAMBDA_RACE", 0) != 0;
+    s.lambda_race_next_tier = 1;
+    s.lambda_race_best_tier = 0;
+    s.lambda_race_exploit_remaining = 0;
+    s.lambda_race_exploit_epochs = env_int("CP_AL_LAMBDA_RACE_EXPLOIT_EPOCHS",
+                                           use_mp_race_profile ? 2 : (use_grid4_race_profile ? 9999 : 3));
+    s.lambda_race_cooldown_remaining = 0;
+    s.lambda_race_cooldown_epochs = env_int("CP_AL_LAMBDA_RACE_COOLDOWN_EPOCHS",
+                                            use_mp_race_profile ? 3 : (use_grid4_race_profile ? 1000000000 : 0));
+    s.lambda_race_best_score = -INFINITY;
+    s.lambda_race_cycle_best_tier = 0;
+    s.lambda_race_cycle_best_score = -INFINITY;
+    s.lambda_race_bad_ratio = env_double("CP_AL_LAMBDA_RACE_BAD_RATIO",
+                                         use_mp_race_profile ? 3.0 : (use_grid4_race_profile ? 10.0 : INFINITY));
+    s.lambda_min_time_sec = env_double("CP_AL_LAMBDA_MIN_TIME", 0.0);
+    s.lambda_feas_plateau_tol = env_double("CP_AL_LAMBDA_FEAS_PLATEAU", 0.98);
+    s.lambda_gap_plateau_tol = env_double("CP_AL_LAMBDA_GAP_PLATEAU", 0.95);
+    s.lambda_fp_plateau_tol = env_double("CP_AL_LAMBDA_FP_PLATEAU", 0.98);
+    s.lambda_feas_active_tol = env_double("CP_AL_LAMBDA_FEAS_ACTIVE", 1e-6);
+    s.lambda_gap_active_tol = env_double("CP_AL_LAMBDA_GAP_ACTIVE", 1e-6);
+    s.lambda_fp_active_tol = env_double("CP_AL_LAMBDA_FP_ACTIVE", 1e-8);
+    s.lambda_gap_dominance = env_double("CP_AL_LAMBDA_GAP_DOMINANCE", 100.0);
+
+    s.lambda_hold_epochs = s.lambda_hold_epochs < 0 ? 0 : s.lambda_hold_epochs;
+    s.lambda_min_restarts = s.lambda_min_restarts < 0 ? 0 : s.lambda_min_restarts;
+    s.lambda_min_iterations = s.lambda_min_iterations < 0 ? 0 : s.lambda_min_iterations;
+    s.lambda_force_restart_iters = s.lambda_force_restart_iters < 0 ? 0 : s.lambda_force_restart_iters;
+    s.lambda_race_exploit_epochs = s.lambda_race_exploit_epochs < 0 ? 0 : s.lambda_race_exploit_epochs;
+    s.lambda_race_cooldown_epochs = s.lambda_race_cooldown_epochs < 0 ? 0 : s.lambda_race_cooldown_epochs;
+    s.lambda_race_bad_ratio = std::fmax(s.lambda_race_bad_ratio, 1.0);
+    s.lambda_min_time_sec = std::fmax(s.lambda_min_time_sec, 0.0);
+
+    if (s.adaptive_lambda) s.al_lambda = s.lambda_values[0];  // lambda_base
+    s.lambda_epoch_feas_start = INFINITY;
+    s.lambda_epoch_gap_start = INFINITY;
+    s.lambda_epoch_fp_start = INFINITY;
+    s.lambda_epoch_time_start = controller_seconds(s);
+}
+
+double lambda_from_tier(const State &s, int tier)  // :3264-3276
+{
+    if (s.lambda_num_tiers > 0) {
+        if (tier < 0) tier = 0;
+        if (tier >= s.lambda_num_tiers) tier = s.lambda_num_tiers - 1;
+        return s.lambda_values[tier];
+    }
+    return 0.0;
+}
+
+void update_adaptive_lambda_on_restart(State &s, bool verbose)  // :3278-3536
+{
+    if (!s.adaptive_lambda) return;
+    const double eps = 1e-300;
+    const double now = controller_seconds(s);
+    const double feas_end = std::fmax(s.rel_p, s.rel_d);
+    const double gap_end = std::fmax(std::fabs(s.rel_gap), 0.0);
+    const double fp_end = (std::isfinite(s.fp) && s.fp > 0.0) ? s.fp : INFINITY;
+
+    s.lambda_restart_count += 1;
+    if (!s.adaptive_lambda_metrics_ready) {
+        s.lambda_epoch_feas_start = feas_end;
+        s.lambda_epoch_gap_start = gap_end;
+        s.lambda_epoch_fp_start = fp_end;
+        s.lambda_epoch_time_start = now;
+        s.adaptive_lambda_metrics_ready = true;
+        if (verbose)
+            printf("lambda-adapt init restart %d lambda %.3e feas %.3e gap %.3e fp %.3e\n", s.lambda_restart_count,
+                   s.al_lambda, feas_end, gap_end, fp_end);
+        return;
+    }
+
+    const double feas_start = s.lambda_epoch_feas_start;
+    const double gap_start = s.lambda_epoch_gap_start;
+    const double fp_start = s.lambda_epoch_fp_start;
+    const double feas_ratio = std::fmax(feas_end, eps) / std::fmax(feas_start, eps);
+    const double gap_ratio = std::fmax(gap_end, eps) / std::fmax(gap_start, eps);
+    const double fp_ratio = std::fmax(fp_end, eps) / std::fmax(fp_start, eps);
+    const double epoch_time = std::fmax(now - s.lambda_epoch_time_start, 1e-9);
+    const bool mature_enough = s.lambda_restart_count >= s.lambda_min_restarts &&
+                               s.total_count >= s.lambda_min_iterations && now >= s.lambda_min_time_sec;
+
+    if (s.lambda_race) {
+        const int old_tier = s.lambda_tier;
+        const double old_lambda = s.al_lambda;
+        double score = 0.45 * std::log(std::fmax(feas_start, eps) / std::fmax(feas_end, eps)) +
+                       0.35 * std::log(std::fmax(fp_start, eps) / std::fmax(fp_end, eps)) +
+                       0.20 * std::log(std::fmax(gap_start, eps) / std::fmax(gap_end, eps));
+        score /= epoch_time;
+        const bool bad_positive_epoch =
+            old_tier > 0 && (feas_ratio > s.lambda_race_bad_ratio || gap_ratio > s.lambda_race_bad_ratio ||
+                             fp_ratio > s.lambda_race_bad_ratio);
+        if (!bad_positive_epoch && std::isfinite(score) && score > s.lambda_race_best_score) {
+            s.lambda_race_best_score = score;
+            s.lambda_race_best_tier = old_tier;
+        }
+        if (!bad_positive_epoch && std::isfinite(score) && score > s.lambda_race_cycle_best_score) {
+            s.lambda_race_cycle_best_score = score;
+            s.lambda_race_cycle_best_tier = old_tier;
+        }
+        const double worst_ratio = std::fmax(feas_ratio, std::fmax(gap_ratio, fp_ratio));
+        const bool moderate_instability = worst_ratio < 3.0;
+        const bool feasibility_only_lag = score > 5.0 && s.lambda_restart_count <= 10 && feas_ratio > 1.20 &&
+                                          feas_ratio < 2.00 && gap_ratio < 1.20 && fp_ratio < 1.20;
+        const bool feasibility_gain_tradeoff = score > 5.0 && feas_ratio < 0.50 &&
+                                               std::fmax(gap_ratio, fp_ratio) > 3.00 &&
+                                               worst_ratio < s.lambda_race_bad_ratio;
+        const bool compact_plateau = score < -10.0 && moderate_instability && feas_ratio < 1.10;
+        const bool race_plateau =
+            mature_enough &&
+            ((std::fmax(feas_start, feas_end) > s.lambda_feas_active_tol && feas_ratio > s.lambda_feas_plateau_tol) ||
+             (std::fmax(gap_start, gap_end) > s.lambda_gap_active_tol && gap_ratio > s.lambda_gap_plateau_tol) ||
+             (std::fmax(fp_start, fp_end) > s.lambda_fp_active_tol && fp_ratio > s.lambda_fp_plateau_tol));
+        const bool start_race =
+            old_tier > 0 || race_plateau || feasibility_only_lag || feasibility_gain_tradeoff || compact_plateau;
+
+        int requested_tier = 0;
+        if (bad_positive_epoch) {
+            requested_tier = 0;
+            s.lambda_race_next_tier = 1;
+            s.lambda_race_exploit_remaining = 0;
+            s.lambda_race_cooldown_remaining = s.lambda_race_cooldown_epochs;
+            s.lambda_race_cycle_best_tier = 0;
+            s.lambda_race_cycle_best_score = -INFINITY;
+        } else if (s.lambda_race_cooldown_remaining > 0) {
+            requested_tier = 0;
+            s.lambda_race_cooldown_remaining -= 1;
+            s.lambda_race_next_tier = 1;
+            s.lambda_race_exploit_remaining = 0;
+            s.lambda_race_cycle_best_tier = 0;
+            s.lambda_race_cycle_best_score = std::isfinite(score) ? score : -INFINITY;
+        } else if (!start_race) {
+            requested_tier = 0;
+            s.lambda_race_next_tier = 1;
+            s.lambda_race_exploit_remaining = 0;
+            s.lambda_race_cycle_best_tier = 0;
+            s.lambda_race_cycle_best_score = std::isfinite(score) ? score : -INFINITY;
+        } else if (!mature_enough) {
+            requested_tier = 0;
+            s.lambda_race_cycle_best_tier = 0;
+            s.lambda_race_cycle_best_score = std::isfinite(score) ? score : -INFINITY;
+        } else if (s.lambda_race_next_tier < s.lambda_num_tiers) {
+            requested_tier = s.lambda_race_next_tier;
+            if (s.lambda_race_next_tier == 1) {
+                s.lambda_race_cycle_best_tier = old_tier;
+                s.lambda_race_cycle_best_score = std::isfinite(score) ? score : -INFINITY;
+            }
+            s.lambda_race_next_tier += 1;
+            s.lambda_race_exploit_remaining = s.lambda_race_exploit_epochs;
+        } else if (s.lambda_race_exploit_remaining > 0) {
+            requested_tier = s.lambda_race_cycle_best_tier;
+            s.lambda_race_exploit_remaining -= 1;
+        } else {
+            s.lambda_race_next_tier = 1;
+            requested_tier = s.lambda_race_cycle_best_tier;
+            s.lambda_race_cycle_best_tier = requested_tier;
+            s.lambda_race_cycle_best_score = std::isfinite(score) ? score : -INFINITY;
+        }
+
+        s.lambda_tier = requested_tier;
+        s.lambda_hold_remaining = 0;
+        s.al_lambda = lambda_from_tier(s, s.lambda_tier);
+        s.lambda_epoch_feas_start = feas_end;
+        s.lambda_epoch_gap_start = gap_end;
+        s.lambda_epoch_fp_start = fp_end;
+        s.lambda_epoch_time_start = now;
+        if (verbose || old_tier != s.lambda_tier || std::fabs(old_lambda - s.al_lambda) > 1e-15)
+            printf("lambda-race restart %d score %.3e best(tier=%d score=%.3e) cycle(tier=%d score=%.3e) "
+                   "tier %d->%d lambda %.3e->%.3e next=%d exploit=%d cool=%d bad=%d start=%d mature=%d "
+                   "ratios(feas %.3e gap %.3e fp %.3e)\n",
+                   s.lambda_restart_count, score, s.lambda_race_best_tier, s.lambda_race_best_score,
+                   s.lambda_race_cycle_best_tier, s.lambda_race_cycle_best_score, old_tier, s.lambda_tier, old_lambda,
+                   s.al_lambda, s.lambda_race_next_tier, s.lambda_race_exploit_remaining,
+                   s.lambda_race_cooldown_remaining, (int)bad_positive_epoch, (int)start_race, (int)mature_enough,
+                   feas_ratio, gap_ratio, fp_ratio);
+        return;
+    }
+
+    const bool feas_plateau =
+        std::fmax(feas_start, feas_end) > s.lambda_feas_active_tol && feas_ratio > s.lambda_feas_plateau_tol;
+    const bool gap_plateau =
+        std::fmax(gap_start, gap_end) > s.lambda_gap_active_tol && gap_ratio > s.lambda_gap_plateau_tol;
+    const bool fp_plateau =
+        std::fmax(fp_start, fp_end) > s.lambda_fp_active_tol && fp_ratio > s.lambda_fp_plateau_tol;
+    const bool gap_dominates = gap_end > s.lambda_gap_dominance * std::fmax(feas_end, eps);
+    const int low_tier = s.lambda_num_tiers > 1 ? 1 : 0;
+    const int mid_tier = s.lambda_num_tiers > 2 ? s.lambda_num_tiers / 2 : low_tier;
+    const int high_tier = s.lambda_num_tiers > 0 ? s.lambda_num_tiers - 1 : 0;
+    int requested_tier = 0;
+    if (!mature_enough) requested_tier = 0;
+    else if (gap_plateau && gap_dominates) requested_tier = high_tier;
+    else if (feas_plateau || (gap_plateau && fp_plateau)) requested_tier = mid_tier;
+    else if (gap_plateau || fp_plateau) requested_tier = low_tier;
+
+    const int old_tier = s.lambda_tier;
+    const double old_lambda = s.al_lambda;
+    if (requested_tier > 0) {
+        s.lambda_tier = requested_tier;
+        s.lambda_hold_remaining = s.lambda_hold_epochs;
+    } else if (s.lambda_hold_remaining > 0 && s.lambda_tier > 0) {
+        s.lambda_hold_remaining -= 1;
+    } else {
+        s.lambda_tier = 0;
+        s.lambda_hold_remaining = 0;
+    }
+    s.al_lambda = lambda_from_tier(s, s.lambda_tier);
+    s.lambda_epoch_feas_start = feas_end;
+    s.lambda_epoch_gap_start = gap_end;
+    s.lambda_epoch_fp_start = fp_end;
+    s.lambda_epoch_time_start = now;
+    if (verbose || old_tier != s.lambda_tier || std::fabs(old_lambda - s.al_lambda) > 1e-15)
+        printf("lambda-adapt restart %d ratios(feas %.3e gap %.3e fp %.3e) "
+               "plateau(feas %d gap %d fp %d dom %d mature %d) tier %d->%d hold %d lambda %.3e->%.3e\n",
+               s.lambda_restart_count, feas_ratio, gap_ratio, fp_ratio, (int)feas_plateau, (int)gap_plateau,
+               (int)fp_plateau, (int)gap_dominates, (int)mature_enough, old_tier, s.lambda_tier,
+               s.lambda_hold_remaining, old_lambda, s.al_lambda);
 }
 
 void iteration(State &s, int k_offset, double rho, bool major)
@@ -612,6 +992,7 @@ void perform_restart(State &s, const cp_al_parameters_t *p)  // solver.cu:2821-3
             s.best_gap = g;
             s.best_pw = s.primal_weight;
         }
+        update_adaptive_lambda_on_restart(s, p->verbose);  // :2996
     }
     refresh_step_schedule(s);
     if (target == 1) {  // :3019-3043: anchor = T output = the current iterate, reflected = old T output
@@ -636,6 +1017,7 @@ void perform_restart(State &s, const cp_al_parameters_t *p)  // solver.cu:2821-3
         s.y0 = s.yT;
         s.y = s.yT;
     }
+    s.lambda_last_restart_iter = s.total_count;  // :3098
     s.inner_count = 0;
     s.last_trial_fp = INFINITY;
 }
@@ -1129,6 +1511,7 @@ void trial_start(AlphaTrial &t, State &s, int charged)  // alpha_trial_start, :7
     t.max_iters = 2 * hmax;
     t.max_sec = 2.0 * std::fmax(2.0, std::fmax(t.hist[0].seconds, t.hist[1].seconds));
     s.sigma_mode = false;
+    s.adaptive_lambda = false;
     s.al_lambda = 0.002;
     refresh_step_schedule(s);
     compute_fixed_point_error(s);
@@ -1240,6 +1623,7 @@ void pulse_start(DirectPulse &d, State &s, int charged, int K)  // alpha_direct_
     d.anchor_traj = s.total_count;
     d.anchor_charged = charged;
     s.sigma_mode = false;
+    s.adaptive_lambda = false;
     s.al_lambda = 0.002;
     refresh_step_schedule(s);
     fprintf(stderr, "[alpha-direct-pulse-start] charged_iter=%d trajectory_iter=%d alpha=2.000000000000e-03 "
@@ -1255,6 +1639,7 @@ void pulse_finish(DirectPulse &d, State &s, int charged)  // alpha_direct_pulse_
     d.pos_reason = s.reason;
     s.reason = TERMINATION_REASON_UNSPECIFIED;
     s.sigma_mode = false;
+    s.adaptive_lambda = false;
     s.al_lambda = 0.0;
     refresh_step_schedule(s);
     compute_fixed_point_error(s);
@@ -1478,6 +1863,24 @@ ORACLE_API void lp_problem_free(lp_problem_t *p)
     free(p);
 }
 
+void initialize_sigma_parameters(State &s)  // solver.cu:3141-3164
+{
+    const bool sigma_env = getenv("CP_AL_SIGMA") && getenv("CP_AL_SIGMA")[0] != '\0';
+    s.lambda_param = env_double("CP_AL_LAMBDA_PARAM",
+                                env_double("CP_AL_LAMBDA_FACTOR", 2.01) * std::fmax(s.lambda_est, 1e-12));
+    s.lambda_param = std::fmax(s.lambda_param, (2.0 + 1e-12) * std::fmax(s.lambda_est, 1e-12));
+    s.sigma_base = 1.0 / std::sqrt(std::fmax(s.lambda_param, 1e-16));
+    s.sigma_min = env_double("CP_AL_SIGMA_MIN_FACTOR", 1e-4) * s.sigma_base;
+    s.sigma_max = env_double("CP_AL_SIGMA_MAX_FACTOR", 100.0) * s.sigma_base;
+    s.al_sigma = env_double("CP_AL_SIGMA", env_double("CP_AL_SIGMA_INIT_FACTOR", 6.0) * s.sigma_base);
+    s.al_sigma = std::fmin(std::fmax(s.al_sigma, s.sigma_min), s.sigma_max);
+    s.sigma_best = s.al_sigma;
+    s.sigma_err_sum = 0.0;
+    s.sigma_last_err = 0.0;
+    s.sigma_adaptive = env_int("CP_AL_SIGMA_ADAPTIVE", sigma_env ? 0 : 1) != 0;
+    s.sigma_mode = true;
+}
+
 ORACLE_API cp_al_result_t *optimize(const cp_al_parameters_t *params, lp_problem_t *prob)
 {   // solver.cu:1175-1736
     if (!params || !prob) return nullptr;
@@ -1500,21 +1903,22 @@ ORACLE_API cp_al_result_t *optimize(const cp_al_parameters_t *params, lp_problem
     s.primal_weight = params->bound_objective_rescaling ? 1.0 : (s.c_norm + 1.0) / (s.b_norm + 1.0);
     const bool sigma_env = getenv("CP_AL_SIGMA") && getenv("CP_AL_SIGMA")[0] != '\0';
     s.sigma_mode = sigma_env || env_int("CP_AL_SIGMA_MODE", 0) != 0;
-    if (s.sigma_mode) {  // initialize_sigma_parameters, solver.cu:3141-3164
-        s.lambda_param = env_double("CP_AL_LAMBDA_PARAM",
-                                    env_double("CP_AL_LAMBDA_FACTOR", 2.01) * std::fmax(s.lambda_est, 1e-12));
-        s.lambda_param = std::fmax(s.lambda_param, (2.0 + 1e-12) * std::fmax(s.lambda_est, 1e-12));
-        s.sigma_base = 1.0 / std::sqrt(std::fmax(s.lambda_param, 1e-16));
-        s.sigma_min = env_double("CP_AL_SIGMA_MIN_FACTOR", 1e-4) * s.sigma_base;
-        s.sigma_max = env_double("CP_AL_SIGMA_MAX_FACTOR", 100.0) * s.sigma_base;
-        s.al_sigma = env_double("CP_AL_SIGMA", env_double("CP_AL_SIGMA_INIT_FACTOR", 6.0) * s.sigma_base);
-        s.al_sigma = std::fmin(std::fmax(s.al_sigma, s.sigma_min), s.sigma_max);
-        s.sigma_best = s.al_sigma;
-        s.sigma_adaptive = env_int("CP_AL_SIGMA_ADAPTIVE", sigma_env ? 0 : 1) != 0;
-    }
+    if (s.sigma_mode) initialize_sigma_parameters(s);
     s.al_lambda = env_double("CP_AL_LAMBDA", 0.05);
+    initialize_adaptive_lambda(s);
+    if (s.sigma_mode) s.adaptive_lambda = false;
     refresh_step_schedule(s);
     s.best_pw = s.primal_weight;
+    if (params->verbose) {  // solver.cu:3587-3608
+        fprintf(stderr,
+                "[box-cpal] pw_init=%.3e al_lambda=%.3e sigma_mode=%d sigma_adapt=%d al_sigma=%.3e sigma_base=%.3e "
+                "lambda_param=%.3e eta=%.3e refl=%.3e al_penalty=%.3e lambdaA=%.3e adaptive=%d lambda_grid=",
+                s.primal_weight, s.al_lambda, (int)s.sigma_mode, (int)s.sigma_adaptive, s.al_sigma, s.sigma_base,
+                s.lambda_param, s.step_size, params->reflection_coefficient, s.al_penalty, s.lambda_est,
+                (int)s.adaptive_lambda);
+        for (int i = 0; i < s.lambda_num_tiers; ++i) fprintf(stderr, "%s%.3e", i == 0 ? "" : "/", s.lambda_values[i]);
+        fprintf(stderr, "\n");
+    }
 
     AlphaTrial trial;
     trial_init(trial);
@@ -1530,10 +1934,31 @@ ORACLE_API cp_al_result_t *optimize(const cp_al_parameters_t *params, lp_problem
     const auto t_start = std::chrono::steady_clock::now();
     int charged = s.total_count;
     bool do_restart = false;
-    const int K = params->termination_evaluation_frequency;
     const double rho = params->reflection_coefficient;
     const auto &tc = params->termination_criteria;
+    // dynamic evaluation (solver.cu:1228-1260)
+    const int base_eval_frequency = params->termination_evaluation_frequency;
+    int eval_frequency = base_eval_frequency;
+    int dynamic_eval_fast = env_int("CP_AL_DYNAMIC_EVAL_FAST", 50);
+    dynamic_eval_fast = dynamic_eval_fast < 3 ? 3 : dynamic_eval_fast;
+    dynamic_eval_fast = dynamic_eval_fast > eval_frequency ? eval_frequency : dynamic_eval_fast;
+    const bool use_residual_dynamic_eval = env_int("CP_AL_DYNAMIC_EVAL", 0) != 0;
+    const double dynamic_eval_after_sec = env_double("CP_AL_DYNAMIC_EVAL_AFTER_SEC", -1.0);
+    const bool use_time_dynamic_eval = dynamic_eval_after_sec > 0.0;
+    const int dynamic_eval_after_iters = env_int("CP_AL_DYNAMIC_EVAL_AFTER_ITERS", -1);
+    const bool use_iteration_dynamic_eval = dynamic_eval_after_iters > 0;
+    const bool use_dynamic_eval = (use_residual_dynamic_eval || use_time_dynamic_eval || use_iteration_dynamic_eval) &&
+                                  dynamic_eval_fast < eval_frequency;
+    const int dynamic_eval_min_iters =
+        std::max(env_int("CP_AL_DYNAMIC_EVAL_MIN_ITERS", eval_frequency), eval_frequency);
+    const double dynamic_eval_small_res = env_double("CP_AL_DYNAMIC_EVAL_SMALL_RES", 1e-6);
+    const double dynamic_eval_large_res = env_double("CP_AL_DYNAMIC_EVAL_LARGE_RES", 1e-4);
+    const double dynamic_eval_ratio = env_double("CP_AL_DYNAMIC_EVAL_RATIO", 1e4);
+    const int dynamic_eval_bootstrap_iters = std::max(env_int("CP_AL_DYNAMIC_EVAL_BOOTSTRAP_ITERS", 0), 0);
+    bool dynamic_eval_latched = false;
+    if (use_dynamic_eval && dynamic_eval_bootstrap_iters > 0) eval_frequency = dynamic_eval_fast;
     while (charged < tc.iteration_limit) {
+        const int K = eval_frequency;
         if (do_restart) {
             compute_fixed_point_error(s);
             s.fp0 = s.fp;
@@ -1610,8 +2035,63 @@ ORACLE_API cp_al_result_t *optimize(const cp_al_parameters_t *params, lp_problem
             continue;
         }
 
-        do_restart = should_restart(s, params->restart_params, K);
+        const bool sigma_trigger_enabled = !s.sigma_mode && env_int("CP_AL_SIGMA_TRIGGER", 0) != 0;
+        bool sigma_trigger_restart = false;
+        if (sigma_trigger_enabled && s.total_count >= env_int("CP_AL_SIGMA_TRIGGER_MIN_ITERS", 200)) {  // :1496-1528
+            const double rpr = std::fmax(s.rel_p, 1e-300);
+            const double rdr = std::fmax(s.rel_d, 1e-300);
+            const double trigger_primal = env_double("CP_AL_SIGMA_TRIGGER_PRIMAL", 1e-2);
+            const double trigger_dual = env_double("CP_AL_SIGMA_TRIGGER_DUAL", 1e-6);
+            const double trigger_ratio = env_double("CP_AL_SIGMA_TRIGGER_RATIO", 1e6);
+            if (rpr >= trigger_primal && rdr <= trigger_dual && rpr / rdr >= trigger_ratio) {
+                initialize_sigma_parameters(s);
+                s.adaptive_lambda = false;
+                sigma_trigger_restart = true;
+                if (params->verbose)
+                    printf("sigma-trigger iter %d rpr %.12e rdr %.12e ratio %.12e sigma %.12e base %.12e\n",
+                           s.total_count, s.rel_p, s.rel_d, rpr / rdr, s.al_sigma, s.sigma_base);
+            }
+        }
+        if (use_dynamic_eval && !s.sigma_mode &&
+            (s.total_count >= dynamic_eval_min_iters ||
+             (dynamic_eval_bootstrap_iters > 0 && s.total_count >= dynamic_eval_bootstrap_iters))) {  // :1533-1584
+            const double rpr = std::fmax(s.rel_p, 1e-300);
+            const double rdr = std::fmax(s.rel_d, 1e-300);
+            const double min_res = std::fmin(rpr, rdr);
+            const double max_res = std::fmax(rpr, rdr);
+            const double residual_ratio = max_res / min_res;
+            const bool one_sided_residual = min_res <= dynamic_eval_small_res && max_res >= dynamic_eval_large_res;
+            const bool ratio_lag = residual_ratio >= dynamic_eval_ratio && max_res >= dynamic_eval_large_res;
+            const double now = controller_seconds(s);
+            const bool time_gate = use_time_dynamic_eval && !s.cr_early_stopped && now >= dynamic_eval_after_sec;
+            const bool iteration_gate =
+                use_iteration_dynamic_eval && !s.cr_early_stopped && s.total_count >= dynamic_eval_after_iters;
+            if (time_gate || iteration_gate || (use_residual_dynamic_eval && (one_sided_residual || ratio_lag))) {
+                dynamic_eval_latched = true;
+                if (eval_frequency != dynamic_eval_fast) {
+                    eval_frequency = dynamic_eval_fast;
+                    if (params->verbose || use_time_dynamic_eval || use_iteration_dynamic_eval)
+                        printf("[dynamic-eval] iter=%d solve_sec=%.12e eval_frequency=%d->%d relative_primal=%.12e "
+                               "relative_dual=%.12e residual_ratio=%.12e reason=%s\n",
+                               s.total_count, now, K, eval_frequency, s.rel_p, s.rel_d, residual_ratio,
+                               time_gate ? "time" : iteration_gate ? "iteration" : "residual");
+                }
+            } else if (!dynamic_eval_latched && dynamic_eval_bootstrap_iters > 0 &&
+                       eval_frequency == dynamic_eval_fast && s.total_count >= dynamic_eval_bootstrap_iters) {
+                eval_frequency = base_eval_frequency;
+                if (params->verbose)
+                    printf("dynamic-eval iter %d eval_freq %d->%d rpr %.12e rdr %.12e ratio %.12e\n", s.total_count,
+                           K, eval_frequency, s.rel_p, s.rel_d, residual_ratio);
+            }
+        }
+        do_restart = sigma_trigger_restart || should_restart(s, params->restart_params, K);
+        if (!do_restart && s.adaptive_lambda && s.lambda_race && s.lambda_force_restart_iters > 0 &&
+            s.total_count > 0 && s.total_count - s.lambda_last_restart_iter >= s.lambda_force_restart_iters) {
+            do_restart = true;  // :1590-1600
+            if (params->verbose) printf("lambda-race artificial restart iter %d fp %.12e\n", s.total_count, s.fp);
+        }
         if (do_restart) {  // :1631-1695
+            if (params->verbose) printf("restart iter %d fp %.12e\n", s.total_count, s.fp);
             if (trial.active) {
                 const int pos = charged - trial.t_charged;
                 trial.positive_charged += pos;

@@ -1080,8 +1080,8 @@ void BatchSolver::rescale()
         PDAL_HIP(hipMemsetAsync(e.get(), 0, (size_t)m_ * sizeof(double), stream_));
         PDAL_HIP(hipMemsetAsync(d.get(), 0, (size_t)n_ * sizeof(double), stream_));
         for (int it = 0; it < cr_iters; ++it) {
-            cr_log_update_kernel<<<blocks_for_l(m_), kBlock, 0, stream_>>>(A_.view(), d.get(), e.get());
-            cr_log_update_kernel<<<blocks_for_l(n_), kBlock, 0, stream_>>>(At_.view(), e.get(), d.get());
+            cr_log_update_kernel<<<blocks_for_l(m_), kBlock, 0, stream_>>>(A_.view(), d.get(), e.get(), nullptr);
+            cr_log_update_kernel<<<blocks_for_l(n_), kBlock, 0, stream_>>>(At_.view(), e.get(), d.get(), nullptr);
         }
         cr_exp_kernel<<<blocks_for_l(m_), kBlock, 0, stream_>>>(e.get(), inv_e.get(), E_.get(), mi);
         cr_exp_kernel<<<blocks_for_l(n_), kBlock, 0, stream_>>>(d.get(), inv_d.get(), D_.get(), ni);

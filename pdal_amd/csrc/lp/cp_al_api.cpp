@@ -56,22 +56,25 @@ bool env_set(const char *name)
 void warn_unsupported_controllers(bool batch)
 {
     // Controllers of the reference that are switched on only through these
-    // variables (solver.cu:3166-3536 and the loop's dynamic evaluation); the
-    // restart knobs (utils.cu:324-332) and sigma mode (solver.cu:3141-3164) are
-    // honoured, and the single-instance solver also runs the positive-alpha
-    // controllers (solver.cu:189-1057) and the restart-candidate selection
-    // (:2663-2819), which the batched solver does not.
-    static const char *const knobs[] = {"CP_AL_SIGMA_TRIGGER",
-                                        "CP_AL_ADAPTIVE_LAMBDA",
-                                        "CP_AL_LAMBDA_PROFILE",
-                                        "CP_AL_DYNAMIC_EVAL",
-                                        "CP_AL_DYNAMIC_EVAL_AFTER_SEC",
-                                        "CP_AL_DYNAMIC_EVAL_AFTER_ITERS",
-                                        "CP_AL_LOG_FREQ",
-                                        "CP_AL_ALPHA_ROLLBACK_TRIAL",
-                                        "CP_AL_ALPHA_DIRECT_PULSE",
-                                        "CP_AL_RESTART_SELECT"};
-    const size_t count = batch ? sizeof(knobs) / sizeof(knobs[0]) : 7;
+    // variables.  The single-instance solver runs all of them except the dense
+    // per-iteration trace (CP_AL_LOG_FREQ, solver.cu:321-358, a diagnostic that
+    // prints fixed-point errors between evaluations); the batched solver runs
+    // none of them (it keeps the restart knobs, utils.cu:324-332, Curtis-Reid
+    // without its early stop, and sigma mode).
+    static const char *const single[] = {"CP_AL_LOG_FREQ"};
+    static const char *const batched[] = {"CP_AL_LOG_FREQ",
+                                          "CP_AL_SIGMA_TRIGGER",
+                                          "CP_AL_ADAPTIVE_LAMBDA",
+                                          "CP_AL_LAMBDA_PROFILE",
+                                          "CP_AL_DYNAMIC_EVAL",
+                                          "CP_AL_DYNAMIC_EVAL_AFTER_SEC",
+                                          "CP_AL_DYNAMIC_EVAL_AFTER_ITERS",
+                                          "CP_AL_CR_EARLY_STOP",
+                                          "CP_AL_ALPHA_ROLLBACK_TRIAL",
+                                          "CP_AL_ALPHA_DIRECT_PULSE",
+                                          "CP_AL_RESTART_SELECT"};
+    const char *const *knobs = batch ? batched : single;
+    const size_t count = batch ? sizeof(batched) / sizeof(batched[0]) : sizeof(single) / sizeof(single[0]);
     for (size_t i = 0; i < count; ++i) {
         const char *raw = getenv(knobs[i]);
         if (raw && raw[0] != '\0' && strcmp(raw, "0") != 0)

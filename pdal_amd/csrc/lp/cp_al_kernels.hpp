@@ -605,12 +605,14 @@ static __global__ void __launch_bounds__(kBlock)
 }
 
 static __global__ void __launch_bounds__(kBlock)
-    cr_log_update_kernel(CsrView M, const double *__restrict__ other, double *__restrict__ log_scale)
-{
+    cr_log_update_kernel(CsrView M, const double *__restrict__ other, double *__restrict__ log_scale,
+                         double *__restrict__ log_delta)
+{   // log_delta (may be null): this pass's change of each log scale
     const int r = blockIdx.x * kBlock + threadIdx.x;
     if (r >= M.rows) return;
     const int b = M.rp[r], e = M.rp[r + 1];
     if (e - b <= 0) {
+        if (log_delta) log_delta[r] = -log_scale[r];
         log_scale[r] = 0.0;
         return;
     }
@@ -620,7 +622,27 @@ static __global__ void __launch_bounds__(kBlock)
         if (!isfinite(v) || v < 1e-300) v = 1e-300;
         acc += -log(v) - other[M.ci[k]];
     }
-    log_scale[r] = acc / (double)(e - b);
+    const double next = acc / (double)(e - b);
+    if (log_delta) log_delta[r] = next - log_scale[r];
+    log_scale[r] = next;
+}
+
+// Largest change of a log edge weight, |row change + column change|, over the
+// nonzeros of M in one Curtis-Reid pass: per-block maxima into parts[block]
+// (preconditioner.cu:63-80 and the max reduction at :222-235).
+static __global__ void __launch_bounds__(kBlock)
+    cr_edge_change_kernel(CsrView M, const double *__restrict__ row_delta, const double *__restrict__ col_delta,
+                          double *__restrict__ parts)
+{
+    double q[1] = {0.0};
+    for (int r = blockIdx.x * kBlock + threadIdx.x; r < M.rows; r += gridDim.x * kBlock) {
+        const double dr = row_delta[r];
+        for (int k = M.rp[r]; k < M.rp[r + 1]; ++k) {
+            const double ch = fabs(dr + col_delta[M.ci[k]]);
+            if (isfinite(ch) && ch > q[0]) q[0] = ch;
+        }
+    }
+    block_reduce_store<0, 1>(q, parts + blockIdx.x);
 }
 
 static __global__ void __launch_bounds__(kBlock)

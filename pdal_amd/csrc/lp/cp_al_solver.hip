@@ -19,6 +19,7 @@
 
 #include "../common/csr_host.hpp"
 #include "../common/hip_util.hpp"
+#include "alpha_tiers.hpp"
 #include "cp_al_kernels.hpp"
 
 #include <algorithm>
@@ -85,6 +86,9 @@ class Solver {
     void perform_restart(const Eval &e);
     void refresh_step_schedule();
     void init_sigma();
+    void set_sigma_parameters();
+    void init_alpha();
+    double controller_clock() const;
     double delta_normal_motion(const double *rx);
     cp_al_result_t *make_result(const lp_problem_t *original, int charged);
     void display() const;
@@ -131,6 +135,8 @@ class Solver {
     bool sigma_mode_ = false, sigma_adaptive_ = false;
     double al_sigma_ = 0.0, sigma_base_ = 0.0, sigma_min_ = 0.0, sigma_max_ = 0.0, sigma_best_ = 0.0;
     double sigma_err_sum_ = 0.0, sigma_last_err_ = 0.0, lambda_param_ = 0.0;
+    AlphaTiers tiers_;    // adaptive alpha tiers (solver.cu:3166-3536)
+    bool cr_early_stopped_ = false;  // Curtis-Reid stopped at its fifth pass (preconditioner.cu:244-252)
     double pw_err_sum_ = 0.0, pw_last_err_ = 0.0, best_pw_ = 1.0, best_gap_ = INFINITY;
     int total_ = 0, inner_ = 0;
     double fp_ = 0.0, fp0_ = 0.0, last_trial_fp_ = INFINITY;
@@ -156,6 +162,7 @@ class Solver {
             sigma_last_err, pw_err_sum, pw_last_err, best_pw, best_gap, fp, fp0, last_trial_fp, abs_p, rel_p, abs_d,
             rel_d, p_obj, d_obj, gap, rel_gap, score_current, score_average, score_halpern;
         bool sigma_mode, fp_stale;
+        AlphaTiers tiers;
         int total, inner, restart_target;
         termination_reason_t reason;
         Eval last_eval;
@@ -305,13 +312,41 @@ void Solver::rescale()
     const char *cr_iters_raw = getenv("CP_AL_CR_ITERS");
     if (cr_iters > 0 && cr_iters_raw && cr_iters_raw[0] != '\0') cr_iters = atoi(cr_iters_raw);
     cr_iters = std::max(cr_iters, 0);
+    cr_early_stopped_ = false;
     if (cr_iters > 0) {  // curtis_reid_rescaling, :162-268
         if (P_.verbose) printf("  Curtis-Reid scaling (%d iterations)\n", cr_iters);
+        // the edge-change diagnostic and the early stop after five passes (:171-178, :217-252)
+        const char *diag_raw = getenv("CP_AL_CR_DIAGNOSTIC");
+        const char *early_raw = getenv("CP_AL_CR_EARLY_STOP");
+        const char *thr_raw = getenv("CP_AL_CR_EARLY_STOP_THRESHOLD");
+        const bool diag = diag_raw && atoi(diag_raw) != 0;
+        const bool early = early_raw && atoi(early_raw) != 0 && cr_iters == 20;
+        const double thr = thr_raw ? atof(thr_raw) : 0.1;
+        const bool measure = diag || early;
+        const int egrid = std::max(1, std::min(1024, blocks_for(m_)));
+        DBuf<double> eparts(measure ? egrid : 0), emax(measure ? 1 : 0);
         PDAL_HIP(hipMemsetAsync(e.get(), 0, (size_t)m_ * sizeof(double), stream_));
         PDAL_HIP(hipMemsetAsync(d.get(), 0, (size_t)n_ * sizeof(double), stream_));
         for (int it = 0; it < cr_iters; ++it) {
-            cr_log_update_kernel<<<blocks_for(m_), kBlock, 0, stream_>>>(A_.view(), d.get(), e.get());
-            cr_log_update_kernel<<<blocks_for(n_), kBlock, 0, stream_>>>(At_.view(), e.get(), d.get());
+            cr_log_update_kernel<<<blocks_for(m_), kBlock, 0, stream_>>>(A_.view(), d.get(), e.get(),
+                                                                         measure ? inv_e.get() : nullptr);
+            cr_log_update_kernel<<<blocks_for(n_), kBlock, 0, stream_>>>(At_.view(), e.get(), d.get(),
+                                                                         measure ? inv_d.get() : nullptr);
+            PDAL_LAUNCHED();
+            if (!measure) continue;
+            cr_edge_change_kernel<<<egrid, kBlock, 0, stream_>>>(A_.view(), inv_e.get(), inv_d.get(), eparts.get());
+            reduce_partials_kernel<<<1, kBlock, 0, stream_>>>(eparts.get(), egrid, 1, 0, 1, emax.get());
+            PDAL_LAUNCHED();
+            double mx = 0.0;
+            emax.download(&mx, 1, stream_);
+            PDAL_HIP(hipStreamSynchronize(stream_));
+            if (diag) fprintf(stderr, "[cr-diagnostic] iteration=%d max_edge_log_change=%.12e\n", it + 1, mx);
+            if (early && it + 1 == 5 && mx <= thr) {
+                cr_early_stopped_ = true;
+                printf("[cr-early-stop] selected=%d executed=%d max_edge_log_change=%.12e threshold=%.12e\n",
+                       cr_iters, it + 1, mx, thr);
+                break;
+            }
         }
         cr_exp_kernel<<<blocks_for(m_), kBlock, 0, stream_>>>(e.get(), inv_e.get(), E_.get(), m_);
         cr_exp_kernel<<<blocks_for(n_), kBlock, 0, stream_>>>(d.get(), inv_d.get(), D_.get(), n_);
@@ -544,13 +579,18 @@ void Solver::setup_sliced()
 }
 
 void Solver::init_sigma()
-{   // sigma mode on CP_AL_SIGMA or CP_AL_SIGMA_MODE (solver.cu:3563-3578), parameters :3141-3164
+{   // sigma mode on CP_AL_SIGMA or CP_AL_SIGMA_MODE (solver.cu:3563-3578)
     const bool sigma_env = env_set("CP_AL_SIGMA");
     sigma_mode_ = sigma_env || env_int("CP_AL_SIGMA_MODE", 0) != 0;
     al_sigma_ = sigma_base_ = sigma_min_ = sigma_max_ = sigma_best_ = 0.0;
     sigma_err_sum_ = sigma_last_err_ = lambda_param_ = 0.0;
     sigma_adaptive_ = false;
-    if (!sigma_mode_) return;
+    if (sigma_mode_) set_sigma_parameters();
+}
+
+void Solver::set_sigma_parameters()
+{   // initialize_sigma_parameters, solver.cu:3141-3164 (also the sigma trigger's switch, :1511)
+    const bool sigma_env = env_set("CP_AL_SIGMA");
     lambda_param_ = env_double("CP_AL_LAMBDA_PARAM",
                                env_double("CP_AL_LAMBDA_FACTOR", 2.01) * std::fmax(lambda_est_, 1e-12));
     lambda_param_ = std::fmax(lambda_param_, (2.0 + 1e-12) * std::fmax(lambda_est_, 1e-12));
@@ -560,7 +600,27 @@ void Solver::init_sigma()
     al_sigma_ = env_double("CP_AL_SIGMA", env_double("CP_AL_SIGMA_INIT_FACTOR", 6.0) * sigma_base_);
     al_sigma_ = std::fmin(std::fmax(al_sigma_, sigma_min_), sigma_max_);
     sigma_best_ = al_sigma_;
+    sigma_err_sum_ = 0.0;
+    sigma_last_err_ = 0.0;
     sigma_adaptive_ = env_int("CP_AL_SIGMA_ADAPTIVE", sigma_env ? 0 : 1) != 0;
+    sigma_mode_ = true;
+}
+
+double Solver::controller_clock() const
+{   // The reference's cumulative_time_sec (utils.cu:301) as the adaptive-alpha
+    // epochs and the dynamic evaluation's time gate read it.
+    // PDAL_CONTROLLER_SECONDS_PER_ITERATION > 0 replaces it by the iteration
+    // count times that value, so that a test sees the same decisions in every run.
+    const double per_iter = env_double("PDAL_CONTROLLER_SECONDS_PER_ITERATION", 0.0);
+    return per_iter > 0.0 ? per_iter * (double)total_ : solve_time_;
+}
+
+void Solver::init_alpha()
+{   // the alpha part of initialize_step_size_and_primal_weight, solver.cu:3579-3584
+    al_lambda_ = env_double("CP_AL_LAMBDA", 0.05);
+    al_lambda_ = tiers_.init(al_lambda_, controller_clock());
+    if (sigma_mode_) tiers_.adaptive = false;
+    refresh_step_schedule();
 }
 
 void Solver::refresh_step_schedule()
@@ -603,16 +663,18 @@ void Solver::init_steps()
     }
     pw_ = P_.bound_objective_rescaling ? 1.0 : (c_norm_ + 1.0) / (b_norm_ + 1.0);
     init_sigma();
-    al_lambda_ = env_double("CP_AL_LAMBDA", 0.05);
-    refresh_step_schedule();
+    init_alpha();
     best_pw_ = pw_;
     warn_unsupported_controllers(false);
-    if (P_.verbose)
+    if (P_.verbose) {  // solver.cu:3587-3608
         fprintf(stderr,
                 "[box-cpal] pw_init=%.3e al_lambda=%.3e sigma_mode=%d sigma_adapt=%d al_sigma=%.3e sigma_base=%.3e "
-                "lambda_param=%.3e eta=%.3e refl=%.3e al_penalty=%.3e lambdaA=%.3e\n",
+                "lambda_param=%.3e eta=%.3e refl=%.3e al_penalty=%.3e lambdaA=%.3e adaptive=%d lambda_grid=",
                 pw_, al_lambda_, (int)sigma_mode_, (int)sigma_adaptive_, al_sigma_, sigma_base_, lambda_param_,
-                step_size_, P_.reflection_coefficient, al_penalty_, lambda_est_);
+                step_size_, P_.reflection_coefficient, al_penalty_, lambda_est_, (int)tiers_.adaptive);
+        for (int i = 0; i < tiers_.num; ++i) fprintf(stderr, "%s%.3e", i == 0 ? "" : "/", tiers_.values[i]);
+        fprintf(stderr, "\n");
+    }
 }
 
 void Solver::setup()
@@ -673,9 +735,10 @@ void Solver::reset_iterates()
     for (auto *b : {&xR_, &dslack_}) PDAL_HIP(hipMemsetAsync(b->get(), 0, (size_t)n_ * sizeof(double), stream_));
     PDAL_HIP(hipMemsetAsync(yR_.get(), 0, (size_t)m_ * sizeof(double), stream_));
     pw_ = pw_init_;
+    total_ = 0;
+    solve_time_ = 0.0;
     init_sigma();
-    al_lambda_ = env_double("CP_AL_LAMBDA", 0.05);
-    refresh_step_schedule();
+    init_alpha();
     best_pw_ = pw_;
     pw_err_sum_ = 0.0;
     pw_last_err_ = 0.0;
@@ -990,6 +1053,8 @@ void Solver::perform_restart(const Eval &e)
             best_gap_ = g;
             best_pw_ = pw_;
         }
+        al_lambda_ = tiers_.on_restart(al_lambda_, rel_p_, rel_d_, rel_gap_, fp_, total_, controller_clock(),
+                                       P_.verbose);  // :2996
     }
     refresh_step_schedule();
     auto copy = [&](double *dst, const double *src, int len) {
@@ -1015,6 +1080,7 @@ void Solver::perform_restart(const Eval &e)
                                                                               yT_.get(), y0_.get(), y_.get(), m_);
         PDAL_LAUNCHED();
     }
+    tiers_.last_restart_iter = total_;  // :3098
     inner_ = 0;
     last_trial_fp_ = INFINITY;
 }
@@ -1143,6 +1209,7 @@ Solver::Scalars Solver::save_scalars() const
     v.score_halpern = score_halpern_;
     v.sigma_mode = sigma_mode_;
     v.fp_stale = fp_stale_;
+    v.tiers = tiers_;
     v.total = total_;
     v.inner = inner_;
     v.restart_target = restart_target_;
@@ -1182,6 +1249,7 @@ void Solver::load_scalars(const Scalars &v)
     score_halpern_ = v.score_halpern;
     sigma_mode_ = v.sigma_mode;
     fp_stale_ = v.fp_stale;
+    tiers_ = v.tiers;
     total_ = v.total;
     inner_ = v.inner;
     restart_target_ = v.restart_target;
@@ -1304,6 +1372,7 @@ void Solver::trial_start(int charged)
     t.max_iters = 2 * std::max(t.hist[0].iterations, t.hist[1].iterations);
     t.max_sec = 2.0 * std::fmax(2.0, std::fmax(t.hist[0].seconds, t.hist[1].seconds));
     sigma_mode_ = false;
+    tiers_.adaptive = false;
     al_lambda_ = 0.002;
     refresh_step_schedule();
     refresh_fp();
@@ -1393,6 +1462,7 @@ void Solver::pulse_start(int charged, int K)
     d.phase = kPulsePositive;
     d.pulse_iters = K;
     sigma_mode_ = false;
+    tiers_.adaptive = false;
     al_lambda_ = 0.002;
     refresh_step_schedule();
     fprintf(stderr,
@@ -1411,6 +1481,7 @@ void Solver::pulse_finish(int charged)
     d.pos_reason = reason_;
     reason_ = TERMINATION_REASON_UNSPECIFIED;
     sigma_mode_ = false;
+    tiers_.adaptive = false;
     al_lambda_ = 0.0;
     refresh_step_schedule();
     refresh_fp();
@@ -1520,9 +1591,31 @@ cp_al_result_t *Solver::solve_once(const lp_problem_t *original)
     const auto t_start = std::chrono::steady_clock::now();
     int charged = total_;
     bool do_restart = false;
-    const int K = P_.termination_evaluation_frequency;
     const termination_criteria_t &tc = P_.termination_criteria;
     bool have_eval = false;
+    // dynamic evaluation frequency (solver.cu:1228-1260, 1533-1584)
+    const int base_k = P_.termination_evaluation_frequency;
+    int eval_k = base_k;
+    const int fast_k = std::min(std::max(env_int("CP_AL_DYNAMIC_EVAL_FAST", 50), 3), base_k);
+    const bool dyn_residual = env_int("CP_AL_DYNAMIC_EVAL", 0) != 0;
+    const double dyn_after_sec = env_double("CP_AL_DYNAMIC_EVAL_AFTER_SEC", -1.0);
+    const int dyn_after_iters = env_int("CP_AL_DYNAMIC_EVAL_AFTER_ITERS", -1);
+    const bool dyn_time = dyn_after_sec > 0.0, dyn_iters = dyn_after_iters > 0;
+    const bool dyn = (dyn_residual || dyn_time || dyn_iters) && fast_k < base_k;
+    const int dyn_min_iters = std::max(env_int("CP_AL_DYNAMIC_EVAL_MIN_ITERS", base_k), base_k);
+    const double dyn_small = env_double("CP_AL_DYNAMIC_EVAL_SMALL_RES", 1e-6);
+    const double dyn_large = env_double("CP_AL_DYNAMIC_EVAL_LARGE_RES", 1e-4);
+    const double dyn_ratio = env_double("CP_AL_DYNAMIC_EVAL_RATIO", 1e4);
+    const int dyn_bootstrap = std::max(env_int("CP_AL_DYNAMIC_EVAL_BOOTSTRAP_ITERS", 0), 0);
+    bool dyn_latched = false;
+    if (dyn && dyn_bootstrap > 0) eval_k = fast_k;
+    // sigma trigger (solver.cu:1496-1528): switch to sigma mode once the primal
+    // residual lags the dual one by the given ratio
+    const bool trigger_on = env_int("CP_AL_SIGMA_TRIGGER", 0) != 0;
+    const int trigger_min_iters = env_int("CP_AL_SIGMA_TRIGGER_MIN_ITERS", 200);
+    const double trigger_primal = env_double("CP_AL_SIGMA_TRIGGER_PRIMAL", 1e-2);
+    const double trigger_dual = env_double("CP_AL_SIGMA_TRIGGER_DUAL", 1e-6);
+    const double trigger_ratio = env_double("CP_AL_SIGMA_TRIGGER_RATIO", 1e6);
     auto positive_charge = [&]() {
         const int pos = charged - trial_.t_charged;
         trial_.positive_charged += pos;
@@ -1530,6 +1623,7 @@ cp_al_result_t *Solver::solve_once(const lp_problem_t *original)
         return pos;
     };
     while (charged < tc.iteration_limit) {
+        const int K = eval_k;
         if (do_restart) {  // :1279-1284
             refresh_fp();
             fp0_ = fp_;
@@ -1612,7 +1706,49 @@ cp_al_result_t *Solver::solve_once(const lp_problem_t *original)
             do_restart = true;
             continue;
         }
-        do_restart = should_restart(K);
+        bool trigger_restart = false;
+        if (trigger_on && !sigma_mode_ && total_ >= trigger_min_iters) {  // :1496-1528
+            const double rp = std::fmax(rel_p_, 1e-300), rd = std::fmax(rel_d_, 1e-300);
+            if (rp >= trigger_primal && rd <= trigger_dual && rp / rd >= trigger_ratio) {
+                set_sigma_parameters();
+                tiers_.adaptive = false;
+                trigger_restart = true;
+                if (P_.verbose)
+                    printf("sigma-trigger iter %d rpr %.12e rdr %.12e ratio %.12e sigma %.12e base %.12e\n", total_,
+                           rel_p_, rel_d_, rp / rd, al_sigma_, sigma_base_);
+            }
+        }
+        if (dyn && !sigma_mode_ && (total_ >= dyn_min_iters || (dyn_bootstrap > 0 && total_ >= dyn_bootstrap))) {
+            // :1533-1584
+            const double rp = std::fmax(rel_p_, 1e-300), rd = std::fmax(rel_d_, 1e-300);
+            const double lo = std::fmin(rp, rd), hi = std::fmax(rp, rd), ratio = hi / lo;
+            const bool one_sided = lo <= dyn_small && hi >= dyn_large;
+            const bool lag = ratio >= dyn_ratio && hi >= dyn_large;
+            const double now = controller_clock();
+            const bool time_gate = dyn_time && !cr_early_stopped_ && now >= dyn_after_sec;
+            const bool iter_gate = dyn_iters && !cr_early_stopped_ && total_ >= dyn_after_iters;
+            if (time_gate || iter_gate || (dyn_residual && (one_sided || lag))) {
+                dyn_latched = true;
+                if (eval_k != fast_k) {
+                    eval_k = fast_k;
+                    if (P_.verbose || dyn_time || dyn_iters)
+                        printf("[dynamic-eval] iter=%d solve_sec=%.12e eval_frequency=%d->%d relative_primal=%.12e "
+                               "relative_dual=%.12e residual_ratio=%.12e reason=%s\n",
+                               total_, now, K, eval_k, rel_p_, rel_d_, ratio,
+                               time_gate ? "time" : iter_gate ? "iteration" : "residual");
+                }
+            } else if (!dyn_latched && dyn_bootstrap > 0 && eval_k == fast_k && total_ >= dyn_bootstrap) {
+                eval_k = base_k;
+                if (P_.verbose)
+                    printf("dynamic-eval iter %d eval_freq %d->%d rpr %.12e rdr %.12e ratio %.12e\n", total_, K,
+                           eval_k, rel_p_, rel_d_, ratio);
+            }
+        }
+        do_restart = trigger_restart || should_restart(K);
+        if (!do_restart && tiers_.force_restart(total_)) {  // :1590-1600
+            do_restart = true;
+            if (P_.verbose) printf("lambda-race artificial restart iter %d fp %.12e\n", total_, fp_);
+        }
         if (do_restart) {  // :1631-1695
             if (P_.verbose) printf("restart iter %d fp %.12e\n", total_, fp_);
             if (trial_.active) {

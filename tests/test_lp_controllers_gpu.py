@@ -1,0 +1,147 @@
+"""The environment-driven controllers of the CP_AL loop on the GPU, held to the
+oracle: the sigma trigger (LP/CP_AL/src/solver.cu:1496-1528), the dynamic
+evaluation frequency (:1228-1260, 1533-1584), adaptive lambda's tier ladder and
+race (:3166-3536, forced restart :1590-1600) and Curtis-Reid's early stop
+(LP/CP_AL/src/preconditioner.cu:171-252).  The oracle's side of each controller
+is pinned in tests/test_lp_controllers.py.
+
+Every scenario compares the controller's log lines (the iterations where it
+acted, its integer state and decisions; its floating-point inputs to 1e-6) and
+the iterates after a fixed iteration count.  The race scores per second of
+solve time; both sides read PDAL_CONTROLLER_SECONDS_PER_ITERATION so that the
+race's decisions do not depend on the wall clock.
+"""
+from __future__ import annotations
+
+import ctypes
+import re
+
+import numpy as np
+import pytest
+
+from pdal_amd.generators import planted_lp
+
+pytestmark = pytest.mark.gpu
+
+LIBC = ctypes.CDLL(None)
+ITER_RTOL = 1e-9  # a few thousand iterations through restarts and controller switches
+
+TIERS = {"CP_AL_LAMBDA": "0", "CP_AL_ADAPTIVE_LAMBDA": "1", "CP_AL_LAMBDA_GRID": "0,0.02,0.05,0.1,0.2",
+         "CP_AL_LAMBDA_FEAS_ACTIVE": "0", "CP_AL_LAMBDA_GAP_ACTIVE": "0", "CP_AL_LAMBDA_FP_ACTIVE": "0"}
+RACE = {"CP_AL_LAMBDA": "0", "CP_AL_ADAPTIVE_LAMBDA": "1", "CP_AL_LAMBDA_RACE": "1",
+        "CP_AL_LAMBDA_GRID": "0,0.02,0.05", "CP_AL_LAMBDA_RACE_EXPLOIT_EPOCHS": "1",
+        "CP_AL_LAMBDA_FEAS_ACTIVE": "0", "CP_AL_LAMBDA_GAP_ACTIVE": "0", "CP_AL_LAMBDA_FP_ACTIVE": "0",
+        "PDAL_CONTROLLER_SECONDS_PER_ITERATION": "1e-4"}
+PROFILE = {"CP_AL_LAMBDA": "0", "CP_AL_LAMBDA_PROFILE": "grid4_race", "PDAL_CONTROLLER_SECONDS_PER_ITERATION": "1e-4"}
+TRIGGER = {"CP_AL_LAMBDA": "0", "CP_AL_SIGMA_TRIGGER": "1", "CP_AL_SIGMA_TRIGGER_MIN_ITERS": "250",
+           "CP_AL_SIGMA_TRIGGER_PRIMAL": "0", "CP_AL_SIGMA_TRIGGER_DUAL": "1", "CP_AL_SIGMA_TRIGGER_RATIO": "0"}
+DYN_ITER = {"CP_AL_LAMBDA": "0", "CP_AL_DYNAMIC_EVAL_AFTER_ITERS": "500"}
+DYN_TIME = {"CP_AL_LAMBDA": "0", "CP_AL_DYNAMIC_EVAL_AFTER_SEC": "0.45", "PDAL_CONTROLLER_SECONDS_PER_ITERATION": "1e-3"}
+DYN_RES = {"CP_AL_LAMBDA": "0", "CP_AL_DYNAMIC_EVAL": "1", "CP_AL_DYNAMIC_EVAL_SMALL_RES": "1",
+           "CP_AL_DYNAMIC_EVAL_LARGE_RES": "0", "CP_AL_DYNAMIC_EVAL_MIN_ITERS": "500"}
+DYN_BOOT = {"CP_AL_LAMBDA": "0", "CP_AL_DYNAMIC_EVAL": "1", "CP_AL_DYNAMIC_EVAL_BOOTSTRAP_ITERS": "400",
+            "CP_AL_DYNAMIC_EVAL_SMALL_RES": "0", "CP_AL_DYNAMIC_EVAL_LARGE_RES": "1e300",
+            "CP_AL_DYNAMIC_EVAL_RATIO": "1e300"}
+CR_STOP = {"CP_AL_LAMBDA": "0", "CP_AL_CR_SCALING": "1", "CP_AL_CR_EARLY_STOP": "1", "CP_AL_CR_DIAGNOSTIC": "1",
+           "CP_AL_CR_EARLY_STOP_THRESHOLD": "100", "CP_AL_DYNAMIC_EVAL_AFTER_ITERS": "300"}
+
+# controller log lines: (regex, groups compared exactly, groups compared as floats); floats printed
+# with four significant digits (%.3e) may round apart, so they get 2e-3, the others 1e-6
+LINES = {
+    "restart": (r"^restart iter (\d+) fp (\S+)$", (0,), (1,)),
+    "lambda-adapt": (r"lambda-adapt restart (\d+) ratios\(feas (\S+) gap (\S+) fp (\S+)\) (plateau\(.*\)) "
+                     r"tier (\d+)->(\d+) hold (\d+) lambda (\S+)->(\S+)", (0, 4, 5, 6, 7, 8, 9), ()),
+    "lambda-race": (r"lambda-race restart (\d+) score (\S+) best\(tier=(\d+) score=\S+\) cycle\(tier=(\d+) "
+                    r"score=\S+\) tier (\d+)->(\d+) lambda (\S+)->(\S+) next=(\d+) exploit=(\d+) cool=(\d+) "
+                    r"bad=(\d) start=(\d) mature=(\d)", (0, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13), (1,), 2e-3),
+    "forced": (r"lambda-race artificial restart iter (\d+)", (0,), ()),
+    "trigger": (r"sigma-trigger iter (\d+) rpr (\S+) rdr (\S+) ratio \S+ sigma (\S+) base (\S+)", (0,), (1, 2, 3, 4)),
+    "dynamic": (r"\[dynamic-eval\] iter=(\d+) solve_sec=\S+ eval_frequency=(\d+)->(\d+) .*reason=(\w+)",
+                (0, 1, 2, 3), ()),
+    "boot": (r"^dynamic-eval iter (\d+) eval_freq (\d+)->(\d+)", (0, 1, 2), ()),
+    "cr-stop": (r"\[cr-early-stop\] selected=(\d+) executed=(\d+) max_edge_log_change=(\S+)", (0, 1), (2,)),
+    "cr-diag": (r"\[cr-diagnostic\] iteration=(\d+) max_edge_log_change=(\S+)", (0,), (1,)),
+}
+
+
+def rel_err(a, b):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def run(lib, inst, capfd, env, monkeypatch, **params):
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    lib.reset_rng()
+    capfd.readouterr()
+    r = lib.solve(**inst.lp_args(), params=lib.default_params(verbose=True, **params))
+    LIBC.fflush(None)
+    out = capfd.readouterr()
+    for k in env:
+        monkeypatch.delenv(k, raising=False)
+    return r, out.out + out.err
+
+
+def compare_logs(lo, lg, tags):
+    for tag in tags:
+        pat, exact, approx, *tol = LINES[tag]
+        tol = tol[0] if tol else 1e-6
+        mo = [m.groups() for m in re.finditer(pat, lo, re.M)]
+        mg = [m.groups() for m in re.finditer(pat, lg, re.M)]
+        assert len(mo) == len(mg), (tag, len(mo), len(mg))
+        for a, b in zip(mo, mg):
+            assert [a[i] for i in exact] == [b[i] for i in exact], (tag, a, b)
+            for i in approx:
+                x, y = float(a[i]), float(b[i])
+                assert abs(x - y) <= tol * max(abs(x), abs(y), 1e-300), (tag, a, b)
+    return len(re.findall(LINES[tags[0]][0], lo, re.M))
+
+
+SCENARIOS = {
+    "tiers": (TIERS, dict(iteration_limit=4000, termination_evaluation_frequency=100), ("restart", "lambda-adapt")),
+    "race": (RACE, dict(iteration_limit=4000, termination_evaluation_frequency=100), ("restart", "lambda-race")),
+    "grid4_race": (PROFILE, dict(iteration_limit=4000, termination_evaluation_frequency=100),
+                   ("restart", "lambda-race")),
+    "forced_restart": (dict(RACE, CP_AL_LAMBDA_FORCE_RESTART_ITERS="300", CP_AL_RESTART_SUFFICIENT="0",
+                            CP_AL_RESTART_NECESSARY="0", CP_AL_RESTART_ARTIFICIAL="1e9"),
+                       dict(iteration_limit=2000, termination_evaluation_frequency=100), ("forced", "restart")),
+    "trigger": (TRIGGER, dict(iteration_limit=2000, termination_evaluation_frequency=100), ("trigger", "restart")),
+    "dyn_iteration": (DYN_ITER, dict(iteration_limit=1500, termination_evaluation_frequency=200),
+                      ("dynamic", "restart")),
+    "dyn_time": (DYN_TIME, dict(iteration_limit=1500, termination_evaluation_frequency=200), ("dynamic", "restart")),
+    "dyn_residual": (DYN_RES, dict(iteration_limit=1500, termination_evaluation_frequency=200),
+                     ("dynamic", "restart")),
+    "dyn_bootstrap": (DYN_BOOT, dict(iteration_limit=1000, termination_evaluation_frequency=200),
+                      ("boot", "restart")),
+    "cr_early_stop": (CR_STOP, dict(iteration_limit=1000, termination_evaluation_frequency=200),
+                      ("cr-stop", "cr-diag", "restart")),
+}
+
+
+@pytest.mark.parametrize("name", list(SCENARIOS))
+def test_controller_matches_oracle(oracle_lp, gpu_lp, capfd, monkeypatch, name):
+    env, params, tags = SCENARIOS[name]
+    inst = planted_lp(300, 600, 8, seed=3)
+    p = dict(eps_optimal_relative=0.0, eps_feasible_relative=0.0, **params)
+    ro, lo = run(oracle_lp, inst, capfd, env, monkeypatch, **p)
+    rg, lg = run(gpu_lp, inst, capfd, env, monkeypatch, **p)
+    count = compare_logs(lo, lg, tags)
+    assert count >= 1, f"{name}: the controller never acted"
+    assert ro.iterations == rg.iterations and ro.status == rg.status
+    ex, ey = rel_err(rg.x, ro.x), rel_err(rg.y, ro.y)
+    print(f"{name}: {count} controller lines, x rel err {ex:.2e}, y rel err {ey:.2e}")
+    assert ex <= ITER_RTOL and ey <= ITER_RTOL
+
+
+@pytest.mark.parametrize("name", ["tiers", "race", "trigger", "dyn_residual"])
+def test_controller_solves_match_oracle(oracle_lp, gpu_lp, capfd, monkeypatch, name):
+    env, params, _ = SCENARIOS[name]
+    inst = planted_lp(300, 600, 8, seed=3)
+    p = dict(params, iteration_limit=100000, eps_optimal_relative=1e-8, eps_feasible_relative=1e-8)
+    ro, _ = run(oracle_lp, inst, capfd, env, monkeypatch, **p)
+    rg, _ = run(gpu_lp, inst, capfd, env, monkeypatch, **p)
+    assert ro.status == rg.status == "OPTIMAL"
+    assert ro.iterations == rg.iterations
+    assert abs(rg.primal_objective - ro.primal_objective) <= 1e-8 * (1 + abs(ro.primal_objective))
+    opt = inst.optimal_value
+    assert abs(rg.primal_objective - opt) <= 1e-6 * (1 + abs(opt))
