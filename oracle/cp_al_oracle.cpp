@@ -995,6 +995,14 @@ void perform_restart(State &s, const cp_al_parameters_t *p)  // solver.cu:2821-3
         update_adaptive_lambda_on_restart(s, p->verbose);  // :2996
     }
     refresh_step_schedule(s);
+    const int select_mode = env_int("CP_AL_RESTART_SELECT", 0);
+    if (select_mode != 0 && (p->verbose || env_int("CP_AL_RESTART_SELECT_TRACE", 0) != 0))  // :3000-3015
+        fprintf(stderr,
+                "[restart-select] iter=%d target=%s score_current=%.12e score_reflected_halpern=%.12e "
+                "score_halpern=%.12e candidate=%.12e margin=%.6e mode=%d\n",
+                s.total_count, target == 1 ? "reflected_halpern" : target == 2 ? "halpern" : "current",
+                s.score_current, s.score_average, s.score_halpern, s.fp,
+                env_double("CP_AL_RESTART_SELECT_MARGIN", 1.0), select_mode);
     if (target == 1) {  // :3019-3043: anchor = T output = the current iterate, reflected = old T output
         s.xR = s.xT;
         s.yR = s.yT;

@@ -1057,6 +1057,13 @@ void Solver::perform_restart(const Eval &e)
                                        P_.verbose);  // :2996
     }
     refresh_step_schedule();
+    const int select_mode = env_int("CP_AL_RESTART_SELECT", 0);
+    if (select_mode != 0 && (P_.verbose || env_int("CP_AL_RESTART_SELECT_TRACE", 0) != 0))  // :3000-3015
+        fprintf(stderr,
+                "[restart-select] iter=%d target=%s score_current=%.12e score_reflected_halpern=%.12e "
+                "score_halpern=%.12e candidate=%.12e margin=%.6e mode=%d\n",
+                total_, target == 1 ? "reflected_halpern" : target == 2 ? "halpern" : "current", score_current_,
+                score_average_, score_halpern_, fp_, env_double("CP_AL_RESTART_SELECT_MARGIN", 1.0), select_mode);
     auto copy = [&](double *dst, const double *src, int len) {
         if (len > 0)
             PDAL_HIP(hipMemcpyAsync(dst, src, (size_t)len * sizeof(double), hipMemcpyDeviceToDevice, stream_));

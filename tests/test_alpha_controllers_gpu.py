@@ -162,11 +162,20 @@ def test_restart_selection_matches_oracle(oracle_lp, gpu_lp, monkeypatch, capfd,
     monkeypatch.setenv("CP_AL_LAMBDA", "0")
     monkeypatch.setenv("CP_AL_RESTART_SELECT", mode)
     monkeypatch.setenv("CP_AL_RESTART_SELECT_MARGIN", "0.9")
+    monkeypatch.setenv("CP_AL_RESTART_SELECT_TRACE", "1")
     inst = planted_lp(300, 600, 8, seed=7)
     p = oracle_lp.default_params(termination_evaluation_frequency=64, iteration_limit=100000,
                                  eps_optimal_relative=1e-8, eps_feasible_relative=1e-8)
-    ro, _ = run(oracle_lp, inst, p, capfd)
-    rg, _ = run(gpu_lp, inst, p, capfd)
+    ro, lo = run(oracle_lp, inst, p, capfd)
+    rg, lg = run(gpu_lp, inst, p, capfd)
+    # every restart's candidates, scores and choice (the scores are pinned in closed form on the oracle)
+    so, sg = fields(lo, "restart-select"), fields(lg, "restart-select")
+    assert so and len(so) == len(sg)
+    for a, b in zip(so, sg):
+        assert (a["iter"], a["target"]) == (b["iter"], b["target"])
+        for k in ("score_current", "score_reflected_halpern", "score_halpern", "candidate"):
+            x, y = float(a[k]), float(b[k])
+            assert x == y or abs(x - y) <= 1e-6 * max(abs(x), abs(y)), (k, a, b)
     assert ro.status == rg.status == "OPTIMAL" and ro.iterations == rg.iterations
     assert rel_err(rg.x, ro.x) <= 1e-6
     assert abs(rg.primal_objective - inst.optimal_value) <= 1e-6 * (1 + abs(inst.optimal_value))
