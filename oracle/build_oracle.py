@@ -6,8 +6,9 @@ summation-order noise floor of a trajectory) and liblin_cp_al_oracle.so (the
 linearized CP-AL QP solver, the FA_CP source built with -DORACLE_LIN_CP_AL) and
 libplanted_fa_cp_oracle.so (the planted-KKT FA_CP, -DORACLE_PLANTED_FA_CP) and
 libplanted_cp_al_oracle.so (the planted-KKT CP-AL, -DORACLE_PLANTED_CP_AL), each
-with a reverse-summing build, and libcp_al_oracle_omp.so (the LP oracle at -O3
-with OpenMP: the CPU baseline of the benchmark).  The oracle exports the same C names
+with a reverse-summing build, libcp_al_oracle_omp.so (the LP oracle at -O3
+with OpenMP: the CPU baseline of the benchmark) and cp_al_oracle_cli (the LP
+runner's own source linked against the LP oracle).  The oracle exports the same C names
 as the product library; it is built with hidden visibility and -Bsymbolic so a
 process can load both without the two binding to each other's symbols.
 
@@ -51,7 +52,19 @@ def build() -> Path:
     _one("fa_cp_oracle.cpp", "libplanted_fa_cp_oracle_rev.so", ["-DORACLE_PLANTED_FA_CP", "-DFA_CP_ORACLE_REVERSE_SUMS"])
     _one("fa_cp_oracle.cpp", "libplanted_cp_al_oracle.so", ["-DORACLE_PLANTED_CP_AL"])
     _one("fa_cp_oracle.cpp", "libplanted_cp_al_oracle_rev.so", ["-DORACLE_PLANTED_CP_AL", "-DFA_CP_ORACLE_REVERSE_SUMS"])
-    return _one("cp_al_oracle.cpp", "libcp_al_oracle.so")
+    lib = _one("cp_al_oracle.cpp", "libcp_al_oracle.so")
+    # the LP runner's source (policy layer, MPS reader, output files) linked
+    # against the oracle: the runner's routes and phases with oracle solves,
+    # the checker of the GPU runner
+    csrc = ROOT / "pdal_amd" / "csrc" / "lp"
+    cli = OUT / "cp_al_oracle_cli"
+    cmd = ["g++", "-O2", "-std=c++17", "-ffp-contract=off", f"-I{ROOT / 'include'}", str(csrc / "cli.cpp"),
+           str(csrc / "mps_parser.cpp"), f"-L{OUT}", "-l:libcp_al_oracle.so", "-Wl,-rpath,$ORIGIN", "-lz",
+           "-o", str(cli) + ".tmp"]
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(str(cli) + ".tmp", cli)
+    return lib
 
 
 if __name__ == "__main__":

@@ -1846,6 +1846,9 @@ ORACLE_API void set_start_values(lp_problem_t *p, const double *primal, const do
     }
 }
 
+// utils.cu:230-253; the LP runner prints it (an oracle-backed runner links this library)
+ORACLE_API const char *termination_reason_to_string(termination_reason_t r) { return reason_name(r); }
+
 ORACLE_API void cp_al_result_free(cp_al_result_t *r)
 {
     if (!r) return;
