@@ -95,6 +95,15 @@ def measured_traffic(tag: str, profile: str = "batch_staged_spmv_4096_instances.
     return None
 
 
+def planted_bar_ratio(r, fam, theta, tol: float) -> float:
+    """|p - planted optimum| / the bar the KKT test at tol implies for member theta."""
+    from pdal_amd.generators import kkt_objective_bar
+    xs, ys, _ = fam.solution(theta)
+    inst = fam.instance(theta)
+    bar = kkt_objective_bar(r.primal_objective, r.dual_objective, xs, ys, inst["con_lb"], inst["c"], tol)
+    return abs(r.primal_objective - fam.optimal_value(theta)) / bar
+
+
 def roofline(bytes_per_launch: int, ms: float, kernel: str, traffic=None) -> dict:
     gbs = bytes_per_launch / (ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -157,7 +166,7 @@ def cpu_baseline_batch(insts, gpu_iters: dict, sample: int, iters_cap: int) -> d
 
 def run_config3(args, lib) -> dict:
     import numpy as np
-    from pdal_amd.generators import config3_instance, config3_sizes
+    from pdal_amd.generators import config3_instance, config3_sizes, kkt_objective_bar
     from pdal_amd.lp import LpBatch, LpSession
     count = args.instances
     t0 = time.perf_counter()
@@ -204,6 +213,10 @@ def run_config3(args, lib) -> dict:
     value = count * args.steps / total
     obj_err = max(abs(r.primal_objective - insts[k].optimal_value) / (1.0 + abs(insts[k].optimal_value))
                   for r, k in zip(res, (0, count // 2, count - 1)))
+    bar_ratio = max(abs(r.primal_objective - insts[k].optimal_value)
+                    / kkt_objective_bar(r.primal_objective, r.dual_objective, insts[k].x_star, insts[k].y_star,
+                                        insts[k].b, insts[k].c, args.tol)
+                    for r, k in zip(res, (0, count // 2, count - 1)))
     # plain single-instance GPU path on a random sample of the same instances
     rng = np.random.default_rng(0)
     sample = rng.choice(count, size=min(args.plain_sample, count), replace=False)
@@ -225,6 +238,7 @@ def run_config3(args, lib) -> dict:
         "seconds_until_99pct_stopped": round(st["p99_stopped_s"], 2),
         "plain_solves_per_s": round(plain, 3), "speedup_vs_plain": round(value / plain, 2),
         "plain_sample": int(len(sample)), "objective_rel_err_vs_planted_max3": obj_err,
+        "planted_objective_over_kkt_bar_max3": round(bar_ratio, 4),
     })
     if not args.no_cpu:
         log("CPU baseline (oracle, 1 thread) ...")
@@ -435,7 +449,7 @@ def run_config4(args, lib) -> dict:
 
     for w in range(args.warmup):
         warm_step(*batches[w])
-    times, fallbacks, splits, updates, obj_err, warm_its = [], 0, [], [], 0.0, []
+    times, fallbacks, splits, updates, obj_err, warm_its, bar_ratio = [], 0, [], [], 0.0, [], 0.0
     for s in range(args.steps):
         th, bvec, cvec = batches[args.warmup + s]
         t1 = time.perf_counter()
@@ -449,6 +463,8 @@ def run_config4(args, lib) -> dict:
         warm_its += [r.iterations for r in res]
         obj_err = max(obj_err, max(abs(r.primal_objective - fam.optimal_value(t)) / (1.0 + abs(fam.optimal_value(t)))
                                    for r, t in zip(res, th) if r.status == "OPTIMAL"))
+        bar_ratio = max(bar_ratio, max(planted_bar_ratio(r, fam, t, args.tol) for r, t in zip(res, th)
+                                       if r.status == "OPTIMAL"))
         log(f"warm step {s}: {count} fresh held-out solves in {times[-1]:.3f} s (vector update {t_up * 1e3:.1f} ms, "
             f"solve {st['solve_s']:.3f} s, half stopped {st['half_stopped_s']:.3f} s, "
             f"99 % {st['p99_stopped_s']:.3f} s, fallbacks {fb})")
@@ -484,6 +500,8 @@ def run_config4(args, lib) -> dict:
         "plain_iterations_mean": float(np.mean(plain_its)),
         "speedup_vs_plain": round(value / plain, 2), "speedup_cold_batched_vs_plain": round(count / cold_s / plain, 2),
         "objective_rel_err_vs_planted_max": obj_err,
+        # |p - planted optimum| over the bar the KKT test at tol implies (generators.kkt_objective_bar); <= 1 passes
+        "planted_objective_over_kkt_bar_max": round(bar_ratio, 4),
         "training": {"instances": args.train, "tolerance": 1e-8, "iteration_cap": args.train_iters,
                      "solve_s": round(tst["solve_s"], 2), "fit_s": round(fit_s, 2), "epochs": args.epochs,
                      "head_rel_resid": info.get("head_rel_resid")},

@@ -151,7 +151,9 @@ static __global__ void __launch_bounds__(kBlock)
 
 // ---------------------------------------------------------------------------
 // Q.x products (solver_state.cu:371-442).  MODE: 0 sparse Q, 1 R^T(Rx) from a
-// precomputed rx, 2 Q x (precomputed in qx) + R^T rx.  Plain form writes qx.
+// precomputed rx, 2 Q x (precomputed in qx) + R^T rx, 5 R^T rx + Q x with the
+// sparse part summed in the same pass (the BB gradient of the low-rank-plus-
+// sparse form: one launch less per inner step).  Plain form writes qx.
 template <int G, int MODE>
 __device__ __forceinline__ double qrow(const CsrView &Qm, const CsrView &Rt, int row, int lane,
                                        const double *__restrict__ xv, const double *__restrict__ rx,
@@ -160,6 +162,7 @@ __device__ __forceinline__ double qrow(const CsrView &Qm, const CsrView &Rt, int
     if (MODE == 0) return row_dot<G>(Qm, row, lane, xv);
     const double t = row_dot<G>(Rt, row, lane, rx);
     if (MODE == 1) return t;
+    if (MODE == 5) return t + row_dot<G>(Qm, row, lane, xv);
     return t + qx_in[row];
 }
 

@@ -731,14 +731,9 @@ void Solver::bb_inner_step(const StepScalars &sc, bool test_first)
         spmv(R_, xT_.get(), rx_.get(), bb);
         dispatch_group(Rt_.G, [&](auto g) { grad(g, std::integral_constant<int, 1>{}, Rt_); });
         break;
-    case FA_CP_LOW_RANK_PLUS_SPARSE_Q:
-        dispatch_group(Q_.G, [&](auto g) {
-            constexpr int G = decltype(g)::value;
-            qx_kernel<G, 0><<<blocks_for((long)n_ * G), kBlock, 0, stream_>>>(Q_.view(), Rt_.view(), xT_.get(),
-                                                                             nullptr, qx_.get(), n_, bb);
-        });
+    case FA_CP_LOW_RANK_PLUS_SPARSE_Q:  // the sparse part's product inside the gradient kernel
         spmv(R_, xT_.get(), rx_.get(), bb);
-        dispatch_group(Rt_.G, [&](auto g) { grad(g, std::integral_constant<int, 2>{}, Rt_); });
+        dispatch_group(Rt_.G, [&](auto g) { grad(g, std::integral_constant<int, 5>{}, Rt_); });
         break;
     case FA_CP_DIAG_Q: grad(std::integral_constant<int, 1>{}, std::integral_constant<int, 4>{}, Rt_); break;
     default: grad(std::integral_constant<int, 1>{}, std::integral_constant<int, 3>{}, Rt_); break;

@@ -403,3 +403,17 @@ def config4_family(seed: int = 4, scale: float = 1.0) -> PlantedFamily:
 def config4_thetas(count: int, seed: int, p: int = 32) -> np.ndarray:
     """theta ~ N(0, I_2p), one row per instance."""
     return np.random.default_rng(seed).standard_normal((count, 2 * p))
+
+
+def kkt_objective_bar(primal_obj: float, dual_obj: float, x_star, y_star, b, c, tol: float) -> float:
+    """How far a solve that passes the relative KKT test at `tol` (L2 norms,
+    the reference's definitions: |p - d| < tol (1 + |p| + |d|), |Ax - b| < tol
+    (1 + |b|), |c - A^T y - lambda| < tol (1 + |c|)) may put its primal
+    objective from the optimum of an LP with the optimal pair (x*, y*): the gap
+    allowance plus what the feasibility residuals can move an objective,
+    |y*| tol (1 + |b|) on the primal side and |x*| tol (1 + |c|) on the dual
+    side (weak duality through the optimal pair)."""
+    gap = tol * (1.0 + abs(primal_obj) + abs(dual_obj))
+    side = tol * max(float(np.linalg.norm(y_star)) * (1.0 + float(np.linalg.norm(b))),
+                     float(np.linalg.norm(x_star)) * (1.0 + float(np.linalg.norm(c))))
+    return gap + side
