@@ -26,7 +26,9 @@ The same JSON line carries:
     log-uniform on [200, 20000], n_i = 2 m_i, 8 nonzeros per column; cold),
     solves/s and its own roofline;
   * spmv: the plain path's fused SpMV kernels on the large planted LP
-    (m = 2e6, n = 4e6, Zipf(1.5) column counts) with their HBM GB/s;
+    (m = 2e6, n = 4e6, Zipf(1.5) column counts) with their HBM GB/s, and one
+    full plain solve of it to rel KKT 1e-6 (iterations/s, solves/s, the
+    planted objective against the bar the KKT test implies);
   * cpu_baseline: the FP64 oracle (-O3, OpenMP over the host's threads, cold)
     on a bounded sample of held-out members, scaled to solves/s with the GPU
     cold iteration counts, with the one-thread figure, the thread count and
@@ -111,17 +113,34 @@ def roofline(bytes_per_launch: int, ms: float, kernel: str, traffic=None) -> dic
             "bytes_per_launch": bytes_per_launch, "avg_launch_ms": ms}
 
 
-def spmv_leg(lib, kernel_iters: int) -> dict:
-    """Plain-path fused SpMV kernels on the large planted LP (kernel timing only)."""
-    from pdal_amd.generators import config1
+def spmv_leg(lib, kernel_iters: int, tol: float = 1e-6, full_solve: bool = True) -> dict:
+    """Plain-path fused SpMV kernels on the large planted LP, and (full_solve)
+    one plain single-instance solve of it to rel KKT tol on the same device
+    session: iterations/s and solves/s of the plain path at full size."""
+    from pdal_amd.generators import config1, kkt_objective_bar
     from pdal_amd.lp import LpSession
     t0 = time.perf_counter()
     inst = config1(seed=1)
     m, n, nnz = inst.m, inst.n, int(inst.A.nnz)
     log(f"large LP m={m} n={n} nnz={nnz} generated in {time.perf_counter() - t0:.1f} s")
-    with LpSession(lib, **inst.lp_args(), params=lib.default_params()) as sess:
+    solve = None
+    with LpSession(lib, **inst.lp_args(), params=lib.default_params(eps_optimal_relative=tol,
+                                                                    eps_feasible_relative=tol)) as sess:
         kt = sess.time_kernels(kernel_iters)
         setup = sess.setup_seconds
+        if full_solve:
+            t1 = time.perf_counter()
+            r = sess.solve()
+            wall = time.perf_counter() - t1
+            opt = inst.optimal_value
+            bar = kkt_objective_bar(r.primal_objective, r.dual_objective, inst.x_star, inst.y_star, inst.b, inst.c,
+                                    tol)
+            solve = {"status": r.status, "iterations": r.iterations, "solve_s": round(r.solve_time_sec, 3),
+                     "wall_s": round(wall, 3), "iterations_per_s": round(r.iterations / r.solve_time_sec, 1),
+                     "solves_per_s": round(1.0 / wall, 5), "kkt": r.kkt, "tolerance": tol,
+                     "objective_rel_err_vs_planted": abs(r.primal_objective - opt) / abs(opt),
+                     "planted_objective_over_kkt_bar": round(abs(r.primal_objective - opt) / bar, 4)}
+            log(f"large LP full solve: {solve}")
     bp, bd = fused_primal_bytes(nnz, n, m), fused_dual_bytes(nnz, n, m)
     out = {"workload": "large planted LP m=2e6 n=4e6 (Zipf(1.5) column counts in [2,64])", "nnz": nnz,
            "primal_At_gbs": round(bp / (kt["primal_ms"] * 1e-3) / 1e9, 1),
@@ -137,6 +156,8 @@ def spmv_leg(lib, kernel_iters: int) -> dict:
     out["gather_floor_ms"] = {"primal_At": round(g_at, 4), "dual_A": round(g_a, 4)}
     out["spmv_over_gather_floor"] = {"primal_At": round(kt["primal_ms"] / g_at, 3), "dual_A": round(kt["dual_ms"] / g_a, 3)}
     log(f"large LP kernels: {out}")
+    if solve is not None:
+        out["full_solve"] = solve
     return out
 
 
@@ -635,6 +656,7 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=16, help="CPU baseline: at most this many host threads")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-spmv", action="store_true", help="skip the large-LP SpMV leg")
+    ap.add_argument("--no-c1-solve", action="store_true", help="large-LP leg: time the kernels only, no full solve")
     ap.add_argument("--no-batch-leg", action="store_true", help="config4: skip the 4096-instance batch leg")
     ap.add_argument("--kernels-only", action="store_true", help="set up and time kernels only (profiling)")
     ap.add_argument("--no-qp", action="store_true", help="config4: skip the FA_CP config-2 leg")
@@ -695,7 +717,7 @@ def main() -> int:
         leg["workload"] = "4096 planted LPs, m log-uniform [200,20000], n=2m, 8 nnz/col (uneven sizes, cold)"
         out["batch_4096"] = leg
     if args.workload in ("config4", "config3") and not args.no_spmv:
-        out["spmv"] = spmv_leg(lib, args.kernel_iters)
+        out["spmv"] = spmv_leg(lib, args.kernel_iters, args.tol, not args.no_c1_solve)
     if args.workload == "config4" and not args.no_qp:
         out["qp_config2"] = run_config2_leg(args)
     out["roofline"] = roof
