@@ -5,8 +5,10 @@
  * every struct below keeps the reference's field order and field types so a
  * caller compiled against the reference header links against libcp_al.so
  * unchanged.  The reference also includes "PSLP_stats.h" (third-party presolve,
- * not vendored); no public struct uses a PSLP type, so the include is dropped
- * and `presolve` stays a field that is accepted and reported as unsupported.
+ * not vendored); no public struct uses a PSLP type, so the include is dropped.
+ * `presolve` runs this library's own reductions (empty rows, fixed and empty
+ * columns, singleton rows); `presolve_status` holds the flags 0 UNCHANGED,
+ * 1 REDUCED, 2 INFEASIBLE, 4 INFEASIBLE_OR_UNBOUNDED.
  */
 #pragma once
 

@@ -59,8 +59,9 @@
  *
  * Deliberate differences from the reference, shared with the HIP path:
  *   - timing uses steady_clock wall time, not clock() CPU time;
- *   - presolve is not available (PSLP is not vendored): a request is ignored
- *     with a warning;
+ *   - presolve: PSLP is not vendored; the HIP path runs its own reductions
+ *     (checked against HiGHS and the KKT conditions, not against this file),
+ *     and this oracle ignores a request with a warning;
  *   - result->absolute_{primal,dual}_residual are filled in;
  *   - errors return NULL instead of exit().
  */

@@ -43,6 +43,7 @@ CP_AL_SOURCES = [
     CSRC / "lp" / "cp_al_solver.hip",
     CSRC / "lp" / "batch_solver.hip",
     CSRC / "lp" / "mps_parser.cpp",
+    CSRC / "lp" / "presolve.cpp",
 ]
 
 

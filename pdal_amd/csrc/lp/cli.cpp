@@ -286,6 +286,17 @@ void save_solution(const double *v, int n, const char *dir, const std::string &n
     fclose(f);
 }
 
+const char *presolve_status_name(int status)
+{   // get_presolve_status_str, presolve.c:16-31 (the library's status flags)
+    switch (status) {
+    case 0: return "UNCHANGED";
+    case 1: return "REDUCED";
+    case 2: return "INFEASIBLE";
+    case 4: return "INFEASIBLE_OR_UNBOUNDED";
+    default: return "UNKNOWN_STATUS";
+    }
+}
+
 void save_summary(const cp_al_result_t *r, const char *dir, const std::string &name)
 {   // save_solver_summary, cli.c:501-586 (same keys, parsed by the benchmark collectors)
     const std::string path = out_path(dir, name, "_summary.txt");
@@ -295,6 +306,13 @@ void save_summary(const cp_al_result_t *r, const char *dir, const std::string &n
         return;
     }
     fprintf(f, "Termination Reason: %s\n", termination_reason_to_string(r->termination_reason));
+    if (r->presolve_time > 0.0) {  // cli.c:542-549
+        fprintf(f, "Presolve Status: %s\n", presolve_status_name(r->presolve_status));
+        fprintf(f, "Presolve Time (sec): %e\n", r->presolve_time);
+        fprintf(f, "Reduced Rows: %d\n", r->num_reduced_constraints);
+        fprintf(f, "Reduced Columns: %d\n", r->num_reduced_variables);
+        fprintf(f, "Reduced Nonzeros: %d\n", r->num_reduced_nonzeros);
+    }
     fprintf(f, "Precondition time (sec): %e\n", r->rescaling_time_sec);
     fprintf(f, "Runtime (sec): %e\n", r->cumulative_time_sec);
     fprintf(f, "Iterations Count: %d\n", r->total_count);

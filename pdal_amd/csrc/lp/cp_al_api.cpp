@@ -7,6 +7,7 @@
 #include "cp_al.h"
 #include "cp_al_internal.hpp"
 #include "pdal_batch.h"
+#include "presolve.hpp"
 #include "pdal_lp.h"
 
 #include <algorithm>
@@ -388,10 +389,80 @@ PDAL_API cp_al_result_t *optimize(const cp_al_parameters_t *params, lp_problem_t
         return nullptr;
     }
     print_initial_info(params, prob);
-    if (params->presolve && !quiet())
-        fprintf(stderr, "[cp_al-mi355x] presolve (PSLP) is not available in this build; solving the original "
-                        "problem\n");
     cp_al_result_t *r = nullptr;
+    if (params->presolve) {  // solver.cu:1183-1194, 1726-1730; presolve.c:61-226
+        if (prob->primal_start || prob->dual_start)
+            printf("Warning: Warm-starting is currently not supported when presolve is enabled.\n"
+                   "The provided initial solutions will be ignored.\n");
+        if (params->verbose) printf("\nRunning presolver (built-in reductions)...\n");
+        Presolved ps;
+        presolve(prob, ps);
+        if (params->verbose) {
+            printf("  %-15s : %s\n", "status", presolve_status_name(ps.status));
+            printf("  %-15s : %.3g sec\n", "presolve time", ps.seconds);
+            printf("  %-15s : %d rows, %d columns, %d nonzeros\n", "reduced problem", (int)ps.rows.size(),
+                   (int)ps.cols.size(), ps.reduced ? ps.reduced->constraint_matrix_num_nonzeros : 0);
+        }
+        const int n = prob->num_variables, m = prob->num_constraints;
+        if (!ps.reduced) {  // solved during presolve: infeasible, unbounded, or no column left
+            r = (cp_al_result_t *)calloc(1, sizeof(cp_al_result_t));
+            r->num_variables = n;
+            r->num_constraints = m;
+            r->num_nonzeros = prob->constraint_matrix_num_nonzeros;
+            r->num_reduced_variables = (int)ps.cols.size();
+            r->num_reduced_constraints = (int)ps.rows.size();
+            r->num_reduced_nonzeros = 0;
+            r->presolve_status = ps.status;
+            r->presolve_time = ps.seconds;
+            r->primal_solution = (double *)calloc((size_t)std::max(n, 1), sizeof(double));
+            r->dual_solution = (double *)calloc((size_t)std::max(m, 1), sizeof(double));
+            r->reduced_cost = (double *)calloc((size_t)std::max(n, 1), sizeof(double));
+            if (ps.status & (kPresolveInfeasible | kPresolveUnbndOrInfeas)) {
+                r->termination_reason = (ps.status & kPresolveInfeasible) ? TERMINATION_REASON_PRIMAL_INFEASIBLE
+                                                                          : TERMINATION_REASON_INFEASIBLE_OR_UNBOUNDED;
+                r->absolute_primal_residual = r->relative_primal_residual = INFINITY;
+                r->absolute_dual_residual = r->relative_dual_residual = INFINITY;
+                r->primal_objective_value = INFINITY;
+                r->dual_objective_value = -INFINITY;
+                r->objective_gap = r->relative_objective_gap = INFINITY;
+            } else {
+                r->termination_reason = TERMINATION_REASON_OPTIMAL;
+                postsolve(ps, prob, nullptr, nullptr, r->primal_solution, r->dual_solution, r->reduced_cost);
+                double obj = prob->objective_constant;
+                for (int j = 0; j < n; ++j) obj += prob->objective_vector[j] * r->primal_solution[j];
+                r->primal_objective_value = r->dual_objective_value = obj;
+            }
+            if (!quiet()) final_log(r, params);
+            return r;
+        }
+        try {
+            r = optimize_gpu(params, ps.reduced);
+        } catch (const std::exception &e) {
+            fprintf(stderr, "[cp_al-mi355x] solve failed: %s\n", e.what());
+            return nullptr;
+        }
+        if (!r) return nullptr;
+        double *x = (double *)malloc((size_t)std::max(n, 1) * sizeof(double));
+        double *y = (double *)malloc((size_t)std::max(m, 1) * sizeof(double));
+        double *z = (double *)malloc((size_t)std::max(n, 1) * sizeof(double));
+        postsolve(ps, prob, r->primal_solution, r->dual_solution, x, y, z);
+        free(r->primal_solution);
+        free(r->dual_solution);
+        free(r->reduced_cost);
+        r->primal_solution = x;
+        r->dual_solution = y;
+        r->reduced_cost = z;
+        r->num_reduced_variables = ps.reduced->num_variables;
+        r->num_reduced_constraints = ps.reduced->num_constraints;
+        r->num_reduced_nonzeros = ps.reduced->constraint_matrix_num_nonzeros;
+        r->num_variables = n;
+        r->num_constraints = m;
+        r->num_nonzeros = prob->constraint_matrix_num_nonzeros;
+        r->presolve_status = ps.status;
+        r->presolve_time = ps.seconds;
+        if (!quiet()) final_log(r, params);
+        return r;
+    }
     try {
         r = optimize_gpu(params, prob);
     } catch (const std::exception &e) {

@@ -383,7 +383,11 @@ class CpAl:
                 rescaling_time_sec=r.rescaling_time_sec,
                 solve_time_sec=r.cumulative_time_sec,
                 extra={"feasibility_polishing_time": r.feasibility_polishing_time,
-                       "feasibility_iteration": r.feasibility_iteration},
+                       "feasibility_iteration": r.feasibility_iteration,
+                       "presolve_status": r.presolve_status, "presolve_time": r.presolve_time,
+                       "num_reduced_variables": r.num_reduced_variables,
+                       "num_reduced_constraints": r.num_reduced_constraints,
+                       "num_reduced_nonzeros": r.num_reduced_nonzeros},
             )
         finally:
             self.lib.cp_al_result_free(res)
