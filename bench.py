@@ -10,7 +10,7 @@ on their POD coefficients and final primal weights, and the batch of 256
 held-out members set up in HBM (shared-A engine).  A step = predict the starts
 and primal weights of the 256 held-out members (MLP forward + affine head,
 POD expansion written into the batch's start buffers on the device), solve all
-256 to relative KKT 1e-6 (alpha = 0, blocks of 64 iterations), cold re-solve
+256 to relative KKT 1e-6 (alpha = 0, blocks of 16 iterations), cold re-solve
 of any member the warm pass leaves unsolved.
 `value` = held-out solves per second over the K timed steps.
 
@@ -56,6 +56,10 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 K_BATCH = 64
+# evaluation frequency of the warm-started held-out solves: a warm member needs about 80 iterations, so
+# a block of 64 makes most of them run to 128; measured steps at 256 members: K = 64 / 32 / 16 / 12 / 8
+# give 1364 / 1797 / 1995 / 1918 / 1690 solves/s (the evaluation pass costs about one iteration)
+K_WARM = 16
 
 
 def log(msg: str) -> None:
@@ -282,14 +286,14 @@ def host_cpu() -> dict:
     return {"threads": cores, "model": model}
 
 
-def cpu_baseline_family_parallel(insts, gpu_iters: list, iters_cap: int, threads: int) -> dict:
+def cpu_baseline_family_parallel(insts, gpu_iters: list, iters_cap: int, threads: int, k_eval: int = K_BATCH) -> dict:
     """Oracle (FP64 port, -O3, OpenMP) on the host: the first held-out member
     on one thread, and `threads` members at once on `threads` threads, each
     capped at `iters_cap` iterations; each member's full-solve time is its GPU
     cold iteration count over the measured iterations per second on it."""
     from pdal_amd.lp import CpAl
     oracle = CpAl(ROOT / "oracle" / "_build" / "libcp_al_oracle_omp.so")
-    p = oracle.default_params(iteration_limit=iters_cap, termination_evaluation_frequency=K_BATCH,
+    p = oracle.default_params(iteration_limit=iters_cap, termination_evaluation_frequency=k_eval,
                               eps_optimal_relative=1e-6, eps_feasible_relative=1e-6)
     # iterations per second from each solve's own clock (started after its set-up)
     t0 = time.perf_counter()
@@ -385,7 +389,7 @@ def run_config4(args, lib) -> dict:
     te = config4_thetas(count, 2)
     test_inst = [fam.instance(t) for t in te]
     log(f"family m={fam.m} n={fam.n} nnz={fam.base.A.nnz}; {count} held-out instances in {time.perf_counter() - t0:.1f} s")
-    params = lib.default_params(termination_evaluation_frequency=K_BATCH, eps_optimal_relative=args.tol,
+    params = lib.default_params(termination_evaluation_frequency=args.eval_freq, eps_optimal_relative=args.tol,
                                 eps_feasible_relative=args.tol, iteration_limit=1000000, time_sec_limit=300.0)
     t0 = time.perf_counter()
     batch = LpBatch(lib, test_inst, params)
@@ -508,6 +512,19 @@ def run_config4(args, lib) -> dict:
             plain_its.append(r.iterations)
     plain = args.plain_sample / plain_t
     log(f"plain single-instance GPU: {args.plain_sample} solves in {plain_t:.2f} s -> {plain:.2f} solves/s")
+    plain_k64 = None
+    if args.eval_freq != K_BATCH:  # the plain path at the evaluation frequency that suits it best
+        p64 = lib.default_params(termination_evaluation_frequency=K_BATCH, eps_optimal_relative=args.tol,
+                                 eps_feasible_relative=args.tol, iteration_limit=1000000, time_sec_limit=300.0)
+        t64 = 0.0
+        for k in range(args.plain_sample):
+            with LpSession(lib, **test_inst[k], params=p64) as sess:
+                lib.reset_rng()
+                t1 = time.perf_counter()
+                sess.solve()
+                t64 += time.perf_counter() - t1
+        plain_k64 = args.plain_sample / t64
+        log(f"plain single-instance GPU at K={K_BATCH}: {plain_k64:.2f} solves/s")
     result.update({
         "value": value, "ms_per_step": 1e3 * total / args.steps, "optimal": int(sum(r.status == "OPTIMAL" for r in res)),
         "fallbacks": fallbacks, "warm_iterations": {"mean": float(warm_its.mean()), "max": int(warm_its.max())},
@@ -520,6 +537,8 @@ def run_config4(args, lib) -> dict:
         "plain_solves_per_s": round(plain, 3), "plain_sample": args.plain_sample,
         "plain_iterations_mean": float(np.mean(plain_its)),
         "speedup_vs_plain": round(value / plain, 2), "speedup_cold_batched_vs_plain": round(count / cold_s / plain, 2),
+        "plain_solves_per_s_eval64": None if plain_k64 is None else round(plain_k64, 3),
+        "speedup_vs_plain_eval64": None if plain_k64 is None else round(value / plain_k64, 2),
         "objective_rel_err_vs_planted_max": obj_err,
         # |p - planted optimum| over the bar the KKT test at tol implies (generators.kkt_objective_bar); <= 1 passes
         "planted_objective_over_kkt_bar_max": round(bar_ratio, 4),
@@ -531,7 +550,7 @@ def run_config4(args, lib) -> dict:
         threads = min(host_cpu()["threads"], args.cpu_threads)
         log(f"CPU baseline (oracle, 1 and {threads} threads) ...")
         result["cpu_baseline"] = cpu_baseline_family_parallel(test_inst, [int(i) for i in cold_its],
-                                                              args.cpu_family_iters, threads)
+                                                              args.cpu_family_iters, threads, args.eval_freq)
         result["cpu_config0_full_solves"] = cpu_config0_full_solves(lib, threads)
     return result
 
@@ -663,6 +682,8 @@ def main() -> int:
     ap.add_argument("--qp-time", type=float, default=25.0, help="config-2 leg: time cap per FA_CP solve (s)")
     ap.add_argument("--qp-iters", type=int, default=10_000_000, help="config-2 leg: iteration cap per FA_CP solve")
     ap.add_argument("--kernel-iters", type=int, default=20)
+    ap.add_argument("--eval-freq", type=int, default=K_WARM,
+                    help="config4: evaluation frequency of the held-out solves (warm, cold and plain)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -686,7 +707,7 @@ def main() -> int:
         config = {"workload": "fixed-A LP family m=5e4 n=1e5 8 nnz/col, b=b0+U theta_b, c=c0+V theta_c (32+32 "
                               "parameters, planted optimum); 256 held-out instances per batch, learned warm start",
                   "heldout": args.heldout, "train": args.train, "tolerance": args.tol, "alpha": 0.0,
-                  "eval_freq": K_BATCH}
+                  "eval_freq": args.eval_freq}
     elif args.workload == "config3":
         res = run_config3(args, lib)
         metric = "solves/sec to rel-KKT 1e-6 (batched CP_AL, 4096 planted LPs, one GPU)"

@@ -55,7 +55,8 @@
  *   PDAL_CONTROLLER_SECONDS_PER_ITERATION (a test knob shared with the HIP
  *   path) replaces the wall clock of the adaptive-lambda epochs and the
  *   dynamic evaluation's time gate by the iteration count times its value.
- *   The dense per-iteration trace (CP_AL_LOG_FREQ) is not restated.
+ *   With verbose: the dense trace (CP_AL_LOG_FREQ), the checkpoint lines and the
+ *   progress lines (CP_AL_PROGRESS_LOG_FREQ), solver.cu:321-375.
  *
  * Deliberate differences from the reference, shared with the HIP path:
  *   - timing uses steady_clock wall time, not clock() CPU time;
@@ -1969,6 +1970,13 @@ ORACLE_API cp_al_result_t *optimize(const cp_al_parameters_t *params, lp_problem
     const int dynamic_eval_bootstrap_iters = std::max(env_int("CP_AL_DYNAMIC_EVAL_BOOTSTRAP_ITERS", 0), 0);
     bool dynamic_eval_latched = false;
     if (use_dynamic_eval && dynamic_eval_bootstrap_iters > 0) eval_frequency = dynamic_eval_fast;
+    auto log_freq = [](const char *name) {  // get_trace_log_frequency / get_progress_log_frequency, :321-375
+        const char *raw = getenv(name);
+        const int v = (raw && raw[0]) ? atoi(raw) : 0;
+        return v > 0 ? v : 0;
+    };
+    const int trace_freq = params->verbose ? log_freq("CP_AL_LOG_FREQ") : 0;
+    const int progress_freq = params->verbose ? log_freq("CP_AL_PROGRESS_LOG_FREQ") : 0;
     while (charged < tc.iteration_limit) {
         const int K = eval_frequency;
         if (do_restart) {
@@ -1976,13 +1984,30 @@ ORACLE_API cp_al_result_t *optimize(const cp_al_parameters_t *params, lp_problem
             s.fp0 = s.fp;
             do_restart = false;
         }
-        for (int i = 1; i <= K; ++i) iteration(s, i, rho, i == 1 || i == K);
+        for (int i = 1; i <= K; ++i) {
+            iteration(s, i, rho, i == 1 || i == K);
+            const long long it = (long long)s.total_count + i;
+            if (trace_freq > 0 && it % trace_freq == 0 && it % K != 0) {  // should_emit_trace, emit_trace_fp
+                compute_fixed_point_error(s);
+                if (std::isfinite(s.fp) && s.fp > 0.0) printf("trace iter %lld fp %.12e\n", it, s.fp);
+            }
+        }
         compute_fixed_point_error(s);
         choose_restart_candidate(s);  // solver.cu:1351
         compute_residual(s, params->optimality_norm);
         s.inner_count += K;
         s.total_count += K;
         charged += K;
+        if (params->verbose) {  // emit_checkpoint_fp, emit_progress_metrics (solver.cu:1356-1363, 329-375)
+            if (std::isfinite(s.fp) && s.fp > 0.0) printf("checkpoint iter %lld fp %.12e\n", (long long)s.total_count, s.fp);
+            if (progress_freq > 0 && s.total_count % progress_freq == 0) {
+                const double bal = std::fabs(std::log10(std::fmax(s.rel_d, 1e-300) / std::fmax(s.rel_p, 1e-300)));
+                printf("progress iter %lld time %.6e rpr %.12e rdr %.12e rgap %.12e fp %.12e pw %.12e pw_best %.12e "
+                       "bal %.12e\n",
+                       (long long)s.total_count, s.solve_time, s.rel_p, s.rel_d, s.rel_gap,
+                       (std::isfinite(s.fp) && s.fp > 0.0) ? s.fp : NAN, s.primal_weight, s.best_pw, bal);
+            }
+        }
 
         // check_termination_criteria, utils.cu:299-317, and the controllers' test limits (:1372-1391)
         s.solve_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();

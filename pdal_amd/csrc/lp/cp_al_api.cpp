@@ -57,12 +57,9 @@ bool env_set(const char *name)
 void warn_unsupported_controllers(bool batch)
 {
     // Controllers of the reference that are switched on only through these
-    // variables.  The single-instance solver runs all of them except the dense
-    // per-iteration trace (CP_AL_LOG_FREQ, solver.cu:321-358, a diagnostic that
-    // prints fixed-point errors between evaluations); the batched solver runs
-    // none of them (it keeps the restart knobs, utils.cu:324-332, Curtis-Reid
-    // without its early stop, and sigma mode).
-    static const char *const single[] = {"CP_AL_LOG_FREQ"};
+    // variables.  The single-instance solver runs all of them; the batched
+    // solver runs none of them (it keeps the restart knobs, utils.cu:324-332,
+    // Curtis-Reid without its early stop, and sigma mode).
     static const char *const batched[] = {"CP_AL_LOG_FREQ",
                                           "CP_AL_SIGMA_TRIGGER",
                                           "CP_AL_ADAPTIVE_LAMBDA",
@@ -74,13 +71,13 @@ void warn_unsupported_controllers(bool batch)
                                           "CP_AL_ALPHA_ROLLBACK_TRIAL",
                                           "CP_AL_ALPHA_DIRECT_PULSE",
                                           "CP_AL_RESTART_SELECT"};
-    const char *const *knobs = batch ? batched : single;
-    const size_t count = batch ? sizeof(batched) / sizeof(batched[0]) : sizeof(single) / sizeof(single[0]);
-    for (size_t i = 0; i < count; ++i) {
-        const char *raw = getenv(knobs[i]);
+    if (!batch) return;
+    for (const char *knob : batched) {
+        const char *raw = getenv(knob);
         if (raw && raw[0] != '\0' && strcmp(raw, "0") != 0)
-            fprintf(stderr, "[cp_al-mi355x] %s=%s requests a controller this %s does not implement; ignored\n",
-                    knobs[i], raw, batch ? "batched solver" : "build");
+            fprintf(stderr, "[cp_al-mi355x] %s=%s requests a controller the batched solver does not implement; "
+                            "ignored\n",
+                    knob, raw);
     }
 }
 

@@ -77,6 +77,7 @@ class Solver {
     void launch_predictor(const PredictorArgs &a);
     void launch_eval();
     void launch_block(int K);
+    void launch_block_traced(int K, int trace_freq);
     void record_graph(int K);
     void sync_scalars();
     Eval read_eval();
@@ -888,6 +889,21 @@ void Solver::launch_block(int K)
     PDAL_HIP(hipGraphLaunch(graph_exec_, stream_));
 }
 
+void Solver::launch_block_traced(int K, int trace_freq)
+{   // the dense trace of solver.cu:1286-1300, 321-358: the block launched iteration by iteration, and at
+    // every trace iteration that is not an evaluation the fixed-point metric of x_R - x_T, y_R - y_T as
+    // the buffers hold them then (x_R of this iteration; x_T, y_T, y_R of the block's last major step)
+    for (int i = 1; i <= K; ++i) {
+        launch_iteration(i, i == 1 || i == K);
+        const long long it = (long long)total_ + i;
+        if (it % trace_freq == 0 && it % K != 0) {
+            const double f = delta_score(xR_.get(), xT_.get(), yR_.get(), yT_.get());
+            if (std::isfinite(f) && f > 0.0) printf("trace iter %lld fp %.12e\n", it, f);
+        }
+    }
+    launch_eval();
+}
+
 Eval Solver::read_eval()
 {
     PDAL_HIP(hipMemcpyAsync(h_eval_.get(), evalq_.get(), 16 * sizeof(double), hipMemcpyDeviceToHost, stream_));
@@ -1623,6 +1639,14 @@ cp_al_result_t *Solver::solve_once(const lp_problem_t *original)
     const double trigger_primal = env_double("CP_AL_SIGMA_TRIGGER_PRIMAL", 1e-2);
     const double trigger_dual = env_double("CP_AL_SIGMA_TRIGGER_DUAL", 1e-6);
     const double trigger_ratio = env_double("CP_AL_SIGMA_TRIGGER_RATIO", 1e6);
+    // verbose diagnostics: the dense trace and the progress lines (solver.cu:321-375)
+    auto log_freq = [](const char *name) {
+        const char *raw = getenv(name);
+        const int v = (raw && raw[0]) ? atoi(raw) : 0;
+        return v > 0 ? v : 0;
+    };
+    const int trace_freq = P_.verbose ? log_freq("CP_AL_LOG_FREQ") : 0;
+    const int progress_freq = P_.verbose ? log_freq("CP_AL_PROGRESS_LOG_FREQ") : 0;
     auto positive_charge = [&]() {
         const int pos = charged - trial_.t_charged;
         trial_.positive_charged += pos;
@@ -1637,7 +1661,8 @@ cp_al_result_t *Solver::solve_once(const lp_problem_t *original)
             do_restart = false;
         }
         sync_scalars();
-        launch_block(K);
+        if (trace_freq > 0) launch_block_traced(K, trace_freq);
+        else launch_block(K);
         const Eval e = read_eval();
         last_eval_ = e;
         fp_stale_ = false;
@@ -1648,6 +1673,16 @@ cp_al_result_t *Solver::solve_once(const lp_problem_t *original)
         inner_ += K;
         total_ += K;
         charged += K;
+        if (P_.verbose) {  // :1356-1363, 329-375
+            if (std::isfinite(fp_) && fp_ > 0.0) printf("checkpoint iter %lld fp %.12e\n", (long long)total_, fp_);
+            if (progress_freq > 0 && total_ % progress_freq == 0) {
+                const double bal = std::fabs(std::log10(std::fmax(rel_d_, 1e-300) / std::fmax(rel_p_, 1e-300)));
+                printf("progress iter %lld time %.6e rpr %.12e rdr %.12e rgap %.12e fp %.12e pw %.12e pw_best %.12e "
+                       "bal %.12e\n",
+                       (long long)total_, solve_time_, rel_p_, rel_d_, rel_gap_,
+                       (std::isfinite(fp_) && fp_ > 0.0) ? fp_ : NAN, pw_, best_pw_, bal);
+            }
+        }
         if (total_ % print_frequency(total_) == 0) display();
         solve_time_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
         if (rel_d_ < tc.eps_feasible_relative && rel_p_ < tc.eps_feasible_relative &&

@@ -61,6 +61,10 @@ LINES = {
     "boot": (r"^dynamic-eval iter (\d+) eval_freq (\d+)->(\d+)", (0, 1, 2), ()),
     "cr-stop": (r"\[cr-early-stop\] selected=(\d+) executed=(\d+) max_edge_log_change=(\S+)", (0, 1), (2,)),
     "cr-diag": (r"\[cr-diagnostic\] iteration=(\d+) max_edge_log_change=(\S+)", (0,), (1,)),
+    "trace": (r"^trace iter (\d+) fp (\S+)$", (0,), (1,)),
+    "checkpoint": (r"^checkpoint iter (\d+) fp (\S+)$", (0,), (1,)),
+    "progress": (r"^progress iter (\d+) time \S+ rpr (\S+) rdr (\S+) rgap (\S+) fp (\S+) pw (\S+) pw_best (\S+)",
+                 (0,), (1, 2, 3, 4, 5, 6)),
 }
 
 
@@ -115,6 +119,10 @@ SCENARIOS = {
                       ("boot", "restart")),
     "cr_early_stop": (CR_STOP, dict(iteration_limit=1000, termination_evaluation_frequency=200),
                       ("cr-stop", "cr-diag", "restart")),
+    # the verbose diagnostics: the dense trace between evaluations, checkpoint and progress lines
+    "dense_trace": ({"CP_AL_LAMBDA": "0", "CP_AL_LOG_FREQ": "7", "CP_AL_PROGRESS_LOG_FREQ": "200"},
+                    dict(iteration_limit=1200, termination_evaluation_frequency=100),
+                    ("trace", "checkpoint", "progress", "restart")),
 }
 
 

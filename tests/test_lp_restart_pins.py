@@ -170,3 +170,50 @@ def test_trial_gate_follows_from_its_inputs(oracle_lp, monkeypatch, capfd, force
     else:
         assert dec["accept"] == str(int(gate))
         assert dec["reason"] == ("accepted" if gate else "acceptance-gate")
+
+
+def test_dense_trace_in_closed_form(oracle_lp, monkeypatch, capfd):
+    """CP_AL_LOG_FREQ with verbose (solver.cu:321-358): inside a block the trace
+    prints the fixed-point metric of what the buffers hold at that iteration:
+    x_R of the iteration itself, x_T, y_T and y_R of the block's first (major)
+    iteration.  The checkpoint line at the evaluation is the block's fixed-point
+    error and the progress line carries the evaluation's residuals."""
+    monkeypatch.setenv("CP_AL_LAMBDA", "0")
+    monkeypatch.setenv("CP_AL_LOG_FREQ", "5")
+    monkeypatch.setenv("CP_AL_PROGRESS_LOG_FREQ", "25")
+    m, n, s, K = 10, 16, 1.7, 25
+    A = _orthogonal_rows(m, n, s, seed=3)
+    rng = np.random.default_rng(4)
+    b = rng.standard_normal(m)
+    c = A.T @ rng.standard_normal(m)
+    eta = np.sqrt(0.998 ** 2 / s)
+    pw = (np.linalg.norm(c) + 1) / (np.linalg.norm(b) + 1)
+    M, d = _affine(A, b, c, eta / pw, eta * pw)
+    z0 = np.zeros(n + m)
+    T0 = M @ z0 + d                         # the major iteration 1: x_T, y_T
+    yR1 = 2 * T0[n:] - z0[n:]
+    oracle_lp.reset_rng()
+    capfd.readouterr()
+    r = oracle_lp.solve(A=sp.csr_matrix(A), c=c, con_lb=b, con_ub=b, params=oracle_lp.default_params(
+        verbose=True, l_inf_ruiz_iterations=0, has_pock_chambolle_alpha=False, bound_objective_rescaling=False,
+        termination_evaluation_frequency=K, iteration_limit=K, eps_optimal_relative=1e-15,
+        eps_feasible_relative=1e-15))
+    LIBC.fflush(None)
+    log = capfd.readouterr().out
+    trace = [(int(a), float(b_)) for a, b_ in re.findall(r"^trace iter (\d+) fp (\S+)$", log, re.M)]
+    assert [t for t, _ in trace] == [5, 10, 15, 20]  # every 5th iteration but not the evaluation's
+    for t, fp in trace:
+        z_prev, Tt, _ = _block(M, d, z0, t)  # z_{t-1} and T(z_{t-1})
+        xR = 2 * Tt[:n] - z_prev[:n]
+        want = _metric(A, pw, eta, np.concatenate([xR - T0[:n], yR1 - T0[n:]]))
+        assert fp == pytest.approx(want, rel=1e-9), t
+    z_km1, zT, _ = _block(M, d, z0, K)
+    cp = re.findall(r"^checkpoint iter (\d+) fp (\S+)$", log, re.M)
+    assert len(cp) == 1 and cp[0][0] == str(K)
+    assert float(cp[0][1]) == pytest.approx(_metric(A, pw, eta, zT - z_km1), rel=1e-9)
+    pg = re.findall(r"^progress iter (\d+) time \S+ rpr (\S+) rdr (\S+) rgap (\S+) fp \S+ pw (\S+)", log, re.M)
+    assert len(pg) == 1 and pg[0][0] == str(K)
+    assert float(pg[0][1]) == pytest.approx(r.relative_primal_residual, rel=1e-11)
+    assert float(pg[0][2]) == pytest.approx(r.relative_dual_residual, rel=1e-11)
+    assert float(pg[0][3]) == pytest.approx(r.relative_objective_gap, rel=1e-11)
+    assert float(pg[0][4]) == pytest.approx(pw, rel=1e-11)
