@@ -13,7 +13,10 @@ Bars:
     held to ITER_RTOL up to the first restart (the low-rank forms already
     split at the first BB solve after it, by 1e-8 between the two oracle
     builds) and, later, to the distance between the two oracle builds
-    (NOISE_FACTOR times it);
+    (NOISE_FACTOR times it); resumed from the device's own state after 1000
+    iterations, both sides are held to ITER_RTOL again with equal inner step
+    counts, over the lengths the two oracle builds stay within 3e-11 (48-200
+    iterations);
   * same termination status and iteration count, objective within 1e-6
     relative of the oracle and of the planted optimum when solved;
   * the documented known answers, infeasibility detection, warm start,
@@ -151,6 +154,35 @@ def test_bb_iterates_within_summation_noise(oracle_qp, oracle_qp_rev, gpu_qp, ki
     print(f"[noise] kind={kind} iters={iters} oracle-vs-reversed={floor:.3e} gpu-vs-oracle={dist:.3e}")
     assert dist <= NOISE_FACTOR * floor + ITER_RTOL
     assert abs(g.inner_iterations - o.inner_iterations) <= 0.1 * o.inner_iterations
+
+
+RESUMED = {"sparse": 48, "lowrank": 48, "lowrank_plus_sparse": 200}
+
+
+@pytest.mark.parametrize("kind", BB_KINDS)
+def test_bb_resumed_from_device_state_tracks_oracle(oracle_qp, gpu_qp, kind):
+    """The BB path on late iterates at 1e-10: the device's own x, y after 1000
+    iterations (several restarts in) start both the oracle and the device
+    (set_start_values), which then run RESUMED[kind] iterations.  Measured from
+    that state, the forward and reversed-sum oracles differ by 1.1e-11 (sparse,
+    48), 2.5e-11 (low rank, 48) and 1.4e-13 (low rank plus sparse, 200: 446
+    inner steps, the inner tolerance tightened at the evaluations of
+    iterations 64, 128 and 192), the device from the oracle by 3.1e-12,
+    2.2e-11 and 3.2e-14, with equal inner step counts; ten iterations later the
+    sparse and low-rank trajectories have drifted apart by 1e-10 in both
+    oracle builds alike."""
+    args = _qp_instance(kind)
+    gpu_qp.reset_rng()
+    print(f"[case] device state after 1000 iterations kind={kind}", flush=True)
+    base = gpu_qp.solve(**args, params=gpu_qp.default_params(iteration_limit=1000, eps_optimal_relative=1e-14,
+                                                             eps_feasible_relative=1e-14))
+    assert base.iterations == 1000
+    it = RESUMED[kind]
+    o, g = run_both(oracle_qp, gpu_qp, args, start=(base.x, base.y), iteration_limit=it,
+                    eps_optimal_relative=1e-14, eps_feasible_relative=1e-14)
+    assert o.iterations == g.iterations == it
+    assert g.inner_iterations == o.inner_iterations
+    assert_tracks(o, g)
 
 
 @pytest.mark.parametrize("iters", [200, 600])
