@@ -188,6 +188,7 @@ class BatchSolver {
     template <int NK>
     void sh_passes(const CsrView &A, const CsrView &At, const double *yv, const ShVecs &v, const int *ids, int k,
                    bool major, bool primal, bool dual);
+
     bool same_matrices() const;
     void detect_shared();
     void sh_alloc(int set, int width);
@@ -204,9 +205,12 @@ class BatchSolver {
     // The last few running instances go to the block-diagonal engine: a
     // shared-A pass costs about the same for 2 running slots as for 64 (it is
     // latency-bound), while a block-diagonal pass costs in proportion to the
-    // instances it holds (PDAL_BATCH_NO_HANDOFF=1: never).
-    static constexpr int kShHandoff = 8;
+    // instances it holds (PDAL_BATCH_NO_HANDOFF=1: never; PDAL_BATCH_HANDOFF
+    // sets the count).
+    int sh_handoff_at_ = 16;
+    int sh_eval_blocks_ = kShEvalBlocks;  // workgroups of the evaluation passes (PDAL_SH_EVAL_BLOCKS)
     bool handoff_ = true;
+    bool trace_ = false;  // PDAL_BATCH_TRACE=1: one stderr line per checked block
     int handoffs_ = 0;
     bool res_ready_ = false;
     DBuf<double> rc_all_, xo_all_, yo_all_;
@@ -524,7 +528,8 @@ void BatchSolver::detect_shared()
     Bp_full_ = Bp_ = nk * kWave;
     detect_vector_flags();
     sh_alloc(0, Bp_);
-    sh_parts_.alloc((size_t)kShEvalBlocks * std::max(kAtQ, kAQ) * Bp_);
+    if (const char *raw = getenv("PDAL_SH_EVAL_BLOCKS")) sh_eval_blocks_ = std::max(1, std::min(8192, atoi(raw)));
+    sh_parts_.alloc((size_t)sh_eval_blocks_ * std::max(kAtQ, kAQ) * Bp_);
     for (auto &ids : sh_ids_) ids.alloc(Bp_);
     h_ids_.resize(Bp_);
     for (int s = 0; s < Bp_; ++s) h_ids_[s] = s < N_ ? s : -1;
@@ -654,14 +659,15 @@ void BatchSolver::sh_launch_block()
     At.rows = n;
     const ShVecs v = sh_vecs();
     const int *ids = sh_ids_[cur_].get();
-    const int nb = kShEvalBlocks, rb = (Bp_ + kBlock - 1) / kBlock;
+    const int nb = sh_eval_blocks_;
+    const int rb_at = (kAtQ * Bp_ + kBlock - 1) / kBlock, rb_a = (kAQ * Bp_ + kBlock - 1) / kBlock;
     sh_dispatch(NK_, [&](auto nk) {
         constexpr int NK = decltype(nk)::value;
         sh_eval_at_kernel<NK><<<nb, kBlock, 0, stream_>>>(At, v, D_.get(), st_.get(), ids, Bp_, sh_parts_.get());
-        sh_reduce_kernel<<<rb, kBlock, 0, stream_>>>(sh_parts_.get(), nb, kAtQ, kAtSums, Bp_, ids, 0, evalq_.get(),
+        sh_reduce_kernel<<<rb_at, kBlock, 0, stream_>>>(sh_parts_.get(), nb, kAtQ, kAtSums, Bp_, ids, 0, evalq_.get(),
                                                      st_.get());
         sh_eval_a_kernel<NK><<<nb, kBlock, 0, stream_>>>(A, v, E_.get(), st_.get(), ids, Bp_, sh_parts_.get());
-        sh_reduce_kernel<<<rb, kBlock, 0, stream_>>>(sh_parts_.get(), nb, kAQ, kASums, Bp_, ids, 8, evalq_.get(),
+        sh_reduce_kernel<<<rb_a, kBlock, 0, stream_>>>(sh_parts_.get(), nb, kAQ, kASums, Bp_, ids, 8, evalq_.get(),
                                                      st_.get());
     });
     control_kernel<<<blocks_for_l(N_), kBlock, 0, stream_>>>(st_.get(), evalq_.get(), N_, cp_);
@@ -830,6 +836,8 @@ void BatchSolver::setup()
     use_graph_ = !(getenv("PDAL_NO_GRAPH") && atoi(getenv("PDAL_NO_GRAPH")) != 0);
     compact_ = !(getenv("PDAL_BATCH_NO_COMPACT") && atoi(getenv("PDAL_BATCH_NO_COMPACT")) != 0);
     handoff_ = !(getenv("PDAL_BATCH_NO_HANDOFF") && atoi(getenv("PDAL_BATCH_NO_HANDOFF")) != 0);
+    if (const char *raw = getenv("PDAL_BATCH_HANDOFF")) sh_handoff_at_ = std::max(1, atoi(raw));
+    trace_ = getenv("PDAL_BATCH_TRACE") && atoi(getenv("PDAL_BATCH_TRACE")) != 0;
     xcd_ = (getenv("PDAL_BATCH_NO_XCD") && atoi(getenv("PDAL_BATCH_NO_XCD")) != 0) ? 0 : 1;
     staged_ = !(getenv("PDAL_BATCH_NO_STAGE") && atoi(getenv("PDAL_BATCH_NO_STAGE")) != 0);
     if (const char *raw = getenv("PDAL_BATCH_MIN_STAGED")) min_staged_ = std::max(1, atoi(raw));
@@ -1439,6 +1447,11 @@ int BatchSolver::solve()
                 next_report = el + 20.0;
             }
         }
+        if (trace_)
+            fprintf(stderr, "[batch-trace] %.3f ms block %ld: %d running, %s %d\n",
+                    1e3 * std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(), checked,
+                    active, shared_ && !handed ? "shared width" : "block-diagonal instances",
+                    shared_ && !handed ? Bp_ : table_instances_);
         if (t_half_ == 0.0 && active <= N_ / 2)
             t_half_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         if (t_p99_ == 0.0 && active <= N_ / 100)
@@ -1453,7 +1466,7 @@ int BatchSolver::solve()
             checked = launched;
             if (still == 0) break;
             ++compactions_;
-        } else if (shared_ && !handed && handoff_ && active <= kShHandoff) {
+        } else if (shared_ && !handed && handoff_ && active <= sh_handoff_at_) {
             const int still = sh_handoff();
             handed = true;
             ++handoffs_;
