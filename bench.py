@@ -41,6 +41,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
        python bench.py --workload config3   (the 4096-instance batch alone)
        python bench.py --workload config1   (plain single-instance solve of the large LP)
        python bench.py --kernels-only       (set-up and kernel timing only, for rocprofv3)
+       python bench.py --kernels-only --solve-blocks 10   (+ ten whole cold blocks of the held-out batch)
 """
 from __future__ import annotations
 
@@ -409,6 +410,13 @@ def run_config4(args, lib) -> dict:
     result = {"roofline": roof, "setup_s": round(setup_s, 3), "shared_bounds": st["shared_bounds"],
               "shared_kernel_ms": {"primal": kt["primal_ms"], "dual": kt["dual_ms"]}}
     if args.kernels_only:
+        if args.solve_blocks > 0:  # a fixed number of whole cold blocks of the held-out batch (profiling)
+            with LpBatch(lib, test_inst, lib.default_params(
+                    termination_evaluation_frequency=args.eval_freq, eps_optimal_relative=args.tol,
+                    eps_feasible_relative=args.tol, iteration_limit=args.solve_blocks * args.eval_freq)) as blk:
+                blk.solve()
+                log(f"{args.solve_blocks} cold blocks of {args.eval_freq} iterations, {count} members: "
+                    f"{blk.stats()['solve_s'] * 1e3:.1f} ms")
         batch.close()
         return result
     # training: the solver's own solutions of other family members (outside the timed region)
@@ -682,6 +690,8 @@ def main() -> int:
     ap.add_argument("--qp-time", type=float, default=25.0, help="config-2 leg: time cap per FA_CP solve (s)")
     ap.add_argument("--qp-iters", type=int, default=10_000_000, help="config-2 leg: iteration cap per FA_CP solve")
     ap.add_argument("--kernel-iters", type=int, default=20)
+    ap.add_argument("--solve-blocks", type=int, default=0,
+                    help="with --kernels-only (config4): also solve this many whole cold blocks of the held-out batch")
     ap.add_argument("--eval-freq", type=int, default=K_WARM,
                     help="config4: evaluation frequency of the held-out solves (warm, cold and plain)")
     args = ap.parse_args()
