@@ -702,7 +702,7 @@ void BatchSolver::sh_begin()
     const long m = m_ / N_, n = n_ / N_;
     const int *ids = sh_ids_[0].get();
     auto gather = [&](const double *cat, double *out, long rows) {
-        sh_gather_kernel<<<blocks_for_l(rows * Bp_), kBlock, 0, stream_>>>(cat, out, rows, Bp_, ids);
+        sh_gather_kernel<<<sh_tile_grid(rows, Bp_), kBlock, 0, stream_>>>(cat, out, rows, Bp_, ids);
     };
     SharedBufs &b = sh_[0];
     gather(c_.get(), b.c.get(), n);
@@ -731,8 +731,8 @@ void BatchSolver::sh_scatter_current()
 {   // x_T, y_T of every instance the current layout holds -> the concatenated vectors the results read
     const long m = m_ / N_, n = n_ / N_;
     const int *ids = sh_ids_[cur_].get();
-    sh_scatter_kernel<<<blocks_for_l(n * Bp_), kBlock, 0, stream_>>>(sh_[cur_].xT.get(), xT_.get(), n, Bp_, ids);
-    sh_scatter_kernel<<<blocks_for_l(m * Bp_), kBlock, 0, stream_>>>(sh_[cur_].yT.get(), yT_.get(), m, Bp_, ids);
+    sh_scatter_kernel<<<sh_tile_grid(n, Bp_), kBlock, 0, stream_>>>(sh_[cur_].xT.get(), xT_.get(), n, Bp_, ids);
+    sh_scatter_kernel<<<sh_tile_grid(m, Bp_), kBlock, 0, stream_>>>(sh_[cur_].yT.get(), yT_.get(), m, Bp_, ids);
     PDAL_LAUNCHED();
 }
 
@@ -751,7 +751,7 @@ int BatchSolver::sh_handoff()
     const int *ids = sh_ids_[cur_].get();
     SharedBufs &b = sh_[cur_];
     auto put = [&](const DBuf<double> &in, DBuf<double> &cat, long rows) {
-        sh_scatter_kernel<<<blocks_for_l(rows * Bp_), kBlock, 0, stream_>>>(in.get(), cat.get(), rows, Bp_, ids);
+        sh_scatter_kernel<<<sh_tile_grid(rows, Bp_), kBlock, 0, stream_>>>(in.get(), cat.get(), rows, Bp_, ids);
     };
     put(b.x0, x0_, n);
     put(b.x, x_, n);
