@@ -35,7 +35,8 @@ The same JSON line carries:
     the CPU model; beside it, full oracle and GPU solves of the small planted
     LP at 1e-4 and 1e-8 (cpu_config0_full_solves);
   * qp_config2: FA_CP on the planted QP (explicit and low-rank Q), one solve
-    each to 1e-6 within a time cap, time and bytes per inner BB step.
+    each to 1e-6 within a time cap, time and bytes per inner BB step; and a
+    whole explicit-Q solve of the same family at a fifth of the size.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
        python bench.py --workload config3   (the 4096-instance batch alone)
@@ -658,6 +659,23 @@ def run_config2_leg(args) -> dict:
         log(f"config2 {form}: {out[form]}")
     if all(k in objs for k in ("explicit", "lowrank")):
         out["explicit_vs_lowrank_objective_rel"] = abs(objs["explicit"] - objs["lowrank"]) / abs(objs["explicit"])
+    # a whole solve within the cap: the same family at a fifth of the size
+    # (n = 2e4, m = 1e4, M 4e3 x n, so Q keeps about 126 nonzeros per row)
+    from pdal_amd.generators import planted_qp
+    small = planted_qp(20000, 10000, 4000, seed=7)
+    p = qlib.default_params(eps_optimal_relative=args.tol, eps_feasible_relative=args.tol, verbose=0,
+                            time_sec_limit=args.qp_time, iteration_limit=args.qp_iters)
+    qlib.reset_rng()
+    t0 = time.perf_counter()
+    r = qlib.solve(**small.qp_args("explicit"), params=p)
+    wall = time.perf_counter() - t0
+    out["explicit_fifth_size"] = {
+        "workload": "planted QP n=2e4 m=1e4, Q=M^T M+1e-3 I (M 4e3 x n, 5 nnz/col), explicit Q",
+        "status": r.status, "iterations": r.iterations, "inner_steps": r.inner_iterations,
+        "solve_s": round(r.solve_time_sec, 3), "wall_s": round(wall, 3), "solves_per_s": round(1.0 / wall, 4),
+        "kkt": r.kkt, "us_per_inner_step": round(1e6 * r.solve_time_sec / max(r.inner_iterations, 1), 3),
+        "objective_rel_err_vs_planted": abs(r.primal_objective - small.optimal_value) / abs(small.optimal_value)}
+    log(f"config2 fifth size: {out['explicit_fifth_size']}")
     return out
 
 
